@@ -1,0 +1,466 @@
+/*
+ * CPU oracle body, instantiated once per packed word type.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a single-threaded restatement of the
+ * reference SparsePauliVector engine (PauliOperators.jl v3.1.0).  It is the
+ * checker for the HIP kernels and the CPU baseline timed beside them; the
+ * product library never links or calls it.
+ *
+ * Instantiation: define WT (uint64_t or u128) and SFX (a name suffix), then
+ * include this file.  Every function cites the reference file:line it restates.
+ */
+
+#define CAT2(a, b) a##b
+#define CAT(a, b) CAT2(a, b)
+#define F(name) CAT(name, SFX)
+
+typedef struct {
+    WT z, x;
+    double c;
+} F(or_trip);
+
+/* type_SparsePauliVector.jl:69-84 — live / append / scratch SoA buffers plus
+ * the AoS sort workspace and its ping-pong twin. */
+typedef struct {
+    int N;
+    WT *z, *x;
+    double *c;
+    int64_t n, live_cap;
+    WT *az, *ax;
+    double *ac;
+    int64_t an, app_cap;
+    WT *sz, *sx;
+    double *sc;
+    F(or_trip) *ws;
+    int64_t ws_cap;
+    F(or_trip) *ws2;
+    int64_t ws2_cap;
+} F(or_spv);
+
+/* spv_kernels.jl:14-16 — lexicographic (z, x) order, z-major. */
+static inline int F(key_lt)(WT az, WT ax, WT bz, WT bx) {
+    return (az < bz) | ((az == bz) & (ax < bx));
+}
+static inline int F(trip_lt)(const F(or_trip) *a, const F(or_trip) *b) {
+    return F(key_lt)(a->z, a->x, b->z, b->x);
+}
+
+/* spv_kernels.jl:51-62 — branchless suffix-parity Majorana weight. */
+static inline int F(majorana_bits)(WT z, WT x) {
+    WT zonly = z & ~x;
+    WT S = x;
+    for (int shift = 1; shift < 8 * (int)sizeof(WT); shift <<= 1) S ^= S >> shift;
+    WT ctrl = ~(S ^ x);
+    return F(popc)(x) + 2 * F(popc)(zonly & ctrl) + 2 * F(popc)(~(z | x) & ~ctrl);
+}
+
+/* spv_kernels.jl:64-79 — per-term drop predicate.  The exp() factors come from
+ * the filter's host-computed tables (exp(-alpha*w) for w = 0..256). */
+static inline int F(should_drop)(const or_filter_t *f, WT z, WT x, double absc) {
+    if (absc <= f->thresh) return 1;
+    int w = F(popc)(z | x);
+    if ((int64_t)w > f->wmax) return 1;
+    int xw = F(popc)(x);
+    if ((int64_t)xw > f->xwmax) return 1;
+    if (f->mwmax != INT64_MAX) {
+        if ((int64_t)F(majorana_bits)(z, x) > f->mwmax) return 1;
+    }
+    if (f->dthresh_w >= 0.0) {
+        if (absc * f->exptab_w[w] <= f->dthresh_w) return 1;
+    }
+    if (f->dthresh_xw >= 0.0) {
+        if (absc * f->exptab_xw[xw] <= f->dthresh_xw) return 1;
+    }
+    return 0;
+}
+
+/* type_SparsePauliVector.jl:94-104 — allocate all buffers. */
+static F(or_spv) *F(or_alloc)(int N, int64_t live_cap, int64_t app_cap) {
+    F(or_spv) *v = (F(or_spv) *)calloc(1, sizeof(F(or_spv)));
+    v->N = N;
+    v->live_cap = live_cap;
+    v->app_cap = app_cap;
+    v->z = (WT *)calloc((size_t)live_cap, sizeof(WT));
+    v->x = (WT *)calloc((size_t)live_cap, sizeof(WT));
+    v->c = (double *)calloc((size_t)live_cap, sizeof(double));
+    v->sz = (WT *)calloc((size_t)live_cap, sizeof(WT));
+    v->sx = (WT *)calloc((size_t)live_cap, sizeof(WT));
+    v->sc = (double *)calloc((size_t)live_cap, sizeof(double));
+    v->az = (WT *)calloc((size_t)app_cap, sizeof(WT));
+    v->ax = (WT *)calloc((size_t)app_cap, sizeof(WT));
+    v->ac = (double *)calloc((size_t)app_cap, sizeof(double));
+    v->ws_cap = live_cap > app_cap ? live_cap : app_cap;
+    v->ws = (F(or_trip) *)malloc((size_t)v->ws_cap * sizeof(F(or_trip)));
+    v->ws2 = NULL;
+    v->ws2_cap = 0;
+    return v;
+}
+
+static void F(or_free)(F(or_spv) *v) {
+    free(v->z); free(v->x); free(v->c);
+    free(v->sz); free(v->sx); free(v->sc);
+    free(v->az); free(v->ax); free(v->ac);
+    free(v->ws); free(v->ws2);
+    free(v);
+}
+
+static void *F(xrealloc)(void *p, size_t n) {
+    void *q = realloc(p, n);
+    if (!q && n) { fprintf(stderr, "oracle: out of memory\n"); abort(); }
+    return q;
+}
+
+/* type_SparsePauliVector.jl:107-117 — chunked doubling of live + scratch. */
+static void F(grow_live)(F(or_spv) *v, int64_t need) {
+    int64_t newcap = 2 * v->live_cap > need ? 2 * v->live_cap : need;
+    v->z = (WT *)F(xrealloc)(v->z, (size_t)newcap * sizeof(WT));
+    v->x = (WT *)F(xrealloc)(v->x, (size_t)newcap * sizeof(WT));
+    v->c = (double *)F(xrealloc)(v->c, (size_t)newcap * sizeof(double));
+    v->sz = (WT *)F(xrealloc)(v->sz, (size_t)newcap * sizeof(WT));
+    v->sx = (WT *)F(xrealloc)(v->sx, (size_t)newcap * sizeof(WT));
+    v->sc = (double *)F(xrealloc)(v->sc, (size_t)newcap * sizeof(double));
+    v->live_cap = newcap;
+    if (v->ws_cap < newcap) {
+        v->ws = (F(or_trip) *)F(xrealloc)(v->ws, (size_t)newcap * sizeof(F(or_trip)));
+        v->ws_cap = newcap;
+    }
+}
+
+/* type_SparsePauliVector.jl:119-126 — chunked doubling of the append buffer. */
+static void F(grow_append)(F(or_spv) *v, int64_t need) {
+    int64_t newcap = 2 * v->app_cap > need ? 2 * v->app_cap : need;
+    v->az = (WT *)F(xrealloc)(v->az, (size_t)newcap * sizeof(WT));
+    v->ax = (WT *)F(xrealloc)(v->ax, (size_t)newcap * sizeof(WT));
+    v->ac = (double *)F(xrealloc)(v->ac, (size_t)newcap * sizeof(double));
+    v->app_cap = newcap;
+    if (v->ws_cap < newcap) {
+        v->ws = (F(or_trip) *)F(xrealloc)(v->ws, (size_t)newcap * sizeof(F(or_trip)));
+        v->ws_cap = newcap;
+    }
+}
+
+/* spv_kernels.jl:146-177 — rotation sweep of terms [lo, hi) (0-based) of one
+ * buffer set, appending sin-branch children at *cur. */
+static void F(rotate_range)(WT *z, WT *x, double *c, int64_t lo, int64_t hi,
+                            WT gz, WT gx, int ng, double cosv, double sinv,
+                            WT *dz, WT *dx, double *dc, int64_t *cur, int64_t cap_end,
+                            const or_filter_t *f, int64_t *created, int *overflowed) {
+    for (int64_t i = lo; i < hi; i++) {
+        WT zi = z[i], xi = x[i];
+        int m1 = F(popc)(gx & zi);
+        int m2 = F(popc)(gz & xi);
+        if (((m1 - m2) & 1) == 0) continue;
+        WT zp = gz ^ zi;
+        WT xp = gx ^ xi;
+        int k = (F(popc)(zp & xp) - ng - F(popc)(zi & xi) + 2 * m1) & 3;
+        double cnew = ((double)(k - 2) * sinv) * c[i];
+        c[i] = c[i] * cosv;
+        if (F(should_drop)(f, zp, xp, fabs(cnew))) continue;
+        if (*cur >= cap_end) {
+            *overflowed = 1;
+            continue;
+        }
+        dz[*cur] = zp;
+        dx[*cur] = xp;
+        dc[*cur] = cnew;
+        *cur += 1;
+        *created += 1;
+    }
+}
+
+/* spv_kernels.jl:187-193 */
+static int64_t F(gather_append)(F(or_spv) *v) {
+    int64_t m = v->an;
+    for (int64_t i = 0; i < m; i++) {
+        v->ws[i].z = v->az[i];
+        v->ws[i].x = v->ax[i];
+        v->ws[i].c = v->ac[i];
+    }
+    return m;
+}
+
+/* spv_kernels.jl:202-247 — in-place quicksort, median of 3, insertion sort
+ * below 24 elements; lo, hi inclusive (0-based). */
+static void F(sort_ws)(F(or_trip) *ws, int64_t lo, int64_t hi) {
+    F(or_trip) t;
+#define SWAPT(a, b) do { t = ws[a]; ws[a] = ws[b]; ws[b] = t; } while (0)
+    while (hi - lo >= 24) {
+        int64_t mid = (int64_t)(((uint64_t)lo + (uint64_t)hi) >> 1);
+        if (F(trip_lt)(&ws[mid], &ws[lo])) SWAPT(mid, lo);
+        if (F(trip_lt)(&ws[hi], &ws[mid])) {
+            SWAPT(hi, mid);
+            if (F(trip_lt)(&ws[mid], &ws[lo])) SWAPT(mid, lo);
+        }
+        F(or_trip) pivot = ws[mid];
+        int64_t i = lo, j = hi;
+        while (i <= j) {
+            while (F(trip_lt)(&ws[i], &pivot)) i++;
+            while (F(trip_lt)(&pivot, &ws[j])) j--;
+            if (i <= j) {
+                SWAPT(i, j);
+                i++;
+                j--;
+            }
+        }
+        if (j - lo < hi - i) {
+            F(sort_ws)(ws, lo, j);
+            lo = i;
+        } else {
+            F(sort_ws)(ws, i, hi);
+            hi = j;
+        }
+    }
+#undef SWAPT
+    for (int64_t k = lo + 1; k <= hi; k++) {
+        F(or_trip) val = ws[k];
+        int64_t m = k - 1;
+        while (m >= lo && F(trip_lt)(&val, &ws[m])) {
+            ws[m + 1] = ws[m];
+            m--;
+        }
+        ws[m + 1] = val;
+    }
+}
+
+/* spv_kernels.jl:253-286 — one stable block-swap pass of the unshuffle cascade
+ * for bit b of word `which` (1 = z, 2 = x). */
+static void F(unshuffle_pass)(const F(or_trip) *src, F(or_trip) *dst, int64_t m, int b, int which) {
+    /* bits above b; Julia's shift by the full width yields 0, so hb = 0 there */
+    WT hb = (b + 1 >= 8 * (int)sizeof(WT)) ? (WT)0 : ~((((WT)1) << (b + 1)) - (WT)1);
+    int64_t i = 0;
+    while (i < m) {
+        WT hp1 = which == 1 ? (src[i].z & hb) : src[i].z;
+        WT hp2 = which == 1 ? (WT)0 : (src[i].x & hb);
+        int64_t j = i;
+        while (j + 1 < m) {
+            WT q1 = which == 1 ? (src[j + 1].z & hb) : src[j + 1].z;
+            WT q2 = which == 1 ? (WT)0 : (src[j + 1].x & hb);
+            if (q1 == hp1 && q2 == hp2) j++;
+            else break;
+        }
+        int64_t f = i;
+        while (f <= j) {
+            WT w = which == 1 ? src[f].z : src[f].x;
+            if (((w >> b) & (WT)1) == (WT)1) f++;
+            else break;
+        }
+        int64_t k = i;
+        for (int64_t s = f; s <= j; s++) dst[k++] = src[s];
+        for (int64_t s = i; s < f; s++) dst[k++] = src[s];
+        i = j + 1;
+    }
+}
+
+/* spv_kernels.jl:300-320 */
+static void F(unshuffle_ws)(F(or_spv) *v, int64_t m, WT gz, WT gx) {
+    if (m <= 1) return;
+    if (F(popc)(gz) + F(popc)(gx) == 0) return;
+    if (v->ws2_cap < m) {
+        v->ws2 = (F(or_trip) *)F(xrealloc)(v->ws2, (size_t)v->ws_cap * sizeof(F(or_trip)));
+        v->ws2_cap = v->ws_cap;
+    }
+    F(or_trip) *src = v->ws, *dst = v->ws2, *tmp;
+    int flips = 0;
+    for (int b = 8 * (int)sizeof(WT) - 1; b >= 0; b--) {
+        if (((gz >> b) & (WT)1) != (WT)1) continue;
+        F(unshuffle_pass)(src, dst, m, b, 1);
+        tmp = src; src = dst; dst = tmp;
+        flips++;
+    }
+    for (int b = 8 * (int)sizeof(WT) - 1; b >= 0; b--) {
+        if (((gx >> b) & (WT)1) != (WT)1) continue;
+        F(unshuffle_pass)(src, dst, m, b, 2);
+        tmp = src; src = dst; dst = tmp;
+        flips++;
+    }
+    if (flips & 1) memcpy(v->ws, v->ws2, (size_t)m * sizeof(F(or_trip)));
+}
+
+/* spv_kernels.jl:326-363 — two-pointer merge of live + m sorted workspace
+ * triples into scratch; ties take the live term first, then every equal
+ * workspace triple in order.  Returns 1 on overflow (live untouched). */
+static int F(try_merge)(F(or_spv) *v, int64_t m, const or_filter_t *f, int64_t *out_n) {
+    int64_t n = v->n, cap = v->live_cap, out = 0, i = 0, j = 0;
+    WT *z = v->z, *x = v->x, *sz = v->sz, *sx = v->sx;
+    double *c = v->c, *sc = v->sc;
+    F(or_trip) *ws = v->ws;
+    while (i < n || j < m) {
+        WT kz, kx;
+        double acc;
+        if (j >= m || (i < n && !F(key_lt)(ws[j].z, ws[j].x, z[i], x[i]))) {
+            kz = z[i]; kx = x[i]; acc = c[i];
+            i++;
+        } else {
+            kz = ws[j].z; kx = ws[j].x; acc = ws[j].c;
+            j++;
+        }
+        while (j < m && ws[j].z == kz && ws[j].x == kx) {
+            acc = acc + ws[j].c;
+            j++;
+        }
+        if (F(should_drop)(f, kz, kx, fabs(acc))) continue;
+        out++;
+        if (out > cap) { *out_n = out; return 1; }
+        sz[out - 1] = kz;
+        sx[out - 1] = kx;
+        sc[out - 1] = acc;
+    }
+    v->z = sz; v->sz = z;
+    v->x = sx; v->sx = x;
+    v->c = sc; v->sc = c;
+    v->n = out;
+    *out_n = out;
+    return 0;
+}
+
+/* spv_kernels.jl:377-386 */
+static void F(merge_spv)(F(or_spv) *v, int64_t m, const or_filter_t *f, int64_t *n_in, int64_t *n_out) {
+    *n_in = v->n + m;
+    int64_t out;
+    while (F(try_merge)(v, m, f, &out)) F(grow_live)(v, v->n + m);
+    v->an = 0;
+    *n_out = out;
+}
+
+/* spv_kernels.jl:395-408 — order-preserving in-place compaction. */
+static void F(compact)(F(or_spv) *v, const or_filter_t *f) {
+    int64_t out = 0;
+    for (int64_t i = 0; i < v->n; i++) {
+        if (F(should_drop)(f, v->z[i], v->x[i], fabs(v->c[i]))) continue;
+        if (out != i) {
+            v->z[out] = v->z[i];
+            v->x[out] = v->x[i];
+            v->c[out] = v->c[i];
+        }
+        out++;
+    }
+    v->n = out;
+}
+
+/* spv_kernels.jl:441-452 — sequential ⟨k|O|k⟩ over live then appends. */
+static double F(expectation)(const F(or_spv) *v, WT kv) {
+    double acc = 0.0;
+    for (int64_t i = 0; i < v->n; i++) {
+        if (v->x[i] != (WT)0) continue;
+        acc = acc + (double)(1 - 2 * (F(popc)(v->z[i] & kv) & 1)) * v->c[i];
+    }
+    for (int64_t i = 0; i < v->an; i++) {
+        if (v->ax[i] != (WT)0) continue;
+        acc = acc + (double)(1 - 2 * (F(popc)(v->az[i] & kv) & 1)) * v->ac[i];
+    }
+    return acc;
+}
+
+/* type_SparsePauliVector.jl:195-204 — construction-time quicksort of live. */
+static void F(sort_live)(F(or_spv) *v) {
+    if (v->ws_cap < v->n) {
+        v->ws = (F(or_trip) *)F(xrealloc)(v->ws, (size_t)v->n * sizeof(F(or_trip)));
+        v->ws_cap = v->n;
+    }
+    for (int64_t i = 0; i < v->n; i++) {
+        v->ws[i].z = v->z[i];
+        v->ws[i].x = v->x[i];
+        v->ws[i].c = v->c[i];
+    }
+    if (v->n > 0) F(sort_ws)(v->ws, 0, v->n - 1);
+    for (int64_t i = 0; i < v->n; i++) {
+        v->z[i] = v->ws[i].z;
+        v->x[i] = v->ws[i].x;
+        v->c[i] = v->ws[i].c;
+    }
+}
+
+/* spv_evolve.jl:207-218 — rotate live, then the pending appends present at
+ * entry; children land in the append buffer. */
+static int F(rotate_spv)(F(or_spv) *v, WT gz, WT gx, int ng, double cosv, double sinv,
+                         const or_filter_t *f, int64_t *created) {
+    int64_t hi = v->an;
+    int64_t cur = v->an;
+    int ovf = 0;
+    *created = 0;
+    F(rotate_range)(v->z, v->x, v->c, 0, v->n, gz, gx, ng, cosv, sinv,
+                    v->az, v->ax, v->ac, &cur, v->app_cap, f, created, &ovf);
+    F(rotate_range)(v->az, v->ax, v->ac, 0, hi, gz, gx, ng, cosv, sinv,
+                    v->az, v->ax, v->ac, &cur, v->app_cap, f, created, &ovf);
+    v->an = cur;
+    return ovf;
+}
+
+/* truncation.jl:222-233 / spv_evolve.jl:103-114 — AdaptiveTruncation on
+ * merged state: exact sort-based top-k threshold. */
+static int F(cmp_dbl)(const void *a, const void *b) {
+    double x = *(const double *)a, y = *(const double *)b;
+    return (x > y) - (x < y);
+}
+static void F(apply_adaptive)(F(or_spv) *v, int64_t max_terms, double min_thresh) {
+    or_filter_t f;
+    or_filter_init_nofilter(&f);
+    if (v->n > max_terms) {
+        double *coeffs = (double *)malloc((size_t)v->n * sizeof(double));
+        for (int64_t i = 0; i < v->n; i++) coeffs[i] = fabs(v->c[i]);
+        qsort(coeffs, (size_t)v->n, sizeof(double), F(cmp_dbl));
+        /* Julia coeffs[end - max_terms] (1-based) = index n - max_terms - 1 (0-based) */
+        f.thresh = coeffs[v->n - max_terms - 1];
+        free(coeffs);
+    } else {
+        f.thresh = min_thresh;
+    }
+    F(compact)(v, &f);
+}
+
+/* spv_evolve.jl:258-284 — merge boundary.  Corrections are not restated. */
+static void F(boundary)(F(or_spv) *v, const or_filter_t *f, const or_strategy_t *s,
+                        int have_mask, WT lz, WT lx, int64_t *n_in, int64_t *n_out) {
+    int64_t m = F(gather_append)(v);
+    if (have_mask) F(unshuffle_ws)(v, m, lz, lx);
+    else if (m > 0) F(sort_ws)(v->ws, 0, m - 1);
+    F(merge_spv)(v, m, f, n_in, n_out);
+    if (s->kind == OR_STRAT_ADAPTIVE) F(apply_adaptive)(v, s->max_terms, s->min_thresh);
+}
+
+/* spv_evolve.jl:312-387 — the windowed driver. */
+static int F(evolve_seq)(F(or_spv) *v, const WT *gz, const WT *gx, const double *cosv,
+                         const double *sinv, int64_t L, int64_t window,
+                         const or_filter_t *strict, const or_strategy_t *strat,
+                         const or_filter_t *local, or_counters_t *cnt) {
+    if (window < 1) return -1;
+    int since = 0;
+    WT lz = 0, lx = 0;
+    for (int64_t i = 0; i < L; i++) {
+        int64_t w = i / window; /* 0-based window index */
+        int64_t n_in, n_out;
+        if (2 * v->an + v->n > v->app_cap) {
+            F(boundary)(v, strict, strat, since == 1, lz, lx, &n_in, &n_out);
+            since = 0;
+            if (cnt) {
+                cnt->early_merges += 1;
+                cnt->merge_in += n_in;
+                cnt->merge_out += n_out;
+            }
+            (void)w;
+            if (v->n > v->app_cap) F(grow_append)(v, v->n);
+        }
+        int ng = F(popc)(gz[i] & gx[i]);
+        int64_t swept = v->n + v->an;
+        int64_t created;
+        int ovf = F(rotate_spv)(v, gz[i], gx[i], ng, cosv[i], sinv[i], local, &created);
+        if (ovf) return -2;
+        lz = gz[i];
+        lx = gx[i];
+        since += 1;
+        if (cnt) {
+            cnt->terms_created += created;
+            cnt->rotations += 1;
+            cnt->terms_swept += swept; /* terms entering the sweep (window=1: live n_in) */
+        }
+        if ((i + 1) % window == 0 || i == L - 1) {
+            F(boundary)(v, strict, strat, since == 1, lz, lx, &n_in, &n_out);
+            since = 0;
+            if (cnt) {
+                cnt->merge_in += n_in;
+                cnt->merge_out += n_out;
+            }
+        }
+    }
+    return 0;
+}
