@@ -1,0 +1,122 @@
+/*
+ * pauliprop.h — C ABI of the MI355X SparsePauliVector engine (libpauliprop.so).
+ *
+ * The reference (PauliOperators.jl v3.1.0) has no FFI: its interface is Julia
+ * dispatch on SparsePauliVector{N,W,T} (src/PauliOperators.jl:42-110).  Each
+ * entry point below replaces one of those methods for Float64 coefficients;
+ * the Julia/Python wrappers validate arguments and raise the reference's
+ * exception types before calling in.
+ *
+ * Conventions
+ *  - Keys cross the ABI as W little-endian uint64 words per term for z and
+ *    for x (W = 1 for N <= 64, 2 for N <= 128), so a Julia Vector{UInt64} or
+ *    Vector{UInt128} passes unchanged (type_SparsePauliVector.jl:23-34).
+ *  - Host pointers are borrowed for the duration of a call; device state is
+ *    owned by the handle.  One handle per host thread; calls are ordered on
+ *    the handle's HIP stream.
+ *  - Every function returns PP_OK or an error code; pp_last_error() gives the
+ *    message of the last failure on the calling thread.
+ */
+#ifndef PAULIPROP_H
+#define PAULIPROP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PP_OK = 0,
+    PP_EARG = 1,         /* invalid argument (reference: ArgumentError / DimensionMismatch) */
+    PP_ENOMEM = 2,       /* device allocation failed */
+    PP_EHIP = 3,         /* HIP runtime error */
+    PP_EOVERFLOW = 4,    /* capacity overflow that growth could not resolve */
+    PP_EUNSUPPORTED = 5  /* mode not available in this build */
+};
+
+typedef struct pp_handle pp_handle;
+
+/* Byte layout of MergeFilter (src/spv_kernels.jl:33-42). */
+typedef struct {
+    int64_t wmax, xwmax, mwmax;
+    double thresh, alpha_w, dthresh_w, alpha_xw, dthresh_xw;
+} pp_filter;
+
+/* WindowCounters (src/spv_evolve.jl:149-161), summed over one pp_evolve call. */
+typedef struct {
+    int64_t rotations;      /* rotations applied */
+    int64_t terms_swept;    /* sum over rotations of the live count entering it */
+    int64_t terms_created;  /* children appended (anticommuting, kept by local filter) */
+    int64_t merge_in;       /* sum of n + m entering each merge */
+    int64_t merge_out;      /* sum of terms leaving each merge */
+    int64_t early_merges;   /* capacity-forced merges (spv_evolve.jl:356-362) */
+    double t_rotate;        /* seconds in sweep kernels (HIP events) */
+    double t_merge;         /* seconds in the merge kernel (HIP events) */
+} pp_counters;
+
+/* SparsePauliVector(N, Float64) with the reference's capacity bookkeeping
+ * (type_SparsePauliVector.jl:94-104,167-188): live_cap and append_cap are the
+ * reference's buffer lengths, which decide when an early merge happens.  Device
+ * memory is sized separately and grows on demand. */
+int pp_create(int N, int64_t live_cap, int64_t append_cap, pp_handle** out);
+int pp_destroy(pp_handle* h);
+
+/* Replace the live set by n terms.  sorted = 1: keys already strictly
+ * increasing; sorted = 0: sorted here (the constructor's _sort_live!,
+ * type_SparsePauliVector.jl:195-204); duplicate keys are an error. */
+int pp_upload(pp_handle* h, const uint64_t* z, const uint64_t* x, const double* c, int64_t n, int sorted);
+/* Copy the live set out (PauliSum(v), type_SparsePauliVector.jl:212-221). */
+int pp_download(pp_handle* h, uint64_t* z, uint64_t* x, double* c, int64_t cap, int64_t* n);
+int pp_size(pp_handle* h, int64_t* n);
+int pp_capacities(pp_handle* h, int64_t* live_cap, int64_t* append_cap);
+
+/* Sequence evolution (spv_evolve.jl:312-387): L rotations exp(iθ/2 G) O exp(-iθ/2 G),
+ * applied left to right.  gz, gx: L generators of W words each; cosv, sinv:
+ * the caller's cos θ and sin θ (spv_evolve.jl:341-342).  strict: the compiled
+ * truncation applied at every merge; local: the filter applied to each child
+ * as it is created.  window must be 1 in this build (the reference default).
+ * counters may be NULL. */
+int pp_evolve(pp_handle* h, const uint64_t* gz, const uint64_t* gx, const double* cosv, const double* sinv,
+              int64_t L, int64_t window, const pp_filter* strict, const pp_filter* local, pp_counters* counters);
+
+/* Single rotation without truncation (spv_evolve.jl:228-236). */
+int pp_evolve_single(pp_handle* h, const uint64_t* gz, const uint64_t* gx, double cosv, double sinv);
+
+/* Order-preserving compaction under a filter (spv_kernels.jl:395-408): the
+ * SPV clip family (spv_evolve.jl:13-26) and truncate! for compilable
+ * strategies. */
+int pp_compact(pp_handle* h, const pp_filter* f);
+
+/* Diagonal Pauli channel (channels.jl:203-265).  kind: 0 depolarizing
+ * popcount((z|x)&M), 1 dephasing popcount(x&M), 2 bit flip popcount(z&M),
+ * 3 bit-phase flip popcount((x^z)&M), 4 general: t0[nX]*t1[nY]*t2[nZ].
+ * Tables hold λ^k for k = 0..N as the caller computes them (Julia's
+ * Float64^Int gives bit parity); t1, t2 are used by kind 4 only. */
+int pp_scale_by_popcount(pp_handle* h, int kind, const uint64_t* mask, const double* t0, const double* t1,
+                         const double* t2);
+
+/* ⟨ψ|v|ψ⟩ for a computational-basis ket (spv_evolve.jl:133-134 ->
+ * spv_kernels.jl:441-452); ket is W words.  Synchronises. */
+int pp_expectation_ket(pp_handle* h, const uint64_t* ket, double* out);
+
+/* check_spv key-order invariant (type_SparsePauliVector.jl:263-278): *ok = 1
+ * if the live keys are strictly increasing. */
+int pp_check(pp_handle* h, int* ok);
+
+/* Julia's Float64^Int (compensated power by squaring, base/math.jl), for
+ * callers that are not Julia and want the same channel tables. */
+double pp_julia_powi(double x, int64_t n);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* pp_last_error(void);
+
+/* Library version string. */
+const char* pp_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PAULIPROP_H */

@@ -11,3 +11,8 @@ from .truncation import (AdaptiveTruncation, CoeffTruncation, CompositeTruncatio
                          TruncationStrategy, WeightDampedTruncation, WeightTruncation,
                          XWeightDampedTruncation, XWeightTruncation, compile_filter,
                          is_compilable, NOFILTER)
+from ._lib import ArgumentError, DimensionMismatch, PPCounters, PPError
+from .spv import (SparsePauliVector, bit_flip_channel, bit_flip_channel_, bit_phase_flip_channel,
+                  bit_phase_flip_channel_, dephasing_channel, dephasing_channel_, depolarizing_channel,
+                  depolarizing_channel_, depolarizing_p_for_weight_decay, evolve, evolve_, expectation_value,
+                  pauli_channel, pauli_channel_, phase_flip_channel_, truncate_)

@@ -1,0 +1,101 @@
+"""ctypes binding of libpauliprop.so (declarations: include/pauliprop.h).
+
+The library is built in-tree by ``make -C paulioperators.jl_amd``.  There is
+no fallback: if the library is missing, or a call fails, an exception is
+raised.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+from .truncation import PPFilter
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "_build", "libpauliprop.so")
+
+PP_OK, PP_EARG, PP_ENOMEM, PP_EHIP, PP_EOVERFLOW, PP_EUNSUPPORTED = range(6)
+
+EXPORTS = ("pp_create", "pp_destroy", "pp_upload", "pp_download", "pp_size", "pp_capacities", "pp_evolve",
+           "pp_evolve_single", "pp_compact", "pp_scale_by_popcount", "pp_expectation_ket", "pp_check",
+           "pp_julia_powi", "pp_last_error", "pp_version")
+
+
+class PPCounters(ctypes.Structure):
+    """WindowCounters (spv_evolve.jl:149-161) summed over one call."""
+
+    _fields_ = [("rotations", ctypes.c_int64), ("terms_swept", ctypes.c_int64),
+                ("terms_created", ctypes.c_int64), ("merge_in", ctypes.c_int64),
+                ("merge_out", ctypes.c_int64), ("early_merges", ctypes.c_int64),
+                ("t_rotate", ctypes.c_double), ("t_merge", ctypes.c_double)]
+
+    def asdict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class PPError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"pauliprop error {code}: {msg}")
+        self.code = code
+
+
+class ArgumentError(ValueError):
+    """The reference's ArgumentError."""
+
+
+class DimensionMismatch(ValueError):
+    """The reference's DimensionMismatch."""
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built (run `make -C {PKG_DIR}` or __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    vp = ctypes.c_void_p
+    u64p = P(ctypes.c_uint64)
+    dp = P(ctypes.c_double)
+    i64 = ctypes.c_int64
+    L.pp_create.argtypes = [ctypes.c_int, i64, i64, P(vp)]
+    L.pp_destroy.argtypes = [vp]
+    L.pp_upload.argtypes = [vp, u64p, u64p, dp, i64, ctypes.c_int]
+    L.pp_download.argtypes = [vp, u64p, u64p, dp, i64, P(i64)]
+    L.pp_size.argtypes = [vp, P(i64)]
+    L.pp_capacities.argtypes = [vp, P(i64), P(i64)]
+    L.pp_evolve.argtypes = [vp, u64p, u64p, dp, dp, i64, i64, P(PPFilter), P(PPFilter), P(PPCounters)]
+    L.pp_evolve_single.argtypes = [vp, u64p, u64p, ctypes.c_double, ctypes.c_double]
+    L.pp_compact.argtypes = [vp, P(PPFilter)]
+    L.pp_scale_by_popcount.argtypes = [vp, ctypes.c_int, u64p, dp, dp, dp]
+    L.pp_expectation_ket.argtypes = [vp, u64p, dp]
+    L.pp_check.argtypes = [vp, P(ctypes.c_int)]
+    L.pp_julia_powi.argtypes = [ctypes.c_double, i64]
+    L.pp_julia_powi.restype = ctypes.c_double
+    L.pp_last_error.restype = ctypes.c_char_p
+    L.pp_version.restype = ctypes.c_char_p
+    for name in EXPORTS:
+        if name not in ("pp_julia_powi", "pp_last_error", "pp_version"):
+            getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def check(rc: int):
+    if rc != PP_OK:
+        msg = lib().pp_last_error().decode(errors="replace")
+        if rc == PP_EARG:
+            raise ArgumentError(msg)
+        raise PPError(rc, msg)
+
+
+def u64p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+
+
+def dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))

@@ -1,0 +1,385 @@
+"""SparsePauliVector on an MI355X: the drop-in for PauliOperators.jl's SPV path.
+
+Each method mirrors a reference method (cited) with the same argument meaning
+and error behaviour; Julia's ``f!`` is spelled ``f_``.  Coefficients are
+Float64 (the reference's ``T = Float64`` case, which all workloads use).
+The live terms stay in device memory between calls.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from bisect import bisect_left
+from typing import Iterable, List, Optional, Sequence, Tuple, Union
+
+import numpy as np
+
+from . import _lib
+from ._lib import ArgumentError, DimensionMismatch, PPCounters, check, dp, u64p
+from .basis import Ket, PauliBasis, PauliSum, word_count
+from .truncation import (AdaptiveTruncation, CoeffTruncation, CompositeTruncation, CorrectionAccumulator,
+                         MajoranaWeightTruncation, NoCorrection, NoTruncation, StochasticCoeffTruncation,
+                         StochasticSamplingTruncation, TruncationStrategy, WeightDampedTruncation,
+                         WeightTruncation, XWeightDampedTruncation, XWeightTruncation, compile_filter,
+                         is_compilable, NOFILTER)
+
+M64 = (1 << 64) - 1
+
+
+def _pack(v: int, W: int) -> np.ndarray:
+    return np.array([(v >> (64 * k)) & M64 for k in range(W)], dtype=np.uint64)
+
+
+def _unpack_rows(a: np.ndarray) -> List[int]:
+    if a.shape[1] == 1:
+        return [int(v) for v in a[:, 0]]
+    return [int(lo) | (int(hi) << 64) for lo, hi in a]
+
+
+class SparsePauliVector:
+    """SparsePauliVector{N,W,Float64} (type_SparsePauliVector.jl:69-84), device-resident."""
+
+    def __init__(self, N: int, terms=None, *, capacity_factor: float = 2.0, append_factor: float = 1.0,
+                 min_capacity: int = 16, imag_tol: float = 1e-10, _caps: Optional[Tuple[int, int]] = None):
+        """``SparsePauliVector(O::PauliSum; capacity_factor, append_factor, min_capacity, imag_tol)``
+        (type_SparsePauliVector.jl:167-188).  ``terms`` is a PauliSum/dict
+        {PauliBasis: coeff}, a list of (PauliBasis, coeff), or a tuple of word
+        arrays (z, x, c) already in key order."""
+        if capacity_factor < 1:
+            raise ArgumentError("capacity_factor must be >= 1")
+        self.N = int(N)
+        self.W = word_count(self.N)
+        arrays = None
+        if isinstance(terms, tuple) and len(terms) == 3 and isinstance(terms[0], np.ndarray):
+            arrays = terms
+            nterms = len(terms[2])
+            presorted = True
+        else:
+            items = list(terms.items()) if isinstance(terms, dict) else list(terms or [])
+            nterms = len(items)
+            presorted = False
+        live_cap = max(min_capacity, math.ceil(capacity_factor * max(nterms, 1)))
+        app_cap = max(min_capacity, math.ceil(append_factor * live_cap))
+        if _caps is not None:
+            live_cap, app_cap = _caps
+        h = ctypes.c_void_p()
+        check(_lib.lib().pp_create(self.N, live_cap, app_cap, ctypes.byref(h)))
+        self._h = h
+        if arrays is None:
+            z = np.zeros((max(nterms, 1), self.W), dtype=np.uint64)
+            x = np.zeros((max(nterms, 1), self.W), dtype=np.uint64)
+            c = np.zeros(max(nterms, 1), dtype=np.float64)
+            for i, (p, coef) in enumerate(items):
+                if isinstance(p, PauliBasis):
+                    if p.N != self.N:
+                        raise ArgumentError("qubit count mismatch")
+                    pz, px = p.z, p.x
+                else:
+                    pz, px = p
+                cc = complex(coef)
+                # type_SparsePauliVector.jl:144-149
+                if abs(cc.imag) > imag_tol * max(1.0, abs(cc)):
+                    raise ValueError(f"coefficient {coef} has an imaginary part; Float64 storage needs real input")
+                z[i] = _pack(pz, self.W)
+                x[i] = _pack(px, self.W)
+                c[i] = cc.real
+            arrays = (z[:nterms], x[:nterms], c[:nterms])
+        if nterms:
+            z, x, c = (np.ascontiguousarray(a) for a in arrays)
+            check(_lib.lib().pp_upload(self._h, u64p(z), u64p(x), dp(c), nterms, 1 if presorted else 0))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.lib().pp_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    # --- size / conversion ------------------------------------------------------------
+    def __len__(self) -> int:
+        n = ctypes.c_int64()
+        check(_lib.lib().pp_size(self._h, ctypes.byref(n)))
+        return n.value
+
+    def capacities(self) -> Tuple[int, int]:
+        a = ctypes.c_int64()
+        b = ctypes.c_int64()
+        check(_lib.lib().pp_capacities(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def arrays(self) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """Live (z, x, c) in key order: z, x as (n, W) uint64, c as float64."""
+        n = len(self)
+        z = np.zeros((max(n, 1), self.W), dtype=np.uint64)
+        x = np.zeros((max(n, 1), self.W), dtype=np.uint64)
+        c = np.zeros(max(n, 1), dtype=np.float64)
+        got = ctypes.c_int64()
+        check(_lib.lib().pp_download(self._h, u64p(z), u64p(x), dp(c), max(n, 1), ctypes.byref(got)))
+        return z[:n], x[:n], c[:n]
+
+    def terms(self) -> List[Tuple[int, int, float]]:
+        z, x, c = self.arrays()
+        return list(zip(_unpack_rows(z), _unpack_rows(x), c.tolist()))
+
+    def to_paulisum(self) -> PauliSum:
+        """``PauliSum(v)`` (type_SparsePauliVector.jl:212-221)."""
+        out = PauliSum(self.N)
+        for z, x, c in self.terms():
+            out[PauliBasis(self.N, z, x)] = c
+        return out
+
+    def copy(self) -> "SparsePauliVector":
+        """``copy(v)`` (type_SparsePauliVector.jl:249-254)."""
+        return SparsePauliVector(self.N, self.arrays(), _caps=self.capacities())
+
+    def check(self) -> bool:
+        """``check_spv`` key-order invariant (type_SparsePauliVector.jl:263-278)."""
+        ok = ctypes.c_int()
+        check(_lib.lib().pp_check(self._h, ctypes.byref(ok)))
+        if not ok.value:
+            raise RuntimeError("live keys not strictly increasing")
+        return True
+
+    def __getitem__(self, p: PauliBasis) -> float:
+        """Dict idiom (type_SparsePauliVector.jl:307-311): KeyError when absent."""
+        keys = [(z, x) for z, x, _ in self.terms()]
+        cs = [c for _, _, c in self.terms()]
+        i = bisect_left(keys, (p.z, p.x))
+        if i < len(keys) and keys[i] == (p.z, p.x):
+            return cs[i]
+        raise KeyError(p)
+
+    def get(self, p: PauliBasis, default=None):
+        try:
+            return self[p]
+        except KeyError:
+            return default
+
+    def __contains__(self, p: PauliBasis) -> bool:
+        return self.get(p) is not None
+
+    # --- evolution -----------------------------------------------------------------------
+    def _gen_arrays(self, gens: Sequence[PauliBasis]):
+        L = len(gens)
+        gz = np.zeros((max(L, 1), self.W), dtype=np.uint64)
+        gx = np.zeros((max(L, 1), self.W), dtype=np.uint64)
+        for i, g in enumerate(gens):
+            if isinstance(g, PauliBasis):
+                if g.N != self.N:
+                    raise ArgumentError("generator qubit count mismatch")
+                z, x = g.z, g.x
+            else:
+                z, x = g
+            gz[i] = _pack(z, self.W)
+            gx[i] = _pack(x, self.W)
+        return gz, gx
+
+    def evolve_(self, generators, angles, *, window: int = 1, truncation: TruncationStrategy = NoTruncation(),
+                local_truncation: TruncationStrategy = NoTruncation(),
+                correction: CorrectionAccumulator = NoCorrection(),
+                counters: Optional[PPCounters] = None) -> "SparsePauliVector":
+        """``evolve!(O, G, θ)`` (spv_evolve.jl:228-236) when given one generator and
+        angle, else ``evolve!(O, generators, angles; window, truncation,
+        local_truncation, correction, counters)`` (spv_evolve.jl:312-387)."""
+        if isinstance(generators, PauliBasis) or (isinstance(generators, tuple) and len(generators) == 2
+                                                  and isinstance(generators[0], int)):
+            gz, gx = self._gen_arrays([generators])
+            th = float(angles)
+            check(_lib.lib().pp_evolve_single(self._h, u64p(gz), u64p(gx), math.cos(th), math.sin(th)))
+            return self
+        gens = list(generators)
+        angs = [float(a) for a in angles]
+        if len(gens) != len(angs):
+            raise DimensionMismatch("generators and angles must have same length")
+        if window < 1:
+            raise ArgumentError("window must be >= 1")
+        if not is_compilable(local_truncation):
+            raise ArgumentError("local_truncation must be a deterministic (weight/coefficient) strategy "
+                                "— it runs per append")
+        if not isinstance(correction, NoCorrection):
+            raise NotImplementedError("correction accumulators are not available in this build")
+        if not is_compilable(truncation):
+            raise NotImplementedError(f"{type(truncation).__name__} at merge boundaries is not available in this build")
+        if not gens:
+            return self
+        gz, gx = self._gen_arrays(gens)
+        cv = np.array([math.cos(a) for a in angs], dtype=np.float64)
+        sv = np.array([math.sin(a) for a in angs], dtype=np.float64)
+        fs = compile_filter(truncation)
+        fl = compile_filter(local_truncation)
+        cnt = counters if counters is not None else PPCounters()
+        check(_lib.lib().pp_evolve(self._h, u64p(gz), u64p(gx), dp(cv), dp(sv), len(gens), int(window),
+                                   ctypes.byref(fs), ctypes.byref(fl), ctypes.byref(cnt)))
+        return self
+
+    def evolve(self, generators, angles, **kw) -> "SparsePauliVector":
+        """Non-mutating ``evolve`` (spv_evolve.jl:243-244, 394-396)."""
+        return self.copy().evolve_(generators, angles, **kw)
+
+    # --- truncation / clips ------------------------------------------------------------------
+    def _compact(self, f):
+        check(_lib.lib().pp_compact(self._h, ctypes.byref(f)))
+        return self
+
+    def coeff_clip_(self, thresh: float):
+        """spv_evolve.jl:13-14 — drop |c| <= thresh."""
+        return self._compact(compile_filter(CoeffTruncation(float(thresh))))
+
+    def clip_(self, thresh: float = 1e-16):
+        return self.coeff_clip_(thresh)
+
+    def weight_clip_(self, max_weight: int):
+        return self._compact(compile_filter(WeightTruncation(int(max_weight))))
+
+    def weight_damped_clip_(self, alpha: float, thresh: float):
+        return self._compact(compile_filter(WeightDampedTruncation(float(alpha), float(thresh))))
+
+    def x_weight_clip_(self, max_weight: int):
+        return self._compact(compile_filter(XWeightTruncation(int(max_weight))))
+
+    def x_weight_damped_clip_(self, alpha: float, thresh: float):
+        return self._compact(compile_filter(XWeightDampedTruncation(float(alpha), float(thresh))))
+
+    def majorana_weight_clip_(self, max_weight: int):
+        return self._compact(compile_filter(MajoranaWeightTruncation(int(max_weight))))
+
+    def truncate_(self, strategy: TruncationStrategy, corr: CorrectionAccumulator = NoCorrection()):
+        """``truncate!(v, strategy, corr)`` (truncation.jl:330-337) for compilable strategies."""
+        if not isinstance(corr, NoCorrection):
+            raise NotImplementedError("correction accumulators are not available in this build")
+        if is_compilable(strategy):
+            return self._compact(compile_filter(strategy))
+        raise NotImplementedError(f"{type(strategy).__name__} is not available in this build")
+
+    # --- channels (channels.jl:203-265) ------------------------------------------------------
+    def _mask(self, qubits) -> np.ndarray:
+        """_qubit_mask (channels.jl:14-28)."""
+        if qubits is None:
+            M = (1 << self.N) - 1
+        else:
+            qs = [qubits] if isinstance(qubits, int) else list(qubits)
+            M = 0
+            for q in qs:
+                if not (1 <= q <= self.N):
+                    raise ArgumentError(f"qubit index {q} out of range 1:{self.N}")
+                M |= 1 << (q - 1)
+        return _pack(M, self.W)
+
+    def _table(self, lam: float) -> np.ndarray:
+        return np.array([_lib.lib().pp_julia_powi(lam, k) for k in range(self.N + 1)], dtype=np.float64)
+
+    def _scale(self, kind: int, qubits, t0, t1=None, t2=None):
+        M = self._mask(qubits)
+        t1 = t1 if t1 is not None else t0
+        t2 = t2 if t2 is not None else t0
+        check(_lib.lib().pp_scale_by_popcount(self._h, kind, u64p(M), dp(t0), dp(t1), dp(t2)))
+        return self
+
+    def depolarizing_channel_(self, p: float, qubits=None):
+        if not (0 <= p <= 1):
+            raise ArgumentError(f"depolarizing parameter p={p} must be in [0,1]")
+        return self._scale(0, qubits, self._table(1 - 4 * p / 3))
+
+    def dephasing_channel_(self, p: float, qubits=None):
+        if not (0 <= p <= 1):
+            raise ArgumentError(f"dephasing parameter p={p} must be in [0,1]")
+        return self._scale(1, qubits, self._table(1 - 2 * p))
+
+    phase_flip_channel_ = dephasing_channel_
+
+    def bit_flip_channel_(self, p: float, qubits=None):
+        if not (0 <= p <= 1):
+            raise ArgumentError(f"bit_flip parameter p={p} must be in [0,1]")
+        return self._scale(2, qubits, self._table(1 - 2 * p))
+
+    def bit_phase_flip_channel_(self, p: float, qubits=None):
+        if not (0 <= p <= 1):
+            raise ArgumentError(f"bit_phase_flip parameter p={p} must be in [0,1]")
+        return self._scale(3, qubits, self._table(1 - 2 * p))
+
+    def pauli_channel_(self, pX: float, pY: float, pZ: float, qubits=None):
+        if not (pX >= 0 and pY >= 0 and pZ >= 0):
+            raise ArgumentError(f"probabilities must be non-negative (got pX={pX}, pY={pY}, pZ={pZ})")
+        if not (pX + pY + pZ <= 1 + 4 * 2.220446049250313e-16):
+            raise ArgumentError(f"pX+pY+pZ must be <= 1 (got {pX + pY + pZ})")
+        lX = 1 - 2 * (pY + pZ)
+        lY = 1 - 2 * (pX + pZ)
+        lZ = 1 - 2 * (pX + pY)
+        return self._scale(4, qubits, self._table(lX), self._table(lY), self._table(lZ))
+
+    # --- observables ------------------------------------------------------------------------------
+    def expectation_value(self, psi: Union[Ket, int] = 0) -> float:
+        """``expectation_value(v, ψ::Ket)`` (spv_evolve.jl:133-134)."""
+        k = psi.v if isinstance(psi, Ket) else int(psi)
+        ket = _pack(k & ((1 << (64 * self.W)) - 1), self.W)
+        out = ctypes.c_double()
+        check(_lib.lib().pp_expectation_ket(self._h, u64p(ket), ctypes.byref(out)))
+        return out.value
+
+
+# module-level spellings of the reference's free functions --------------------------------------
+
+def evolve_(v: SparsePauliVector, generators, angles, **kw):
+    return v.evolve_(generators, angles, **kw)
+
+
+def evolve(v: SparsePauliVector, generators, angles, **kw):
+    return v.evolve(generators, angles, **kw)
+
+
+def truncate_(v: SparsePauliVector, strategy, corr=NoCorrection()):
+    return v.truncate_(strategy, corr)
+
+
+def expectation_value(v: SparsePauliVector, psi) -> float:
+    return v.expectation_value(psi)
+
+
+def depolarizing_channel_(v, p, qubits=None):
+    return v.depolarizing_channel_(p, qubits)
+
+
+def dephasing_channel_(v, p, qubits=None):
+    return v.dephasing_channel_(p, qubits)
+
+
+phase_flip_channel_ = dephasing_channel_
+
+
+def bit_flip_channel_(v, p, qubits=None):
+    return v.bit_flip_channel_(p, qubits)
+
+
+def bit_phase_flip_channel_(v, p, qubits=None):
+    return v.bit_phase_flip_channel_(p, qubits)
+
+
+def pauli_channel_(v, pX, pY, pZ, qubits=None):
+    return v.pauli_channel_(pX, pY, pZ, qubits)
+
+
+def depolarizing_channel(v, p, qubits=None):
+    return v.copy().depolarizing_channel_(p, qubits)
+
+
+def dephasing_channel(v, p, qubits=None):
+    return v.copy().dephasing_channel_(p, qubits)
+
+
+def bit_flip_channel(v, p, qubits=None):
+    return v.copy().bit_flip_channel_(p, qubits)
+
+
+def bit_phase_flip_channel(v, p, qubits=None):
+    return v.copy().bit_phase_flip_channel_(p, qubits)
+
+
+def pauli_channel(v, pX, pY, pZ, qubits=None):
+    return v.copy().pauli_channel_(pX, pY, pZ, qubits)
+
+
+def depolarizing_p_for_weight_decay(rate: float, dt: float) -> float:
+    """channels.jl:281-282."""
+    return (3 / 4) * (1 - math.exp(-rate * dt))

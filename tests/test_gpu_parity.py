@@ -1,0 +1,268 @@
+"""GPU parity: the HIP path against the CPU oracle, bit for bit.
+
+Every comparison is exact: the surviving key set, key order and every FP64
+coefficient (compared as bit patterns).  ⟨O⟩ is compared with the exactly
+rounded sum of the (bit-identical) coefficients, to 1 ulp, and with the
+oracle's sequential sum within its condition number (1e-12 relative to sum |c|).
+"""
+import math
+import random
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from pauliprop import (CoeffTruncation, CompositeTruncation, MajoranaWeightTruncation, NoTruncation, Pauli,
+                       PauliBasis, PPCounters, SparsePauliVector, WeightDampedTruncation, WeightTruncation,
+                       XWeightDampedTruncation, XWeightTruncation, compile_filter)
+from pauliprop import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def assert_same(gpu: SparsePauliVector, orc: po.OracleSPV, what=""):
+    gz, gx, gc = gpu.arrays()
+    oz, ox, oc = orc.arrays()
+    assert gz.shape == oz.shape, f"{what}: n gpu={len(gc)} oracle={len(oc)}"
+    assert np.array_equal(gz, oz), f"{what}: z keys differ"
+    assert np.array_equal(gx, ox), f"{what}: x keys differ"
+    assert np.array_equal(gc.view(np.uint64), oc.view(np.uint64)), f"{what}: coefficients differ"
+
+
+def random_terms(N, n, rng, support=None):
+    """n distinct random (z, x, c) with bits only inside `support` (a bit mask)."""
+    support = support if support is not None else (1 << N) - 1
+    seen = set()
+    out = []
+    while len(out) < n:
+        z = rng.getrandbits(N) & support
+        x = rng.getrandbits(N) & support
+        if (z, x) in seen:
+            continue
+        seen.add((z, x))
+        out.append((z, x, rng.gauss(0, 1)))
+    return out
+
+
+def pair(N, terms, **kw):
+    g = SparsePauliVector(N, [((z, x), c) for z, x, c in terms], **kw)
+    o = po.OracleSPV(N, terms, **{k: v for k, v in kw.items()})
+    return g, o
+
+
+def spread_support(N, bits, rng):
+    return sum(1 << q for q in rng.sample(range(N), bits))
+
+
+@pytest.mark.parametrize("N", [6, 8, 16, 64, 65, 121, 128])
+def test_single_rotation_random_generators(N):
+    rng = random.Random(1000 + N)
+    for trial in range(6):
+        bits = min(N, 7)
+        sup = spread_support(N, bits, rng)
+        n = rng.choice([1, 2, 63, 64, 65, 2047, 2048, 2049, 5000, 9000])
+        n = min(n, 2 ** (2 * bits) - 1)
+        terms = random_terms(N, n, rng, sup)
+        g, o = pair(N, terms)
+        for k in range(4):
+            kind = rng.randrange(4)
+            if kind == 0:
+                gz, gx = rng.getrandbits(N) & sup, rng.getrandbits(N) & sup
+            elif kind == 1:
+                q = rng.choice([b for b in range(N) if sup >> b & 1])
+                gz, gx = 0, 1 << q
+            elif kind == 2:
+                qs = rng.sample([b for b in range(N) if sup >> b & 1], 2)
+                gz, gx = (1 << qs[0]) | (1 << qs[1]), 0
+            else:
+                gz, gx = rng.getrandbits(N), rng.getrandbits(N)
+            theta = rng.choice([0.0, 0.3, math.pi / 2, math.pi, -1.234, rng.uniform(-3, 3)])
+            log(f"N={N} trial={trial} n={len(terms)} step={k} G=({gz:x},{gx:x}) θ={theta}")
+            g.evolve_(PauliBasis(N, gz, gx), theta)
+            o.evolve_single(gz, gx, theta)
+            assert_same(g, o, f"N={N} trial={trial} step={k}")
+            g.check()
+
+
+def test_dense_small_keyspace_collisions():
+    # N = 6: 4096 possible keys, 3000 live -> most children collide with live terms
+    rng = random.Random(7)
+    N = 6
+    terms = random_terms(N, 3000, rng)
+    g, o = pair(N, terms)
+    for step in range(40):
+        gz, gx = rng.getrandbits(N), rng.getrandbits(N)
+        th = rng.uniform(-3, 3)
+        g.evolve_(PauliBasis(N, gz, gx), th)
+        o.evolve_single(gz, gx, th)
+        assert_same(g, o, f"step {step}")
+
+
+def heisenberg_gens(N, dt=0.05, nsteps=2):
+    terms = []
+    for i in range(1, N):
+        terms += [(Pauli(N, X=[i, i + 1]), 1.0), (Pauli(N, Y=[i, i + 1]), 0.7), (Pauli(N, Z=[i, i + 1]), 1.1)]
+    terms += [(Pauli(N, Z=[i]), 0.3) for i in range(1, N + 1)]
+    gens, angs = [], []
+    for _ in range(nsteps):
+        half = [(p, dt * c) for p, c in terms]
+        for p, a in half + half[::-1]:
+            gens.append(p)
+            angs.append(a)
+    return gens, angs
+
+
+STRATS = [NoTruncation(), CoeffTruncation(1e-4), WeightTruncation(3), XWeightTruncation(2),
+          MajoranaWeightTruncation(6), WeightDampedTruncation(0.5, 1e-4), XWeightDampedTruncation(0.5, 1e-4),
+          CompositeTruncation(WeightTruncation(4), CoeffTruncation(1e-6))]
+
+
+@pytest.mark.parametrize("strat", STRATS, ids=lambda s: type(s).__name__)
+def test_sequence_strategies(strat):
+    for N in (8, 70):
+        gens, angs = heisenberg_gens(N if N == 8 else 8)
+        gens = [PauliBasis(N, p.z, p.x) for p in gens]
+        O0 = [(Pauli(N, Z=[4]).z, 0, 1.0)]
+        g, o = pair(N, O0)
+        g.evolve_(gens, angs, truncation=strat)
+        o.evolve([(p.z, p.x) for p in gens], angs, strict=compile_filter(strat))
+        assert_same(g, o, f"{type(strat).__name__} N={N}")
+
+
+def test_local_truncation_and_counters():
+    N = 10
+    gens, angs = heisenberg_gens(N)
+    O0 = [(Pauli(N, Z=[5]).z, 0, 1.0)]
+    g, o = pair(N, O0)
+    cnt = PPCounters()
+    st = CoeffTruncation(1e-5)
+    loc = WeightTruncation(4)
+    g.evolve_(gens, angs, truncation=st, local_truncation=loc, counters=cnt)
+    oc = o.evolve([(p.z, p.x) for p in gens], angs, strict=compile_filter(st), local=compile_filter(loc))
+    assert_same(g, o, "local truncation")
+    assert cnt.rotations == oc.rotations
+    assert cnt.terms_swept == oc.terms_swept
+    assert cnt.terms_created == oc.terms_created
+    assert cnt.merge_out == oc.merge_out
+    assert cnt.early_merges == oc.early_merges
+
+
+def test_kicked_ising_chain16_end_to_end():
+    w = wl.kicked_ising_16()
+    gens, angs = w.flat()
+    obs = [(p.z, p.x, c) for p, c in w.observable]
+    g, o = pair(w.N, obs)
+    cnt = PPCounters()
+    g.evolve_(gens, angs, truncation=w.truncation, counters=cnt)
+    oc = o.evolve([(p.z, p.x) for p in gens], angs, strict=compile_filter(w.truncation))
+    assert_same(g, o, "kicked ising 16")
+    assert cnt.early_merges == oc.early_merges and oc.early_merges > 0
+    assert g.capacities() == o.capacities()
+    _, _, c = g.arrays()
+    zz, xx, _ = g.arrays()
+    diag = c[np.all(xx == 0, axis=1)]
+    exact = math.fsum(diag.tolist())
+    ev = g.expectation_value(0)
+    assert abs(ev - exact) <= 2 * np.spacing(abs(exact))
+    assert abs(ev - o.expectation(0)) <= 1e-12 * max(1.0, float(np.sum(np.abs(diag))))
+
+
+def run_layers(w, layers, g, o):
+    for li in range(layers):
+        L = w.layers[li]
+        g.evolve_(L.generators, L.angles, truncation=w.truncation)
+        o.evolve([(p.z, p.x) for p in L.generators], L.angles, strict=compile_filter(w.truncation))
+        if w.channel:
+            kind, p = w.channel
+            getattr(g, kind + "_channel_")(p)
+            o.channel(kind, p)
+        log(f"{w.name} layer {li}: n={len(g)}")
+        assert_same(g, o, f"{w.name} layer {li}")
+
+
+def test_chain64_first_layers():
+    w = wl.chain64()
+    obs = [(p.z, p.x, c) for p, c in w.observable]
+    g, o = pair(w.N, obs)
+    run_layers(w, 4, g, o)
+    ev = g.expectation_value(0)
+    assert abs(ev - o.expectation(0)) <= 1e-12
+
+
+def test_lattice121_first_layers():
+    w = wl.lattice121(wl.theta_h_values()[9], n_layers=3)
+    obs = [(p.z, p.x, c) for p, c in w.observable]
+    g, o = pair(w.N, obs)
+    run_layers(w, 3, g, o)
+    assert abs(g.expectation_value(0) - o.expectation(0)) <= 1e-12
+
+
+@pytest.mark.parametrize("kind", ["depolarizing", "dephasing", "bit_flip", "bit_phase_flip", "pauli"])
+def test_channels_bitexact(kind):
+    rng = random.Random(11)
+    for N in (5, 64, 100):
+        terms = random_terms(N, 3000, rng, spread_support(N, 8, rng))
+        g, o = pair(N, terms)
+        qs = sorted(rng.sample(range(1, N + 1), 3))
+        mask = sum(1 << (q - 1) for q in qs)
+        if kind == "pauli":
+            g.pauli_channel_(0.05, 0.1, 0.15, qubits=qs)
+            o.channel("pauli", pX=0.05, pY=0.1, pZ=0.15, mask=mask)
+            g.pauli_channel_(0.01, 0.02, 0.03)
+            o.channel("pauli", pX=0.01, pY=0.02, pZ=0.03)
+        else:
+            getattr(g, kind + "_channel_")(0.13, qubits=qs)
+            o.channel(kind, 0.13, mask=mask)
+            getattr(g, kind + "_channel_")(0.07)
+            o.channel(kind, 0.07)
+        assert_same(g, o, f"{kind} N={N}")
+
+
+def test_clips_bitexact():
+    rng = random.Random(12)
+    for N in (8, 90):
+        terms = random_terms(N, 5000, rng, spread_support(N, 8, rng))
+        for strat in STRATS[1:]:
+            g, o = pair(N, terms)
+            g.truncate_(strat)
+            o.compact(compile_filter(strat))
+            assert_same(g, o, f"{type(strat).__name__} N={N}")
+
+
+def test_edge_cases():
+    N = 12
+    # empty vector: evolution and expectation are no-ops
+    g = SparsePauliVector(N, [])
+    g.evolve_([Pauli(N, X=[1])], [0.4], truncation=CoeffTruncation(1e-6))
+    assert len(g) == 0 and g.expectation_value(0) == 0.0
+    # identity generator and commuting generator: values unchanged
+    terms = [(Pauli(N, Z=[2]).z, 0, 0.5), (Pauli(N, Z=[3]).z, 0, -0.25)]
+    g, o = pair(N, terms)
+    g.evolve_(PauliBasis(N, 0, 0), 0.7)
+    g.evolve_(Pauli(N, Z=[1, 4]), 0.7)
+    o.evolve_single(0, 0, 0.7)
+    p = Pauli(N, Z=[1, 4])
+    o.evolve_single(p.z, p.x, 0.7)
+    assert_same(g, o, "commuting")
+    # everything truncated away
+    g.evolve_([Pauli(N, X=[2])], [0.3], truncation=CoeffTruncation(10.0))
+    assert len(g) == 0
+
+
+def test_tight_capacity_early_merges():
+    N = 8
+    gens, angs = heisenberg_gens(N)
+    O0 = [(Pauli(N, Z=[4]).z, 0, 1.0)]
+    g, o = pair(N, O0, append_factor=0.01, min_capacity=4)
+    cnt = PPCounters()
+    st = CoeffTruncation(1e-4)
+    g.depolarizing_channel_(0.2)
+    o.channel("depolarizing", 0.2)
+    g.evolve_(gens, angs, truncation=st, counters=cnt)
+    oc = o.evolve([(p.z, p.x) for p in gens], angs, strict=compile_filter(st))
+    assert_same(g, o, "tight capacity")
+    assert cnt.early_merges == oc.early_merges > 0
