@@ -158,6 +158,7 @@ struct pp_handle {
     // reference capacity bookkeeping (type_SparsePauliVector.jl:69-126)
     int64_t live_cap_v = 16, app_cap_v = 16;
     int64_t n = 0;  // live count (host copy)
+    double m_ratio = 1.0;  // children per live term in the last rotation
 
     ~pp_handle() {
         for (int s = 0; s < 2; ++s) { dfree(L[s].z); dfree(L[s].x); dfree(L[s].c); }
@@ -184,30 +185,32 @@ struct pp_handle {
         s = DevSet();
     }
 
-    // Live ping-pong buffers hold at least `need` terms; the current live set survives.
-    void ensure_live(int64_t need) {
-        if (L[0].cap >= need && L[1].cap >= need) return;
-        const int64_t cap = std::max<int64_t>(need, 2 * std::max(L[0].cap, L[1].cap));
-        DevSet nw[2];
-        alloc_set(nw[0], cap);
-        alloc_set(nw[1], cap);
-        if (n > 0) {
-            HIPCHK(hipMemcpyAsync(nw[cur].z, L[cur].z, (size_t)n * key_bytes(), hipMemcpyDeviceToDevice, stream));
-            HIPCHK(hipMemcpyAsync(nw[cur].x, L[cur].x, (size_t)n * key_bytes(), hipMemcpyDeviceToDevice, stream));
-            HIPCHK(hipMemcpyAsync(nw[cur].c, L[cur].c, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, stream));
-            HIPCHK(hipStreamSynchronize(stream));
+    // Grow one buffer set to hold `need` terms (+25% headroom).  keep > 0
+    // copies that many leading terms across; otherwise the old contents are
+    // dropped first, which keeps the peak footprint at one copy.
+    void grow_set(DevSet& s, int64_t need, int64_t keep) {
+        if (s.cap >= need) return;
+        const int64_t cap = std::max<int64_t>(need + need / 4, 1024);
+        if (keep <= 0) {
+            free_set(s);
+            alloc_set(s, cap);
+            return;
         }
-        free_set(L[0]);
-        free_set(L[1]);
-        L[0] = nw[0];
-        L[1] = nw[1];
+        DevSet nw;
+        alloc_set(nw, cap);
+        HIPCHK(hipMemcpyAsync(nw.z, s.z, (size_t)keep * key_bytes(), hipMemcpyDeviceToDevice, stream));
+        HIPCHK(hipMemcpyAsync(nw.x, s.x, (size_t)keep * key_bytes(), hipMemcpyDeviceToDevice, stream));
+        HIPCHK(hipMemcpyAsync(nw.c, s.c, (size_t)keep * sizeof(double), hipMemcpyDeviceToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        free_set(s);
+        s = nw;
     }
 
-    void ensure_children(int64_t need) {
-        if (C.cap >= need) return;
-        free_set(C);
-        alloc_set(C, std::max<int64_t>(need, 2 * C.cap));
-    }
+    // Current live buffer holds `need` terms (keeping the live set); the
+    // merge target is sized separately per call.
+    void ensure_live(int64_t need) { grow_set(L[cur], need, n); }
+    void ensure_target(int64_t need) { grow_set(L[cur ^ 1], need, 0); }
+    void ensure_children(int64_t need) { grow_set(C, need, 0); }
 
     void ensure_scratch(int64_t nlive, int nlev) {
         const int64_t words = cdiv(std::max<int64_t>(nlive, 1), 64) + 1;
@@ -276,15 +279,17 @@ struct pp_handle {
         return (int)std::min<int64_t>(need, (int64_t)ncu * max_per_cu);
     }
 
-    // Read back the live count written by the last merge (synchronises).
-    int64_t sync_count(int idx) {
+    // Read back the live count written by the last merge, the child count and
+    // the overflow flag (synchronises).  Internal consistency failures throw.
+    int64_t sync_count(int idx, int32_t* ovf) {
         HIPCHK(hipMemcpyAsync(h_pin, d_cnt + idx, sizeof(int64_t), hipMemcpyDeviceToHost, stream));
         HIPCHK(hipMemcpyAsync(h_pin + 1, d_cnt + 2, sizeof(int64_t), hipMemcpyDeviceToHost, stream));
         HIPCHK(hipMemcpyAsync(h_pin + 2, d_ovf, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
         const int32_t* flags3 = (const int32_t*)(h_pin + 2);
-        if (flags3[2] != 0) throw PPError(PP_EHIP, "internal consistency check failed in a kernel (bits " + std::to_string(flags3[2]) + ")");
-        if (flags3[0] != 0) throw PPError(PP_EOVERFLOW, "device buffer overflow");
+        if (flags3[2] != 0)
+            throw PPError(PP_EHIP, "internal consistency check failed in a kernel (bits " + std::to_string(flags3[2]) + ")");
+        *ovf = flags3[0];
         return h_pin[0];
     }
 
@@ -296,7 +301,7 @@ struct pp_handle {
         const int64_t threads = ntm_ub + 1;
         ChanDev nochan{};
         hipLaunchKernelGGL(k_partition<Wt>, dim3((unsigned)cdiv(threads, kBlock)), dim3(kBlock), 0, stream, src.z,
-                           src.x, d_n, ch.z, ch.x, d_m, split, status, ticket, d_nout, threads);
+                           src.x, d_n, ch.z, ch.x, d_m, split, status, ticket, d_nout, threads, d_ovf);
         const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(ntm_ub, (int64_t)ncu * 4));
         if (timed) HIPCHK(hipEventRecord(ev[2], stream));
         hipLaunchKernelGGL(k_merge<Wt>, dim3(grid), dim3(kBlock), 0, stream, src.z, src.x, src.c, d_n, ch.z, ch.x,
@@ -306,17 +311,56 @@ struct pp_handle {
     }
 
     // One rotation at window = 1 on the current live set (host count n).
+    // Buffers are sized from the previous rotation's child ratio; if the
+    // children or the merged output do not fit, the kernels raise the
+    // overflow flag and write nothing out of bounds, the live set is
+    // untouched, and the rotation is re-run with exact sizes (the reference's
+    // grow-and-retry merge, spv_kernels.jl:377-383).
     template <int Wt>
     void rotate(const GenDev* dg, const GenDev& hg, int local_active, bool use_strict, pp_counters* cnt) {
         ensure_scratch(n, hg.nlev);
-        ensure_live(2 * n + 1);
-        ensure_children(n + 1);
+        const int64_t m_est = std::min<int64_t>(n, (int64_t)(m_ratio * (double)n * 1.15) + 4096);
+        ensure_children(m_est + 1);
+        ensure_target(n + m_est + 1);
+        int64_t n_out = 0, m = 0;
+        for (int attempt = 0;; ++attempt) {
+            launch_rotation<Wt>(dg, hg, local_active, use_strict, cnt != nullptr);
+            int32_t ovf = 0;
+            n_out = sync_count(cur ^ 1, &ovf);
+            m = h_pin[1];
+            if (!ovf) break;
+            if (attempt >= 2) throw PPError(PP_EOVERFLOW, "merge overflow persisted after regrowth");
+            HIPCHK(hipMemsetAsync(d_ovf, 0, sizeof(int32_t), stream));
+            ensure_children(m + 1);
+            ensure_target(n + m + 1);
+        }
+        if (cnt) {
+            float a = 0.f, b = 0.f;
+            HIPCHK(hipEventElapsedTime(&a, ev[0], ev[1]));
+            HIPCHK(hipEventElapsedTime(&b, ev[2], ev[3]));
+            cnt->t_rotate += a * 1e-3;
+            cnt->t_merge += b * 1e-3;
+            cnt->rotations += 1;
+            cnt->terms_swept += n;
+            cnt->terms_created += m;
+            cnt->merge_in += n + m;
+            cnt->merge_out += n_out;
+        }
+        if (n > 0 && hg.nlev > 0) m_ratio = (double)m / (double)n;
+        // _merge_spv! growth of the reference's live buffers (spv_kernels.jl:377-383)
+        if (n_out > live_cap_v) live_cap_v = std::max(2 * live_cap_v, n + m);
+        cur ^= 1;
+        n = n_out;
+    }
+
+    template <int Wt>
+    void launch_rotation(const GenDev* dg, const GenDev& hg, int local_active, bool use_strict, bool timed) {
         DevSet& src = L[cur];
         DevSet& dst = L[cur ^ 1];
         const int64_t* d_n = d_cnt + cur;
         int64_t* d_nout = d_cnt + (cur ^ 1);
         const FilterDev* fstrict = d_filt + (use_strict ? 0 : 2);
-        if (cnt) HIPCHK(hipEventRecord(ev[0], stream));
+        if (timed) HIPCHK(hipEventRecord(ev[0], stream));
         if (hg.nlev > 0 && n > 0) {
             const int64_t ntiles = cdiv(n, kTile);
             const int gt = (int)std::min<int64_t>(ntiles, (int64_t)ncu * 8);
@@ -332,39 +376,23 @@ struct pp_handle {
         } else {
             HIPCHK(hipMemsetAsync(d_cnt + 2, 0, sizeof(int64_t), stream));
         }
-        if (cnt) HIPCHK(hipEventRecord(ev[1], stream));
-        launch_merge<Wt>(dg, fstrict, src, d_n, C, d_cnt + 2, hg.nlev > 0 ? n : 0, dst, d_nout, cnt != nullptr);
-        if (cnt) HIPCHK(hipEventRecord(ev[3], stream));
-        const int64_t n_out = sync_count(cur ^ 1);
-        const int64_t m = h_pin[1];
-        if (cnt) {
-            float a = 0.f, b = 0.f;
-            HIPCHK(hipEventElapsedTime(&a, ev[0], ev[1]));
-            HIPCHK(hipEventElapsedTime(&b, ev[2], ev[3]));
-            cnt->t_rotate += a * 1e-3;
-            cnt->t_merge += b * 1e-3;
-            cnt->rotations += 1;
-            cnt->terms_swept += n;
-            cnt->terms_created += m;
-            cnt->merge_in += n + m;
-            cnt->merge_out += n_out;
-        }
-        // _merge_spv! growth of the reference's live buffers (spv_kernels.jl:377-383)
-        if (n_out > live_cap_v) live_cap_v = std::max(2 * live_cap_v, n + m);
-        cur ^= 1;
-        n = n_out;
+        if (timed) HIPCHK(hipEventRecord(ev[1], stream));
+        launch_merge<Wt>(dg, fstrict, src, d_n, C, d_cnt + 2, hg.nlev > 0 ? n : 0, dst, d_nout, timed);
+        if (timed) HIPCHK(hipEventRecord(ev[3], stream));
     }
 
     template <int Wt>
     void compact(const FilterDev* f) {
         ensure_scratch(n, 0);
-        ensure_live(n + 1);
+        ensure_target(n + 1);
         DevSet empty;
         empty.z = C.z ? C.z : L[cur].z;
         empty.x = C.x ? C.x : L[cur].x;
         empty.c = C.c ? C.c : L[cur].c;
         launch_merge<Wt>(nullptr, f, L[cur], d_cnt + cur, empty, d_cnt + 3, 0, L[cur ^ 1], d_cnt + (cur ^ 1));
-        const int64_t n_out = sync_count(cur ^ 1);
+        int32_t ovf = 0;
+        const int64_t n_out = sync_count(cur ^ 1, &ovf);
+        if (ovf) throw PPError(PP_EOVERFLOW, "compaction overflow");
         cur ^= 1;
         n = n_out;
     }
@@ -518,6 +546,7 @@ int pp_upload(pp_handle* h, const uint64_t* z, const uint64_t* x, const double* 
             throw PPError(PP_EARG, sorted ? "keys not strictly increasing" : "duplicate keys in upload");
     h->n = 0;
     h->ensure_live(std::max<int64_t>(n, 1));
+    h->m_ratio = 1.0;
     DevSet& s = h->L[h->cur];
     if (n > 0) {
         HIPCHK(hipMemcpyAsync(s.z, hz.data(), (size_t)n * W * 8, hipMemcpyHostToDevice, h->stream));

@@ -631,8 +631,12 @@ __global__ void __launch_bounds__(kBlock) k_partition(const uint64_t* __restrict
                                                       const uint64_t* __restrict__ Cx, const int64_t* __restrict__ d_m,
                                                       int64_t* __restrict__ split, uint64_t* __restrict__ status,
                                                       uint32_t* __restrict__ ticket, int64_t* __restrict__ d_nout,
-                                                      int64_t max_threads) {
+                                                      int64_t max_threads, const int32_t* __restrict__ d_ovf) {
     constexpr int TM = MergeTile<W>::kSize;
+    // The child buffer overflowed (m > its capacity): C holds fewer than m
+    // valid terms, so no split may be searched; k_merge skips as well and the
+    // host re-runs the rotation with a larger buffer.
+    if (*d_ovf) return;
     const int64_t n = *d_n, m = *d_m;
     const int64_t tot = n + m;
     const int64_t ntm = (tot + TM - 1) / TM;
@@ -815,36 +819,57 @@ __global__ void __launch_bounds__(kBlock) k_merge(const uint64_t* __restrict__ L
             total += s_wsum[w];
         }
         const int myoff = wpref + incl - cnt;
-        // decoupled look-back for the tile's global base
-        if (tid == 0) {
-            int64_t base = 0;
+        // Decoupled look-back for the tile's global base, one wave scanning 64
+        // predecessors per step.  Each status word carries its own payload
+        // (flag | count), so relaxed atomics suffice: no other memory is
+        // published through it, and no cache write-back/invalidate is needed.
+        if (wave == 0) {
             if (t == 0) {
-                __hip_atomic_store(&status[0], kFlagPre | (uint64_t)total, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                if (lane == 0) {
+                    __hip_atomic_store(&status[0], kFlagPre | (uint64_t)total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    s_base = 0;
+                }
             } else {
-                __hip_atomic_store(&status[t], kFlagAgg | (uint64_t)total, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                int64_t p = t - 1;
+                if (lane == 0)
+                    __hip_atomic_store(&status[t], kFlagAgg | (uint64_t)total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                int64_t excl = 0;
+                int64_t top = t - 1;
                 int64_t spins = 0;
                 while (true) {
-                    const uint64_t s = __hip_atomic_load(&status[p], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                    const int64_t q = top - lane;
+                    const uint64_t s = q >= 0 ? __hip_atomic_load(&status[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                              : kFlagPre;  // before tile 0: prefix 0
                     const uint64_t flag = s & ~kValMask;
-                    if (flag == 0) {
-                        if (++spins > (1ll << 24)) {  // bounded wait: report instead of hanging
-                            atomicOr(d_err, 4);
+                    const uint64_t pre = __ballot(flag == kFlagPre);
+                    const uint64_t notready = __ballot(flag == 0);
+                    const int first = pre ? __ffsll((unsigned long long)pre) - 1 : kWave;
+                    const uint64_t need = first >= kWave - 1 ? ~0ull : ((1ull << (first + 1)) - 1);
+                    if (notready & need) {
+                        if (++spins > (1ll << 22)) {  // bounded wait: report instead of hanging
+                            if (lane == 0) atomicOr(d_err, 4);
                             break;
                         }
                         __builtin_amdgcn_s_sleep(1);
                         continue;
                     }
-                    base += (int64_t)(s & kValMask);
-                    if (flag == kFlagPre) break;
-                    --p;
+                    int64_t v = lane <= first ? (int64_t)(s & kValMask) : 0;
+#pragma unroll
+                    for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kWave);
+                    excl += v;
+                    if (first < kWave) break;
+                    top -= kWave;
                 }
-                __hip_atomic_store(&status[t], kFlagPre | (uint64_t)(base + total), __ATOMIC_RELEASE,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+                if (lane == 0) {
+                    __hip_atomic_store(&status[t], kFlagPre | (uint64_t)(excl + total), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                    s_base = excl;
+                }
             }
-            s_base = base;
-            if (t == ntm - 1) *d_nout = base + total;
-            if (base + total > capOut) *d_ovf = 1;
+            if (lane == 0) {
+                const int64_t b = s_base;
+                if (t == ntm - 1) *d_nout = b + total;
+                if (b + total > capOut) *d_ovf = 1;
+            }
         }
         __syncthreads();
         const int64_t base = s_base;

@@ -205,7 +205,8 @@ def test_lattice121_first_layers():
 def test_channels_bitexact(kind):
     rng = random.Random(11)
     for N in (5, 64, 100):
-        terms = random_terms(N, 3000, rng, spread_support(N, 8, rng))
+        bits = min(N, 8)
+        terms = random_terms(N, min(3000, 4 ** bits // 2), rng, spread_support(N, bits, rng))
         g, o = pair(N, terms)
         qs = sorted(rng.sample(range(1, N + 1), 3))
         mask = sum(1 << (q - 1) for q in qs)
@@ -266,3 +267,29 @@ def test_tight_capacity_early_merges():
     oc = o.evolve([(p.z, p.x) for p in gens], angs, strict=compile_filter(st))
     assert_same(g, o, "tight capacity")
     assert cnt.early_merges == oc.early_merges > 0
+
+
+def test_child_buffer_regrowth_after_ratio_jump():
+    # Rotation 1 anticommutes with ~1% of the terms, so the child buffer is
+    # sized small for rotation 2, which anticommutes with every term: the
+    # overflow must be detected and the rotation re-run with exact sizes.
+    rng = random.Random(21)
+    N = 40
+    q, r = 3, 17
+    terms, seen = [], set()
+    while len(terms) < 300000:
+        z = rng.getrandbits(N) & ~(1 << q)
+        x = rng.getrandbits(N) | (1 << r)
+        if rng.random() < 0.01:
+            z |= 1 << q
+        if (z, x) in seen:
+            continue
+        seen.add((z, x))
+        terms.append((z, x, rng.gauss(0, 1)))
+    g, o = pair(N, terms)
+    steps = [(0, 1 << q, 0.4), (1 << r, 0, 0.7), (1 << (q + 1), 1 << (q + 1), 0.2)]
+    for gz, gx, th in steps:
+        log(f"regrowth step G=({gz:x},{gx:x}) n={len(g)}")
+        g.evolve_(PauliBasis(N, gz, gx), th)
+        o.evolve_single(gz, gx, th)
+        assert_same(g, o, f"regrowth G=({gz:x},{gx:x})")
