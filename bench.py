@@ -1,21 +1,23 @@
 #!/usr/bin/env python3
-"""Benchmark: Pauli terms/s per gate sweep on the 64-qubit chain workload.
+"""Benchmark: Pauli terms/s per gate sweep on the 121-qubit lattice sweep.
 
-Workload (one step): 64-qubit open chain, 30 layers; each layer RZZ on the 63
-bonds, then RX and RZ on the 64 sites, angles i.i.d. U[0, π) from seed 1234;
-depolarizing p = 1e-3 after each layer; truncation |c| <= 1e-6 and weight
-<= 16 after every rotation; observable Z_32; FP64 coefficients.  The state
-starts resident on the GPU; the timed region is the full evolution.
+Workload (one step): 121-qubit 11x11 open square lattice; 20 Trotter layers of
+ZZ(θ_J = -π/2) on the 220 nearest-neighbour bonds then X(θ_h) on all sites;
+θ_h swept over 16 values k (π/2)/15, k = 0..15, each an independent evolution;
+dephasing p = 5e-4 per qubit after each layer; truncation |c| <= 1e-4 and
+weight <= 12 after every rotation; observable (1/121) Σ_i Z_i; FP64.  The 16
+initial operators are resident on the GPU when the timed region starts; the
+16 evolutions run as one batch (one kernel sequence per rotation step for
+all of them).
 
-value = sum over rotations of the live terms entering the rotation / wall
+value = Σ over all rotations of the live terms entering the rotation / wall
 time of the step.  The JSON line also carries the merge kernel's roofline
-(algorithmic bytes = 24 B x (terms in + children + terms out) per launch,
-divided by its HIP-event time) and a CPU baseline: the single-threaded
-oracle on a bounded sample of the same workload.
+(algorithmic bytes per launch = 40 B x (terms in + children + terms out)
+over its HIP-event time) and the CPU baseline: the single-threaded oracle
+(a C restatement of the reference algorithm) on the same sweep.
 """
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -30,46 +32,50 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_workload(w, layers, counters_cls, spv_cls):
-    obs = [((p.z, p.x), c) for p, c in w.observable]
-    v = spv_cls(w.N, obs)
-    tot = counters_cls()
+def sweep_gpu(workloads, timing=True):
+    """All θ_h instances in one batch: per layer one evolve! per instance, then
+    the dephasing channel.  Returns (seconds, total counters, <O> list, merge
+    kernel seconds, merge launches)."""
+    from pauliprop import PPCounters, SparsePauliBatch
+    bt = SparsePauliBatch([w.N for w in workloads],
+                          [[((p.z, p.x), c) for p, c in w.observable] for w in workloads])
+    bt.set_timing(timing)
+    tot = PPCounters()
+    nl = len(workloads[0].layers)
     t0 = time.perf_counter()
-    for li in range(layers):
-        L = w.layers[li]
-        cnt = counters_cls()
-        v.evolve_(L.generators, L.angles, truncation=w.truncation, counters=cnt)
-        if w.channel:
-            kind, p = w.channel
-            getattr(v, kind + "_channel_")(p)
-        for f, _ in cnt._fields_:
-            setattr(tot, f, getattr(tot, f) + getattr(cnt, f))
-        log(f"  layer {li}: n={len(v)} swept={cnt.terms_swept} t={time.perf_counter() - t0:.2f}s")
-    ev = v.expectation_value(0)
-    return time.perf_counter() - t0, tot, ev, len(v)
+    for li in range(nl):
+        cnts = bt.evolve_([w.layers[li].generators for w in workloads], [w.layers[li].angles for w in workloads],
+                          truncation=workloads[0].truncation, counters=True)
+        bt.channel_(workloads[0].channel[0], workloads[0].channel[1])
+        for c in cnts:
+            for f, _ in c._fields_:
+                setattr(tot, f, getattr(tot, f) + getattr(c, f))
+    evs = bt.expectation_values()
+    dt = time.perf_counter() - t0
+    tm, nm = bt.merge_time()
+    return dt, tot, evs, tm, nm
 
 
-def cpu_baseline(w, budget_s=20.0):
-    """Single-threaded oracle on the first layers of the same workload, stopping
-    once the budget is spent (whole layers only)."""
+def sweep_cpu(workloads, budget_s):
     import pyoracle as po
     from pauliprop import compile_filter
-    o = po.OracleSPV(w.N, [(p.z, p.x, c) for p, c in w.observable])
-    f = compile_filter(w.truncation)
     swept = 0
+    done = 0
     t0 = time.perf_counter()
-    layers = 0
-    for L in w.layers:
-        c = o.evolve([(p.z, p.x) for p in L.generators], L.angles, strict=f)
-        if w.channel:
-            kind, p = w.channel
-            o.channel(kind, p)
-        swept += c.terms_swept
-        layers += 1
+    evs = []
+    for w in workloads:
+        o = po.OracleSPV(w.N, [(p.z, p.x, c) for p, c in w.observable])
+        f = compile_filter(w.truncation)
+        for L in w.layers:
+            c = o.evolve([(p.z, p.x) for p in L.generators], L.angles, strict=f)
+            o.channel(w.channel[0], w.channel[1])
+            swept += c.terms_swept
+        evs.append(o.expectation(0))
+        done += 1
         if time.perf_counter() - t0 > budget_s:
             break
     dt = time.perf_counter() - t0
-    return swept / dt, layers, swept, dt
+    return swept / dt, done, swept, dt, evs
 
 
 def main():
@@ -77,8 +83,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--layers", type=int, default=30)
-    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -86,40 +91,45 @@ def main():
               "(the sharding design is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
-    from pauliprop import PPCounters, SparsePauliVector
+    from pauliprop import PPCounters
     from pauliprop import workloads as wl
 
-    w = wl.chain64(n_layers=30)
-    layers = min(args.layers, 30)
-    S = 24  # bytes per term: 2 x 8 B keys + 8 B coefficient (N <= 64)
+    ths = wl.theta_h_values()
+    workloads = [wl.lattice121(t) for t in ths]
+    S = 40  # bytes per term: 2 x 16 B keys + 8 B coefficient (65 <= N <= 128)
 
     for i in range(args.warmup):
         log(f"warmup {i}")
-        run_workload(w, min(layers, 8), PPCounters, SparsePauliVector)
+        sweep_gpu(workloads, timing=False)
 
     tot_time = 0.0
     agg = PPCounters()
-    ev = None
-    nfin = 0
+    evs = None
+    t_merge = 0.0
+    n_merge = 0
     for i in range(args.steps):
         log(f"step {i}")
-        dt, cnt, ev, nfin = run_workload(w, layers, PPCounters, SparsePauliVector)
+        dt, cnt, evs, tm, nm = sweep_gpu(workloads)
+        log(f"  {dt:.3f} s, {cnt.terms_swept} term-rotations, {cnt.rotations} rotations, "
+            f"merge kernel {tm:.3f} s over {nm} launches")
         tot_time += dt
+        t_merge += tm
+        n_merge += nm
         for f, _ in cnt._fields_:
             setattr(agg, f, getattr(agg, f) + getattr(cnt, f))
 
     value = agg.terms_swept / tot_time
     merge_bytes = S * (agg.merge_in + agg.merge_out)
-    achieved = merge_bytes / agg.t_merge / 1e9 if agg.t_merge > 0 else None
-    rot_bytes = S * (agg.terms_swept + agg.merge_out)
-    rot_gbs = rot_bytes / (agg.t_rotate + agg.t_merge) / 1e9 if agg.t_merge > 0 else None
+    achieved = merge_bytes / t_merge / 1e9 if t_merge > 0 else None
 
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle, 1 thread)")
-        cv, cl, cs, cdt = cpu_baseline(w, args.cpu_budget)
+        cv, done, cs, cdt, cevs = sweep_cpu(workloads, args.cpu_budget)
         cpu = {"value": cv, "unit": "terms/s", "cores": 1, "kind": "port",
-               "sample": f"chain64 layers 1-{cl} from the start ({cs} term-rotations, {cdt:.1f} s)"}
+               "sample": f"the same sweep, first {done} of 16 theta_h values ({cs} term-rotations, {cdt:.1f} s)"}
+        for a, b in zip(evs, cevs):
+            assert abs(a - b) <= 1e-12, ("<O> mismatch vs oracle", a, b)
 
     out = {
         "metric": "Pauli terms/s per gate sweep",
@@ -134,12 +144,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "chain64_depol: 64-qubit open chain, 30 layers RZZ+RX+RZ, seed 1234, "
-                               "depolarizing 1e-3, |c|<=1e-6, weight<=16, observable Z_32",
-                   "layers": layers, "rotations": agg.rotations, "final_terms": nfin, "expectation": ev},
+        "config": {"workload": "lattice121_sweep: 121-qubit 11x11 open lattice, 20 layers ZZ(-pi/2) on 220 bonds "
+                               "+ X(theta_h) on 121 sites, 16 theta_h in [0, pi/2], dephasing 5e-4, |c|<=1e-4, "
+                               "weight<=12, observable mean Z",
+                   "rotations": agg.rotations, "term_rotations": agg.terms_swept,
+                   "expectation_by_theta_h": evs},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
-                     "kernel": "k_merge<1>", "rotation_gbs": rot_gbs},
+                     "kernel": "kb_merge<2>", "launches": n_merge,
+                     "avg_launch_us": 1e6 * t_merge / max(n_merge, 1),
+                     "bytes_per_launch": merge_bytes / max(n_merge, 1)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

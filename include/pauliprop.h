@@ -105,6 +105,40 @@ int pp_expectation_ket(pp_handle* h, const uint64_t* ket, double* out);
  * if the live keys are strictly increasing. */
 int pp_check(pp_handle* h, int* ok);
 
+/* ------------------------------------------------------------------------
+ * Batches: B independent SparsePauliVectors (qubit counts N[b] <= 128, one
+ * word count for the batch) evolved together, one kernel sequence per
+ * rotation step for all of them.  This has no counterpart in the reference,
+ * which evolves one vector per call; each vector keeps exactly the semantics
+ * of the single-vector calls above (one evolve! call per vector per
+ * pp_batch_evolve, with the reference's capacity bookkeeping per vector).
+ * live_cap / append_cap as in pp_create, per vector.
+ */
+typedef struct pp_batch pp_batch;
+int pp_batch_create(int B, const int* N, const int64_t* live_cap, const int64_t* append_cap, pp_batch** out);
+int pp_batch_destroy(pp_batch* p);
+/* Words per key of this batch (1 or 2). */
+int pp_batch_words(pp_batch* p, int* W);
+int pp_batch_upload(pp_batch* p, int b, const uint64_t* z, const uint64_t* x, const double* c, int64_t n, int sorted);
+int pp_batch_download(pp_batch* p, int b, uint64_t* z, uint64_t* x, double* c, int64_t cap, int64_t* n);
+/* n[b] = live terms of vector b. */
+int pp_batch_sizes(pp_batch* p, int64_t* n);
+/* nsteps rotation steps; step s of vector b uses generator gz/gx[(s*B+b)*W ..]
+ * and cosv/sinv[s*B+b]; active (may be NULL = all) switches a vector off for
+ * a step.  Per vector: strict[b] and local[b] (local may be NULL) as in
+ * pp_evolve; counters (may be NULL) receive B records. */
+int pp_batch_evolve(pp_batch* p, int64_t nsteps, const uint64_t* gz, const uint64_t* gx, const double* cosv,
+                    const double* sinv, const uint8_t* active, const pp_filter* strict, const pp_filter* local,
+                    pp_counters* counters);
+/* Per-vector channel as pp_scale_by_popcount: kind[b] in -1 (none) .. 4,
+ * mask[b*W ..], tables[(b*3 + t)*129 + k] = table t, entry k. */
+int pp_batch_scale_by_popcount(pp_batch* p, const int* kind, const uint64_t* mask, const double* tables);
+/* out[b] = ⟨ψ_b|v_b|ψ_b⟩ for kets[b*W ..]. */
+int pp_batch_expectation_ket(pp_batch* p, const uint64_t* kets, double* out);
+/* HIP-event timing of every merge-kernel launch (off by default). */
+int pp_batch_set_timing(pp_batch* p, int on);
+int pp_batch_merge_time(pp_batch* p, double* seconds, int64_t* launches, int reset);
+
 /* Julia's Float64^Int (compensated power by squaring, base/math.jl), for
  * callers that are not Julia and want the same channel tables. */
 double pp_julia_powi(double x, int64_t n);

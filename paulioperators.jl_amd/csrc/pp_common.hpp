@@ -39,4 +39,52 @@ struct FilterDev {
     double exptab_xw[kExpTab];
 };
 
+// Per-instance device buffers.  A batch holds B independent SparsePauliVectors
+// of the same word count; the single-vector API is a batch of one.
+struct SegDesc {
+    uint64_t* Lz[2];      // live terms, ping-pong (W words per term)
+    uint64_t* Lx[2];
+    double* Lc[2];
+    uint64_t* Cz;         // children of the current rotation, in key order
+    uint64_t* Cx;
+    double* Cc;
+    uint64_t* flags;      // one bit per live term
+    uint16_t* wpre;       // in-tile exclusive prefix per flag word
+    int32_t* tilecnt;     // flagged terms per sweep tile
+    int64_t* tileoff;     // exclusive scan of tilecnt (ntiles + 1 entries)
+    int64_t* bnd;         // boundary segment counts, 4 per (tile, level)
+    int64_t* split;       // merge-path splits
+    uint64_t* status;     // merge look-back words
+    const struct FilterDev* fs;  // strict filter
+    const struct FilterDev* fl;  // local filter
+    int64_t cap;          // capacity of each live buffer (terms)
+    int64_t capC;         // capacity of the child buffer (terms)
+    int32_t maxlev;       // levels per tile held by bnd
+    int32_t local_active; // local filter can drop children
+};
+
+// Per-instance device state, advanced by the kernels without host round trips.
+struct SegState {
+    int64_t n[2];         // live counts of Lz[0] / Lz[1]
+    int64_t m;            // children of the current rotation
+    int64_t app_cap_v;    // the reference's append-buffer length (decides early merges)
+    int64_t live_cap_v;   // the reference's live-buffer length
+    int64_t rotations, swept, created, merge_in, merge_out, early;  // WindowCounters sums
+    int64_t stall_step;   // step at which the instance stalled
+    int64_t need;         // size that did not fit
+    uint32_t ticket;      // merge tile ticket
+    int32_t cur;          // live buffer parity
+    int32_t flip;         // a finished merge whose parity flip is pending
+    int32_t stalled;      // 1: child buffer too small, 2: live buffer too small
+    int32_t compact;      // early-merge compaction requested at this layer start
+    int32_t err;          // internal consistency bits
+};
+
+// One rotation of one instance at one step.
+struct StepDev {
+    GenDev g;
+    int32_t active;       // 0: the instance skips this step
+    int32_t precheck;     // run the reference's capacity precheck before it
+};
+
 }  // namespace pp

@@ -1,0 +1,301 @@
+// Device helpers of the SparsePauliVector engine: packed keys, the
+// reference's key order, filters, the rotation algebra, channel factors and
+// compensated sums.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pp_common.hpp"
+
+namespace pp {
+
+template <int W>
+struct Key {
+    uint64_t z[W];
+    uint64_t x[W];
+};
+
+__device__ __forceinline__ int popc64(uint64_t v) { return __popcll(v); }
+
+template <int W>
+__device__ __forceinline__ Key<W> load_key(const uint64_t* __restrict__ Z, const uint64_t* __restrict__ X, int64_t i) {
+    Key<W> k;
+    if constexpr (W == 1) {
+        k.z[0] = Z[i];
+        k.x[0] = X[i];
+    } else {
+        const ulonglong2 a = reinterpret_cast<const ulonglong2*>(Z)[i];
+        const ulonglong2 b = reinterpret_cast<const ulonglong2*>(X)[i];
+        k.z[0] = a.x; k.z[1] = a.y;
+        k.x[0] = b.x; k.x[1] = b.y;
+    }
+    return k;
+}
+
+template <int W>
+__device__ __forceinline__ void store_key(uint64_t* __restrict__ Z, uint64_t* __restrict__ X, int64_t i, const Key<W>& k) {
+    if constexpr (W == 1) {
+        Z[i] = k.z[0];
+        X[i] = k.x[0];
+    } else {
+        reinterpret_cast<ulonglong2*>(Z)[i] = make_ulonglong2(k.z[0], k.z[1]);
+        reinterpret_cast<ulonglong2*>(X)[i] = make_ulonglong2(k.x[0], k.x[1]);
+    }
+}
+
+// spv_kernels.jl:14-16: z-major, high word first.
+template <int W>
+__device__ __forceinline__ bool key_lt(const Key<W>& a, const Key<W>& b) {
+#pragma unroll
+    for (int w = W - 1; w >= 0; --w)
+        if (a.z[w] != b.z[w]) return a.z[w] < b.z[w];
+#pragma unroll
+    for (int w = W - 1; w >= 0; --w)
+        if (a.x[w] != b.x[w]) return a.x[w] < b.x[w];
+    return false;
+}
+template <int W>
+__device__ __forceinline__ bool key_le(const Key<W>& a, const Key<W>& b) { return !key_lt<W>(b, a); }
+template <int W>
+__device__ __forceinline__ bool key_eq(const Key<W>& a, const Key<W>& b) {
+    bool e = true;
+#pragma unroll
+    for (int w = 0; w < W; ++w) e = e && a.z[w] == b.z[w] && a.x[w] == b.x[w];
+    return e;
+}
+
+// Highest differing bit of the combined key (x bits at [0, 64W), z bits above);
+// -1 when equal.
+template <int W>
+__device__ __forceinline__ int hdb(const Key<W>& a, const Key<W>& b) {
+#pragma unroll
+    for (int w = W - 1; w >= 0; --w) {
+        const uint64_t d = a.z[w] ^ b.z[w];
+        if (d) return 64 * W + 64 * w + 63 - __clzll((long long)d);
+    }
+#pragma unroll
+    for (int w = W - 1; w >= 0; --w) {
+        const uint64_t d = a.x[w] ^ b.x[w];
+        if (d) return 64 * w + 63 - __clzll((long long)d);
+    }
+    return -1;
+}
+
+template <int W>
+__device__ __forceinline__ int key_bit(const Key<W>& k, int b) {
+    if (b >= 64 * W) {
+        const int bb = b - 64 * W;
+        return (int)((k.z[bb >> 6] >> (bb & 63)) & 1ull);
+    }
+    return (int)((k.x[b >> 6] >> (b & 63)) & 1ull);
+}
+
+template <int W>
+__device__ __forceinline__ int popc_or(const Key<W>& k) {
+    int s = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) s += popc64(k.z[w] | k.x[w]);
+    return s;
+}
+template <int W>
+__device__ __forceinline__ int popc_x(const Key<W>& k) {
+    int s = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) s += popc64(k.x[w]);
+    return s;
+}
+
+// spv_kernels.jl:51-62 on the word type of width 64 W.
+template <int W>
+__device__ __forceinline__ int majorana_weight(const Key<W>& k) {
+    if constexpr (W == 1) {
+        const uint64_t z = k.z[0], x = k.x[0];
+        const uint64_t zonly = z & ~x;
+        uint64_t S = x;
+#pragma unroll
+        for (int sh = 1; sh < 64; sh <<= 1) S ^= S >> sh;
+        const uint64_t ctrl = ~(S ^ x);
+        return popc64(x) + 2 * popc64(zonly & ctrl) + 2 * popc64(~(z | x) & ~ctrl);
+    } else {
+        typedef unsigned __int128 u128;
+        const u128 z = ((u128)k.z[1] << 64) | k.z[0];
+        const u128 x = ((u128)k.x[1] << 64) | k.x[0];
+        const u128 zonly = z & ~x;
+        u128 S = x;
+#pragma unroll
+        for (int sh = 1; sh < 128; sh <<= 1) S ^= S >> sh;
+        const u128 ctrl = ~(S ^ x);
+        auto pc = [](u128 v) { return popc64((uint64_t)v) + popc64((uint64_t)(v >> 64)); };
+        return pc(x) + 2 * pc(zonly & ctrl) + 2 * pc(~(z | x) & ~ctrl);
+    }
+}
+
+// spv_kernels.jl:64-79.
+template <int W>
+__device__ __forceinline__ bool should_drop(const FilterDev* __restrict__ f, const Key<W>& k, double absc) {
+    if (absc <= f->thresh) return true;
+    const int w = popc_or<W>(k);
+    if ((int64_t)w > f->wmax) return true;
+    const int xw = popc_x<W>(k);
+    if ((int64_t)xw > f->xwmax) return true;
+    if (f->mwmax != INT64_MAX) {
+        if ((int64_t)majorana_weight<W>(k) > f->mwmax) return true;
+    }
+    if (f->dthresh_w >= 0.0) {
+        if (absc * f->exptab_w[w] <= f->dthresh_w) return true;
+    }
+    if (f->dthresh_xw >= 0.0) {
+        if (absc * f->exptab_xw[xw] <= f->dthresh_xw) return true;
+    }
+    return false;
+}
+
+template <int W>
+struct GenReg {
+    uint64_t gz[W], gx[W];
+    double cosv, sinv;
+    int ng;
+    __device__ __forceinline__ void load(const GenDev* g) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            gz[w] = g->gz[w];
+            gx[w] = g->gx[w];
+        }
+        cosv = g->cosv;
+        sinv = g->sinv;
+        ng = g->ng;
+    }
+    // popcount(gx & z) - popcount(gz & x) odd  (spv_kernels.jl:157-159)
+    __device__ __forceinline__ bool anti(const Key<W>& k) const {
+        int m = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) m += popc64(gx[w] & k.z[w]) + popc64(gz[w] & k.x[w]);
+        return (m & 1) != 0;
+    }
+    // spv_kernels.jl:160-163: child key and coefficient from the OLD c.
+    __device__ __forceinline__ double child(const Key<W>& k, double c, Key<W>& out) const {
+        int m1 = 0, nyp = 0, ny = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            out.z[w] = gz[w] ^ k.z[w];
+            out.x[w] = gx[w] ^ k.x[w];
+            m1 += popc64(gx[w] & k.z[w]);
+            nyp += popc64(out.z[w] & out.x[w]);
+            ny += popc64(k.z[w] & k.x[w]);
+        }
+        const int kk = (nyp - ng - ny + 2 * m1) & 3;
+        return ((double)(kk - 2) * sinv) * c;
+    }
+};
+
+
+// Global exclusive A-prefix at live position p (0 <= p <= n).
+__device__ __forceinline__ int64_t a_prefix(int64_t p, int64_t n, int64_t ntiles, const uint64_t* __restrict__ flags,
+                                            const uint16_t* __restrict__ wpre, const int64_t* __restrict__ tileoff) {
+    if (p >= n) return tileoff[ntiles];
+    const int64_t w = p >> 6;
+    const uint64_t lowmask = (p & 63) ? ((~0ull) >> (64 - (p & 63))) : 0ull;
+    return tileoff[p / kTile] + (int64_t)wpre[w] + popc64(flags[w] & lowmask);
+}
+
+// First q <= p whose key agrees with K_p on every bit >= b.
+template <int W>
+__device__ int64_t seg_start(const uint64_t* __restrict__ Z, const uint64_t* __restrict__ X, int64_t p, int b) {
+    const Key<W> kp = load_key<W>(Z, X, p);
+    int64_t in = p, out = -1, step = 1;
+    while (true) {
+        const int64_t c = p - step;
+        if (c < 0) { out = -1; break; }
+        if (hdb<W>(load_key<W>(Z, X, c), kp) < b) {
+            in = c;
+            step <<= 1;
+        } else {
+            out = c;
+            break;
+        }
+    }
+    while (in - out > 1) {
+        const int64_t mid = out + (in - out) / 2;
+        if (hdb<W>(load_key<W>(Z, X, mid), kp) < b) in = mid;
+        else out = mid;
+    }
+    return in;
+}
+
+// First r > p whose key differs from K_p on some bit >= b (n if none).
+template <int W>
+__device__ int64_t seg_end(const uint64_t* __restrict__ Z, const uint64_t* __restrict__ X, int64_t p, int64_t n, int b) {
+    const Key<W> kp = load_key<W>(Z, X, p);
+    int64_t in = p, out = n, step = 1;
+    while (true) {
+        const int64_t c = p + step;
+        if (c >= n) { out = n; break; }
+        if (hdb<W>(load_key<W>(Z, X, c), kp) < b) {
+            in = c;
+            step <<= 1;
+        } else {
+            out = c;
+            break;
+        }
+    }
+    while (out - in > 1) {
+        const int64_t mid = in + (out - in) / 2;
+        if (hdb<W>(load_key<W>(Z, X, mid), kp) < b) in = mid;
+        else out = mid;
+    }
+    return out;
+}
+
+
+struct ChanDev {
+    int kind;
+    uint64_t mask[2];
+    const double* t0;
+    const double* t1;
+    const double* t2;
+};
+
+
+template <int W>
+__device__ __forceinline__ double chan_factor(const ChanDev& ch, const Key<W>& k) {
+    int a = 0, b = 0, c = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        const uint64_t z = k.z[w], x = k.x[w], M = ch.mask[w];
+        switch (ch.kind) {
+            case 0: a += popc64((z | x) & M); break;
+            case 1: a += popc64(x & M); break;
+            case 2: a += popc64(z & M); break;
+            case 3: a += popc64((x ^ z) & M); break;
+            default:
+                a += popc64(x & M & ~z);
+                b += popc64(x & M & z);
+                c += popc64(z & M & ~x);
+                break;
+        }
+    }
+    if (ch.kind != 4) return ch.t0[a];
+    double f = ch.t0[a] * ch.t1[b];
+    return f * ch.t2[c];
+}
+
+constexpr uint64_t kFlagAgg = 1ull << 62;
+constexpr uint64_t kFlagPre = 2ull << 62;
+constexpr uint64_t kValMask = (1ull << 62) - 1;
+
+
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+__device__ __forceinline__ void dd_add(double& h, double& l, double bh, double bl) {
+    double s, e;
+    two_sum(h, bh, s, e);
+    e = e + (l + bl);
+    h = s + e;
+    l = e - (h - s);
+}
+
+
+}  // namespace pp

@@ -1,11 +1,15 @@
 // libpauliprop: host engine and C ABI of the MI355X SparsePauliVector path.
 //
-// The driver keeps the reference's semantics (spv_evolve.jl:312-387 at
-// window = 1): before every rotation the append-capacity precheck
-// `n > length(az)` may force an early merge (a strict-filter compaction) and
-// grows the virtual append capacity; each rotation is sweep + merge under
-// the strict filter.  Device buffers are sized independently of those
-// virtual capacities and grow on demand.
+// A Batch holds B independent vectors with the same word count; every
+// operation on it is a short, fixed sequence of kernel launches whose grids
+// depend only on buffer capacities, so a whole rotation sequence is enqueued
+// without waiting on the device.  Live counts, the reference's capacity
+// bookkeeping (spv_evolve.jl:312-387 at window = 1: the append-capacity
+// precheck that may force an early merge before a rotation) and the
+// WindowCounters sums live in device memory (SegState).  A vector whose
+// buffers are too small freezes; after the sequence the host regrows its
+// buffers and replays it from the step that stalled.  The single-vector API
+// is a batch of one.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -14,6 +18,7 @@
 #include <cstring>
 #include <numeric>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/pauliprop.h"
@@ -31,6 +36,12 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
+struct PPError {
+    int code;
+    std::string msg;
+    PPError(int c, std::string m) : code(c), msg(std::move(m)) {}
+};
+
 #define HIPCHK(expr)                                                                              \
     do {                                                                                          \
         hipError_t _e = (expr);                                                                   \
@@ -38,12 +49,6 @@ int fail(int code, const std::string& msg) {
             throw PPError(_e == hipErrorOutOfMemory ? PP_ENOMEM : PP_EHIP,                        \
                           std::string(#expr) + ": " + hipGetErrorString(_e));                     \
     } while (0)
-
-struct PPError {
-    int code;
-    std::string msg;
-    PPError(int c, std::string m) : code(c), msg(std::move(m)) {}
-};
 
 template <typename T>
 T* dalloc(size_t count) {
@@ -57,13 +62,6 @@ T* dalloc(size_t count) {
 void dfree(void* p) {
     if (p) (void)hipFree(p);
 }
-
-struct DevSet {
-    uint64_t* z = nullptr;
-    uint64_t* x = nullptr;
-    double* c = nullptr;
-    int64_t cap = 0;
-};
 
 int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
@@ -124,55 +122,94 @@ bool filter_active(const pp_filter* f) {
            f->dthresh_w >= 0.0 || f->dthresh_xw >= 0.0;
 }
 
-}  // namespace
+struct DevSet {
+    uint64_t* z = nullptr;
+    uint64_t* x = nullptr;
+    double* c = nullptr;
+    int64_t cap = 0;
+};
 
-struct pp_handle {
-    int N = 0, W = 1;
-    int ncu = 256;
-    hipStream_t stream = nullptr;
+// Per-vector device buffers (owned).
+struct InstBuf {
     DevSet L[2];
-    int cur = 0;
     DevSet C;
     uint64_t* flags = nullptr;
     uint16_t* wpre = nullptr;
-    int64_t cap_words = 0;
     int32_t* tilecnt = nullptr;
     int64_t* tileoff = nullptr;
-    int64_t cap_tiles = 0;
     int64_t* bnd = nullptr;
-    int64_t cap_bnd = 0;
     int64_t* split = nullptr;
     uint64_t* status = nullptr;
-    int64_t cap_split = 0;
-    uint32_t* ticket = nullptr;
-    int64_t* d_cnt = nullptr;   // [0], [1] live counts of L[0], L[1]; [2] m; [3] constant 0
-    int32_t* d_ovf = nullptr;   // [0] overflow, [1] check flag, [2] internal error bits
-    GenDev* d_gens = nullptr;
-    int64_t cap_gens = 0;
-    FilterDev* d_filt = nullptr;  // [0] strict, [1] local, [2] NOFILTER
-    double* d_tabs = nullptr;     // 3 x 257 channel tables
-    double2* d_partial = nullptr;
-    double* d_out = nullptr;
-    int64_t* h_pin = nullptr;     // pinned scalars
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    // reference capacity bookkeeping (type_SparsePauliVector.jl:69-126)
-    int64_t live_cap_v = 16, app_cap_v = 16;
-    int64_t n = 0;  // live count (host copy)
-    double m_ratio = 1.0;  // children per live term in the last rotation
+    int64_t cap_tiles = 0;   // sweep-tile capacity of flags/wpre/tilecnt/tileoff
+    int64_t cap_bnd = 0;     // entries of bnd
+    int64_t cap_split = 0;   // entries of split/status
+    int maxlev = 1;
+};
 
-    ~pp_handle() {
-        for (int s = 0; s < 2; ++s) { dfree(L[s].z); dfree(L[s].x); dfree(L[s].c); }
-        dfree(C.z); dfree(C.x); dfree(C.c);
-        dfree(flags); dfree(wpre); dfree(tilecnt); dfree(tileoff); dfree(bnd); dfree(split); dfree(status);
-        dfree(ticket); dfree(d_cnt); dfree(d_ovf); dfree(d_gens); dfree(d_filt); dfree(d_tabs);
-        dfree(d_partial); dfree(d_out);
-        if (h_pin) (void)hipHostFree(h_pin);
-        for (auto& e : ev)
-            if (e) (void)hipEventDestroy(e);
+}  // namespace
+
+// ---------------------------------------------------------------------------
+struct Batch {
+    int W = 1, B = 0, ncu = 256;
+    hipStream_t stream = nullptr;
+    std::vector<int> N;
+    std::vector<InstBuf> ib;
+    std::vector<int64_t> n_known;  // live count at the last host sync
+    SegDesc* d_desc = nullptr;
+    SegState* d_state = nullptr;
+    int64_t* d_maps = nullptr;     // 4 x (B + 1) block prefixes: tile, bounds, partition, merge
+    std::vector<int64_t> h_maps;
+    FilterDev* d_filt = nullptr;   // [2B]: strict, local
+    std::vector<int32_t> local_active;
+    ChanDev* d_chan = nullptr;     // [B]
+    double* d_tabs = nullptr;      // [B][3][kExpTab]
+    int32_t* d_en = nullptr;       // [B]
+    StepDev* d_steps = nullptr;
+    int64_t cap_steps = 0;
+    double2* d_partial = nullptr;
+    int64_t cap_partial = 0;
+    double* d_out = nullptr;
+    uint64_t* d_kets = nullptr;
+    int32_t* d_bad = nullptr;
+    SegState* h_state = nullptr;   // pinned mirror
+    // Optional HIP-event timing of every kb_merge launch (for roofline figures).
+    bool timing = false;
+    std::vector<hipEvent_t> evpool;
+    size_t evused = 0;
+    double t_merge = 0.0;
+    int64_t merge_launches = 0;
+
+    ~Batch() {
+        for (auto& b : ib) free_inst(b);
+        dfree(d_desc); dfree(d_state); dfree(d_maps); dfree(d_filt); dfree(d_chan); dfree(d_tabs);
+        dfree(d_en); dfree(d_steps); dfree(d_partial); dfree(d_out); dfree(d_kets); dfree(d_bad);
+        if (h_state) (void)hipHostFree(h_state);
+        for (auto e : evpool) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
-    size_t key_bytes() const { return sizeof(uint64_t) * (size_t)W; }
+    hipEvent_t next_event() {
+        if (evused == evpool.size()) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            evpool.push_back(e);
+        }
+        return evpool[evused++];
+    }
+
+    // Sum the recorded kb_merge intervals (call after a synchronisation).
+    void harvest_timing() {
+        for (size_t i = 0; i + 1 < evused; i += 2) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, evpool[i], evpool[i + 1]));
+            t_merge += ms * 1e-3;
+            merge_launches += 1;
+        }
+        evused = 0;
+    }
+
+    size_t kb() const { return sizeof(uint64_t) * (size_t)W; }
+    int tm() const { return W == 1 ? MergeTile<1>::kSize : MergeTile<2>::kSize; }
 
     void alloc_set(DevSet& s, int64_t cap) {
         s.z = dalloc<uint64_t>((size_t)cap * W);
@@ -180,332 +217,531 @@ struct pp_handle {
         s.c = dalloc<double>((size_t)cap);
         s.cap = cap;
     }
-    void free_set(DevSet& s) {
+    static void free_set(DevSet& s) {
         dfree(s.z); dfree(s.x); dfree(s.c);
         s = DevSet();
     }
+    static void free_inst(InstBuf& b) {
+        free_set(b.L[0]); free_set(b.L[1]); free_set(b.C);
+        dfree(b.flags); dfree(b.wpre); dfree(b.tilecnt); dfree(b.tileoff); dfree(b.bnd); dfree(b.split); dfree(b.status);
+        b = InstBuf();
+    }
 
-    // Grow one buffer set to hold `need` terms (+25% headroom).  keep > 0
-    // copies that many leading terms across; otherwise the old contents are
-    // dropped first, which keeps the peak footprint at one copy.
-    void grow_set(DevSet& s, int64_t need, int64_t keep) {
-        if (s.cap >= need) return;
-        const int64_t cap = std::max<int64_t>(need + need / 4, 1024);
-        if (keep <= 0) {
-            free_set(s);
-            alloc_set(s, cap);
-            return;
+    void init(int W_, const std::vector<int>& Ns, const std::vector<int64_t>& live_cap_v,
+              const std::vector<int64_t>& app_cap_v) {
+        W = W_;
+        B = (int)Ns.size();
+        N = Ns;
+        int dev = 0;
+        HIPCHK(hipGetDevice(&dev));
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, dev));
+        ncu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        ib.assign(B, InstBuf());
+        n_known.assign(B, 0);
+        local_active.assign(B, 0);
+        d_desc = dalloc<SegDesc>(B);
+        d_state = dalloc<SegState>(B);
+        d_maps = dalloc<int64_t>(4 * (size_t)(B + 1));
+        h_maps.assign(4 * (size_t)(B + 1), 0);
+        d_filt = dalloc<FilterDev>(2 * (size_t)B);
+        d_chan = dalloc<ChanDev>(B);
+        d_tabs = dalloc<double>((size_t)B * 3 * kExpTab);
+        d_en = dalloc<int32_t>(B);
+        d_out = dalloc<double>(B);
+        d_kets = dalloc<uint64_t>((size_t)B * W);
+        d_bad = dalloc<int32_t>(B);
+        HIPCHK(hipHostMalloc((void**)&h_state, sizeof(SegState) * (size_t)B, hipHostMallocDefault));
+        std::memset(h_state, 0, sizeof(SegState) * (size_t)B);
+        for (int b = 0; b < B; ++b) {
+            h_state[b].app_cap_v = app_cap_v[b];
+            h_state[b].live_cap_v = live_cap_v[b];
         }
-        DevSet nw;
-        alloc_set(nw, cap);
-        HIPCHK(hipMemcpyAsync(nw.z, s.z, (size_t)keep * key_bytes(), hipMemcpyDeviceToDevice, stream));
-        HIPCHK(hipMemcpyAsync(nw.x, s.x, (size_t)keep * key_bytes(), hipMemcpyDeviceToDevice, stream));
-        HIPCHK(hipMemcpyAsync(nw.c, s.c, (size_t)keep * sizeof(double), hipMemcpyDeviceToDevice, stream));
+        HIPCHK(hipMemcpyAsync(d_state, h_state, sizeof(SegState) * (size_t)B, hipMemcpyHostToDevice, stream));
+        std::vector<FilterDev> f(2 * (size_t)B, make_filter_dev(nullptr));
+        HIPCHK(hipMemcpyAsync(d_filt, f.data(), f.size() * sizeof(FilterDev), hipMemcpyHostToDevice, stream));
+        for (int b = 0; b < B; ++b) ensure_caps(b, 4096, 4096, 2, 0);
         HIPCHK(hipStreamSynchronize(stream));
-        free_set(s);
-        s = nw;
+        upload_desc();
     }
 
-    // Current live buffer holds `need` terms (keeping the live set); the
-    // merge target is sized separately per call.
-    void ensure_live(int64_t need) { grow_set(L[cur], need, n); }
-    void ensure_target(int64_t need) { grow_set(L[cur ^ 1], need, 0); }
-    void ensure_children(int64_t need) { grow_set(C, need, 0); }
-
-    void ensure_scratch(int64_t nlive, int nlev) {
-        const int64_t words = cdiv(std::max<int64_t>(nlive, 1), 64) + 1;
-        if (cap_words < words) {
-            dfree(flags); dfree(wpre);
-            cap_words = std::max(words, 2 * cap_words);
-            flags = dalloc<uint64_t>(cap_words);
-            wpre = dalloc<uint16_t>(cap_words);
+    // Buffers of vector b hold `cap` live terms and `capC` children; keep > 0
+    // preserves that many terms of the current live buffer `cur`.
+    void ensure_caps(int b, int64_t cap, int64_t capC, int maxlev, int64_t keep, int cur = 0) {
+        InstBuf& I = ib[b];
+        if (I.L[0].cap < cap || I.L[1].cap < cap) {
+            const int64_t nc = std::max<int64_t>(cap, 1024);
+            DevSet keepset;
+            if (keep > 0) {
+                alloc_set(keepset, nc);
+                HIPCHK(hipMemcpyAsync(keepset.z, I.L[cur].z, (size_t)keep * kb(), hipMemcpyDeviceToDevice, stream));
+                HIPCHK(hipMemcpyAsync(keepset.x, I.L[cur].x, (size_t)keep * kb(), hipMemcpyDeviceToDevice, stream));
+                HIPCHK(hipMemcpyAsync(keepset.c, I.L[cur].c, (size_t)keep * sizeof(double), hipMemcpyDeviceToDevice, stream));
+                HIPCHK(hipStreamSynchronize(stream));
+            }
+            free_set(I.L[0]);
+            free_set(I.L[1]);
+            if (keep > 0) I.L[cur] = keepset;
+            else alloc_set(I.L[cur], nc);
+            alloc_set(I.L[cur ^ 1], nc);
         }
-        const int64_t tiles = cdiv(std::max<int64_t>(nlive, 1), kTile) + 2;
-        if (cap_tiles < tiles) {
-            dfree(tilecnt); dfree(tileoff);
-            cap_tiles = std::max(tiles, 2 * cap_tiles);
-            tilecnt = dalloc<int32_t>(cap_tiles);
-            tileoff = dalloc<int64_t>(cap_tiles);
+        if (I.C.cap < capC) {
+            free_set(I.C);
+            alloc_set(I.C, std::max<int64_t>(capC, 1024));
         }
-        const int64_t nb = tiles * std::max(nlev, 1) * 4;
-        if (cap_bnd < nb) {
-            dfree(bnd);
-            cap_bnd = std::max(nb, 2 * cap_bnd);
-            bnd = dalloc<int64_t>(cap_bnd);
+        const int64_t tiles = cdiv(I.L[0].cap, kTile) + 2;
+        if (I.cap_tiles < tiles) {
+            dfree(I.flags); dfree(I.wpre); dfree(I.tilecnt); dfree(I.tileoff);
+            I.cap_tiles = tiles;
+            I.flags = dalloc<uint64_t>((size_t)tiles * kTileWords);
+            I.wpre = dalloc<uint16_t>((size_t)tiles * kTileWords);
+            I.tilecnt = dalloc<int32_t>(tiles);
+            I.tileoff = dalloc<int64_t>(tiles + 1);
         }
-        // merge tiles: outputs <= 2 n (live + at most one child per live term)
-        const int TM = W == 1 ? MergeTile<1>::kSize : MergeTile<2>::kSize;
-        const int64_t sp = cdiv(2 * std::max<int64_t>(nlive, 1), TM) + 2;
-        if (cap_split < sp) {
-            dfree(split); dfree(status);
-            cap_split = std::max(sp, 2 * cap_split);
-            split = dalloc<int64_t>(cap_split);
-            status = dalloc<uint64_t>(cap_split);
+        const int ml = std::min(std::max(maxlev, I.maxlev), kMaxLevels);
+        if (I.cap_bnd < tiles * ml * 4) {
+            dfree(I.bnd);
+            I.cap_bnd = tiles * ml * 4;
+            I.bnd = dalloc<int64_t>(I.cap_bnd);
+        }
+        I.maxlev = ml;
+        const int64_t sp = cdiv(I.L[0].cap + I.C.cap, tm()) + 2;
+        if (I.cap_split < sp) {
+            dfree(I.split); dfree(I.status);
+            I.cap_split = sp;
+            I.split = dalloc<int64_t>(sp);
+            I.status = dalloc<uint64_t>(sp);
         }
     }
 
-    void ensure_gens(int64_t Lcount) {
-        if (cap_gens >= Lcount) return;
-        dfree(d_gens);
-        cap_gens = std::max<int64_t>(Lcount, 2 * cap_gens);
-        d_gens = dalloc<GenDev>(cap_gens);
-    }
-
-    GenDev make_gen(const uint64_t* gz, const uint64_t* gx, double cosv, double sinv) const {
-        GenDev g;
-        std::memset(&g, 0, sizeof(g));
-        int ng = 0;
-        for (int w = 0; w < W; ++w) {
-            g.gz[w] = gz[w];
-            g.gx[w] = gx[w];
-            ng += __builtin_popcountll(gz[w] & gx[w]);
+    void upload_desc() {
+        std::vector<SegDesc> h((size_t)B);
+        for (int b = 0; b < B; ++b) {
+            InstBuf& I = ib[b];
+            SegDesc& d = h[b];
+            std::memset(&d, 0, sizeof(d));
+            for (int p = 0; p < 2; ++p) {
+                d.Lz[p] = I.L[p].z;
+                d.Lx[p] = I.L[p].x;
+                d.Lc[p] = I.L[p].c;
+            }
+            d.Cz = I.C.z; d.Cx = I.C.x; d.Cc = I.C.c;
+            d.flags = I.flags; d.wpre = I.wpre; d.tilecnt = I.tilecnt; d.tileoff = I.tileoff;
+            d.bnd = I.bnd; d.split = I.split; d.status = I.status;
+            d.fs = d_filt + 2 * b;
+            d.fl = d_filt + 2 * b + 1;
+            d.cap = std::min(I.L[0].cap, I.L[1].cap);
+            d.capC = I.C.cap;
+            d.maxlev = I.maxlev;
+            d.local_active = local_active[b];
         }
-        g.ng = ng;
-        g.cosv = cosv;
-        g.sinv = sinv;
-        int nlev = 0;
-        for (int w = 0; w < W; ++w)  // x bits: combined positions [0, 64 W)
-            for (int b = 0; b < 64; ++b)
-                if ((gx[w] >> b) & 1ull) g.lev[nlev++] = (uint8_t)(64 * w + b);
-        for (int w = 0; w < W; ++w)  // z bits above
-            for (int b = 0; b < 64; ++b)
-                if ((gz[w] >> b) & 1ull) g.lev[nlev++] = (uint8_t)(64 * W + 64 * w + b);
-        g.nlev = nlev;
-        return g;
+        HIPCHK(hipMemcpyAsync(d_desc, h.data(), h.size() * sizeof(SegDesc), hipMemcpyHostToDevice, stream));
     }
 
-    int grid_for(int64_t items, int per_block_items, int max_per_cu = 8) const {
-        const int64_t need = cdiv(std::max<int64_t>(items, 1), per_block_items);
-        return (int)std::min<int64_t>(need, (int64_t)ncu * max_per_cu);
+    // Block prefixes per kernel family from the expected live count (blocks
+    // only set parallelism: every kernel loops over the vector's real tiles).
+    void plan() {
+        const int T2 = tm();
+        int64_t acc[4] = {0, 0, 0, 0};
+        for (int b = 0; b <= B; ++b) {
+            for (int k = 0; k < 4; ++k) h_maps[(size_t)k * (B + 1) + b] = acc[k];
+            if (b == B) break;
+            const int64_t cap = std::min(ib[b].L[0].cap, ib[b].L[1].cap);
+            const int64_t nest = std::min<int64_t>(cap, std::max<int64_t>(4 * n_known[b], 8192));
+            acc[0] += std::max<int64_t>(1, std::min<int64_t>(cdiv(nest, kTile), (int64_t)ncu * 8));
+            acc[1] += std::max<int64_t>(1, std::min<int64_t>(cdiv(2 * cdiv(nest, kTile), kBlock), (int64_t)ncu * 4));
+            acc[2] += std::max<int64_t>(1, std::min<int64_t>(cdiv(cdiv(2 * nest, T2) + 1, kBlock), (int64_t)ncu * 4));
+            acc[3] += std::max<int64_t>(1, std::min<int64_t>(cdiv(2 * nest, T2), (int64_t)ncu * 4));
+        }
+        HIPCHK(hipMemcpyAsync(d_maps, h_maps.data(), h_maps.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream));
+        if (cap_partial < acc[0]) {
+            dfree(d_partial);
+            cap_partial = acc[0];
+            d_partial = dalloc<double2>(cap_partial);
+        }
     }
 
-    // Read back the live count written by the last merge, the child count and
-    // the overflow flag (synchronises).  Internal consistency failures throw.
-    int64_t sync_count(int idx, int32_t* ovf) {
-        HIPCHK(hipMemcpyAsync(h_pin, d_cnt + idx, sizeof(int64_t), hipMemcpyDeviceToHost, stream));
-        HIPCHK(hipMemcpyAsync(h_pin + 1, d_cnt + 2, sizeof(int64_t), hipMemcpyDeviceToHost, stream));
-        HIPCHK(hipMemcpyAsync(h_pin + 2, d_ovf, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
-        HIPCHK(hipStreamSynchronize(stream));
-        const int32_t* flags3 = (const int32_t*)(h_pin + 2);
-        if (flags3[2] != 0)
-            throw PPError(PP_EHIP, "internal consistency check failed in a kernel (bits " + std::to_string(flags3[2]) + ")");
-        *ovf = flags3[0];
-        return h_pin[0];
+    BlockMap map(int k) const { return BlockMap{d_maps + (size_t)k * (B + 1), B}; }
+    unsigned blocks(int k) const { return (unsigned)h_maps[(size_t)k * (B + 1) + B]; }
+
+    void set_filters(const pp_filter* strict, const pp_filter* local) {
+        std::vector<FilterDev> f(2 * (size_t)B);
+        for (int b = 0; b < B; ++b) {
+            f[2 * b] = make_filter_dev(strict ? strict + b : nullptr);
+            f[2 * b + 1] = make_filter_dev(local ? local + b : nullptr);
+            local_active[b] = filter_active(local ? local + b : nullptr) ? 1 : 0;
+        }
+        HIPCHK(hipMemcpyAsync(d_filt, f.data(), f.size() * sizeof(FilterDev), hipMemcpyHostToDevice, stream));
+    }
+
+    void ensure_steps(int64_t count) {
+        if (cap_steps >= count) return;
+        dfree(d_steps);
+        cap_steps = std::max<int64_t>(count, 2 * cap_steps);
+        d_steps = dalloc<StepDev>(cap_steps);
     }
 
     template <int Wt>
-    void launch_merge(const GenDev* g, const FilterDev* f, const DevSet& src, const int64_t* d_n, const DevSet& ch,
-                      const int64_t* d_m, int64_t mbound, DevSet& dst, int64_t* d_nout, bool timed = false) {
-        constexpr int TM = MergeTile<Wt>::kSize;
-        const int64_t ntm_ub = cdiv(n + mbound, TM);
-        const int64_t threads = ntm_ub + 1;
-        ChanDev nochan{};
-        hipLaunchKernelGGL(k_partition<Wt>, dim3((unsigned)cdiv(threads, kBlock)), dim3(kBlock), 0, stream, src.z,
-                           src.x, d_n, ch.z, ch.x, d_m, split, status, ticket, d_nout, threads, d_ovf);
-        const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(ntm_ub, (int64_t)ncu * 4));
-        if (timed) HIPCHK(hipEventRecord(ev[2], stream));
-        hipLaunchKernelGGL(k_merge<Wt>, dim3(grid), dim3(kBlock), 0, stream, src.z, src.x, src.c, d_n, ch.z, ch.x,
-                           ch.c, d_m, split, g, f, nochan, 0, dst.z, dst.x, dst.c, dst.cap, status, ticket, d_nout,
-                           d_ovf, d_ovf + 2);
+    void launch_step(const StepDev* dsteps, int64_t step_index) {
+        hipLaunchKernelGGL(kb_flags<Wt>, dim3(blocks(0)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(0));
+        hipLaunchKernelGGL(kb_scan, dim3(B), dim3(1024), 0, stream, d_desc, d_state, dsteps, step_index);
+        hipLaunchKernelGGL(kb_bounds<Wt>, dim3(blocks(1)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(1));
+        hipLaunchKernelGGL(kb_emit<Wt>, dim3(blocks(0)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(0));
+        hipLaunchKernelGGL(kb_partition<Wt>, dim3(blocks(2)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(2), 0);
+        if (timing) HIPCHK(hipEventRecord(next_event(), stream));
+        hipLaunchKernelGGL(kb_merge<Wt>, dim3(blocks(3)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(3), 0,
+                           step_index);
+        if (timing) HIPCHK(hipEventRecord(next_event(), stream));
+    }
+
+    template <int Wt>
+    void launch_compact() {
+        // the step table is not read in compaction mode but must be a valid pointer
+        ensure_steps(B);
+        hipLaunchKernelGGL(kb_partition<Wt>, dim3(blocks(2)), dim3(kBlock), 0, stream, d_desc, d_state, d_steps, map(2), 1);
+        hipLaunchKernelGGL(kb_merge<Wt>, dim3(blocks(3)), dim3(kBlock), 0, stream, d_desc, d_state, d_steps, map(3), 1,
+                           (int64_t)-1);
+    }
+
+    void sync_state() {
+        hipLaunchKernelGGL(kb_resolve, dim3((unsigned)cdiv(B, 256)), dim3(256), 0, stream, d_state, B);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(h_state, d_state, sizeof(SegState) * (size_t)B, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        if (timing) harvest_timing();
+        for (int b = 0; b < B; ++b) {
+            if (h_state[b].err)
+                throw PPError(PP_EHIP, "internal consistency check failed in a kernel (vector " + std::to_string(b) +
+                                           ", bits " + std::to_string(h_state[b].err) + ")");
+            n_known[b] = h_state[b].n[h_state[b].cur];
+        }
+    }
+
+    void put_state() {
+        HIPCHK(hipMemcpyAsync(d_state, h_state, sizeof(SegState) * (size_t)B, hipMemcpyHostToDevice, stream));
+    }
+
+    // One sequence: for every vector b, the rotations steps[s * B + b] with
+    // active != 0, in step order (one evolve! call per vector), preceded by
+    // the reference's capacity precheck.  Stalled vectors are regrown and
+    // replayed until every vector has finished the sequence.
+    template <int Wt>
+    void run_sequence(std::vector<StepDev>& steps, int64_t nsteps, const std::vector<int32_t>& en) {
+        int maxlev = 1;
+        for (const StepDev& s : steps) maxlev = std::max(maxlev, (int)s.g.nlev);
+        for (int b = 0; b < B; ++b) {
+            if (ib[b].maxlev < maxlev) {
+                const int64_t cap = std::min(ib[b].L[0].cap, ib[b].L[1].cap);
+                ensure_caps(b, cap, ib[b].C.cap, maxlev, 0, h_state[b].cur);
+            }
+        }
+        upload_desc();
+        plan();
+        ensure_steps(nsteps * B);
+        HIPCHK(hipMemcpyAsync(d_steps, steps.data(), (size_t)(nsteps * B) * sizeof(StepDev), hipMemcpyHostToDevice,
+                              stream));
+        HIPCHK(hipMemcpyAsync(d_en, en.data(), (size_t)B * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL(kb_precheck, dim3((unsigned)cdiv(B, 256)), dim3(256), 0, stream, d_state, d_en, B);
+        launch_compact<Wt>();
+        for (int64_t s = 0; s < nsteps; ++s) launch_step<Wt>(d_steps + s * B, s);
+        HIPCHK(hipGetLastError());
+        sync_state();
+        for (int round = 0;; ++round) {
+            std::vector<int> stalled;
+            for (int b = 0; b < B; ++b)
+                if (h_state[b].stalled) stalled.push_back(b);
+            if (stalled.empty()) break;
+            if (round > 64) throw PPError(PP_EOVERFLOW, "vector kept stalling after regrowth");
+            int64_t first = nsteps;
+            for (int b : stalled) {
+                SegState& st = h_state[b];
+                const int64_t cap = std::min(ib[b].L[0].cap, ib[b].L[1].cap);
+                const int64_t n = st.n[st.cur];
+                const int64_t need = st.need + st.need / 2 + 1024;
+                if (st.stalled == 1) ensure_caps(b, std::max(cap, n + need), need, maxlev, n, st.cur);
+                else ensure_caps(b, need, ib[b].C.cap, maxlev, n, st.cur);
+                first = std::min(first, st.stall_step);
+            }
+            // replay table: only the stalled vectors, each from its own stall
+            // step, whose capacity precheck already ran
+            std::vector<StepDev> rs((size_t)((nsteps - first) * B));
+            std::vector<char> is_st(B, 0);
+            for (int b : stalled) is_st[b] = 1;
+            for (int64_t s = first; s < nsteps; ++s)
+                for (int b = 0; b < B; ++b) {
+                    StepDev sd = steps[(size_t)(s * B + b)];
+                    const bool on = is_st[b] && s >= h_state[b].stall_step && sd.active;
+                    sd.active = on ? 1 : 0;
+                    if (s == h_state[b].stall_step) sd.precheck = 0;
+                    rs[(size_t)((s - first) * B + b)] = sd;
+                }
+            for (int b : stalled) {
+                h_state[b].stalled = 0;
+                h_state[b].flip = 0;
+            }
+            put_state();
+            upload_desc();
+            plan();
+            HIPCHK(hipMemcpyAsync(d_steps, rs.data(), rs.size() * sizeof(StepDev), hipMemcpyHostToDevice, stream));
+            for (int64_t s = first; s < nsteps; ++s) launch_step<Wt>(d_steps + (s - first) * B, s);
+            HIPCHK(hipGetLastError());
+            sync_state();
+        }
+    }
+
+    template <int Wt>
+    void channel(const std::vector<ChanDev>& ch, const std::vector<double>& tabs, const std::vector<int32_t>& en) {
+        HIPCHK(hipMemcpyAsync(d_tabs, tabs.data(), tabs.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(d_chan, ch.data(), ch.size() * sizeof(ChanDev), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(d_en, en.data(), en.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+        plan();
+        hipLaunchKernelGGL(kb_channel<Wt>, dim3(blocks(0)), dim3(kBlock), 0, stream, d_desc, d_state, d_chan, d_en, map(0));
         HIPCHK(hipGetLastError());
     }
 
-    // One rotation at window = 1 on the current live set (host count n).
-    // Buffers are sized from the previous rotation's child ratio; if the
-    // children or the merged output do not fit, the kernels raise the
-    // overflow flag and write nothing out of bounds, the live set is
-    // untouched, and the rotation is re-run with exact sizes (the reference's
-    // grow-and-retry merge, spv_kernels.jl:377-383).
+    // Compaction of the selected vectors under their strict filter (clips).
     template <int Wt>
-    void rotate(const GenDev* dg, const GenDev& hg, int local_active, bool use_strict, pp_counters* cnt) {
-        ensure_scratch(n, hg.nlev);
-        const int64_t m_est = std::min<int64_t>(n, (int64_t)(m_ratio * (double)n * 1.15) + 4096);
-        ensure_children(m_est + 1);
-        ensure_target(n + m_est + 1);
-        int64_t n_out = 0, m = 0;
-        for (int attempt = 0;; ++attempt) {
-            launch_rotation<Wt>(dg, hg, local_active, use_strict, cnt != nullptr);
-            int32_t ovf = 0;
-            n_out = sync_count(cur ^ 1, &ovf);
-            m = h_pin[1];
-            if (!ovf) break;
-            if (attempt >= 2) throw PPError(PP_EOVERFLOW, "merge overflow persisted after regrowth");
-            HIPCHK(hipMemsetAsync(d_ovf, 0, sizeof(int32_t), stream));
-            ensure_children(m + 1);
-            ensure_target(n + m + 1);
-        }
-        if (cnt) {
-            float a = 0.f, b = 0.f;
-            HIPCHK(hipEventElapsedTime(&a, ev[0], ev[1]));
-            HIPCHK(hipEventElapsedTime(&b, ev[2], ev[3]));
-            cnt->t_rotate += a * 1e-3;
-            cnt->t_merge += b * 1e-3;
-            cnt->rotations += 1;
-            cnt->terms_swept += n;
-            cnt->terms_created += m;
-            cnt->merge_in += n + m;
-            cnt->merge_out += n_out;
-        }
-        if (n > 0 && hg.nlev > 0) m_ratio = (double)m / (double)n;
-        // _merge_spv! growth of the reference's live buffers (spv_kernels.jl:377-383)
-        if (n_out > live_cap_v) live_cap_v = std::max(2 * live_cap_v, n + m);
-        cur ^= 1;
-        n = n_out;
+    void compact(const std::vector<int32_t>& sel) {
+        sync_state();
+        for (int b = 0; b < B; ++b) h_state[b].compact = sel[b] ? 1 : 0;
+        put_state();
+        upload_desc();
+        plan();
+        launch_compact<Wt>();
+        HIPCHK(hipGetLastError());
+        sync_state();
     }
 
     template <int Wt>
-    void launch_rotation(const GenDev* dg, const GenDev& hg, int local_active, bool use_strict, bool timed) {
-        DevSet& src = L[cur];
-        DevSet& dst = L[cur ^ 1];
-        const int64_t* d_n = d_cnt + cur;
-        int64_t* d_nout = d_cnt + (cur ^ 1);
-        const FilterDev* fstrict = d_filt + (use_strict ? 0 : 2);
-        if (timed) HIPCHK(hipEventRecord(ev[0], stream));
-        if (hg.nlev > 0 && n > 0) {
-            const int64_t ntiles = cdiv(n, kTile);
-            const int gt = (int)std::min<int64_t>(ntiles, (int64_t)ncu * 8);
-            hipLaunchKernelGGL(k_flags<Wt>, dim3(gt), dim3(kBlock), 0, stream, src.z, src.x, src.c, d_n, dg,
-                               d_filt + 1, local_active, flags, wpre, tilecnt);
-            hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, stream, tilecnt, d_n, tileoff, d_cnt + 2, C.cap, d_ovf);
-            const int64_t nb = ntiles * hg.nlev * 2;
-            hipLaunchKernelGGL(k_bounds<Wt>, dim3((unsigned)std::min<int64_t>(cdiv(nb, kBlock), (int64_t)ncu * 16)),
-                               dim3(kBlock), 0, stream, src.z, src.x, d_n, dg, flags, wpre, tileoff, bnd);
-            hipLaunchKernelGGL(k_emit<Wt>, dim3(gt), dim3(kBlock), 0, stream, src.z, src.x, src.c, d_n, dg, flags,
-                               wpre, tileoff, bnd, C.z, C.x, C.c, d_ovf, d_ovf + 2);
-            HIPCHK(hipGetLastError());
-        } else {
-            HIPCHK(hipMemsetAsync(d_cnt + 2, 0, sizeof(int64_t), stream));
-        }
-        if (timed) HIPCHK(hipEventRecord(ev[1], stream));
-        launch_merge<Wt>(dg, fstrict, src, d_n, C, d_cnt + 2, hg.nlev > 0 ? n : 0, dst, d_nout, timed);
-        if (timed) HIPCHK(hipEventRecord(ev[3], stream));
+    void expectation(const uint64_t* kets, double* out) {
+        HIPCHK(hipMemcpyAsync(d_kets, kets, (size_t)B * W * sizeof(uint64_t), hipMemcpyHostToDevice, stream));
+        plan();
+        hipLaunchKernelGGL(kb_expect<Wt>, dim3(blocks(0)), dim3(kBlock), 0, stream, d_desc, d_state, d_kets, map(0),
+                           d_partial);
+        hipLaunchKernelGGL(kb_expect_final, dim3(B), dim3(kBlock), 0, stream, d_partial, map(0), d_out);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out, d_out, (size_t)B * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
     }
 
     template <int Wt>
-    void compact(const FilterDev* f) {
-        ensure_scratch(n, 0);
-        ensure_target(n + 1);
-        DevSet empty;
-        empty.z = C.z ? C.z : L[cur].z;
-        empty.x = C.x ? C.x : L[cur].x;
-        empty.c = C.c ? C.c : L[cur].c;
-        launch_merge<Wt>(nullptr, f, L[cur], d_cnt + cur, empty, d_cnt + 3, 0, L[cur ^ 1], d_cnt + (cur ^ 1));
-        int32_t ovf = 0;
-        const int64_t n_out = sync_count(cur ^ 1, &ovf);
-        if (ovf) throw PPError(PP_EOVERFLOW, "compaction overflow");
-        cur ^= 1;
-        n = n_out;
+    void check(int32_t* ok) {
+        HIPCHK(hipMemsetAsync(d_bad, 0, (size_t)B * sizeof(int32_t), stream));
+        plan();
+        hipLaunchKernelGGL(kb_check<Wt>, dim3(blocks(0)), dim3(kBlock), 0, stream, d_desc, d_state, map(0), d_bad);
+        HIPCHK(hipGetLastError());
+        std::vector<int32_t> bad((size_t)B);
+        HIPCHK(hipMemcpyAsync(bad.data(), d_bad, (size_t)B * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        for (int b = 0; b < B; ++b) ok[b] = bad[b] ? 0 : 1;
     }
 
-    void upload_filters(const pp_filter* strict, const pp_filter* local) {
-        FilterDev f[3] = {make_filter_dev(strict), make_filter_dev(local), make_filter_dev(nullptr)};
-        HIPCHK(hipMemcpyAsync(d_filt, f, sizeof(f), hipMemcpyHostToDevice, stream));
-    }
-
-    template <int Wt>
-    void evolve(const uint64_t* gz, const uint64_t* gx, const double* cosv, const double* sinv, int64_t Lcount,
-                const pp_filter* strict, const pp_filter* local, pp_counters* cnt) {
-        std::vector<GenDev> hg((size_t)Lcount);
-        for (int64_t i = 0; i < Lcount; ++i) hg[i] = make_gen(gz + i * W, gx + i * W, cosv[i], sinv[i]);
-        ensure_gens(Lcount);
-        HIPCHK(hipMemcpyAsync(d_gens, hg.data(), (size_t)Lcount * sizeof(GenDev), hipMemcpyHostToDevice, stream));
-        upload_filters(strict, local);
-        const int local_active = filter_active(local) ? 1 : 0;
-        for (int64_t i = 0; i < Lcount; ++i) {
-            // capacity precheck, spv_evolve.jl:356-362 (no pending appends at window = 1)
-            if (n > app_cap_v) {
-                const int64_t before = n;
-                compact<Wt>(d_filt + 0);
-                if (cnt) {
-                    cnt->early_merges += 1;
-                    cnt->merge_in += before;
-                    cnt->merge_out += n;
+    void upload(int b, const uint64_t* z, const uint64_t* x, const double* c, int64_t n, bool sorted) {
+        const int Wl = W;
+        std::vector<uint64_t> hz(z, z + n * Wl), hx(x, x + n * Wl);
+        std::vector<double> hc(c, c + n);
+        auto less = [&](int64_t a, int64_t q) {
+            for (int w = Wl - 1; w >= 0; --w)
+                if (hz[a * Wl + w] != hz[q * Wl + w]) return hz[a * Wl + w] < hz[q * Wl + w];
+            for (int w = Wl - 1; w >= 0; --w)
+                if (hx[a * Wl + w] != hx[q * Wl + w]) return hx[a * Wl + w] < hx[q * Wl + w];
+            return false;
+        };
+        if (!sorted) {
+            std::vector<int64_t> perm((size_t)n);
+            std::iota(perm.begin(), perm.end(), 0);
+            std::sort(perm.begin(), perm.end(), less);
+            std::vector<uint64_t> sz((size_t)n * Wl), sx((size_t)n * Wl);
+            std::vector<double> sc((size_t)n);
+            for (int64_t i = 0; i < n; ++i) {
+                for (int w = 0; w < Wl; ++w) {
+                    sz[i * Wl + w] = hz[perm[i] * Wl + w];
+                    sx[i * Wl + w] = hx[perm[i] * Wl + w];
                 }
-                if (n > app_cap_v) app_cap_v = std::max(2 * app_cap_v, n);
+                sc[i] = hc[perm[i]];
             }
-            rotate<Wt>(d_gens + i, hg[i], local_active, true, cnt);
+            hz.swap(sz);
+            hx.swap(sx);
+            hc.swap(sc);
         }
+        for (int64_t i = 1; i < n; ++i)
+            if (!less(i - 1, i))
+                throw PPError(PP_EARG, sorted ? "keys not strictly increasing" : "duplicate keys in upload");
+        sync_state();
+        const int64_t cap = std::max<int64_t>(2 * n + 1024, 4096);
+        ensure_caps(b, cap, std::max<int64_t>(n + 1024, ib[b].C.cap), ib[b].maxlev, 0, 0);
+        DevSet& s = ib[b].L[0];
+        if (n > 0) {
+            HIPCHK(hipMemcpyAsync(s.z, hz.data(), (size_t)n * Wl * 8, hipMemcpyHostToDevice, stream));
+            HIPCHK(hipMemcpyAsync(s.x, hx.data(), (size_t)n * Wl * 8, hipMemcpyHostToDevice, stream));
+            HIPCHK(hipMemcpyAsync(s.c, hc.data(), (size_t)n * 8, hipMemcpyHostToDevice, stream));
+        }
+        SegState& st = h_state[b];
+        st.n[0] = n;
+        st.n[1] = 0;
+        st.cur = 0;
+        st.flip = 0;
+        st.m = 0;
+        st.stalled = 0;
+        st.compact = 0;
+        if (n > st.live_cap_v) st.live_cap_v = n;
+        n_known[b] = n;
+        put_state();
+        upload_desc();
+        HIPCHK(hipStreamSynchronize(stream));
     }
 
-    template <int Wt>
-    void evolve_single(const uint64_t* gz, const uint64_t* gx, double cosv, double sinv) {
-        GenDev g = make_gen(gz, gx, cosv, sinv);
-        ensure_gens(1);
-        HIPCHK(hipMemcpyAsync(d_gens, &g, sizeof(GenDev), hipMemcpyHostToDevice, stream));
-        upload_filters(nullptr, nullptr);
-        // spv_evolve.jl:231
-        if (n > app_cap_v) app_cap_v = std::max(2 * app_cap_v, n);
-        rotate<Wt>(d_gens, g, 0, false, nullptr);
+    int64_t download(int b, uint64_t* z, uint64_t* x, double* c, int64_t cap) {
+        sync_state();
+        const SegState& st = h_state[b];
+        const int64_t n = st.n[st.cur];
+        if (cap < n) throw PPError(PP_EARG, "output capacity too small");
+        const DevSet& s = ib[b].L[st.cur];
+        if (n > 0) {
+            HIPCHK(hipMemcpyAsync(z, s.z, (size_t)n * W * 8, hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipMemcpyAsync(x, s.x, (size_t)n * W * 8, hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipMemcpyAsync(c, s.c, (size_t)n * 8, hipMemcpyDeviceToHost, stream));
+        }
+        HIPCHK(hipStreamSynchronize(stream));
+        return n;
     }
 };
 
 // ---------------------------------------------------------------------------
+// Rotation tables
+
+static GenDev make_gen(int W, const uint64_t* gz, const uint64_t* gx, double cosv, double sinv) {
+    GenDev g;
+    std::memset(&g, 0, sizeof(g));
+    int ng = 0;
+    for (int w = 0; w < W; ++w) {
+        g.gz[w] = gz[w];
+        g.gx[w] = gx[w];
+        ng += __builtin_popcountll(gz[w] & gx[w]);
+    }
+    g.ng = ng;
+    g.cosv = cosv;
+    g.sinv = sinv;
+    int nlev = 0;
+    for (int w = 0; w < W; ++w)  // x bits: combined positions [0, 64 W)
+        for (int b = 0; b < 64; ++b)
+            if ((gx[w] >> b) & 1ull) g.lev[nlev++] = (uint8_t)(64 * w + b);
+    for (int w = 0; w < W; ++w)  // z bits above
+        for (int b = 0; b < 64; ++b)
+            if ((gz[w] >> b) & 1ull) g.lev[nlev++] = (uint8_t)(64 * W + 64 * w + b);
+    g.nlev = nlev;
+    return g;
+}
+
+// Steps for B vectors: generator i of vector b is gz[(i * B + b) * W]; active
+// may be null (all active).  The first active rotation of each vector skips
+// the per-step precheck (the sequence-start precheck covers it).
+static std::vector<StepDev> make_steps(int W, int B, int64_t nsteps, const uint64_t* gz, const uint64_t* gx,
+                                       const double* cosv, const double* sinv, const uint8_t* active,
+                                       std::vector<int32_t>& en) {
+    std::vector<StepDev> st((size_t)(nsteps * B));
+    en.assign(B, 0);
+    std::vector<char> seen(B, 0);
+    for (int64_t s = 0; s < nsteps; ++s)
+        for (int b = 0; b < B; ++b) {
+            const size_t k = (size_t)(s * B + b);
+            StepDev& d = st[k];
+            d.g = make_gen(W, gz + k * W, gx + k * W, cosv[k], sinv[k]);
+            d.active = active ? (active[k] ? 1 : 0) : 1;
+            d.precheck = 0;
+            if (d.active) {
+                d.precheck = seen[b] ? 1 : 0;
+                seen[b] = 1;
+                en[b] = 1;
+            }
+        }
+    return st;
+}
+
+static void add_counters(pp_counters* c, const SegState& a, const SegState& b) {
+    c->rotations += b.rotations - a.rotations;
+    c->terms_swept += b.swept - a.swept;
+    c->terms_created += b.created - a.created;
+    c->merge_in += b.merge_in - a.merge_in;
+    c->merge_out += b.merge_out - a.merge_out;
+    c->early_merges += b.early - a.early;
+}
+
+struct pp_handle {
+    Batch bt;
+    int N = 0;
+};
+
+struct pp_batch {
+    Batch bt;
+};
 
 #define API_BEGIN try {
-#define API_END                                           \
-    }                                                     \
-    catch (const PPError& e) {                            \
-        return fail(e.code, e.msg);                       \
-    }                                                     \
-    catch (const std::exception& e) {                     \
-        return fail(PP_EHIP, e.what());                   \
-    }                                                     \
-    g_last_error.clear();                                 \
+#define API_END                             \
+    }                                       \
+    catch (const PPError& e) {              \
+        return fail(e.code, e.msg);         \
+    }                                       \
+    catch (const std::exception& e) {       \
+        return fail(PP_EHIP, e.what());     \
+    }                                       \
+    g_last_error.clear();                   \
     return PP_OK;
+
+template <typename F>
+static void dispatch_w(int W, F&& f) {
+    if (W == 1) f(std::integral_constant<int, 1>());
+    else f(std::integral_constant<int, 2>());
+}
+
+static void batch_evolve(Batch& bt, int64_t nsteps, const uint64_t* gz, const uint64_t* gx, const double* cosv,
+                         const double* sinv, const uint8_t* active, const pp_filter* strict, const pp_filter* local,
+                         pp_counters* counters) {
+    std::vector<int32_t> en;
+    std::vector<StepDev> st = make_steps(bt.W, bt.B, nsteps, gz, gx, cosv, sinv, active, en);
+    bt.sync_state();
+    std::vector<SegState> before(bt.h_state, bt.h_state + bt.B);
+    bt.set_filters(strict, local);
+    dispatch_w(bt.W, [&](auto wc) { bt.run_sequence<decltype(wc)::value>(st, nsteps, en); });
+    if (counters)
+        for (int b = 0; b < bt.B; ++b) {
+            std::memset(counters + b, 0, sizeof(pp_counters));
+            add_counters(counters + b, before[b], bt.h_state[b]);
+        }
+}
 
 extern "C" {
 
 const char* pp_last_error(void) { return g_last_error.c_str(); }
 
-const char* pp_version(void) { return "pauliprop 0.1 gfx950"; }
+const char* pp_version(void) { return "pauliprop 0.2 gfx950"; }
 
 double pp_julia_powi(double x, int64_t n) { return julia_powi(x, n); }
+
+// --- single vector ------------------------------------------------------------
 
 int pp_create(int N, int64_t live_cap, int64_t append_cap, pp_handle** out) {
     if (!out) return fail(PP_EARG, "out is NULL");
     *out = nullptr;
     if (N < 1 || N > 128) return fail(PP_EARG, "N must be in 1:128");
     if (live_cap < 1 || append_cap < 1) return fail(PP_EARG, "capacities must be >= 1");
-    pp_handle* h = nullptr;
-    API_BEGIN
-    h = new pp_handle();
-    h->N = N;
-    h->W = N <= 64 ? 1 : 2;
-    h->live_cap_v = live_cap;
-    h->app_cap_v = append_cap;
-    int dev = 0;
-    HIPCHK(hipGetDevice(&dev));
-    hipDeviceProp_t prop;
-    HIPCHK(hipGetDeviceProperties(&prop, dev));
-    h->ncu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
-    HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-    for (auto& e : h->ev) HIPCHK(hipEventCreate(&e));
-    h->ticket = dalloc<uint32_t>(1);
-    h->d_cnt = dalloc<int64_t>(4);
-    h->d_ovf = dalloc<int32_t>(4);
-    h->d_filt = dalloc<FilterDev>(3);
-    h->d_tabs = dalloc<double>(3 * kExpTab);
-    h->d_partial = dalloc<double2>(4096);
-    h->d_out = dalloc<double>(2);
-    HIPCHK(hipHostMalloc((void**)&h->h_pin, 8 * sizeof(int64_t), hipHostMallocDefault));
-    HIPCHK(hipMemsetAsync(h->d_cnt, 0, 4 * sizeof(int64_t), h->stream));
-    HIPCHK(hipMemsetAsync(h->d_ovf, 0, 4 * sizeof(int32_t), h->stream));
-    h->ensure_live(std::max<int64_t>(live_cap, 16));
-    HIPCHK(hipStreamSynchronize(h->stream));
-    *out = h;
-    h = nullptr;
-    }
-    catch (const PPError& e) {
+    pp_handle* h = new pp_handle();
+    try {
+        h->N = N;
+        h->bt.init(N <= 64 ? 1 : 2, {N}, {live_cap}, {append_cap});
+    } catch (const PPError& e) {
         delete h;
         return fail(e.code, e.msg);
     }
+    *out = h;
     g_last_error.clear();
     return PP_OK;
 }
 
 int pp_destroy(pp_handle* h) {
     if (!h) return PP_OK;
-    (void)hipStreamSynchronize(h->stream);
+    (void)hipStreamSynchronize(h->bt.stream);
     delete h;
     return PP_OK;
 }
@@ -514,79 +750,32 @@ int pp_upload(pp_handle* h, const uint64_t* z, const uint64_t* x, const double* 
     if (!h) return fail(PP_EARG, "null handle");
     if (n < 0 || (n > 0 && (!z || !x || !c))) return fail(PP_EARG, "bad arrays");
     API_BEGIN
-    const int W = h->W;
-    std::vector<uint64_t> hz(z, z + n * W), hx(x, x + n * W);
-    std::vector<double> hc(c, c + n);
-    auto less = [&](int64_t a, int64_t b) {
-        for (int w = W - 1; w >= 0; --w)
-            if (hz[a * W + w] != hz[b * W + w]) return hz[a * W + w] < hz[b * W + w];
-        for (int w = W - 1; w >= 0; --w)
-            if (hx[a * W + w] != hx[b * W + w]) return hx[a * W + w] < hx[b * W + w];
-        return false;
-    };
-    if (!sorted) {
-        std::vector<int64_t> perm((size_t)n);
-        std::iota(perm.begin(), perm.end(), 0);
-        std::sort(perm.begin(), perm.end(), less);
-        std::vector<uint64_t> sz((size_t)n * W), sx((size_t)n * W);
-        std::vector<double> sc((size_t)n);
-        for (int64_t i = 0; i < n; ++i) {
-            for (int w = 0; w < W; ++w) {
-                sz[i * W + w] = hz[perm[i] * W + w];
-                sx[i * W + w] = hx[perm[i] * W + w];
-            }
-            sc[i] = hc[perm[i]];
-        }
-        hz.swap(sz);
-        hx.swap(sx);
-        hc.swap(sc);
-    }
-    for (int64_t i = 1; i < n; ++i)
-        if (!less(i - 1, i))
-            throw PPError(PP_EARG, sorted ? "keys not strictly increasing" : "duplicate keys in upload");
-    h->n = 0;
-    h->ensure_live(std::max<int64_t>(n, 1));
-    h->m_ratio = 1.0;
-    DevSet& s = h->L[h->cur];
-    if (n > 0) {
-        HIPCHK(hipMemcpyAsync(s.z, hz.data(), (size_t)n * W * 8, hipMemcpyHostToDevice, h->stream));
-        HIPCHK(hipMemcpyAsync(s.x, hx.data(), (size_t)n * W * 8, hipMemcpyHostToDevice, h->stream));
-        HIPCHK(hipMemcpyAsync(s.c, hc.data(), (size_t)n * 8, hipMemcpyHostToDevice, h->stream));
-    }
-    h->h_pin[3] = n;
-    HIPCHK(hipMemcpyAsync(h->d_cnt + h->cur, &h->h_pin[3], sizeof(int64_t), hipMemcpyHostToDevice, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
-    h->n = n;
-    if (n > h->live_cap_v) h->live_cap_v = n;
+    h->bt.upload(0, z, x, c, n, sorted != 0);
     API_END
 }
 
 int pp_download(pp_handle* h, uint64_t* z, uint64_t* x, double* c, int64_t cap, int64_t* n) {
     if (!h || !n) return fail(PP_EARG, "null argument");
     API_BEGIN
-    *n = h->n;
-    if (cap < h->n) throw PPError(PP_EARG, "output capacity too small");
-    const DevSet& s = h->L[h->cur];
-    if (h->n > 0) {
-        HIPCHK(hipMemcpyAsync(z, s.z, (size_t)h->n * h->W * 8, hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(hipMemcpyAsync(x, s.x, (size_t)h->n * h->W * 8, hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(hipMemcpyAsync(c, s.c, (size_t)h->n * 8, hipMemcpyDeviceToHost, h->stream));
-    }
-    HIPCHK(hipStreamSynchronize(h->stream));
+    *n = h->bt.download(0, z, x, c, cap);
     API_END
 }
 
 int pp_size(pp_handle* h, int64_t* n) {
     if (!h || !n) return fail(PP_EARG, "null argument");
-    *n = h->n;
-    return PP_OK;
+    API_BEGIN
+    h->bt.sync_state();
+    *n = h->bt.n_known[0];
+    API_END
 }
 
 int pp_capacities(pp_handle* h, int64_t* live_cap, int64_t* append_cap) {
     if (!h || !live_cap || !append_cap) return fail(PP_EARG, "null argument");
-    *live_cap = h->live_cap_v;
-    *append_cap = h->app_cap_v;
-    return PP_OK;
+    API_BEGIN
+    h->bt.sync_state();
+    *live_cap = h->bt.h_state[0].live_cap_v;
+    *append_cap = h->bt.h_state[0].app_cap_v;
+    API_END
 }
 
 int pp_evolve(pp_handle* h, const uint64_t* gz, const uint64_t* gx, const double* cosv, const double* sinv,
@@ -599,25 +788,54 @@ int pp_evolve(pp_handle* h, const uint64_t* gz, const uint64_t* gx, const double
     if (counters) std::memset(counters, 0, sizeof(*counters));
     if (L == 0) return PP_OK;
     API_BEGIN
-    if (h->W == 1) h->evolve<1>(gz, gx, cosv, sinv, L, strict, local, counters);
-    else h->evolve<2>(gz, gx, cosv, sinv, L, strict, local, counters);
+    Batch& bt = h->bt;
+    bt.timing = counters != nullptr;
+    bt.t_merge = 0.0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (counters) {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, bt.stream));
+    }
+    batch_evolve(bt, L, gz, gx, cosv, sinv, nullptr, strict, local, counters);
+    if (counters) {
+        HIPCHK(hipEventRecord(e1, bt.stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        counters->t_merge = bt.t_merge;
+        counters->t_rotate = ms * 1e-3 - bt.t_merge;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    bt.timing = false;
     API_END
 }
 
 int pp_evolve_single(pp_handle* h, const uint64_t* gz, const uint64_t* gx, double cosv, double sinv) {
     if (!h || !gz || !gx) return fail(PP_EARG, "null argument");
     API_BEGIN
-    if (h->W == 1) h->evolve_single<1>(gz, gx, cosv, sinv);
-    else h->evolve_single<2>(gz, gx, cosv, sinv);
+    // spv_evolve.jl:228-236: grow the append length if short, rotate, merge
+    // with NOFILTER; no early merge.
+    Batch& bt = h->bt;
+    bt.sync_state();
+    SegState& st = bt.h_state[0];
+    const int64_t n = st.n[st.cur];
+    if (n > st.app_cap_v) st.app_cap_v = std::max(2 * st.app_cap_v, n);
+    bt.put_state();
+    std::vector<int32_t> en;
+    std::vector<StepDev> steps = make_steps(bt.W, 1, 1, gz, gx, &cosv, &sinv, nullptr, en);
+    bt.set_filters(nullptr, nullptr);
+    // app_cap_v >= n now, so the sequence-start precheck cannot fire
+    dispatch_w(bt.W, [&](auto wc) { bt.run_sequence<decltype(wc)::value>(steps, 1, en); });
     API_END
 }
 
 int pp_compact(pp_handle* h, const pp_filter* f) {
     if (!h || !f) return fail(PP_EARG, "null argument");
     API_BEGIN
-    h->upload_filters(f, nullptr);
-    if (h->W == 1) h->compact<1>(h->d_filt + 0);
-    else h->compact<2>(h->d_filt + 0);
+    h->bt.set_filters(f, nullptr);
+    dispatch_w(h->bt.W, [&](auto wc) { h->bt.compact<decltype(wc)::value>({1}); });
     API_END
 }
 
@@ -627,69 +845,163 @@ int pp_scale_by_popcount(pp_handle* h, int kind, const uint64_t* mask, const dou
     if (kind < 0 || kind > 4) return fail(PP_EARG, "kind must be in 0:4");
     if (kind == 4 && (!t1 || !t2)) return fail(PP_EARG, "general channel needs three tables");
     API_BEGIN
-    const int nt = h->N + 1;
+    Batch& bt = h->bt;
     std::vector<double> tabs(3 * kExpTab, 0.0);
-    for (int k = 0; k < nt; ++k) {
+    for (int k = 0; k <= h->N; ++k) {
         tabs[k] = t0[k];
         if (kind == 4) {
             tabs[kExpTab + k] = t1[k];
             tabs[2 * kExpTab + k] = t2[k];
         }
     }
-    HIPCHK(hipMemcpyAsync(h->d_tabs, tabs.data(), tabs.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
     ChanDev ch{};
     ch.kind = kind;
-    for (int w = 0; w < h->W; ++w) ch.mask[w] = mask[w];
-    ch.t0 = h->d_tabs;
-    ch.t1 = h->d_tabs + kExpTab;
-    ch.t2 = h->d_tabs + 2 * kExpTab;
-    DevSet& s = h->L[h->cur];
-    if (h->n > 0) {
-        const int grid = h->grid_for(h->n, kBlock * 4);
-        if (h->W == 1)
-            hipLaunchKernelGGL(k_channel<1>, dim3(grid), dim3(kBlock), 0, h->stream, s.z, s.x, s.c, h->d_cnt + h->cur, ch);
-        else
-            hipLaunchKernelGGL(k_channel<2>, dim3(grid), dim3(kBlock), 0, h->stream, s.z, s.x, s.c, h->d_cnt + h->cur, ch);
-        HIPCHK(hipGetLastError());
-    }
-    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int w = 0; w < bt.W; ++w) ch.mask[w] = mask[w];
+    ch.t0 = bt.d_tabs;
+    ch.t1 = bt.d_tabs + kExpTab;
+    ch.t2 = bt.d_tabs + 2 * kExpTab;
+    dispatch_w(bt.W, [&](auto wc) { bt.channel<decltype(wc)::value>({ch}, tabs, {1}); });
+    HIPCHK(hipStreamSynchronize(bt.stream));
     API_END
 }
 
 int pp_expectation_ket(pp_handle* h, const uint64_t* ket, double* out) {
     if (!h || !ket || !out) return fail(PP_EARG, "null argument");
     API_BEGIN
-    const DevSet& s = h->L[h->cur];
-    const int np = (int)std::min<int64_t>(std::max<int64_t>(1, cdiv(h->n, kBlock * 8)), 4096);
-    const uint64_t k0 = ket[0], k1 = h->W == 2 ? ket[1] : 0;
-    if (h->W == 1)
-        hipLaunchKernelGGL(k_expect<1>, dim3(np), dim3(kBlock), 0, h->stream, s.z, s.x, s.c, h->d_cnt + h->cur, k0, k1,
-                           h->d_partial);
-    else
-        hipLaunchKernelGGL(k_expect<2>, dim3(np), dim3(kBlock), 0, h->stream, s.z, s.x, s.c, h->d_cnt + h->cur, k0, k1,
-                           h->d_partial);
-    hipLaunchKernelGGL(k_expect_final, dim3(1), dim3(kBlock), 0, h->stream, h->d_partial, np, h->d_out);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(out, h->d_out, sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
+    dispatch_w(h->bt.W, [&](auto wc) { h->bt.expectation<decltype(wc)::value>(ket, out); });
     API_END
 }
 
 int pp_check(pp_handle* h, int* ok) {
     if (!h || !ok) return fail(PP_EARG, "null argument");
     API_BEGIN
-    HIPCHK(hipMemsetAsync(h->d_ovf + 1, 0, sizeof(int32_t), h->stream));
-    const DevSet& s = h->L[h->cur];
-    const int grid = h->grid_for(h->n, kBlock * 4);
-    if (h->W == 1)
-        hipLaunchKernelGGL(k_check<1>, dim3(grid), dim3(kBlock), 0, h->stream, s.z, s.x, h->d_cnt + h->cur, h->d_ovf + 1);
-    else
-        hipLaunchKernelGGL(k_check<2>, dim3(grid), dim3(kBlock), 0, h->stream, s.z, s.x, h->d_cnt + h->cur, h->d_ovf + 1);
-    HIPCHK(hipGetLastError());
-    int32_t bad = 0;
-    HIPCHK(hipMemcpyAsync(&bad, h->d_ovf + 1, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
-    *ok = bad ? 0 : 1;
+    int32_t o = 0;
+    dispatch_w(h->bt.W, [&](auto wc) { h->bt.check<decltype(wc)::value>(&o); });
+    *ok = o;
+    API_END
+}
+
+// --- batches -----------------------------------------------------------------
+
+int pp_batch_create(int B, const int* N, const int64_t* live_cap, const int64_t* append_cap, pp_batch** out) {
+    if (!out || !N || !live_cap || !append_cap) return fail(PP_EARG, "null argument");
+    *out = nullptr;
+    if (B < 1) return fail(PP_EARG, "B must be >= 1");
+    int maxN = 0;
+    for (int b = 0; b < B; ++b) {
+        if (N[b] < 1 || N[b] > 128) return fail(PP_EARG, "N must be in 1:128");
+        if (live_cap[b] < 1 || append_cap[b] < 1) return fail(PP_EARG, "capacities must be >= 1");
+        maxN = std::max(maxN, N[b]);
+    }
+    pp_batch* p = new pp_batch();
+    try {
+        p->bt.init(maxN <= 64 ? 1 : 2, std::vector<int>(N, N + B), std::vector<int64_t>(live_cap, live_cap + B),
+                   std::vector<int64_t>(append_cap, append_cap + B));
+    } catch (const PPError& e) {
+        delete p;
+        return fail(e.code, e.msg);
+    }
+    *out = p;
+    g_last_error.clear();
+    return PP_OK;
+}
+
+int pp_batch_destroy(pp_batch* p) {
+    if (!p) return PP_OK;
+    (void)hipStreamSynchronize(p->bt.stream);
+    delete p;
+    return PP_OK;
+}
+
+int pp_batch_words(pp_batch* p, int* W) {
+    if (!p || !W) return fail(PP_EARG, "null argument");
+    *W = p->bt.W;
+    return PP_OK;
+}
+
+int pp_batch_upload(pp_batch* p, int b, const uint64_t* z, const uint64_t* x, const double* c, int64_t n, int sorted) {
+    if (!p) return fail(PP_EARG, "null handle");
+    if (b < 0 || b >= p->bt.B) return fail(PP_EARG, "vector index out of range");
+    if (n < 0 || (n > 0 && (!z || !x || !c))) return fail(PP_EARG, "bad arrays");
+    API_BEGIN
+    p->bt.upload(b, z, x, c, n, sorted != 0);
+    API_END
+}
+
+int pp_batch_download(pp_batch* p, int b, uint64_t* z, uint64_t* x, double* c, int64_t cap, int64_t* n) {
+    if (!p || !n) return fail(PP_EARG, "null argument");
+    if (b < 0 || b >= p->bt.B) return fail(PP_EARG, "vector index out of range");
+    API_BEGIN
+    *n = p->bt.download(b, z, x, c, cap);
+    API_END
+}
+
+int pp_batch_sizes(pp_batch* p, int64_t* n) {
+    if (!p || !n) return fail(PP_EARG, "null argument");
+    API_BEGIN
+    p->bt.sync_state();
+    for (int b = 0; b < p->bt.B; ++b) n[b] = p->bt.n_known[b];
+    API_END
+}
+
+int pp_batch_evolve(pp_batch* p, int64_t nsteps, const uint64_t* gz, const uint64_t* gx, const double* cosv,
+                    const double* sinv, const uint8_t* active, const pp_filter* strict, const pp_filter* local,
+                    pp_counters* counters) {
+    if (!p) return fail(PP_EARG, "null handle");
+    if (nsteps < 0) return fail(PP_EARG, "nsteps must be >= 0");
+    if (nsteps > 0 && (!gz || !gx || !cosv || !sinv)) return fail(PP_EARG, "null generator arrays");
+    if (counters) std::memset(counters, 0, sizeof(pp_counters) * (size_t)p->bt.B);
+    if (nsteps == 0) return PP_OK;
+    API_BEGIN
+    batch_evolve(p->bt, nsteps, gz, gx, cosv, sinv, active, strict, local, counters);
+    API_END
+}
+
+int pp_batch_scale_by_popcount(pp_batch* p, const int* kind, const uint64_t* mask, const double* tables) {
+    if (!p || !kind || !mask || !tables) return fail(PP_EARG, "null argument");
+    API_BEGIN
+    Batch& bt = p->bt;
+    std::vector<double> tabs((size_t)bt.B * 3 * kExpTab, 0.0);
+    std::vector<ChanDev> ch((size_t)bt.B);
+    std::vector<int32_t> en((size_t)bt.B, 0);
+    for (int b = 0; b < bt.B; ++b) {
+        if (kind[b] < -1 || kind[b] > 4) throw PPError(PP_EARG, "kind must be in -1:4");
+        ch[b].kind = kind[b];
+        en[b] = kind[b] >= 0;
+        for (int w = 0; w < bt.W; ++w) ch[b].mask[w] = mask[(size_t)b * bt.W + w];
+        for (int t = 0; t < 3; ++t)
+            for (int k = 0; k <= bt.N[b]; ++k)
+                tabs[((size_t)b * 3 + t) * kExpTab + k] = tables[((size_t)b * 3 + t) * 129 + k];
+        ch[b].t0 = bt.d_tabs + ((size_t)b * 3 + 0) * kExpTab;
+        ch[b].t1 = bt.d_tabs + ((size_t)b * 3 + 1) * kExpTab;
+        ch[b].t2 = bt.d_tabs + ((size_t)b * 3 + 2) * kExpTab;
+    }
+    dispatch_w(bt.W, [&](auto wc) { bt.channel<decltype(wc)::value>(ch, tabs, en); });
+    HIPCHK(hipStreamSynchronize(bt.stream));
+    API_END
+}
+
+int pp_batch_set_timing(pp_batch* p, int on) {
+    if (!p) return fail(PP_EARG, "null handle");
+    p->bt.timing = on != 0;
+    return PP_OK;
+}
+
+int pp_batch_merge_time(pp_batch* p, double* seconds, int64_t* launches, int reset) {
+    if (!p || !seconds || !launches) return fail(PP_EARG, "null argument");
+    *seconds = p->bt.t_merge;
+    *launches = p->bt.merge_launches;
+    if (reset) {
+        p->bt.t_merge = 0.0;
+        p->bt.merge_launches = 0;
+    }
+    return PP_OK;
+}
+
+int pp_batch_expectation_ket(pp_batch* p, const uint64_t* kets, double* out) {
+    if (!p || !kets || !out) return fail(PP_EARG, "null argument");
+    API_BEGIN
+    dispatch_w(p->bt.W, [&](auto wc) { p->bt.expectation<decltype(wc)::value>(kets, out); });
     API_END
 }
 

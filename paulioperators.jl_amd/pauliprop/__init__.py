@@ -16,3 +16,4 @@ from .spv import (SparsePauliVector, bit_flip_channel, bit_flip_channel_, bit_ph
                   bit_phase_flip_channel_, dephasing_channel, dephasing_channel_, depolarizing_channel,
                   depolarizing_channel_, depolarizing_p_for_weight_decay, evolve, evolve_, expectation_value,
                   pauli_channel, pauli_channel_, phase_flip_channel_, truncate_)
+from .batch import SparsePauliBatch

@@ -18,6 +18,9 @@ PP_OK, PP_EARG, PP_ENOMEM, PP_EHIP, PP_EOVERFLOW, PP_EUNSUPPORTED = range(6)
 
 EXPORTS = ("pp_create", "pp_destroy", "pp_upload", "pp_download", "pp_size", "pp_capacities", "pp_evolve",
            "pp_evolve_single", "pp_compact", "pp_scale_by_popcount", "pp_expectation_ket", "pp_check",
+           "pp_batch_create", "pp_batch_destroy", "pp_batch_words", "pp_batch_upload", "pp_batch_download",
+           "pp_batch_sizes", "pp_batch_evolve", "pp_batch_scale_by_popcount", "pp_batch_expectation_ket",
+           "pp_batch_set_timing", "pp_batch_merge_time",
            "pp_julia_powi", "pp_last_error", "pp_version")
 
 
@@ -74,6 +77,19 @@ def lib():
     L.pp_scale_by_popcount.argtypes = [vp, ctypes.c_int, u64p, dp, dp, dp]
     L.pp_expectation_ket.argtypes = [vp, u64p, dp]
     L.pp_check.argtypes = [vp, P(ctypes.c_int)]
+    ip = P(ctypes.c_int)
+    L.pp_batch_create.argtypes = [ctypes.c_int, ip, P(i64), P(i64), P(vp)]
+    L.pp_batch_destroy.argtypes = [vp]
+    L.pp_batch_words.argtypes = [vp, ip]
+    L.pp_batch_upload.argtypes = [vp, ctypes.c_int, u64p, u64p, dp, i64, ctypes.c_int]
+    L.pp_batch_download.argtypes = [vp, ctypes.c_int, u64p, u64p, dp, i64, P(i64)]
+    L.pp_batch_sizes.argtypes = [vp, P(i64)]
+    L.pp_batch_evolve.argtypes = [vp, i64, u64p, u64p, dp, dp, P(ctypes.c_uint8), P(PPFilter), P(PPFilter),
+                                  P(PPCounters)]
+    L.pp_batch_scale_by_popcount.argtypes = [vp, ip, u64p, dp]
+    L.pp_batch_expectation_ket.argtypes = [vp, u64p, dp]
+    L.pp_batch_set_timing.argtypes = [vp, ctypes.c_int]
+    L.pp_batch_merge_time.argtypes = [vp, dp, P(i64), ctypes.c_int]
     L.pp_julia_powi.argtypes = [ctypes.c_double, i64]
     L.pp_julia_powi.restype = ctypes.c_double
     L.pp_last_error.restype = ctypes.c_char_p

@@ -293,3 +293,45 @@ def test_child_buffer_regrowth_after_ratio_jump():
         g.evolve_(PauliBasis(N, gz, gx), th)
         o.evolve_single(gz, gx, th)
         assert_same(g, o, f"regrowth G=({gz:x},{gx:x})")
+
+
+def test_batch_uneven_vectors_bitexact():
+    # Vectors of different qubit counts (one word type), circuits of different
+    # lengths, per-vector truncation and channels: each must equal its own
+    # single-vector oracle run.
+    from pauliprop import SparsePauliBatch
+    rng = random.Random(33)
+    Ns = [8, 20, 64, 40, 13, 64]
+    B = len(Ns)
+    gens, angs, strats = [], [], []
+    obs = []
+    for b, N in enumerate(Ns):
+        q = rng.randint(1, N)
+        obs.append([(Pauli(N, Z=[q]).z, 0, 1.0)])
+        L = rng.randint(30, 200)
+        g, a = [], []
+        for _ in range(L):
+            kind = rng.randrange(3)
+            j = rng.randint(1, N - 1)
+            p = Pauli(N, Z=[j, j + 1]) if kind == 0 else (Pauli(N, X=[j]) if kind == 1 else Pauli(N, Z=[j]))
+            g.append(p)
+            a.append(rng.uniform(0, math.pi))
+        gens.append(g)
+        angs.append(a)
+        strats.append(CompositeTruncation(CoeffTruncation(10 ** rng.uniform(-6, -3)), WeightTruncation(rng.randint(4, 12))))
+    bt = SparsePauliBatch(Ns, [[((z, x), c) for z, x, c in o] for o in obs])
+    orcs = [po.OracleSPV(N, o) for N, o in zip(Ns, obs)]
+    for layer in range(3):
+        log(f"batch layer {layer} sizes={bt.sizes()}")
+        bt.evolve_(gens, angs, truncation=strats)
+        bt.channel_(["depolarizing" if b % 2 == 0 else "dephasing" for b in range(B)], [1e-3] * B)
+        for b in range(B):
+            orcs[b].evolve([(p.z, p.x) for p in gens[b]], angs[b], strict=compile_filter(strats[b]))
+            orcs[b].channel("depolarizing" if b % 2 == 0 else "dephasing", 1e-3)
+    evs = bt.expectation_values()
+    for b in range(B):
+        gz, gx, gc = bt.arrays(b)
+        oz, ox, oc = orcs[b].arrays()
+        assert np.array_equal(gz, oz) and np.array_equal(gx, ox), f"vector {b} keys"
+        assert np.array_equal(gc.view(np.uint64), oc.view(np.uint64)), f"vector {b} coefficients"
+        assert abs(evs[b] - orcs[b].expectation(0)) <= 1e-12
