@@ -39,13 +39,15 @@ def sweep_gpu(workloads, timing=True):
     from pauliprop import PPCounters, SparsePauliBatch
     bt = SparsePauliBatch([w.N for w in workloads],
                           [[((p.z, p.x), c) for p, c in w.observable] for w in workloads])
+    bt.reserve(1 << 18)
     bt.set_timing(timing)
     tot = PPCounters()
     nl = len(workloads[0].layers)
+    packed = [bt.pack([w.layers[li].generators for w in workloads], [w.layers[li].angles for w in workloads])
+              for li in range(nl)]
     t0 = time.perf_counter()
     for li in range(nl):
-        cnts = bt.evolve_([w.layers[li].generators for w in workloads], [w.layers[li].angles for w in workloads],
-                          truncation=workloads[0].truncation, counters=True)
+        cnts = bt.evolve_packed(packed[li], truncation=workloads[0].truncation, counters=True)
         bt.channel_(workloads[0].channel[0], workloads[0].channel[1])
         for c in cnts:
             for f, _ in c._fields_:

@@ -135,6 +135,12 @@ int pp_batch_evolve(pp_batch* p, int64_t nsteps, const uint64_t* gz, const uint6
 int pp_batch_scale_by_popcount(pp_batch* p, const int* kind, const uint64_t* mask, const double* tables);
 /* out[b] = ⟨ψ_b|v_b|ψ_b⟩ for kets[b*W ..]. */
 int pp_batch_expectation_ket(pp_batch* p, const uint64_t* kets, double* out);
+/* Pre-size device buffers: vector b holds cap[b] live terms without regrowth
+ * (buffers grow on demand regardless; this only avoids stall-and-replay). */
+int pp_batch_reserve(pp_batch* p, const int64_t* cap);
+/* Tile shapes: sweep tile 512 or 2048 terms, merge tile 512, 1024 or 2048
+ * outputs (2048 only for N <= 64); 0 = chosen from the live counts. */
+int pp_batch_set_tiles(pp_batch* p, int tile, int merge_tile);
 /* HIP-event timing of every merge-kernel launch (off by default). */
 int pp_batch_set_timing(pp_batch* p, int on);
 int pp_batch_merge_time(pp_batch* p, double* seconds, int64_t* launches, int reset);

@@ -6,17 +6,16 @@ namespace pp {
 
 constexpr int kWave = 64;          // CDNA wavefront width
 constexpr int kBlock = 256;        // 4 waves per workgroup
-constexpr int kTile = 2048;        // sweep tile: 32 flag words, 8 elements per lane
-constexpr int kTileWords = kTile / 64;
+constexpr int kTileMin = 512;      // smallest sweep tile (scratch is sized for it)
 constexpr int kMaxLevels = 256;    // set bits of a combined (z, x) mask, N <= 128
 constexpr int kExpTab = 257;       // exp(-alpha * w) for w = 0..256
 constexpr int16_t kHdbNone = 0x3fff;  // "no neighbour": above every bit position
 
-// Merge tile (outputs per workgroup) per word count: the LDS tile holds the L
-// and C slices of one merge-path window (keys + coefficients).
-template <int W> struct MergeTile;
-template <> struct MergeTile<1> { static constexpr int kSize = 2048; };
-template <> struct MergeTile<2> { static constexpr int kSize = 1024; };
+// Tile shapes are template parameters of the kernels: the sweep tile T
+// (terms per workgroup in kb_flags/kb_emit: 512 or 2048) and the merge tile
+// TM (outputs per workgroup in kb_merge: 512, 1024, 2048 for W = 1; 512, 1024
+// for W = 2, whose LDS window holds TM x 40 B).  The host picks them per call
+// from the autotuned table.
 
 // One rotation, prepared on the host (spv_evolve.jl:333-343 computes the same
 // packed generator, n_g = popcount(gz & gx), cos θ and sin θ once per generator).

@@ -190,12 +190,13 @@ struct GenReg {
 
 
 // Global exclusive A-prefix at live position p (0 <= p <= n).
+template <int T>
 __device__ __forceinline__ int64_t a_prefix(int64_t p, int64_t n, int64_t ntiles, const uint64_t* __restrict__ flags,
                                             const uint16_t* __restrict__ wpre, const int64_t* __restrict__ tileoff) {
     if (p >= n) return tileoff[ntiles];
     const int64_t w = p >> 6;
     const uint64_t lowmask = (p & 63) ? ((~0ull) >> (64 - (p & 63))) : 0ull;
-    return tileoff[p / kTile] + (int64_t)wpre[w] + popc64(flags[w] & lowmask);
+    return tileoff[p / T] + (int64_t)wpre[w] + popc64(flags[w] & lowmask);
 }
 
 // First q <= p whose key agrees with K_p on every bit >= b.
@@ -247,6 +248,93 @@ __device__ int64_t seg_end(const uint64_t* __restrict__ Z, const uint64_t* __res
 }
 
 
+// Wave-cooperative versions: all 64 lanes call with the same arguments and
+// get the same result.  Each step probes 64 positions at once (a galloping
+// search with stride 64^k, then a 64-ary refinement), so a segment of length
+// L costs about 2 log_64(L) dependent load rounds instead of 2 log_2(L).
+
+// First q <= p whose key agrees with kp = K_p on every bit >= b.
+template <int W>
+__device__ int64_t seg_start_wave(const uint64_t* __restrict__ Z, const uint64_t* __restrict__ X, int64_t p, int b,
+                                  const Key<W>& kp) {
+    const int lane = threadIdx.x % kWave;
+    int64_t in = p, out = -1, stride = 1;
+    while (true) {
+        const int64_t q = in - (int64_t)(lane + 1) * stride;
+        const bool inside = q >= 0 && hdb<W>(load_key<W>(Z, X, q), kp) < b;
+        const uint64_t outm = __ballot(!inside);
+        if (outm == 0) {
+            in -= (int64_t)kWave * stride;
+            stride *= kWave;
+            continue;
+        }
+        const int f = __ffsll((unsigned long long)outm) - 1;
+        const int64_t qf = in - (int64_t)(f + 1) * stride;
+        out = qf < 0 ? -1 : qf;
+        in = in - (int64_t)f * stride;
+        break;
+    }
+    while (in - out > 1) {
+        const int64_t step = (in - out - 1 + kWave - 1) / kWave;
+        const int64_t q = out + step * (int64_t)(lane + 1);
+        const bool valid = q < in;
+        const bool inside = valid && hdb<W>(load_key<W>(Z, X, q), kp) < b;
+        const uint64_t inm = __ballot(inside);
+        if (inm == 0) {
+            int64_t kmax = (in - out - 1) / step;
+            if (kmax > kWave) kmax = kWave;
+            out = out + step * kmax;
+        } else {
+            const int g = __ffsll((unsigned long long)inm) - 1;
+            const int64_t qg = out + step * (int64_t)(g + 1);
+            out = g == 0 ? out : out + step * (int64_t)g;
+            in = qg;
+        }
+    }
+    return in;
+}
+
+// First r > p whose key differs from kp = K_p on some bit >= b (n if none).
+template <int W>
+__device__ int64_t seg_end_wave(const uint64_t* __restrict__ Z, const uint64_t* __restrict__ X, int64_t p, int64_t n,
+                                int b, const Key<W>& kp) {
+    const int lane = threadIdx.x % kWave;
+    int64_t in = p, out = n, stride = 1;
+    while (true) {
+        const int64_t q = in + (int64_t)(lane + 1) * stride;
+        const bool inside = q < n && hdb<W>(load_key<W>(Z, X, q), kp) < b;
+        const uint64_t outm = __ballot(!inside);
+        if (outm == 0) {
+            in += (int64_t)kWave * stride;
+            stride *= kWave;
+            continue;
+        }
+        const int f = __ffsll((unsigned long long)outm) - 1;
+        const int64_t qf = in + (int64_t)(f + 1) * stride;
+        out = qf > n ? n : qf;
+        in = in + (int64_t)f * stride;
+        break;
+    }
+    while (out - in > 1) {
+        const int64_t step = (out - in - 1 + kWave - 1) / kWave;
+        const int64_t q = in + step * (int64_t)(lane + 1);
+        const bool valid = q < out;
+        const bool outside = valid && !(hdb<W>(load_key<W>(Z, X, q), kp) < b);
+        const uint64_t om = __ballot(outside);
+        if (om == 0) {
+            int64_t kmax = (out - in - 1) / step;
+            if (kmax > kWave) kmax = kWave;
+            in = in + step * kmax;
+        } else {
+            const int h = __ffsll((unsigned long long)om) - 1;
+            const int64_t qh = in + step * (int64_t)(h + 1);
+            in = h == 0 ? in : in + step * (int64_t)h;
+            out = qh;
+        }
+    }
+    return out;
+}
+
 struct ChanDev {
     int kind;
     uint64_t mask[2];
@@ -279,9 +367,43 @@ __device__ __forceinline__ double chan_factor(const ChanDev& ch, const Key<W>& k
     return f * ch.t2[c];
 }
 
+// Merge look-back words: [flag:2][epoch:22][count:40].  The epoch is the
+// launch number, so words left by earlier launches read as "not ready" and
+// the array never needs clearing.
 constexpr uint64_t kFlagAgg = 1ull << 62;
 constexpr uint64_t kFlagPre = 2ull << 62;
-constexpr uint64_t kValMask = (1ull << 62) - 1;
+constexpr int kEpochShift = 40;
+constexpr uint64_t kEpochMask = (1ull << 22) - 1;
+constexpr uint64_t kCountMask = (1ull << kEpochShift) - 1;
+
+// Merge-path split: the number of live terms among the first d outputs of
+// the merge of L (n) and C (m), live first on equal keys.  Wave-cooperative
+// 64-ary search over the monotone predicate L[i] <= C[d-1-i].
+template <int W>
+__device__ int64_t merge_split_wave(const uint64_t* __restrict__ Lz, const uint64_t* __restrict__ Lx, int64_t n,
+                                    const uint64_t* __restrict__ Cz, const uint64_t* __restrict__ Cx, int64_t m,
+                                    int64_t d) {
+    const int lane = threadIdx.x % kWave;
+    int64_t lo = d - m > 0 ? d - m : 0;
+    int64_t hi = d < n ? d : n;
+    while (lo < hi) {
+        const int64_t step = (hi - lo + kWave - 1) / kWave;
+        const int64_t i = lo + step * (int64_t)lane;
+        const bool valid = i < hi;
+        const bool pred = valid && key_le<W>(load_key<W>(Lz, Lx, i), load_key<W>(Cz, Cx, d - 1 - i));
+        const uint64_t tm = __ballot(pred);
+        const uint64_t vm = __ballot(valid);
+        if (tm == vm) {
+            lo = lo + step * (int64_t)(__popcll(vm) - 1) + 1;
+        } else {
+            const int g = __ffsll((unsigned long long)(~tm & vm)) - 1;
+            const int64_t ig = lo + step * (int64_t)g;
+            lo = g == 0 ? lo : lo + step * (int64_t)(g - 1) + 1;
+            hi = ig;
+        }
+    }
+    return lo;
+}
 
 
 __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {

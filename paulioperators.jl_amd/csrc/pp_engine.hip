@@ -178,6 +178,7 @@ struct Batch {
     size_t evused = 0;
     double t_merge = 0.0;
     int64_t merge_launches = 0;
+    uint32_t epoch = 0;  // merge launch number, tags the look-back words
 
     ~Batch() {
         for (auto& b : ib) free_inst(b);
@@ -209,7 +210,17 @@ struct Batch {
     }
 
     size_t kb() const { return sizeof(uint64_t) * (size_t)W; }
-    int tm() const { return W == 1 ? MergeTile<1>::kSize : MergeTile<2>::kSize; }
+    // Tile shapes of the next launches (0 = choose from the live counts).
+    int tile_req = 0, mtile_req = 0;
+    int tile = 2048, mtile = 1024;
+    void choose_tiles() {
+        int64_t nmax = 0;
+        for (int64_t v : n_known) nmax = std::max(nmax, v);
+        const bool small = nmax < (1 << 17);
+        tile = tile_req ? tile_req : (small ? 512 : 2048);
+        mtile = mtile_req ? mtile_req : (small ? (W == 1 ? 1024 : 512) : (W == 1 ? 2048 : 1024));
+    }
+    int tm() const { return mtile; }
 
     void alloc_set(DevSet& s, int64_t cap) {
         s.z = dalloc<uint64_t>((size_t)cap * W);
@@ -261,6 +272,7 @@ struct Batch {
         HIPCHK(hipMemcpyAsync(d_state, h_state, sizeof(SegState) * (size_t)B, hipMemcpyHostToDevice, stream));
         std::vector<FilterDev> f(2 * (size_t)B, make_filter_dev(nullptr));
         HIPCHK(hipMemcpyAsync(d_filt, f.data(), f.size() * sizeof(FilterDev), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
         for (int b = 0; b < B; ++b) ensure_caps(b, 4096, 4096, 2, 0);
         HIPCHK(hipStreamSynchronize(stream));
         upload_desc();
@@ -290,12 +302,12 @@ struct Batch {
             free_set(I.C);
             alloc_set(I.C, std::max<int64_t>(capC, 1024));
         }
-        const int64_t tiles = cdiv(I.L[0].cap, kTile) + 2;
+        const int64_t tiles = cdiv(I.L[0].cap, kTileMin) + 2;
         if (I.cap_tiles < tiles) {
             dfree(I.flags); dfree(I.wpre); dfree(I.tilecnt); dfree(I.tileoff);
             I.cap_tiles = tiles;
-            I.flags = dalloc<uint64_t>((size_t)tiles * kTileWords);
-            I.wpre = dalloc<uint16_t>((size_t)tiles * kTileWords);
+            I.flags = dalloc<uint64_t>((size_t)tiles * (kTileMin / 64));
+            I.wpre = dalloc<uint16_t>((size_t)tiles * (kTileMin / 64));
             I.tilecnt = dalloc<int32_t>(tiles);
             I.tileoff = dalloc<int64_t>(tiles + 1);
         }
@@ -306,7 +318,7 @@ struct Batch {
             I.bnd = dalloc<int64_t>(I.cap_bnd);
         }
         I.maxlev = ml;
-        const int64_t sp = cdiv(I.L[0].cap + I.C.cap, tm()) + 2;
+        const int64_t sp = cdiv(I.L[0].cap + I.C.cap, 512) + 2;
         if (I.cap_split < sp) {
             dfree(I.split); dfree(I.status);
             I.cap_split = sp;
@@ -342,7 +354,9 @@ struct Batch {
     // Block prefixes per kernel family from the expected live count (blocks
     // only set parallelism: every kernel loops over the vector's real tiles).
     void plan() {
-        const int T2 = tm();
+        choose_tiles();
+        const int T2 = mtile;
+        const int kTile = tile;
         int64_t acc[4] = {0, 0, 0, 0};
         for (int b = 0; b <= B; ++b) {
             for (int k = 0; k < 4; ++k) h_maps[(size_t)k * (B + 1) + b] = acc[k];
@@ -350,7 +364,7 @@ struct Batch {
             const int64_t cap = std::min(ib[b].L[0].cap, ib[b].L[1].cap);
             const int64_t nest = std::min<int64_t>(cap, std::max<int64_t>(4 * n_known[b], 8192));
             acc[0] += std::max<int64_t>(1, std::min<int64_t>(cdiv(nest, kTile), (int64_t)ncu * 8));
-            acc[1] += std::max<int64_t>(1, std::min<int64_t>(cdiv(2 * cdiv(nest, kTile), kBlock), (int64_t)ncu * 4));
+            acc[1] += std::max<int64_t>(1, std::min<int64_t>(cdiv(2 * cdiv(nest, kTile), kBlock / kWave), (int64_t)ncu * 4));
             acc[2] += std::max<int64_t>(1, std::min<int64_t>(cdiv(cdiv(2 * nest, T2) + 1, kBlock), (int64_t)ncu * 4));
             acc[3] += std::max<int64_t>(1, std::min<int64_t>(cdiv(2 * nest, T2), (int64_t)ncu * 4));
         }
@@ -365,7 +379,19 @@ struct Batch {
     BlockMap map(int k) const { return BlockMap{d_maps + (size_t)k * (B + 1), B}; }
     unsigned blocks(int k) const { return (unsigned)h_maps[(size_t)k * (B + 1) + B]; }
 
+    std::vector<pp_filter> last_filters;  // what d_filt holds (strict, local per vector)
+
     void set_filters(const pp_filter* strict, const pp_filter* local) {
+        static const pp_filter kNo = {INT64_MAX, INT64_MAX, INT64_MAX, -1.0, 0.0, -1.0, 0.0, -1.0};
+        std::vector<pp_filter> want(2 * (size_t)B);
+        for (int b = 0; b < B; ++b) {
+            want[2 * b] = strict ? strict[b] : kNo;
+            want[2 * b + 1] = local ? local[b] : kNo;
+        }
+        if (want.size() == last_filters.size() &&
+            std::memcmp(want.data(), last_filters.data(), want.size() * sizeof(pp_filter)) == 0)
+            return;
+        last_filters = want;
         std::vector<FilterDev> f(2 * (size_t)B);
         for (int b = 0; b < B; ++b) {
             f[2 * b] = make_filter_dev(strict ? strict + b : nullptr);
@@ -382,26 +408,57 @@ struct Batch {
         d_steps = dalloc<StepDev>(cap_steps);
     }
 
-    template <int Wt>
+    // Next merge epoch; when the 22-bit tag wraps, every look-back array is
+    // cleared first so that no word of an older launch can carry the new tag.
+    uint32_t next_epoch() {
+        epoch = (epoch + 1) & (uint32_t)kEpochMask;
+        if (epoch == 0) {
+            for (auto& I : ib)
+                if (I.status) HIPCHK(hipMemsetAsync(I.status, 0, (size_t)I.cap_split * sizeof(uint64_t), stream));
+            epoch = 1;
+        }
+        return epoch;
+    }
+
+    template <int Wt, int T, int TM>
     void launch_step(const StepDev* dsteps, int64_t step_index) {
-        hipLaunchKernelGGL(kb_flags<Wt>, dim3(blocks(0)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(0));
-        hipLaunchKernelGGL(kb_scan, dim3(B), dim3(1024), 0, stream, d_desc, d_state, dsteps, step_index);
-        hipLaunchKernelGGL(kb_bounds<Wt>, dim3(blocks(1)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(1));
-        hipLaunchKernelGGL(kb_emit<Wt>, dim3(blocks(0)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(0));
-        hipLaunchKernelGGL(kb_partition<Wt>, dim3(blocks(2)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(2), 0);
+        hipLaunchKernelGGL((kb_flags<Wt, T>), dim3(blocks(0)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(0));
+        hipLaunchKernelGGL(kb_scan<T>, dim3(B), dim3(1024), 0, stream, d_desc, d_state, dsteps, step_index);
+        hipLaunchKernelGGL((kb_bounds<Wt, T>), dim3(blocks(1)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(1));
+        hipLaunchKernelGGL((kb_emit<Wt, T>), dim3(blocks(0)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(0));
+        const uint32_t ep = next_epoch();
         if (timing) HIPCHK(hipEventRecord(next_event(), stream));
-        hipLaunchKernelGGL(kb_merge<Wt>, dim3(blocks(3)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(3), 0,
-                           step_index);
+        hipLaunchKernelGGL((kb_merge<Wt, TM>), dim3(blocks(3)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(3),
+                           0, step_index, ep);
         if (timing) HIPCHK(hipEventRecord(next_event(), stream));
     }
 
-    template <int Wt>
+    template <int Wt, int TM>
     void launch_compact() {
         // the step table is not read in compaction mode but must be a valid pointer
         ensure_steps(B);
-        hipLaunchKernelGGL(kb_partition<Wt>, dim3(blocks(2)), dim3(kBlock), 0, stream, d_desc, d_state, d_steps, map(2), 1);
-        hipLaunchKernelGGL(kb_merge<Wt>, dim3(blocks(3)), dim3(kBlock), 0, stream, d_desc, d_state, d_steps, map(3), 1,
-                           (int64_t)-1);
+        const uint32_t ep = next_epoch();
+        hipLaunchKernelGGL((kb_merge<Wt, TM>), dim3(blocks(3)), dim3(kBlock), 0, stream, d_desc, d_state, d_steps, map(3),
+                           1, (int64_t)-1, ep);
+    }
+
+    // Run f<T, TM>() with the current tile shapes as template arguments.
+    template <int Wt, typename F>
+    void with_tiles(F&& f) {
+        using I = std::integral_constant<int, 512>;
+        using J = std::integral_constant<int, 1024>;
+        using K = std::integral_constant<int, 2048>;
+        if (tile == 512) {
+            if (mtile == 512) f(I(), I());
+            else if (mtile == 1024) f(I(), J());
+            else if (Wt == 1) f(I(), K());
+            else throw PPError(PP_EARG, "merge tile 2048 needs N <= 64");
+        } else {
+            if (mtile == 512) f(K(), I());
+            else if (mtile == 1024) f(K(), J());
+            else if (Wt == 1) f(K(), K());
+            else throw PPError(PP_EARG, "merge tile 2048 needs N <= 64");
+        }
     }
 
     void sync_state() {
@@ -443,8 +500,10 @@ struct Batch {
                               stream));
         HIPCHK(hipMemcpyAsync(d_en, en.data(), (size_t)B * sizeof(int32_t), hipMemcpyHostToDevice, stream));
         hipLaunchKernelGGL(kb_precheck, dim3((unsigned)cdiv(B, 256)), dim3(256), 0, stream, d_state, d_en, B);
-        launch_compact<Wt>();
-        for (int64_t s = 0; s < nsteps; ++s) launch_step<Wt>(d_steps + s * B, s);
+        with_tiles<Wt>([&](auto T, auto TM) {
+            launch_compact<Wt, decltype(TM)::value>();
+            for (int64_t s = 0; s < nsteps; ++s) launch_step<Wt, decltype(T)::value, decltype(TM)::value>(d_steps + s * B, s);
+        });
         HIPCHK(hipGetLastError());
         sync_state();
         for (int round = 0;; ++round) {
@@ -484,7 +543,10 @@ struct Batch {
             upload_desc();
             plan();
             HIPCHK(hipMemcpyAsync(d_steps, rs.data(), rs.size() * sizeof(StepDev), hipMemcpyHostToDevice, stream));
-            for (int64_t s = first; s < nsteps; ++s) launch_step<Wt>(d_steps + (s - first) * B, s);
+            with_tiles<Wt>([&](auto T, auto TM) {
+                for (int64_t s = first; s < nsteps; ++s)
+                    launch_step<Wt, decltype(T)::value, decltype(TM)::value>(d_steps + (s - first) * B, s);
+            });
             HIPCHK(hipGetLastError());
             sync_state();
         }
@@ -504,11 +566,14 @@ struct Batch {
     template <int Wt>
     void compact(const std::vector<int32_t>& sel) {
         sync_state();
-        for (int b = 0; b < B; ++b) h_state[b].compact = sel[b] ? 1 : 0;
+        for (int b = 0; b < B; ++b) {
+            h_state[b].compact = sel[b] ? 1 : 0;
+            h_state[b].ticket = 0u;
+        }
         put_state();
         upload_desc();
         plan();
-        launch_compact<Wt>();
+        with_tiles<Wt>([&](auto, auto TM) { launch_compact<Wt, decltype(TM)::value>(); });
         HIPCHK(hipGetLastError());
         sync_state();
     }
@@ -979,6 +1044,33 @@ int pp_batch_scale_by_popcount(pp_batch* p, const int* kind, const uint64_t* mas
     dispatch_w(bt.W, [&](auto wc) { bt.channel<decltype(wc)::value>(ch, tabs, en); });
     HIPCHK(hipStreamSynchronize(bt.stream));
     API_END
+}
+
+int pp_batch_reserve(pp_batch* p, const int64_t* cap) {
+    if (!p || !cap) return fail(PP_EARG, "null argument");
+    API_BEGIN
+    Batch& bt = p->bt;
+    bt.sync_state();
+    for (int b = 0; b < bt.B; ++b) {
+        if (cap[b] < 0) throw PPError(PP_EARG, "capacity must be >= 0");
+        const SegState& st = bt.h_state[b];
+        bt.ensure_caps(b, std::max<int64_t>(cap[b], 1), std::max<int64_t>(cap[b] / 2, 1), bt.ib[b].maxlev,
+                       st.n[st.cur], st.cur);
+    }
+    bt.upload_desc();
+    HIPCHK(hipStreamSynchronize(bt.stream));
+    API_END
+}
+
+int pp_batch_set_tiles(pp_batch* p, int tile, int merge_tile) {
+    if (!p) return fail(PP_EARG, "null handle");
+    if (tile != 0 && tile != 512 && tile != 2048) return fail(PP_EARG, "tile must be 0, 512 or 2048");
+    if (merge_tile != 0 && merge_tile != 512 && merge_tile != 1024 && merge_tile != 2048)
+        return fail(PP_EARG, "merge_tile must be 0, 512, 1024 or 2048");
+    if (merge_tile == 2048 && p->bt.W == 2) return fail(PP_EARG, "merge_tile 2048 needs N <= 64");
+    p->bt.tile_req = tile;
+    p->bt.mtile_req = merge_tile;
+    return PP_OK;
 }
 
 int pp_batch_set_timing(pp_batch* p, int on) {

@@ -19,8 +19,8 @@
 //   kb_bounds    per (tile, mask level) segment counts that cross tile edges
 //   kb_emit      child (G xor P, ((k-2) sinθ) c) written straight to its rank
 //                in key order (no sort pass, see below)
-//   kb_partition merge-path splits of live + children
-//   kb_merge     merge, live term first on ties (spv_kernels.jl:338-350),
+//   kb_merge     merge-path merge (each tile finds its own split with a
+//                wave-parallel search), live term first on ties (spv_kernels.jl:338-350),
 //                cos-scale of anticommuting live terms, sum of the (at most
 //                one) child per key, strict filter on every output
 //                (spv_kernels.jl:351), stream compaction with a decoupled
@@ -72,9 +72,11 @@ __device__ __forceinline__ void map_block(const BlockMap& bm, int& b, int64_t& l
 }
 
 // ---------------------------------------------------------------------------
-template <int W>
+template <int W, int T>
 __global__ void __launch_bounds__(kBlock) kb_flags(const SegDesc* __restrict__ D, const SegState* __restrict__ S,
                                                    const StepDev* __restrict__ steps, BlockMap bm) {
+    constexpr int kTile = T;
+    constexpr int kTileWords = T / 64;
     __shared__ int s_wcnt[kTileWords];
     int b;
     int64_t lb, nb;
@@ -161,8 +163,10 @@ __global__ void __launch_bounds__(kBlock) kb_flags(const SegDesc* __restrict__ D
 // rotations after the first of a sequence (where the early merge cannot drop
 // anything: every live term passed the strict filter at the last merge),
 // scans the tile counts and checks the child buffer.
+template <int T>
 __global__ void __launch_bounds__(1024) kb_scan(const SegDesc* __restrict__ D, SegState* __restrict__ S,
                                                 const StepDev* __restrict__ steps, int64_t step_index) {
+    constexpr int kTile = T;
     __shared__ int64_t s_w[16];
     __shared__ int64_t s_carry;
     __shared__ int64_t s_n;
@@ -179,6 +183,7 @@ __global__ void __launch_bounds__(1024) kb_scan(const SegDesc* __restrict__ D, S
         s_go = st->active && !s->stalled;
         s_n = s->n[s->cur];
         s_carry = 0;
+        s->ticket = 0u;
         if (s_go && st->precheck && s_n > s->app_cap_v) {
             s->early += 1;
             s->merge_in += s_n;
@@ -231,9 +236,10 @@ __global__ void __launch_bounds__(1024) kb_scan(const SegDesc* __restrict__ D, S
 //   q=2 post_same: A in [te, segend_b(te))
 //   q=3 post_sib : A in the segment just after segend_b(te) when it is the
 //                  bit-1 sibling and K_te has bit b = 0
-template <int W>
+template <int W, int T>
 __global__ void __launch_bounds__(kBlock) kb_bounds(const SegDesc* __restrict__ D, const SegState* __restrict__ S,
                                                     const StepDev* __restrict__ steps, BlockMap bm) {
+    constexpr int kTile = T;
     int b;
     int64_t lb, nb;
     map_block(bm, b, lb, nb);
@@ -249,51 +255,56 @@ __global__ void __launch_bounds__(kBlock) kb_bounds(const SegDesc* __restrict__ 
     const uint64_t* __restrict__ Lx = d.Lx[cur];
     const int nlev = st->g.nlev;
     const int maxlev = d.maxlev;
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int64_t total = ntiles * 2;
-    for (int64_t idx = lb * kBlock + threadIdx.x; idx < total; idx += nb * kBlock) {
+    // one wave per (tile, side); the searches are wave-cooperative
+    for (int64_t idx = lb * (kBlock / kWave) + wave; idx < total; idx += nb * (kBlock / kWave)) {
         const int side = (int)(idx & 1);
         const int64_t t = idx >> 1;
         const int64_t ts = t * kTile;
         const int64_t te = ts + kTile < n ? ts + kTile : n;
+        Key<W> ka, kb2;
+        int h = kHdbNone;
+        if (side == 0 && ts > 0) {
+            ka = load_key<W>(Lz, Lx, ts - 1);
+            h = hdb<W>(ka, load_key<W>(Lz, Lx, ts));
+        } else if (side == 1 && te < n) {
+            ka = load_key<W>(Lz, Lx, te);
+            h = hdb<W>(load_key<W>(Lz, Lx, te - 1), ka);
+        }
         for (int li = 0; li < nlev; ++li) {
             const int lvl = st->g.lev[li];
             int64_t same = 0, sib = 0;
-            if (side == 0) {
-                if (ts > 0) {
-                    const Key<W> kp = load_key<W>(Lz, Lx, ts - 1);
-                    const int h = hdb<W>(kp, load_key<W>(Lz, Lx, ts));
-                    if (h <= lvl) {
-                        const int64_t q = seg_start<W>(Lz, Lx, ts - 1, lvl);
-                        same = a_prefix(ts, n, ntiles, d.flags, d.wpre, d.tileoff) -
-                               a_prefix(q, n, ntiles, d.flags, d.wpre, d.tileoff);
-                        if (h < lvl && key_bit<W>(kp, lvl) == 1 && q > 0 &&
-                            hdb<W>(load_key<W>(Lz, Lx, q - 1), load_key<W>(Lz, Lx, q)) == lvl) {
-                            const int64_t q2 = seg_start<W>(Lz, Lx, q - 1, lvl);
-                            sib = a_prefix(q, n, ntiles, d.flags, d.wpre, d.tileoff) -
-                                  a_prefix(q2, n, ntiles, d.flags, d.wpre, d.tileoff);
+            if (h <= lvl) {
+                if (side == 0) {
+                    const int64_t q = seg_start_wave<W>(Lz, Lx, ts - 1, lvl, ka);
+                    same = a_prefix<T>(ts, n, ntiles, d.flags, d.wpre, d.tileoff) -
+                           a_prefix<T>(q, n, ntiles, d.flags, d.wpre, d.tileoff);
+                    if (h < lvl && key_bit<W>(ka, lvl) == 1 && q > 0) {
+                        kb2 = load_key<W>(Lz, Lx, q - 1);
+                        if (hdb<W>(kb2, load_key<W>(Lz, Lx, q)) == lvl) {
+                            const int64_t q2 = seg_start_wave<W>(Lz, Lx, q - 1, lvl, kb2);
+                            sib = a_prefix<T>(q, n, ntiles, d.flags, d.wpre, d.tileoff) -
+                                  a_prefix<T>(q2, n, ntiles, d.flags, d.wpre, d.tileoff);
+                        }
+                    }
+                } else {
+                    const int64_t r = seg_end_wave<W>(Lz, Lx, te, n, lvl, ka);
+                    same = a_prefix<T>(r, n, ntiles, d.flags, d.wpre, d.tileoff) -
+                           a_prefix<T>(te, n, ntiles, d.flags, d.wpre, d.tileoff);
+                    if (h < lvl && key_bit<W>(ka, lvl) == 0 && r < n) {
+                        kb2 = load_key<W>(Lz, Lx, r);
+                        if (hdb<W>(load_key<W>(Lz, Lx, r - 1), kb2) == lvl) {
+                            const int64_t r2 = seg_end_wave<W>(Lz, Lx, r, n, lvl, kb2);
+                            sib = a_prefix<T>(r2, n, ntiles, d.flags, d.wpre, d.tileoff) -
+                                  a_prefix<T>(r, n, ntiles, d.flags, d.wpre, d.tileoff);
                         }
                     }
                 }
-                d.bnd[(t * maxlev + li) * 4 + 0] = same;
-                d.bnd[(t * maxlev + li) * 4 + 1] = sib;
-            } else {
-                if (te < n) {
-                    const Key<W> kn = load_key<W>(Lz, Lx, te);
-                    const int h = hdb<W>(load_key<W>(Lz, Lx, te - 1), kn);
-                    if (h <= lvl) {
-                        const int64_t r = seg_end<W>(Lz, Lx, te, n, lvl);
-                        same = a_prefix(r, n, ntiles, d.flags, d.wpre, d.tileoff) -
-                               a_prefix(te, n, ntiles, d.flags, d.wpre, d.tileoff);
-                        if (h < lvl && key_bit<W>(kn, lvl) == 0 && r < n &&
-                            hdb<W>(load_key<W>(Lz, Lx, r - 1), load_key<W>(Lz, Lx, r)) == lvl) {
-                            const int64_t r2 = seg_end<W>(Lz, Lx, r, n, lvl);
-                            sib = a_prefix(r2, n, ntiles, d.flags, d.wpre, d.tileoff) -
-                                  a_prefix(r, n, ntiles, d.flags, d.wpre, d.tileoff);
-                        }
-                    }
-                }
-                d.bnd[(t * maxlev + li) * 4 + 2] = same;
-                d.bnd[(t * maxlev + li) * 4 + 3] = sib;
+            }
+            if (lane == 0) {
+                d.bnd[(t * maxlev + li) * 4 + 2 * side + 0] = same;
+                d.bnd[(t * maxlev + li) * 4 + 2 * side + 1] = sib;
             }
         }
     }
@@ -301,12 +312,13 @@ __global__ void __launch_bounds__(kBlock) kb_bounds(const SegDesc* __restrict__ 
 
 // ---------------------------------------------------------------------------
 // kb_emit: children written at their rank in key order.
-// Layout inside a tile: wave w owns elements [512 w, 512 w + 512), round r
+// Layout inside a tile: wave w owns elements [w T/4, (w+1) T/4), round r
 // covers 64 consecutive elements (coalesced loads), lane = element % 64.
-template <int W>
+template <int W, int T>
 __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D, SegState* __restrict__ S,
                                                   const StepDev* __restrict__ steps, BlockMap bm) {
-    constexpr int R = kTile / kBlock;  // 8 rounds
+    constexpr int kTile = T;
+    constexpr int R = kTile / kBlock;  // rounds of 64 consecutive terms per wave
     __shared__ int16_t s_hdb[kTile + 1];
     __shared__ int16_t s_aloc[kTile + 1];
     __shared__ int16_t s_st[kTile];
@@ -483,64 +495,20 @@ __device__ __forceinline__ bool merge_go(const StepDev* st, const SegState* s, i
     return mode == 0 ? st->active != 0 : s->compact != 0;
 }
 
-// kb_partition: merge-path split of diagonal d = t * TM between live L (n)
-// and children C (m), live first on equal keys.  Clears the vector's
-// look-back words and ticket; finalises a vector with nothing to merge.
-template <int W>
-__global__ void __launch_bounds__(kBlock) kb_partition(const SegDesc* __restrict__ D, SegState* __restrict__ S,
-                                                       const StepDev* __restrict__ steps, BlockMap bm, int mode) {
-    constexpr int TM = MergeTile<W>::kSize;
-    int b;
-    int64_t lb, nb;
-    map_block(bm, b, lb, nb);
-    const StepDev* st = steps + b;
-    SegState* s = S + b;
-    if (!merge_go(st, s, mode)) return;
-    const int cur = s->cur;
-    const int64_t n = s->n[cur];
-    const int64_t m = mode == 0 ? s->m : 0;
-    const int64_t tot = n + m;
-    const int64_t ntm = (tot + TM - 1) / TM;
-    const SegDesc& d = D[b];
-    const uint64_t* __restrict__ Lz = d.Lz[cur];
-    const uint64_t* __restrict__ Lx = d.Lx[cur];
-    for (int64_t t = lb * kBlock + threadIdx.x; t <= ntm; t += nb * kBlock) {
-        if (t == 0) {
-            s->ticket = 0u;
-            if (ntm == 0) {  // nothing to merge: the output is empty
-                s->n[cur ^ 1] = 0;
-                if (mode == 1) s->compact = 0;
-                else s->rotations += 1;
-                s->flip = 1;
-            }
-        }
-        if (t < ntm) d.status[t] = 0ull;
-        const int64_t dg = t * TM < tot ? t * TM : tot;
-        int64_t lo = dg - m > 0 ? dg - m : 0;
-        int64_t hi = dg < n ? dg : n;
-        while (lo < hi) {
-            const int64_t mid = (lo + hi) >> 1;
-            if (key_le<W>(load_key<W>(Lz, Lx, mid), load_key<W>(d.Cz, d.Cx, dg - 1 - mid))) lo = mid + 1;
-            else hi = mid;
-        }
-        d.split[t] = lo;
-    }
-}
-
 // ---------------------------------------------------------------------------
 // kb_merge (see the file header).  Each workgroup takes tiles of its vector
 // by ticket, so a tile only ever waits in the look-back on tiles that running
 // workgroups have already claimed.
-template <int W>
+template <int W, int TM>
 __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D, SegState* __restrict__ S,
                                                    const StepDev* __restrict__ steps, BlockMap bm, int mode,
-                                                   int64_t step_index) {
-    constexpr int TM = MergeTile<W>::kSize;
+                                                   int64_t step_index, uint32_t epoch) {
     constexpr int E = TM / kBlock;
     __shared__ Key<W> s_key[TM];
     __shared__ double s_c[TM];
     __shared__ int64_t s_tile;
     __shared__ int64_t s_base;
+    __shared__ int64_t s_split[2];
     __shared__ int s_wsum[4];
     int b;
     int64_t lb, nb;
@@ -549,6 +517,7 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
     SegState* s = S + b;
     if (!merge_go(st, s, mode)) return;
     const int rot = mode == 0;
+    const uint64_t ep = ((uint64_t)epoch & kEpochMask) << kEpochShift;
     const int cur = s->cur;
     const int64_t n = s->n[cur];
     const int64_t m = rot ? s->m : 0;
@@ -568,6 +537,15 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
     const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
     GenReg<W> G;
     if (rot) G.load(&st->g);
+    if (ntm == 0) {  // nothing to merge: the output is empty
+        if (lb == 0 && tid == 0) {
+            s->n[cur ^ 1] = 0;
+            if (rot) s->rotations += 1;
+            else s->compact = 0;
+            s->flip = 1;
+        }
+        return;
+    }
     while (true) {
         if (tid == 0) s_tile = (int64_t)atomicAdd(&s->ticket, 1u);
         __syncthreads();
@@ -575,7 +553,13 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
         if (t >= ntm) break;
         const int64_t d0 = t * TM;
         const int64_t d1 = d0 + TM < tot ? d0 + TM : tot;
-        int64_t i0 = d.split[t], i1 = d.split[t + 1];
+        // merge-path splits of this tile's window: wave 0 the start, wave 1 the end
+        if (wave < 2) {
+            const int64_t sp = merge_split_wave<W>(Lz, Lx, n, Cz, Cx, m, wave == 0 ? d0 : d1);
+            if (lane == 0) s_split[wave] = sp;
+        }
+        __syncthreads();
+        int64_t i0 = s_split[0], i1 = s_split[1];
         int64_t j0 = d0 - i0, j1 = d1 - i1;
         if (i0 < 0 || i1 < i0 || i1 > n || j0 < 0 || j1 < j0 || j1 > m) {  // never expected
             if (tid == 0) atomicOr(&s->err, 2);
@@ -665,20 +649,23 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
         if (wave == 0) {
             if (t == 0) {
                 if (lane == 0) {
-                    __hip_atomic_store(&status[0], kFlagPre | (uint64_t)total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(&status[0], kFlagPre | ep | (uint64_t)total, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
                     s_base = 0;
                 }
             } else {
                 if (lane == 0)
-                    __hip_atomic_store(&status[t], kFlagAgg | (uint64_t)total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(&status[t], kFlagAgg | ep | (uint64_t)total, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
                 int64_t excl = 0;
                 int64_t top = t - 1;
                 int64_t spins = 0;
                 while (true) {
                     const int64_t q = top - lane;
                     const uint64_t sv = q >= 0 ? __hip_atomic_load(&status[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                               : kFlagPre;  // before tile 0: prefix 0
-                    const uint64_t flag = sv & ~kValMask;
+                                               : (kFlagPre | ep);  // before tile 0: prefix 0
+                    const bool fresh = (sv & (kEpochMask << kEpochShift)) == ep;
+                    const uint64_t flag = fresh ? (sv & (3ull << 62)) : 0ull;
                     const uint64_t pre = __ballot(flag == kFlagPre);
                     const uint64_t notready = __ballot(flag == 0);
                     const int first = pre ? __ffsll((unsigned long long)pre) - 1 : kWave;
@@ -691,7 +678,7 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
                         __builtin_amdgcn_s_sleep(1);
                         continue;
                     }
-                    int64_t v = lane <= first ? (int64_t)(sv & kValMask) : 0;
+                    int64_t v = lane <= first ? (int64_t)(sv & kCountMask) : 0;
 #pragma unroll
                     for (int dd = kWave / 2; dd > 0; dd >>= 1) v += __shfl_xor(v, dd, kWave);
                     excl += v;
@@ -699,7 +686,7 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
                     top -= kWave;
                 }
                 if (lane == 0) {
-                    __hip_atomic_store(&status[t], kFlagPre | (uint64_t)(excl + total), __ATOMIC_RELAXED,
+                    __hip_atomic_store(&status[t], kFlagPre | ep | (uint64_t)(excl + total), __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
                     s_base = excl;
                 }
@@ -763,6 +750,7 @@ __global__ void kb_precheck(SegState* __restrict__ S, const int32_t* __restrict_
         s->flip = 0;
     }
     s->compact = 0;
+    s->ticket = 0u;
     if (!en[b] || s->stalled) return;
     const int64_t n = s->n[s->cur];
     if (n > s->app_cap_v) {

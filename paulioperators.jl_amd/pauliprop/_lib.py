@@ -20,7 +20,7 @@ EXPORTS = ("pp_create", "pp_destroy", "pp_upload", "pp_download", "pp_size", "pp
            "pp_evolve_single", "pp_compact", "pp_scale_by_popcount", "pp_expectation_ket", "pp_check",
            "pp_batch_create", "pp_batch_destroy", "pp_batch_words", "pp_batch_upload", "pp_batch_download",
            "pp_batch_sizes", "pp_batch_evolve", "pp_batch_scale_by_popcount", "pp_batch_expectation_ket",
-           "pp_batch_set_timing", "pp_batch_merge_time",
+           "pp_batch_reserve", "pp_batch_set_tiles", "pp_batch_set_timing", "pp_batch_merge_time",
            "pp_julia_powi", "pp_last_error", "pp_version")
 
 
@@ -88,6 +88,8 @@ def lib():
                                   P(PPCounters)]
     L.pp_batch_scale_by_popcount.argtypes = [vp, ip, u64p, dp]
     L.pp_batch_expectation_ket.argtypes = [vp, u64p, dp]
+    L.pp_batch_reserve.argtypes = [vp, P(i64)]
+    L.pp_batch_set_tiles.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.pp_batch_set_timing.argtypes = [vp, ctypes.c_int]
     L.pp_batch_merge_time.argtypes = [vp, dp, P(i64), ctypes.c_int]
     L.pp_julia_powi.argtypes = [ctypes.c_double, i64]

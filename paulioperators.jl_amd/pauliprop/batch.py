@@ -93,28 +93,14 @@ class SparsePauliBatch:
         z, x, c = self.arrays(b)
         return list(zip(_unpack_rows(z), _unpack_rows(x), c.tolist()))
 
-    def evolve_(self, generators: Sequence[Sequence], angles: Sequence[Sequence[float]], *, truncation=NoTruncation(),
-                local_truncation=NoTruncation(), counters: bool = False):
-        """One ``evolve!(v_b, generators[b], angles[b]; truncation, local_truncation)``
-        per vector b (window = 1).  ``truncation`` may be one strategy or one per vector."""
+    def pack(self, generators: Sequence[Sequence], angles: Sequence[Sequence[float]]):
+        """Step arrays for ``evolve_packed``: generator s of vector b at [s, b]."""
         if len(generators) != self.B or len(angles) != self.B:
             raise DimensionMismatch("one generator list and one angle list per vector")
-        strat = list(truncation) if isinstance(truncation, (list, tuple)) else [truncation] * self.B
-        loc = list(local_truncation) if isinstance(local_truncation, (list, tuple)) else [local_truncation] * self.B
-        for s in strat:
-            if not is_compilable(s):
-                raise NotImplementedError(f"{type(s).__name__} at merge boundaries is not available in this build")
-        for s in loc:
-            if not is_compilable(s):
-                raise ArgumentError("local_truncation must be a deterministic (weight/coefficient) strategy "
-                                    "— it runs per append")
-        lens = [len(g) for g in generators]
         for g, a in zip(generators, angles):
             if len(g) != len(a):
                 raise DimensionMismatch("generators and angles must have same length")
-        nsteps = max(lens) if lens else 0
-        if nsteps == 0:
-            return [PPCounters() for _ in range(self.B)] if counters else None
+        nsteps = max((len(g) for g in generators), default=0)
         B, W = self.B, self.W
         gz = np.zeros((nsteps, B, W), dtype=np.uint64)
         gx = np.zeros((nsteps, B, W), dtype=np.uint64)
@@ -129,6 +115,32 @@ class SparsePauliBatch:
                 cv[s, b] = math.cos(a)
                 sv[s, b] = math.sin(a)
                 act[s, b] = 1
+        return gz, gx, cv, sv, act
+
+    def evolve_(self, generators: Sequence[Sequence], angles: Sequence[Sequence[float]], *, truncation=NoTruncation(),
+                local_truncation=NoTruncation(), counters: bool = False):
+        """One ``evolve!(v_b, generators[b], angles[b]; truncation, local_truncation)``
+        per vector b (window = 1).  ``truncation`` may be one strategy or one per vector."""
+        return self.evolve_packed(self.pack(generators, angles), truncation=truncation,
+                                  local_truncation=local_truncation, counters=counters)
+
+    def evolve_packed(self, packed, *, truncation=NoTruncation(), local_truncation=NoTruncation(),
+                      counters: bool = False):
+        """``evolve_`` on arrays from ``pack`` (cos/sin already applied)."""
+        strat = list(truncation) if isinstance(truncation, (list, tuple)) else [truncation] * self.B
+        loc = list(local_truncation) if isinstance(local_truncation, (list, tuple)) else [local_truncation] * self.B
+        for s in strat:
+            if not is_compilable(s):
+                raise NotImplementedError(f"{type(s).__name__} at merge boundaries is not available in this build")
+        for s in loc:
+            if not is_compilable(s):
+                raise ArgumentError("local_truncation must be a deterministic (weight/coefficient) strategy "
+                                    "— it runs per append")
+        gz, gx, cv, sv, act = packed
+        nsteps = gz.shape[0]
+        if nsteps == 0:
+            return [PPCounters() for _ in range(self.B)] if counters else None
+        B = self.B
         fs = (PPFilter * B)(*[compile_filter(s) for s in strat])
         fl = (PPFilter * B)(*[compile_filter(s) for s in loc])
         cnt = (PPCounters * B)()
@@ -181,6 +193,18 @@ class SparsePauliBatch:
         out = np.zeros(self.B, dtype=np.float64)
         check(_lib.lib().pp_batch_expectation_ket(self._h, u64p(k), dp(out)))
         return out.tolist()
+
+    def reserve(self, cap):
+        """Pre-size device buffers to hold ``cap`` (int or one per vector) live terms."""
+        caps = list(cap) if isinstance(cap, (list, tuple)) else [int(cap)] * self.B
+        arr = (ctypes.c_int64 * self.B)(*caps)
+        check(_lib.lib().pp_batch_reserve(self._h, arr))
+        return self
+
+    def set_tiles(self, tile: int = 0, merge_tile: int = 0):
+        """Sweep tile (512/2048) and merge tile (512/1024/2048); 0 = automatic."""
+        check(_lib.lib().pp_batch_set_tiles(self._h, int(tile), int(merge_tile)))
+        return self
 
     def set_timing(self, on: bool):
         check(_lib.lib().pp_batch_set_timing(self._h, 1 if on else 0))
