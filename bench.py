@@ -87,7 +87,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a 1 GiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": 1 << 30, "launches": 3}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; runs on more than one GPU are not part of it "
               "(the sharding design is described in DESIGN.md)", file=sys.stderr)
@@ -123,6 +136,11 @@ def main():
     value = agg.terms_swept / tot_time
     merge_bytes = S * (agg.merge_in + agg.merge_out)
     achieved = merge_bytes / t_merge / 1e9 if t_merge > 0 else None
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "merge_traffic_lattice121.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
 
     cpu = None
     if not args.no_cpu:
@@ -152,7 +170,7 @@ def main():
                    "rotations": agg.rotations, "term_rotations": agg.terms_swept,
                    "expectation_by_theta_h": evs},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
                      "kernel": "kb_merge<2>", "launches": n_merge,
                      "avg_launch_us": 1e6 * t_merge / max(n_merge, 1),
                      "bytes_per_launch": merge_bytes / max(n_merge, 1)},

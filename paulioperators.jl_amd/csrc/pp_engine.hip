@@ -362,11 +362,14 @@ struct Batch {
             for (int k = 0; k < 4; ++k) h_maps[(size_t)k * (B + 1) + b] = acc[k];
             if (b == B) break;
             const int64_t cap = std::min(ib[b].L[0].cap, ib[b].L[1].cap);
-            const int64_t nest = std::min<int64_t>(cap, std::max<int64_t>(4 * n_known[b], 8192));
-            acc[0] += std::max<int64_t>(1, std::min<int64_t>(cdiv(nest, kTile), (int64_t)ncu * 8));
-            acc[1] += std::max<int64_t>(1, std::min<int64_t>(cdiv(2 * cdiv(nest, kTile), kBlock / kWave), (int64_t)ncu * 4));
-            acc[2] += std::max<int64_t>(1, std::min<int64_t>(cdiv(cdiv(2 * nest, T2) + 1, kBlock), (int64_t)ncu * 4));
-            acc[3] += std::max<int64_t>(1, std::min<int64_t>(cdiv(2 * nest, T2), (int64_t)ncu * 4));
+            // expected live count during the coming sequence: blocks beyond the
+            // real tiles only exit (sweeps) or contend for the ticket (merge)
+            const int64_t nest = std::min<int64_t>(cap, std::max<int64_t>(n_known[b] + n_known[b] / 2, 1024));
+            const int64_t tiles = cdiv(nest, kTile);
+            acc[0] += std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)ncu * 8));
+            acc[1] += std::max<int64_t>(1, std::min<int64_t>(cdiv(2 * tiles, kBlock / kWave), (int64_t)ncu * 4));
+            acc[2] += 1;
+            acc[3] += std::max<int64_t>(1, std::min<int64_t>(cdiv(nest + nest / 2, T2), (int64_t)ncu * 4));
         }
         HIPCHK(hipMemcpyAsync(d_maps, h_maps.data(), h_maps.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream));
         if (cap_partial < acc[0]) {
