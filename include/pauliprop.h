@@ -56,6 +56,27 @@ typedef struct {
     double t_merge;         /* seconds in the merge kernel (HIP events) */
 } pp_counters;
 
+/* One step of a boundary strategy that does not compile into a single
+ * filter (truncation.jl:37-159; _apply! on SparsePauliVector,
+ * spv_evolve.jl:59-119).  A CompositeTruncation becomes its members in order
+ * (_apply_tup!), compilable members as PP_OP_FILTER.
+ *   PP_OP_FILTER            compaction under `filter` (the clip family)
+ *   PP_OP_ADAPTIVE          AdaptiveTruncation(max_terms, min_thresh = value):
+ *                           exact top-k threshold (spv_evolve.jl:103-114)
+ *   PP_OP_STOCHASTIC_COEFF  StochasticCoeffTruncation(epsilon = value):
+ *                           stochastic_clip! (spv_evolve.jl:35-52) with a
+ *                           counter-based generator keyed by `seed` (the Julia
+ *                           RNG stream is not reproduced) */
+enum { PP_OP_FILTER = 0, PP_OP_ADAPTIVE = 1, PP_OP_STOCHASTIC_COEFF = 2 };
+typedef struct {
+    int32_t kind;
+    int32_t reserved;
+    pp_filter filter;
+    int64_t max_terms;
+    double value;
+    uint64_t seed;
+} pp_strategy_op;
+
 /* SparsePauliVector(N, Float64) with the reference's capacity bookkeeping
  * (type_SparsePauliVector.jl:94-104,167-188): live_cap and append_cap are the
  * reference's buffer lengths, which decide when an early merge happens.  Device
@@ -76,10 +97,27 @@ int pp_capacities(pp_handle* h, int64_t* live_cap, int64_t* append_cap);
  * applied left to right.  gz, gx: L generators of W words each; cosv, sinv:
  * the caller's cos θ and sin θ (spv_evolve.jl:341-342).  strict: the compiled
  * truncation applied at every merge; local: the filter applied to each child
- * as it is created.  window must be 1 in this build (the reference default).
+ * as it is created; window >= 1 rotations between merges (window > 1 sorts
+ * the pending appends with a stable radix sort, so equal keys are summed in
+ * append order where the reference's quicksort leaves the order unspecified).
  * counters may be NULL. */
 int pp_evolve(pp_handle* h, const uint64_t* gz, const uint64_t* gx, const double* cosv, const double* sinv,
               int64_t L, int64_t window, const pp_filter* strict, const pp_filter* local, pp_counters* counters);
+
+/* evolve!(v, gens, angs; window, truncation, local_truncation, correction)
+ * with any strategy (spv_evolve.jl:312-387): nops > 0 merges with NOFILTER
+ * and applies ops[0..nops) after each merge (the `compiled || _apply!` branch
+ * of _boundary!, spv_evolve.jl:258-284); nops == 0 uses `strict`.
+ * energy_ket != NULL is EnergyCorrection(ψ) (truncation.jl:262-306): the
+ * change of ⟨ψ|v|ψ⟩ across every boundary, measured before the merge (live +
+ * pending appends) and after the strategy, is added to *energy_acc. */
+int pp_evolve_ex(pp_handle* h, const uint64_t* gz, const uint64_t* gx, const double* cosv, const double* sinv,
+                 int64_t L, int64_t window, const pp_filter* strict, const pp_strategy_op* ops, int nops,
+                 const pp_filter* local, const uint64_t* energy_ket, double* energy_acc, pp_counters* counters);
+
+/* truncate!(v, strategy, corr) (truncation.jl:330-337) for any strategy given
+ * as operations; energy_ket / energy_acc as in pp_evolve_ex (may be NULL). */
+int pp_truncate(pp_handle* h, const pp_strategy_op* ops, int nops, const uint64_t* energy_ket, double* energy_acc);
 
 /* Single rotation without truncation (spv_evolve.jl:228-236). */
 int pp_evolve_single(pp_handle* h, const uint64_t* gz, const uint64_t* gx, double cosv, double sinv);

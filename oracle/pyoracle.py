@@ -67,6 +67,10 @@ def lib():
         L.or_evolve.argtypes = [ctypes.c_void_p, u64p, u64p, dp, dp, ctypes.c_int64, ctypes.c_int64,
                                 P(OrFilter), ctypes.c_int, ctypes.c_int64, ctypes.c_double,
                                 P(OrFilter), P(OrCounters)]
+        L.or_evolve_ex.argtypes = [ctypes.c_void_p, u64p, u64p, dp, dp, ctypes.c_int64, ctypes.c_int64,
+                                   P(OrFilter), ctypes.c_int, ctypes.c_int64, ctypes.c_double,
+                                   P(OrFilter), u64p, P(ctypes.c_double), P(OrCounters)]
+        L.or_set_sort_mode.argtypes = [ctypes.c_int]
         L.or_compact.argtypes = [ctypes.c_void_p, P(OrFilter)]
         L.or_adaptive.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_double]
         L.or_channel.argtypes = [ctypes.c_void_p, ctypes.c_int, u64p, ctypes.c_double, ctypes.c_double,
@@ -86,6 +90,12 @@ def lib():
                                      ctypes.c_double, ctypes.c_double]
         _lib = L
     return _lib
+
+
+def set_sort_mode(stable: bool):
+    """Boundary sort with more than one contributing rotation: the reference's
+    quicksort (False) or a stable merge sort (True, the device's order)."""
+    lib().or_set_sort_mode(1 if stable else 0)
 
 
 def _u64p(a: np.ndarray):
@@ -199,7 +209,10 @@ class OracleSPV:
 
     def evolve(self, gens: Sequence[Tuple[int, int]], angles: Sequence[float], window: int = 1,
                strict=None, adaptive: Optional[Tuple[int, float]] = None, local=None,
-               cosv=None, sinv=None) -> OrCounters:
+               cosv=None, sinv=None, energy: Optional[list] = None) -> OrCounters:
+        """spv_evolve.jl:312-387.  ``energy``: a one-element list [ket] or
+        [ket, accumulated]; on return energy[1] holds EnergyCorrection's
+        accumulated_energy (truncation.jl:262-310)."""
         from_struct = lambda f: filter_from(f) if f is not None else None
         L = len(gens)
         gz = np.zeros((max(L, 1), self.W), dtype=np.uint64)
@@ -216,8 +229,17 @@ class OracleSPV:
         cnt = OrCounters()
         kind = 1 if adaptive is not None else 0
         mt, mth = adaptive if adaptive is not None else (0, 0.0)
-        rc = lib().or_evolve(self.h, _u64p(gz), _u64p(gx), _dp(cv), _dp(sv), L, window,
-                             ctypes.byref(fs), kind, mt, mth, ctypes.byref(fl), ctypes.byref(cnt))
+        if energy is None:
+            rc = lib().or_evolve(self.h, _u64p(gz), _u64p(gx), _dp(cv), _dp(sv), L, window,
+                                 ctypes.byref(fs), kind, mt, mth, ctypes.byref(fl), ctypes.byref(cnt))
+        else:
+            ket = pack_int(int(energy[0]), 2)
+            acc = ctypes.c_double(energy[1] if len(energy) > 1 else 0.0)
+            rc = lib().or_evolve_ex(self.h, _u64p(gz), _u64p(gx), _dp(cv), _dp(sv), L, window,
+                                    ctypes.byref(fs), kind, mt, mth, ctypes.byref(fl), _u64p(ket),
+                                    ctypes.byref(acc), ctypes.byref(cnt))
+            del energy[1:]
+            energy.append(acc.value)
         if rc:
             raise RuntimeError(f"oracle evolve rc={rc}")
         return cnt

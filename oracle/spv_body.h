@@ -222,6 +222,36 @@ static void F(sort_ws)(F(or_trip) *ws, int64_t lo, int64_t hi) {
     }
 }
 
+/* Stable alternative to sort_ws (or_sort_mode == 1): top-down merge sort of
+ * ws[lo..hi] (inclusive) through tmp; equal keys keep their order.  Not the
+ * reference's order for duplicates: it is the device radix sort's, so that
+ * the two can be compared bit for bit. */
+static void F(msort_ws)(F(or_trip) *ws, F(or_trip) *tmp, int64_t lo, int64_t hi) {
+    if (hi - lo < 1) return;
+    int64_t mid = lo + (hi - lo) / 2;
+    F(msort_ws)(ws, tmp, lo, mid);
+    F(msort_ws)(ws, tmp, mid + 1, hi);
+    int64_t i = lo, j = mid + 1, k = lo;
+    while (i <= mid && j <= hi) {
+        if (F(trip_lt)(&ws[j], &ws[i])) tmp[k++] = ws[j++];
+        else tmp[k++] = ws[i++];
+    }
+    while (i <= mid) tmp[k++] = ws[i++];
+    while (j <= hi) tmp[k++] = ws[j++];
+    memcpy(ws + lo, tmp + lo, (size_t)(hi - lo + 1) * sizeof(F(or_trip)));
+}
+
+static void F(sort_pending)(F(or_trip) *ws, int64_t m) {
+    if (m <= 1) return;
+    if (or_sort_mode == 1) {
+        F(or_trip) *tmp = (F(or_trip) *)malloc((size_t)m * sizeof(F(or_trip)));
+        F(msort_ws)(ws, tmp, 0, m - 1);
+        free(tmp);
+    } else {
+        F(sort_ws)(ws, 0, m - 1);
+    }
+}
+
 /* spv_kernels.jl:253-286 — one stable block-swap pass of the unshuffle cascade
  * for bit b of word `which` (1 = z, 2 = x). */
 static void F(unshuffle_pass)(const F(or_trip) *src, F(or_trip) *dst, int64_t m, int b, int which) {
@@ -408,20 +438,28 @@ static void F(apply_adaptive)(F(or_spv) *v, int64_t max_terms, double min_thresh
     F(compact)(v, &f);
 }
 
-/* spv_evolve.jl:258-284 — merge boundary.  Corrections are not restated. */
-static void F(boundary)(F(or_spv) *v, const or_filter_t *f, const or_strategy_t *s,
+/* spv_evolve.jl:258-284 — merge boundary, with the EnergyCorrection
+ * accumulator (the non-linear EnergyVarianceCorrection is not restated). */
+static void F(boundary)(F(or_spv) *v, const or_filter_t *f, or_strategy_t *s,
                         int have_mask, WT lz, WT lx, int64_t *n_in, int64_t *n_out) {
+    const WT kv = (WT)((((u128)s->ket[1]) << 64) | (u128)s->ket[0]);
+    /* EnergyCorrection measures the pre-merge state (spv_evolve.jl:246-251) */
+    const double before = s->energy ? F(expectation)(v, kv) : 0.0;
     int64_t m = F(gather_append)(v);
     if (have_mask) F(unshuffle_ws)(v, m, lz, lx);
-    else if (m > 0) F(sort_ws)(v->ws, 0, m - 1);
+    else F(sort_pending)(v->ws, m);
     F(merge_spv)(v, m, f, n_in, n_out);
     if (s->kind == OR_STRAT_ADAPTIVE) F(apply_adaptive)(v, s->max_terms, s->min_thresh);
+    if (s->energy) {
+        const double after = F(expectation)(v, kv);
+        s->energy_acc += after - before;  /* truncation.jl:304-306 */
+    }
 }
 
 /* spv_evolve.jl:312-387 — the windowed driver. */
 static int F(evolve_seq)(F(or_spv) *v, const WT *gz, const WT *gx, const double *cosv,
                          const double *sinv, int64_t L, int64_t window,
-                         const or_filter_t *strict, const or_strategy_t *strat,
+                         const or_filter_t *strict, or_strategy_t *strat,
                          const or_filter_t *local, or_counters_t *cnt) {
     if (window < 1) return -1;
     int since = 0;

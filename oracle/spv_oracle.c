@@ -33,11 +33,25 @@ typedef struct {
 } or_filter_t;
 
 enum { OR_STRAT_COMPILED = 0, OR_STRAT_ADAPTIVE = 1 };
+/* The boundary strategy plus the EnergyCorrection accumulator
+ * (truncation.jl:262-310): energy != 0 measures <ket|O|ket> over live +
+ * pending appends before the merge and over the result after it, and adds
+ * after - before to energy_acc. */
 typedef struct {
     int kind;
     int64_t max_terms;
     double min_thresh;
+    int energy;
+    uint64_t ket[2];
+    double energy_acc;
 } or_strategy_t;
+
+/* Sort used at merge boundaries with more than one contributing rotation.
+ * 0: the reference's unstable quicksort (spv_kernels.jl:202-247).
+ * 1: a stable merge sort — equal keys keep their append order.  The two
+ *    orders differ only in how equal keys are summed (a last-ulp effect);
+ *    mode 1 is the order the device's stable radix sort produces. */
+static int or_sort_mode = 0;
 
 typedef struct {
     int64_t rotations, terms_swept, terms_created, merge_in, merge_out, early_merges;
@@ -238,18 +252,54 @@ int or_evolve_single(or_handle *h, const uint64_t *gz, const uint64_t *gx, doubl
 
 /* spv_evolve.jl:312-387 — windowed sequence driver.  gz/gx hold L generators
  * of W words each; cosv/sinv are the caller's cos/sin of each angle. */
+static int or_evolve_strat(or_handle *h, const uint64_t *gz, const uint64_t *gx, const double *cosv,
+                           const double *sinv, int64_t L, int64_t window, const or_filter_t *strict,
+                           or_strategy_t *sp, const or_filter_t *local, or_counters_t *cnt);
+
+int or_evolve_ex(or_handle *h, const uint64_t *gz, const uint64_t *gx, const double *cosv,
+                 const double *sinv, int64_t L, int64_t window, const or_filter_t *strict,
+                 int strat_kind, int64_t max_terms, double min_thresh, const or_filter_t *local,
+                 const uint64_t *energy_ket, double *energy_acc, or_counters_t *cnt) {
+    if (window < 1) return -1;
+    if (L == 0) return 0;
+    or_strategy_t s;
+    memset(&s, 0, sizeof(s));
+    s.kind = strat_kind;
+    s.max_terms = max_terms;
+    s.min_thresh = min_thresh;
+    if (energy_ket) {
+        s.energy = 1;
+        s.ket[0] = energy_ket[0];
+        s.ket[1] = h->W == 2 ? energy_ket[1] : 0;
+        s.energy_acc = energy_acc ? *energy_acc : 0.0;
+    }
+    int rc = or_evolve_strat(h, gz, gx, cosv, sinv, L, window, strict, &s, local, cnt);
+    if (energy_acc) *energy_acc = s.energy_acc;
+    return rc;
+}
+
+void or_set_sort_mode(int mode) { or_sort_mode = mode; }
+
 int or_evolve(or_handle *h, const uint64_t *gz, const uint64_t *gx, const double *cosv,
               const double *sinv, int64_t L, int64_t window, const or_filter_t *strict,
               int strat_kind, int64_t max_terms, double min_thresh, const or_filter_t *local,
               or_counters_t *cnt) {
-    if (window < 1) return -1;
-    if (L == 0) return 0;
-    or_strategy_t s = {strat_kind, max_terms, min_thresh};
+    return or_evolve_ex(h, gz, gx, cosv, sinv, L, window, strict, strat_kind, max_terms, min_thresh,
+                        local, NULL, NULL, cnt);
+}
+
+static int or_evolve_strat(or_handle *h, const uint64_t *gz, const uint64_t *gx, const double *cosv,
+                           const double *sinv, int64_t L, int64_t window, const or_filter_t *strict,
+                           or_strategy_t *sp, const or_filter_t *local, or_counters_t *cnt) {
+    const int strat_kind = sp->kind;
+    or_strategy_t s = *sp;
     or_filter_t nof;
     or_filter_init_nofilter(&nof);
     const or_filter_t *f = strat_kind == OR_STRAT_COMPILED ? strict : &nof;
     if (h->W == 1) {
-        return evolve_seq_w1((or_spv_w1 *)h->v, gz, gx, cosv, sinv, L, window, f, &s, local, cnt);
+        int rc1 = evolve_seq_w1((or_spv_w1 *)h->v, gz, gx, cosv, sinv, L, window, f, &s, local, cnt);
+        *sp = s;
+        return rc1;
     }
     u128 *z = (u128 *)malloc((size_t)L * sizeof(u128));
     u128 *x = (u128 *)malloc((size_t)L * sizeof(u128));
@@ -257,6 +307,7 @@ int or_evolve(or_handle *h, const uint64_t *gz, const uint64_t *gx, const double
     int rc = evolve_seq_w2((or_spv_w2 *)h->v, z, x, cosv, sinv, L, window, f, &s, local, cnt);
     free(z);
     free(x);
+    *sp = s;
     return rc;
 }
 

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -24,6 +25,7 @@
 #include "../../include/pauliprop.h"
 #include "pp_common.hpp"
 #include "pp_kernels.hpp"
+#include "pp_window.hpp"
 
 using namespace pp;
 
@@ -146,6 +148,34 @@ struct InstBuf {
     int maxlev = 1;
 };
 
+// Buffers of the windowed path (one vector): the reference's append buffer,
+// the radix-sort ping-pong twin and the scan / selection scratch.
+struct WinBuf {
+    DevSet A, T;
+    int64_t* bcount = nullptr;
+    int64_t* boff = nullptr;
+    int64_t cap_blocks = 0;
+    int64_t* bsum = nullptr;
+    int64_t cap_bsum = 0;
+    int64_t* hist = nullptr;
+    int64_t* hoff = nullptr;
+    int64_t cap_hist = 0;
+    uint64_t* status = nullptr;
+    int64_t cap_status = 0;
+    double2* partial = nullptr;
+    GenDev* gens = nullptr;
+    int64_t cap_gens = 0;
+    FilterDev* filt = nullptr;         // [0] strict (or NOFILTER), [1] local, [2] compaction
+    int64_t* small = nullptr;          // [0] scan total, [1] merge output count
+    int32_t* err = nullptr;
+    uint32_t* ticket = nullptr;
+    unsigned long long* orand = nullptr;  // [4 W]: OR then AND of the key words
+    unsigned long long* qhist = nullptr;  // [256]
+    double* out = nullptr;
+    uint32_t epoch = 0;
+    uint64_t rng_calls = 0;
+};
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -180,8 +210,11 @@ struct Batch {
     int64_t merge_launches = 0;
     uint32_t epoch = 0;  // merge launch number, tags the look-back words
 
+    WinBuf win;
+
     ~Batch() {
         for (auto& b : ib) free_inst(b);
+        free_win();
         dfree(d_desc); dfree(d_state); dfree(d_maps); dfree(d_filt); dfree(d_chan); dfree(d_tabs);
         dfree(d_en); dfree(d_steps); dfree(d_partial); dfree(d_out); dfree(d_kets); dfree(d_bad);
         if (h_state) (void)hipHostFree(h_state);
@@ -605,6 +638,388 @@ struct Batch {
         for (int b = 0; b < B; ++b) ok[b] = bad[b] ? 0 : 1;
     }
 
+    // ----------------------------------------------------------------------
+    // Windowed path (window > 1, non-compilable boundary strategies,
+    // EnergyCorrection): one vector, host-driven, spv_evolve.jl:312-387.
+
+    void free_win() {
+        WinBuf& w = win;
+        free_set(w.A); free_set(w.T);
+        dfree(w.bcount); dfree(w.boff); dfree(w.bsum); dfree(w.hist); dfree(w.hoff); dfree(w.status);
+        dfree(w.partial); dfree(w.gens); dfree(w.filt); dfree(w.small); dfree(w.err); dfree(w.ticket);
+        dfree(w.orand); dfree(w.qhist); dfree(w.out);
+        w = WinBuf();
+    }
+
+    void win_init() {
+        WinBuf& w = win;
+        if (w.filt) return;
+        w.filt = dalloc<FilterDev>(3);
+        w.small = dalloc<int64_t>(2);
+        w.err = dalloc<int32_t>(1);
+        w.ticket = dalloc<uint32_t>(1);
+        w.orand = dalloc<unsigned long long>(4 * (size_t)W);
+        w.qhist = dalloc<unsigned long long>(kRadix);
+        w.out = dalloc<double>(1);
+        w.partial = dalloc<double2>(1024);
+        HIPCHK(hipMemsetAsync(w.err, 0, sizeof(int32_t), stream));
+    }
+
+    // Grow a set to hold `need` terms, keeping the first `keep`.
+    void grow_set(DevSet& s, int64_t need, int64_t keep) {
+        if (s.cap >= need) return;
+        DevSet n2;
+        alloc_set(n2, std::max<int64_t>(need + need / 2, 1024));
+        if (keep > 0) {
+            HIPCHK(hipMemcpyAsync(n2.z, s.z, (size_t)keep * kb(), hipMemcpyDeviceToDevice, stream));
+            HIPCHK(hipMemcpyAsync(n2.x, s.x, (size_t)keep * kb(), hipMemcpyDeviceToDevice, stream));
+            HIPCHK(hipMemcpyAsync(n2.c, s.c, (size_t)keep * sizeof(double), hipMemcpyDeviceToDevice, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+        }
+        free_set(s);
+        s = n2;
+    }
+
+    void win_blocks(int64_t nb) {
+        WinBuf& w = win;
+        if (w.cap_blocks >= nb) return;
+        dfree(w.bcount); dfree(w.boff);
+        w.cap_blocks = std::max<int64_t>(nb, 2 * w.cap_blocks);
+        w.bcount = dalloc<int64_t>(w.cap_blocks);
+        w.boff = dalloc<int64_t>(w.cap_blocks);
+    }
+
+    // Exclusive scan of count int64 values; the sum lands in *total (device).
+    void scan_i64(const int64_t* in, int64_t count, int64_t* out, int64_t* total) {
+        WinBuf& w = win;
+        if (count == 0) {
+            HIPCHK(hipMemsetAsync(total, 0, sizeof(int64_t), stream));
+            return;
+        }
+        const int64_t nb = cdiv(count, (int64_t)kBlock * 8);
+        if (w.cap_bsum < nb) {
+            dfree(w.bsum);
+            w.cap_bsum = std::max<int64_t>(nb, 2 * w.cap_bsum);
+            w.bsum = dalloc<int64_t>(w.cap_bsum);
+        }
+        hipLaunchKernelGGL(kx_scan_blocks, dim3((unsigned)nb), dim3(kBlock), 0, stream, in, count, out, w.bsum);
+        hipLaunchKernelGGL(kx_scan_sums, dim3(1), dim3(1024), 0, stream, w.bsum, nb, total);
+        hipLaunchKernelGGL(kx_scan_add, dim3((unsigned)nb), dim3(kBlock), 0, stream, out, count, w.bsum);
+    }
+
+    int64_t read_small(int i) {
+        int64_t v = 0;
+        HIPCHK(hipMemcpyAsync(&v, win.small + i, sizeof(int64_t), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        return v;
+    }
+
+    void check_win_err() {
+        int32_t e = 0;
+        HIPCHK(hipMemcpyAsync(&e, win.err, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        if (e) throw PPError(PP_EHIP, "internal consistency check failed in a windowed-path kernel (bits " +
+                                          std::to_string(e) + ")");
+    }
+
+    // One rotation over [live (n) ; appends (an)]; returns the children appended.
+    template <int Wt>
+    int64_t win_rotate(const GenDev* g, int64_t n, int64_t an, int cur) {
+        WinBuf& w = win;
+        const int64_t tot = n + an;
+        if (tot == 0) return 0;
+        grow_set(w.A, an + tot, an);
+        const int64_t nb = cdiv(tot, kRB);
+        win_blocks(nb);
+        const DevSet& Ls = ib[0].L[cur];
+        hipLaunchKernelGGL(kw_rotate<Wt>, dim3((unsigned)nb), dim3(kBlock), 0, stream, Ls.z, Ls.x, Ls.c, n, w.A.z,
+                           w.A.x, w.A.c, an, g, w.filt + 1, 1, w.bcount, (const int64_t*)nullptr, w.A.cap);
+        scan_i64(w.bcount, nb, w.boff, w.small);
+        hipLaunchKernelGGL(kw_rotate<Wt>, dim3((unsigned)nb), dim3(kBlock), 0, stream, Ls.z, Ls.x, Ls.c, n, w.A.z,
+                           w.A.x, w.A.c, an, g, w.filt + 1, 0, w.bcount, (const int64_t*)w.boff, w.A.cap);
+        HIPCHK(hipGetLastError());
+        return read_small(0);
+    }
+
+    // Stable LSD radix sort of the m appends; returns the buffer holding them.
+    template <int Wt>
+    DevSet* win_sort(int64_t m) {
+        WinBuf& w = win;
+        if (m <= 1) return &w.A;
+        grow_set(w.T, m, 0);
+        HIPCHK(hipMemsetAsync(w.orand, 0, 2 * (size_t)Wt * 8, stream));
+        HIPCHK(hipMemsetAsync(w.orand + 2 * Wt, 0xff, 2 * (size_t)Wt * 8, stream));
+        hipLaunchKernelGGL(ks_orand<Wt>, dim3((unsigned)std::min<int64_t>(cdiv(m, kBlock), 1024)), dim3(kBlock), 0,
+                           stream, w.A.z, w.A.x, m, w.orand);
+        unsigned long long oa[4 * 2];
+        HIPCHK(hipMemcpyAsync(oa, w.orand, 4 * (size_t)Wt * 8, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        const int64_t nb = cdiv(m, kRB);
+        const int64_t nh = (int64_t)kRadix * nb;
+        if (w.cap_hist < nh) {
+            dfree(w.hist); dfree(w.hoff);
+            w.cap_hist = std::max<int64_t>(nh, 2 * w.cap_hist);
+            w.hist = dalloc<int64_t>(w.cap_hist);
+            w.hoff = dalloc<int64_t>(w.cap_hist);
+        }
+        DevSet* src = &w.A;
+        DevSet* dst = &w.T;
+        for (int dd = 0; dd < 16 * Wt; ++dd) {
+            const int cw = (dd * 8) >> 6;  // combined word: x words low, z words high
+            const uint64_t vary = (uint64_t)(oa[cw] ^ oa[2 * Wt + cw]);
+            if (((vary >> ((dd * 8) & 63)) & 0xffull) == 0) continue;
+            hipLaunchKernelGGL(ks_hist<Wt>, dim3((unsigned)nb), dim3(kBlock), 0, stream, src->z, src->x, m, dd, w.hist,
+                               nb);
+            scan_i64(w.hist, nh, w.hoff, w.small);
+            hipLaunchKernelGGL(ks_scatter<Wt>, dim3((unsigned)nb), dim3(kBlock), 0, stream, src->z, src->x, src->c, m,
+                               dd, w.hoff, nb, dst->z, dst->x, dst->c);
+            std::swap(src, dst);
+        }
+        HIPCHK(hipGetLastError());
+        return src;
+    }
+
+    // Merge live (n, buffer cur) with the sorted appends under filter slot
+    // fslot; the result lands in buffer cur ^ 1.  Returns the output count.
+    template <int Wt>
+    int64_t win_merge(int64_t n, int cur, const DevSet* S, int64_t m, int fslot) {
+        constexpr int TM = 1024;
+        WinBuf& w = win;
+        ensure_caps(0, n + m, ib[0].C.cap, ib[0].maxlev, n, cur);
+        const int64_t tiles = cdiv(n + m, TM);
+        if (w.cap_status < tiles + 1) {
+            dfree(w.status);
+            w.cap_status = std::max<int64_t>(tiles + 1, 2 * w.cap_status);
+            w.status = dalloc<uint64_t>(w.cap_status);
+            HIPCHK(hipMemsetAsync(w.status, 0, (size_t)w.cap_status * sizeof(uint64_t), stream));
+        }
+        w.epoch = (w.epoch + 1) & (uint32_t)kEpochMask;
+        if (w.epoch == 0) {
+            HIPCHK(hipMemsetAsync(w.status, 0, (size_t)w.cap_status * sizeof(uint64_t), stream));
+            w.epoch = 1;
+        }
+        HIPCHK(hipMemsetAsync(w.ticket, 0, sizeof(uint32_t), stream));
+        const DevSet& Li = ib[0].L[cur];
+        const DevSet& Lo = ib[0].L[cur ^ 1];
+        const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)ncu * 4));
+        hipLaunchKernelGGL((kw_merge<Wt, TM>), dim3(grid), dim3(kBlock), 0, stream, Li.z, Li.x, Li.c, n, S->z, S->x,
+                           S->c, m, w.filt + fslot, Lo.z, Lo.x, Lo.c, Lo.cap, w.status, w.ticket, w.epoch,
+                           w.small + 1, w.err);
+        HIPCHK(hipGetLastError());
+        const int64_t out = read_small(1);
+        check_win_err();
+        return out;
+    }
+
+    // Order-preserving compaction of the live set (mode 0: filter slot 2;
+    // mode 1: stochastic clip).  Returns the new count; flips cur.
+    template <int Wt>
+    int64_t win_compact(int64_t n, int& cur, int mode, double eps, uint64_t seed) {
+        WinBuf& w = win;
+        if (n == 0) return 0;
+        ensure_caps(0, n, ib[0].C.cap, ib[0].maxlev, n, cur);
+        const int64_t nb = cdiv(n, kRB);
+        win_blocks(nb);
+        const DevSet& Li = ib[0].L[cur];
+        const DevSet& Lo = ib[0].L[cur ^ 1];
+        hipLaunchKernelGGL(kw_compact<Wt>, dim3((unsigned)nb), dim3(kBlock), 0, stream, Li.z, Li.x, Li.c, n,
+                           w.filt + 2, mode, eps, seed, 1, w.bcount, (const int64_t*)nullptr, Lo.z, Lo.x, Lo.c);
+        scan_i64(w.bcount, nb, w.boff, w.small);
+        hipLaunchKernelGGL(kw_compact<Wt>, dim3((unsigned)nb), dim3(kBlock), 0, stream, Li.z, Li.x, Li.c, n,
+                           w.filt + 2, mode, eps, seed, 0, w.bcount, (const int64_t*)w.boff, Lo.z, Lo.x, Lo.c);
+        HIPCHK(hipGetLastError());
+        cur ^= 1;
+        return read_small(0);
+    }
+
+    // k-th smallest |c| (0-based) of the live set by an 8-pass radix select
+    // over the IEEE bit patterns (non-negative doubles order as integers).
+    double win_select_abs(int64_t n, int cur, int64_t k) {
+        WinBuf& w = win;
+        const DevSet& Li = ib[0].L[cur];
+        uint64_t prefix = 0, pmask = 0;
+        const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(n, kBlock), 2048));
+        unsigned long long h[kRadix];
+        for (int shift = 56; shift >= 0; shift -= 8) {
+            HIPCHK(hipMemsetAsync(w.qhist, 0, kRadix * sizeof(unsigned long long), stream));
+            hipLaunchKernelGGL(kq_hist, dim3(grid), dim3(kBlock), 0, stream, Li.c, n, shift, prefix, pmask, w.qhist);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(h, w.qhist, sizeof(h), hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+            int d = 0;
+            for (; d < kRadix; ++d) {
+                if ((int64_t)h[d] > k) break;
+                k -= (int64_t)h[d];
+            }
+            if (d == kRadix) throw PPError(PP_EHIP, "radix select lost its rank");
+            prefix |= (uint64_t)d << shift;
+            pmask |= 0xffull << shift;
+        }
+        double v;
+        std::memcpy(&v, &prefix, sizeof(v));
+        return v;
+    }
+
+    void win_filter(int slot, const pp_filter* f) {
+        FilterDev d = make_filter_dev(f);
+        HIPCHK(hipMemcpyAsync(win.filt + slot, &d, sizeof(d), hipMemcpyHostToDevice, stream));
+    }
+
+    template <int Wt>
+    double win_expect(int64_t n, int cur, int64_t an, const uint64_t* ket) {
+        WinBuf& w = win;
+        Key<Wt> kk;
+        for (int q = 0; q < Wt; ++q) {
+            kk.z[q] = ket[q];
+            kk.x[q] = 0;
+        }
+        const DevSet& Li = ib[0].L[cur];
+        const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(n + an, kBlock), 1024));
+        hipLaunchKernelGGL(kw_expect<Wt>, dim3(grid), dim3(kBlock), 0, stream, Li.z, Li.x, Li.c, n, w.A.z, w.A.x,
+                           w.A.c, an, kk, w.partial);
+        hipLaunchKernelGGL(kw_expect_final, dim3(1), dim3(kBlock), 0, stream, w.partial, (int64_t)grid, w.out);
+        HIPCHK(hipGetLastError());
+        double v = 0.0;
+        HIPCHK(hipMemcpyAsync(&v, w.out, sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        return v;
+    }
+
+    // _apply! of a non-compilable strategy: its operations in order.
+    template <int Wt>
+    int64_t win_apply_ops(int64_t n, int& cur, const pp_strategy_op* ops, int nops) {
+        for (int o = 0; o < nops; ++o) {
+            const pp_strategy_op& op = ops[o];
+            if (op.kind == PP_OP_FILTER) {
+                win_filter(2, &op.filter);
+                n = win_compact<Wt>(n, cur, 0, 0.0, 0);
+            } else if (op.kind == PP_OP_ADAPTIVE) {
+                // spv_evolve.jl:103-114: threshold = the (n - max_terms)-th
+                // smallest |c| (1-based), then coeff_clip!
+                pp_filter f = {INT64_MAX, INT64_MAX, INT64_MAX, op.value, 0.0, -1.0, 0.0, -1.0};
+                if (n > op.max_terms) f.thresh = win_select_abs(n, cur, n - op.max_terms - 1);
+                win_filter(2, &f);
+                n = win_compact<Wt>(n, cur, 0, 0.0, 0);
+            } else if (op.kind == PP_OP_STOCHASTIC_COEFF) {
+                const uint64_t seed = op.seed ^ (0x9e3779b97f4a7c15ull * (++win.rng_calls));
+                n = win_compact<Wt>(n, cur, 1, op.value, seed);
+            } else {
+                throw PPError(PP_EARG, "unknown strategy operation kind " + std::to_string(op.kind));
+            }
+        }
+        return n;
+    }
+
+    struct WinArgs {
+        const pp_strategy_op* ops = nullptr;  // non-compilable boundary strategy (nops > 0)
+        int nops = 0;
+        const uint64_t* ket = nullptr;        // EnergyCorrection ket (null: none)
+        double energy = 0.0;
+    };
+
+    template <int Wt>
+    void evolve_window(const std::vector<GenDev>& gens, int64_t window, const pp_filter* strict, const pp_filter* local,
+                       WinArgs& a, pp_counters* cnt) {
+        win_init();
+        sync_state();
+        SegState& st = h_state[0];
+        int cur = st.cur;
+        int64_t n = st.n[cur];
+        int64_t an = 0;
+        int64_t app_cap = st.app_cap_v, live_cap = st.live_cap_v;
+        const int64_t L = (int64_t)gens.size();
+        WinBuf& w = win;
+        if (w.cap_gens < L) {
+            dfree(w.gens);
+            w.cap_gens = L;
+            w.gens = dalloc<GenDev>(L);
+        }
+        HIPCHK(hipMemcpyAsync(w.gens, gens.data(), (size_t)L * sizeof(GenDev), hipMemcpyHostToDevice, stream));
+        win_filter(0, a.nops > 0 ? nullptr : strict);
+        win_filter(1, local);
+        pp_counters c{};
+        double t_merge = 0.0;
+        auto boundary = [&](bool early) {
+            const auto t0 = std::chrono::steady_clock::now();
+            const double before = a.ket ? win_expect<Wt>(n, cur, an, a.ket) : 0.0;
+            const int64_t m = an;
+            DevSet* S = win_sort<Wt>(m);
+            const int64_t out = win_merge<Wt>(n, cur, S, m, 0);
+            // _try_merge! overflow -> _grow_live!(v, n + m) (spv_kernels.jl:377-386)
+            if (out > live_cap) live_cap = std::max(2 * live_cap, n + m);
+            c.merge_in += n + m;
+            c.merge_out += out;
+            cur ^= 1;
+            n = out;
+            an = 0;
+            if (a.nops > 0) n = win_apply_ops<Wt>(n, cur, a.ops, a.nops);
+            if (a.ket) a.energy += win_expect<Wt>(n, cur, 0, a.ket) - before;
+            if (early) c.early_merges += 1;
+            t_merge += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        };
+        int since = 0;
+        const auto tstart = std::chrono::steady_clock::now();
+        for (int64_t i = 0; i < L; ++i) {
+            if (2 * an + n > app_cap) {  // spv_evolve.jl:356-362
+                boundary(true);
+                since = 0;
+                if (n > app_cap) app_cap = std::max(2 * app_cap, n);
+            }
+            c.terms_swept += n + an;
+            c.rotations += 1;
+            const int64_t created = win_rotate<Wt>(w.gens + i, n, an, cur);
+            c.terms_created += created;
+            an += created;
+            since += 1;
+            if ((i + 1) % window == 0 || i == L - 1) {
+                boundary(false);
+                since = 0;
+            }
+        }
+        (void)since;
+        const double total = std::chrono::duration<double>(std::chrono::steady_clock::now() - tstart).count();
+        c.t_merge = t_merge;
+        c.t_rotate = total - t_merge;
+        st.cur = cur;
+        st.n[cur] = n;
+        st.n[cur ^ 1] = 0;
+        st.flip = 0;
+        st.app_cap_v = app_cap;
+        st.live_cap_v = live_cap;
+        st.rotations += c.rotations;
+        st.swept += c.terms_swept;
+        st.created += c.terms_created;
+        st.merge_in += c.merge_in;
+        st.merge_out += c.merge_out;
+        st.early += c.early_merges;
+        n_known[0] = n;
+        put_state();
+        upload_desc();
+        HIPCHK(hipStreamSynchronize(stream));
+        if (cnt) *cnt = c;
+    }
+
+    // truncate!(v, strategy, corr) for any strategy (truncation.jl:330-337).
+    template <int Wt>
+    void truncate_ops(const pp_strategy_op* ops, int nops, const uint64_t* ket, double* energy) {
+        win_init();
+        sync_state();
+        SegState& st = h_state[0];
+        int cur = st.cur;
+        int64_t n = st.n[cur];
+        const double before = ket ? win_expect<Wt>(n, cur, 0, ket) : 0.0;
+        n = win_apply_ops<Wt>(n, cur, ops, nops);
+        if (ket) *energy += win_expect<Wt>(n, cur, 0, ket) - before;
+        st.cur = cur;
+        st.n[cur] = n;
+        st.n[cur ^ 1] = 0;
+        st.flip = 0;
+        n_known[0] = n;
+        put_state();
+        upload_desc();
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
     void upload(int b, const uint64_t* z, const uint64_t* x, const double* c, int64_t n, bool sorted) {
         const int Wl = W;
         std::vector<uint64_t> hz(z, z + n * Wl), hx(x, x + n * Wl);
@@ -852,9 +1267,10 @@ int pp_evolve(pp_handle* h, const uint64_t* gz, const uint64_t* gx, const double
     if (window < 1) return fail(PP_EARG, "window must be >= 1");
     if (L < 0) return fail(PP_EARG, "L must be >= 0");
     if (L > 0 && (!gz || !gx || !cosv || !sinv)) return fail(PP_EARG, "null generator arrays");
-    if (window != 1) return fail(PP_EUNSUPPORTED, "window > 1 is not available in this build");
     if (counters) std::memset(counters, 0, sizeof(*counters));
     if (L == 0) return PP_OK;
+    if (window != 1) return pp_evolve_ex(h, gz, gx, cosv, sinv, L, window, strict, nullptr, 0, local, nullptr,
+                                         nullptr, counters);
     API_BEGIN
     Batch& bt = h->bt;
     bt.timing = counters != nullptr;
@@ -877,6 +1293,53 @@ int pp_evolve(pp_handle* h, const uint64_t* gz, const uint64_t* gx, const double
         (void)hipEventDestroy(e1);
     }
     bt.timing = false;
+    API_END
+}
+
+static void check_ops(const pp_strategy_op* ops, int nops) {
+    if (nops < 0 || (nops > 0 && !ops)) throw PPError(PP_EARG, "bad strategy operations");
+    for (int o = 0; o < nops; ++o) {
+        const pp_strategy_op& op = ops[o];
+        if (op.kind == PP_OP_ADAPTIVE && op.max_terms < 0) throw PPError(PP_EARG, "max_terms must be >= 0");
+        if (op.kind == PP_OP_STOCHASTIC_COEFF && !(op.value > 0.0)) throw PPError(PP_EARG, "epsilon must be > 0");
+        if (op.kind < PP_OP_FILTER || op.kind > PP_OP_STOCHASTIC_COEFF)
+            throw PPError(PP_EARG, "unknown strategy operation kind " + std::to_string(op.kind));
+    }
+}
+
+int pp_evolve_ex(pp_handle* h, const uint64_t* gz, const uint64_t* gx, const double* cosv, const double* sinv,
+                 int64_t L, int64_t window, const pp_filter* strict, const pp_strategy_op* ops, int nops,
+                 const pp_filter* local, const uint64_t* energy_ket, double* energy_acc, pp_counters* counters) {
+    if (!h) return fail(PP_EARG, "null handle");
+    if (window < 1) return fail(PP_EARG, "window must be >= 1");
+    if (L < 0) return fail(PP_EARG, "L must be >= 0");
+    if (L > 0 && (!gz || !gx || !cosv || !sinv)) return fail(PP_EARG, "null generator arrays");
+    if (energy_ket && !energy_acc) return fail(PP_EARG, "energy_acc is NULL");
+    if (counters) std::memset(counters, 0, sizeof(*counters));
+    if (L == 0) return PP_OK;
+    API_BEGIN
+    check_ops(ops, nops);
+    Batch& bt = h->bt;
+    std::vector<GenDev> gens((size_t)L);
+    for (int64_t i = 0; i < L; ++i) gens[i] = make_gen(bt.W, gz + i * bt.W, gx + i * bt.W, cosv[i], sinv[i]);
+    Batch::WinArgs a;
+    a.ops = ops;
+    a.nops = nops;
+    a.ket = energy_ket;
+    a.energy = energy_ket ? *energy_acc : 0.0;
+    dispatch_w(bt.W, [&](auto wc) { bt.evolve_window<decltype(wc)::value>(gens, window, strict, local, a, counters); });
+    if (energy_ket) *energy_acc = a.energy;
+    API_END
+}
+
+int pp_truncate(pp_handle* h, const pp_strategy_op* ops, int nops, const uint64_t* energy_ket, double* energy_acc) {
+    if (!h) return fail(PP_EARG, "null handle");
+    if (energy_ket && !energy_acc) return fail(PP_EARG, "energy_acc is NULL");
+    API_BEGIN
+    check_ops(ops, nops);
+    double e = energy_ket ? *energy_acc : 0.0;
+    dispatch_w(h->bt.W, [&](auto wc) { h->bt.truncate_ops<decltype(wc)::value>(ops, nops, energy_ket, &e); });
+    if (energy_ket) *energy_acc = e;
     API_END
 }
 

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(PKG_DIR, "_build", "libpauliprop.so")
 PP_OK, PP_EARG, PP_ENOMEM, PP_EHIP, PP_EOVERFLOW, PP_EUNSUPPORTED = range(6)
 
 EXPORTS = ("pp_create", "pp_destroy", "pp_upload", "pp_download", "pp_size", "pp_capacities", "pp_evolve",
-           "pp_evolve_single", "pp_compact", "pp_scale_by_popcount", "pp_expectation_ket", "pp_check",
+           "pp_evolve_ex", "pp_truncate", "pp_evolve_single", "pp_compact", "pp_scale_by_popcount", "pp_expectation_ket", "pp_check",
            "pp_batch_create", "pp_batch_destroy", "pp_batch_words", "pp_batch_upload", "pp_batch_download",
            "pp_batch_sizes", "pp_batch_evolve", "pp_batch_scale_by_popcount", "pp_batch_expectation_ket",
            "pp_batch_reserve", "pp_batch_set_tiles", "pp_batch_set_timing", "pp_batch_merge_time",
@@ -34,6 +34,16 @@ class PPCounters(ctypes.Structure):
 
     def asdict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+PP_OP_FILTER, PP_OP_ADAPTIVE, PP_OP_STOCHASTIC_COEFF = range(3)
+
+
+class PPStrategyOp(ctypes.Structure):
+    """pp_strategy_op: one step of a non-compilable boundary strategy."""
+
+    _fields_ = [("kind", ctypes.c_int32), ("reserved", ctypes.c_int32), ("filter", PPFilter),
+                ("max_terms", ctypes.c_int64), ("value", ctypes.c_double), ("seed", ctypes.c_uint64)]
 
 
 class PPError(RuntimeError):
@@ -72,6 +82,9 @@ def lib():
     L.pp_size.argtypes = [vp, P(i64)]
     L.pp_capacities.argtypes = [vp, P(i64), P(i64)]
     L.pp_evolve.argtypes = [vp, u64p, u64p, dp, dp, i64, i64, P(PPFilter), P(PPFilter), P(PPCounters)]
+    L.pp_evolve_ex.argtypes = [vp, u64p, u64p, dp, dp, i64, i64, P(PPFilter), P(PPStrategyOp), ctypes.c_int,
+                               P(PPFilter), u64p, dp, P(PPCounters)]
+    L.pp_truncate.argtypes = [vp, P(PPStrategyOp), ctypes.c_int, u64p, dp]
     L.pp_evolve_single.argtypes = [vp, u64p, u64p, ctypes.c_double, ctypes.c_double]
     L.pp_compact.argtypes = [vp, P(PPFilter)]
     L.pp_scale_by_popcount.argtypes = [vp, ctypes.c_int, u64p, dp, dp, dp]

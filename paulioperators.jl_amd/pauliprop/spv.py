@@ -18,12 +18,54 @@ from . import _lib
 from ._lib import ArgumentError, DimensionMismatch, PPCounters, check, dp, u64p
 from .basis import Ket, PauliBasis, PauliSum, word_count
 from .truncation import (AdaptiveTruncation, CoeffTruncation, CompositeTruncation, CorrectionAccumulator,
-                         MajoranaWeightTruncation, NoCorrection, NoTruncation, StochasticCoeffTruncation,
+                         EnergyCorrection, MajoranaWeightTruncation, NoCorrection, NoTruncation,
+                         StochasticCoeffTruncation, strategy_steps,
                          StochasticSamplingTruncation, TruncationStrategy, WeightDampedTruncation,
                          WeightTruncation, XWeightDampedTruncation, XWeightTruncation, compile_filter,
                          is_compilable, NOFILTER)
 
 M64 = (1 << 64) - 1
+
+
+def _rng_seed(rng) -> int:
+    """A 64-bit key for the device's counter-based generator: an int is used
+    as given, a numpy Generator / random.Random draws one, None draws from
+    the OS (the reference's Random.default_rng() default)."""
+    if rng is None:
+        return int.from_bytes(__import__("os").urandom(8), "little")
+    if isinstance(rng, (int, np.integer)):
+        return int(rng) & M64
+    if hasattr(rng, "integers"):
+        return int(rng.integers(0, 1 << 63)) | (int(rng.integers(0, 2)) << 63)
+    return int(rng.getrandbits(64))
+
+
+def _strategy_ops(strategy):
+    """pp_strategy_op array for a strategy's _apply! steps."""
+    steps = strategy_steps(strategy)
+    ops = (_lib.PPStrategyOp * max(len(steps), 1))()
+    for i, st in enumerate(steps):
+        if st[0] == "filter":
+            ops[i].kind = _lib.PP_OP_FILTER
+            ops[i].filter = st[1]
+        elif st[0] == "adaptive":
+            ops[i].kind = _lib.PP_OP_ADAPTIVE
+            ops[i].max_terms = st[1]
+            ops[i].value = st[2]
+        else:
+            ops[i].kind = _lib.PP_OP_STOCHASTIC_COEFF
+            ops[i].value = st[1]
+            ops[i].seed = _rng_seed(st[2])
+    return ops, len(steps)
+
+
+def _energy_args(correction, W):
+    if isinstance(correction, NoCorrection):
+        return None, None
+    if isinstance(correction, EnergyCorrection):
+        ket = _pack(correction.ket_int() & ((1 << (64 * W)) - 1), W)
+        return ket, ctypes.c_double(correction.accumulated_energy)
+    raise NotImplementedError(f"{type(correction).__name__} is not available on the device path in this build")
 
 
 def _pack(v: int, W: int) -> np.ndarray:
@@ -198,20 +240,27 @@ class SparsePauliVector:
         if not is_compilable(local_truncation):
             raise ArgumentError("local_truncation must be a deterministic (weight/coefficient) strategy "
                                 "— it runs per append")
-        if not isinstance(correction, NoCorrection):
-            raise NotImplementedError("correction accumulators are not available in this build")
-        if not is_compilable(truncation):
-            raise NotImplementedError(f"{type(truncation).__name__} at merge boundaries is not available in this build")
+        ket, acc = _energy_args(correction, self.W)
+        compiled = is_compilable(truncation)
+        ops, nops = (None, 0) if compiled else _strategy_ops(truncation)
         if not gens:
             return self
         gz, gx = self._gen_arrays(gens)
         cv = np.array([math.cos(a) for a in angs], dtype=np.float64)
         sv = np.array([math.sin(a) for a in angs], dtype=np.float64)
-        fs = compile_filter(truncation)
+        fs = compile_filter(truncation) if compiled else NOFILTER()
         fl = compile_filter(local_truncation)
         cnt = counters if counters is not None else PPCounters()
-        check(_lib.lib().pp_evolve(self._h, u64p(gz), u64p(gx), dp(cv), dp(sv), len(gens), int(window),
-                                   ctypes.byref(fs), ctypes.byref(fl), ctypes.byref(cnt)))
+        if compiled and ket is None:
+            check(_lib.lib().pp_evolve(self._h, u64p(gz), u64p(gx), dp(cv), dp(sv), len(gens), int(window),
+                                       ctypes.byref(fs), ctypes.byref(fl), ctypes.byref(cnt)))
+        else:
+            check(_lib.lib().pp_evolve_ex(self._h, u64p(gz), u64p(gx), dp(cv), dp(sv), len(gens), int(window),
+                                          ctypes.byref(fs), ops, nops, ctypes.byref(fl),
+                                          u64p(ket) if ket is not None else None,
+                                          ctypes.byref(acc) if acc is not None else None, ctypes.byref(cnt)))
+            if acc is not None:
+                correction.accumulated_energy = acc.value
         return self
 
     def evolve(self, generators, angles, **kw) -> "SparsePauliVector":
@@ -246,12 +295,23 @@ class SparsePauliVector:
         return self._compact(compile_filter(MajoranaWeightTruncation(int(max_weight))))
 
     def truncate_(self, strategy: TruncationStrategy, corr: CorrectionAccumulator = NoCorrection()):
-        """``truncate!(v, strategy, corr)`` (truncation.jl:330-337) for compilable strategies."""
-        if not isinstance(corr, NoCorrection):
-            raise NotImplementedError("correction accumulators are not available in this build")
-        if is_compilable(strategy):
+        """``truncate!(v, strategy, corr)`` (truncation.jl:330-337)."""
+        ket, acc = _energy_args(corr, self.W)
+        if is_compilable(strategy) and ket is None:
             return self._compact(compile_filter(strategy))
-        raise NotImplementedError(f"{type(strategy).__name__} is not available in this build")
+        ops, nops = _strategy_ops(strategy)
+        check(_lib.lib().pp_truncate(self._h, ops, nops, u64p(ket) if ket is not None else None,
+                                     ctypes.byref(acc) if acc is not None else None))
+        if acc is not None:
+            corr.accumulated_energy = acc.value
+        return self
+
+    def stochastic_clip_(self, epsilon: float, rng=None):
+        """``stochastic_clip!(v, ε; rng)`` (spv_evolve.jl:35-52): |c| < ε survives
+        with probability |c|/ε as ε·sign(c).  The uniform variates come from a
+        counter-based generator keyed by ``rng`` (an int seed, a numpy
+        Generator or random.Random, or None), not from Julia's stream."""
+        return self.truncate_(StochasticCoeffTruncation(float(epsilon), rng))
 
     # --- channels (channels.jl:203-265) ------------------------------------------------------
     def _mask(self, qubits) -> np.ndarray:
@@ -331,6 +391,10 @@ def evolve(v: SparsePauliVector, generators, angles, **kw):
 
 def truncate_(v: SparsePauliVector, strategy, corr=NoCorrection()):
     return v.truncate_(strategy, corr)
+
+
+def stochastic_clip_(v: SparsePauliVector, epsilon: float, rng=None):
+    return v.stochastic_clip_(epsilon, rng)
 
 
 def expectation_value(v: SparsePauliVector, psi) -> float:

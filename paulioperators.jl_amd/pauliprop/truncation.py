@@ -184,3 +184,45 @@ class CorrectionAccumulator:
 
 class NoCorrection(CorrectionAccumulator):
     pass
+
+
+class EnergyCorrection(CorrectionAccumulator):
+    """truncation.jl:262-266: accumulates the change of ⟨ψ|O|ψ⟩ caused by
+    every truncation (after - before, truncation.jl:304-306)."""
+
+    def __init__(self, psi, accumulated_energy: float = 0.0):
+        self.psi = psi
+        self.accumulated_energy = float(accumulated_energy)
+
+    def ket_int(self) -> int:
+        v = getattr(self.psi, "v", self.psi)
+        return int(v)
+
+
+class EnergyVarianceCorrection(CorrectionAccumulator):
+    """truncation.jl:273-278.  Its variance measure needs the merged state and
+    O² (statistics.jl:6); not available on the device path in this build."""
+
+    def __init__(self, psi):
+        self.psi = psi
+        self.accumulated_energy = 0.0
+        self.accumulated_variance = 0.0
+
+
+def strategy_steps(s: TruncationStrategy) -> list:
+    """A strategy as the ordered steps of its _apply! (spv_evolve.jl:59-119;
+    CompositeTruncation applies its members in order, truncation.jl:236-243):
+    ("filter", PPFilter) | ("adaptive", max_terms, min_thresh) |
+    ("stochastic_coeff", epsilon, rng)."""
+    if is_compilable(s):
+        return [("filter", compile_filter(s))]
+    if isinstance(s, AdaptiveTruncation):
+        return [("adaptive", int(s.max_terms), float(s.min_thresh))]
+    if isinstance(s, StochasticCoeffTruncation):
+        return [("stochastic_coeff", float(s.epsilon), s.rng)]
+    if isinstance(s, CompositeTruncation):
+        out = []
+        for t in s.strategies:
+            out.extend(strategy_steps(t))
+        return out
+    raise NotImplementedError(f"{type(s).__name__} is not available on the device path in this build")

@@ -226,3 +226,68 @@ def statevector_expectation(N: int, O: Dict[Key, float], gens: Sequence[Key], an
     for k, c in O.items():
         acc += c * np.vdot(psi, apply_pauli_state(N, psi, k))
     return acc.real
+
+
+# --- windowed flat-storage restatement (spv_evolve.jl:312-387), stable sort ----
+
+def window_evolve(N: int, terms: Sequence[Tuple[Key, float]], gens: Sequence[Key], angs: Sequence[float],
+                  window: int, drop_strict, drop_local, energy_ket=None):
+    """Pure-Python restatement of the windowed SparsePauliVector driver with
+    unlimited append capacity (no early merges) and a *stable* boundary sort:
+    rotations sweep live terms then the pending appends present at entry
+    (spv_evolve.jl:207-218), appending children under ``drop_local``; every
+    ``window`` rotations the appends are sorted (equal keys in append order)
+    and merged, live first on ties, equal appends summed left to right, with
+    ``drop_strict`` on every output (spv_kernels.jl:326-363).
+    drop_*(key, |c|) -> bool.  Returns (sorted [(key, c)], accumulated energy)."""
+    live = sorted(terms, key=lambda t: (t[0][0], t[0][1]))
+    app: List[Tuple[Key, float]] = []
+    acc_e = 0.0
+
+    def expect(items):
+        s = 0.0
+        for (z, x), c in items:
+            if x == 0:
+                s = s + (1 - 2 * (bin(z & energy_ket).count("1") & 1)) * c
+        return s
+
+    def boundary():
+        nonlocal live, app, acc_e
+        before = expect(live + app) if energy_ket is not None else 0.0
+        srt = sorted(app, key=lambda t: (t[0][0], t[0][1]))  # stable
+        out = []
+        i = j = 0
+        while i < len(live) or j < len(srt):
+            if j >= len(srt) or (i < len(live) and not (srt[j][0] < live[i][0])):
+                k, a = live[i]
+                i += 1
+            else:
+                k, a = srt[j]
+                j += 1
+            while j < len(srt) and srt[j][0] == k:
+                a = a + srt[j][1]
+                j += 1
+            if not drop_strict(k, abs(a)):
+                out.append((k, a))
+        live, app = out, []
+        if energy_ket is not None:
+            acc_e += expect(live) - before
+
+    for i, (g, th) in enumerate(zip(gens, angs)):
+        c_, s_ = math.cos(th), math.sin(th)
+        new_children = []
+        for seq in (live, app):
+            for idx in range(len(seq)):
+                p, c = seq[idx]
+                if not anticommute(p, g):
+                    continue
+                phase, k = pauli_product(N, g, p)
+                f = 1j * phase
+                v = c * s_ if f.real > 0 else -(c * s_)
+                seq[idx] = (p, c * c_)
+                if not drop_local(k, abs(v)):
+                    new_children.append((k, v))
+        app.extend(new_children)
+        if (i + 1) % window == 0 or i + 1 == len(gens):
+            boundary()
+    return live, acc_e

@@ -46,3 +46,14 @@ def test_julia_powi_matches_oracle():
 
 def test_version_string():
     assert _lib.lib().pp_version().startswith(b"pauliprop")
+
+
+def test_strategy_op_struct_layout():
+    # pp_strategy_op: int32 kind, int32 reserved, pp_filter (64 B), int64, double, uint64
+    assert ctypes.sizeof(_lib.PPStrategyOp) == 96
+    assert _lib.PPStrategyOp.filter.offset == 8
+    assert _lib.PPStrategyOp.max_terms.offset == 72
+    assert _lib.PPStrategyOp.seed.offset == 88
+    hdr = open(os.path.join(ROOT, "include", "pauliprop.h")).read()
+    assert re.search(r"PP_OP_FILTER = 0, PP_OP_ADAPTIVE = 1, PP_OP_STOCHASTIC_COEFF = 2", hdr)
+    assert (_lib.PP_OP_FILTER, _lib.PP_OP_ADAPTIVE, _lib.PP_OP_STOCHASTIC_COEFF) == (0, 1, 2)

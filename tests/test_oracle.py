@@ -397,3 +397,118 @@ def test_statevector_expectation_end_to_end():
     got = v.expectation(ket)
     exp = sum(c * (-1) ** bin(z & ket).count("1") for z, x, c in v.terms() if x == 0)
     assert abs(got - exp) < 1e-12
+
+
+# --- windowed mode: boundary sort order and EnergyCorrection ---------------------
+
+def _rand_circuit(N, L, seed, lo=1):
+    """L random 1- and 2-site generators on sites lo..min(lo+11, N)."""
+    rng = random.Random(seed)
+    hi = min(lo + 11, N)
+    gens, angs = [], []
+    for _ in range(L):
+        q = rng.randrange(lo, hi + 1)
+        kind = rng.randrange(3)
+        if kind == 0 and q < hi:
+            p = Pauli(N, Z=[q, q + 1])
+        elif kind == 1 or q == hi:
+            p = Pauli(N, X=[q])
+        else:
+            p = Pauli(N, Y=[q], X=[q + 1])
+        gens.append((p.z, p.x))
+        angs.append(rng.uniform(-math.pi, math.pi))
+    return gens, angs
+
+
+def _pydrop(f):
+    return lambda k, a: po.should_drop(_N_CUR[0], f, k[0], k[1], a)
+
+
+_N_CUR = [0]
+
+
+@pytest.mark.parametrize("N,window", [(10, 3), (70, 5), (12, 64)])
+def test_window_stable_mode_bitexact_vs_python_restatement(N, window):
+    """Stable-sort mode of the oracle against the independent Python restatement
+    of the windowed driver: same string set, same coefficients bit for bit
+    (duplicates summed in append order), same EnergyCorrection accumulation."""
+    _N_CUR[0] = N
+    lo = max(1, N - 11)
+    gens, angs = _rand_circuit(N, 100, seed=N + window, lo=lo)
+    fs = compile_filter(CompositeTruncation(CoeffTruncation(1e-7), WeightTruncation(6)))
+    fl = compile_filter(WeightTruncation(7))
+    terms = [((Pauli(N, Z=[lo + 5]).z, 0), 1.0), ((Pauli(N, X=[lo]).z, Pauli(N, X=[lo]).x), 0.5)]
+    ket = random.Random(1).getrandbits(N)
+    ref, ref_e = dp.window_evolve(N, list(terms), gens, angs, window, _pydrop(fs), _pydrop(fl), energy_ket=ket)
+    v = po.OracleSPV(N, [(k[0], k[1], c) for k, c in terms], capacity_factor=1e5, append_factor=64)
+    po.set_sort_mode(True)
+    try:
+        e = [ket]
+        cnt = v.evolve(gens, angs, window=window, strict=fs, local=fl, energy=e)
+    finally:
+        po.set_sort_mode(False)
+    assert cnt.early_merges == 0
+    got = [((z, x), c) for z, x, c in v.terms()]
+    assert [k for k, _ in got] == [k for k, _ in ref]
+    assert all(a == b for (_, a), (_, b) in zip(got, ref))
+    assert e[1] == ref_e
+    assert len(ref) > 30
+
+
+def test_window_quicksort_and_stable_agree_to_ulp():
+    """The reference's quicksort and the stable sort differ only in the order
+    equal keys are summed: coefficients agree to 1e-12 relative and the string
+    sets agree away from the cut."""
+    N = 12
+    gens, angs = _rand_circuit(N, 120, seed=3)
+    eps = 1e-6
+    fs = compile_filter(CoeffTruncation(eps))
+    res = []
+    for stable in (False, True):
+        v = po.OracleSPV(N, [(Pauli(N, Z=[6]).z, 0, 1.0)], append_factor=8)
+        po.set_sort_mode(stable)
+        try:
+            v.evolve(gens, angs, window=16, strict=fs)
+        finally:
+            po.set_sort_mode(False)
+        res.append({(z, x): c for z, x, c in v.terms()})
+    a, b = res
+    for k in set(a) | set(b):
+        if k in a and k in b:
+            assert abs(a[k] - b[k]) <= 1e-12 * abs(a[k])
+        else:
+            c = a.get(k, b.get(k))
+            assert abs(abs(c) - eps) <= 1e-12 * eps
+    assert len(a) > 100
+
+
+def test_energy_correction_window1_matches_dict_path():
+    """EnergyCorrection at window = 1 against evolve!; truncate!(O, s, corr) per
+    rotation on the Dict path (truncation.jl:330-337): accumulated energy
+    agrees to rounding; without truncation it is zero up to rounding."""
+    N = 8
+    gens, angs = heisenberg_trotter(N, dt=0.3, nsteps=3)
+    O0 = {(Pauli(N, Z=[4]).z, 0): 1.0}
+    strat = CoeffTruncation(1e-2)
+    ket = 0b10110010
+    ref = dict(O0)
+    acc = 0.0
+
+    def ev(O):
+        return sum(c * (1 - 2 * (bin(z & ket).count("1") & 1)) for (z, x), c in O.items() if x == 0)
+
+    for g, a in zip(gens, angs):
+        dp.dict_evolve_single(N, ref, g, a)
+        before = ev(ref)
+        dp.dict_truncate(ref, strat)
+        acc += ev(ref) - before
+    v = _oracle_from(N, O0)
+    e = [ket]
+    v.evolve(gens, angs, strict=compile_filter(strat), energy=e)
+    assert abs(acc) > 1e-4
+    assert abs(e[1] - acc) < 1e-13
+    assert abs(v.expectation(ket) - ev(ref)) < 1e-13
+    v0 = _oracle_from(N, O0)
+    e0 = [ket]
+    v0.evolve(gens, angs, energy=e0)
+    assert abs(e0[1]) < 1e-14
