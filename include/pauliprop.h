@@ -183,6 +183,37 @@ int pp_batch_set_tiles(pp_batch* p, int tile, int merge_tile);
 int pp_batch_set_timing(pp_batch* p, int on);
 int pp_batch_merge_time(pp_batch* p, double* seconds, int64_t* launches, int reset);
 
+/* ---- Learned surrogate of ⟨O⟩(θ) -------------------------------------------
+ * No counterpart in the reference: an ensemble of E MLPs
+ * din -> hidden -> ... (layers hidden layers, GELU) -> 1, trained with Adam on
+ * ⟨O⟩ labels produced by the propagation path above, used with an ensemble
+ * disagreement gate that falls back to exact propagation.  BF16 matrix-core
+ * GEMMs with FP32 accumulation; FP32 master weights and optimizer state.
+ * din <= 64; hidden a multiple of 64; batch sizes multiples of 64.
+ * Host parameter layout per member (floats, pp_mlp_param_count of them):
+ * W_1[hidden][din], b_1[hidden], then (W_l[hidden][hidden], b_l[hidden]) for
+ * each further hidden layer, then w_out[hidden], b_out. */
+typedef struct pp_mlp pp_mlp;
+int pp_mlp_create(int E, int din, int hidden, int layers, pp_mlp** out);
+int pp_mlp_destroy(pp_mlp* h);
+int pp_mlp_param_count(pp_mlp* h, int64_t* count);
+/* params[e*count ..] for every member; resets the Adam state. */
+int pp_mlp_set_params(pp_mlp* h, const float* params);
+int pp_mlp_get_params(pp_mlp* h, float* params);
+/* Training set: x[n][din], y[n], copied to the device. */
+int pp_mlp_load_data(pp_mlp* h, const float* x, const float* y, int64_t n);
+/* One epoch of floor(n/batch) Adam steps (lr; betas 0.9, 0.999; eps 1e-8) on
+ * the mean squared error; member e takes rows in the order perm[e*n ..].
+ * loss[e]: mean minibatch loss of member e over the epoch. */
+int pp_mlp_train_epoch(pp_mlp* h, const int32_t* perm, int batch, float lr, float* loss);
+/* Gradients of each member's mean squared error on one batch x[batch][din],
+ * t[batch] (grads[e*count ..], host layout) and the losses (may be NULL). */
+int pp_mlp_gradients(pp_mlp* h, const float* x, const float* t, int batch, float* grads, float* loss);
+/* Predictions for x[n][din]: members[e*n + i] (may be NULL), ensemble mean
+ * and population standard deviation; seconds (may be NULL) = device time. */
+int pp_mlp_predict(pp_mlp* h, const float* x, int64_t n, float* members, double* mean, double* sd,
+                   double* seconds);
+
 /* Julia's Float64^Int (compensated power by squaring, base/math.jl), for
  * callers that are not Julia and want the same channel tables. */
 double pp_julia_powi(double x, int64_t n);

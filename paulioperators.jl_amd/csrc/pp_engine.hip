@@ -178,6 +178,10 @@ struct WinBuf {
 
 }  // namespace
 
+namespace pp {
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace pp
+
 // ---------------------------------------------------------------------------
 struct Batch {
     int W = 1, B = 0, ncu = 256;

@@ -1,0 +1,750 @@
+// libpauliprop: learned surrogate of <O>(theta) — an ensemble of E MLPs
+// din -> H -> ... -> H -> 1 (GELU), trained with Adam on labels produced by
+// the propagation engine.  BF16 operands on the matrix cores (gfx950
+// v_mfma_f32_32x32x16_bf16), FP32 accumulation, FP32 master weights and
+// optimizer state.  This part has no counterpart in the reference.
+//
+// Layouts (per member e, row-major):
+//   input       X[b][KIN]          bf16, KIN = din rounded up to 64 (zero pad)
+//   hidden l    W_l[H][K_l] bf16 copy of the FP32 master, b_l[H] fp32
+//   output      w_o[H], b_o        fp32
+//   activations Z_l[b][H] fp32 (pre-activation), A_l[b][H] bf16 = gelu(Z_l)
+// Every GEMM is C[m][n] = sum_k A(m,k) B(n,k) with element strides, so the
+// forward (X W^T), the input gradient (dZ W) and the weight gradient
+// (dZ^T X) use one kernel; operands are staged through LDS in 64 x 32 tiles.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pauliprop.h"
+
+namespace pp {
+void set_last_error(const std::string& msg);  // pp_last_error() of the library
+}
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+constexpr int kBM = 64, kBN = 64, kBK = 32, kLDK = kBK + 8;
+constexpr int kKin = 64;
+
+struct MlpError {
+    int code;
+    std::string msg;
+};
+
+#define MCHK(expr)                                                                                   \
+    do {                                                                                             \
+        hipError_t _e = (expr);                                                                      \
+        if (_e != hipSuccess)                                                                        \
+            throw MlpError{_e == hipErrorOutOfMemory ? PP_ENOMEM : PP_EHIP,                          \
+                           std::string(#expr) + ": " + hipGetErrorString(_e)};                       \
+    } while (0)
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    // round to nearest even (inputs are finite)
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float x) {
+    return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
+}
+
+struct GemmArgs {
+    const uint16_t* A;
+    int64_t sam, sak, sae;
+    const uint16_t* B;
+    int64_t sbn, sbk, sbe;
+    int M, N, K;
+    float* C;  // EPI 0: C = acc; EPI 1: C = acc + bias (pre-activation)
+    int64_t ldc, sce;
+    uint16_t* H;  // EPI 1: gelu(acc + bias); EPI 2: acc * gelu'(Zp); EPI 3: gelu(acc + bias), no C
+    int64_t ldh, she;
+    const float* bias;
+    int64_t sbias;
+    const float* Zp;
+    int64_t ldz, sze;
+};
+
+// Stage a 64 x 32 tile (rows r0.., k0..) of a bf16 operand into LDS as
+// dst[row][k].  Either the k stride or the row stride is 1 (checked by the
+// host); 8 contiguous elements per thread, 16-byte loads.
+__device__ __forceinline__ void stage_tile(const uint16_t* __restrict__ src, int64_t srow, int64_t sk, int r0, int k0,
+                                           uint16_t* __restrict__ dst) {
+    const int t = threadIdx.x;
+    if (sk == 1) {
+        const int row = t >> 2, kc = (t & 3) * 8;
+        const uint4 v = *reinterpret_cast<const uint4*>(src + (int64_t)(r0 + row) * srow + k0 + kc);
+        *reinterpret_cast<uint4*>(dst + row * kLDK + kc) = v;
+    } else {
+        const int k = t >> 3, rc = (t & 7) * 8;
+        const uint4 v = *reinterpret_cast<const uint4*>(src + (int64_t)(k0 + k) * sk + r0 + rc);
+        const uint16_t* h = reinterpret_cast<const uint16_t*>(&v);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) dst[(rc + j) * kLDK + k] = h[j];
+    }
+}
+
+template <int EPI>
+__global__ void __launch_bounds__(256) kg_gemm(GemmArgs g) {
+    __shared__ __attribute__((aligned(16))) uint16_t sA[kBM * kLDK];
+    __shared__ __attribute__((aligned(16))) uint16_t sB[kBN * kLDK];
+    const int e = blockIdx.z;
+    const int m0 = blockIdx.y * kBM, n0 = blockIdx.x * kBN;
+    const uint16_t* A = g.A + e * g.sae;
+    const uint16_t* B = g.B + e * g.sbe;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    floatx16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+    for (int k0 = 0; k0 < g.K; k0 += kBK) {
+        stage_tile(A, g.sam, g.sak, m0, k0, sA);
+        stage_tile(B, g.sbn, g.sbk, n0, k0, sB);
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kBK; kk += 16) {
+            // lane l supplies row (l mod 32), k = kk + 8 (l div 32) .. +7
+            const bf16x8 a = *reinterpret_cast<const bf16x8*>(sA + (wm * 32 + (lane & 31)) * kLDK + kk + 8 * (lane >> 5));
+            const bf16x8 b = *reinterpret_cast<const bf16x8*>(sB + (wn * 32 + (lane & 31)) * kLDK + kk + 8 * (lane >> 5));
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // accumulator r of lane l: row 8 (r div 4) + 4 (l div 32) + (r mod 4), column l mod 32
+    const int col = n0 + wn * 32 + (lane & 31);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int row = m0 + wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const float v = acc[r];
+        if (EPI == 0) {
+            g.C[e * g.sce + (int64_t)row * g.ldc + col] = v;
+        } else if (EPI == 1 || EPI == 3) {
+            const float z = v + g.bias[e * g.sbias + col];
+            if (EPI == 1) g.C[e * g.sce + (int64_t)row * g.ldc + col] = z;
+            g.H[e * g.she + (int64_t)row * g.ldh + col] = f2bf(gelu(z));
+        } else {
+            const float zp = g.Zp[e * g.sze + (int64_t)row * g.ldz + col];
+            g.H[e * g.she + (int64_t)row * g.ldh + col] = f2bf(v * gelu_grad(zp));
+        }
+    }
+}
+
+// Minibatch gather: Xb[e][i][:] = X[perm[e][off + i]][:], Yb[e][i] likewise.
+__global__ void kg_gather(const uint16_t* __restrict__ X, const float* __restrict__ Y, const int32_t* __restrict__ perm,
+                          int64_t n, int64_t off, int Bsz, uint16_t* __restrict__ Xb, float* __restrict__ Yb) {
+    const int e = blockIdx.y;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int c = threadIdx.x & 63;
+    if (i >= Bsz) return;
+    const int64_t src = perm[(int64_t)e * n + off + i];
+    Xb[((int64_t)e * Bsz + i) * kKin + c] = X[src * kKin + c];
+    if (c == 0) Yb[(int64_t)e * Bsz + i] = Y[src];
+}
+
+// y[e][b] = sum_o A[e][b][o] w[e][o] + bo[e]: one wave per row.
+__global__ void kg_out_fwd(const uint16_t* __restrict__ A, int H, int Bsz, const float* __restrict__ w, int64_t swe,
+                           const float* __restrict__ bo, int64_t sbo, float* __restrict__ y) {
+    const int e = blockIdx.y;
+    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (b >= Bsz) return;
+    const uint16_t* a = A + ((int64_t)e * Bsz + b) * H;
+    const float* we = w + e * swe;
+    float s = 0.0f;
+    for (int o = lane; o < H; o += 64) s += bf2f(a[o]) * we[o];
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
+    if (lane == 0) y[(int64_t)e * Bsz + b] = s + bo[e * sbo];
+}
+
+// dy = 2 (y - t) / B (mean squared error per member), loss += (y - t)^2 / B.
+__global__ void kg_loss(const float* __restrict__ y, const float* __restrict__ t, int Bsz, float* __restrict__ dy,
+                        float* __restrict__ loss) {
+    const int e = blockIdx.y;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    float l = 0.0f;
+    if (b < Bsz) {
+        const float r = y[(int64_t)e * Bsz + b] - t[(int64_t)e * Bsz + b];
+        dy[(int64_t)e * Bsz + b] = 2.0f * r / (float)Bsz;
+        l = r * r / (float)Bsz;
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) l += __shfl_xor(l, d, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(loss + e, l);
+}
+
+// dZ[e][b][o] = bf16(dy[e][b] * w[e][o] * gelu'(Z[e][b][o]))
+__global__ void kg_dz_out(const float* __restrict__ dy, const float* __restrict__ w, int64_t swe,
+                          const float* __restrict__ Z, int H, int Bsz, uint16_t* __restrict__ dZ) {
+    const int e = blockIdx.y;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)Bsz * H) return;
+    const int b = (int)(i / H), o = (int)(i % H);
+    const int64_t q = (int64_t)e * Bsz * H + i;
+    dZ[q] = f2bf(dy[(int64_t)e * Bsz + b] * w[e * swe + o] * gelu_grad(Z[q]));
+}
+
+// out[e][o] += sum over rows b in this block's chunk of U[e][b][o] * (v ? v[e][b] : 1)
+__global__ void kg_colsum(const uint16_t* __restrict__ U, const float* __restrict__ v, int H, int Bsz, int rows_per,
+                          float* __restrict__ out, int64_t soe) {
+    const int e = blockIdx.z;
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= H) return;
+    const int b0 = blockIdx.y * rows_per;
+    const int b1 = min(Bsz, b0 + rows_per);
+    float s = 0.0f;
+    for (int b = b0; b < b1; ++b) {
+        const float u = bf2f(U[((int64_t)e * Bsz + b) * H + o]);
+        s += v ? u * v[(int64_t)e * Bsz + b] : u;
+    }
+    atomicAdd(out + e * soe + o, s);
+}
+
+// out[e] += sum_b v[e][b]
+__global__ void kg_sum(const float* __restrict__ v, int Bsz, float* __restrict__ out, int64_t soe) {
+    const int e = blockIdx.y;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    float s = b < Bsz ? v[(int64_t)e * Bsz + b] : 0.0f;
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out + e * soe, s);
+}
+
+// Adam (Kingma & Ba 2015) over the flat FP32 parameter array.
+__global__ void kg_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                        int64_t n, float lr, float b1, float b2, float eps, float c1, float c2) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// bf16 copy of a weight block [rows][cols] (row stride ld in the master).
+__global__ void kg_cast(const float* __restrict__ src, int64_t ld_src, int64_t se_src, uint16_t* __restrict__ dst,
+                        int64_t ld_dst, int64_t se_dst, int rows, int cols) {
+    const int e = blockIdx.y;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)rows * cols) return;
+    const int r = (int)(i / cols), c = (int)(i % cols);
+    dst[e * se_dst + (int64_t)r * ld_dst + c] = f2bf(src[e * se_src + (int64_t)r * ld_src + c]);
+}
+
+// fp32 [n][din] -> bf16 [n][kKin] with zero padding
+__global__ void kg_pack_input(const float* __restrict__ x, int64_t n, int din, uint16_t* __restrict__ X) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * kKin) return;
+    const int64_t r = i / kKin;
+    const int c = (int)(i % kKin);
+    X[i] = c < din ? f2bf(x[r * din + c]) : (uint16_t)0;
+}
+
+// mean and (population) standard deviation over the E members of each row
+__global__ void kg_ensemble(const float* __restrict__ y, int E, int64_t n, int64_t stride, double* __restrict__ mean,
+                            double* __restrict__ sd) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int e = 0; e < E; ++e) s += (double)y[e * stride + i];
+    const double mu = s / E;
+    double q = 0.0;
+    for (int e = 0; e < E; ++e) {
+        const double d = (double)y[e * stride + i] - mu;
+        q += d * d;
+    }
+    mean[i] = mu;
+    sd[i] = sqrt(q / E);
+}
+
+template <typename T>
+T* mal(size_t count) {
+    void* p = nullptr;
+    if (count == 0) count = 1;
+    MCHK(hipMalloc(&p, count * sizeof(T)));
+    return static_cast<T*>(p);
+}
+
+int64_t cdiv64(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+struct pp_mlp {
+    int E = 0, din = 0, H = 0, L = 0;  // L hidden layers
+    hipStream_t stream = nullptr;
+    // FP32 master layout per member (padded): W_1[H][kKin], b_1[H],
+    // then (W_l[H][H], b_l[H]) for l = 2..L, then w_o[H], b_o[1]
+    std::vector<int64_t> offW, offb;  // per hidden layer
+    int64_t offwo = 0, offbo = 0, P = 0;  // P: floats per member
+    float* p = nullptr;
+    float* g = nullptr;
+    float* m = nullptr;
+    float* v = nullptr;
+    int64_t step = 0;
+    std::vector<uint16_t*> Wb;  // bf16 copies per hidden layer: [E][H][K_l]
+    // data
+    uint16_t* X = nullptr;
+    float* Y = nullptr;
+    int64_t n = 0;
+    int32_t* perm = nullptr;
+    // minibatch workspace (capacity cap_b rows)
+    int cap_b = 0;
+    uint16_t* Xb = nullptr;
+    float* Yb = nullptr;
+    std::vector<float*> Z;       // [E][B][H] per layer
+    std::vector<uint16_t*> A;    // [E][B][H] per layer
+    uint16_t* dZa = nullptr;     // [E][B][H] ping
+    uint16_t* dZb = nullptr;     // [E][B][H] pong
+    float* y = nullptr;
+    float* dy = nullptr;
+    float* loss = nullptr;
+    // inference workspace
+    int64_t cap_q = 0;
+    uint16_t* Xq = nullptr;
+    uint16_t* Qa = nullptr;
+    uint16_t* Qb = nullptr;
+    float* yq = nullptr;
+    float* xq_f = nullptr;
+    double* mq = nullptr;
+    double* sq = nullptr;
+
+    int K(int l) const { return l == 0 ? kKin : H; }
+
+    ~pp_mlp() {
+        auto F = [](void* q) {
+            if (q) (void)hipFree(q);
+        };
+        F(p); F(g); F(m); F(v);
+        for (auto q : Wb) F(q);
+        F(X); F(Y); F(perm); F(Xb); F(Yb);
+        for (auto q : Z) F(q);
+        for (auto q : A) F(q);
+        F(dZa); F(dZb); F(y); F(dy); F(loss);
+        F(Xq); F(Qa); F(Qb); F(yq); F(xq_f); F(mq); F(sq);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void init(int E_, int din_, int H_, int L_) {
+        E = E_; din = din_; H = H_; L = L_;
+        MCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        int64_t off = 0;
+        for (int l = 0; l < L; ++l) {
+            offW.push_back(off);
+            off += (int64_t)H * K(l);
+            offb.push_back(off);
+            off += H;
+        }
+        offwo = off;
+        off += H;
+        offbo = off;
+        off += 1;
+        P = (off + 63) / 64 * 64;
+        p = mal<float>((size_t)E * P);
+        g = mal<float>((size_t)E * P);
+        m = mal<float>((size_t)E * P);
+        v = mal<float>((size_t)E * P);
+        MCHK(hipMemsetAsync(p, 0, sizeof(float) * E * P, stream));
+        MCHK(hipMemsetAsync(m, 0, sizeof(float) * E * P, stream));
+        MCHK(hipMemsetAsync(v, 0, sizeof(float) * E * P, stream));
+        for (int l = 0; l < L; ++l) Wb.push_back(mal<uint16_t>((size_t)E * H * K(l)));
+        loss = mal<float>(E);
+        MCHK(hipStreamSynchronize(stream));
+    }
+
+    void ensure_batch(int Bsz) {
+        if (cap_b >= Bsz) return;
+        auto F = [](void* q) {
+            if (q) (void)hipFree(q);
+        };
+        F(Xb); F(Yb); F(dZa); F(dZb); F(y); F(dy);
+        for (auto q : Z) F(q);
+        for (auto q : A) F(q);
+        Z.clear();
+        A.clear();
+        cap_b = Bsz;
+        const size_t act = (size_t)E * Bsz * H;
+        Xb = mal<uint16_t>((size_t)E * Bsz * kKin);
+        Yb = mal<float>((size_t)E * Bsz);
+        for (int l = 0; l < L; ++l) {
+            Z.push_back(mal<float>(act));
+            A.push_back(mal<uint16_t>(act));
+        }
+        dZa = mal<uint16_t>(act);
+        dZb = mal<uint16_t>(act);
+        y = mal<float>((size_t)E * Bsz);
+        dy = mal<float>((size_t)E * Bsz);
+    }
+
+    void cast_weights() {
+        for (int l = 0; l < L; ++l) {
+            const int64_t cnt = (int64_t)H * K(l);
+            hipLaunchKernelGGL(kg_cast, dim3((unsigned)cdiv64(cnt, 256), E), dim3(256), 0, stream, p + offW[l],
+                               (int64_t)K(l), P, Wb[l], (int64_t)K(l), (int64_t)H * K(l), H, K(l));
+        }
+        MCHK(hipGetLastError());
+    }
+
+    template <int EPI>
+    void gemm(const uint16_t* Aop, int64_t sam, int64_t sak, int64_t sae, const uint16_t* Bop, int64_t sbn, int64_t sbk,
+              int64_t sbe, int M, int N, int Kd, float* C, int64_t ldc, int64_t sce, uint16_t* Hout, int64_t ldh,
+              int64_t she, const float* bias, int64_t sbias, const float* Zp, int64_t ldz, int64_t sze) {
+        if (M % kBM || N % kBN || Kd % kBK) throw MlpError{PP_EARG, "GEMM shape not a multiple of the tile"};
+        if ((sak != 1 && sam != 1) || (sbk != 1 && sbn != 1)) throw MlpError{PP_EARG, "GEMM operand not contiguous"};
+        GemmArgs a{Aop, sam, sak, sae, Bop, sbn, sbk, sbe, M, N, Kd, C, ldc, sce, Hout, ldh, she, bias, sbias, Zp, ldz, sze};
+        hipLaunchKernelGGL(kg_gemm<EPI>, dim3(N / kBN, M / kBM, E), dim3(256), 0, stream, a);
+        MCHK(hipGetLastError());
+    }
+
+    // Forward over a minibatch already gathered in Xb (train: keep Z).
+    void forward_train(int Bsz) {
+        const uint16_t* in = Xb;
+        int64_t ld_in = kKin;
+        for (int l = 0; l < L; ++l) {
+            gemm<1>(in, ld_in, 1, (int64_t)Bsz * ld_in, Wb[l], K(l), 1, (int64_t)H * K(l), Bsz, H, K(l), Z[l], H,
+                    (int64_t)Bsz * H, A[l], H, (int64_t)Bsz * H, p + offb[l], P, nullptr, 0, 0);
+            in = A[l];
+            ld_in = H;
+        }
+        hipLaunchKernelGGL(kg_out_fwd, dim3((unsigned)cdiv64(Bsz, 4), E), dim3(256), 0, stream, A[L - 1], H, Bsz,
+                           p + offwo, P, p + offbo, P, y);
+        MCHK(hipGetLastError());
+    }
+
+    // Loss, gradients into g (zeroed first), for the minibatch in Xb / Yb.
+    void backward(int Bsz) {
+        MCHK(hipMemsetAsync(g, 0, sizeof(float) * E * P, stream));
+        MCHK(hipMemsetAsync(loss, 0, sizeof(float) * E, stream));
+        hipLaunchKernelGGL(kg_loss, dim3((unsigned)cdiv64(Bsz, 256), E), dim3(256), 0, stream, y, Yb, Bsz, dy, loss);
+        // output layer: dw_o = sum_b dy A_L, db_o = sum_b dy
+        const int rows = 64;
+        hipLaunchKernelGGL(kg_colsum, dim3((unsigned)cdiv64(H, 256), (unsigned)cdiv64(Bsz, rows), E), dim3(256), 0,
+                           stream, A[L - 1], dy, H, Bsz, rows, g + offwo, P);
+        hipLaunchKernelGGL(kg_sum, dim3((unsigned)cdiv64(Bsz, 256), E), dim3(256), 0, stream, dy, Bsz, g + offbo, P);
+        const int64_t act = (int64_t)Bsz * H;
+        hipLaunchKernelGGL(kg_dz_out, dim3((unsigned)cdiv64(act, 256), E), dim3(256), 0, stream, dy, p + offwo, P,
+                           Z[L - 1], H, Bsz, dZa);
+        MCHK(hipGetLastError());
+        uint16_t* dz = dZa;
+        uint16_t* dzn = dZb;
+        for (int l = L - 1; l >= 0; --l) {
+            const uint16_t* in = l == 0 ? Xb : A[l - 1];
+            const int64_t ld_in = l == 0 ? kKin : H;
+            // dW_l[o][i] = sum_b dZ[b][o] in[b][i]: M = H (o), N = K_l (i), K = B
+            gemm<0>(dz, 1, H, act, in, 1, ld_in, (int64_t)Bsz * ld_in, H, K(l), Bsz, g + offW[l], K(l), P, nullptr, 0,
+                    0, nullptr, 0, nullptr, 0, 0);
+            hipLaunchKernelGGL(kg_colsum, dim3((unsigned)cdiv64(H, 256), (unsigned)cdiv64(Bsz, rows), E), dim3(256), 0,
+                               stream, dz, (const float*)nullptr, H, Bsz, rows, g + offb[l], P);
+            MCHK(hipGetLastError());
+            if (l > 0) {
+                // dZ_{l-1}[b][i] = (sum_o dZ[b][o] W_l[o][i]) gelu'(Z_{l-1}[b][i]): M = B, N = H (i), K = H (o)
+                gemm<2>(dz, H, 1, act, Wb[l], 1, K(l), (int64_t)H * K(l), Bsz, H, H, nullptr, 0, 0, dzn, H, act,
+                        nullptr, 0, Z[l - 1], H, act);
+                std::swap(dz, dzn);
+            }
+        }
+    }
+
+    void adam(float lr) {
+        step += 1;
+        const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+        const float c1 = 1.0f - (float)std::pow((double)b1, (double)step);
+        const float c2 = 1.0f - (float)std::pow((double)b2, (double)step);
+        const int64_t cnt = (int64_t)E * P;
+        hipLaunchKernelGGL(kg_adam, dim3((unsigned)cdiv64(cnt, 256)), dim3(256), 0, stream, p, g, m, v, cnt, lr, b1, b2,
+                           eps, c1, c2);
+        MCHK(hipGetLastError());
+        cast_weights();
+    }
+
+    void ensure_query(int64_t cq) {
+        if (cap_q >= cq) return;
+        auto F = [](void* q) {
+            if (q) (void)hipFree(q);
+        };
+        F(Xq); F(Qa); F(Qb); F(yq); F(xq_f); F(mq); F(sq);
+        cap_q = cq;
+        Xq = mal<uint16_t>((size_t)cq * kKin);
+        Qa = mal<uint16_t>((size_t)E * cq * H);
+        Qb = mal<uint16_t>((size_t)E * cq * H);
+        yq = mal<float>((size_t)E * cq);
+        xq_f = mal<float>((size_t)cq * din);
+        mq = mal<double>((size_t)cq);
+        sq = mal<double>((size_t)cq);
+    }
+
+    // Forward of `cnt` rows (multiple of 64) of Xq into yq[E][cnt].
+    void forward_query(int64_t cnt) {
+        const uint16_t* in = Xq;
+        int64_t ld_in = kKin, se_in = 0;  // the input is shared by every member
+        uint16_t* outs[2] = {Qa, Qb};
+        for (int l = 0; l < L; ++l) {
+            uint16_t* o = outs[l & 1];
+            gemm<3>(in, ld_in, 1, se_in, Wb[l], K(l), 1, (int64_t)H * K(l), (int)cnt, H, K(l), nullptr, 0, 0, o, H,
+                    cnt * H, p + offb[l], P, nullptr, 0, 0);
+            in = o;
+            ld_in = H;
+            se_in = cnt * H;
+        }
+        hipLaunchKernelGGL(kg_out_fwd, dim3((unsigned)cdiv64(cnt, 4), E), dim3(256), 0, stream, in, H, (int)cnt,
+                           p + offwo, P, p + offbo, P, yq);
+        MCHK(hipGetLastError());
+    }
+};
+
+namespace {
+
+int mfail(int code, const std::string& msg) {
+    pp::set_last_error(msg);
+    return code;
+}
+
+}  // namespace
+
+#define MLP_BEGIN try {
+#define MLP_END                                 \
+    }                                           \
+    catch (const MlpError& e) {                 \
+        return mfail(e.code, e.msg);            \
+    }                                           \
+    catch (const std::exception& e) {           \
+        return mfail(PP_EHIP, e.what());        \
+    }                                           \
+    pp::set_last_error("");                     \
+    return PP_OK;
+
+extern "C" {
+
+int pp_mlp_create(int E, int din, int hidden, int layers, pp_mlp** out) {
+    if (!out) return mfail(PP_EARG, "out is NULL");
+    *out = nullptr;
+    if (E < 1 || E > 64) return mfail(PP_EARG, "E must be in 1:64");
+    if (din < 1 || din > kKin) return mfail(PP_EARG, "din must be in 1:64");
+    if (hidden < 64 || hidden % 64) return mfail(PP_EARG, "hidden must be a positive multiple of 64");
+    if (layers < 1 || layers > 16) return mfail(PP_EARG, "layers must be in 1:16");
+    pp_mlp* h = new pp_mlp();
+    try {
+        h->init(E, din, hidden, layers);
+    } catch (const MlpError& e) {
+        delete h;
+        return mfail(e.code, e.msg);
+    }
+    *out = h;
+    return PP_OK;
+}
+
+int pp_mlp_destroy(pp_mlp* h) {
+    if (!h) return PP_OK;
+    (void)hipStreamSynchronize(h->stream);
+    delete h;
+    return PP_OK;
+}
+
+int pp_mlp_param_count(pp_mlp* h, int64_t* count) {
+    if (!h || !count) return mfail(PP_EARG, "null argument");
+    // host layout per member: W_1[H][din], b_1, (W_l[H][H], b_l)..., w_o[H], b_o
+    int64_t c = (int64_t)h->H * h->din + h->H;
+    for (int l = 1; l < h->L; ++l) c += (int64_t)h->H * h->H + h->H;
+    *count = c + h->H + 1;
+    return PP_OK;
+}
+
+static void host_layout(pp_mlp* h, const float* src, std::vector<float>& dst) {
+    // host layout -> padded device layout of one member
+    dst.assign((size_t)h->P, 0.0f);
+    int64_t q = 0;
+    for (int l = 0; l < h->L; ++l) {
+        const int kin = l == 0 ? h->din : h->H;
+        for (int r = 0; r < h->H; ++r)
+            for (int c = 0; c < kin; ++c) dst[h->offW[l] + (int64_t)r * h->K(l) + c] = src[q++];
+        for (int r = 0; r < h->H; ++r) dst[h->offb[l] + r] = src[q++];
+    }
+    for (int r = 0; r < h->H; ++r) dst[h->offwo + r] = src[q++];
+    dst[h->offbo] = src[q++];
+}
+
+static void device_layout(pp_mlp* h, const float* src, float* dst) {
+    int64_t q = 0;
+    for (int l = 0; l < h->L; ++l) {
+        const int kin = l == 0 ? h->din : h->H;
+        for (int r = 0; r < h->H; ++r)
+            for (int c = 0; c < kin; ++c) dst[q++] = src[h->offW[l] + (int64_t)r * h->K(l) + c];
+        for (int r = 0; r < h->H; ++r) dst[q++] = src[h->offb[l] + r];
+    }
+    for (int r = 0; r < h->H; ++r) dst[q++] = src[h->offwo + r];
+    dst[q++] = src[h->offbo];
+}
+
+int pp_mlp_set_params(pp_mlp* h, const float* params) {
+    if (!h || !params) return mfail(PP_EARG, "null argument");
+    MLP_BEGIN
+    int64_t cnt = 0;
+    pp_mlp_param_count(h, &cnt);
+    std::vector<float> all((size_t)h->E * h->P), one;
+    for (int e = 0; e < h->E; ++e) {
+        host_layout(h, params + (size_t)e * cnt, one);
+        std::copy(one.begin(), one.end(), all.begin() + (size_t)e * h->P);
+    }
+    MCHK(hipMemcpyAsync(h->p, all.data(), all.size() * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    MCHK(hipMemsetAsync(h->m, 0, sizeof(float) * h->E * h->P, h->stream));
+    MCHK(hipMemsetAsync(h->v, 0, sizeof(float) * h->E * h->P, h->stream));
+    h->step = 0;
+    h->cast_weights();
+    MCHK(hipStreamSynchronize(h->stream));
+    MLP_END
+}
+
+int pp_mlp_get_params(pp_mlp* h, float* params) {
+    if (!h || !params) return mfail(PP_EARG, "null argument");
+    MLP_BEGIN
+    int64_t cnt = 0;
+    pp_mlp_param_count(h, &cnt);
+    std::vector<float> all((size_t)h->E * h->P);
+    MCHK(hipMemcpyAsync(all.data(), h->p, all.size() * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    MCHK(hipStreamSynchronize(h->stream));
+    for (int e = 0; e < h->E; ++e) device_layout(h, all.data() + (size_t)e * h->P, params + (size_t)e * cnt);
+    MLP_END
+}
+
+int pp_mlp_load_data(pp_mlp* h, const float* x, const float* y, int64_t n) {
+    if (!h || !x || !y || n < 1) return mfail(PP_EARG, "bad arguments");
+    MLP_BEGIN
+    if (h->X) (void)hipFree(h->X);
+    if (h->Y) (void)hipFree(h->Y);
+    if (h->perm) (void)hipFree(h->perm);
+    h->X = mal<uint16_t>((size_t)n * kKin);
+    h->Y = mal<float>((size_t)n);
+    h->perm = mal<int32_t>((size_t)h->E * n);
+    h->n = n;
+    float* xf = mal<float>((size_t)n * h->din);
+    MCHK(hipMemcpyAsync(xf, x, sizeof(float) * n * h->din, hipMemcpyHostToDevice, h->stream));
+    MCHK(hipMemcpyAsync(h->Y, y, sizeof(float) * n, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(kg_pack_input, dim3((unsigned)cdiv64(n * kKin, 256)), dim3(256), 0, h->stream, xf, n, h->din,
+                       h->X);
+    MCHK(hipGetLastError());
+    MCHK(hipStreamSynchronize(h->stream));
+    (void)hipFree(xf);
+    MLP_END
+}
+
+// One epoch: floor(n / batch) Adam steps; member e visits rows in the order
+// perm[e*n .. e*n + n).  loss[e] = mean minibatch loss over the epoch.
+int pp_mlp_train_epoch(pp_mlp* h, const int32_t* perm, int batch, float lr, float* loss_out) {
+    if (!h || !perm || !loss_out) return mfail(PP_EARG, "null argument");
+    if (batch < 64 || batch % 64) return mfail(PP_EARG, "batch must be a positive multiple of 64");
+    if (!h->X) return mfail(PP_EARG, "no training data loaded");
+    if (batch > h->n) return mfail(PP_EARG, "batch larger than the data set");
+    MLP_BEGIN
+    for (int64_t i = 0; i < (int64_t)h->E * h->n; ++i)
+        if (perm[i] < 0 || perm[i] >= h->n) throw MlpError{PP_EARG, "perm entry out of range"};
+    MCHK(hipMemcpyAsync(h->perm, perm, sizeof(int32_t) * h->E * h->n, hipMemcpyHostToDevice, h->stream));
+    h->ensure_batch(batch);
+    const int64_t steps = h->n / batch;
+    std::vector<double> acc((size_t)h->E, 0.0);
+    std::vector<float> l((size_t)h->E);
+    for (int64_t s = 0; s < steps; ++s) {
+        hipLaunchKernelGGL(kg_gather, dim3((unsigned)cdiv64(batch, 4), h->E), dim3(256), 0, h->stream, h->X, h->Y,
+                           h->perm, h->n, s * batch, batch, h->Xb, h->Yb);
+        h->forward_train(batch);
+        h->backward(batch);
+        h->adam(lr);
+        MCHK(hipMemcpyAsync(l.data(), h->loss, sizeof(float) * h->E, hipMemcpyDeviceToHost, h->stream));
+        MCHK(hipStreamSynchronize(h->stream));
+        for (int e = 0; e < h->E; ++e) acc[e] += l[e];
+    }
+    for (int e = 0; e < h->E; ++e) loss_out[e] = (float)(acc[e] / (double)std::max<int64_t>(steps, 1));
+    MLP_END
+}
+
+// Gradients of the mean squared error of every member on one batch (x[b][din],
+// t[b], shared by all members), in the host parameter layout: grads[e][count].
+int pp_mlp_gradients(pp_mlp* h, const float* x, const float* t, int batch, float* grads, float* loss_out) {
+    if (!h || !x || !t || !grads) return mfail(PP_EARG, "null argument");
+    if (batch < 64 || batch % 64) return mfail(PP_EARG, "batch must be a positive multiple of 64");
+    MLP_BEGIN
+    h->ensure_batch(batch);
+    h->ensure_query(batch);
+    MCHK(hipMemcpyAsync(h->xq_f, x, sizeof(float) * batch * h->din, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(kg_pack_input, dim3((unsigned)cdiv64((int64_t)batch * kKin, 256)), dim3(256), 0, h->stream,
+                       h->xq_f, (int64_t)batch, h->din, h->Xq);
+    for (int e = 0; e < h->E; ++e) {
+        MCHK(hipMemcpyAsync(h->Xb + (size_t)e * batch * kKin, h->Xq, sizeof(uint16_t) * batch * kKin,
+                            hipMemcpyDeviceToDevice, h->stream));
+        MCHK(hipMemcpyAsync(h->Yb + (size_t)e * batch, t, sizeof(float) * batch, hipMemcpyHostToDevice, h->stream));
+    }
+    h->forward_train(batch);
+    h->backward(batch);
+    std::vector<float> all((size_t)h->E * h->P);
+    MCHK(hipMemcpyAsync(all.data(), h->g, all.size() * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    std::vector<float> l((size_t)h->E);
+    MCHK(hipMemcpyAsync(l.data(), h->loss, sizeof(float) * h->E, hipMemcpyDeviceToHost, h->stream));
+    MCHK(hipStreamSynchronize(h->stream));
+    int64_t cnt = 0;
+    pp_mlp_param_count(h, &cnt);
+    for (int e = 0; e < h->E; ++e) device_layout(h, all.data() + (size_t)e * h->P, grads + (size_t)e * cnt);
+    if (loss_out)
+        for (int e = 0; e < h->E; ++e) loss_out[e] = l[e];
+    MLP_END
+}
+
+// Ensemble prediction for n query rows x[n][din]: per-member outputs
+// (members[e*n + i], may be NULL), mean and population std over members.
+// seconds (may be NULL): device time of the forward passes (HIP events).
+int pp_mlp_predict(pp_mlp* h, const float* x, int64_t n, float* members, double* mean, double* sd, double* seconds) {
+    if (!h || !x || !mean || !sd || n < 0) return mfail(PP_EARG, "bad arguments");
+    if (n == 0) return PP_OK;
+    MLP_BEGIN
+    const int64_t chunk = std::min<int64_t>(65536, (n + 63) / 64 * 64);
+    h->ensure_query(chunk);
+    hipEvent_t e0, e1;
+    MCHK(hipEventCreate(&e0));
+    MCHK(hipEventCreate(&e1));
+    double t = 0.0;
+    std::vector<float> yh;
+    for (int64_t r0 = 0; r0 < n; r0 += chunk) {
+        const int64_t cnt = std::min(chunk, n - r0);
+        const int64_t cpad = (cnt + 63) / 64 * 64;
+        MCHK(hipMemsetAsync(h->xq_f, 0, sizeof(float) * cpad * h->din, h->stream));
+        MCHK(hipMemcpyAsync(h->xq_f, x + r0 * h->din, sizeof(float) * cnt * h->din, hipMemcpyHostToDevice, h->stream));
+        MCHK(hipEventRecord(e0, h->stream));
+        hipLaunchKernelGGL(kg_pack_input, dim3((unsigned)cdiv64(cpad * kKin, 256)), dim3(256), 0, h->stream, h->xq_f,
+                           cpad, h->din, h->Xq);
+        h->forward_query(cpad);
+        hipLaunchKernelGGL(kg_ensemble, dim3((unsigned)cdiv64(cpad, 256)), dim3(256), 0, h->stream, h->yq, h->E, cpad,
+                           cpad, h->mq, h->sq);
+        MCHK(hipGetLastError());
+        MCHK(hipEventRecord(e1, h->stream));
+        MCHK(hipMemcpyAsync(mean + r0, h->mq, sizeof(double) * cnt, hipMemcpyDeviceToHost, h->stream));
+        MCHK(hipMemcpyAsync(sd + r0, h->sq, sizeof(double) * cnt, hipMemcpyDeviceToHost, h->stream));
+        if (members) {
+            yh.resize((size_t)h->E * cpad);
+            MCHK(hipMemcpyAsync(yh.data(), h->yq, sizeof(float) * h->E * cpad, hipMemcpyDeviceToHost, h->stream));
+        }
+        MCHK(hipStreamSynchronize(h->stream));
+        float ms = 0.f;
+        MCHK(hipEventElapsedTime(&ms, e0, e1));
+        t += ms * 1e-3;
+        if (members)
+            for (int e = 0; e < h->E; ++e)
+                std::memcpy(members + (size_t)e * n + r0, yh.data() + (size_t)e * cpad, sizeof(float) * cnt);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (seconds) *seconds = t;
+    MLP_END
+}
+
+}  // extern "C"

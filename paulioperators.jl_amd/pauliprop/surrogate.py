@@ -1,0 +1,264 @@
+"""Learned surrogate of ⟨O⟩(θ) with an ensemble-disagreement gate.
+
+The exact method is the propagation engine itself: ⟨0|O(θ)|0⟩ of the
+36-qubit chain (10 layers, per-layer shared RZZ and RX angles, θ ∈ [-π, π]^20,
+observable Z_18, |c| <= 1e-6) computed by SparsePauliBatch on the GPU.  The
+surrogate is an ensemble of E MLPs (20 -> 512 -> 512 -> 512 -> 1, GELU)
+trained with Adam on labels the engine produced; at query time a θ whose
+ensemble standard deviation exceeds ``tol`` is sent back to exact
+propagation.  Training and inference run in libpauliprop's BF16
+matrix-core kernels (pp_mlp_*).  No part of this has a counterpart in the
+reference.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import time
+from typing import Callable, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import ArgumentError, check
+from . import workloads as wl
+
+DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+LABELS_PATH = os.path.join(DATA_DIR, "chain36_labels.npz")
+N_TRAIN, N_HELDOUT = 65536, 4096
+
+_fp = ctypes.POINTER(ctypes.c_float)
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def features(thetas) -> np.ndarray:
+    """Network input: θ / π, float32 [n][20]."""
+    return (np.asarray(thetas, dtype=np.float64) / math.pi).astype(np.float32)
+
+
+class MLPEnsemble:
+    """E independent MLPs din -> hidden^layers -> 1 on the device."""
+
+    def __init__(self, E: int = 5, din: int = 20, hidden: int = 512, layers: int = 3, seed: int = 0):
+        self.E, self.din, self.hidden, self.layers = E, din, hidden, layers
+        h = ctypes.c_void_p()
+        check(_lib.lib().pp_mlp_create(E, din, hidden, layers, ctypes.byref(h)))
+        self._h = h
+        c = ctypes.c_int64()
+        check(_lib.lib().pp_mlp_param_count(self._h, ctypes.byref(c)))
+        self.count = c.value
+        self.set_params(self.init_params(seed))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.lib().pp_mlp_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def init_params(self, seed: int) -> np.ndarray:
+        """Uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)) weights and biases per layer,
+        one independent draw per member."""
+        rng = np.random.default_rng(seed)
+        out = np.empty((self.E, self.count), dtype=np.float32)
+        H = self.hidden
+        for e in range(self.E):
+            parts = []
+            for l in range(self.layers):
+                kin = self.din if l == 0 else H
+                bound = 1.0 / math.sqrt(kin)
+                parts.append(rng.uniform(-bound, bound, size=H * kin))
+                parts.append(rng.uniform(-bound, bound, size=H))
+            bound = 1.0 / math.sqrt(H)
+            parts.append(rng.uniform(-bound, bound, size=H))
+            parts.append(rng.uniform(-bound, bound, size=1))
+            out[e] = np.concatenate(parts)
+        return out
+
+    def set_params(self, params: np.ndarray):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.shape != (self.E, self.count):
+            raise ArgumentError(f"params must have shape {(self.E, self.count)}")
+        check(_lib.lib().pp_mlp_set_params(self._h, p.ctypes.data_as(_fp)))
+
+    def get_params(self) -> np.ndarray:
+        p = np.empty((self.E, self.count), dtype=np.float32)
+        check(_lib.lib().pp_mlp_get_params(self._h, p.ctypes.data_as(_fp)))
+        return p
+
+    def gradients(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.empty((self.E, self.count), dtype=np.float32)
+        loss = np.empty(self.E, dtype=np.float32)
+        check(_lib.lib().pp_mlp_gradients(self._h, x.ctypes.data_as(_fp), t.ctypes.data_as(_fp), len(t),
+                                          g.ctypes.data_as(_fp), loss.ctypes.data_as(_fp)))
+        return g, loss
+
+    def fit(self, x, y, epochs: int = 60, batch: int = 1024, lr: float = 1e-3, lr_min: float = 1e-5,
+            seed: int = 1, log: Optional[Callable[[str], None]] = None):
+        """Adam with a cosine learning-rate decay from lr to lr_min; each member
+        sees its own shuffle of the data every epoch.  Returns the per-epoch
+        mean losses [epochs][E] and the wall time."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        n = len(y)
+        check(_lib.lib().pp_mlp_load_data(self._h, x.ctypes.data_as(_fp), y.ctypes.data_as(_fp), n))
+        rng = np.random.default_rng(seed)
+        hist = []
+        t0 = time.perf_counter()
+        for ep in range(epochs):
+            perm = np.stack([rng.permutation(n) for _ in range(self.E)]).astype(np.int32)
+            rate = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * ep / max(epochs - 1, 1)))
+            loss = np.empty(self.E, dtype=np.float32)
+            check(_lib.lib().pp_mlp_train_epoch(self._h, perm.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                                batch, ctypes.c_float(rate), loss.ctypes.data_as(_fp)))
+            hist.append(loss.copy())
+            if log and (ep % 10 == 0 or ep == epochs - 1):
+                log(f"epoch {ep}: lr {rate:.2e} loss {np.array2string(loss, precision=3)}")
+        return np.array(hist), time.perf_counter() - t0
+
+    def predict(self, x, members: bool = False):
+        """(mean, std, members or None, device seconds) for x[n][din]."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        n = len(x)
+        mean = np.empty(n, dtype=np.float64)
+        sd = np.empty(n, dtype=np.float64)
+        mem = np.empty((self.E, n), dtype=np.float32) if members else None
+        secs = ctypes.c_double()
+        check(_lib.lib().pp_mlp_predict(self._h, x.ctypes.data_as(_fp), n,
+                                        mem.ctypes.data_as(_fp) if members else None,
+                                        mean.ctypes.data_as(_dp), sd.ctypes.data_as(_dp), ctypes.byref(secs)))
+        return mean, sd, mem, secs.value
+
+
+# --- exact method: the propagation engine ----------------------------------------
+
+def _chain36_layer_arrays(thetas, li: int, W: int):
+    """Packed steps of layer li for every θ (the circuits share generators;
+    RZZ angle θ[2 li] on the 35 bonds, RX angle θ[2 li + 1] on the 36 sites)."""
+    from .spv import _pack
+    N = 36
+    gens = [(p.z, p.x) for p in wl.surrogate_chain36_layers([0.0] * 20).layers[li].generators]
+    B = len(thetas)
+    S = len(gens)
+    gz = np.zeros((S, B, W), dtype=np.uint64)
+    gx = np.zeros((S, B, W), dtype=np.uint64)
+    for s, (z, x) in enumerate(gens):
+        gz[s, :] = _pack(z, W)
+        gx[s, :] = _pack(x, W)
+    nb = N - 1
+    cz = np.array([math.cos(t[2 * li]) for t in thetas])
+    sz = np.array([math.sin(t[2 * li]) for t in thetas])
+    cx = np.array([math.cos(t[2 * li + 1]) for t in thetas])
+    sx = np.array([math.sin(t[2 * li + 1]) for t in thetas])
+    cv = np.empty((S, B))
+    sv = np.empty((S, B))
+    cv[:nb], sv[:nb] = cz, sz
+    cv[nb:], sv[nb:] = cx, sx
+    return gz, gx, cv, sv, np.ones((S, B), dtype=np.uint8)
+
+
+def chain36_batch_expectations(thetas: Sequence[Sequence[float]], batch: int = 512,
+                               log: Optional[Callable[[str], None]] = None) -> np.ndarray:
+    """⟨0|O(θ)|0⟩ for every θ by batched GPU propagation (SparsePauliBatch):
+    `batch` circuits per batch, one evolve! per layer per circuit."""
+    from .batch import SparsePauliBatch
+    from .truncation import CoeffTruncation
+    out = np.empty(len(thetas), dtype=np.float64)
+    obs = wl.surrogate_chain36_layers([0.0] * 20).observable
+    for b0 in range(0, len(thetas), batch):
+        th = [list(t) for t in thetas[b0:b0 + batch]]
+        bt = SparsePauliBatch([36] * len(th), [[((p.z, p.x), c) for p, c in obs]] * len(th))
+        for li in range(10):
+            bt.evolve_packed(_chain36_layer_arrays(th, li, bt.W), truncation=CoeffTruncation(1e-6))
+        out[b0:b0 + len(th)] = bt.expectation_values()
+        if log:
+            log(f"labels {b0 + len(th)}/{len(thetas)}")
+    return out
+
+
+def chain36_plain_expectation(theta: Sequence[float]) -> float:
+    """One θ at a time: a SparsePauliVector and one evolve! per layer."""
+    from .spv import SparsePauliVector
+    w = wl.surrogate_chain36_layers(list(theta))
+    v = SparsePauliVector(w.N, [((p.z, p.x), c) for p, c in w.observable])
+    for L in w.layers:
+        v.evolve_(L.generators, L.angles, truncation=w.truncation)
+    return v.expectation_value(0)
+
+
+class GatedSurrogate:
+    """Ensemble mean where the ensemble agrees (std <= tol), exact propagation
+    elsewhere."""
+
+    def __init__(self, ensemble: MLPEnsemble, exact: Callable = chain36_batch_expectations, tol: float = 1e-3):
+        self.ens, self.exact, self.tol = ensemble, exact, tol
+
+    def evaluate(self, thetas):
+        thetas = np.asarray(thetas, dtype=np.float64)
+        t0 = time.perf_counter()
+        mean, sd, _, dev_s = self.ens.predict(features(thetas))
+        t1 = time.perf_counter()
+        fall = sd > self.tol
+        vals = mean.copy()
+        if fall.any():
+            vals[fall] = self.exact([list(t) for t in thetas[fall]])
+        t2 = time.perf_counter()
+        return vals, fall, {"surrogate_s": t1 - t0, "surrogate_device_s": dev_s, "exact_s": t2 - t1}
+
+
+# --- labels --------------------------------------------------------------------------
+
+def training_thetas():
+    """The 65,536 training and 4,096 held-out θ: the first 69,632 draws of the
+    seed-99 stream, in that order."""
+    th = np.array(wl.surrogate_thetas(N_TRAIN + N_HELDOUT, seed=99), dtype=np.float64)
+    return th[:N_TRAIN], th[N_TRAIN:]
+
+
+def query_thetas(count: int = 1_000_000, seed: int = 100) -> np.ndarray:
+    """Query θ for inference throughput: uniform [-π, π]^20 (numpy, seed 100)."""
+    return np.random.default_rng(seed).uniform(-math.pi, math.pi, size=(count, 20))
+
+
+def generate_labels(path: str = LABELS_PATH, start: int = 0, stop: Optional[int] = None, batch: int = 512,
+                    budget_s: float = float("inf"), log: Optional[Callable[[str], None]] = None):
+    """Label θ rows [start, stop) of training + held-out θ by GPU propagation,
+    resuming from the .npz at `path` (key `labels`, NaN where not computed)
+    and saving it after every batch; stops after `budget_s` seconds."""
+    tr, ho = training_thetas()
+    allth = np.concatenate([tr, ho])
+    stop = len(allth) if stop is None else stop
+    labels = np.full(len(allth), np.nan)
+    if os.path.exists(path):
+        labels = np.load(path)["labels"]
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    t0 = time.perf_counter()
+    todo = [i for i in range(start, stop) if np.isnan(labels[i])]
+    for b0 in range(0, len(todo), batch):
+        idx = todo[b0:b0 + batch]
+        t1 = time.perf_counter()
+        labels[idx] = chain36_batch_expectations(allth[idx], batch=batch)
+        np.savez_compressed(path, labels=labels)
+        if log:
+            log(f"labels: {b0 + len(idx)}/{len(todo)} done, batch {time.perf_counter() - t1:.2f} s")
+        if time.perf_counter() - t0 > budget_s:
+            break
+    return labels
+
+
+if __name__ == "__main__":
+    import argparse
+    import sys
+    ap = argparse.ArgumentParser(description="generate chain36 surrogate labels by GPU propagation")
+    ap.add_argument("--start", type=int, default=0)
+    ap.add_argument("--stop", type=int, default=None)
+    ap.add_argument("--batch", type=int, default=512)
+    ap.add_argument("--out", default=LABELS_PATH)
+    ap.add_argument("--budget", type=float, default=float("inf"), help="seconds")
+    a = ap.parse_args()
+    generate_labels(a.out, a.start, a.stop, a.batch, a.budget, log=lambda m: print(m, file=sys.stderr, flush=True))

@@ -1,0 +1,123 @@
+"""GPU tests of the learned surrogate: the BF16 matrix-core forward and
+backward kernels against the CPU oracle (oracle/mlp_oracle.py, device
+rounding mode), Adam training, the exact labels against the propagation
+oracle, and what the trained, gated ensemble gains over exact propagation."""
+import math
+import os
+import time
+
+import numpy as np
+import pytest
+
+import mlp_oracle as mo
+import pyoracle as po
+from pauliprop import compile_filter
+from pauliprop import surrogate as sg
+from pauliprop import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+@pytest.mark.parametrize("E,din,H,L,B", [(2, 20, 64, 3, 128), (5, 20, 512, 3, 256), (1, 7, 128, 1, 64)])
+def test_forward_and_gradients_match_oracle(E, din, H, L, B):
+    log(f"mlp parity E={E} din={din} H={H} L={L} B={B}")
+    ens = sg.MLPEnsemble(E=E, din=din, hidden=H, layers=L, seed=11)
+    orc = mo.MLPOracle(din, H, L)
+    assert ens.count == orc.count()
+    p = ens.get_params()
+    rng = np.random.default_rng(5)
+    x = rng.uniform(-1, 1, size=(B, din)).astype(np.float32)
+    t = rng.uniform(-1, 1, size=B).astype(np.float32)
+    mean, sd, mem, _ = ens.predict(x, members=True)
+    g, loss = ens.gradients(x, t)
+    for e in range(E):
+        y_ref, _ = orc.forward(p[e], x)
+        scale = max(1e-3, float(np.max(np.abs(y_ref))))
+        assert np.max(np.abs(mem[e] - y_ref)) <= 2e-3 * scale, e
+        g_ref, l_ref = orc.gradients(p[e], x, t)
+        assert abs(loss[e] - l_ref) <= 1e-3 * l_ref
+        # per parameter block: norm-wise relative difference (bf16 rounding
+        # ties can flip one ulp where fp32 and fp64 accumulations differ)
+        Wd, bd, wod, bod = orc.unpack(g[e])
+        Wr, br, wor, bor = orc.unpack(g_ref)
+        for l in range(L):
+            assert rel(Wd[l], Wr[l]) < 1e-2, ("W", e, l, rel(Wd[l], Wr[l]))
+            assert rel(bd[l], br[l]) < 1e-2, ("b", e, l)
+        assert rel(wod, wor) < 1e-3
+        assert abs(bod - bor) <= 1e-4 * max(abs(bor), 1e-6)
+    assert np.allclose(mean, mem.astype(np.float64).mean(axis=0), atol=1e-6)
+    assert np.allclose(sd, mem.astype(np.float64).std(axis=0), atol=1e-6)
+
+
+def test_set_get_params_roundtrip_and_padding():
+    ens = sg.MLPEnsemble(E=3, din=20, hidden=64, layers=2, seed=1)
+    p = np.random.default_rng(2).normal(size=(3, ens.count)).astype(np.float32)
+    ens.set_params(p)
+    assert np.array_equal(ens.get_params(), p)
+
+
+def test_training_fits_a_smooth_function():
+    rng = np.random.default_rng(7)
+    n = 8192
+    x = rng.uniform(-1, 1, size=(n, 20)).astype(np.float32)
+    y = (np.sin(2 * x[:, 0]) * np.cos(x[:, 1]) + 0.3 * x[:, 2] * x[:, 3]).astype(np.float32)
+    ens = sg.MLPEnsemble(E=2, din=20, hidden=256, layers=3, seed=3)
+    hist, secs = ens.fit(x, y, epochs=40, batch=512, lr=2e-3, lr_min=1e-4, log=log)
+    log(f"training 40 epochs: {secs:.2f} s, first {hist[0]}, last {hist[-1]}")
+    assert np.all(hist[-1] < 0.02 * hist[0])
+    xt = rng.uniform(-1, 1, size=(1024, 20)).astype(np.float32)
+    yt = np.sin(2 * xt[:, 0]) * np.cos(xt[:, 1]) + 0.3 * xt[:, 2] * xt[:, 3]
+    mean, sd, _, _ = ens.predict(xt)
+    assert np.sqrt(np.mean((mean - yt) ** 2)) < 0.05
+
+
+def test_labels_match_the_propagation_oracle():
+    """Batched GPU labels equal the CPU oracle's ⟨O⟩ on the same θ (1e-12)."""
+    th = np.array(wl.surrogate_thetas(4, seed=99))
+    got = sg.chain36_batch_expectations(th, batch=4)
+    for t, gval in zip(th, got):
+        w = wl.surrogate_chain36_layers(list(t))
+        o = po.OracleSPV(w.N, [(p.z, p.x, c) for p, c in w.observable])
+        f = compile_filter(w.truncation)
+        for L in w.layers:
+            o.evolve([(p.z, p.x) for p in L.generators], L.angles, strict=f)
+        assert abs(gval - o.expectation(0)) <= 1e-12
+    assert abs(sg.chain36_plain_expectation(th[3]) - got[3]) <= 1e-12
+
+
+@pytest.mark.skipif(not os.path.exists(sg.LABELS_PATH), reason="chain36 labels not generated")
+def test_gated_surrogate_gain_on_heldout():
+    """Train the 5-member ensemble on the GPU labels, then on the held-out
+    4,096 θ: every value the gate lets through is within 1e-3 of exact
+    propagation, and the gated evaluation is faster per ⟨O⟩ than exact
+    propagation of the same θ."""
+    lab = np.load(sg.LABELS_PATH)["labels"]
+    tr, ho = sg.training_thetas()
+    ytr, yho = lab[:len(tr)], lab[len(tr):]
+    assert not np.isnan(lab).any()
+    ens = sg.MLPEnsemble(E=5, din=20, hidden=512, layers=3, seed=0)
+    hist, secs = ens.fit(sg.features(tr), ytr, epochs=30, batch=1024, lr=1e-3, lr_min=1e-5, log=log)
+    log(f"trained in {secs:.1f} s; final losses {hist[-1]}")
+    gate = sg.GatedSurrogate(ens, exact=lambda th: sg.chain36_batch_expectations(th, batch=512), tol=1e-3)
+    sub = ho[:1024]
+    t0 = time.perf_counter()
+    vals, fall, tm = gate.evaluate(sub)
+    t_gate = time.perf_counter() - t0
+    err = np.abs(vals - yho[:1024])
+    served = ~fall
+    log(f"gate: served {served.sum()}/1024 by the network, max err served "
+        f"{err[served].max() if served.any() else 0:.2e}, time {t_gate:.2f} s ({tm})")
+    assert err.max() <= 1e-3
+    t0 = time.perf_counter()
+    sg.chain36_batch_expectations(sub, batch=512)
+    t_exact = time.perf_counter() - t0
+    log(f"exact batched: {t_exact:.2f} s")
+    assert t_gate <= t_exact * 1.05
