@@ -32,15 +32,17 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def sweep_gpu(workloads, timing=True):
+def sweep_gpu(workloads, timing=True, graphs=False):
     """All θ_h instances in one batch: per layer one evolve! per instance, then
     the dephasing channel.  Returns (seconds, total counters, <O> list, merge
-    kernel seconds, merge launches)."""
+    kernel seconds, merge launches).  timing: HIP events around every merge
+    launch; graphs: replay captured layer sequences (not combined with timing)."""
     from pauliprop import PPCounters, SparsePauliBatch
     bt = SparsePauliBatch([w.N for w in workloads],
                           [[((p.z, p.x), c) for p, c in w.observable] for w in workloads])
     bt.reserve(1 << 18)
     bt.set_timing(timing)
+    bt.set_graphs(graphs)
     tot = PPCounters()
     nl = len(workloads[0].layers)
     packed = [bt.pack([w.layers[li].generators for w in workloads], [w.layers[li].angles for w in workloads])
@@ -87,6 +89,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graphs", type=int, default=1, help="replay hipGraph-captured layer sequences (1/0)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -113,28 +116,29 @@ def main():
     workloads = [wl.lattice121(t) for t in ths]
     S = 40  # bytes per term: 2 x 16 B keys + 8 B coefficient (65 <= N <= 128)
 
+    graphs = bool(args.graphs)
     for i in range(args.warmup):
         log(f"warmup {i}")
-        sweep_gpu(workloads, timing=False)
+        sweep_gpu(workloads, timing=False, graphs=graphs)
 
     tot_time = 0.0
     agg = PPCounters()
     evs = None
-    t_merge = 0.0
-    n_merge = 0
     for i in range(args.steps):
         log(f"step {i}")
-        dt, cnt, evs, tm, nm = sweep_gpu(workloads)
-        log(f"  {dt:.3f} s, {cnt.terms_swept} term-rotations, {cnt.rotations} rotations, "
-            f"merge kernel {tm:.3f} s over {nm} launches")
+        dt, cnt, evs, _, _ = sweep_gpu(workloads, timing=False, graphs=graphs)
+        log(f"  {dt:.3f} s, {cnt.terms_swept} term-rotations, {cnt.rotations} rotations")
         tot_time += dt
-        t_merge += tm
-        n_merge += nm
         for f, _ in cnt._fields_:
             setattr(agg, f, getattr(agg, f) + getattr(cnt, f))
+    # merge-kernel roofline: one more (untimed) sweep with HIP events around
+    # every merge launch on the batch's stream
+    _, pcnt, pevs, t_merge, n_merge = sweep_gpu(workloads, timing=True, graphs=False)
+    assert pevs == evs, "profiled sweep differs from the timed one"
+    log(f"  profiled sweep: merge kernel {t_merge:.3f} s over {n_merge} launches")
 
     value = agg.terms_swept / tot_time
-    merge_bytes = S * (agg.merge_in + agg.merge_out)
+    merge_bytes = S * (pcnt.merge_in + pcnt.merge_out)
     achieved = merge_bytes / t_merge / 1e9 if t_merge > 0 else None
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "merge_traffic_lattice121.json")
@@ -167,7 +171,7 @@ def main():
         "config": {"workload": "lattice121_sweep: 121-qubit 11x11 open lattice, 20 layers ZZ(-pi/2) on 220 bonds "
                                "+ X(theta_h) on 121 sites, 16 theta_h in [0, pi/2], dephasing 5e-4, |c|<=1e-4, "
                                "weight<=12, observable mean Z",
-                   "rotations": agg.rotations, "term_rotations": agg.terms_swept,
+                   "rotations": agg.rotations, "term_rotations": agg.terms_swept, "graphs": graphs,
                    "expectation_by_theta_h": evs},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,

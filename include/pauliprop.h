@@ -179,6 +179,12 @@ int pp_batch_reserve(pp_batch* p, const int64_t* cap);
 /* Tile shapes: sweep tile 512 or 2048 terms, merge tile 512, 1024 or 2048
  * outputs (2048 only for N <= 64); 0 = chosen from the live counts. */
 int pp_batch_set_tiles(pp_batch* p, int tile, int merge_tile);
+/* Capture each evolve call's launch sequence (capacity precheck, compaction,
+ * every rotation step) in a hipGraph and replay it on later calls with the
+ * same shape (off by default; no effect while timing is on). */
+int pp_batch_set_graphs(pp_batch* p, int on);
+/* Graphs captured and graph launches so far. */
+int pp_batch_graph_stats(pp_batch* p, int64_t* captures, int64_t* launches);
 /* HIP-event timing of every merge-kernel launch (off by default). */
 int pp_batch_set_timing(pp_batch* p, int on);
 int pp_batch_merge_time(pp_batch* p, double* seconds, int64_t* launches, int reset);

@@ -212,7 +212,25 @@ struct Batch {
     size_t evused = 0;
     double t_merge = 0.0;
     int64_t merge_launches = 0;
-    uint32_t epoch = 0;  // merge launch number, tags the look-back words
+    // Look-back tags: the base lives in device memory (so captured graphs can
+    // be replayed) and is advanced by every sequence; epoch_used mirrors it.
+    uint32_t* d_epoch = nullptr;
+    uint64_t epoch_used = 0;
+    // Captured launch sequences (hipGraph), keyed by everything a replay bakes
+    // in: step count, tile shapes, grid sizes and the step-table address.
+    bool graphs = false;
+    struct GraphKey {
+        int64_t nsteps, first;
+        int tile, mtile, W;
+        unsigned g0, g1, g3;
+        const void* steps;
+        bool operator==(const GraphKey& o) const {
+            return nsteps == o.nsteps && first == o.first && tile == o.tile && mtile == o.mtile && W == o.W &&
+                   g0 == o.g0 && g1 == o.g1 && g3 == o.g3 && steps == o.steps;
+        }
+    };
+    std::vector<std::pair<GraphKey, hipGraphExec_t>> gcache;
+    int64_t graph_launches = 0, graph_captures = 0;
 
     WinBuf win;
 
@@ -223,6 +241,8 @@ struct Batch {
         dfree(d_en); dfree(d_steps); dfree(d_partial); dfree(d_out); dfree(d_kets); dfree(d_bad);
         if (h_state) (void)hipHostFree(h_state);
         for (auto e : evpool) (void)hipEventDestroy(e);
+        for (auto& g : gcache) (void)hipGraphExecDestroy(g.second);
+        dfree(d_epoch);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -300,6 +320,8 @@ struct Batch {
         d_out = dalloc<double>(B);
         d_kets = dalloc<uint64_t>((size_t)B * W);
         d_bad = dalloc<int32_t>(B);
+        d_epoch = dalloc<uint32_t>(1);
+        HIPCHK(hipMemsetAsync(d_epoch, 0, sizeof(uint32_t), stream));
         HIPCHK(hipHostMalloc((void**)&h_state, sizeof(SegState) * (size_t)B, hipHostMallocDefault));
         std::memset(h_state, 0, sizeof(SegState) * (size_t)B);
         for (int b = 0; b < B; ++b) {
@@ -361,6 +383,9 @@ struct Batch {
             I.cap_split = sp;
             I.split = dalloc<int64_t>(sp);
             I.status = dalloc<uint64_t>(sp);
+            // look-back words must start "not ready": freed memory may hold
+            // words of another handle whose tags collide with this one's
+            HIPCHK(hipMemsetAsync(I.status, 0, (size_t)sp * sizeof(uint64_t), stream));
         }
     }
 
@@ -403,10 +428,19 @@ struct Batch {
             // real tiles only exit (sweeps) or contend for the ticket (merge)
             const int64_t nest = std::min<int64_t>(cap, std::max<int64_t>(n_known[b] + n_known[b] / 2, 1024));
             const int64_t tiles = cdiv(nest, kTile);
-            acc[0] += std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)ncu * 8));
-            acc[1] += std::max<int64_t>(1, std::min<int64_t>(cdiv(2 * tiles, kBlock / kWave), (int64_t)ncu * 4));
+            // with captured graphs, counts are rounded up to powers of two so
+            // that the grids (baked into a graph) change rarely; surplus
+            // blocks find no tile and exit
+            auto q = [&](int64_t v) {
+                if (!graphs) return v;
+                int64_t r = 1;
+                while (r < v) r <<= 1;
+                return r;
+            };
+            acc[0] += q(std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)ncu * 8)));
+            acc[1] += q(std::max<int64_t>(1, std::min<int64_t>(cdiv(2 * tiles, kBlock / kWave), (int64_t)ncu * 4)));
             acc[2] += 1;
-            acc[3] += std::max<int64_t>(1, std::min<int64_t>(cdiv(nest + nest / 2, T2), (int64_t)ncu * 4));
+            acc[3] += q(std::max<int64_t>(1, std::min<int64_t>(cdiv(nest + nest / 2, T2), (int64_t)ncu * 4)));
         }
         HIPCHK(hipMemcpyAsync(d_maps, h_maps.data(), h_maps.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream));
         if (cap_partial < acc[0]) {
@@ -448,38 +482,93 @@ struct Batch {
         d_steps = dalloc<StepDev>(cap_steps);
     }
 
-    // Next merge epoch; when the 22-bit tag wraps, every look-back array is
-    // cleared first so that no word of an older launch can carry the new tag.
-    uint32_t next_epoch() {
-        epoch = (epoch + 1) & (uint32_t)kEpochMask;
-        if (epoch == 0) {
+    // Reserve `count` look-back tags for the next sequence.  Before the 22-bit
+    // tag would wrap, every look-back array is cleared and the base reset, so
+    // no word of an older launch can carry a tag in use.
+    void reserve_epochs(uint64_t count) {
+        if (epoch_used + count + 1 > (uint64_t)kEpochMask) {
             for (auto& I : ib)
                 if (I.status) HIPCHK(hipMemsetAsync(I.status, 0, (size_t)I.cap_split * sizeof(uint64_t), stream));
-            epoch = 1;
+            static const uint32_t one = 1;
+            HIPCHK(hipMemcpyAsync(d_epoch, &one, sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+            epoch_used = 1;
         }
-        return epoch;
+        epoch_used += count;
     }
 
     template <int Wt, int T, int TM>
-    void launch_step(const StepDev* dsteps, int64_t step_index) {
+    void launch_step(const StepDev* dsteps, int64_t step_index, uint32_t ep_off) {
         hipLaunchKernelGGL((kb_flags<Wt, T>), dim3(blocks(0)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(0));
         hipLaunchKernelGGL(kb_scan<T>, dim3(B), dim3(1024), 0, stream, d_desc, d_state, dsteps, step_index);
         hipLaunchKernelGGL((kb_bounds<Wt, T>), dim3(blocks(1)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(1));
         hipLaunchKernelGGL((kb_emit<Wt, T>), dim3(blocks(0)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(0));
-        const uint32_t ep = next_epoch();
         if (timing) HIPCHK(hipEventRecord(next_event(), stream));
         hipLaunchKernelGGL((kb_merge<Wt, TM>), dim3(blocks(3)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(3),
-                           0, step_index, ep);
+                           0, step_index, (const uint32_t*)d_epoch, ep_off);
         if (timing) HIPCHK(hipEventRecord(next_event(), stream));
     }
 
+    // Compaction launch (the step table is not read in this mode but must be
+    // a valid pointer: the caller has sized it to at least B entries).
     template <int Wt, int TM>
-    void launch_compact() {
-        // the step table is not read in compaction mode but must be a valid pointer
-        ensure_steps(B);
-        const uint32_t ep = next_epoch();
+    void launch_compact(uint32_t ep_off) {
         hipLaunchKernelGGL((kb_merge<Wt, TM>), dim3(blocks(3)), dim3(kBlock), 0, stream, d_desc, d_state, d_steps, map(3),
-                           1, (int64_t)-1, ep);
+                           1, (int64_t)-1, (const uint32_t*)d_epoch, ep_off);
+    }
+
+    // Enqueue a sequence: optional precheck + compaction, steps [first,
+    // nsteps) from the table at dsteps, then the tag advance.  In graph mode
+    // the launches are captured once per key and replayed.
+    template <int Wt>
+    void enqueue_sequence(bool with_precheck, const StepDev* dsteps, int64_t first, int64_t nsteps, bool use_graph) {
+        const uint32_t count = (uint32_t)(nsteps - first + (with_precheck ? 1 : 0));
+        reserve_epochs(count);
+        auto body = [&]() {
+            if (with_precheck)
+                hipLaunchKernelGGL(kb_precheck, dim3((unsigned)cdiv(B, 256)), dim3(256), 0, stream, d_state,
+                                   (const int32_t*)d_en, B);
+            with_tiles<Wt>([&](auto T, auto TM) {
+                uint32_t off = 0;
+                if (with_precheck) launch_compact<Wt, decltype(TM)::value>(off++);
+                for (int64_t s = first; s < nsteps; ++s)
+                    launch_step<Wt, decltype(T)::value, decltype(TM)::value>(dsteps + (s - first) * B, s, off++);
+            });
+            hipLaunchKernelGGL(kb_epoch_advance, dim3(1), dim3(1), 0, stream, d_epoch, count);
+        };
+        if (!use_graph || timing) {
+            body();
+            HIPCHK(hipGetLastError());
+            return;
+        }
+        GraphKey key{nsteps, with_precheck ? -1 - first : first, tile, mtile, Wt, blocks(0), blocks(1), blocks(3),
+                     (const void*)dsteps};
+        for (auto& g : gcache)
+            if (g.first == key) {
+                HIPCHK(hipGraphLaunch(g.second, stream));
+                graph_launches += 1;
+                return;
+            }
+        if (gcache.size() >= 16) {
+            HIPCHK(hipStreamSynchronize(stream));
+            for (auto& g : gcache) (void)hipGraphExecDestroy(g.second);
+            gcache.clear();
+        }
+        hipGraph_t graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        body();
+        const hipError_t le = hipGetLastError();
+        const hipError_t ce = hipStreamEndCapture(stream, &graph);
+        HIPCHK(le);
+        HIPCHK(ce);
+        hipGraphExec_t exec = nullptr;
+        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPCHK(ie);
+        gcache.emplace_back(key, exec);
+        graph_captures += 1;
+        HIPCHK(hipGraphLaunch(exec, stream));
+        graph_launches += 1;
     }
 
     // Run f<T, TM>() with the current tile shapes as template arguments.
@@ -539,12 +628,7 @@ struct Batch {
         HIPCHK(hipMemcpyAsync(d_steps, steps.data(), (size_t)(nsteps * B) * sizeof(StepDev), hipMemcpyHostToDevice,
                               stream));
         HIPCHK(hipMemcpyAsync(d_en, en.data(), (size_t)B * sizeof(int32_t), hipMemcpyHostToDevice, stream));
-        hipLaunchKernelGGL(kb_precheck, dim3((unsigned)cdiv(B, 256)), dim3(256), 0, stream, d_state, d_en, B);
-        with_tiles<Wt>([&](auto T, auto TM) {
-            launch_compact<Wt, decltype(TM)::value>();
-            for (int64_t s = 0; s < nsteps; ++s) launch_step<Wt, decltype(T)::value, decltype(TM)::value>(d_steps + s * B, s);
-        });
-        HIPCHK(hipGetLastError());
+        enqueue_sequence<Wt>(true, d_steps, 0, nsteps, graphs);
         sync_state();
         for (int round = 0;; ++round) {
             std::vector<int> stalled;
@@ -583,11 +667,7 @@ struct Batch {
             upload_desc();
             plan();
             HIPCHK(hipMemcpyAsync(d_steps, rs.data(), rs.size() * sizeof(StepDev), hipMemcpyHostToDevice, stream));
-            with_tiles<Wt>([&](auto T, auto TM) {
-                for (int64_t s = first; s < nsteps; ++s)
-                    launch_step<Wt, decltype(T)::value, decltype(TM)::value>(d_steps + (s - first) * B, s);
-            });
-            HIPCHK(hipGetLastError());
+            enqueue_sequence<Wt>(false, d_steps, first, nsteps, false);
             sync_state();
         }
     }
@@ -613,7 +693,10 @@ struct Batch {
         put_state();
         upload_desc();
         plan();
-        with_tiles<Wt>([&](auto, auto TM) { launch_compact<Wt, decltype(TM)::value>(); });
+        ensure_steps(B);
+        reserve_epochs(1);
+        with_tiles<Wt>([&](auto, auto TM) { launch_compact<Wt, decltype(TM)::value>(0); });
+        hipLaunchKernelGGL(kb_epoch_advance, dim3(1), dim3(1), 0, stream, d_epoch, 1u);
         HIPCHK(hipGetLastError());
         sync_state();
     }
@@ -1540,6 +1623,19 @@ int pp_batch_set_tiles(pp_batch* p, int tile, int merge_tile) {
     if (merge_tile == 2048 && p->bt.W == 2) return fail(PP_EARG, "merge_tile 2048 needs N <= 64");
     p->bt.tile_req = tile;
     p->bt.mtile_req = merge_tile;
+    return PP_OK;
+}
+
+int pp_batch_set_graphs(pp_batch* p, int on) {
+    if (!p) return fail(PP_EARG, "null handle");
+    p->bt.graphs = on != 0;
+    return PP_OK;
+}
+
+int pp_batch_graph_stats(pp_batch* p, int64_t* captures, int64_t* launches) {
+    if (!p || !captures || !launches) return fail(PP_EARG, "null argument");
+    *captures = p->bt.graph_captures;
+    *launches = p->bt.graph_launches;
     return PP_OK;
 }
 

@@ -502,7 +502,8 @@ __device__ __forceinline__ bool merge_go(const StepDev* st, const SegState* s, i
 template <int W, int TM>
 __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D, SegState* __restrict__ S,
                                                    const StepDev* __restrict__ steps, BlockMap bm, int mode,
-                                                   int64_t step_index, uint32_t epoch) {
+                                                   int64_t step_index, const uint32_t* __restrict__ d_epoch,
+                                                   uint32_t ep_off) {
     constexpr int E = TM / kBlock;
     __shared__ Key<W> s_key[TM];
     __shared__ double s_c[TM];
@@ -517,7 +518,9 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
     SegState* s = S + b;
     if (!merge_go(st, s, mode)) return;
     const int rot = mode == 0;
-    const uint64_t ep = ((uint64_t)epoch & kEpochMask) << kEpochShift;
+    // launch tag of the look-back words: the sequence's base (device memory,
+    // advanced after every sequence) plus this launch's offset in it
+    const uint64_t ep = ((uint64_t)(*d_epoch + ep_off) & kEpochMask) << kEpochShift;
     const int cur = s->cur;
     const int64_t n = s->n[cur];
     const int64_t m = rot ? s->m : 0;
@@ -758,6 +761,12 @@ __global__ void kb_precheck(SegState* __restrict__ S, const int32_t* __restrict_
         s->early += 1;
         s->merge_in += n;
     }
+}
+
+// Advance the look-back tag base past the `count` merge launches of the
+// sequence just enqueued.
+__global__ void kb_epoch_advance(uint32_t* __restrict__ d_epoch, uint32_t count) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *d_epoch += count;
 }
 
 // Resolve pending parity flips (before host reads and between sequences).

@@ -75,6 +75,21 @@ class SparsePauliBatch:
             c[i] = float(coef)
         check(_lib.lib().pp_batch_upload(self._h, b, u64p(z), u64p(x), dp(c), n, 0))
 
+    def upload_arrays(self, b: int, z: np.ndarray, x: np.ndarray, c: np.ndarray, sorted_keys: bool = True):
+        """Replace vector b's live set by word arrays z[n][W], x[n][W], c[n]
+        (keys strictly increasing when sorted_keys)."""
+        z = np.ascontiguousarray(z, dtype=np.uint64).reshape(-1, self.W)
+        x = np.ascontiguousarray(x, dtype=np.uint64).reshape(-1, self.W)
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        check(_lib.lib().pp_batch_upload(self._h, b, u64p(z), u64p(x), dp(c), len(c), 1 if sorted_keys else 0))
+        return self
+
+    def use_tuning(self, table):
+        """Pick tile shapes before every evolve call from an autotuner table
+        (pauliprop.autotune) by the current largest live size."""
+        self._tuning = table
+        return self
+
     def sizes(self) -> List[int]:
         n = (ctypes.c_int64 * self.B)()
         check(_lib.lib().pp_batch_sizes(self._h, n))
@@ -141,6 +156,9 @@ class SparsePauliBatch:
         if nsteps == 0:
             return [PPCounters() for _ in range(self.B)] if counters else None
         B = self.B
+        if getattr(self, "_tuning", None):
+            from .autotune import lookup
+            self.set_tiles(*lookup(self._tuning, self.W, max(self.sizes())))
         fs = (PPFilter * B)(*[compile_filter(s) for s in strat])
         fl = (PPFilter * B)(*[compile_filter(s) for s in loc])
         cnt = (PPCounters * B)()
@@ -205,6 +223,18 @@ class SparsePauliBatch:
         """Sweep tile (512/2048) and merge tile (512/1024/2048); 0 = automatic."""
         check(_lib.lib().pp_batch_set_tiles(self._h, int(tile), int(merge_tile)))
         return self
+
+    def set_graphs(self, on: bool):
+        """Capture each evolve call's launch sequence in a hipGraph and replay it
+        on later calls of the same shape."""
+        check(_lib.lib().pp_batch_set_graphs(self._h, 1 if on else 0))
+        return self
+
+    def graph_stats(self):
+        c = ctypes.c_int64()
+        n = ctypes.c_int64()
+        check(_lib.lib().pp_batch_graph_stats(self._h, ctypes.byref(c), ctypes.byref(n)))
+        return c.value, n.value
 
     def set_timing(self, on: bool):
         check(_lib.lib().pp_batch_set_timing(self._h, 1 if on else 0))

@@ -192,18 +192,41 @@ def chain36_plain_expectation(theta: Sequence[float]) -> float:
 
 
 class GatedSurrogate:
-    """Ensemble mean where the ensemble agrees (std <= tol), exact propagation
-    elsewhere."""
+    """Ensemble mean where the ensemble agrees, exact propagation elsewhere.
+
+    A θ is served by the network when its ensemble standard deviation is at
+    most ``threshold``.  ``calibrate`` sets the threshold from labelled θ the
+    network was not trained on: the largest disagreement s (at most ``tol``)
+    such that every calibration point with std <= s is within ``tol`` of its
+    exact value; when even the most confident calibration point misses, the
+    threshold is 0 and every query goes to exact propagation.  (Ensemble
+    members that all collapse to the same wrong answer agree closely, so
+    disagreement alone is not a safe gate.)"""
 
     def __init__(self, ensemble: MLPEnsemble, exact: Callable = chain36_batch_expectations, tol: float = 1e-3):
         self.ens, self.exact, self.tol = ensemble, exact, tol
+        self.threshold = tol
+
+    def calibrate(self, thetas, labels):
+        mean, sd, _, _ = self.ens.predict(features(thetas))
+        err = np.abs(mean - np.asarray(labels))
+        order = np.argsort(sd, kind="stable")
+        bad = np.nonzero(err[order] > self.tol)[0]
+        k = bad[0] if len(bad) else len(order)  # points [0, k) of the std order are all within tol
+        if k == 0:
+            self.threshold = 0.0
+        else:
+            nxt = sd[order[k]] if k < len(order) else np.inf
+            self.threshold = float(min(self.tol, sd[order[k - 1]] if nxt > sd[order[k - 1]] else 0.0))
+        return {"threshold": self.threshold, "calibration_points": int(len(order)), "within_tol_prefix": int(k),
+                "calibration_rmse": float(np.sqrt(np.mean(err ** 2)))}
 
     def evaluate(self, thetas):
         thetas = np.asarray(thetas, dtype=np.float64)
         t0 = time.perf_counter()
         mean, sd, _, dev_s = self.ens.predict(features(thetas))
         t1 = time.perf_counter()
-        fall = sd > self.tol
+        fall = ~(sd <= self.threshold) if self.threshold > 0 else np.ones(len(thetas), dtype=bool)
         vals = mean.copy()
         if fall.any():
             vals[fall] = self.exact([list(t) for t in thetas[fall]])

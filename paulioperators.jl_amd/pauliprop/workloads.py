@@ -182,3 +182,50 @@ def surrogate_chain36_layers(theta: List[float]) -> Workload:
 def surrogate_thetas(count: int, seed: int = 99) -> List[List[float]]:
     rng = Xoshiro256ss(seed)
     return [[-math.pi + 2 * math.pi * rng.uniform() for _ in range(20)] for _ in range(count)]
+
+
+def grid_bonds(N: int) -> Tuple[int, int, List[Tuple[int, int]]]:
+    """Near-square open grid holding N sites: rows = floor(sqrt N), cols =
+    ceil(N / rows), sites row-major q = cols*r + c + 1 <= N (the last row may be
+    partial); nearest-neighbour bonds between present sites, ordered
+    lexicographically by (min site, max site)."""
+    rows = max(1, int(math.isqrt(N)))
+    cols = -(-N // rows)
+    bonds = []
+    for q in range(1, N + 1):
+        r, c = divmod(q - 1, cols)
+        if c + 1 < cols and q + 1 <= N:
+            bonds.append((q, q + 1))
+        if q + cols <= N:
+            bonds.append((q, q + cols))
+    bonds.sort()
+    return rows, cols, bonds
+
+
+def batched_instances(count: int = 256, seed: int = 7, max_depth: Optional[int] = None) -> List[Workload]:
+    """The batched workload: `count` independent circuits from one seed-7
+    stream, drawn instance by instance in this order: N ~ U{24..128},
+    depth ~ U{5..40}, topology bit (0: open chain, 1: near-square grid,
+    grid_bonds), observable site ~ U{1..N}; then per layer RZZ on every bond,
+    RX on every site, RZ on every site, each angle i.i.d. U[0, π) in that
+    order.  Observable Z_site; CoeffTruncation(1e-5); FP64.  max_depth (for
+    tests) caps the number of layers kept without changing the draws of
+    later instances."""
+    rng = Xoshiro256ss(seed)
+    out = []
+    for i in range(count):
+        N = rng.randint(24, 128)
+        depth = rng.randint(5, 40)
+        grid = rng.next() >> 63
+        site = rng.randint(1, N)
+        bonds = grid_bonds(N)[2] if grid else chain_bonds(N)
+        gens = ([Pauli(N, Z=[a, b]) for a, b in bonds] + [Pauli(N, X=[j]) for j in range(1, N + 1)]
+                + [Pauli(N, Z=[j]) for j in range(1, N + 1)])
+        layers = []
+        for _ in range(depth):
+            layers.append(Layer(gens, [rng.uniform() * math.pi for _ in gens]))
+        keep = layers if max_depth is None else layers[:max_depth]
+        out.append(Workload(f"batched_{i}", N, [(Pauli(N, Z=[site]), 1.0)], keep, CoeffTruncation(1e-5),
+                            meta={"index": i, "depth": depth, "topology": "grid" if grid else "chain",
+                                  "site": site, "bonds": len(bonds)}))
+    return out

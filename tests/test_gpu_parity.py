@@ -335,3 +335,86 @@ def test_batch_uneven_vectors_bitexact():
         assert np.array_equal(gz, oz) and np.array_equal(gx, ox), f"vector {b} keys"
         assert np.array_equal(gc.view(np.uint64), oc.view(np.uint64)), f"vector {b} coefficients"
         assert abs(evs[b] - orcs[b].expectation(0)) <= 1e-12
+
+
+def test_batch_graph_replays_match_eager():
+    """hipGraph-captured sequences: the same layer circuit replayed over
+    several calls, with eager channel work and host synchronisation between
+    the replays; every replay's output equals an eager twin batch and the
+    oracle bit for bit."""
+    from pauliprop import SparsePauliBatch
+    Ns = [64, 40, 64, 24]
+    B = len(Ns)
+    rng = random.Random(71)
+    gens, angs = [], []
+    for N in Ns:
+        g = [Pauli(N, Z=[j, j + 1]) for j in range(1, N)] + [Pauli(N, X=[j]) for j in range(1, N + 1)]
+        gens.append(g + [Pauli(N, Z=[1])] * (127 - len(g)))  # equal step counts per call
+        angs.append([rng.uniform(0, math.pi) for _ in range(127)])
+    obs = [[(Pauli(N, Z=[N // 2]).z, 0, 1.0)] for N in Ns]
+    st = CoeffTruncation(1e-5)
+    g_bt = SparsePauliBatch(Ns, [[((z, x), c) for z, x, c in o] for o in obs]).set_graphs(True)
+    e_bt = SparsePauliBatch(Ns, [[((z, x), c) for z, x, c in o] for o in obs])
+    g_bt.reserve(1 << 16)
+    e_bt.reserve(1 << 16)
+    orcs = [po.OracleSPV(N, o) for N, o in zip(Ns, obs)]
+    for layer in range(6):
+        g_bt.evolve_(gens, angs, truncation=st)
+        e_bt.evolve_(gens, angs, truncation=st)
+        g_bt.channel_("depolarizing", 1e-3)
+        e_bt.channel_("depolarizing", 1e-3)
+        for b in range(B):
+            orcs[b].evolve([(p.z, p.x) for p in gens[b]], angs[b], strict=compile_filter(st))
+            orcs[b].channel("depolarizing", 1e-3)
+        assert g_bt.expectation_values() == e_bt.expectation_values()
+        for b in range(B):
+            gz, gx, gc = g_bt.arrays(b)
+            ez, ex, ec = e_bt.arrays(b)
+            assert np.array_equal(gz, ez) and np.array_equal(gx, ex), f"layer {layer} vector {b} keys"
+            assert np.array_equal(gc.view(np.uint64), ec.view(np.uint64)), f"layer {layer} vector {b} coefficients"
+            oz, ox, oc = orcs[b].arrays()
+            assert np.array_equal(gz, oz) and np.array_equal(gc.view(np.uint64), oc.view(np.uint64))
+        log(f"graph layer {layer}: sizes {g_bt.sizes()} stats {g_bt.graph_stats()}")
+    caps, launches = g_bt.graph_stats()
+    assert launches == 6
+    # zero angles: every child has c = 0 and is cut, sizes and grids stay
+    # fixed, so the captured sequence is replayed
+    zero = [[0.0] * 127 for _ in range(B)]
+    for rep in range(3):
+        g_bt.evolve_(gens, zero, truncation=st)
+        e_bt.evolve_(gens, zero, truncation=st)
+        assert g_bt.expectation_values() == e_bt.expectation_values()
+        for b in range(B):
+            assert np.array_equal(g_bt.arrays(b)[2].view(np.uint64), e_bt.arrays(b)[2].view(np.uint64))
+    caps2, launches2 = g_bt.graph_stats()
+    log(f"graph stats after replays: {caps2} captures, {launches2} launches")
+    assert launches2 == 9 and caps2 <= caps + 1
+
+
+def test_batched_instances_bitexact_with_graphs_and_chunks():
+    """The first 12 instances of the seed-7 batched set (2 layers each), run
+    as in the batched driver (key-width groups, 64-step chunks, captured
+    graphs), equal their single-vector oracle runs bit for bit."""
+    from pauliprop import SparsePauliBatch
+    from pauliprop import batched_bench as bb
+    ws = wl.batched_instances(12, seed=7, max_depth=2)
+    for sel in ([w for w in ws if w.N <= 64], [w for w in ws if w.N > 64]):
+        if not sel:
+            continue
+        bt = SparsePauliBatch([w.N for w in sel], [[((p.z, p.x), c) for p, c in w.observable] for w in sel])
+        bt.set_graphs(True)
+        gz, gx, layers = bb._group_arrays(sel, bt.W)
+        orcs = [po.OracleSPV(w.N, [(p.z, p.x, c) for p, c in w.observable]) for w in sel]
+        for li, (cv, sv, act) in enumerate(layers):
+            for s0 in range(0, act.shape[0], 64):
+                bt.evolve_packed((gz[s0:s0 + 64], gx[s0:s0 + 64], cv[s0:s0 + 64], sv[s0:s0 + 64],
+                                  np.ascontiguousarray(act[s0:s0 + 64])), truncation=sel[0].truncation)
+            for w, o in zip(sel, orcs):
+                L = w.layers[li]
+                o.evolve([(p.z, p.x) for p in L.generators], L.angles, strict=compile_filter(w.truncation))
+            log(f"batched instances W={bt.W} layer {li}: sizes {bt.sizes()}")
+        for b, o in enumerate(orcs):
+            gz_, gx_, gc_ = bt.arrays(b)
+            oz, ox, oc = o.arrays()
+            assert np.array_equal(gz_, oz) and np.array_equal(gx_, ox), f"instance {sel[b].name} keys"
+            assert np.array_equal(gc_.view(np.uint64), oc.view(np.uint64)), f"instance {sel[b].name} coefficients"
