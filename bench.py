@@ -89,7 +89,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--graphs", type=int, default=1, help="replay hipGraph-captured layer sequences (1/0)")
+    ap.add_argument("--graphs", type=int, default=0, help="replay hipGraph-captured layer sequences (1/0)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
