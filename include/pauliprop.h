@@ -66,8 +66,12 @@ typedef struct {
  *   PP_OP_STOCHASTIC_COEFF  StochasticCoeffTruncation(epsilon = value):
  *                           stochastic_clip! (spv_evolve.jl:35-52) with a
  *                           counter-based generator keyed by `seed` (the Julia
- *                           RNG stream is not reproduced) */
-enum { PP_OP_FILTER = 0, PP_OP_ADAPTIVE = 1, PP_OP_STOCHASTIC_COEFF = 2 };
+ *                           RNG stream is not reproduced)
+ *   PP_OP_STOCHASTIC_SAMPLING  StochasticSamplingTruncation(n_keep = max_terms):
+ *                           exponential keys u^(1/|c|^2), the n_keep largest
+ *                           kept (ties: lower index), rescaled to the original
+ *                           Σ|c|^2 (spv_evolve.jl:69-99); same generator */
+enum { PP_OP_FILTER = 0, PP_OP_ADAPTIVE = 1, PP_OP_STOCHASTIC_COEFF = 2, PP_OP_STOCHASTIC_SAMPLING = 3 };
 typedef struct {
     int32_t kind;
     int32_t reserved;
@@ -118,6 +122,18 @@ int pp_evolve_ex(pp_handle* h, const uint64_t* gz, const uint64_t* gx, const dou
 /* truncate!(v, strategy, corr) (truncation.jl:330-337) for any strategy given
  * as operations; energy_ket / energy_acc as in pp_evolve_ex (may be NULL). */
 int pp_truncate(pp_handle* h, const pp_strategy_op* ops, int nops, const uint64_t* energy_ket, double* energy_acc);
+
+/* get / getindex / haskey (type_SparsePauliVector.jl:286-311): for `count`
+ * query keys (W words each in z, x), found[i] = 1 and c[i] = the coefficient
+ * when the key is live, else found[i] = 0 and c[i] = 0.  Device binary search. */
+int pp_lookup(pp_handle* h, const uint64_t* z, const uint64_t* x, int64_t count, double* c, int32_t* found);
+/* norm(v, p) (spv_ops.jl:57-84): p = 2, 1, Inf or any p > 0. */
+int pp_norm(pp_handle* h, double p, double* out);
+/* inner_product(a, b) (spv_ops.jl:378-396): Σ over shared keys of c_a c_b. */
+int pp_inner_product(pp_handle* a, pp_handle* b, double* out);
+/* The distance isapprox(a, b; atol, rtol) tests (spv_ops.jl:88-107):
+ * sqrt(Σ over the union of keys of (c_a - c_b)^2), absent = 0. */
+int pp_diff_norm(pp_handle* a, pp_handle* b, double* out);
 
 /* Single rotation without truncation (spv_evolve.jl:228-236). */
 int pp_evolve_single(pp_handle* h, const uint64_t* gz, const uint64_t* gx, double cosv, double sinv);

@@ -26,6 +26,7 @@
 #include "pp_common.hpp"
 #include "pp_kernels.hpp"
 #include "pp_window.hpp"
+#include "pp_ops.hpp"
 
 using namespace pp;
 
@@ -901,7 +902,7 @@ struct Batch {
     // Order-preserving compaction of the live set (mode 0: filter slot 2;
     // mode 1: stochastic clip).  Returns the new count; flips cur.
     template <int Wt>
-    int64_t win_compact(int64_t n, int& cur, int mode, double eps, uint64_t seed) {
+    int64_t win_compact(int64_t n, int& cur, int mode, double eps, uint64_t seed, SampleSel ss = SampleSel{}) {
         WinBuf& w = win;
         if (n == 0) return 0;
         ensure_caps(0, n, ib[0].C.cap, ib[0].maxlev, n, cur);
@@ -910,10 +911,10 @@ struct Batch {
         const DevSet& Li = ib[0].L[cur];
         const DevSet& Lo = ib[0].L[cur ^ 1];
         hipLaunchKernelGGL(kw_compact<Wt>, dim3((unsigned)nb), dim3(kBlock), 0, stream, Li.z, Li.x, Li.c, n,
-                           w.filt + 2, mode, eps, seed, 1, w.bcount, (const int64_t*)nullptr, Lo.z, Lo.x, Lo.c);
+                           w.filt + 2, mode, eps, seed, 1, w.bcount, (const int64_t*)nullptr, Lo.z, Lo.x, Lo.c, ss);
         scan_i64(w.bcount, nb, w.boff, w.small);
         hipLaunchKernelGGL(kw_compact<Wt>, dim3((unsigned)nb), dim3(kBlock), 0, stream, Li.z, Li.x, Li.c, n,
-                           w.filt + 2, mode, eps, seed, 0, w.bcount, (const int64_t*)w.boff, Lo.z, Lo.x, Lo.c);
+                           w.filt + 2, mode, eps, seed, 0, w.bcount, (const int64_t*)w.boff, Lo.z, Lo.x, Lo.c, ss);
         HIPCHK(hipGetLastError());
         cur ^= 1;
         return read_small(0);
@@ -945,6 +946,61 @@ struct Batch {
         double v;
         std::memcpy(&v, &prefix, sizeof(v));
         return v;
+    }
+
+    // Radix select of the k-th largest (key, n - 1 - i) pair (0-based from the
+    // top): eight byte passes over the keys, then over the index among the
+    // terms whose key equals the selected one.
+    std::pair<uint64_t, int64_t> win_select_sample(const uint64_t* key, int64_t n, int64_t k) {
+        WinBuf& w = win;
+        const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(n, kBlock), 2048));
+        unsigned long long h[kRadix];
+        uint64_t val[2] = {0, 0};
+        for (int stage = 0; stage < 2; ++stage) {
+            uint64_t prefix = 0, pmask = 0;
+            for (int shift = 56; shift >= 0; shift -= 8) {
+                HIPCHK(hipMemsetAsync(w.qhist, 0, kRadix * sizeof(unsigned long long), stream));
+                hipLaunchKernelGGL(kq_hist_sel, dim3(grid), dim3(kBlock), 0, stream, key, n, stage, val[0], shift,
+                                   prefix, pmask, w.qhist);
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipMemcpyAsync(h, w.qhist, sizeof(h), hipMemcpyDeviceToHost, stream));
+                HIPCHK(hipStreamSynchronize(stream));
+                int d = kRadix - 1;
+                for (; d >= 0; --d) {  // from the largest byte value down
+                    if ((int64_t)h[d] > k) break;
+                    k -= (int64_t)h[d];
+                }
+                if (d < 0) throw PPError(PP_EHIP, "radix select lost its rank");
+                prefix |= (uint64_t)d << shift;
+                pmask |= 0xffull << shift;
+            }
+            val[stage] = prefix;
+        }
+        return {val[0], (int64_t)val[1]};
+    }
+
+    // StochasticSamplingTruncation (spv_evolve.jl:69-99): keep n_keep terms
+    // drawn by the exponential-key method, rescaled to the original Σ|c|^2.
+    template <int Wt>
+    int64_t win_sample(int64_t n, int& cur, int64_t n_keep, uint64_t seed) {
+        if (n <= n_keep) return n;
+        const double norm_sq = norm_sum_of(ib[0].L[cur].c, n, 0, 2.0);
+        uint64_t* key = dalloc<uint64_t>((size_t)n);
+        const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(n, kBlock), 2048));
+        hipLaunchKernelGGL(kw_sample_keys, dim3(grid), dim3(kBlock), 0, stream, ib[0].L[cur].c, n, seed, key);
+        HIPCHK(hipGetLastError());
+        const auto sel = win_select_sample(key, n, n_keep - 1);
+        SampleSel ss{key, sel.first, sel.second, n};
+        n = win_compact<Wt>(n, cur, 2, 0.0, 0, ss);
+        HIPCHK(hipStreamSynchronize(stream));
+        dfree(key);
+        const double kept = norm_sum_of(ib[0].L[cur].c, n, 0, 2.0);
+        if (kept > 0.0) {
+            hipLaunchKernelGGL(kw_scale, dim3(grid), dim3(kBlock), 0, stream, ib[0].L[cur].c, n,
+                               std::sqrt(norm_sq / kept));
+            HIPCHK(hipGetLastError());
+        }
+        return n;
     }
 
     void win_filter(int slot, const pp_filter* f) {
@@ -990,6 +1046,9 @@ struct Batch {
             } else if (op.kind == PP_OP_STOCHASTIC_COEFF) {
                 const uint64_t seed = op.seed ^ (0x9e3779b97f4a7c15ull * (++win.rng_calls));
                 n = win_compact<Wt>(n, cur, 1, op.value, seed);
+            } else if (op.kind == PP_OP_STOCHASTIC_SAMPLING) {
+                const uint64_t seed = op.seed ^ (0x9e3779b97f4a7c15ull * (++win.rng_calls));
+                n = win_sample<Wt>(n, cur, op.max_terms, seed);
             } else {
                 throw PPError(PP_EARG, "unknown strategy operation kind " + std::to_string(op.kind));
             }
@@ -1105,6 +1164,82 @@ struct Batch {
         put_state();
         upload_desc();
         HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    // ----------------------------------------------------------------------
+    // Lookups and reductions on vector 0 (spv_ops.jl, type_SparsePauliVector.jl)
+
+    const DevSet& live0(int64_t& n) {
+        sync_state();
+        const SegState& st = h_state[0];
+        n = st.n[st.cur];
+        return ib[0].L[st.cur];
+    }
+
+    template <int Wt>
+    void lookup(const uint64_t* z, const uint64_t* x, int64_t count, double* out, int32_t* found) {
+        if (count <= 0) return;
+        int64_t n = 0;
+        const DevSet& L = live0(n);
+        uint64_t* qz = dalloc<uint64_t>((size_t)count * Wt);
+        uint64_t* qx = dalloc<uint64_t>((size_t)count * Wt);
+        double* dout = dalloc<double>((size_t)count);
+        int32_t* dfound = dalloc<int32_t>((size_t)count);
+        HIPCHK(hipMemcpyAsync(qz, z, (size_t)count * Wt * 8, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(qx, x, (size_t)count * Wt * 8, hipMemcpyHostToDevice, stream));
+        const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(count, kBlock), 4096));
+        hipLaunchKernelGGL(ko_lookup<Wt>, dim3(grid), dim3(kBlock), 0, stream, L.z, L.x, L.c, n, qz, qx, count, dout,
+                           dfound);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out, dout, (size_t)count * 8, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(found, dfound, (size_t)count * 4, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        dfree(qz); dfree(qx); dfree(dout); dfree(dfound);
+    }
+
+    double reduce_partials(double2* part, unsigned grid, bool is_max) {
+        double* dout = dalloc<double>(1);
+        if (is_max) hipLaunchKernelGGL(ko_max_final, dim3(1), dim3(kBlock), 0, stream, part, (int64_t)grid, dout);
+        else hipLaunchKernelGGL(kw_expect_final, dim3(1), dim3(kBlock), 0, stream, part, (int64_t)grid, dout);
+        HIPCHK(hipGetLastError());
+        double v = 0.0;
+        HIPCHK(hipMemcpyAsync(&v, dout, sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        dfree(dout);
+        return v;
+    }
+
+    // kind 0: Σ c^2, 1: Σ |c|, 2: max |c|, 3: Σ |c|^p
+    double norm_sum(int kind, double p) {
+        int64_t n = 0;
+        const DevSet& L = live0(n);
+        return norm_sum_of(L.c, n, kind, p);
+    }
+
+    double norm_sum_of(const double* c, int64_t n, int kind, double p) {
+        const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(n, kBlock), 1024));
+        double2* part = dalloc<double2>(grid);
+        hipLaunchKernelGGL(ko_norm, dim3(grid), dim3(kBlock), 0, stream, c, n, kind, p, part);
+        HIPCHK(hipGetLastError());
+        const double v = reduce_partials(part, grid, kind == 2);
+        dfree(part);
+        return v;
+    }
+
+    // Σ over this vector's terms of ko_join's mode term against `other`.
+    template <int Wt>
+    double join(Batch& other, int mode) {
+        int64_t na = 0, nb = 0;
+        const DevSet& A = live0(na);
+        const DevSet& Bs = other.live0(nb);
+        const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(na, kBlock), 1024));
+        double2* part = dalloc<double2>(grid);
+        hipLaunchKernelGGL(ko_join<Wt>, dim3(grid), dim3(kBlock), 0, stream, A.z, A.x, A.c, na, Bs.z, Bs.x, Bs.c, nb,
+                           mode, part);
+        HIPCHK(hipGetLastError());
+        const double v = reduce_partials(part, grid, false);
+        dfree(part);
+        return v;
     }
 
     void upload(int b, const uint64_t* z, const uint64_t* x, const double* c, int64_t n, bool sorted) {
@@ -1389,7 +1524,8 @@ static void check_ops(const pp_strategy_op* ops, int nops) {
         const pp_strategy_op& op = ops[o];
         if (op.kind == PP_OP_ADAPTIVE && op.max_terms < 0) throw PPError(PP_EARG, "max_terms must be >= 0");
         if (op.kind == PP_OP_STOCHASTIC_COEFF && !(op.value > 0.0)) throw PPError(PP_EARG, "epsilon must be > 0");
-        if (op.kind < PP_OP_FILTER || op.kind > PP_OP_STOCHASTIC_COEFF)
+        if (op.kind == PP_OP_STOCHASTIC_SAMPLING && op.max_terms < 0) throw PPError(PP_EARG, "n_keep must be >= 0");
+        if (op.kind < PP_OP_FILTER || op.kind > PP_OP_STOCHASTIC_SAMPLING)
             throw PPError(PP_EARG, "unknown strategy operation kind " + std::to_string(op.kind));
     }
 }
@@ -1427,6 +1563,45 @@ int pp_truncate(pp_handle* h, const pp_strategy_op* ops, int nops, const uint64_
     double e = energy_ket ? *energy_acc : 0.0;
     dispatch_w(h->bt.W, [&](auto wc) { h->bt.truncate_ops<decltype(wc)::value>(ops, nops, energy_ket, &e); });
     if (energy_ket) *energy_acc = e;
+    API_END
+}
+
+int pp_lookup(pp_handle* h, const uint64_t* z, const uint64_t* x, int64_t count, double* c, int32_t* found) {
+    if (!h || count < 0 || (count > 0 && (!z || !x || !c || !found))) return fail(PP_EARG, "bad arguments");
+    API_BEGIN
+    dispatch_w(h->bt.W, [&](auto wc) { h->bt.lookup<decltype(wc)::value>(z, x, count, c, found); });
+    API_END
+}
+
+int pp_norm(pp_handle* h, double p, double* out) {
+    if (!h || !out) return fail(PP_EARG, "null argument");
+    if (!(p > 0)) return fail(PP_EARG, "p must be > 0");
+    API_BEGIN
+    if (p == 2.0) *out = std::sqrt(h->bt.norm_sum(0, 2.0));
+    else if (p == 1.0) *out = h->bt.norm_sum(1, 1.0);
+    else if (std::isinf(p)) *out = h->bt.norm_sum(2, p);
+    else *out = std::pow(h->bt.norm_sum(3, p), 1.0 / p);
+    API_END
+}
+
+int pp_inner_product(pp_handle* a, pp_handle* b, double* out) {
+    if (!a || !b || !out) return fail(PP_EARG, "null argument");
+    if (a->N != b->N) return fail(PP_EARG, "qubit counts differ");
+    API_BEGIN
+    dispatch_w(a->bt.W, [&](auto wc) { *out = a->bt.join<decltype(wc)::value>(b->bt, 0); });
+    API_END
+}
+
+int pp_diff_norm(pp_handle* a, pp_handle* b, double* out) {
+    if (!a || !b || !out) return fail(PP_EARG, "null argument");
+    if (a->N != b->N) return fail(PP_EARG, "qubit counts differ");
+    API_BEGIN
+    dispatch_w(a->bt.W, [&](auto wc) {
+        constexpr int Wt = decltype(wc)::value;
+        const double s1 = a->bt.join<Wt>(b->bt, 1);
+        const double s2 = b->bt.join<Wt>(a->bt, 2);
+        *out = std::sqrt(s1 + s2);
+    });
     API_END
 }
 

@@ -509,10 +509,23 @@ __device__ __forceinline__ double counter_uniform(uint64_t seed, uint64_t i) {
     return (double)(h >> 11) * 0x1.0p-53;
 }
 
+// Selection of mode 2: the sampling key of term i (monotone encoding) and the
+// threshold pair: keep when (key, n - 1 - i) >= (eT, jT) lexicographically.
+struct SampleSel {
+    const uint64_t* key;
+    uint64_t eT;
+    int64_t jT;
+    int64_t n;
+};
+
 template <int W>
 __device__ __forceinline__ bool compact_keep(int mode, const FilterDev* f, double eps, uint64_t seed, int64_t i,
-                                             const Key<W>& k, double& c) {
+                                             const Key<W>& k, double& c, const SampleSel& ss) {
     if (mode == 0) return !should_drop<W>(f, k, fabs(c));
+    if (mode == 2) {
+        const uint64_t e = ss.key[i];
+        return e > ss.eT || (e == ss.eT && ss.n - 1 - i >= ss.jT);
+    }
     const double ac = fabs(c);
     if (ac < eps) {
         if (!(counter_uniform(seed, (uint64_t)i) < ac / eps)) return false;
@@ -527,7 +540,7 @@ __global__ void __launch_bounds__(kBlock) kw_compact(const uint64_t* __restrict_
                                                      const FilterDev* __restrict__ f, int mode, double eps,
                                                      uint64_t seed, int count_only, int64_t* __restrict__ bcount,
                                                      const int64_t* __restrict__ boff, uint64_t* __restrict__ Oz,
-                                                     uint64_t* __restrict__ Ox, double* __restrict__ Oc) {
+                                                     uint64_t* __restrict__ Ox, double* __restrict__ Oc, SampleSel ss) {
     __shared__ int s_w[4];
     const int64_t base = (int64_t)blockIdx.x * kRB;
     const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
@@ -541,7 +554,7 @@ __global__ void __launch_bounds__(kBlock) kw_compact(const uint64_t* __restrict_
         if (i < n) {
             k = load_key<W>(Iz, Ix, i);
             c = Ic[i];
-            keep = compact_keep<W>(mode, f, eps, seed, i, k, c);
+            keep = compact_keep<W>(mode, f, eps, seed, i, k, c, ss);
         }
         const uint64_t mask = __ballot(keep);
         const int before = __popcll(mask & (lane ? (~0ull >> (64 - lane)) : 0ull));
@@ -562,6 +575,47 @@ __global__ void __launch_bounds__(kBlock) kw_compact(const uint64_t* __restrict_
         __syncthreads();
     }
     if (count_only && tid == 0) bcount[blockIdx.x] = blockcnt;
+}
+
+// StochasticSamplingTruncation keys (spv_evolve.jl:69-99): term i gets
+// log(u_i) / |c_i|^2, which orders the terms as u_i^(1/|c_i|^2) does (zero
+// weight: -inf), encoded as an unsigned integer with the same order.
+__global__ void __launch_bounds__(kBlock) kw_sample_keys(const double* __restrict__ C, int64_t n, uint64_t seed,
+                                                         uint64_t* __restrict__ key) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        const double w = C[i] * C[i];
+        const double u = counter_uniform(seed, (uint64_t)i);
+        const double k = w > 0.0 ? log(u) / w : -INFINITY;
+        const uint64_t b = (uint64_t)__double_as_longlong(k);
+        key[i] = (b >> 63) ? ~b : (b | (1ull << 63));
+    }
+}
+
+// Byte histogram for the radix select over (key, n - 1 - i): stage 0 bins
+// key byte `shift` of the keys matching `prefix` on `pmask`; stage 1 bins byte
+// `shift` of n - 1 - i among the terms whose key equals eT.
+__global__ void __launch_bounds__(kBlock) kq_hist_sel(const uint64_t* __restrict__ key, int64_t n, int stage,
+                                                      uint64_t eT, int shift, uint64_t prefix, uint64_t pmask,
+                                                      unsigned long long* __restrict__ hist) {
+    __shared__ int s_h[kRadix];
+    for (int i = threadIdx.x; i < kRadix; i += kBlock) s_h[i] = 0;
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t e = key[i];
+        if (stage == 0) {
+            if ((e & pmask) == prefix) atomicAdd(&s_h[(int)((e >> shift) & 0xffull)], 1);
+        } else if (e == eT) {
+            const uint64_t j = (uint64_t)(n - 1 - i);
+            if ((j & pmask) == prefix) atomicAdd(&s_h[(int)((j >> shift) & 0xffull)], 1);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kRadix; i += kBlock)
+        if (s_h[i]) atomicAdd(hist + i, (unsigned long long)s_h[i]);
+}
+
+__global__ void __launch_bounds__(kBlock) kw_scale(double* __restrict__ C, int64_t n, double f) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) C[i] *= f;
 }
 
 // <ket|O|ket> partial sums over [A (na) ; B (nb)] (live then pending appends,

@@ -16,5 +16,6 @@ from ._lib import ArgumentError, DimensionMismatch, PPCounters, PPError, PPStrat
 from .spv import (SparsePauliVector, bit_flip_channel, bit_flip_channel_, bit_phase_flip_channel,
                   bit_phase_flip_channel_, dephasing_channel, dephasing_channel_, depolarizing_channel,
                   depolarizing_channel_, depolarizing_p_for_weight_decay, evolve, evolve_, expectation_value,
+                  inner_product, isapprox, norm,
                   pauli_channel, pauli_channel_, phase_flip_channel_, stochastic_clip_, truncate_)
 from .batch import SparsePauliBatch

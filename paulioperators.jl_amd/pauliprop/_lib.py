@@ -17,7 +17,8 @@ LIB_PATH = os.path.join(PKG_DIR, "_build", "libpauliprop.so")
 PP_OK, PP_EARG, PP_ENOMEM, PP_EHIP, PP_EOVERFLOW, PP_EUNSUPPORTED = range(6)
 
 EXPORTS = ("pp_create", "pp_destroy", "pp_upload", "pp_download", "pp_size", "pp_capacities", "pp_evolve",
-           "pp_evolve_ex", "pp_truncate", "pp_evolve_single", "pp_compact", "pp_scale_by_popcount", "pp_expectation_ket", "pp_check",
+           "pp_evolve_ex", "pp_truncate", "pp_evolve_single", "pp_compact", "pp_lookup", "pp_norm",
+           "pp_inner_product", "pp_diff_norm", "pp_scale_by_popcount", "pp_expectation_ket", "pp_check",
            "pp_batch_create", "pp_batch_destroy", "pp_batch_words", "pp_batch_upload", "pp_batch_download",
            "pp_batch_sizes", "pp_batch_evolve", "pp_batch_scale_by_popcount", "pp_batch_expectation_ket",
            "pp_batch_reserve", "pp_batch_set_tiles", "pp_batch_set_timing", "pp_batch_merge_time",
@@ -39,7 +40,7 @@ class PPCounters(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
-PP_OP_FILTER, PP_OP_ADAPTIVE, PP_OP_STOCHASTIC_COEFF = range(3)
+PP_OP_FILTER, PP_OP_ADAPTIVE, PP_OP_STOCHASTIC_COEFF, PP_OP_STOCHASTIC_SAMPLING = range(4)
 
 
 class PPStrategyOp(ctypes.Structure):
@@ -89,6 +90,10 @@ def lib():
                                P(PPFilter), u64p, dp, P(PPCounters)]
     L.pp_truncate.argtypes = [vp, P(PPStrategyOp), ctypes.c_int, u64p, dp]
     L.pp_evolve_single.argtypes = [vp, u64p, u64p, ctypes.c_double, ctypes.c_double]
+    L.pp_lookup.argtypes = [vp, u64p, u64p, i64, dp, P(ctypes.c_int32)]
+    L.pp_norm.argtypes = [vp, ctypes.c_double, dp]
+    L.pp_inner_product.argtypes = [vp, vp, dp]
+    L.pp_diff_norm.argtypes = [vp, vp, dp]
     L.pp_compact.argtypes = [vp, P(PPFilter)]
     L.pp_scale_by_popcount.argtypes = [vp, ctypes.c_int, u64p, dp, dp, dp]
     L.pp_expectation_ket.argtypes = [vp, u64p, dp]

@@ -9,7 +9,6 @@ from __future__ import annotations
 
 import ctypes
 import math
-from bisect import bisect_left
 from typing import Iterable, List, Optional, Sequence, Tuple, Union
 
 import numpy as np
@@ -52,9 +51,13 @@ def _strategy_ops(strategy):
             ops[i].kind = _lib.PP_OP_ADAPTIVE
             ops[i].max_terms = st[1]
             ops[i].value = st[2]
-        else:
+        elif st[0] == "stochastic_coeff":
             ops[i].kind = _lib.PP_OP_STOCHASTIC_COEFF
             ops[i].value = st[1]
+            ops[i].seed = _rng_seed(st[2])
+        else:
+            ops[i].kind = _lib.PP_OP_STOCHASTIC_SAMPLING
+            ops[i].max_terms = st[1]
             ops[i].seed = _rng_seed(st[2])
     return ops, len(steps)
 
@@ -184,23 +187,55 @@ class SparsePauliVector:
             raise RuntimeError("live keys not strictly increasing")
         return True
 
+    def _key_arrays(self, keys):
+        z = np.zeros((max(len(keys), 1), self.W), dtype=np.uint64)
+        x = np.zeros((max(len(keys), 1), self.W), dtype=np.uint64)
+        for i, p in enumerate(keys):
+            pz, px = (p.z, p.x) if isinstance(p, PauliBasis) else p
+            z[i] = _pack(pz, self.W)
+            x[i] = _pack(px, self.W)
+        return z, x
+
+    def lookup(self, keys):
+        """Coefficients of many keys at once by device binary search
+        (type_SparsePauliVector.jl:286-299 _find): (values, found) arrays."""
+        keys = list(keys)
+        z, x = self._key_arrays(keys)
+        vals = np.zeros(max(len(keys), 1), dtype=np.float64)
+        found = np.zeros(max(len(keys), 1), dtype=np.int32)
+        check(_lib.lib().pp_lookup(self._h, u64p(z), u64p(x), len(keys), dp(vals),
+                                   found.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return vals[:len(keys)], found[:len(keys)].astype(bool)
+
     def __getitem__(self, p: PauliBasis) -> float:
-        """Dict idiom (type_SparsePauliVector.jl:307-311): KeyError when absent."""
-        keys = [(z, x) for z, x, _ in self.terms()]
-        cs = [c for _, _, c in self.terms()]
-        i = bisect_left(keys, (p.z, p.x))
-        if i < len(keys) and keys[i] == (p.z, p.x):
-            return cs[i]
-        raise KeyError(p)
+        """``getindex`` (type_SparsePauliVector.jl:307-311): KeyError when absent."""
+        if isinstance(p, str):
+            p = PauliBasis.from_string(p)
+        v, f = self.lookup([p])
+        if not f[0]:
+            raise KeyError(p)
+        return float(v[0])
 
     def get(self, p: PauliBasis, default=None):
-        try:
-            return self[p]
-        except KeyError:
-            return default
+        """``get(v, p, default)`` (type_SparsePauliVector.jl:301-305)."""
+        v, f = self.lookup([p])
+        return float(v[0]) if f[0] else default
+
+    def haskey(self, p: PauliBasis) -> bool:
+        return bool(self.lookup([p])[1][0])
 
     def __contains__(self, p: PauliBasis) -> bool:
-        return self.get(p) is not None
+        return self.haskey(p)
+
+    def norm(self, p: float = 2) -> float:
+        """``norm(v, p)`` (spv_ops.jl:57-84)."""
+        out = ctypes.c_double()
+        check(_lib.lib().pp_norm(self._h, float(p), ctypes.byref(out)))
+        return out.value
+
+    def tr(self) -> float:
+        """``tr(v)`` (spv_ops.jl:52-55): the identity coefficient times 2^N."""
+        return (self.get((0, 0), 0.0)) * 2.0 ** self.N
 
     # --- evolution -----------------------------------------------------------------------
     def _gen_arrays(self, gens: Sequence[PauliBasis]):
@@ -391,6 +426,29 @@ def evolve(v: SparsePauliVector, generators, angles, **kw):
 
 def truncate_(v: SparsePauliVector, strategy, corr=NoCorrection()):
     return v.truncate_(strategy, corr)
+
+
+def norm(v: SparsePauliVector, p: float = 2) -> float:
+    return v.norm(p)
+
+
+def inner_product(v1: SparsePauliVector, v2: SparsePauliVector) -> float:
+    """``inner_product(v1, v2)`` (spv_ops.jl:378-396): Σ over shared keys of c1 c2."""
+    if v1.N != v2.N:
+        raise DimensionMismatch("qubit counts differ")
+    out = ctypes.c_double()
+    check(_lib.lib().pp_inner_product(v1._h, v2._h, ctypes.byref(out)))
+    return out.value
+
+
+def isapprox(v1: SparsePauliVector, v2: SparsePauliVector, atol: float = 1e-14, rtol: float = 0.0) -> bool:
+    """``isapprox(v1, v2; atol, rtol)`` (spv_ops.jl:88-107): the L2 norm of the
+    difference over the union of keys against atol + rtol max(norm)."""
+    if v1.N != v2.N:
+        raise DimensionMismatch("qubit counts differ")
+    out = ctypes.c_double()
+    check(_lib.lib().pp_diff_norm(v1._h, v2._h, ctypes.byref(out)))
+    return out.value <= atol + rtol * max(v1.norm(), v2.norm())
 
 
 def stochastic_clip_(v: SparsePauliVector, epsilon: float, rng=None):

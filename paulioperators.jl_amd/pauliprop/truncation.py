@@ -99,14 +99,14 @@ class AdaptiveTruncation(TruncationStrategy):
 
 @dataclass(frozen=True)
 class StochasticCoeffTruncation(TruncationStrategy):
-    """truncation.jl:113-117 (Russian roulette).  Host-side only here."""
+    """truncation.jl:113-117 (Russian roulette): stochastic_clip! on the device."""
     epsilon: float
     rng: Any = None
 
 
 @dataclass(frozen=True)
 class StochasticSamplingTruncation(TruncationStrategy):
-    """truncation.jl:125-129.  Host-side only here."""
+    """truncation.jl:125-129: exponential-key sampling of n_keep terms on the device."""
     n_keep: int
     rng: Any = None
 
@@ -213,13 +213,15 @@ def strategy_steps(s: TruncationStrategy) -> list:
     """A strategy as the ordered steps of its _apply! (spv_evolve.jl:59-119;
     CompositeTruncation applies its members in order, truncation.jl:236-243):
     ("filter", PPFilter) | ("adaptive", max_terms, min_thresh) |
-    ("stochastic_coeff", epsilon, rng)."""
+    ("stochastic_coeff", epsilon, rng) | ("stochastic_sampling", n_keep, rng)."""
     if is_compilable(s):
         return [("filter", compile_filter(s))]
     if isinstance(s, AdaptiveTruncation):
         return [("adaptive", int(s.max_terms), float(s.min_thresh))]
     if isinstance(s, StochasticCoeffTruncation):
         return [("stochastic_coeff", float(s.epsilon), s.rng)]
+    if isinstance(s, StochasticSamplingTruncation):
+        return [("stochastic_sampling", int(s.n_keep), s.rng)]
     if isinstance(s, CompositeTruncation):
         out = []
         for t in s.strategies:

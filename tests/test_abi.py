@@ -55,5 +55,7 @@ def test_strategy_op_struct_layout():
     assert _lib.PPStrategyOp.max_terms.offset == 72
     assert _lib.PPStrategyOp.seed.offset == 88
     hdr = open(os.path.join(ROOT, "include", "pauliprop.h")).read()
-    assert re.search(r"PP_OP_FILTER = 0, PP_OP_ADAPTIVE = 1, PP_OP_STOCHASTIC_COEFF = 2", hdr)
-    assert (_lib.PP_OP_FILTER, _lib.PP_OP_ADAPTIVE, _lib.PP_OP_STOCHASTIC_COEFF) == (0, 1, 2)
+    assert re.search(r"PP_OP_FILTER = 0, PP_OP_ADAPTIVE = 1, PP_OP_STOCHASTIC_COEFF = 2, "
+                     r"PP_OP_STOCHASTIC_SAMPLING = 3", hdr)
+    assert (_lib.PP_OP_FILTER, _lib.PP_OP_ADAPTIVE, _lib.PP_OP_STOCHASTIC_COEFF,
+            _lib.PP_OP_STOCHASTIC_SAMPLING) == (0, 1, 2, 3)

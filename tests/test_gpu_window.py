@@ -269,3 +269,35 @@ def test_stochastic_strategy_in_evolve_runs_and_keeps_order():
     assert v.check()
     cs = [abs(c) for _, _, c in v.terms()]
     assert min(cs) >= 1e-4 * (1 - 1e-15)
+
+
+def test_stochastic_sampling_truncation():
+    """StochasticSamplingTruncation (spv_evolve.jl:69-99): exactly n_keep terms
+    survive, in key order, each with its coefficient times one common factor
+    that restores Σ|c|^2; large-|c| terms survive far more often than small
+    ones (keys u^(1/|c|^2)); below n_keep nothing changes; same seed, same
+    result.  Julia's RNG stream is not reproduced (individual draws unpinned)."""
+    from pauliprop import StochasticSamplingTruncation
+    rng = random.Random(5)
+    N = 50
+    d = {}
+    while len(d) < 20000:
+        d[(rng.getrandbits(N), rng.getrandbits(N))] = rng.choice([1.0, 1e-3]) * rng.choice([1, -1])
+    items = sorted(d.items())
+    v = SparsePauliVector(N, items)
+    v.truncate_(StochasticSamplingTruncation(5000, 42))
+    t = v.terms()
+    assert len(t) == 5000 and v.check()
+    norm0 = math.fsum(c * c for _, c in items)
+    scale = math.sqrt(norm0 / math.fsum(d[(z, x)] ** 2 for z, x, _ in t))
+    for z, x, c in t:
+        assert abs(c - d[(z, x)] * scale) <= 1e-12 * abs(c)
+    big = sum(1 for z, x, _ in t if abs(d[(z, x)]) == 1.0)
+    nbig = sum(1 for _, c in items if abs(c) == 1.0)
+    assert big == min(nbig, 5000) or big > 0.99 * min(nbig, 5000)
+    v2 = SparsePauliVector(N, items)
+    v2.truncate_(StochasticSamplingTruncation(5000, 42))
+    assert v2.terms() == t
+    v3 = SparsePauliVector(N, items)
+    v3.truncate_(StochasticSamplingTruncation(30000, 1))
+    assert len(v3) == 20000 and v3.terms() == SparsePauliVector(N, items).terms()
