@@ -13,8 +13,11 @@ all of them).
 value = Σ over all rotations of the live terms entering the rotation / wall
 time of the step.  The JSON line also carries the merge kernel's roofline
 (algorithmic bytes per launch = 40 B x (terms in + children + terms out)
-over its HIP-event time) and the CPU baseline: the single-threaded oracle
-(a C restatement of the reference algorithm) on the same sweep.
+over its HIP-event time; this sweep is latency-bound), the same kernel's
+roofline where it is bandwidth-bound (256 circuits of a 36-qubit chain with
+up to ~1e6 live terms each, 24 B per term), and the CPU baseline: the
+single-threaded oracle (a C restatement of the reference algorithm) on the
+same sweep.
 """
 import argparse
 import json
@@ -58,6 +61,30 @@ def sweep_gpu(workloads, timing=True, graphs=False):
     dt = time.perf_counter() - t0
     tm, nm = bt.merge_time()
     return dt, tot, evs, tm, nm
+
+
+def merge_bandwidth_regime(count=256):
+    """The merge kernel where it is bandwidth-bound: one batch of `count`
+    circuits of the 36-qubit chain (10 layers, RZZ/RX angles from the seed-99
+    stream, |c| <= 1e-6; up to ~1e6 live terms each).  Returns (algorithmic
+    GB/s, average launch µs, bytes per launch, launches) from HIP events
+    around every merge launch and the device merge counters (24 B per term)."""
+    from pauliprop import SparsePauliBatch, workloads as wl
+    from pauliprop.surrogate import _chain36_layer_arrays
+    from pauliprop.truncation import CoeffTruncation
+    th = [list(t) for t in wl.surrogate_thetas(count, seed=99)]
+    obs = wl.surrogate_chain36_layers([0.0] * 20).observable
+    bt = SparsePauliBatch([36] * count, [[((p.z, p.x), c) for p, c in obs]] * count)
+    packed = [_chain36_layer_arrays(th, li, bt.W) for li in range(10)]
+    bt.set_timing(True)
+    mi = mo = 0
+    for li in range(10):
+        for c in bt.evolve_packed(packed[li], truncation=CoeffTruncation(1e-6), counters=True):
+            mi += c.merge_in
+            mo += c.merge_out
+    t, n = bt.merge_time()
+    b = 24 * (mi + mo)
+    return b / t / 1e9, 1e6 * t / n, b / n, n
 
 
 def sweep_cpu(workloads, budget_s):
@@ -146,6 +173,10 @@ def main():
         with open(tpath) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
 
+    log("merge kernel in the bandwidth-bound regime (36-qubit chain, 256 circuits)")
+    bw_gbs, bw_us, bw_bytes, bw_n = merge_bandwidth_regime()
+    log(f"  {bw_gbs:.0f} GB/s algorithmic, {bw_us:.1f} us per launch over {bw_n} launches")
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle, 1 thread)")
@@ -178,6 +209,10 @@ def main():
                      "kernel": "kb_merge<2>", "launches": n_merge,
                      "avg_launch_us": 1e6 * t_merge / max(n_merge, 1),
                      "bytes_per_launch": merge_bytes / max(n_merge, 1)},
+        "roofline_bandwidth_regime": {"bound": "hbm", "achieved": bw_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                      "frac": bw_gbs / HBM_PEAK_GBS, "kernel": "kb_merge<1,2048>",
+                                      "workload": "36-qubit chain, 10 layers, 256 circuits in one batch, |c|<=1e-6",
+                                      "launches": bw_n, "avg_launch_us": bw_us, "bytes_per_launch": bw_bytes},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -127,6 +127,14 @@ int pp_truncate(pp_handle* h, const pp_strategy_op* ops, int nops, const uint64_
  * query keys (W words each in z, x), found[i] = 1 and c[i] = the coefficient
  * when the key is live, else found[i] = 0 and c[i] = 0.  Device binary search. */
 int pp_lookup(pp_handle* h, const uint64_t* z, const uint64_t* x, int64_t count, double* c, int32_t* found);
+/* setindex! for `count` keys (type_SparsePauliVector.jl:315-335): present
+ * keys get the new coefficient, absent keys are inserted in key order (the
+ * reference's live-capacity growth is applied per inserted key); for repeated
+ * keys the last value wins. */
+int pp_setindex(pp_handle* h, const uint64_t* z, const uint64_t* x, const double* c, int64_t count);
+/* delete! for `count` keys (type_SparsePauliVector.jl:345-358); absent keys
+ * are ignored; order-preserving. */
+int pp_delete(pp_handle* h, const uint64_t* z, const uint64_t* x, int64_t count);
 /* norm(v, p) (spv_ops.jl:57-84): p = 2, 1, Inf or any p > 0. */
 int pp_norm(pp_handle* h, double p, double* out);
 /* inner_product(a, b) (spv_ops.jl:378-396): Σ over shared keys of c_a c_b. */

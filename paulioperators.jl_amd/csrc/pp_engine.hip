@@ -268,6 +268,14 @@ struct Batch {
     }
 
     size_t kb() const { return sizeof(uint64_t) * (size_t)W; }
+
+    // Whether `terms` more terms of this word size fit in half of the free
+    // device memory (growth beyond what is needed is only taken when cheap).
+    bool fits_device(int64_t terms) const {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) return false;
+        return (double)terms * (double)(2 * kb() + 8) < 0.5 * (double)fr;
+    }
     // Tile shapes of the next launches (0 = choose from the live counts).
     int tile_req = 0, mtile_req = 0;
     int tile = 2048, mtile = 1024;
@@ -618,8 +626,14 @@ struct Batch {
         int maxlev = 1;
         for (const StepDev& s : steps) maxlev = std::max(maxlev, (int)s.g.nlev);
         for (int b = 0; b < B; ++b) {
-            if (ib[b].maxlev < maxlev) {
-                const int64_t cap = std::min(ib[b].L[0].cap, ib[b].L[1].cap);
+            const int64_t cap = std::min(ib[b].L[0].cap, ib[b].L[1].cap);
+            const int64_t n = n_known[b];
+            // headroom for growth within the call: a vector with less than 2x
+            // room is grown to 4x (children to 2x) when device memory allows,
+            // so that a stall and replay is the exception
+            if (en[b] && cap < 2 * n && fits_device(3 * 4 * n)) {
+                ensure_caps(b, 4 * n, std::max<int64_t>(ib[b].C.cap, 2 * n), maxlev, n, h_state[b].cur);
+            } else if (ib[b].maxlev < maxlev) {
                 ensure_caps(b, cap, ib[b].C.cap, maxlev, 0, h_state[b].cur);
             }
         }
@@ -642,9 +656,13 @@ struct Batch {
                 SegState& st = h_state[b];
                 const int64_t cap = std::min(ib[b].L[0].cap, ib[b].L[1].cap);
                 const int64_t n = st.n[st.cur];
-                const int64_t need = st.need + st.need / 2 + 1024;
+                // a vector that outgrew its buffers mid-sequence usually keeps
+                // growing: regrow 4x (children and live together) so that
+                // the replay rarely stalls again
+                const int64_t big = 4 * st.need + 1024;
+                const int64_t need = fits_device(3 * big) ? big : st.need + st.need / 2 + 1024;
                 if (st.stalled == 1) ensure_caps(b, std::max(cap, n + need), need, maxlev, n, st.cur);
-                else ensure_caps(b, need, ib[b].C.cap, maxlev, n, st.cur);
+                else ensure_caps(b, need, std::max<int64_t>(ib[b].C.cap, need / 2), maxlev, n, st.cur);
                 first = std::min(first, st.stall_step);
             }
             // replay table: only the stalled vectors, each from its own stall
@@ -1189,12 +1207,165 @@ struct Batch {
         HIPCHK(hipMemcpyAsync(qx, x, (size_t)count * Wt * 8, hipMemcpyHostToDevice, stream));
         const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(count, kBlock), 4096));
         hipLaunchKernelGGL(ko_lookup<Wt>, dim3(grid), dim3(kBlock), 0, stream, L.z, L.x, L.c, n, qz, qx, count, dout,
-                           dfound);
+                           dfound, (int64_t*)nullptr);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(out, dout, (size_t)count * 8, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipMemcpyAsync(found, dfound, (size_t)count * 4, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
         dfree(qz); dfree(qx); dfree(dout); dfree(dfound);
+    }
+
+    // Device indices of `count` keys in the live set of vector 0 (-1: absent).
+    template <int Wt>
+    std::vector<int64_t> find_indices(const uint64_t* z, const uint64_t* x, int64_t count, int64_t* d_idx_out) {
+        int64_t n = 0;
+        const DevSet& L = live0(n);
+        uint64_t* qz = dalloc<uint64_t>((size_t)count * Wt);
+        uint64_t* qx = dalloc<uint64_t>((size_t)count * Wt);
+        double* dout = dalloc<double>((size_t)count);
+        int32_t* dfound = dalloc<int32_t>((size_t)count);
+        HIPCHK(hipMemcpyAsync(qz, z, (size_t)count * Wt * 8, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(qx, x, (size_t)count * Wt * 8, hipMemcpyHostToDevice, stream));
+        const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(count, kBlock), 4096));
+        hipLaunchKernelGGL(ko_lookup<Wt>, dim3(grid), dim3(kBlock), 0, stream, L.z, L.x, L.c, n, qz, qx, count, dout,
+                           dfound, d_idx_out);
+        HIPCHK(hipGetLastError());
+        std::vector<int64_t> idx((size_t)count);
+        HIPCHK(hipMemcpyAsync(idx.data(), d_idx_out, (size_t)count * 8, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        dfree(qz); dfree(qx); dfree(dout); dfree(dfound);
+        return idx;
+    }
+
+    void commit_live0(int cur, int64_t n, int64_t live_cap) {
+        SegState& st = h_state[0];
+        st.cur = cur;
+        st.n[cur] = n;
+        st.n[cur ^ 1] = 0;
+        st.flip = 0;
+        st.live_cap_v = live_cap;
+        n_known[0] = n;
+        put_state();
+        upload_desc();
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    // setindex! for many keys (type_SparsePauliVector.jl:315-335); the last
+    // of repeated keys wins, as with sequential calls.
+    template <int Wt>
+    void setindex(const uint64_t* z, const uint64_t* x, const double* c, int64_t count) {
+        if (count <= 0) return;
+        // last occurrence of each key
+        std::vector<int64_t> order((size_t)count);
+        std::iota(order.begin(), order.end(), 0);
+        auto kless = [&](int64_t a, int64_t b) {
+            for (int w = Wt - 1; w >= 0; --w)
+                if (z[a * Wt + w] != z[b * Wt + w]) return z[a * Wt + w] < z[b * Wt + w];
+            for (int w = Wt - 1; w >= 0; --w)
+                if (x[a * Wt + w] != x[b * Wt + w]) return x[a * Wt + w] < x[b * Wt + w];
+            return false;
+        };
+        std::stable_sort(order.begin(), order.end(), kless);
+        std::vector<int64_t> uniq;
+        for (size_t i = 0; i < order.size(); ++i)
+            if (i + 1 == order.size() || kless(order[i], order[i + 1])) uniq.push_back(order[i]);
+        const int64_t u = (int64_t)uniq.size();
+        std::vector<uint64_t> uz((size_t)u * Wt), ux((size_t)u * Wt);
+        std::vector<double> uc((size_t)u);
+        for (int64_t i = 0; i < u; ++i) {
+            for (int w = 0; w < Wt; ++w) {
+                uz[i * Wt + w] = z[uniq[i] * Wt + w];
+                ux[i * Wt + w] = x[uniq[i] * Wt + w];
+            }
+            uc[i] = c[uniq[i]];
+        }
+        int64_t* didx = dalloc<int64_t>((size_t)u);
+        const std::vector<int64_t> idx = find_indices<Wt>(uz.data(), ux.data(), u, didx);
+        dfree(didx);
+        std::vector<int64_t> fi;
+        std::vector<double> fv;
+        std::vector<uint64_t> nz, nx;
+        std::vector<double> nc;
+        for (int64_t i = 0; i < u; ++i) {
+            if (idx[i] >= 0) {
+                fi.push_back(idx[i]);
+                fv.push_back(uc[i]);
+            } else {
+                for (int w = 0; w < Wt; ++w) {
+                    nz.push_back(uz[i * Wt + w]);
+                    nx.push_back(ux[i * Wt + w]);
+                }
+                nc.push_back(uc[i]);
+            }
+        }
+        SegState& st = h_state[0];
+        int cur = st.cur;
+        int64_t n = st.n[cur];
+        int64_t live_cap = st.live_cap_v;
+        if (!fi.empty()) {
+            int64_t* di = dalloc<int64_t>(fi.size());
+            double* dv = dalloc<double>(fv.size());
+            HIPCHK(hipMemcpyAsync(di, fi.data(), fi.size() * 8, hipMemcpyHostToDevice, stream));
+            HIPCHK(hipMemcpyAsync(dv, fv.data(), fv.size() * 8, hipMemcpyHostToDevice, stream));
+            const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv((int64_t)fi.size(), kBlock), 4096));
+            hipLaunchKernelGGL(ko_scatter, dim3(grid), dim3(kBlock), 0, stream, ib[0].L[cur].c, (const int64_t*)di,
+                               (const double*)dv, (int64_t)fi.size());
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(stream));
+            dfree(di);
+            dfree(dv);
+        }
+        const int64_t m = (int64_t)nc.size();
+        if (m > 0) {
+            // the reference inserts one key at a time, growing when full
+            for (int64_t i = 0; i < m; ++i) {
+                if (n + i == live_cap) live_cap = std::max(2 * live_cap, n + i + 1);
+            }
+            win_init();
+            WinBuf& w = win;
+            grow_set(w.A, m, 0);
+            HIPCHK(hipMemcpyAsync(w.A.z, nz.data(), (size_t)m * Wt * 8, hipMemcpyHostToDevice, stream));
+            HIPCHK(hipMemcpyAsync(w.A.x, nx.data(), (size_t)m * Wt * 8, hipMemcpyHostToDevice, stream));
+            HIPCHK(hipMemcpyAsync(w.A.c, nc.data(), (size_t)m * 8, hipMemcpyHostToDevice, stream));
+            DevSet* S = win_sort<Wt>(m);
+            win_filter(0, nullptr);
+            n = win_merge<Wt>(n, cur, S, m, 0);
+            cur ^= 1;
+        }
+        commit_live0(cur, n, live_cap);
+    }
+
+    // delete! for many keys (type_SparsePauliVector.jl:345-358); absent keys
+    // are ignored.
+    template <int Wt>
+    void delete_keys(const uint64_t* z, const uint64_t* x, int64_t count) {
+        if (count <= 0) return;
+        int64_t* didx = dalloc<int64_t>((size_t)count);
+        const std::vector<int64_t> idx = find_indices<Wt>(z, x, count, didx);
+        std::vector<int64_t> hit;
+        for (int64_t v : idx)
+            if (v >= 0) hit.push_back(v);
+        dfree(didx);
+        SegState& st = h_state[0];
+        int cur = st.cur;
+        int64_t n = st.n[cur];
+        if (!hit.empty()) {
+            uint8_t* mark = dalloc<uint8_t>((size_t)n);
+            int64_t* di = dalloc<int64_t>(hit.size());
+            HIPCHK(hipMemsetAsync(mark, 0, (size_t)n, stream));
+            HIPCHK(hipMemcpyAsync(di, hit.data(), hit.size() * 8, hipMemcpyHostToDevice, stream));
+            const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv((int64_t)hit.size(), kBlock), 4096));
+            hipLaunchKernelGGL(ko_mark, dim3(grid), dim3(kBlock), 0, stream, mark, (const int64_t*)di, (int64_t)hit.size());
+            HIPCHK(hipGetLastError());
+            win_init();
+            SampleSel ss{};
+            ss.mark = mark;
+            n = win_compact<Wt>(n, cur, 3, 0.0, 0, ss);
+            HIPCHK(hipStreamSynchronize(stream));
+            dfree(mark);
+            dfree(di);
+        }
+        commit_live0(cur, n, st.live_cap_v);
     }
 
     double reduce_partials(double2* part, unsigned grid, bool is_max) {
@@ -1570,6 +1741,20 @@ int pp_lookup(pp_handle* h, const uint64_t* z, const uint64_t* x, int64_t count,
     if (!h || count < 0 || (count > 0 && (!z || !x || !c || !found))) return fail(PP_EARG, "bad arguments");
     API_BEGIN
     dispatch_w(h->bt.W, [&](auto wc) { h->bt.lookup<decltype(wc)::value>(z, x, count, c, found); });
+    API_END
+}
+
+int pp_setindex(pp_handle* h, const uint64_t* z, const uint64_t* x, const double* c, int64_t count) {
+    if (!h || count < 0 || (count > 0 && (!z || !x || !c))) return fail(PP_EARG, "bad arguments");
+    API_BEGIN
+    dispatch_w(h->bt.W, [&](auto wc) { h->bt.setindex<decltype(wc)::value>(z, x, c, count); });
+    API_END
+}
+
+int pp_delete(pp_handle* h, const uint64_t* z, const uint64_t* x, int64_t count) {
+    if (!h || count < 0 || (count > 0 && (!z || !x))) return fail(PP_EARG, "bad arguments");
+    API_BEGIN
+    dispatch_w(h->bt.W, [&](auto wc) { h->bt.delete_keys<decltype(wc)::value>(z, x, count); });
     API_END
 }
 

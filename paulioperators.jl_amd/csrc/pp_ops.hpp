@@ -33,12 +33,27 @@ __global__ void __launch_bounds__(kBlock) ko_lookup(const uint64_t* __restrict__
                                                     const double* __restrict__ C, int64_t n,
                                                     const uint64_t* __restrict__ Qz, const uint64_t* __restrict__ Qx,
                                                     int64_t count, double* __restrict__ out,
-                                                    int32_t* __restrict__ found) {
+                                                    int32_t* __restrict__ found, int64_t* __restrict__ idx) {
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < count; i += (int64_t)gridDim.x * kBlock) {
         const int64_t j = find_key<W>(Z, X, n, load_key<W>(Qz, Qx, i));
         found[i] = j >= 0 ? 1 : 0;
         out[i] = j >= 0 ? C[j] : 0.0;
+        if (idx) idx[i] = j;
     }
+}
+
+// setindex! on present keys: C[idx[i]] = v[i]
+__global__ void __launch_bounds__(kBlock) ko_scatter(double* __restrict__ C, const int64_t* __restrict__ idx,
+                                                     const double* __restrict__ v, int64_t count) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < count; i += (int64_t)gridDim.x * kBlock)
+        C[idx[i]] = v[i];
+}
+
+// delete!: mark[idx[i]] = 1
+__global__ void __launch_bounds__(kBlock) ko_mark(uint8_t* __restrict__ mark, const int64_t* __restrict__ idx,
+                                                  int64_t count) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < count; i += (int64_t)gridDim.x * kBlock)
+        mark[idx[i]] = 1;
 }
 
 __device__ __forceinline__ void block_dd_reduce(double h, double l, double2* __restrict__ partial) {

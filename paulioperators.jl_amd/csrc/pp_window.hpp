@@ -516,6 +516,7 @@ struct SampleSel {
     uint64_t eT;
     int64_t jT;
     int64_t n;
+    const uint8_t* mark;  // mode 3: terms to delete
 };
 
 template <int W>
@@ -526,6 +527,7 @@ __device__ __forceinline__ bool compact_keep(int mode, const FilterDev* f, doubl
         const uint64_t e = ss.key[i];
         return e > ss.eT || (e == ss.eT && ss.n - 1 - i >= ss.jT);
     }
+    if (mode == 3) return ss.mark[i] == 0;
     const double ac = fabs(c);
     if (ac < eps) {
         if (!(counter_uniform(seed, (uint64_t)i) < ac / eps)) return false;

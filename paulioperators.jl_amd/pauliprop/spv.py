@@ -227,6 +227,29 @@ class SparsePauliVector:
     def __contains__(self, p: PauliBasis) -> bool:
         return self.haskey(p)
 
+    def setindex_(self, items):
+        """``v[p] = c`` for many (key, c) pairs (type_SparsePauliVector.jl:315-335):
+        present keys take the new value, absent keys are inserted; for a key
+        given twice the last value wins."""
+        items = list(items.items() if isinstance(items, dict) else items)
+        z, x = self._key_arrays([k for k, _ in items])
+        c = np.array([float(v) for _, v in items] or [0.0], dtype=np.float64)
+        check(_lib.lib().pp_setindex(self._h, u64p(z), u64p(x), dp(c), len(items)))
+        return self
+
+    def __setitem__(self, p: PauliBasis, val: float):
+        self.setindex_([(p, val)])
+
+    def delete_(self, keys):
+        """``delete!(v, p)`` for many keys (type_SparsePauliVector.jl:345-358)."""
+        keys = list(keys)
+        z, x = self._key_arrays(keys)
+        check(_lib.lib().pp_delete(self._h, u64p(z), u64p(x), len(keys)))
+        return self
+
+    def __delitem__(self, p: PauliBasis):
+        self.delete_([p])
+
     def norm(self, p: float = 2) -> float:
         """``norm(v, p)`` (spv_ops.jl:57-84)."""
         out = ctypes.c_double()

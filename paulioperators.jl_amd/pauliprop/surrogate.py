@@ -162,7 +162,7 @@ def _chain36_layer_arrays(thetas, li: int, W: int):
     return gz, gx, cv, sv, np.ones((S, B), dtype=np.uint8)
 
 
-def chain36_batch_expectations(thetas: Sequence[Sequence[float]], batch: int = 512,
+def chain36_batch_expectations(thetas: Sequence[Sequence[float]], batch: int = 256,
                                log: Optional[Callable[[str], None]] = None) -> np.ndarray:
     """⟨0|O(θ)|0⟩ for every θ by batched GPU propagation (SparsePauliBatch):
     `batch` circuits per batch, one evolve! per layer per circuit."""

@@ -35,7 +35,7 @@ def run(n_plain: int = 8, n_heldout: int = 4096, epochs: int = 60, n_query: int 
     log(f"plain: {n_plain} in {t_plain:.2f} s")
     # batched exact
     t0 = time.perf_counter()
-    bvals = sg.chain36_batch_expectations(ho[:1024], batch=512)
+    bvals = sg.chain36_batch_expectations(ho[:1024], batch=256)
     t_b = time.perf_counter() - t0
     res["batched_evals_per_s"] = 1024 / t_b
     res["batched_max_abs_err"] = float(np.max(np.abs(bvals - yho[:1024])))
@@ -57,7 +57,7 @@ def run(n_plain: int = 8, n_heldout: int = 4096, epochs: int = 60, n_query: int 
     res["ensemble_rmse_heldout"] = float(np.sqrt(np.mean(err ** 2)))
     res["ensemble_std_median"] = float(np.median(sd))
     # gated evaluation of the held-out θ
-    gate = sg.GatedSurrogate(ens, exact=lambda th: sg.chain36_batch_expectations(th, batch=512), tol=tol)
+    gate = sg.GatedSurrogate(ens, exact=lambda th: sg.chain36_batch_expectations(th, batch=256), tol=tol)
     res["gate_calibration"] = gate.calibrate(tr[-ncal:], ytr[-ncal:])
     t0 = time.perf_counter()
     vals, fall, tm = gate.evaluate(ho)

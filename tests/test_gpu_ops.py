@@ -93,3 +93,50 @@ def test_inner_product_and_isapprox(N):
     va2 = SparsePauliVector(N, list(a.items()))
     assert isapprox(va, va2, atol=0.0)
     assert isapprox(va, vb, rtol=1.0)
+
+
+@pytest.mark.parametrize("N", [16, 100])
+def test_setindex_and_delete(N):
+    """setindex! / delete! (type_SparsePauliVector.jl:315-358) against dict
+    semantics, with the reference's live-capacity growth per inserted key."""
+    rng = random.Random(100 + N)
+    d = rand_vec(N, 3000, rng)
+    v = SparsePauliVector(N, list(d.items()))
+    cap0 = v.capacities()[0]
+    keys = list(d)
+    upd = []
+    for k in rng.sample(keys, 500):
+        upd.append((k, rng.gauss(0, 1)))
+    new = []
+    while len(new) < 7000:
+        k = (rng.getrandbits(N), rng.getrandbits(N))
+        if k not in d:
+            new.append((k, rng.gauss(0, 1)))
+    batch = upd + new + [(new[0][0], 42.0)]  # repeated key: last value wins
+    rng.shuffle(batch)
+    batch.append((new[1][0], -3.0))
+    ref = dict(d)
+    n, cap = len(ref), cap0
+    for k, c in batch:
+        if k not in ref:
+            if n == cap:
+                cap = max(2 * cap, n + 1)
+            n += 1
+        ref[k] = c
+    v.setindex_(batch)
+    assert v.check()
+    got = {(z, x): c for z, x, c in v.terms()}
+    assert got == ref
+    assert v.capacities()[0] == cap
+    p = PauliBasis(N, *keys[0])
+    v[p] = 0.5
+    ref[keys[0]] = 0.5
+    assert v[p] == 0.5
+    dele = rng.sample(list(ref), 1000) + [(rng.getrandbits(N) | 1, 0)]
+    v.delete_(dele)
+    for k in dele:
+        ref.pop(k, None)
+    got = {(z, x): c for z, x, c in v.terms()}
+    assert got == ref and v.check()
+    del v[PauliBasis(N, *next(iter(ref)))]
+    assert len(v) == len(ref) - 1
