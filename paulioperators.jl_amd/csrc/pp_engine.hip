@@ -10,6 +10,7 @@
 // buffers are too small freezes; after the sequence the host regrows its
 // buffers and replays it from the step that stalled.  The single-vector API
 // is a batch of one.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -512,10 +513,17 @@ struct Batch {
         hipLaunchKernelGGL(kb_scan<T>, dim3(B), dim3(1024), 0, stream, d_desc, d_state, dsteps, step_index);
         hipLaunchKernelGGL((kb_bounds<Wt, T>), dim3(blocks(1)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(1));
         hipLaunchKernelGGL((kb_emit<Wt, T>), dim3(blocks(0)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(0));
-        if (timing) HIPCHK(hipEventRecord(next_event(), stream));
-        hipLaunchKernelGGL((kb_merge<Wt, TM>), dim3(blocks(3)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(3),
-                           0, step_index, (const uint32_t*)d_epoch, ep_off);
-        if (timing) HIPCHK(hipEventRecord(next_event(), stream));
+        if (timing) {
+            // events recorded by the launch itself at the kernel's start and
+            // end (not around it), so the interval is the kernel's duration
+            hipEvent_t e0 = next_event();
+            hipEvent_t e1 = next_event();
+            hipExtLaunchKernelGGL((kb_merge<Wt, TM>), dim3(blocks(3)), dim3(kBlock), 0, stream, e0, e1, 0, d_desc,
+                                  d_state, dsteps, map(3), 0, step_index, (const uint32_t*)d_epoch, ep_off);
+        } else {
+            hipLaunchKernelGGL((kb_merge<Wt, TM>), dim3(blocks(3)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps,
+                               map(3), 0, step_index, (const uint32_t*)d_epoch, ep_off);
+        }
     }
 
     // Compaction launch (the step table is not read in this mode but must be
