@@ -135,6 +135,11 @@ int pp_setindex(pp_handle* h, const uint64_t* z, const uint64_t* x, const double
 /* delete! for `count` keys (type_SparsePauliVector.jl:345-358); absent keys
  * are ignored; order-preserving. */
 int pp_delete(pp_handle* h, const uint64_t* z, const uint64_t* x, int64_t count);
+/* sum!(a, b) (spv_ops.jl:182-193): a += b, a merge of b's sorted terms into
+ * a with NOFILTER (a and b distinct handles of the same N). */
+int pp_sum(pp_handle* a, pp_handle* b);
+/* mul!(v, s) (spv_ops.jl:12-17): every coefficient times s. */
+int pp_scale(pp_handle* h, double s);
 /* norm(v, p) (spv_ops.jl:57-84): p = 2, 1, Inf or any p > 0. */
 int pp_norm(pp_handle* h, double p, double* out);
 /* inner_product(a, b) (spv_ops.jl:378-396): Σ over shared keys of c_a c_b. */

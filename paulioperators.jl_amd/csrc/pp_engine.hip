@@ -1343,6 +1343,41 @@ struct Batch {
         commit_live0(cur, n, live_cap);
     }
 
+    // sum!(v1, v2) (spv_ops.jl:182-193): v2's sorted live terms merged into
+    // vector 0 with NOFILTER; _try_merge! growth when the union overflows.
+    template <int Wt>
+    void sum_from(Batch& other) {
+        int64_t n2 = 0;
+        const DevSet& L2 = other.live0(n2);
+        sync_state();
+        SegState& st = h_state[0];
+        int cur = st.cur;
+        const int64_t n = st.n[cur];
+        int64_t live_cap = st.live_cap_v;
+        if (n2 == 0) return;
+        win_init();
+        WinBuf& w = win;
+        grow_set(w.A, n2, 0);
+        HIPCHK(hipMemcpyAsync(w.A.z, L2.z, (size_t)n2 * kb(), hipMemcpyDeviceToDevice, stream));
+        HIPCHK(hipMemcpyAsync(w.A.x, L2.x, (size_t)n2 * kb(), hipMemcpyDeviceToDevice, stream));
+        HIPCHK(hipMemcpyAsync(w.A.c, L2.c, (size_t)n2 * sizeof(double), hipMemcpyDeviceToDevice, stream));
+        win_filter(0, nullptr);
+        const int64_t out = win_merge<Wt>(n, cur, &w.A, n2, 0);
+        if (out > live_cap) live_cap = std::max(2 * live_cap, n + n2);
+        commit_live0(cur ^ 1, out, live_cap);
+    }
+
+    // mul!(v, a) (spv_ops.jl:12-17)
+    void scale(double a) {
+        int64_t n = 0;
+        const DevSet& L = live0(n);
+        if (n == 0) return;
+        const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(n, kBlock), 2048));
+        hipLaunchKernelGGL(kw_scale, dim3(grid), dim3(kBlock), 0, stream, L.c, n, a);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
     // delete! for many keys (type_SparsePauliVector.jl:345-358); absent keys
     // are ignored.
     template <int Wt>
@@ -1763,6 +1798,22 @@ int pp_delete(pp_handle* h, const uint64_t* z, const uint64_t* x, int64_t count)
     if (!h || count < 0 || (count > 0 && (!z || !x))) return fail(PP_EARG, "bad arguments");
     API_BEGIN
     dispatch_w(h->bt.W, [&](auto wc) { h->bt.delete_keys<decltype(wc)::value>(z, x, count); });
+    API_END
+}
+
+int pp_sum(pp_handle* a, pp_handle* b) {
+    if (!a || !b) return fail(PP_EARG, "null argument");
+    if (a == b) return fail(PP_EARG, "sum!(v, v) needs two distinct vectors");
+    if (a->N != b->N) return fail(PP_EARG, "qubit counts differ");
+    API_BEGIN
+    dispatch_w(a->bt.W, [&](auto wc) { a->bt.sum_from<decltype(wc)::value>(b->bt); });
+    API_END
+}
+
+int pp_scale(pp_handle* h, double a) {
+    if (!h) return fail(PP_EARG, "null handle");
+    API_BEGIN
+    h->bt.scale(a);
     API_END
 }
 

@@ -18,7 +18,7 @@ PP_OK, PP_EARG, PP_ENOMEM, PP_EHIP, PP_EOVERFLOW, PP_EUNSUPPORTED = range(6)
 
 EXPORTS = ("pp_create", "pp_destroy", "pp_upload", "pp_download", "pp_size", "pp_capacities", "pp_evolve",
            "pp_evolve_ex", "pp_truncate", "pp_evolve_single", "pp_compact", "pp_lookup", "pp_norm",
-           "pp_inner_product", "pp_diff_norm", "pp_setindex", "pp_delete", "pp_scale_by_popcount", "pp_expectation_ket", "pp_check",
+           "pp_inner_product", "pp_diff_norm", "pp_setindex", "pp_delete", "pp_sum", "pp_scale", "pp_scale_by_popcount", "pp_expectation_ket", "pp_check",
            "pp_batch_create", "pp_batch_destroy", "pp_batch_words", "pp_batch_upload", "pp_batch_download",
            "pp_batch_sizes", "pp_batch_evolve", "pp_batch_scale_by_popcount", "pp_batch_expectation_ket",
            "pp_batch_reserve", "pp_batch_set_tiles", "pp_batch_set_timing", "pp_batch_merge_time",
@@ -93,6 +93,8 @@ def lib():
     L.pp_lookup.argtypes = [vp, u64p, u64p, i64, dp, P(ctypes.c_int32)]
     L.pp_norm.argtypes = [vp, ctypes.c_double, dp]
     L.pp_setindex.argtypes = [vp, u64p, u64p, dp, i64]
+    L.pp_sum.argtypes = [vp, vp]
+    L.pp_scale.argtypes = [vp, ctypes.c_double]
     L.pp_delete.argtypes = [vp, u64p, u64p, i64]
     L.pp_inner_product.argtypes = [vp, vp, dp]
     L.pp_diff_norm.argtypes = [vp, vp, dp]

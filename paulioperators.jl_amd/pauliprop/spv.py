@@ -250,6 +250,50 @@ class SparsePauliVector:
     def __delitem__(self, p: PauliBasis):
         self.delete_([p])
 
+    def filter_(self, f):
+        """``filter!(f, v)`` (type_SparsePauliVector.jl:389-404): keeps the
+        pairs (PauliBasis, c) for which the host predicate f is true, in key
+        order; the rejected keys are removed on the device."""
+        drop = [(z, x) for z, x, c in self.terms() if not f((PauliBasis(self.N, z, x), c))]
+        return self.delete_(drop) if drop else self
+
+    def filter(self, f) -> "SparsePauliVector":
+        return self.copy().filter_(f)
+
+    def mul_(self, a: float):
+        """``mul!(v, a)`` (spv_ops.jl:12-17)."""
+        check(_lib.lib().pp_scale(self._h, float(a)))
+        return self
+
+    def sum_(self, other: "SparsePauliVector"):
+        """``sum!(v, other)`` (spv_ops.jl:182-193): v += other."""
+        if other.N != self.N:
+            raise DimensionMismatch("qubit counts differ")
+        if other is self:
+            return self.mul_(2.0)
+        check(_lib.lib().pp_sum(self._h, other._h))
+        return self
+
+    def __mul__(self, a: float) -> "SparsePauliVector":
+        return self.copy().mul_(a)
+
+    __rmul__ = __mul__
+
+    def __neg__(self) -> "SparsePauliVector":
+        return self.copy().mul_(-1.0)
+
+    def __add__(self, other: "SparsePauliVector") -> "SparsePauliVector":
+        return self.copy().sum_(other)
+
+    def __eq__(self, other) -> bool:
+        """``==`` (type_SparsePauliVector.jl:380-388): same keys and values."""
+        if not isinstance(other, SparsePauliVector) or other.N != self.N:
+            return NotImplemented
+        a, b = self.arrays(), other.arrays()
+        return all(np.array_equal(p, q) for p, q in zip(a, b))
+
+    __hash__ = None
+
     def norm(self, p: float = 2) -> float:
         """``norm(v, p)`` (spv_ops.jl:57-84)."""
         out = ctypes.c_double()

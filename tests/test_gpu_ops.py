@@ -140,3 +140,49 @@ def test_setindex_and_delete(N):
     assert got == ref and v.check()
     del v[PauliBasis(N, *next(iter(ref)))]
     assert len(v) == len(ref) - 1
+
+
+@pytest.mark.parametrize("N", [40, 120])
+def test_sum_mul_eq(N):
+    """sum! (spv_ops.jl:182-193, a merge summing shared keys), mul! (:12-17),
+    == (type_SparsePauliVector.jl:380-388) against dict arithmetic; shared
+    keys are summed once (c1 + c2), so the values are bit-exact."""
+    rng = random.Random(50 + N)
+    a = rand_vec(N, 20000, rng)
+    b = {k: rng.gauss(0, 1) for k in rng.sample(list(a), 5000)}
+    b.update(rand_vec(N, 8000, rng))
+    for k in rng.sample(list(a), 300):
+        b[k] = -a[k]  # NOFILTER keeps the exact zeros
+    va = SparsePauliVector(N, list(a.items()))
+    vb = SparsePauliVector(N, list(b.items()))
+    cap = va.capacities()[0]
+    va.sum_(vb)
+    ref = dict(a)
+    for k, c in b.items():
+        ref[k] = ref[k] + c if k in ref else c
+    got = {(z, x): c for z, x, c in va.terms()}
+    assert got == ref and va.check()
+    assert sum(1 for c in got.values() if c == 0.0) >= 300
+    n_in = len(a) + len(b)
+    assert va.capacities()[0] == (max(2 * cap, n_in) if len(ref) > cap else cap)
+    vb.mul_(-0.5)
+    assert {(z, x): c for z, x, c in vb.terms()} == {k: c * -0.5 for k, c in b.items()}
+    vs = va + vb
+    assert len(vs) == len(set(ref) | set(b))
+    v2 = SparsePauliVector(N, list(a.items()))
+    assert v2 == SparsePauliVector(N, list(a.items()))
+    assert not (v2 == va)
+    assert (-v2).terms()[0][2] == -v2.terms()[0][2]
+
+
+def test_filter():
+    """filter! (type_SparsePauliVector.jl:389-404) with a host predicate."""
+    rng = random.Random(77)
+    d = rand_vec(70, 5000, rng)
+    v = SparsePauliVector(70, list(d.items()))
+    keep = lambda pr: abs(pr[1]) > 1e-3 and pr[0].z % 3 != 0
+    w = v.filter(keep)
+    assert len(v) == len(d)
+    ref = {k: c for k, c in d.items() if abs(c) > 1e-3 and k[0] % 3 != 0}
+    assert {(z, x): c for z, x, c in w.terms()} == ref and w.check()
+    assert v.filter_(lambda pr: True) == SparsePauliVector(70, list(d.items()))
