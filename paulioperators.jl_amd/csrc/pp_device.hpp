@@ -81,13 +81,24 @@ __device__ __forceinline__ int hdb(const Key<W>& a, const Key<W>& b) {
     return -1;
 }
 
+// Word q of the combined key (x words 0..W-1, then z words).  Selects, not
+// an indexed access: a key indexed by a run-time word number would be placed
+// in per-thread LDS, whose addressing reads the workgroup shape from the
+// dispatch packet (a host-memory read at every launch).
+template <int W>
+__device__ __forceinline__ uint64_t key_word(const Key<W>& k, int q) {
+    uint64_t v = k.x[0];
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        v = q == w ? k.x[w] : v;
+        v = q == W + w ? k.z[w] : v;
+    }
+    return v;
+}
+
 template <int W>
 __device__ __forceinline__ int key_bit(const Key<W>& k, int b) {
-    if (b >= 64 * W) {
-        const int bb = b - 64 * W;
-        return (int)((k.z[bb >> 6] >> (bb & 63)) & 1ull);
-    }
-    return (int)((k.x[b >> 6] >> (b & 63)) & 1ull);
+    return (int)((key_word<W>(k, b >> 6) >> (b & 63)) & 1ull);
 }
 
 template <int W>

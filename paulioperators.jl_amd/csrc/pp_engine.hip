@@ -142,6 +142,7 @@ struct InstBuf {
     int32_t* tilecnt = nullptr;
     int64_t* tileoff = nullptr;
     int64_t* bnd = nullptr;
+    uint64_t* agg = nullptr;
     int64_t* split = nullptr;
     uint64_t* status = nullptr;
     int64_t cap_tiles = 0;   // sweep-tile capacity of flags/wpre/tilecnt/tileoff
@@ -194,6 +195,8 @@ struct Batch {
     SegDesc* d_desc = nullptr;
     SegState* d_state = nullptr;
     int64_t* d_maps = nullptr;     // 4 x (B + 1) block prefixes: tile, bounds, partition, merge
+    uint64_t* d_blk = nullptr;     // per-block (vector, local block, blocks) tables of the 4 maps
+    int64_t blk_off[4] = {0, 0, 0, 0};
     std::vector<int64_t> h_maps;
     FilterDev* d_filt = nullptr;   // [2B]: strict, local
     std::vector<int32_t> local_active;
@@ -226,11 +229,15 @@ struct Batch {
         int tile, mtile, W;
         unsigned g0, g1, g3;
         const void* steps;
+        uint64_t bounds_hash;  // which steps launch kb_bounds
         bool operator==(const GraphKey& o) const {
             return nsteps == o.nsteps && first == o.first && tile == o.tile && mtile == o.mtile && W == o.W &&
-                   g0 == o.g0 && g1 == o.g1 && g3 == o.g3 && steps == o.steps;
+                   g0 == o.g0 && g1 == o.g1 && g3 == o.g3 && steps == o.steps && bounds_hash == o.bounds_hash;
         }
     };
+    // per step of the current sequence: 1 when some vector's mask has more
+    // levels than kb_flags summarises, so kb_bounds must run
+    std::vector<uint8_t> step_bounds;
     std::vector<std::pair<GraphKey, hipGraphExec_t>> gcache;
     int64_t graph_launches = 0, graph_captures = 0;
 
@@ -239,7 +246,7 @@ struct Batch {
     ~Batch() {
         for (auto& b : ib) free_inst(b);
         free_win();
-        dfree(d_desc); dfree(d_state); dfree(d_maps); dfree(d_filt); dfree(d_chan); dfree(d_tabs);
+        dfree(d_desc); dfree(d_state); dfree(d_maps); dfree(d_blk); dfree(d_filt); dfree(d_chan); dfree(d_tabs);
         dfree(d_en); dfree(d_steps); dfree(d_partial); dfree(d_out); dfree(d_kets); dfree(d_bad);
         if (h_state) (void)hipHostFree(h_state);
         for (auto e : evpool) (void)hipEventDestroy(e);
@@ -301,7 +308,7 @@ struct Batch {
     }
     static void free_inst(InstBuf& b) {
         free_set(b.L[0]); free_set(b.L[1]); free_set(b.C);
-        dfree(b.flags); dfree(b.wpre); dfree(b.tilecnt); dfree(b.tileoff); dfree(b.bnd); dfree(b.split); dfree(b.status);
+        dfree(b.flags); dfree(b.wpre); dfree(b.tilecnt); dfree(b.tileoff); dfree(b.bnd); dfree(b.agg); dfree(b.split); dfree(b.status);
         b = InstBuf();
     }
 
@@ -323,6 +330,22 @@ struct Batch {
         d_state = dalloc<SegState>(B);
         d_maps = dalloc<int64_t>(4 * (size_t)(B + 1));
         h_maps.assign(4 * (size_t)(B + 1), 0);
+        {
+            // fixed table layout (a captured graph keeps these addresses):
+            // each map's per-vector block count is bounded by plan()'s caps
+            auto p2 = [](int64_t v) {
+                int64_t r = 1;
+                while (r < v) r <<= 1;
+                return r;
+            };
+            const int64_t per[4] = {p2((int64_t)ncu * 8), p2((int64_t)ncu * 4), 1, p2((int64_t)ncu * 4)};
+            int64_t off = 0;
+            for (int k = 0; k < 4; ++k) {
+                blk_off[k] = off;
+                off += per[k] * B;
+            }
+            d_blk = dalloc<uint64_t>((size_t)off);
+        }
         d_filt = dalloc<FilterDev>(2 * (size_t)B);
         d_chan = dalloc<ChanDev>(B);
         d_tabs = dalloc<double>((size_t)B * 3 * kExpTab);
@@ -373,8 +396,9 @@ struct Batch {
         }
         const int64_t tiles = cdiv(I.L[0].cap, kTileMin) + 2;
         if (I.cap_tiles < tiles) {
-            dfree(I.flags); dfree(I.wpre); dfree(I.tilecnt); dfree(I.tileoff);
+            dfree(I.flags); dfree(I.wpre); dfree(I.tilecnt); dfree(I.tileoff); dfree(I.agg);
             I.cap_tiles = tiles;
+            I.agg = dalloc<uint64_t>((size_t)tiles * kAggLevels);
             I.flags = dalloc<uint64_t>((size_t)tiles * (kTileMin / 64));
             I.wpre = dalloc<uint16_t>((size_t)tiles * (kTileMin / 64));
             I.tilecnt = dalloc<int32_t>(tiles);
@@ -412,7 +436,7 @@ struct Batch {
             }
             d.Cz = I.C.z; d.Cx = I.C.x; d.Cc = I.C.c;
             d.flags = I.flags; d.wpre = I.wpre; d.tilecnt = I.tilecnt; d.tileoff = I.tileoff;
-            d.bnd = I.bnd; d.split = I.split; d.status = I.status;
+            d.bnd = I.bnd; d.agg = I.agg; d.split = I.split; d.status = I.status;
             d.fs = d_filt + 2 * b;
             d.fl = d_filt + 2 * b + 1;
             d.cap = std::min(I.L[0].cap, I.L[1].cap);
@@ -453,6 +477,12 @@ struct Batch {
             acc[3] += q(std::max<int64_t>(1, std::min<int64_t>(cdiv(nest + nest / 2, T2), (int64_t)ncu * 4)));
         }
         HIPCHK(hipMemcpyAsync(d_maps, h_maps.data(), h_maps.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream));
+        for (int k = 0; k < 4; ++k) {
+            const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv((int64_t)blocks(k), 256), 1024));
+            hipLaunchKernelGGL(kb_block_table, dim3(g), dim3(256), 0, stream, (const int64_t*)(d_maps + (size_t)k * (B + 1)),
+                               B, d_blk + blk_off[k]);
+        }
+        HIPCHK(hipGetLastError());
         if (cap_partial < acc[0]) {
             dfree(d_partial);
             cap_partial = acc[0];
@@ -460,7 +490,7 @@ struct Batch {
         }
     }
 
-    BlockMap map(int k) const { return BlockMap{d_maps + (size_t)k * (B + 1), B}; }
+    BlockMap map(int k) const { return BlockMap{d_maps + (size_t)k * (B + 1), d_blk + blk_off[k], B}; }
     unsigned blocks(int k) const { return (unsigned)h_maps[(size_t)k * (B + 1) + B]; }
 
     std::vector<pp_filter> last_filters;  // what d_filt holds (strict, local per vector)
@@ -508,10 +538,12 @@ struct Batch {
     }
 
     template <int Wt, int T, int TM>
-    void launch_step(const StepDev* dsteps, int64_t step_index, uint32_t ep_off) {
+    void launch_step(const StepDev* dsteps, int64_t step_index, uint32_t ep_off, bool bounds) {
         hipLaunchKernelGGL((kb_flags<Wt, T>), dim3(blocks(0)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(0));
         hipLaunchKernelGGL(kb_scan<T>, dim3(B), dim3(1024), 0, stream, d_desc, d_state, dsteps, step_index);
-        hipLaunchKernelGGL((kb_bounds<Wt, T>), dim3(blocks(1)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(1));
+        if (bounds)
+            hipLaunchKernelGGL((kb_bounds<Wt, T>), dim3(blocks(1)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps,
+                               map(1));
         hipLaunchKernelGGL((kb_emit<Wt, T>), dim3(blocks(0)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(0));
         if (timing) {
             // events recorded by the launch itself at the kernel's start and
@@ -549,7 +581,8 @@ struct Batch {
                 uint32_t off = 0;
                 if (with_precheck) launch_compact<Wt, decltype(TM)::value>(off++);
                 for (int64_t s = first; s < nsteps; ++s)
-                    launch_step<Wt, decltype(T)::value, decltype(TM)::value>(dsteps + (s - first) * B, s, off++);
+                    launch_step<Wt, decltype(T)::value, decltype(TM)::value>(dsteps + (s - first) * B, s, off++,
+                                                                             step_bounds[(size_t)s] != 0);
             });
             hipLaunchKernelGGL(kb_epoch_advance, dim3(1), dim3(1), 0, stream, d_epoch, count);
         };
@@ -558,8 +591,10 @@ struct Batch {
             HIPCHK(hipGetLastError());
             return;
         }
+        uint64_t bh = 1469598103934665603ull;
+        for (int64_t s = first; s < nsteps; ++s) bh = (bh ^ step_bounds[(size_t)s]) * 1099511628211ull;
         GraphKey key{nsteps, with_precheck ? -1 - first : first, tile, mtile, Wt, blocks(0), blocks(1), blocks(3),
-                     (const void*)dsteps};
+                     (const void*)dsteps, bh};
         for (auto& g : gcache)
             if (g.first == key) {
                 HIPCHK(hipGraphLaunch(g.second, stream));
@@ -597,13 +632,17 @@ struct Batch {
         if (tile == 512) {
             if (mtile == 512) f(I(), I());
             else if (mtile == 1024) f(I(), J());
-            else if (Wt == 1) f(I(), K());
-            else throw PPError(PP_EARG, "merge tile 2048 needs N <= 64");
+            else {
+                if constexpr (Wt == 1) f(I(), K());
+                else throw PPError(PP_EARG, "merge tile 2048 needs N <= 64");
+            }
         } else {
             if (mtile == 512) f(K(), I());
             else if (mtile == 1024) f(K(), J());
-            else if (Wt == 1) f(K(), K());
-            else throw PPError(PP_EARG, "merge tile 2048 needs N <= 64");
+            else {
+                if constexpr (Wt == 1) f(K(), K());
+                else throw PPError(PP_EARG, "merge tile 2048 needs N <= 64");
+            }
         }
     }
 
@@ -633,6 +672,12 @@ struct Batch {
     void run_sequence(std::vector<StepDev>& steps, int64_t nsteps, const std::vector<int32_t>& en) {
         int maxlev = 1;
         for (const StepDev& s : steps) maxlev = std::max(maxlev, (int)s.g.nlev);
+        step_bounds.assign((size_t)nsteps, 0);
+        for (int64_t s = 0; s < nsteps; ++s)
+            for (int b = 0; b < B; ++b) {
+                const StepDev& sd = steps[(size_t)(s * B + b)];
+                if (sd.active && sd.g.nlev > kAggLevels) step_bounds[(size_t)s] = 1;
+            }
         for (int b = 0; b < B; ++b) {
             const int64_t cap = std::min(ib[b].L[0].cap, ib[b].L[1].cap);
             const int64_t n = n_known[b];
@@ -1923,7 +1968,7 @@ int pp_check(pp_handle* h, int* ok) {
 int pp_batch_create(int B, const int* N, const int64_t* live_cap, const int64_t* append_cap, pp_batch** out) {
     if (!out || !N || !live_cap || !append_cap) return fail(PP_EARG, "null argument");
     *out = nullptr;
-    if (B < 1) return fail(PP_EARG, "B must be >= 1");
+    if (B < 1 || B >= (1 << pp::kBlkVecBits)) return fail(PP_EARG, "B must be in [1, 2^20)");
     int maxN = 0;
     for (int b = 0; b < B; ++b) {
         if (N[b] < 1 || N[b] > 128) return fail(PP_EARG, "N must be in 1:128");

@@ -59,16 +59,221 @@ __device__ __forceinline__ int find_seg(const int64_t* __restrict__ prefix, int 
     return lo;
 }
 
-// Grid mapping shared by every kernel: blocks [pre[b], pre[b+1]) serve vector b.
+// Grid mapping shared by every kernel: blocks [pre[b], pre[b+1]) serve vector
+// b.  blk holds, per block, b | (block - pre[b]) << 20 | (pre[b+1] - pre[b]) << 42,
+// so a workgroup finds its vector with one load instead of a search.
 struct BlockMap {
     const int64_t* pre;
+    const uint64_t* blk;
     int B;
 };
 
+constexpr int kBlkVecBits = 20;   // vectors per batch < 2^20
+constexpr int kBlkLocBits = 22;   // blocks per vector < 2^22
+
 __device__ __forceinline__ void map_block(const BlockMap& bm, int& b, int64_t& lb, int64_t& nb) {
-    b = find_seg(bm.pre, bm.B, blockIdx.x);
-    lb = (int64_t)blockIdx.x - bm.pre[b];
-    nb = bm.pre[b + 1] - bm.pre[b];
+    const uint64_t e = bm.blk[blockIdx.x];
+    b = (int)(e & ((1ull << kBlkVecBits) - 1));
+    lb = (int64_t)((e >> kBlkVecBits) & ((1ull << kBlkLocBits) - 1));
+    nb = (int64_t)(e >> (kBlkVecBits + kBlkLocBits));
+}
+
+// Fills the per-block table of a map from its prefix (once per call).
+__global__ void kb_block_table(const int64_t* __restrict__ pre, int B, uint64_t* __restrict__ blk) {
+    const int64_t total = pre[B];
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (int64_t)gridDim.x * blockDim.x) {
+        const int b = find_seg(pre, B, g);
+        blk[g] = (uint64_t)b | ((uint64_t)(g - pre[b]) << kBlkVecBits) |
+                 ((uint64_t)(pre[b + 1] - pre[b]) << (kBlkVecBits + kBlkLocBits));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Segment summaries.  For a mask level b, a "head" is a live position j with
+// hdb(K_{j-1}, K_j) >= b (position 0 always): the runs between heads are the
+// trie segments of level b, and two adjacent runs are siblings when the head
+// between them has hdb exactly b.  kb_flags stores, per sweep tile and level,
+// the A-counts around the tile's first two and last two heads; kb_emit then
+// finds the A-counts of the segments that cross its tile's edges by scanning
+// the summaries of neighbouring tiles (kb_bounds does the same with key
+// searches for masks of more than kAggLevels levels).
+//   bits  0-11 a_first  A in [tile start, first head)   (whole tile if none)
+//   bits 12-23 a_last   A in [last head, tile end)
+//   bits 24-35 a_12     A in [first head, second head)
+//   bits 36-47 a_l2     A in [second-last head, last head)
+//   bits 48-49 heads    0, 1, or 2 for two or more
+//   bit  50    eq_first hdb at the first head == b
+//   bit  51    eq_last  hdb at the last head == b
+constexpr int kAggLevels = 4;
+
+__device__ __forceinline__ int agg_heads(uint64_t a) { return (int)((a >> 48) & 3u); }
+__device__ __forceinline__ int64_t agg_first(uint64_t a) { return (int64_t)(a & 4095u); }
+__device__ __forceinline__ int64_t agg_last(uint64_t a) { return (int64_t)((a >> 12) & 4095u); }
+__device__ __forceinline__ int64_t agg_12(uint64_t a) { return (int64_t)((a >> 24) & 4095u); }
+__device__ __forceinline__ int64_t agg_l2(uint64_t a) { return (int64_t)((a >> 36) & 4095u); }
+__device__ __forceinline__ bool agg_eq_first(uint64_t a) { return (a >> 50) & 1u; }
+__device__ __forceinline__ bool agg_eq_last(uint64_t a) { return (a >> 51) & 1u; }
+
+// One thread: the summary of one level of one tile from the per-word head,
+// equal-level and A bit masks and the per-word A counts.
+template <int kWords>
+__device__ uint64_t tile_summary(const uint64_t* hm, const uint64_t* em, const uint64_t* am, const int* wcnt) {
+    int f1 = -1, f2 = -1, l1 = -1, l2 = -1;
+    for (int w = 0; w < kWords && f2 < 0; ++w) {
+        uint64_t m = hm[w];
+        while (m && f2 < 0) {
+            const int p = w * 64 + __ffsll((unsigned long long)m) - 1;
+            if (f1 < 0) f1 = p;
+            else f2 = p;
+            m &= m - 1;
+        }
+    }
+    for (int w = kWords - 1; w >= 0 && l2 < 0; --w) {
+        uint64_t m = hm[w];
+        while (m && l2 < 0) {
+            const int q = 63 - __clzll((long long)m);
+            const int p = w * 64 + q;
+            if (l1 < 0) l1 = p;
+            else l2 = p;
+            m &= ~(1ull << q);
+        }
+    }
+    // A-count of [tile start, p)
+    auto apos = [&](int p) {
+        int c = 0;
+        for (int w = 0; w < (p >> 6); ++w) c += wcnt[w];
+        const uint64_t low = (p & 63) ? ((~0ull) >> (64 - (p & 63))) : 0ull;
+        return c + popc64(am[p >> 6] & low);
+    };
+    int total = 0;
+    for (int w = 0; w < kWords; ++w) total += wcnt[w];
+    const int af = f1 < 0 ? total : apos(f1);
+    const int al = l1 < 0 ? 0 : total - apos(l1);
+    const int a12 = f2 < 0 ? 0 : apos(f2) - af;
+    const int al2 = l2 < 0 ? 0 : apos(l1) - apos(l2);
+    const uint64_t heads = f1 < 0 ? 0u : (f2 < 0 ? 1u : 2u);
+    const uint64_t eqf = f1 >= 0 ? ((em[f1 >> 6] >> (f1 & 63)) & 1u) : 0u;
+    const uint64_t eql = l1 >= 0 ? ((em[l1 >> 6] >> (l1 & 63)) & 1u) : 0u;
+    return (uint64_t)af | ((uint64_t)al << 12) | ((uint64_t)a12 << 24) | ((uint64_t)al2 << 36) | (heads << 48) |
+           (eqf << 50) | (eql << 51);
+}
+
+struct AggHit {
+    int64_t u;      // tile holding the head (ntiles when none)
+    int64_t sum;    // A-counts of the tiles passed over
+    uint64_t a;     // that tile's summary
+};
+
+__device__ __forceinline__ int64_t wave_sum64(int64_t v) {
+#pragma unroll
+    for (int dd = kWave / 2; dd > 0; dd >>= 1) v += __shfl_xor(v, dd, kWave);
+    return v;
+}
+
+// Wave-cooperative: the largest tile u <= u0 with a head at level li, and the
+// A-count of the tiles in (u, u0].  Tile 0 always has a head (position 0).
+__device__ AggHit agg_back(const uint64_t* __restrict__ agg, const int32_t* __restrict__ tilecnt, int64_t u0, int li) {
+    const int lane = threadIdx.x % kWave;
+    int64_t base = u0, sum = 0;
+    while (base >= 0) {
+        const int64_t u = base - lane;
+        uint64_t a = 0;
+        int64_t c = 0;
+        if (u >= 0) {
+            a = agg[u * kAggLevels + li];
+            c = tilecnt[u];
+        }
+        const uint64_t hm = __ballot(u >= 0 && agg_heads(a) != 0);
+        if (hm) {
+            const int f = __ffsll((unsigned long long)hm) - 1;
+            sum += wave_sum64(lane < f ? c : 0);
+            const uint64_t af = __shfl(a, f, kWave);
+            return AggHit{base - f, sum, af};
+        }
+        sum += wave_sum64(c);
+        base -= kWave;
+    }
+    return AggHit{-1, sum, 0};
+}
+
+// Wave-cooperative: the smallest tile v >= v0 with a head at level li
+// (ntiles if none), and the A-count of the tiles in [v0, v).
+__device__ AggHit agg_fwd(const uint64_t* __restrict__ agg, const int32_t* __restrict__ tilecnt, int64_t v0,
+                          int64_t ntiles, int li) {
+    const int lane = threadIdx.x % kWave;
+    int64_t base = v0, sum = 0;
+    while (base < ntiles) {
+        const int64_t u = base + lane;
+        uint64_t a = 0;
+        int64_t c = 0;
+        if (u < ntiles) {
+            a = agg[u * kAggLevels + li];
+            c = tilecnt[u];
+        }
+        const uint64_t hm = __ballot(u < ntiles && agg_heads(a) != 0);
+        if (hm) {
+            const int f = __ffsll((unsigned long long)hm) - 1;
+            sum += wave_sum64(lane < f ? c : 0);
+            const uint64_t af = __shfl(a, f, kWave);
+            return AggHit{base + f, sum, af};
+        }
+        sum += wave_sum64(c);
+        base += kWave;
+    }
+    return AggHit{ntiles, sum, 0};
+}
+
+// The four cross-edge counts kb_bounds computes (same layout: pre_same,
+// pre_sib, post_same, post_sib), for one side of tile t at level lvl, from
+// the summaries.  Wave-uniform; `out` receives two values.
+template <int W>
+__device__ void edge_counts(int side, int li, int lvl, int64_t t, int64_t ntiles, int64_t n, int64_t ts, int64_t te,
+                            const uint64_t* __restrict__ Lz, const uint64_t* __restrict__ Lx,
+                            const uint64_t* __restrict__ agg, const int32_t* __restrict__ tilecnt, int64_t& same,
+                            int64_t& sib) {
+    same = 0;
+    sib = 0;
+    if (side == 0) {
+        if (ts == 0) return;
+        const Key<W> ka = load_key<W>(Lz, Lx, ts - 1);
+        const int h = hdb<W>(ka, load_key<W>(Lz, Lx, ts));
+        if (h > lvl) return;
+        const AggHit q = agg_back(agg, tilecnt, t - 1, li);        // q: last head <= ts - 1
+        same = agg_last(q.a) + q.sum;
+        if (h < lvl && key_bit<W>(ka, lvl) == 1 && agg_eq_last(q.a)) {
+            if (agg_heads(q.a) >= 2) {
+                sib = agg_l2(q.a);
+            } else {
+                const AggHit q2 = agg_back(agg, tilecnt, q.u - 1, li);
+                sib = agg_first(q.a) + q2.sum + agg_last(q2.a);
+            }
+        }
+    } else {
+        if (te >= n) return;
+        const Key<W> ka = load_key<W>(Lz, Lx, te);
+        const int h = hdb<W>(load_key<W>(Lz, Lx, te - 1), ka);
+        if (h > lvl) return;
+        if (h == lvl) {  // te is a head: the run [te, next head)
+            const uint64_t a = agg[(t + 1) * kAggLevels + li];
+            if (agg_heads(a) >= 2) {
+                same = agg_12(a);
+            } else {
+                const AggHit r = agg_fwd(agg, tilecnt, t + 2, ntiles, li);
+                same = (int64_t)tilecnt[t + 1] + r.sum + (r.u < ntiles ? agg_first(r.a) : 0);
+            }
+            return;
+        }
+        const AggHit r = agg_fwd(agg, tilecnt, t + 1, ntiles, li);  // r: first head > te
+        same = r.sum + (r.u < ntiles ? agg_first(r.a) : 0);
+        if (r.u < ntiles && key_bit<W>(ka, lvl) == 0 && agg_eq_first(r.a)) {
+            if (agg_heads(r.a) >= 2) {
+                sib = agg_12(r.a);
+            } else {
+                const AggHit r2 = agg_fwd(agg, tilecnt, r.u + 1, ntiles, li);
+                sib = agg_last(r.a) + r2.sum + (r2.u < ntiles ? agg_first(r2.a) : 0);
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -78,39 +283,61 @@ __global__ void __launch_bounds__(kBlock) kb_flags(const SegDesc* __restrict__ D
     constexpr int kTile = T;
     constexpr int kTileWords = T / 64;
     __shared__ int s_wcnt[kTileWords];
+    __shared__ uint64_t s_am[kTileWords];                   // A bits per word
+    __shared__ uint64_t s_hm[kAggLevels][kTileWords];       // run heads per level
+    __shared__ uint64_t s_em[kAggLevels][kTileWords];       // heads whose hdb equals the level
     int b;
     int64_t lb, nb;
     map_block(bm, b, lb, nb);
     const StepDev* st = steps + b;
-    if (!st->active) return;
     const SegState* s = S + b;
-    if (s->stalled) return;
-    const int cur = s->cur ^ s->flip;
-    const int64_t n = s->n[cur];
-    const int64_t ntiles = (n + kTile - 1) / kTile;
     const SegDesc& d = D[b];
-    const uint64_t* __restrict__ Lz = d.Lz[cur];
-    const uint64_t* __restrict__ Lx = d.Lx[cur];
-    const double* __restrict__ Lc = d.Lc[cur];
-    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    // every per-vector value is loaded before the first test, so the loads
+    // share one memory round trip (the empty asm takes them all as operands,
+    // which keeps the compiler from sinking a load past a branch)
+    const int active = st->active, stalled = s->stalled, cur0 = s->cur, flip = s->flip, nlev = st->g.nlev;
+    const int64_t n0 = s->n[0], n1 = s->n[1];
+    const uint64_t *Lz0 = d.Lz[0], *Lz1 = d.Lz[1], *Lx0 = d.Lx[0], *Lx1 = d.Lx[1];
+    const double *Lc0 = d.Lc[0], *Lc1 = d.Lc[1];
+    uint64_t* __restrict__ flags = d.flags;
+    uint16_t* __restrict__ wpre = d.wpre;
+    int32_t* __restrict__ tilecnt = d.tilecnt;
+    uint64_t* __restrict__ agg = d.agg;
     GenReg<W> G;
     G.load(&st->g);
     const int local_active = d.local_active;
+    const uint32_t lev4 = *reinterpret_cast<const uint32_t*>(st->g.lev);  // the first four levels
+    asm volatile("" ::"s"(active), "s"(stalled), "s"(cur0), "s"(flip), "s"(nlev), "s"(n0), "s"(n1), "s"(Lz0),
+                 "s"(Lz1), "s"(Lx0), "s"(Lx1), "s"(Lc0), "s"(Lc1), "s"(flags), "s"(wpre), "s"(tilecnt), "s"(agg),
+                 "s"(local_active), "s"(lev4), "s"(G.gz[0]), "s"(G.gx[0]), "s"(G.gz[W - 1]), "s"(G.gx[W - 1]));
+    if (!active || stalled) return;
+    const int cur = cur0 ^ flip;
+    const int64_t n = cur ? n1 : n0;
+    const uint64_t* __restrict__ Lz = cur ? Lz1 : Lz0;
+    const uint64_t* __restrict__ Lx = cur ? Lx1 : Lx0;
+    const double* __restrict__ Lc = cur ? Lc1 : Lc0;
+    const int64_t ntiles = (n + kTile - 1) / kTile;
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    // segment summaries for kb_emit when the mask has 1..kAggLevels levels
+    const bool summ = nlev >= 1 && nlev <= kAggLevels;
     bool need_z = false, need_x = false;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
         need_z |= G.gx[w] != 0;
         need_x |= G.gz[w] != 0;
     }
-    if (local_active) need_z = need_x = true;
+    if (local_active || summ) need_z = need_x = true;
     for (int64_t t = lb; t < ntiles; t += nb) {
 #pragma unroll
         for (int r = 0; r < kTileWords / 4; ++r) {
             const int wi = wave * (kTileWords / 4) + r;
             const int64_t i = t * kTile + wi * 64 + lane;
+            const bool valid = i < n;
             bool flag = false;
-            if (i < n) {
-                Key<W> k;
+            Key<W> k;
+#pragma unroll
+            for (int w = 0; w < W; ++w) k.z[w] = k.x[w] = 0;
+            if (valid) {
                 if constexpr (W == 1) {
                     k.z[0] = need_z ? Lz[i] : 0;
                     k.x[0] = need_x ? Lx[i] : 0;
@@ -118,14 +345,10 @@ __global__ void __launch_bounds__(kBlock) kb_flags(const SegDesc* __restrict__ D
                     if (need_z) {
                         const ulonglong2 a = reinterpret_cast<const ulonglong2*>(Lz)[i];
                         k.z[0] = a.x; k.z[1] = a.y;
-                    } else {
-                        k.z[0] = k.z[1] = 0;
                     }
                     if (need_x) {
                         const ulonglong2 c2 = reinterpret_cast<const ulonglong2*>(Lx)[i];
                         k.x[0] = c2.x; k.x[1] = c2.y;
-                    } else {
-                        k.x[0] = k.x[1] = 0;
                     }
                 }
                 flag = G.anti(k);
@@ -137,8 +360,32 @@ __global__ void __launch_bounds__(kBlock) kb_flags(const SegDesc* __restrict__ D
             }
             const uint64_t mask = __ballot(flag);
             if (lane == 0) {
-                if (t * kTile + wi * 64 < n) d.flags[t * kTileWords + wi] = mask;
+                if (t * kTile + wi * 64 < n) flags[t * kTileWords + wi] = mask;
                 s_wcnt[wi] = popc64(mask);
+                s_am[wi] = mask;
+            }
+            if (summ) {
+                // hdb against the previous key: from the lane below, lane 0 loads it
+                Key<W> kp;
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    kp.z[w] = __shfl_up(k.z[w], 1, kWave);
+                    kp.x[w] = __shfl_up(k.x[w], 1, kWave);
+                }
+                if (lane == 0 && valid && i > 0) kp = load_key<W>(Lz, Lx, i - 1);
+                const int hd = i == 0 ? (int)kHdbNone : hdb<W>(kp, k);
+#pragma unroll
+                for (int li = 0; li < kAggLevels; ++li) {
+                    if (li < nlev) {
+                        const int lvl = (int)((lev4 >> (8 * li)) & 255u);
+                        const uint64_t hm = __ballot(valid && hd >= lvl);
+                        const uint64_t em = __ballot(valid && hd == lvl);
+                        if (lane == 0) {
+                            s_hm[li][wi] = hm;
+                            s_em[li][wi] = em;
+                        }
+                    }
+                }
             }
         }
         __syncthreads();
@@ -150,8 +397,10 @@ __global__ void __launch_bounds__(kBlock) kb_flags(const SegDesc* __restrict__ D
                 const int u = __shfl_up(incl, dd, kWave);
                 if (lane >= dd) incl += u;
             }
-            if (lane < kTileWords && t * kTile + lane * 64 < n) d.wpre[t * kTileWords + lane] = (uint16_t)(incl - v);
-            if (lane == kTileWords - 1) d.tilecnt[t] = incl;
+            if (lane < kTileWords && t * kTile + lane * 64 < n) wpre[t * kTileWords + lane] = (uint16_t)(incl - v);
+            if (lane == kTileWords - 1) tilecnt[t] = incl;
+        } else if (wave == 1 && summ && lane < nlev) {
+            agg[t * kAggLevels + lane] = tile_summary<kTileWords>(s_hm[lane], s_em[lane], s_am, s_wcnt);
         }
         __syncthreads();
     }
@@ -169,36 +418,42 @@ __global__ void __launch_bounds__(1024) kb_scan(const SegDesc* __restrict__ D, S
     constexpr int kTile = T;
     __shared__ int64_t s_w[16];
     __shared__ int64_t s_carry;
-    __shared__ int64_t s_n;
-    __shared__ int s_go;
     const int b = blockIdx.x;
     SegState* s = S + b;
     const StepDev* st = steps + b;
+    const SegDesc& d = D[b];
     const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
+    // per-vector values in one round trip (see kb_flags)
+    const int active = st->active, precheck = st->precheck, stalled = s->stalled, cur0 = s->cur, flip = s->flip;
+    const int64_t n0 = s->n[0], n1 = s->n[1], appcap = s->app_cap_v;
+    const int32_t* __restrict__ tilecnt = d.tilecnt;
+    int64_t* __restrict__ tileoff = d.tileoff;
+    const int64_t capC = d.capC;
+    asm volatile("" ::"s"(active), "s"(precheck), "s"(stalled), "s"(cur0), "s"(flip), "s"(n0), "s"(n1), "s"(appcap),
+                 "s"(tilecnt), "s"(tileoff), "s"(capC));
+    const int cur = cur0 ^ flip;
+    const int64_t n = cur ? n1 : n0;
+    const bool go = active && !stalled;
     if (tid == 0) {
-        if (s->flip) {
-            s->cur ^= 1;
+        if (flip) {
+            s->cur = cur;
             s->flip = 0;
         }
-        s_go = st->active && !s->stalled;
-        s_n = s->n[s->cur];
         s_carry = 0;
         s->ticket = 0u;
-        if (s_go && st->precheck && s_n > s->app_cap_v) {
+        if (go && precheck && n > appcap) {
             s->early += 1;
-            s->merge_in += s_n;
-            s->merge_out += s_n;
-            s->app_cap_v = 2 * s->app_cap_v > s_n ? 2 * s->app_cap_v : s_n;
+            s->merge_in += n;
+            s->merge_out += n;
+            s->app_cap_v = 2 * appcap > n ? 2 * appcap : n;
         }
     }
     __syncthreads();
-    if (!s_go) return;
-    const SegDesc& d = D[b];
-    const int64_t n = s_n;
+    if (!go) return;
     const int64_t ntiles = (n + kTile - 1) / kTile;
     for (int64_t base = 0; base < ntiles; base += 1024) {
         const int64_t t = base + tid;
-        const int64_t v = t < ntiles ? d.tilecnt[t] : 0;
+        const int64_t v = t < ntiles ? tilecnt[t] : 0;
         int64_t incl = v;
 #pragma unroll
         for (int dd = 1; dd < kWave; dd <<= 1) {
@@ -210,16 +465,16 @@ __global__ void __launch_bounds__(1024) kb_scan(const SegDesc* __restrict__ D, S
         int64_t wpref = 0;
         for (int w = 0; w < wave; ++w) wpref += s_w[w];
         const int64_t carry = s_carry;
-        if (t < ntiles) d.tileoff[t] = carry + wpref + incl - v;
+        if (t < ntiles) tileoff[t] = carry + wpref + incl - v;
         __syncthreads();
         if (tid == 1023) s_carry = carry + wpref + incl;
         __syncthreads();
     }
     if (tid == 0) {
         const int64_t m = s_carry;
-        d.tileoff[ntiles] = m;
+        tileoff[ntiles] = m;
         s->m = m;
-        if (m > d.capC) {
+        if (m > capC) {
             s->stalled = 1;
             s->stall_step = step_index;
             s->need = m;
@@ -244,17 +499,24 @@ __global__ void __launch_bounds__(kBlock) kb_bounds(const SegDesc* __restrict__ 
     int64_t lb, nb;
     map_block(bm, b, lb, nb);
     const StepDev* st = steps + b;
-    if (!st->active) return;
     const SegState* s = S + b;
-    if (s->stalled) return;
-    const int cur = s->cur;
-    const int64_t n = s->n[cur];
-    const int64_t ntiles = (n + kTile - 1) / kTile;
     const SegDesc& d = D[b];
-    const uint64_t* __restrict__ Lz = d.Lz[cur];
-    const uint64_t* __restrict__ Lx = d.Lx[cur];
-    const int nlev = st->g.nlev;
-    const int maxlev = d.maxlev;
+    // per-vector values in one round trip (see kb_flags)
+    const int active = st->active, stalled = s->stalled, cur = s->cur, nlev = st->g.nlev, maxlev = d.maxlev;
+    const int64_t n0 = s->n[0], n1 = s->n[1];
+    const uint64_t *Lz0 = d.Lz[0], *Lz1 = d.Lz[1], *Lx0 = d.Lx[0], *Lx1 = d.Lx[1];
+    const uint64_t* __restrict__ flags = d.flags;
+    const uint16_t* __restrict__ wpre = d.wpre;
+    const int64_t* __restrict__ tileoff = d.tileoff;
+    int64_t* __restrict__ bnd = d.bnd;
+    const uint32_t lev4 = *reinterpret_cast<const uint32_t*>(st->g.lev);  // the first four levels
+    asm volatile("" ::"s"(active), "s"(stalled), "s"(cur), "s"(nlev), "s"(maxlev), "s"(n0), "s"(n1), "s"(Lz0),
+                 "s"(Lz1), "s"(Lx0), "s"(Lx1), "s"(flags), "s"(wpre), "s"(tileoff), "s"(bnd), "s"(lev4));
+    if (!active || stalled) return;
+    const int64_t n = cur ? n1 : n0;
+    const int64_t ntiles = (n + kTile - 1) / kTile;
+    const uint64_t* __restrict__ Lz = cur ? Lz1 : Lz0;
+    const uint64_t* __restrict__ Lx = cur ? Lx1 : Lx0;
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int64_t total = ntiles * 2;
     // one wave per (tile, side); the searches are wave-cooperative
@@ -273,38 +535,38 @@ __global__ void __launch_bounds__(kBlock) kb_bounds(const SegDesc* __restrict__ 
             h = hdb<W>(load_key<W>(Lz, Lx, te - 1), ka);
         }
         for (int li = 0; li < nlev; ++li) {
-            const int lvl = st->g.lev[li];
+            const int lvl = li < 4 ? (int)((lev4 >> (8 * li)) & 255u) : (int)st->g.lev[li];
             int64_t same = 0, sib = 0;
             if (h <= lvl) {
                 if (side == 0) {
                     const int64_t q = seg_start_wave<W>(Lz, Lx, ts - 1, lvl, ka);
-                    same = a_prefix<T>(ts, n, ntiles, d.flags, d.wpre, d.tileoff) -
-                           a_prefix<T>(q, n, ntiles, d.flags, d.wpre, d.tileoff);
+                    same = a_prefix<T>(ts, n, ntiles, flags, wpre, tileoff) -
+                           a_prefix<T>(q, n, ntiles, flags, wpre, tileoff);
                     if (h < lvl && key_bit<W>(ka, lvl) == 1 && q > 0) {
                         kb2 = load_key<W>(Lz, Lx, q - 1);
                         if (hdb<W>(kb2, load_key<W>(Lz, Lx, q)) == lvl) {
                             const int64_t q2 = seg_start_wave<W>(Lz, Lx, q - 1, lvl, kb2);
-                            sib = a_prefix<T>(q, n, ntiles, d.flags, d.wpre, d.tileoff) -
-                                  a_prefix<T>(q2, n, ntiles, d.flags, d.wpre, d.tileoff);
+                            sib = a_prefix<T>(q, n, ntiles, flags, wpre, tileoff) -
+                                  a_prefix<T>(q2, n, ntiles, flags, wpre, tileoff);
                         }
                     }
                 } else {
                     const int64_t r = seg_end_wave<W>(Lz, Lx, te, n, lvl, ka);
-                    same = a_prefix<T>(r, n, ntiles, d.flags, d.wpre, d.tileoff) -
-                           a_prefix<T>(te, n, ntiles, d.flags, d.wpre, d.tileoff);
+                    same = a_prefix<T>(r, n, ntiles, flags, wpre, tileoff) -
+                           a_prefix<T>(te, n, ntiles, flags, wpre, tileoff);
                     if (h < lvl && key_bit<W>(ka, lvl) == 0 && r < n) {
                         kb2 = load_key<W>(Lz, Lx, r);
                         if (hdb<W>(load_key<W>(Lz, Lx, r - 1), kb2) == lvl) {
                             const int64_t r2 = seg_end_wave<W>(Lz, Lx, r, n, lvl, kb2);
-                            sib = a_prefix<T>(r2, n, ntiles, d.flags, d.wpre, d.tileoff) -
-                                  a_prefix<T>(r, n, ntiles, d.flags, d.wpre, d.tileoff);
+                            sib = a_prefix<T>(r2, n, ntiles, flags, wpre, tileoff) -
+                                  a_prefix<T>(r, n, ntiles, flags, wpre, tileoff);
                         }
                     }
                 }
             }
             if (lane == 0) {
-                d.bnd[(t * maxlev + li) * 4 + 2 * side + 0] = same;
-                d.bnd[(t * maxlev + li) * 4 + 2 * side + 1] = sib;
+                bnd[(t * maxlev + li) * 4 + 2 * side + 0] = same;
+                bnd[(t * maxlev + li) * 4 + 2 * side + 1] = sib;
             }
         }
     }
@@ -324,26 +586,44 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
     __shared__ int16_t s_st[kTile];
     __shared__ int16_t s_e[kTile];
     __shared__ int s_wred[4];
+    __shared__ int64_t s_bq[kAggLevels][4];
     int b;
     int64_t lb, nb;
     map_block(bm, b, lb, nb);
     const StepDev* st = steps + b;
-    if (!st->active) return;
     SegState* s = S + b;
-    if (s->stalled) return;
-    const int cur = s->cur;
-    const int64_t n = s->n[cur];
-    const int64_t ntiles = (n + kTile - 1) / kTile;
     const SegDesc& d = D[b];
-    const uint64_t* __restrict__ Lz = d.Lz[cur];
-    const uint64_t* __restrict__ Lx = d.Lx[cur];
-    const double* __restrict__ Lc = d.Lc[cur];
-    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    // per-vector values in one round trip (see kb_flags); m = tileoff[ntiles]
+    const int active = st->active, stalled = s->stalled, cur = s->cur, nlev = st->g.nlev, maxlev = d.maxlev;
+    const int64_t n0 = s->n[0], n1 = s->n[1], mtot = s->m;
+    const uint64_t *Lz0 = d.Lz[0], *Lz1 = d.Lz[1], *Lx0 = d.Lx[0], *Lx1 = d.Lx[1];
+    const double *Lc0 = d.Lc[0], *Lc1 = d.Lc[1];
+    const uint64_t* __restrict__ flags = d.flags;
+    const uint16_t* __restrict__ wpre = d.wpre;
+    const int64_t* __restrict__ tileoff = d.tileoff;
+    const int64_t* __restrict__ bnd = d.bnd;
+    const uint64_t* __restrict__ agg = d.agg;
+    const int32_t* __restrict__ tilecnt = d.tilecnt;
+    uint64_t* __restrict__ Cz = d.Cz;
+    uint64_t* __restrict__ Cx = d.Cx;
+    double* __restrict__ Cc = d.Cc;
     GenReg<W> G;
     G.load(&st->g);
-    const int nlev = st->g.nlev;
-    const int maxlev = d.maxlev;
-    const int64_t mtot = d.tileoff[ntiles];
+    const uint32_t lev4 = *reinterpret_cast<const uint32_t*>(st->g.lev);  // the first four levels
+    asm volatile("" ::"s"(active), "s"(stalled), "s"(cur), "s"(nlev), "s"(maxlev), "s"(n0), "s"(n1), "s"(mtot),
+                 "s"(Lz0), "s"(Lz1), "s"(Lx0), "s"(Lx1), "s"(Lc0), "s"(Lc1), "s"(flags), "s"(wpre), "s"(tileoff),
+                 "s"(bnd), "s"(Cz), "s"(Cx), "s"(Cc), "s"(G.gz[0]), "s"(G.gx[0]), "s"(G.gz[W - 1]),
+                 "s"(G.gx[W - 1]), "s"(G.sinv), "s"(G.ng), "s"(lev4), "s"(agg), "s"(tilecnt));
+    if (!active || stalled) return;
+    const int64_t n = cur ? n1 : n0;
+    const int64_t ntiles = (n + kTile - 1) / kTile;
+    const uint64_t* __restrict__ Lz = cur ? Lz1 : Lz0;
+    const uint64_t* __restrict__ Lx = cur ? Lx1 : Lx0;
+    const double* __restrict__ Lc = cur ? Lc1 : Lc0;
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    // cross-edge segment counts from kb_flags' summaries (1..kAggLevels
+    // levels) or from kb_bounds' table
+    const bool summ = nlev >= 1 && nlev <= kAggLevels;
     for (int64_t t = lb; t < ntiles; t += nb) {
         const int64_t ts = t * kTile;
         const int Teff = (int)((n - ts) < kTile ? (n - ts) : kTile);
@@ -351,7 +631,18 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
         double C[R];
         int64_t rank[R];
         bool fl[R];
-        const int64_t toff = d.tileoff[t];
+        const int64_t toff = tileoff[t];
+        // boundary counts of the first four levels, loaded with the tile
+        // (index clamped into the table when there are fewer levels)
+        int64_t bq4[4][4];
+        if (!summ) {
+#pragma unroll
+            for (int li = 0; li < 4; ++li) {
+                const int lc = li < nlev ? li : 0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) bq4[li][q] = bnd[(t * maxlev + lc) * 4 + q];
+            }
+        }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int j = wave * (kTile / 4) + r * kWave + lane;
@@ -361,10 +652,10 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
             C[r] = 0.0;
             if (j < Teff) {
                 K[r] = load_key<W>(Lz, Lx, i);
-                const uint64_t word = d.flags[i >> 6];
+                const uint64_t word = flags[i >> 6];
                 fl[r] = (word >> lane) & 1ull;
                 const uint64_t lowmask = lane ? ((~0ull) >> (64 - lane)) : 0ull;
-                const int aloc = (int)d.wpre[i >> 6] + popc64(word & lowmask);
+                const int aloc = (int)wpre[i >> 6] + popc64(word & lowmask);
                 s_aloc[j] = (int16_t)aloc;
                 s_hdb[j] = (int16_t)(i == 0 ? kHdbNone : hdb<W>(load_key<W>(Lz, Lx, i - 1), K[r]));
                 rank[r] = toff + aloc;
@@ -374,15 +665,37 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
                 for (int w = 0; w < W; ++w) K[r].z[w] = K[r].x[w] = 0;
             }
         }
+        if (summ) {
+            // wave w: the tile's start edge (w even) or end edge (w odd) at
+            // levels w / 2 and w / 2 + 2
+            const int side = wave & 1;
+            for (int li = wave >> 1; li < nlev; li += 2) {
+                const int lvl = (int)((lev4 >> (8 * li)) & 255u);
+                int64_t same, sib;
+                edge_counts<W>(side, li, lvl, t, ntiles, n, ts, ts + Teff, Lz, Lx, agg, tilecnt, same, sib);
+                if (lane == 0) {
+                    s_bq[li][2 * side] = same;
+                    s_bq[li][2 * side + 1] = sib;
+                }
+            }
+        }
         if (threadIdx.x == 0) {
-            s_aloc[Teff] = (int16_t)(d.tileoff[t + 1] - toff);
+            s_aloc[Teff] = (int16_t)(tileoff[t + 1] - toff);
             s_hdb[Teff] = (int16_t)(ts + Teff < n ? hdb<W>(load_key<W>(Lz, Lx, ts + Teff - 1), load_key<W>(Lz, Lx, ts + Teff))
                                                   : kHdbNone);
         }
         __syncthreads();
         for (int li = 0; li < nlev; ++li) {
-            const int lvl = st->g.lev[li];
-            const int64_t* bq = d.bnd + (t * maxlev + li) * 4;
+            const int lvl = li < 4 ? (int)((lev4 >> (8 * li)) & 255u) : (int)st->g.lev[li];
+            int64_t bq[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                bq[q] = summ      ? s_bq[li][q]
+                        : li == 0 ? bq4[0][q]
+                        : li == 1 ? bq4[1][q]
+                        : li == 2 ? bq4[2][q]
+                        : li == 3 ? bq4[3][q]
+                                  : bnd[(t * maxlev + li) * 4 + q];
             // forward: st_j = last head <= j (j = 0 is always a head)
             int stv[R];
             int carry = -1;
@@ -480,25 +793,19 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
             }
             Key<W> ch;
             const double cn = G.child(K[r], C[r], ch);
-            store_key<W>(d.Cz, d.Cx, rank[r], ch);
-            d.Cc[rank[r]] = cn;
+            store_key<W>(Cz, Cx, rank[r], ch);
+            Cc[rank[r]] = cn;
         }
         __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------------------
-// mode 0: rotation (live + children); mode 1: early-merge compaction (live
-// only, no rotation) for the vectors kb_precheck flagged.
-__device__ __forceinline__ bool merge_go(const StepDev* st, const SegState* s, int mode) {
-    if (s->stalled) return false;
-    return mode == 0 ? st->active != 0 : s->compact != 0;
-}
-
-// ---------------------------------------------------------------------------
-// kb_merge (see the file header).  Each workgroup takes tiles of its vector
-// by ticket, so a tile only ever waits in the look-back on tiles that running
-// workgroups have already claimed.
+// kb_merge (see the file header).  mode 0: rotation (live + children);
+// mode 1: early-merge compaction (live only, no rotation) for the vectors
+// kb_precheck flagged.  A tile only ever waits in the look-back on tiles of
+// workgroups that have started: with a workgroup per tile, tile = local
+// workgroup index; with fewer workgroups, tiles are claimed by ticket.
 template <int W, int TM>
 __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D, SegState* __restrict__ S,
                                                    const StepDev* __restrict__ steps, BlockMap bm, int mode,
@@ -516,30 +823,40 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
     map_block(bm, b, lb, nb);
     const StepDev* st = steps + b;
     SegState* s = S + b;
-    if (!merge_go(st, s, mode)) return;
-    const int rot = mode == 0;
-    // launch tag of the look-back words: the sequence's base (device memory,
-    // advanced after every sequence) plus this launch's offset in it
-    const uint64_t ep = ((uint64_t)(*d_epoch + ep_off) & kEpochMask) << kEpochShift;
-    const int cur = s->cur;
-    const int64_t n = s->n[cur];
-    const int64_t m = rot ? s->m : 0;
-    const int64_t tot = n + m;
-    const int64_t ntm = (tot + TM - 1) / TM;
     const SegDesc& d = D[b];
-    const uint64_t* __restrict__ Lz = d.Lz[cur];
-    const uint64_t* __restrict__ Lx = d.Lx[cur];
-    const double* __restrict__ Lc = d.Lc[cur];
-    uint64_t* __restrict__ Oz = d.Lz[cur ^ 1];
-    uint64_t* __restrict__ Ox = d.Lx[cur ^ 1];
-    double* __restrict__ Oc = d.Lc[cur ^ 1];
+    const int rot = mode == 0;
+    // per-vector values in one round trip (see kb_flags)
+    const int active = st->active, stalled = s->stalled, compact = s->compact, cur = s->cur;
+    const uint32_t ep0 = *d_epoch;
+    const int64_t n0 = s->n[0], n1 = s->n[1], m0 = s->m, capOut = d.cap;
+    uint64_t *Lz0 = d.Lz[0], *Lz1 = d.Lz[1], *Lx0 = d.Lx[0], *Lx1 = d.Lx[1];
+    double *Lc0 = d.Lc[0], *Lc1 = d.Lc[1];
     const uint64_t* __restrict__ Cz = d.Cz;
     const uint64_t* __restrict__ Cx = d.Cx;
     const double* __restrict__ Cc = d.Cc;
-    const int64_t capOut = d.cap;
-    const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
+    uint64_t* __restrict__ status = d.status;
+    const FilterDev* __restrict__ fs = d.fs;
     GenReg<W> G;
-    if (rot) G.load(&st->g);
+    G.load(&st->g);
+    asm volatile("" ::"s"(active), "s"(stalled), "s"(compact), "s"(cur), "s"(ep0), "s"(n0), "s"(n1), "s"(m0),
+                 "s"(capOut), "s"(Lz0), "s"(Lz1), "s"(Lx0), "s"(Lx1), "s"(Lc0), "s"(Lc1), "s"(Cz), "s"(Cx), "s"(Cc),
+                 "s"(status), "s"(fs), "s"(G.gz[0]), "s"(G.gx[0]), "s"(G.gz[W - 1]), "s"(G.gx[W - 1]),
+                 "s"(G.cosv));
+    if (stalled || !(rot ? active : compact)) return;
+    // launch tag of the look-back words: the sequence's base (device memory,
+    // advanced after every sequence) plus this launch's offset in it
+    const uint64_t ep = ((uint64_t)(ep0 + ep_off) & kEpochMask) << kEpochShift;
+    const int64_t n = cur ? n1 : n0;
+    const int64_t m = rot ? m0 : 0;
+    const int64_t tot = n + m;
+    const int64_t ntm = (tot + TM - 1) / TM;
+    const uint64_t* __restrict__ Lz = cur ? Lz1 : Lz0;
+    const uint64_t* __restrict__ Lx = cur ? Lx1 : Lx0;
+    const double* __restrict__ Lc = cur ? Lc1 : Lc0;
+    uint64_t* __restrict__ Oz = cur ? Lz0 : Lz1;
+    uint64_t* __restrict__ Ox = cur ? Lx0 : Lx1;
+    double* __restrict__ Oc = cur ? Lc0 : Lc1;
+    const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
     if (ntm == 0) {  // nothing to merge: the output is empty
         if (lb == 0 && tid == 0) {
             s->n[cur ^ 1] = 0;
@@ -549,10 +866,20 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
         }
         return;
     }
-    while (true) {
-        if (tid == 0) s_tile = (int64_t)atomicAdd(&s->ticket, 1u);
-        __syncthreads();
-        const int64_t t = s_tile;
+    // Tiles: when the vector has a workgroup per tile, workgroup lb takes
+    // tile lb (workgroups start in index order, so every tile a look-back
+    // waits on belongs to a started workgroup); otherwise workgroups take
+    // tiles by ticket.
+    const bool by_ticket = nb < ntm;
+    for (int64_t it = 0;; ++it) {
+        int64_t t;
+        if (by_ticket) {
+            if (tid == 0) s_tile = (int64_t)atomicAdd(&s->ticket, 1u);
+            __syncthreads();
+            t = s_tile;
+        } else {
+            t = it == 0 ? lb : ntm;
+        }
         if (t >= ntm) break;
         const int64_t d0 = t * TM;
         const int64_t d1 = d0 + TM < tot ? d0 + TM : tot;
@@ -624,7 +951,7 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
                     if (key_eq<W>(load_key<W>(Lz, Lx, i0 - 1), K)) emit = false;
                 }
             }
-            if (emit && !should_drop<W>(d.fs, K, fabs(acc))) {
+            if (emit && !should_drop<W>(fs, K, fabs(acc))) {
                 outK[cnt] = K;
                 outC[cnt] = acc;
                 ++cnt;
@@ -648,7 +975,6 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
         // predecessors per step.  Each status word carries its own payload
         // (flag | count), so relaxed atomics suffice: no other memory is
         // published through it, and no cache write-back/invalidate is needed.
-        uint64_t* __restrict__ status = d.status;
         if (wave == 0) {
             if (t == 0) {
                 if (lane == 0) {

@@ -170,10 +170,7 @@ __global__ void __launch_bounds__(kBlock) kw_rotate(uint64_t* __restrict__ Lz, u
 template <int W>
 __device__ __forceinline__ int key_digit(const Key<W>& k, int dd) {
     const int bit = dd * 8;
-    uint64_t word;
-    if (bit >= 64 * W) word = k.z[(bit - 64 * W) >> 6];
-    else word = k.x[bit >> 6];
-    return (int)((word >> (bit & 63)) & 0xffull);
+    return (int)((key_word<W>(k, bit >> 6) >> (bit & 63)) & 0xffull);
 }
 
 // OR and AND of every key word (constant digits are skipped).

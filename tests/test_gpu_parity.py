@@ -337,6 +337,56 @@ def test_batch_uneven_vectors_bitexact():
         assert abs(evs[b] - orcs[b].expectation(0)) <= 1e-12
 
 
+def random_two_qubit_pauli(N, sup, rng):
+    """A Pauli with random letters on two qubits of the support: 1 to 4 mask levels."""
+    qs = rng.sample([b for b in range(N) if sup >> b & 1], 2)
+    gz = gx = 0
+    for q in qs:
+        letter = rng.randint(1, 3)  # 1 X, 2 Z, 3 Y
+        gx |= (letter & 1) << q
+        gz |= (letter >> 1) << q
+    return gz, gx
+
+
+@pytest.mark.parametrize("N,tile,mtile", [(40, 512, 512), (40, 2048, 2048), (100, 512, 1024), (100, 2048, 1024)])
+def test_segment_summaries_mixed_levels(N, tile, mtile):
+    """Masks of 1 to 4 levels (children ranked from kb_flags' per-tile segment
+    summaries) and many-level masks (kb_bounds' key searches) in the same
+    steps of one batch, sweep tiles of 512 and 2048 terms, segments spanning
+    many tiles: every vector equals its oracle run bit for bit."""
+    from pauliprop import SparsePauliBatch
+    rng = random.Random(N * 7 + tile)
+    sizes = [20000, 9000, 600, 3000]
+    B = len(sizes)
+    sups = [spread_support(N, 9, rng) for _ in range(B)]
+    init = [random_terms(N, n, rng, sup) for n, sup in zip(sizes, sups)]
+    gens, angs = [], []
+    for b in range(B):
+        g, a = [], []
+        for k in range(24):
+            if b == 3 and k % 3 == 0:
+                gz, gx = rng.getrandbits(N) & sups[b], rng.getrandbits(N) & sups[b]
+            else:
+                gz, gx = random_two_qubit_pauli(N, sups[b], rng)
+            g.append(PauliBasis(N, gz, gx))
+            a.append(rng.uniform(-3, 3))
+        gens.append(g)
+        angs.append(a)
+    strat = CoeffTruncation(1e-9)
+    bt = SparsePauliBatch([N] * B, [[((z, x), c) for z, x, c in t] for t in init])
+    bt.set_tiles(tile, mtile)
+    log(f"N={N} tiles {tile}/{mtile}: sizes {bt.sizes()}")
+    bt.evolve_(gens, angs, truncation=[strat] * B)
+    log(f"sizes after {bt.sizes()}")
+    for b in range(B):
+        o = po.OracleSPV(N, init[b])
+        o.evolve([(p.z, p.x) for p in gens[b]], angs[b], strict=compile_filter(strat))
+        gz, gx, gc = bt.arrays(b)
+        oz, ox, oc = o.arrays()
+        assert np.array_equal(gz, oz) and np.array_equal(gx, ox), f"vector {b} keys"
+        assert np.array_equal(gc.view(np.uint64), oc.view(np.uint64)), f"vector {b} coefficients"
+
+
 def test_batch_graph_replays_match_eager():
     """hipGraph-captured sequences: the same layer circuit replayed over
     several calls, with eager channel work and host synchronisation between
