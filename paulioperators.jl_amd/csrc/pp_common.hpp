@@ -72,9 +72,10 @@ struct SegState {
     int64_t rotations, swept, created, merge_in, merge_out, early;  // WindowCounters sums
     int64_t stall_step;   // step at which the instance stalled
     int64_t need;         // size that did not fit
+    int64_t nstep;        // live count entering the current rotation or compaction
     uint32_t ticket;      // merge tile ticket
     int32_t cur;          // live buffer parity
-    int32_t flip;         // a finished merge whose parity flip is pending
+    int32_t flip;         // live parity = cur ^ flip (a merge toggles flip)
     int32_t stalled;      // 1: child buffer too small, 2: live buffer too small
     int32_t compact;      // early-merge compaction requested at this layer start
     int32_t err;          // internal consistency bits

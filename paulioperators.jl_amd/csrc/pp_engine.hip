@@ -339,10 +339,11 @@ struct Batch {
                 return r;
             };
             const int64_t per[4] = {p2((int64_t)ncu * 8), p2((int64_t)ncu * 4), 1, p2((int64_t)ncu * 4)};
+            // graph mode rounds each total up to a power of two
             int64_t off = 0;
             for (int k = 0; k < 4; ++k) {
                 blk_off[k] = off;
-                off += per[k] * B;
+                off += p2(per[k] * B);
             }
             d_blk = dalloc<uint64_t>((size_t)off);
         }
@@ -462,19 +463,23 @@ struct Batch {
             // real tiles only exit (sweeps) or contend for the ticket (merge)
             const int64_t nest = std::min<int64_t>(cap, std::max<int64_t>(n_known[b] + n_known[b] / 2, 1024));
             const int64_t tiles = cdiv(nest, kTile);
-            // with captured graphs, counts are rounded up to powers of two so
-            // that the grids (baked into a graph) change rarely; surplus
-            // blocks find no tile and exit
-            auto q = [&](int64_t v) {
-                if (!graphs) return v;
-                int64_t r = 1;
-                while (r < v) r <<= 1;
-                return r;
-            };
-            acc[0] += q(std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)ncu * 8)));
-            acc[1] += q(std::max<int64_t>(1, std::min<int64_t>(cdiv(2 * tiles, kBlock / kWave), (int64_t)ncu * 4)));
+            acc[0] += std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)ncu * 8));
+            acc[1] += std::max<int64_t>(1, std::min<int64_t>(cdiv(2 * tiles, kBlock / kWave), (int64_t)ncu * 4));
             acc[2] += 1;
-            acc[3] += q(std::max<int64_t>(1, std::min<int64_t>(cdiv(nest + nest / 2, T2), (int64_t)ncu * 4)));
+            acc[3] += std::max<int64_t>(1, std::min<int64_t>(cdiv(nest + nest / 2, T2), (int64_t)ncu * 4));
+        }
+        if (graphs) {
+            // with captured graphs each grid total is rounded up to a power of
+            // two (the surplus blocks go to the last vector, find no tile and
+            // exit), so that a graph, whose grids are fixed, is replayed by
+            // most calls; the per-vector split is data read at run time
+            for (int k = 0; k < 4; ++k) {
+                if (k == 2) continue;
+                int64_t r = 1;
+                while (r < acc[k]) r <<= 1;
+                const int64_t last = h_maps[(size_t)k * (B + 1) + B - 1];
+                if (r - last < (int64_t(1) << kBlkLocBits)) h_maps[(size_t)k * (B + 1) + B] = r;
+            }
         }
         HIPCHK(hipMemcpyAsync(d_maps, h_maps.data(), h_maps.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream));
         for (int k = 0; k < 4; ++k) {
@@ -537,14 +542,26 @@ struct Batch {
         epoch_used += count;
     }
 
+    // kb_scan runs when kb_bounds needs the tile offsets or a vector may
+    // hold more than kSelfOffTiles sweep tiles; otherwise kb_emit sums the
+    // tile counts itself (one kernel less per rotation).
+    static constexpr int64_t kSelfOffTiles = 1024;
+    bool scan_needed(bool bounds) const {
+        if (bounds) return true;
+        for (const InstBuf& I : ib)
+            if (cdiv(std::min(I.L[0].cap, I.L[1].cap), (int64_t)tile) > kSelfOffTiles) return true;
+        return false;
+    }
+
     template <int Wt, int T, int TM>
-    void launch_step(const StepDev* dsteps, int64_t step_index, uint32_t ep_off, bool bounds) {
+    void launch_step(const StepDev* dsteps, int64_t step_index, uint32_t ep_off, bool bounds, bool scan) {
         hipLaunchKernelGGL((kb_flags<Wt, T>), dim3(blocks(0)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(0));
-        hipLaunchKernelGGL(kb_scan<T>, dim3(B), dim3(1024), 0, stream, d_desc, d_state, dsteps, step_index);
+        if (scan) hipLaunchKernelGGL(kb_scan<T>, dim3(B), dim3(1024), 0, stream, d_desc, d_state, dsteps, step_index);
         if (bounds)
             hipLaunchKernelGGL((kb_bounds<Wt, T>), dim3(blocks(1)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps,
                                map(1));
-        hipLaunchKernelGGL((kb_emit<Wt, T>), dim3(blocks(0)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(0));
+        hipLaunchKernelGGL((kb_emit<Wt, T>), dim3(blocks(0)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(0),
+                           scan ? 0 : 1, step_index);
         if (timing) {
             // events recorded by the launch itself at the kernel's start and
             // end (not around it), so the interval is the kernel's duration
@@ -580,9 +597,11 @@ struct Batch {
             with_tiles<Wt>([&](auto T, auto TM) {
                 uint32_t off = 0;
                 if (with_precheck) launch_compact<Wt, decltype(TM)::value>(off++);
-                for (int64_t s = first; s < nsteps; ++s)
-                    launch_step<Wt, decltype(T)::value, decltype(TM)::value>(dsteps + (s - first) * B, s, off++,
-                                                                             step_bounds[(size_t)s] != 0);
+                for (int64_t s = first; s < nsteps; ++s) {
+                    const bool bnd = step_bounds[(size_t)s] != 0;
+                    launch_step<Wt, decltype(T)::value, decltype(TM)::value>(dsteps + (s - first) * B, s, off++, bnd,
+                                                                             scan_needed(bnd));
+                }
             });
             hipLaunchKernelGGL(kb_epoch_advance, dim3(1), dim3(1), 0, stream, d_epoch, count);
         };
@@ -593,6 +612,7 @@ struct Batch {
         }
         uint64_t bh = 1469598103934665603ull;
         for (int64_t s = first; s < nsteps; ++s) bh = (bh ^ step_bounds[(size_t)s]) * 1099511628211ull;
+        bh = (bh ^ (scan_needed(false) ? 2u : 0u)) * 1099511628211ull;
         GraphKey key{nsteps, with_precheck ? -1 - first : first, tile, mtile, Wt, blocks(0), blocks(1), blocks(3),
                      (const void*)dsteps, bh};
         for (auto& g : gcache)
@@ -761,6 +781,7 @@ struct Batch {
         for (int b = 0; b < B; ++b) {
             h_state[b].compact = sel[b] ? 1 : 0;
             h_state[b].ticket = 0u;
+            h_state[b].nstep = h_state[b].n[h_state[b].cur];  // the merge's live count
         }
         put_state();
         upload_desc();

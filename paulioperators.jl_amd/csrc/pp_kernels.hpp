@@ -14,8 +14,9 @@
 //
 // One rotation at window = 1 (spv_evolve.jl:207-218 + 258-284):
 //   kb_flags     anticommutation test per term -> bit mask + per-tile counts
-//   kb_scan      exclusive scan of tile counts -> child offsets, m; applies
-//                the reference's append-capacity precheck bookkeeping
+//   kb_scan      exclusive scan of tile counts -> child offsets and m (only
+//                for very long vectors or kb_bounds steps: otherwise kb_emit
+//                sums the tile counts itself)
 //   kb_bounds    per (tile, mask level) segment counts that cross tile edges
 //   kb_emit      child (G xor P, ((k-2) sinθ) c) written straight to its rank
 //                in key order (no sort pass, see below)
@@ -278,7 +279,7 @@ __device__ void edge_counts(int side, int li, int lvl, int64_t t, int64_t ntiles
 
 // ---------------------------------------------------------------------------
 template <int W, int T>
-__global__ void __launch_bounds__(kBlock) kb_flags(const SegDesc* __restrict__ D, const SegState* __restrict__ S,
+__global__ void __launch_bounds__(kBlock) kb_flags(const SegDesc* __restrict__ D, SegState* __restrict__ S,
                                                    const StepDev* __restrict__ steps, BlockMap bm) {
     constexpr int kTile = T;
     constexpr int kTileWords = T / 64;
@@ -290,13 +291,14 @@ __global__ void __launch_bounds__(kBlock) kb_flags(const SegDesc* __restrict__ D
     int64_t lb, nb;
     map_block(bm, b, lb, nb);
     const StepDev* st = steps + b;
-    const SegState* s = S + b;
+    SegState* s = S + b;
     const SegDesc& d = D[b];
     // every per-vector value is loaded before the first test, so the loads
     // share one memory round trip (the empty asm takes them all as operands,
     // which keeps the compiler from sinking a load past a branch)
     const int active = st->active, stalled = s->stalled, cur0 = s->cur, flip = s->flip, nlev = st->g.nlev;
-    const int64_t n0 = s->n[0], n1 = s->n[1];
+    const int precheck = st->precheck;
+    const int64_t n0 = s->n[0], n1 = s->n[1], appcap = s->app_cap_v;
     const uint64_t *Lz0 = d.Lz[0], *Lz1 = d.Lz[1], *Lx0 = d.Lx[0], *Lx1 = d.Lx[1];
     const double *Lc0 = d.Lc[0], *Lc1 = d.Lc[1];
     uint64_t* __restrict__ flags = d.flags;
@@ -309,10 +311,23 @@ __global__ void __launch_bounds__(kBlock) kb_flags(const SegDesc* __restrict__ D
     const uint32_t lev4 = *reinterpret_cast<const uint32_t*>(st->g.lev);  // the first four levels
     asm volatile("" ::"s"(active), "s"(stalled), "s"(cur0), "s"(flip), "s"(nlev), "s"(n0), "s"(n1), "s"(Lz0),
                  "s"(Lz1), "s"(Lx0), "s"(Lx1), "s"(Lc0), "s"(Lc1), "s"(flags), "s"(wpre), "s"(tilecnt), "s"(agg),
-                 "s"(local_active), "s"(lev4), "s"(G.gz[0]), "s"(G.gx[0]), "s"(G.gz[W - 1]), "s"(G.gx[W - 1]));
+                 "s"(local_active), "s"(lev4), "s"(G.gz[0]), "s"(G.gx[0]), "s"(G.gz[W - 1]), "s"(G.gx[W - 1]),
+                 "s"(precheck), "s"(appcap));
     if (!active || stalled) return;
     const int cur = cur0 ^ flip;
     const int64_t n = cur ? n1 : n0;
+    if (lb == 0 && threadIdx.x == 0) {
+        s->nstep = n;
+        // the capacity precheck of spv_evolve.jl:356-362 for rotations after
+        // the first of a sequence: the early merge it triggers drops nothing
+        // (every live term passed the strict filter at the last merge)
+        if (precheck && n > appcap) {
+            s->early += 1;
+            s->merge_in += n;
+            s->merge_out += n;
+            s->app_cap_v = 2 * appcap > n ? 2 * appcap : n;
+        }
+    }
     const uint64_t* __restrict__ Lz = cur ? Lz1 : Lz0;
     const uint64_t* __restrict__ Lx = cur ? Lx1 : Lx0;
     const double* __restrict__ Lc = cur ? Lc1 : Lc0;
@@ -407,11 +422,8 @@ __global__ void __launch_bounds__(kBlock) kb_flags(const SegDesc* __restrict__ D
 }
 
 // ---------------------------------------------------------------------------
-// kb_scan: one workgroup per vector.  Resolves the previous merge's parity
-// flip, runs the capacity precheck bookkeeping of spv_evolve.jl:356-362 for
-// rotations after the first of a sequence (where the early merge cannot drop
-// anything: every live term passed the strict filter at the last merge),
-// scans the tile counts and checks the child buffer.
+// kb_scan: one workgroup per vector.  Scans the tile counts, checks the
+// child buffer and resets the merge ticket.
 template <int T>
 __global__ void __launch_bounds__(1024) kb_scan(const SegDesc* __restrict__ D, SegState* __restrict__ S,
                                                 const StepDev* __restrict__ steps, int64_t step_index) {
@@ -424,32 +436,19 @@ __global__ void __launch_bounds__(1024) kb_scan(const SegDesc* __restrict__ D, S
     const SegDesc& d = D[b];
     const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
     // per-vector values in one round trip (see kb_flags)
-    const int active = st->active, precheck = st->precheck, stalled = s->stalled, cur0 = s->cur, flip = s->flip;
-    const int64_t n0 = s->n[0], n1 = s->n[1], appcap = s->app_cap_v;
+    const int active = st->active, stalled = s->stalled;
+    const int64_t n = s->nstep;  // written by kb_flags
     const int32_t* __restrict__ tilecnt = d.tilecnt;
     int64_t* __restrict__ tileoff = d.tileoff;
     const int64_t capC = d.capC;
-    asm volatile("" ::"s"(active), "s"(precheck), "s"(stalled), "s"(cur0), "s"(flip), "s"(n0), "s"(n1), "s"(appcap),
-                 "s"(tilecnt), "s"(tileoff), "s"(capC));
-    const int cur = cur0 ^ flip;
-    const int64_t n = cur ? n1 : n0;
+    asm volatile("" ::"s"(active), "s"(stalled), "s"(n), "s"(tilecnt), "s"(tileoff), "s"(capC));
     const bool go = active && !stalled;
+    if (!go) return;
     if (tid == 0) {
-        if (flip) {
-            s->cur = cur;
-            s->flip = 0;
-        }
         s_carry = 0;
         s->ticket = 0u;
-        if (go && precheck && n > appcap) {
-            s->early += 1;
-            s->merge_in += n;
-            s->merge_out += n;
-            s->app_cap_v = 2 * appcap > n ? 2 * appcap : n;
-        }
     }
     __syncthreads();
-    if (!go) return;
     const int64_t ntiles = (n + kTile - 1) / kTile;
     for (int64_t base = 0; base < ntiles; base += 1024) {
         const int64_t t = base + tid;
@@ -502,7 +501,8 @@ __global__ void __launch_bounds__(kBlock) kb_bounds(const SegDesc* __restrict__ 
     const SegState* s = S + b;
     const SegDesc& d = D[b];
     // per-vector values in one round trip (see kb_flags)
-    const int active = st->active, stalled = s->stalled, cur = s->cur, nlev = st->g.nlev, maxlev = d.maxlev;
+    const int active = st->active, stalled = s->stalled, cur = s->cur ^ s->flip, nlev = st->g.nlev,
+              maxlev = d.maxlev;
     const int64_t n0 = s->n[0], n1 = s->n[1];
     const uint64_t *Lz0 = d.Lz[0], *Lz1 = d.Lz[1], *Lx0 = d.Lx[0], *Lx1 = d.Lx[1];
     const uint64_t* __restrict__ flags = d.flags;
@@ -578,7 +578,8 @@ __global__ void __launch_bounds__(kBlock) kb_bounds(const SegDesc* __restrict__ 
 // covers 64 consecutive elements (coalesced loads), lane = element % 64.
 template <int W, int T>
 __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D, SegState* __restrict__ S,
-                                                  const StepDev* __restrict__ steps, BlockMap bm) {
+                                                  const StepDev* __restrict__ steps, BlockMap bm, int self_off,
+                                                  int64_t step_index) {
     constexpr int kTile = T;
     constexpr int R = kTile / kBlock;  // rounds of 64 consecutive terms per wave
     __shared__ int16_t s_hdb[kTile + 1];
@@ -594,8 +595,9 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
     SegState* s = S + b;
     const SegDesc& d = D[b];
     // per-vector values in one round trip (see kb_flags); m = tileoff[ntiles]
-    const int active = st->active, stalled = s->stalled, cur = s->cur, nlev = st->g.nlev, maxlev = d.maxlev;
-    const int64_t n0 = s->n[0], n1 = s->n[1], mtot = s->m;
+    const int active = st->active, stalled = s->stalled, cur = s->cur ^ s->flip, nlev = st->g.nlev,
+              maxlev = d.maxlev;
+    const int64_t n0 = s->n[0], n1 = s->n[1], m_scan = s->m, capC = d.capC;
     const uint64_t *Lz0 = d.Lz[0], *Lz1 = d.Lz[1], *Lx0 = d.Lx[0], *Lx1 = d.Lx[1];
     const double *Lc0 = d.Lc[0], *Lc1 = d.Lc[1];
     const uint64_t* __restrict__ flags = d.flags;
@@ -610,13 +612,38 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
     GenReg<W> G;
     G.load(&st->g);
     const uint32_t lev4 = *reinterpret_cast<const uint32_t*>(st->g.lev);  // the first four levels
-    asm volatile("" ::"s"(active), "s"(stalled), "s"(cur), "s"(nlev), "s"(maxlev), "s"(n0), "s"(n1), "s"(mtot),
+    asm volatile("" ::"s"(active), "s"(stalled), "s"(cur), "s"(nlev), "s"(maxlev), "s"(n0), "s"(n1), "s"(m_scan),
                  "s"(Lz0), "s"(Lz1), "s"(Lx0), "s"(Lx1), "s"(Lc0), "s"(Lc1), "s"(flags), "s"(wpre), "s"(tileoff),
                  "s"(bnd), "s"(Cz), "s"(Cx), "s"(Cc), "s"(G.gz[0]), "s"(G.gx[0]), "s"(G.gz[W - 1]),
                  "s"(G.gx[W - 1]), "s"(G.sinv), "s"(G.ng), "s"(lev4), "s"(agg), "s"(tilecnt));
     if (!active || stalled) return;
     const int64_t n = cur ? n1 : n0;
     const int64_t ntiles = (n + kTile - 1) / kTile;
+    const int lane0 = threadIdx.x % kWave;
+    // Child offsets: from kb_scan, or (self_off) summed here from the tile
+    // counts, with kb_scan's bookkeeping: m, the child-buffer check, the
+    // merge ticket.  Every wave sums for itself (no barrier needed).
+    int64_t mtot = m_scan, toff_first = 0;
+    if (self_off) {
+        int64_t all = 0, before = 0;
+        for (int64_t u = lane0; u < ntiles; u += kWave) {
+            const int64_t c = tilecnt[u];
+            all += c;
+            before += u < lb ? c : 0;
+        }
+        mtot = wave_sum64(all);
+        toff_first = wave_sum64(before);
+        if (lb == 0 && threadIdx.x == 0) {
+            s->m = mtot;
+            s->ticket = 0u;
+            if (mtot > capC) {
+                s->stalled = 1;
+                s->stall_step = step_index;
+                s->need = mtot;
+            }
+        }
+        if (mtot > capC) return;
+    }
     const uint64_t* __restrict__ Lz = cur ? Lz1 : Lz0;
     const uint64_t* __restrict__ Lx = cur ? Lx1 : Lx0;
     const double* __restrict__ Lc = cur ? Lc1 : Lc0;
@@ -624,6 +651,7 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
     // cross-edge segment counts from kb_flags' summaries (1..kAggLevels
     // levels) or from kb_bounds' table
     const bool summ = nlev >= 1 && nlev <= kAggLevels;
+    int64_t toff_run = toff_first;  // self_off: offset of tile t, advanced over the tiles skipped
     for (int64_t t = lb; t < ntiles; t += nb) {
         const int64_t ts = t * kTile;
         const int Teff = (int)((n - ts) < kTile ? (n - ts) : kTile);
@@ -631,7 +659,12 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
         double C[R];
         int64_t rank[R];
         bool fl[R];
-        const int64_t toff = tileoff[t];
+        if (self_off && t > lb) {  // a further tile of this workgroup: add the counts of [t - nb, t)
+            int64_t add = 0;
+            for (int64_t u = t - nb + lane; u < t; u += kWave) add += tilecnt[u];
+            toff_run += wave_sum64(add);
+        }
+        const int64_t toff = self_off ? toff_run : tileoff[t];
         // boundary counts of the first four levels, loaded with the tile
         // (index clamped into the table when there are fewer levels)
         int64_t bq4[4][4];
@@ -680,7 +713,7 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
             }
         }
         if (threadIdx.x == 0) {
-            s_aloc[Teff] = (int16_t)(tileoff[t + 1] - toff);
+            s_aloc[Teff] = (int16_t)(self_off ? tilecnt[t] : tileoff[t + 1] - toff);
             s_hdb[Teff] = (int16_t)(ts + Teff < n ? hdb<W>(load_key<W>(Lz, Lx, ts + Teff - 1), load_key<W>(Lz, Lx, ts + Teff))
                                                   : kHdbNone);
         }
@@ -826,9 +859,9 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
     const SegDesc& d = D[b];
     const int rot = mode == 0;
     // per-vector values in one round trip (see kb_flags)
-    const int active = st->active, stalled = s->stalled, compact = s->compact, cur = s->cur;
+    const int active = st->active, stalled = s->stalled, compact = s->compact, cur = s->cur ^ s->flip;
     const uint32_t ep0 = *d_epoch;
-    const int64_t n0 = s->n[0], n1 = s->n[1], m0 = s->m, capOut = d.cap;
+    const int64_t nstep = s->nstep, m0 = s->m, capOut = d.cap;
     uint64_t *Lz0 = d.Lz[0], *Lz1 = d.Lz[1], *Lx0 = d.Lx[0], *Lx1 = d.Lx[1];
     double *Lc0 = d.Lc[0], *Lc1 = d.Lc[1];
     const uint64_t* __restrict__ Cz = d.Cz;
@@ -838,7 +871,7 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
     const FilterDev* __restrict__ fs = d.fs;
     GenReg<W> G;
     G.load(&st->g);
-    asm volatile("" ::"s"(active), "s"(stalled), "s"(compact), "s"(cur), "s"(ep0), "s"(n0), "s"(n1), "s"(m0),
+    asm volatile("" ::"s"(active), "s"(stalled), "s"(compact), "s"(cur), "s"(ep0), "s"(nstep), "s"(m0),
                  "s"(capOut), "s"(Lz0), "s"(Lz1), "s"(Lx0), "s"(Lx1), "s"(Lc0), "s"(Lc1), "s"(Cz), "s"(Cx), "s"(Cc),
                  "s"(status), "s"(fs), "s"(G.gz[0]), "s"(G.gx[0]), "s"(G.gz[W - 1]), "s"(G.gx[W - 1]),
                  "s"(G.cosv));
@@ -846,7 +879,11 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
     // launch tag of the look-back words: the sequence's base (device memory,
     // advanced after every sequence) plus this launch's offset in it
     const uint64_t ep = ((uint64_t)(ep0 + ep_off) & kEpochMask) << kEpochShift;
-    const int64_t n = cur ? n1 : n0;
+    // n from the step's snapshot, not n[cur]: the last tile toggles the
+    // parity, and a workgroup without a tile may read it afterwards; its tile
+    // count must not change (the parity is used only by workgroups with a
+    // tile, which read it before the last tile can finish)
+    const int64_t n = nstep;
     const int64_t m = rot ? m0 : 0;
     const int64_t tot = n + m;
     const int64_t ntm = (tot + TM - 1) / TM;
@@ -862,7 +899,7 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
             s->n[cur ^ 1] = 0;
             if (rot) s->rotations += 1;
             else s->compact = 0;
-            s->flip = 1;
+            s->flip ^= 1;
         }
         return;
     }
@@ -1044,7 +1081,7 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
                         if (n_out > s->app_cap_v) s->app_cap_v = 2 * s->app_cap_v > n_out ? 2 * s->app_cap_v : n_out;
                         s->compact = 0;
                     }
-                    s->flip = 1;
+                    s->flip ^= 1;
                 }
             }
         }
@@ -1082,6 +1119,7 @@ __global__ void kb_precheck(SegState* __restrict__ S, const int32_t* __restrict_
     s->ticket = 0u;
     if (!en[b] || s->stalled) return;
     const int64_t n = s->n[s->cur];
+    s->nstep = n;
     if (n > s->app_cap_v) {
         s->compact = 1;
         s->early += 1;
