@@ -543,13 +543,12 @@ struct Batch {
     }
 
     // kb_scan runs when kb_bounds needs the tile offsets or a vector may
-    // hold more than kSelfOffTiles sweep tiles; otherwise kb_emit sums the
+    // hold more than kEmitSelfTiles sweep tiles; otherwise kb_emit sums the
     // tile counts itself (one kernel less per rotation).
-    static constexpr int64_t kSelfOffTiles = 1024;
     bool scan_needed(bool bounds) const {
         if (bounds) return true;
         for (const InstBuf& I : ib)
-            if (cdiv(std::min(I.L[0].cap, I.L[1].cap), (int64_t)tile) > kSelfOffTiles) return true;
+            if (cdiv(std::min(I.L[0].cap, I.L[1].cap), (int64_t)tile) > kEmitSelfTiles) return true;
         return false;
     }
 

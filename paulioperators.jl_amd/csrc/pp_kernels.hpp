@@ -106,6 +106,7 @@ __global__ void kb_block_table(const int64_t* __restrict__ pre, int B, uint64_t*
 //   bit  50    eq_first hdb at the first head == b
 //   bit  51    eq_last  hdb at the last head == b
 constexpr int kAggLevels = 4;
+constexpr int kEmitSelfTiles = 1024;  // kb_emit sums the tile counts itself up to this many tiles
 
 __device__ __forceinline__ int agg_heads(uint64_t a) { return (int)((a >> 48) & 3u); }
 __device__ __forceinline__ int64_t agg_first(uint64_t a) { return (int64_t)(a & 4095u); }
@@ -115,46 +116,71 @@ __device__ __forceinline__ int64_t agg_l2(uint64_t a) { return (int64_t)((a >> 3
 __device__ __forceinline__ bool agg_eq_first(uint64_t a) { return (a >> 50) & 1u; }
 __device__ __forceinline__ bool agg_eq_last(uint64_t a) { return (a >> 51) & 1u; }
 
-// One thread: the summary of one level of one tile from the per-word head,
-// equal-level and A bit masks and the per-word A counts.
+// One wave: the summary of one level of one tile from the per-word head,
+// equal-level and A bit masks and the per-word A counts (lane k holds word k).
 template <int kWords>
-__device__ uint64_t tile_summary(const uint64_t* hm, const uint64_t* em, const uint64_t* am, const int* wcnt) {
-    int f1 = -1, f2 = -1, l1 = -1, l2 = -1;
-    for (int w = 0; w < kWords && f2 < 0; ++w) {
-        uint64_t m = hm[w];
-        while (m && f2 < 0) {
-            const int p = w * 64 + __ffsll((unsigned long long)m) - 1;
-            if (f1 < 0) f1 = p;
-            else f2 = p;
-            m &= m - 1;
-        }
+__device__ uint64_t tile_summary(const uint64_t* hm_w, const uint64_t* em_w, const uint64_t* am_w, const int* wcnt) {
+    static_assert(kWords <= kWave, "one word per lane");
+    const int lane = threadIdx.x % kWave;
+    const bool in = lane < kWords;
+    const uint64_t hm = in ? hm_w[lane] : 0ull;
+    const uint64_t em = in ? em_w[lane] : 0ull;
+    const uint64_t am = in ? am_w[lane] : 0ull;
+    const int cnt = in ? wcnt[lane] : 0;
+    int incl = cnt;
+#pragma unroll
+    for (int dd = 1; dd < kWords; dd <<= 1) {
+        const int u = __shfl_up(incl, dd, kWave);
+        if (lane >= dd) incl += u;
     }
-    for (int w = kWords - 1; w >= 0 && l2 < 0; --w) {
-        uint64_t m = hm[w];
-        while (m && l2 < 0) {
-            const int q = 63 - __clzll((long long)m);
-            const int p = w * 64 + q;
-            if (l1 < 0) l1 = p;
-            else l2 = p;
-            m &= ~(1ull << q);
-        }
-    }
-    // A-count of [tile start, p)
+    const int pre = incl - cnt;
+    const int total = __shfl(incl, kWords - 1, kWave);
+    // A-count of [tile start, p) for a wave-uniform p < kWords * 64
     auto apos = [&](int p) {
-        int c = 0;
-        for (int w = 0; w < (p >> 6); ++w) c += wcnt[w];
+        const int w = p >> 6;
+        const int pw = __shfl(pre, w, kWave);
+        const uint64_t aw = __shfl(am, w, kWave);
         const uint64_t low = (p & 63) ? ((~0ull) >> (64 - (p & 63))) : 0ull;
-        return c + popc64(am[p >> 6] & low);
+        return pw + popc64(aw & low);
     };
-    int total = 0;
-    for (int w = 0; w < kWords; ++w) total += wcnt[w];
+    const uint64_t hw = __ballot(hm != 0ull);  // words holding a head
+    int f1 = -1, f2 = -1, l1 = -1, l2 = -1;
+    if (hw) {
+        const int wf = __ffsll((unsigned long long)hw) - 1;
+        const uint64_t m0 = __shfl(hm, wf, kWave);
+        f1 = wf * 64 + __ffsll((unsigned long long)m0) - 1;
+        const uint64_t m1 = m0 & (m0 - 1);
+        if (m1) {
+            f2 = wf * 64 + __ffsll((unsigned long long)m1) - 1;
+        } else {
+            const uint64_t hw2 = hw & (hw - 1);
+            if (hw2) {
+                const int w2 = __ffsll((unsigned long long)hw2) - 1;
+                f2 = w2 * 64 + __ffsll((unsigned long long)__shfl(hm, w2, kWave)) - 1;
+            }
+        }
+        const int wl = 63 - __clzll((long long)hw);
+        const uint64_t n0 = __shfl(hm, wl, kWave);
+        const int q = 63 - __clzll((long long)n0);
+        l1 = wl * 64 + q;
+        const uint64_t n1 = n0 & ~(1ull << q);
+        if (n1) {
+            l2 = wl * 64 + 63 - __clzll((long long)n1);
+        } else {
+            const uint64_t hw3 = hw & ~(1ull << wl);
+            if (hw3) {
+                const int w3 = 63 - __clzll((long long)hw3);
+                l2 = w3 * 64 + 63 - __clzll((long long)__shfl(hm, w3, kWave));
+            }
+        }
+    }
     const int af = f1 < 0 ? total : apos(f1);
     const int al = l1 < 0 ? 0 : total - apos(l1);
     const int a12 = f2 < 0 ? 0 : apos(f2) - af;
     const int al2 = l2 < 0 ? 0 : apos(l1) - apos(l2);
     const uint64_t heads = f1 < 0 ? 0u : (f2 < 0 ? 1u : 2u);
-    const uint64_t eqf = f1 >= 0 ? ((em[f1 >> 6] >> (f1 & 63)) & 1u) : 0u;
-    const uint64_t eql = l1 >= 0 ? ((em[l1 >> 6] >> (l1 & 63)) & 1u) : 0u;
+    const uint64_t eqf = f1 >= 0 ? ((__shfl(em, f1 >> 6, kWave) >> (f1 & 63)) & 1u) : 0u;
+    const uint64_t eql = l1 >= 0 ? ((__shfl(em, l1 >> 6, kWave) >> (l1 & 63)) & 1u) : 0u;
     return (uint64_t)af | ((uint64_t)al << 12) | ((uint64_t)a12 << 24) | ((uint64_t)al2 << 36) | (heads << 48) |
            (eqf << 50) | (eql << 51);
 }
@@ -414,8 +440,12 @@ __global__ void __launch_bounds__(kBlock) kb_flags(const SegDesc* __restrict__ D
             }
             if (lane < kTileWords && t * kTile + lane * 64 < n) wpre[t * kTileWords + lane] = (uint16_t)(incl - v);
             if (lane == kTileWords - 1) tilecnt[t] = incl;
-        } else if (wave == 1 && summ && lane < nlev) {
-            agg[t * kAggLevels + lane] = tile_summary<kTileWords>(s_hm[lane], s_em[lane], s_am, s_wcnt);
+        } else if (summ) {
+            // waves 1-3: one level each (wave 1 also the fourth)
+            for (int li = wave - 1; li < nlev; li += 3) {
+                const uint64_t a = tile_summary<kTileWords>(s_hm[li], s_em[li], s_am, s_wcnt);
+                if (lane == 0) agg[t * kAggLevels + li] = a;
+            }
         }
         __syncthreads();
     }
@@ -572,6 +602,25 @@ __global__ void __launch_bounds__(kBlock) kb_bounds(const SegDesc* __restrict__ 
     }
 }
 
+// Run heads of one level in a tile as bit masks (word c: positions
+// 64c..64c+63; position 0 is always a head): the first head after p (Teff if
+// none) and the last head at or before p.
+__device__ __forceinline__ int next_head(const uint64_t* hm, int p, int Teff, int nwords) {
+    int c = p >> 6;
+    uint64_t m = (p & 63) == 63 ? 0ull : hm[c] & (~0ull << ((p & 63) + 1));
+    while (!m) {
+        if (++c >= nwords) return Teff;
+        m = hm[c];
+    }
+    return c * 64 + __ffsll((unsigned long long)m) - 1;
+}
+__device__ __forceinline__ int last_head(const uint64_t* hm, int p) {
+    int c = p >> 6;
+    uint64_t m = hm[c] & ((p & 63) == 63 ? ~0ull : ((2ull << (p & 63)) - 1));
+    while (!m) m = hm[--c];
+    return c * 64 + 63 - __clzll((long long)m);
+}
+
 // ---------------------------------------------------------------------------
 // kb_emit: children written at their rank in key order.
 // Layout inside a tile: wave w owns elements [w T/4, (w+1) T/4), round r
@@ -588,6 +637,7 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
     __shared__ int16_t s_e[kTile];
     __shared__ int s_wred[4];
     __shared__ int64_t s_bq[kAggLevels][4];
+    __shared__ uint64_t s_lhm[kAggLevels][kTile / 64];  // run heads per level (summary path)
     int b;
     int64_t lb, nb;
     map_block(bm, b, lb, nb);
@@ -621,28 +671,22 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
     const int64_t ntiles = (n + kTile - 1) / kTile;
     const int lane0 = threadIdx.x % kWave;
     // Child offsets: from kb_scan, or (self_off) summed here from the tile
-    // counts, with kb_scan's bookkeeping: m, the child-buffer check, the
-    // merge ticket.  Every wave sums for itself (no barrier needed).
-    int64_t mtot = m_scan, toff_first = 0;
-    if (self_off) {
-        int64_t all = 0, before = 0;
-        for (int64_t u = lane0; u < ntiles; u += kWave) {
-            const int64_t c = tilecnt[u];
-            all += c;
-            before += u < lb ? c : 0;
-        }
-        mtot = wave_sum64(all);
-        toff_first = wave_sum64(before);
+    // counts, with kb_scan's bookkeeping (m, the child-buffer check, the merge
+    // ticket).  The counts (at most kEmitSelfTiles) are loaded now and summed
+    // after the first tile's own loads are issued: both share one memory
+    // round trip.  Every wave sums for itself.
+    int32_t cpre[kEmitSelfTiles / kWave];
+#pragma unroll
+    for (int k = 0; k < kEmitSelfTiles / kWave; ++k) {
+        const int64_t u = lane0 + (int64_t)k * kWave;
+        cpre[k] = (self_off && u < ntiles) ? tilecnt[u] : 0;
+    }
+    if (self_off && ntiles == 0) {
         if (lb == 0 && threadIdx.x == 0) {
-            s->m = mtot;
+            s->m = 0;
             s->ticket = 0u;
-            if (mtot > capC) {
-                s->stalled = 1;
-                s->stall_step = step_index;
-                s->need = mtot;
-            }
         }
-        if (mtot > capC) return;
+        return;
     }
     const uint64_t* __restrict__ Lz = cur ? Lz1 : Lz0;
     const uint64_t* __restrict__ Lx = cur ? Lx1 : Lx0;
@@ -651,7 +695,7 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
     // cross-edge segment counts from kb_flags' summaries (1..kAggLevels
     // levels) or from kb_bounds' table
     const bool summ = nlev >= 1 && nlev <= kAggLevels;
-    int64_t toff_run = toff_first;  // self_off: offset of tile t, advanced over the tiles skipped
+    int64_t mtot = m_scan;
     for (int64_t t = lb; t < ntiles; t += nb) {
         const int64_t ts = t * kTile;
         const int Teff = (int)((n - ts) < kTile ? (n - ts) : kTile);
@@ -659,12 +703,8 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
         double C[R];
         int64_t rank[R];
         bool fl[R];
-        if (self_off && t > lb) {  // a further tile of this workgroup: add the counts of [t - nb, t)
-            int64_t add = 0;
-            for (int64_t u = t - nb + lane; u < t; u += kWave) add += tilecnt[u];
-            toff_run += wave_sum64(add);
-        }
-        const int64_t toff = self_off ? toff_run : tileoff[t];
+        int hdv[R];
+        const int64_t toff_scan = self_off ? 0 : tileoff[t];
         // boundary counts of the first four levels, loaded with the tile
         // (index clamped into the table when there are fewer levels)
         int64_t bq4[4][4];
@@ -690,14 +730,54 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
                 const uint64_t lowmask = lane ? ((~0ull) >> (64 - lane)) : 0ull;
                 const int aloc = (int)wpre[i >> 6] + popc64(word & lowmask);
                 s_aloc[j] = (int16_t)aloc;
-                s_hdb[j] = (int16_t)(i == 0 ? kHdbNone : hdb<W>(load_key<W>(Lz, Lx, i - 1), K[r]));
-                rank[r] = toff + aloc;
+                hdv[r] = i == 0 ? (int)kHdbNone : hdb<W>(load_key<W>(Lz, Lx, i - 1), K[r]);
+                s_hdb[j] = (int16_t)hdv[r];
+                rank[r] = aloc;
                 if (fl[r]) C[r] = Lc[i];
             } else {
+                hdv[r] = -1;
 #pragma unroll
                 for (int w = 0; w < W; ++w) K[r].z[w] = K[r].x[w] = 0;
             }
         }
+        if (summ) {
+            // head masks per level (a tile position 0 is a head of the tile's view)
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int j = wave * (kTile / 4) + r * kWave + lane;
+#pragma unroll
+                for (int li = 0; li < kAggLevels; ++li) {
+                    if (li < nlev) {
+                        const int lvl = (int)((lev4 >> (8 * li)) & 255u);
+                        const uint64_t hm = __ballot(j < Teff && (j == 0 || hdv[r] >= lvl));
+                        if (lane == 0) s_lhm[li][wave * R + r] = hm;
+                    }
+                }
+            }
+        }
+        int64_t toff = toff_scan;
+        if (self_off) {
+            int64_t all = 0, before = 0;
+#pragma unroll
+            for (int k = 0; k < kEmitSelfTiles / kWave; ++k) {
+                all += cpre[k];
+                before += lane + (int64_t)k * kWave < t ? cpre[k] : 0;
+            }
+            mtot = wave_sum64(all);
+            toff = wave_sum64(before);
+            if (t == 0 && threadIdx.x == 0) {
+                s->m = mtot;
+                s->ticket = 0u;
+                if (mtot > capC) {
+                    s->stalled = 1;
+                    s->stall_step = step_index;
+                    s->need = mtot;
+                }
+            }
+            if (mtot > capC) return;  // the same for every workgroup of the vector: nothing is written
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) rank[r] += toff;
         if (summ) {
             // wave w: the tile's start edge (w even) or end edge (w odd) at
             // levels w / 2 and w / 2 + 2
@@ -713,18 +793,57 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
             }
         }
         if (threadIdx.x == 0) {
-            s_aloc[Teff] = (int16_t)(self_off ? tilecnt[t] : tileoff[t + 1] - toff);
+            s_aloc[Teff] = (int16_t)(self_off ? tilecnt[t] : tileoff[t + 1] - toff_scan);
             s_hdb[Teff] = (int16_t)(ts + Teff < n ? hdb<W>(load_key<W>(Lz, Lx, ts + Teff - 1), load_key<W>(Lz, Lx, ts + Teff))
                                                   : kHdbNone);
         }
         __syncthreads();
-        for (int li = 0; li < nlev; ++li) {
+        if (summ) {
+            // the level loop on the head masks: no scans, no barriers
+            for (int li = 0; li < nlev; ++li) {
+                const int lvl = (int)((lev4 >> (8 * li)) & 255u);
+                const uint64_t* hm = s_lhm[li];
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if (!fl[r]) continue;
+                    const int j = wave * (kTile / 4) + r * kWave + lane;
+                    int64_t cnt = 0;
+                    if (key_bit<W>(K[r], lvl) == 0) {
+                        const int ee = next_head(hm, j, Teff, kTile / 64);
+                        if (ee < Teff) {
+                            if (s_hdb[ee] == lvl) {
+                                const int f = next_head(hm, ee, Teff, kTile / 64);
+                                cnt = (int64_t)s_aloc[f] - s_aloc[ee];
+                                if (f == Teff && s_hdb[Teff] < lvl) cnt += s_bq[li][2];
+                            }
+                        } else {
+                            const int h = s_hdb[Teff];
+                            cnt = h == lvl ? s_bq[li][2] : (h < lvl ? s_bq[li][3] : 0);
+                        }
+                        rank[r] += cnt;
+                    } else {
+                        const int sv = last_head(hm, j);
+                        if (sv > 0) {
+                            if (s_hdb[sv] == lvl) {
+                                const int s2 = last_head(hm, sv - 1);
+                                cnt = (int64_t)s_aloc[sv] - s_aloc[s2];
+                                if (s2 == 0 && s_hdb[0] < lvl) cnt += s_bq[li][0];
+                            }
+                        } else {
+                            const int h = s_hdb[0];
+                            cnt = h == lvl ? s_bq[li][0] : (h < lvl ? s_bq[li][1] : 0);
+                        }
+                        rank[r] -= cnt;
+                    }
+                }
+            }
+        }
+        for (int li = 0; li < (summ ? 0 : nlev); ++li) {
             const int lvl = li < 4 ? (int)((lev4 >> (8 * li)) & 255u) : (int)st->g.lev[li];
             int64_t bq[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                bq[q] = summ      ? s_bq[li][q]
-                        : li == 0 ? bq4[0][q]
+                bq[q] = li == 0   ? bq4[0][q]
                         : li == 1 ? bq4[1][q]
                         : li == 2 ? bq4[2][q]
                         : li == 3 ? bq4[3][q]
@@ -850,6 +969,8 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
     __shared__ int64_t s_tile;
     __shared__ int64_t s_base;
     __shared__ int64_t s_split[2];
+    __shared__ Key<W> s_edge[2];   // C[j1] (first child after the tile), L[i0 - 1] (last live term before it)
+    __shared__ double s_edge_c;    // c of C[j1]
     __shared__ int s_wsum[4];
     int b;
     int64_t lb, nb;
@@ -943,6 +1064,12 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
                 s_c[k] = Cc[j0 + (k - nL)];
             }
         }
+        // the neighbours across the tile's edges, loaded with the tile
+        if (tid == kBlock - 1 && j1 < m) {
+            s_edge[0] = load_key<W>(Cz, Cx, j1);
+            s_edge_c = Cc[j1];
+        }
+        if (tid == kBlock - 2 && i0 > 0) s_edge[1] = load_key<W>(Lz, Lx, i0 - 1);
         __syncthreads();
         const Key<W>* sL = s_key;
         const Key<W>* sC = s_key + nL;
@@ -975,7 +1102,7 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
                 if (jb < nC) {
                     if (key_eq<W>(sC[jb], K)) acc = acc + sCc[jb];
                 } else if (j1 < m) {
-                    if (key_eq<W>(load_key<W>(Cz, Cx, j1), K)) acc = acc + Cc[j1];
+                    if (key_eq<W>(s_edge[0], K)) acc = acc + s_edge_c;
                 }
             } else {
                 K = sC[jb];
@@ -985,7 +1112,7 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
                 if (ia > 0) {
                     if (key_eq<W>(sL[ia - 1], K)) emit = false;
                 } else if (i0 > 0) {
-                    if (key_eq<W>(load_key<W>(Lz, Lx, i0 - 1), K)) emit = false;
+                    if (key_eq<W>(s_edge[1], K)) emit = false;
                 }
             }
             if (emit && !should_drop<W>(fs, K, fabs(acc))) {
