@@ -142,23 +142,50 @@ __device__ __forceinline__ int majorana_weight(const Key<W>& k) {
 }
 
 // spv_kernels.jl:64-79.
-template <int W>
-__device__ __forceinline__ bool should_drop(const FilterDev* __restrict__ f, const Key<W>& k, double absc) {
-    if (absc <= f->thresh) return true;
+// The scalar criteria of a FilterDev held in registers (loaded once per
+// kernel; the exp tables stay in memory, read only by damped filters).
+struct FilterReg {
+    int64_t wmax, xwmax, mwmax;
+    double thresh, dthresh_w, dthresh_xw;
+    const double* exptab_w;
+    const double* exptab_xw;
+    __device__ __forceinline__ void load(const FilterDev* f) {
+        wmax = f->wmax;
+        xwmax = f->xwmax;
+        mwmax = f->mwmax;
+        thresh = f->thresh;
+        dthresh_w = f->dthresh_w;
+        dthresh_xw = f->dthresh_xw;
+        exptab_w = f->exptab_w;
+        exptab_xw = f->exptab_xw;
+    }
+};
+
+template <int W, typename F>
+__device__ __forceinline__ bool should_drop_f(const F& f, const Key<W>& k, double absc) {
+    if (absc <= f.thresh) return true;
     const int w = popc_or<W>(k);
-    if ((int64_t)w > f->wmax) return true;
+    if ((int64_t)w > f.wmax) return true;
     const int xw = popc_x<W>(k);
-    if ((int64_t)xw > f->xwmax) return true;
-    if (f->mwmax != INT64_MAX) {
-        if ((int64_t)majorana_weight<W>(k) > f->mwmax) return true;
+    if ((int64_t)xw > f.xwmax) return true;
+    if (f.mwmax != INT64_MAX) {
+        if ((int64_t)majorana_weight<W>(k) > f.mwmax) return true;
     }
-    if (f->dthresh_w >= 0.0) {
-        if (absc * f->exptab_w[w] <= f->dthresh_w) return true;
+    if (f.dthresh_w >= 0.0) {
+        if (absc * f.exptab_w[w] <= f.dthresh_w) return true;
     }
-    if (f->dthresh_xw >= 0.0) {
-        if (absc * f->exptab_xw[xw] <= f->dthresh_xw) return true;
+    if (f.dthresh_xw >= 0.0) {
+        if (absc * f.exptab_xw[xw] <= f.dthresh_xw) return true;
     }
     return false;
+}
+template <int W>
+__device__ __forceinline__ bool should_drop(const FilterDev* __restrict__ f, const Key<W>& k, double absc) {
+    return should_drop_f<W>(*f, k, absc);
+}
+template <int W>
+__device__ __forceinline__ bool should_drop(const FilterReg& f, const Key<W>& k, double absc) {
+    return should_drop_f<W>(f, k, absc);
 }
 
 template <int W>

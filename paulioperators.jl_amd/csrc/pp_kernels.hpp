@@ -989,13 +989,14 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
     const uint64_t* __restrict__ Cx = d.Cx;
     const double* __restrict__ Cc = d.Cc;
     uint64_t* __restrict__ status = d.status;
-    const FilterDev* __restrict__ fs = d.fs;
+    FilterReg fs;
+    fs.load(d.fs);
     GenReg<W> G;
     G.load(&st->g);
     asm volatile("" ::"s"(active), "s"(stalled), "s"(compact), "s"(cur), "s"(ep0), "s"(nstep), "s"(m0),
                  "s"(capOut), "s"(Lz0), "s"(Lz1), "s"(Lx0), "s"(Lx1), "s"(Lc0), "s"(Lc1), "s"(Cz), "s"(Cx), "s"(Cc),
-                 "s"(status), "s"(fs), "s"(G.gz[0]), "s"(G.gx[0]), "s"(G.gz[W - 1]), "s"(G.gx[W - 1]),
-                 "s"(G.cosv));
+                 "s"(status), "s"(G.gz[0]), "s"(G.gx[0]), "s"(G.gz[W - 1]), "s"(G.gx[W - 1]), "s"(G.cosv),
+                 "s"(fs.wmax), "s"(fs.xwmax), "s"(fs.mwmax), "s"(fs.thresh), "s"(fs.dthresh_w), "s"(fs.dthresh_xw));
     if (stalled || !(rot ? active : compact)) return;
     // launch tag of the look-back words: the sequence's base (device memory,
     // advanced after every sequence) plus this launch's offset in it
@@ -1084,11 +1085,19 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
             else hi = mid;
         }
         int ia = lo, jb = dk - lo;
+        // outputs in registers indexed by the unrolled position (a run-time
+        // index would place the arrays in scratch memory)
         Key<W> outK[E];
         double outC[E];
-        int cnt = 0;
+        bool outE[E];
         const int stop = dk + E < nL + nC ? dk + E : nL + nC;
-        for (int pos = dk; pos < stop; ++pos) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            outE[e] = false;
+#pragma unroll
+            for (int w = 0; w < W; ++w) outK[e].z[w] = outK[e].x[w] = 0;
+            outC[e] = 0.0;
+            if (dk + e >= stop) continue;
             const bool takeL = ia < nL && (jb >= nC || key_le<W>(sL[ia], sC[jb]));
             Key<W> K;
             double acc;
@@ -1115,12 +1124,13 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
                     if (key_eq<W>(s_edge[1], K)) emit = false;
                 }
             }
-            if (emit && !should_drop<W>(fs, K, fabs(acc))) {
-                outK[cnt] = K;
-                outC[cnt] = acc;
-                ++cnt;
-            }
+            outK[e] = K;
+            outC[e] = acc;
+            outE[e] = emit && !should_drop<W>(fs, K, fabs(acc));
         }
+        int cnt = 0;
+#pragma unroll
+        for (int e = 0; e < E; ++e) cnt += outE[e] ? 1 : 0;
         int incl = cnt;
 #pragma unroll
         for (int dd = 1; dd < kWave; dd <<= 1) {
@@ -1214,9 +1224,16 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
         }
         __syncthreads();
         const int64_t base = s_base;
-        for (int k = 0; k < cnt; ++k) {
-            s_key[myoff + k] = outK[k];
-            s_c[myoff + k] = outC[k];
+        {
+            int k = myoff;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                if (outE[e]) {
+                    s_key[k] = outK[e];
+                    s_c[k] = outC[e];
+                    ++k;
+                }
+            }
         }
         __syncthreads();
         if (base + total <= capOut) {
