@@ -17,15 +17,43 @@ struct Key {
 
 __device__ __forceinline__ int popc64(uint64_t v) { return __popcll(v); }
 
+// Device (global) memory accessors.  Buffers reached through descriptor
+// structs are generic pointers to the compiler, whose accesses become FLAT
+// instructions; FLAT loads also count against the LDS wait counter, so an
+// LDS wait would wait for them too.  Accessing through a global-address-space
+// pointer gives global instructions (every buffer here is device memory).
+#define PP_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ T ldg(const T* p, int64_t i) {
+    return ((const PP_GLOBAL T*)p)[i];
+}
+template <typename T>
+__device__ __forceinline__ void stg(T* p, int64_t i, T v) {
+    ((PP_GLOBAL T*)p)[i] = v;
+}
+// 16-byte key words through the builtin vector type (HIP's ulonglong2 is a
+// class, which does not copy across address spaces)
+typedef unsigned long long pp_u64x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ ulonglong2 ldg(const ulonglong2* p, int64_t i) {
+    const pp_u64x2 v = ((const PP_GLOBAL pp_u64x2*)p)[i];
+    return make_ulonglong2(v.x, v.y);
+}
+__device__ __forceinline__ void stg(ulonglong2* p, int64_t i, ulonglong2 v) {
+    pp_u64x2 w;
+    w.x = v.x;
+    w.y = v.y;
+    ((PP_GLOBAL pp_u64x2*)p)[i] = w;
+}
+
 template <int W>
 __device__ __forceinline__ Key<W> load_key(const uint64_t* __restrict__ Z, const uint64_t* __restrict__ X, int64_t i) {
     Key<W> k;
     if constexpr (W == 1) {
-        k.z[0] = Z[i];
-        k.x[0] = X[i];
+        k.z[0] = ldg(Z, i);
+        k.x[0] = ldg(X, i);
     } else {
-        const ulonglong2 a = reinterpret_cast<const ulonglong2*>(Z)[i];
-        const ulonglong2 b = reinterpret_cast<const ulonglong2*>(X)[i];
+        const ulonglong2 a = ldg(reinterpret_cast<const ulonglong2*>(Z), i);
+        const ulonglong2 b = ldg(reinterpret_cast<const ulonglong2*>(X), i);
         k.z[0] = a.x; k.z[1] = a.y;
         k.x[0] = b.x; k.x[1] = b.y;
     }
@@ -35,11 +63,11 @@ __device__ __forceinline__ Key<W> load_key(const uint64_t* __restrict__ Z, const
 template <int W>
 __device__ __forceinline__ void store_key(uint64_t* __restrict__ Z, uint64_t* __restrict__ X, int64_t i, const Key<W>& k) {
     if constexpr (W == 1) {
-        Z[i] = k.z[0];
-        X[i] = k.x[0];
+        stg(Z, i, k.z[0]);
+        stg(X, i, k.x[0]);
     } else {
-        reinterpret_cast<ulonglong2*>(Z)[i] = make_ulonglong2(k.z[0], k.z[1]);
-        reinterpret_cast<ulonglong2*>(X)[i] = make_ulonglong2(k.x[0], k.x[1]);
+        stg(reinterpret_cast<ulonglong2*>(Z), i, make_ulonglong2(k.z[0], k.z[1]));
+        stg(reinterpret_cast<ulonglong2*>(X), i, make_ulonglong2(k.x[0], k.x[1]));
     }
 }
 
@@ -172,10 +200,10 @@ __device__ __forceinline__ bool should_drop_f(const F& f, const Key<W>& k, doubl
         if ((int64_t)majorana_weight<W>(k) > f.mwmax) return true;
     }
     if (f.dthresh_w >= 0.0) {
-        if (absc * f.exptab_w[w] <= f.dthresh_w) return true;
+        if (absc * ldg((const double*)f.exptab_w, w) <= f.dthresh_w) return true;
     }
     if (f.dthresh_xw >= 0.0) {
-        if (absc * f.exptab_xw[xw] <= f.dthresh_xw) return true;
+        if (absc * ldg((const double*)f.exptab_xw, xw) <= f.dthresh_xw) return true;
     }
     return false;
 }
@@ -231,10 +259,10 @@ struct GenReg {
 template <int T>
 __device__ __forceinline__ int64_t a_prefix(int64_t p, int64_t n, int64_t ntiles, const uint64_t* __restrict__ flags,
                                             const uint16_t* __restrict__ wpre, const int64_t* __restrict__ tileoff) {
-    if (p >= n) return tileoff[ntiles];
+    if (p >= n) return ldg(tileoff, ntiles);
     const int64_t w = p >> 6;
     const uint64_t lowmask = (p & 63) ? ((~0ull) >> (64 - (p & 63))) : 0ull;
-    return tileoff[p / T] + (int64_t)wpre[w] + popc64(flags[w] & lowmask);
+    return ldg(tileoff, p / T) + (int64_t)ldg(wpre, w) + popc64(ldg(flags, w) & lowmask);
 }
 
 // First q <= p whose key agrees with K_p on every bit >= b.

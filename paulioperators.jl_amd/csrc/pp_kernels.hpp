@@ -207,8 +207,8 @@ __device__ AggHit agg_back(const uint64_t* __restrict__ agg, const int32_t* __re
         uint64_t a = 0;
         int64_t c = 0;
         if (u >= 0) {
-            a = agg[u * kAggLevels + li];
-            c = tilecnt[u];
+            a = ldg(agg, u * kAggLevels + li);
+            c = ldg(tilecnt, u);
         }
         const uint64_t hm = __ballot(u >= 0 && agg_heads(a) != 0);
         if (hm) {
@@ -234,8 +234,8 @@ __device__ AggHit agg_fwd(const uint64_t* __restrict__ agg, const int32_t* __res
         uint64_t a = 0;
         int64_t c = 0;
         if (u < ntiles) {
-            a = agg[u * kAggLevels + li];
-            c = tilecnt[u];
+            a = ldg(agg, u * kAggLevels + li);
+            c = ldg(tilecnt, u);
         }
         const uint64_t hm = __ballot(u < ntiles && agg_heads(a) != 0);
         if (hm) {
@@ -281,12 +281,12 @@ __device__ void edge_counts(int side, int li, int lvl, int64_t t, int64_t ntiles
         const int h = hdb<W>(load_key<W>(Lz, Lx, te - 1), ka);
         if (h > lvl) return;
         if (h == lvl) {  // te is a head: the run [te, next head)
-            const uint64_t a = agg[(t + 1) * kAggLevels + li];
+            const uint64_t a = ldg(agg, (t + 1) * kAggLevels + li);
             if (agg_heads(a) >= 2) {
                 same = agg_12(a);
             } else {
                 const AggHit r = agg_fwd(agg, tilecnt, t + 2, ntiles, li);
-                same = (int64_t)tilecnt[t + 1] + r.sum + (r.u < ntiles ? agg_first(r.a) : 0);
+                same = (int64_t)ldg(tilecnt, t + 1) + r.sum + (r.u < ntiles ? agg_first(r.a) : 0);
             }
             return;
         }
@@ -359,6 +359,8 @@ __global__ void __launch_bounds__(kBlock) kb_flags(const SegDesc* __restrict__ D
     const double* __restrict__ Lc = cur ? Lc1 : Lc0;
     const int64_t ntiles = (n + kTile - 1) / kTile;
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    FilterReg floc;  // the local filter, when it can drop children
+    if (local_active) floc.load(d.fl);
     // segment summaries for kb_emit when the mask has 1..kAggLevels levels
     const bool summ = nlev >= 1 && nlev <= kAggLevels;
     bool need_z = false, need_x = false;
@@ -380,28 +382,28 @@ __global__ void __launch_bounds__(kBlock) kb_flags(const SegDesc* __restrict__ D
             for (int w = 0; w < W; ++w) k.z[w] = k.x[w] = 0;
             if (valid) {
                 if constexpr (W == 1) {
-                    k.z[0] = need_z ? Lz[i] : 0;
-                    k.x[0] = need_x ? Lx[i] : 0;
+                    k.z[0] = need_z ? ldg(Lz, i) : 0;
+                    k.x[0] = need_x ? ldg(Lx, i) : 0;
                 } else {
                     if (need_z) {
-                        const ulonglong2 a = reinterpret_cast<const ulonglong2*>(Lz)[i];
+                        const ulonglong2 a = ldg(reinterpret_cast<const ulonglong2*>(Lz), i);
                         k.z[0] = a.x; k.z[1] = a.y;
                     }
                     if (need_x) {
-                        const ulonglong2 c2 = reinterpret_cast<const ulonglong2*>(Lx)[i];
+                        const ulonglong2 c2 = ldg(reinterpret_cast<const ulonglong2*>(Lx), i);
                         k.x[0] = c2.x; k.x[1] = c2.y;
                     }
                 }
                 flag = G.anti(k);
                 if (flag && local_active) {
                     Key<W> ch;
-                    const double cn = G.child(k, Lc[i], ch);
-                    flag = !should_drop<W>(d.fl, ch, fabs(cn));
+                    const double cn = G.child(k, ldg(Lc, i), ch);
+                    flag = !should_drop<W>(floc, ch, fabs(cn));
                 }
             }
             const uint64_t mask = __ballot(flag);
             if (lane == 0) {
-                if (t * kTile + wi * 64 < n) flags[t * kTileWords + wi] = mask;
+                if (t * kTile + wi * 64 < n) stg(flags, t * kTileWords + wi, mask);
                 s_wcnt[wi] = popc64(mask);
                 s_am[wi] = mask;
             }
@@ -438,13 +440,13 @@ __global__ void __launch_bounds__(kBlock) kb_flags(const SegDesc* __restrict__ D
                 const int u = __shfl_up(incl, dd, kWave);
                 if (lane >= dd) incl += u;
             }
-            if (lane < kTileWords && t * kTile + lane * 64 < n) wpre[t * kTileWords + lane] = (uint16_t)(incl - v);
-            if (lane == kTileWords - 1) tilecnt[t] = incl;
+            if (lane < kTileWords && t * kTile + lane * 64 < n) stg(wpre, t * kTileWords + lane, (uint16_t)(incl - v));
+            if (lane == kTileWords - 1) stg(tilecnt, t, (int32_t)incl);
         } else if (summ) {
             // waves 1-3: one level each (wave 1 also the fourth)
             for (int li = wave - 1; li < nlev; li += 3) {
                 const uint64_t a = tile_summary<kTileWords>(s_hm[li], s_em[li], s_am, s_wcnt);
-                if (lane == 0) agg[t * kAggLevels + li] = a;
+                if (lane == 0) stg(agg, t * kAggLevels + li, a);
             }
         }
         __syncthreads();
@@ -482,7 +484,7 @@ __global__ void __launch_bounds__(1024) kb_scan(const SegDesc* __restrict__ D, S
     const int64_t ntiles = (n + kTile - 1) / kTile;
     for (int64_t base = 0; base < ntiles; base += 1024) {
         const int64_t t = base + tid;
-        const int64_t v = t < ntiles ? tilecnt[t] : 0;
+        const int64_t v = t < ntiles ? ldg(tilecnt, t) : 0;
         int64_t incl = v;
 #pragma unroll
         for (int dd = 1; dd < kWave; dd <<= 1) {
@@ -494,14 +496,14 @@ __global__ void __launch_bounds__(1024) kb_scan(const SegDesc* __restrict__ D, S
         int64_t wpref = 0;
         for (int w = 0; w < wave; ++w) wpref += s_w[w];
         const int64_t carry = s_carry;
-        if (t < ntiles) tileoff[t] = carry + wpref + incl - v;
+        if (t < ntiles) stg(tileoff, t, carry + wpref + incl - v);
         __syncthreads();
         if (tid == 1023) s_carry = carry + wpref + incl;
         __syncthreads();
     }
     if (tid == 0) {
         const int64_t m = s_carry;
-        tileoff[ntiles] = m;
+        stg(tileoff, ntiles, m);
         s->m = m;
         if (m > capC) {
             s->stalled = 1;
@@ -595,8 +597,8 @@ __global__ void __launch_bounds__(kBlock) kb_bounds(const SegDesc* __restrict__ 
                 }
             }
             if (lane == 0) {
-                bnd[(t * maxlev + li) * 4 + 2 * side + 0] = same;
-                bnd[(t * maxlev + li) * 4 + 2 * side + 1] = sib;
+                stg(bnd, (t * maxlev + li) * 4 + 2 * side + 0, same);
+                stg(bnd, (t * maxlev + li) * 4 + 2 * side + 1, sib);
             }
         }
     }
@@ -679,7 +681,7 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
 #pragma unroll
     for (int k = 0; k < kEmitSelfTiles / kWave; ++k) {
         const int64_t u = lane0 + (int64_t)k * kWave;
-        cpre[k] = (self_off && u < ntiles) ? tilecnt[u] : 0;
+        cpre[k] = (self_off && u < ntiles) ? ldg(tilecnt, u) : 0;
     }
     if (self_off && ntiles == 0) {
         if (lb == 0 && threadIdx.x == 0) {
@@ -704,7 +706,7 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
         int64_t rank[R];
         bool fl[R];
         int hdv[R];
-        const int64_t toff_scan = self_off ? 0 : tileoff[t];
+        const int64_t toff_scan = self_off ? 0 : ldg(tileoff, t);
         // boundary counts of the first four levels, loaded with the tile
         // (index clamped into the table when there are fewer levels)
         int64_t bq4[4][4];
@@ -713,7 +715,7 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
             for (int li = 0; li < 4; ++li) {
                 const int lc = li < nlev ? li : 0;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) bq4[li][q] = bnd[(t * maxlev + lc) * 4 + q];
+                for (int q = 0; q < 4; ++q) bq4[li][q] = ldg(bnd, (t * maxlev + lc) * 4 + q);
             }
         }
 #pragma unroll
@@ -725,15 +727,15 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
             C[r] = 0.0;
             if (j < Teff) {
                 K[r] = load_key<W>(Lz, Lx, i);
-                const uint64_t word = flags[i >> 6];
+                const uint64_t word = ldg(flags, i >> 6);
                 fl[r] = (word >> lane) & 1ull;
                 const uint64_t lowmask = lane ? ((~0ull) >> (64 - lane)) : 0ull;
-                const int aloc = (int)wpre[i >> 6] + popc64(word & lowmask);
+                const int aloc = (int)ldg(wpre, i >> 6) + popc64(word & lowmask);
                 s_aloc[j] = (int16_t)aloc;
                 hdv[r] = i == 0 ? (int)kHdbNone : hdb<W>(load_key<W>(Lz, Lx, i - 1), K[r]);
                 s_hdb[j] = (int16_t)hdv[r];
                 rank[r] = aloc;
-                if (fl[r]) C[r] = Lc[i];
+                if (fl[r]) C[r] = ldg(Lc, i);
             } else {
                 hdv[r] = -1;
 #pragma unroll
@@ -793,7 +795,7 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
             }
         }
         if (threadIdx.x == 0) {
-            s_aloc[Teff] = (int16_t)(self_off ? tilecnt[t] : tileoff[t + 1] - toff_scan);
+            s_aloc[Teff] = (int16_t)(self_off ? ldg(tilecnt, t) : ldg(tileoff, t + 1) - toff_scan);
             s_hdb[Teff] = (int16_t)(ts + Teff < n ? hdb<W>(load_key<W>(Lz, Lx, ts + Teff - 1), load_key<W>(Lz, Lx, ts + Teff))
                                                   : kHdbNone);
         }
@@ -847,7 +849,7 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
                         : li == 1 ? bq4[1][q]
                         : li == 2 ? bq4[2][q]
                         : li == 3 ? bq4[3][q]
-                                  : bnd[(t * maxlev + li) * 4 + q];
+                                  : ldg(bnd, (t * maxlev + li) * 4 + q);
             // forward: st_j = last head <= j (j = 0 is always a head)
             int stv[R];
             int carry = -1;
@@ -946,7 +948,7 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
             Key<W> ch;
             const double cn = G.child(K[r], C[r], ch);
             store_key<W>(Cz, Cx, rank[r], ch);
-            Cc[rank[r]] = cn;
+            stg(Cc, rank[r], cn);
         }
         __syncthreads();
     }
@@ -1059,16 +1061,16 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
         for (int k = tid; k < nL + nC; k += kBlock) {
             if (k < nL) {
                 s_key[k] = load_key<W>(Lz, Lx, i0 + k);
-                s_c[k] = Lc[i0 + k];
+                s_c[k] = ldg(Lc, i0 + k);
             } else {
                 s_key[k] = load_key<W>(Cz, Cx, j0 + (k - nL));
-                s_c[k] = Cc[j0 + (k - nL)];
+                s_c[k] = ldg(Cc, j0 + (k - nL));
             }
         }
         // the neighbours across the tile's edges, loaded with the tile
         if (tid == kBlock - 1 && j1 < m) {
             s_edge[0] = load_key<W>(Cz, Cx, j1);
-            s_edge_c = Cc[j1];
+            s_edge_c = ldg(Cc, j1);
         }
         if (tid == kBlock - 2 && i0 > 0) s_edge[1] = load_key<W>(Lz, Lx, i0 - 1);
         __syncthreads();
@@ -1152,20 +1154,20 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
         if (wave == 0) {
             if (t == 0) {
                 if (lane == 0) {
-                    __hip_atomic_store(&status[0], kFlagPre | ep | (uint64_t)total, __ATOMIC_RELAXED,
+                    __hip_atomic_store((PP_GLOBAL uint64_t*)&status[0], kFlagPre | ep | (uint64_t)total, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
                     s_base = 0;
                 }
             } else {
                 if (lane == 0)
-                    __hip_atomic_store(&status[t], kFlagAgg | ep | (uint64_t)total, __ATOMIC_RELAXED,
+                    __hip_atomic_store((PP_GLOBAL uint64_t*)&status[t], kFlagAgg | ep | (uint64_t)total, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
                 int64_t excl = 0;
                 int64_t top = t - 1;
                 int64_t spins = 0;
                 while (true) {
                     const int64_t q = top - lane;
-                    const uint64_t sv = q >= 0 ? __hip_atomic_load(&status[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                    const uint64_t sv = q >= 0 ? __hip_atomic_load((PP_GLOBAL uint64_t*)&status[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                                                : (kFlagPre | ep);  // before tile 0: prefix 0
                     const bool fresh = (sv & (kEpochMask << kEpochShift)) == ep;
                     const uint64_t flag = fresh ? (sv & (3ull << 62)) : 0ull;
@@ -1189,7 +1191,7 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
                     top -= kWave;
                 }
                 if (lane == 0) {
-                    __hip_atomic_store(&status[t], kFlagPre | ep | (uint64_t)(excl + total), __ATOMIC_RELAXED,
+                    __hip_atomic_store((PP_GLOBAL uint64_t*)&status[t], kFlagPre | ep | (uint64_t)(excl + total), __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
                     s_base = excl;
                 }
@@ -1239,7 +1241,7 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
         if (base + total <= capOut) {
             for (int k = tid; k < total; k += kBlock) {
                 store_key<W>(Oz, Ox, base + k, s_key[k]);
-                Oc[base + k] = s_c[k];
+                stg(Oc, base + k, s_c[k]);
             }
         }
         __syncthreads();
