@@ -605,22 +605,25 @@ __global__ void __launch_bounds__(kBlock) kb_bounds(const SegDesc* __restrict__ 
 }
 
 // Run heads of one level in a tile as bit masks (word c: positions
-// 64c..64c+63; position 0 is always a head): the first head after p (Teff if
-// none) and the last head at or before p.
-__device__ __forceinline__ int next_head(const uint64_t* hm, int p, int Teff, int nwords) {
-    int c = p >> 6;
-    uint64_t m = (p & 63) == 63 ? 0ull : hm[c] & (~0ull << ((p & 63) + 1));
-    while (!m) {
-        if (++c >= nwords) return Teff;
-        m = hm[c];
-    }
-    return c * 64 + __ffsll((unsigned long long)m) - 1;
+// 64c..64c+63; position 0 is always a head) plus a mask of the words that
+// hold a head: the first head after p (Teff if none) and the last head at or
+// before p, each in O(1).
+__device__ __forceinline__ int next_head(const uint64_t* hm, uint64_t words, int p, int Teff) {
+    const int c = p >> 6;
+    const uint64_t m = (p & 63) == 63 ? 0ull : hm[c] & (~0ull << ((p & 63) + 1));
+    if (m) return c * 64 + __ffsll((unsigned long long)m) - 1;
+    const uint64_t wm = c == 63 ? 0ull : words & (~0ull << (c + 1));
+    if (!wm) return Teff;
+    const int c2 = __ffsll((unsigned long long)wm) - 1;
+    return c2 * 64 + __ffsll((unsigned long long)hm[c2]) - 1;
 }
-__device__ __forceinline__ int last_head(const uint64_t* hm, int p) {
-    int c = p >> 6;
-    uint64_t m = hm[c] & ((p & 63) == 63 ? ~0ull : ((2ull << (p & 63)) - 1));
-    while (!m) m = hm[--c];
-    return c * 64 + 63 - __clzll((long long)m);
+__device__ __forceinline__ int last_head(const uint64_t* hm, uint64_t words, int p) {
+    const int c = p >> 6;
+    const uint64_t m = hm[c] & ((p & 63) == 63 ? ~0ull : ((2ull << (p & 63)) - 1));
+    if (m) return c * 64 + 63 - __clzll((long long)m);
+    const uint64_t wm = words & ((1ull << c) - 1);  // c >= 1 here: word 0 holds position 0
+    const int c2 = 63 - __clzll((long long)wm);
+    return c2 * 64 + 63 - __clzll((long long)hm[c2]);
 }
 
 // ---------------------------------------------------------------------------
@@ -640,6 +643,7 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
     __shared__ int s_wred[4];
     __shared__ int64_t s_bq[kAggLevels][4];
     __shared__ uint64_t s_lhm[kAggLevels][kTile / 64];  // run heads per level (summary path)
+    __shared__ unsigned long long s_lwords[2][kAggLevels];  // words holding a head, by tile parity
     int b;
     int64_t lb, nb;
     map_block(bm, b, lb, nb);
@@ -697,8 +701,11 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
     // cross-edge segment counts from kb_flags' summaries (1..kAggLevels
     // levels) or from kb_bounds' table
     const bool summ = nlev >= 1 && nlev <= kAggLevels;
+    if (threadIdx.x < 2 * kAggLevels) s_lwords[threadIdx.x / kAggLevels][threadIdx.x % kAggLevels] = 0ull;
+    __syncthreads();
     int64_t mtot = m_scan;
-    for (int64_t t = lb; t < ntiles; t += nb) {
+    int par = 0;  // tile parity: which s_lwords row this tile uses
+    for (int64_t t = lb; t < ntiles; t += nb, par ^= 1) {
         const int64_t ts = t * kTile;
         const int Teff = (int)((n - ts) < kTile ? (n - ts) : kTile);
         Key<W> K[R];
@@ -752,7 +759,10 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
                     if (li < nlev) {
                         const int lvl = (int)((lev4 >> (8 * li)) & 255u);
                         const uint64_t hm = __ballot(j < Teff && (j == 0 || hdv[r] >= lvl));
-                        if (lane == 0) s_lhm[li][wave * R + r] = hm;
+                        if (lane == 0) {
+                            s_lhm[li][wave * R + r] = hm;
+                            if (hm) atomicOr(&s_lwords[par][li], 1ull << (wave * R + r));
+                        }
                     }
                 }
             }
@@ -805,16 +815,17 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
             for (int li = 0; li < nlev; ++li) {
                 const int lvl = (int)((lev4 >> (8 * li)) & 255u);
                 const uint64_t* hm = s_lhm[li];
+                const uint64_t words = s_lwords[par][li];
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     if (!fl[r]) continue;
                     const int j = wave * (kTile / 4) + r * kWave + lane;
                     int64_t cnt = 0;
                     if (key_bit<W>(K[r], lvl) == 0) {
-                        const int ee = next_head(hm, j, Teff, kTile / 64);
+                        const int ee = next_head(hm, words, j, Teff);
                         if (ee < Teff) {
                             if (s_hdb[ee] == lvl) {
-                                const int f = next_head(hm, ee, Teff, kTile / 64);
+                                const int f = next_head(hm, words, ee, Teff);
                                 cnt = (int64_t)s_aloc[f] - s_aloc[ee];
                                 if (f == Teff && s_hdb[Teff] < lvl) cnt += s_bq[li][2];
                             }
@@ -824,10 +835,10 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
                         }
                         rank[r] += cnt;
                     } else {
-                        const int sv = last_head(hm, j);
+                        const int sv = last_head(hm, words, j);
                         if (sv > 0) {
                             if (s_hdb[sv] == lvl) {
-                                const int s2 = last_head(hm, sv - 1);
+                                const int s2 = last_head(hm, words, sv - 1);
                                 cnt = (int64_t)s_aloc[sv] - s_aloc[s2];
                                 if (s2 == 0 && s_hdb[0] < lvl) cnt += s_bq[li][0];
                             }
@@ -950,6 +961,7 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
             store_key<W>(Cz, Cx, rank[r], ch);
             stg(Cc, rank[r], cn);
         }
+        if (threadIdx.x < kAggLevels) s_lwords[par ^ 1][threadIdx.x] = 0ull;  // for the next tile
         __syncthreads();
     }
 }

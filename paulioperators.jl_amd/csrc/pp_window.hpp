@@ -342,11 +342,19 @@ __global__ void __launch_bounds__(kBlock) kw_merge(const uint64_t* __restrict__ 
             else hi = mid;
         }
         int ia = lo, jb = dk - lo;
+        // outputs in registers indexed by the unrolled position (a run-time
+        // index would place the arrays in scratch memory)
         Key<W> outK[E];
         double outC[E];
-        int cnt = 0;
+        bool outE[E];
         const int stop = dk + E < nL + nC ? dk + E : nL + nC;
-        for (int pos = dk; pos < stop; ++pos) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            outE[e] = false;
+#pragma unroll
+            for (int w = 0; w < W; ++w) outK[e].z[w] = outK[e].x[w] = 0;
+            outC[e] = 0.0;
+            if (dk + e >= stop) continue;
             const bool takeL = ia < nL && (jb >= nC || key_le<W>(sL[ia], sC[jb]));
             Key<W> K;
             double acc;
@@ -386,12 +394,13 @@ __global__ void __launch_bounds__(kBlock) kw_merge(const uint64_t* __restrict__ 
                 acc = acc + (inl ? sCc[lj] : Cc[jg]);
                 ++jg;
             }
-            if (!should_drop<W>(f, K, fabs(acc))) {
-                outK[cnt] = K;
-                outC[cnt] = acc;
-                ++cnt;
-            }
+            outK[e] = K;
+            outC[e] = acc;
+            outE[e] = !should_drop<W>(f, K, fabs(acc));
         }
+        int cnt = 0;
+#pragma unroll
+        for (int e = 0; e < E; ++e) cnt += outE[e] ? 1 : 0;
         int incl = cnt;
 #pragma unroll
         for (int dd = 1; dd < kWave; dd <<= 1) {
@@ -453,9 +462,16 @@ __global__ void __launch_bounds__(kBlock) kw_merge(const uint64_t* __restrict__ 
         }
         __syncthreads();
         const int64_t base = s_base;
-        for (int k = 0; k < cnt; ++k) {
-            s_key[myoff + k] = outK[k];
-            s_c[myoff + k] = outC[k];
+        {
+            int k = myoff;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                if (outE[e]) {
+                    s_key[k] = outK[e];
+                    s_c[k] = outC[e];
+                    ++k;
+                }
+            }
         }
         __syncthreads();
         if (base + total <= capOut) {
