@@ -545,6 +545,14 @@ struct Batch {
     // kb_scan runs when kb_bounds needs the tile offsets or a vector may
     // hold more than kEmitSelfTiles sweep tiles; otherwise kb_emit sums the
     // tile counts itself (one kernel less per rotation).
+    // kb_bounds also runs when a vector may hold more than kSummMaxTiles
+    // sweep tiles in this call (those vectors then use its table)
+    bool big_call() const {
+        for (int b = 0; b < B; ++b)
+            if (cdiv(n_known[b] + n_known[b] / 2, (int64_t)tile) > kSummMaxTiles) return true;
+        return false;
+    }
+
     bool scan_needed(bool bounds) const {
         if (bounds) return true;
         for (const InstBuf& I : ib)
@@ -554,13 +562,18 @@ struct Batch {
 
     template <int Wt, int T, int TM>
     void launch_step(const StepDev* dsteps, int64_t step_index, uint32_t ep_off, bool bounds, bool scan) {
-        hipLaunchKernelGGL((kb_flags<Wt, T>), dim3(blocks(0)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(0));
+        hipLaunchKernelGGL((kb_flags<Wt, T>), dim3(blocks(0)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(0),
+                           bounds ? 1 : 0);
         if (scan) hipLaunchKernelGGL(kb_scan<T>, dim3(B), dim3(1024), 0, stream, d_desc, d_state, dsteps, step_index);
         if (bounds)
             hipLaunchKernelGGL((kb_bounds<Wt, T>), dim3(blocks(1)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps,
                                map(1));
-        hipLaunchKernelGGL((kb_emit<Wt, T>), dim3(blocks(0)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(0),
-                           scan ? 0 : 1, step_index);
+        if (bounds)
+            hipLaunchKernelGGL((kb_emit<Wt, T, true>), dim3(blocks(0)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps,
+                               map(0), 0, step_index);
+        else
+            hipLaunchKernelGGL((kb_emit<Wt, T, false>), dim3(blocks(0)), dim3(kBlock), 0, stream, d_desc, d_state,
+                               dsteps, map(0), scan ? 0 : 1, step_index);
         if (timing) {
             // events recorded by the launch itself at the kernel's start and
             // end (not around it), so the interval is the kernel's duration
@@ -596,8 +609,9 @@ struct Batch {
             with_tiles<Wt>([&](auto T, auto TM) {
                 uint32_t off = 0;
                 if (with_precheck) launch_compact<Wt, decltype(TM)::value>(off++);
+                const bool big = big_call();
                 for (int64_t s = first; s < nsteps; ++s) {
-                    const bool bnd = step_bounds[(size_t)s] != 0;
+                    const bool bnd = step_bounds[(size_t)s] != 0 || big;
                     launch_step<Wt, decltype(T)::value, decltype(TM)::value>(dsteps + (s - first) * B, s, off++, bnd,
                                                                              scan_needed(bnd));
                 }
@@ -611,7 +625,7 @@ struct Batch {
         }
         uint64_t bh = 1469598103934665603ull;
         for (int64_t s = first; s < nsteps; ++s) bh = (bh ^ step_bounds[(size_t)s]) * 1099511628211ull;
-        bh = (bh ^ (scan_needed(false) ? 2u : 0u)) * 1099511628211ull;
+        bh = (bh ^ (scan_needed(false) ? 2u : 0u) ^ (big_call() ? 4u : 0u)) * 1099511628211ull;
         GraphKey key{nsteps, with_precheck ? -1 - first : first, tile, mtile, Wt, blocks(0), blocks(1), blocks(3),
                      (const void*)dsteps, bh};
         for (auto& g : gcache)

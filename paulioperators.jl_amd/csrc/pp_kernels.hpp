@@ -106,7 +106,11 @@ __global__ void kb_block_table(const int64_t* __restrict__ pre, int B, uint64_t*
 //   bit  50    eq_first hdb at the first head == b
 //   bit  51    eq_last  hdb at the last head == b
 constexpr int kAggLevels = 4;
-constexpr int kEmitSelfTiles = 1024;  // kb_emit sums the tile counts itself up to this many tiles
+constexpr int kEmitSelfTiles = 512;  // kb_emit sums the tile counts itself up to this many tiles
+// Segment summaries serve calls whose vectors hold at most this many sweep
+// tiles: an edge scan over the summaries grows with the segment length in
+// tiles, kb_bounds' key searches only with its logarithm.
+constexpr int64_t kSummMaxTiles = 256;
 
 __device__ __forceinline__ int agg_heads(uint64_t a) { return (int)((a >> 48) & 3u); }
 __device__ __forceinline__ int64_t agg_first(uint64_t a) { return (int64_t)(a & 4095u); }
@@ -306,7 +310,7 @@ __device__ void edge_counts(int side, int li, int lvl, int64_t t, int64_t ntiles
 // ---------------------------------------------------------------------------
 template <int W, int T>
 __global__ void __launch_bounds__(kBlock) kb_flags(const SegDesc* __restrict__ D, SegState* __restrict__ S,
-                                                   const StepDev* __restrict__ steps, BlockMap bm) {
+                                                   const StepDev* __restrict__ steps, BlockMap bm, int bnd_ok) {
     constexpr int kTile = T;
     constexpr int kTileWords = T / 64;
     __shared__ int s_wcnt[kTileWords];
@@ -362,7 +366,7 @@ __global__ void __launch_bounds__(kBlock) kb_flags(const SegDesc* __restrict__ D
     FilterReg floc;  // the local filter, when it can drop children
     if (local_active) floc.load(d.fl);
     // segment summaries for kb_emit when the mask has 1..kAggLevels levels
-    const bool summ = nlev >= 1 && nlev <= kAggLevels;
+    const bool summ = nlev >= 1 && nlev <= kAggLevels && !bnd_ok;  // kb_emit's summary variant runs
     bool need_z = false, need_x = false;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
@@ -630,7 +634,7 @@ __device__ __forceinline__ int last_head(const uint64_t* hm, uint64_t words, int
 // kb_emit: children written at their rank in key order.
 // Layout inside a tile: wave w owns elements [w T/4, (w+1) T/4), round r
 // covers 64 consecutive elements (coalesced loads), lane = element % 64.
-template <int W, int T>
+template <int W, int T, bool TABLE>
 __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D, SegState* __restrict__ S,
                                                   const StepDev* __restrict__ steps, BlockMap bm, int self_off,
                                                   int64_t step_index) {
@@ -681,11 +685,12 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
     // ticket).  The counts (at most kEmitSelfTiles) are loaded now and summed
     // after the first tile's own loads are issued: both share one memory
     // round trip.  Every wave sums for itself.
-    int32_t cpre[kEmitSelfTiles / kWave];
+    constexpr int kCW = TABLE ? 1 : kEmitSelfTiles / kWave;  // the table variant runs after kb_scan
+    int32_t cpre[kCW];
 #pragma unroll
-    for (int k = 0; k < kEmitSelfTiles / kWave; ++k) {
+    for (int k = 0; k < kCW; ++k) {
         const int64_t u = lane0 + (int64_t)k * kWave;
-        cpre[k] = (self_off && u < ntiles) ? ldg(tilecnt, u) : 0;
+        cpre[k] = (!TABLE && self_off && u < ntiles) ? ldg(tilecnt, u) : 0;
     }
     if (self_off && ntiles == 0) {
         if (lb == 0 && threadIdx.x == 0) {
@@ -700,7 +705,14 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     // cross-edge segment counts from kb_flags' summaries (1..kAggLevels
     // levels) or from kb_bounds' table
-    const bool summ = nlev >= 1 && nlev <= kAggLevels;
+    // TABLE: the cross-edge counts come from kb_bounds' table (any number of
+    // levels); otherwise from kb_flags' summaries (at most kAggLevels levels).
+    // Two instantiations keep each variant's register use low.
+    constexpr bool summ = !TABLE;
+    if (!TABLE && nlev > kAggLevels) {  // the host never launches this: report, do not write
+        if (lb == 0 && threadIdx.x == 0) atomicOr(&s->err, 8);
+        return;
+    }
     if (threadIdx.x < 2 * kAggLevels) s_lwords[threadIdx.x / kAggLevels][threadIdx.x % kAggLevels] = 0ull;
     __syncthreads();
     int64_t mtot = m_scan;
@@ -712,19 +724,7 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
         double C[R];
         int64_t rank[R];
         bool fl[R];
-        int hdv[R];
         const int64_t toff_scan = self_off ? 0 : ldg(tileoff, t);
-        // boundary counts of the first four levels, loaded with the tile
-        // (index clamped into the table when there are fewer levels)
-        int64_t bq4[4][4];
-        if (!summ) {
-#pragma unroll
-            for (int li = 0; li < 4; ++li) {
-                const int lc = li < nlev ? li : 0;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) bq4[li][q] = ldg(bnd, (t * maxlev + lc) * 4 + q);
-            }
-        }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int j = wave * (kTile / 4) + r * kWave + lane;
@@ -739,12 +739,10 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
                 const uint64_t lowmask = lane ? ((~0ull) >> (64 - lane)) : 0ull;
                 const int aloc = (int)ldg(wpre, i >> 6) + popc64(word & lowmask);
                 s_aloc[j] = (int16_t)aloc;
-                hdv[r] = i == 0 ? (int)kHdbNone : hdb<W>(load_key<W>(Lz, Lx, i - 1), K[r]);
-                s_hdb[j] = (int16_t)hdv[r];
+                s_hdb[j] = (int16_t)(i == 0 ? kHdbNone : hdb<W>(load_key<W>(Lz, Lx, i - 1), K[r]));
                 rank[r] = aloc;
                 if (fl[r]) C[r] = ldg(Lc, i);
             } else {
-                hdv[r] = -1;
 #pragma unroll
                 for (int w = 0; w < W; ++w) K[r].z[w] = K[r].x[w] = 0;
             }
@@ -758,7 +756,7 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
                 for (int li = 0; li < kAggLevels; ++li) {
                     if (li < nlev) {
                         const int lvl = (int)((lev4 >> (8 * li)) & 255u);
-                        const uint64_t hm = __ballot(j < Teff && (j == 0 || hdv[r] >= lvl));
+                        const uint64_t hm = __ballot(j < Teff && (j == 0 || s_hdb[j] >= lvl));
                         if (lane == 0) {
                             s_lhm[li][wave * R + r] = hm;
                             if (hm) atomicOr(&s_lwords[par][li], 1ull << (wave * R + r));
@@ -768,10 +766,10 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
             }
         }
         int64_t toff = toff_scan;
-        if (self_off) {
+        if (!TABLE && self_off) {
             int64_t all = 0, before = 0;
 #pragma unroll
-            for (int k = 0; k < kEmitSelfTiles / kWave; ++k) {
+            for (int k = 0; k < kCW; ++k) {
                 all += cpre[k];
                 before += lane + (int64_t)k * kWave < t ? cpre[k] : 0;
             }
@@ -855,12 +853,7 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
             const int lvl = li < 4 ? (int)((lev4 >> (8 * li)) & 255u) : (int)st->g.lev[li];
             int64_t bq[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-                bq[q] = li == 0   ? bq4[0][q]
-                        : li == 1 ? bq4[1][q]
-                        : li == 2 ? bq4[2][q]
-                        : li == 3 ? bq4[3][q]
-                                  : ldg(bnd, (t * maxlev + li) * 4 + q);
+            for (int q = 0; q < 4; ++q) bq[q] = ldg(bnd, (t * maxlev + li) * 4 + q);
             // forward: st_j = last head <= j (j = 0 is always a head)
             int stv[R];
             int carry = -1;
