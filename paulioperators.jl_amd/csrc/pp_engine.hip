@@ -196,7 +196,10 @@ struct Batch {
     SegState* d_state = nullptr;
     int64_t* d_maps = nullptr;     // 4 x (B + 1) block prefixes: tile, bounds, partition, merge
     uint64_t* d_blk = nullptr;     // per-block (vector, local block, blocks) tables of the 4 maps
+    std::vector<uint64_t> h_blk;
     int64_t blk_off[4] = {0, 0, 0, 0};
+    int64_t blk_cap[4] = {0, 0, 0, 0};
+    int64_t h_grid[4] = {1, 1, 1, 1};
     std::vector<int64_t> h_maps;
     FilterDev* d_filt = nullptr;   // [2B]: strict, local
     std::vector<int32_t> local_active;
@@ -339,13 +342,17 @@ struct Batch {
                 return r;
             };
             const int64_t per[4] = {p2((int64_t)ncu * 8), p2((int64_t)ncu * 4), 1, p2((int64_t)ncu * 4)};
-            // graph mode rounds each total up to a power of two
+            // a map's table: twice the largest contiguous grid (per-vector
+            // maximum times B, a power of two); plan() falls back to the
+            // contiguous layout when the XCD rows would not fit
             int64_t off = 0;
             for (int k = 0; k < 4; ++k) {
                 blk_off[k] = off;
-                off += p2(per[k] * B);
+                blk_cap[k] = 2 * p2(per[k] * B);
+                off += blk_cap[k];
             }
             d_blk = dalloc<uint64_t>((size_t)off);
+            h_blk.assign((size_t)off, 0);
         }
         d_filt = dalloc<FilterDev>(2 * (size_t)B);
         d_chan = dalloc<ChanDev>(B);
@@ -468,26 +475,62 @@ struct Batch {
             acc[2] += 1;
             acc[3] += std::max<int64_t>(1, std::min<int64_t>(cdiv(nest + nest / 2, T2), (int64_t)ncu * 4));
         }
-        if (graphs) {
-            // with captured graphs each grid total is rounded up to a power of
-            // two (the surplus blocks go to the last vector, find no tile and
-            // exit), so that a graph, whose grids are fixed, is replayed by
-            // most calls; the per-vector split is data read at run time
-            for (int k = 0; k < 4; ++k) {
-                if (k == 2) continue;
-                int64_t r = 1;
-                while (r < acc[k]) r <<= 1;
-                const int64_t last = h_maps[(size_t)k * (B + 1) + B - 1];
-                if (r - last < (int64_t(1) << kBlkLocBits)) h_maps[(size_t)k * (B + 1) + B] = r;
+        // XCD placement: vectors by decreasing size, each to the XCD row with
+        // the least tile blocks so far; a map's grid is kXcds times its
+        // longest row (blocks beyond a row's end have no work)
+        std::vector<int> order((size_t)B), row((size_t)B);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return n_known[a] > n_known[c]; });
+        {
+            int64_t load[kXcds] = {0};
+            for (int b : order) {
+                int x = 0;
+                for (int y = 1; y < kXcds; ++y)
+                    if (load[y] < load[x]) x = y;
+                row[(size_t)b] = x;
+                load[x] += h_maps[(size_t)0 * (B + 1) + b + 1] - h_maps[(size_t)0 * (B + 1) + b];
             }
         }
-        HIPCHK(hipMemcpyAsync(d_maps, h_maps.data(), h_maps.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream));
         for (int k = 0; k < 4; ++k) {
-            const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv((int64_t)blocks(k), 256), 1024));
-            hipLaunchKernelGGL(kb_block_table, dim3(g), dim3(256), 0, stream, (const int64_t*)(d_maps + (size_t)k * (B + 1)),
-                               B, d_blk + blk_off[k]);
+            const int64_t* pre = h_maps.data() + (size_t)k * (B + 1);
+            int64_t start[kXcds] = {0};
+            std::vector<int64_t> st((size_t)B);
+            for (int b : order) {
+                st[(size_t)b] = start[row[(size_t)b]];
+                start[row[(size_t)b]] += pre[b + 1] - pre[b];
+            }
+            int64_t len = 0;
+            for (int x = 0; x < kXcds; ++x) len = std::max(len, start[x]);
+            auto round = [&](int64_t g) {
+                // with captured graphs each grid is rounded up to a power of
+                // two, so that a graph, whose grids are fixed, is replayed by
+                // most calls; the block table itself is data read at run time
+                g = std::max<int64_t>(1, g);
+                if (!graphs) return g;
+                int64_t r = 1;
+                while (r < g) r <<= 1;
+                return r;
+            };
+            int64_t grid = round((int64_t)kXcds * len);
+            // rows pay off where kernels are latency-bound (L2 hits between
+            // the kernels of a step); a call with more tiles than a few per CU
+            // is bandwidth-bound and wants every XCD on every vector
+            const bool rows = grid <= blk_cap[k] && acc[0] <= 16 * (int64_t)ncu;
+            if (!rows) grid = round(pre[B]);  // contiguous layout: pre[B] <= per-vector maximum x B
+            h_grid[k] = grid;
+            uint64_t* tab = h_blk.data() + blk_off[k];
+            std::fill(tab, tab + grid, (uint64_t)kBlkNone);
+            for (int b = 0; b < B; ++b) {
+                const int64_t nbk = pre[b + 1] - pre[b];
+                for (int64_t lb = 0; lb < nbk; ++lb) {
+                    const int64_t g = rows ? kXcds * (st[(size_t)b] + lb) + row[(size_t)b] : pre[b] + lb;
+                    tab[(size_t)g] =
+                        (uint64_t)b | ((uint64_t)lb << kBlkVecBits) | ((uint64_t)nbk << (kBlkVecBits + kBlkLocBits));
+                }
+            }
+            HIPCHK(hipMemcpyAsync(d_blk + blk_off[k], tab, (size_t)grid * sizeof(uint64_t), hipMemcpyHostToDevice, stream));
         }
-        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(d_maps, h_maps.data(), h_maps.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream));
         if (cap_partial < acc[0]) {
             dfree(d_partial);
             cap_partial = acc[0];
@@ -496,7 +539,7 @@ struct Batch {
     }
 
     BlockMap map(int k) const { return BlockMap{d_maps + (size_t)k * (B + 1), d_blk + blk_off[k], B}; }
-    unsigned blocks(int k) const { return (unsigned)h_maps[(size_t)k * (B + 1) + B]; }
+    unsigned blocks(int k) const { return (unsigned)h_grid[k]; }
 
     std::vector<pp_filter> last_filters;  // what d_filt holds (strict, local per vector)
 

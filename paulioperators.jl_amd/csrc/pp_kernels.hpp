@@ -60,9 +60,12 @@ __device__ __forceinline__ int find_seg(const int64_t* __restrict__ prefix, int 
     return lo;
 }
 
-// Grid mapping shared by every kernel: blocks [pre[b], pre[b+1]) serve vector
-// b.  blk holds, per block, b | (block - pre[b]) << 20 | (pre[b+1] - pre[b]) << 42,
-// so a workgroup finds its vector with one load instead of a search.
+// Grid mapping shared by every kernel.  blk holds, per launched block,
+// b | lb << 20 | nb << 42: block lb of the nb blocks that serve vector b, or
+// b = 2^20 - 1 for a block with no work.  A vector's blocks are laid out at
+// indices of one residue modulo kXcds, so that (with workgroups dealt to the
+// XCDs in turn) every kernel runs a vector on one XCD and its L2.  pre is the
+// contiguous prefix of the per-vector block counts (for per-block partials).
 struct BlockMap {
     const int64_t* pre;
     const uint64_t* blk;
@@ -71,22 +74,16 @@ struct BlockMap {
 
 constexpr int kBlkVecBits = 20;   // vectors per batch < 2^20
 constexpr int kBlkLocBits = 22;   // blocks per vector < 2^22
+constexpr uint64_t kBlkNone = (1ull << kBlkVecBits) - 1;
+constexpr int kXcds = 8;          // XCDs of an MI355X
 
-__device__ __forceinline__ void map_block(const BlockMap& bm, int& b, int64_t& lb, int64_t& nb) {
+// False for a block with no work (the caller returns).
+__device__ __forceinline__ bool map_block(const BlockMap& bm, int& b, int64_t& lb, int64_t& nb) {
     const uint64_t e = bm.blk[blockIdx.x];
     b = (int)(e & ((1ull << kBlkVecBits) - 1));
     lb = (int64_t)((e >> kBlkVecBits) & ((1ull << kBlkLocBits) - 1));
     nb = (int64_t)(e >> (kBlkVecBits + kBlkLocBits));
-}
-
-// Fills the per-block table of a map from its prefix (once per call).
-__global__ void kb_block_table(const int64_t* __restrict__ pre, int B, uint64_t* __restrict__ blk) {
-    const int64_t total = pre[B];
-    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (int64_t)gridDim.x * blockDim.x) {
-        const int b = find_seg(pre, B, g);
-        blk[g] = (uint64_t)b | ((uint64_t)(g - pre[b]) << kBlkVecBits) |
-                 ((uint64_t)(pre[b + 1] - pre[b]) << (kBlkVecBits + kBlkLocBits));
-    }
+    return b < bm.B;
 }
 
 // ---------------------------------------------------------------------------
@@ -319,7 +316,7 @@ __global__ void __launch_bounds__(kBlock) kb_flags(const SegDesc* __restrict__ D
     __shared__ uint64_t s_em[kAggLevels][kTileWords];       // heads whose hdb equals the level
     int b;
     int64_t lb, nb;
-    map_block(bm, b, lb, nb);
+    if (!map_block(bm, b, lb, nb)) return;
     const StepDev* st = steps + b;
     SegState* s = S + b;
     const SegDesc& d = D[b];
@@ -532,7 +529,7 @@ __global__ void __launch_bounds__(kBlock) kb_bounds(const SegDesc* __restrict__ 
     constexpr int kTile = T;
     int b;
     int64_t lb, nb;
-    map_block(bm, b, lb, nb);
+    if (!map_block(bm, b, lb, nb)) return;
     const StepDev* st = steps + b;
     const SegState* s = S + b;
     const SegDesc& d = D[b];
@@ -650,7 +647,7 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
     __shared__ unsigned long long s_lwords[2][kAggLevels];  // words holding a head, by tile parity
     int b;
     int64_t lb, nb;
-    map_block(bm, b, lb, nb);
+    if (!map_block(bm, b, lb, nb)) return;
     const StepDev* st = steps + b;
     SegState* s = S + b;
     const SegDesc& d = D[b];
@@ -981,7 +978,7 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
     __shared__ int s_wsum[4];
     int b;
     int64_t lb, nb;
-    map_block(bm, b, lb, nb);
+    if (!map_block(bm, b, lb, nb)) return;
     const StepDev* st = steps + b;
     SegState* s = S + b;
     const SegDesc& d = D[b];
@@ -1303,7 +1300,7 @@ __global__ void __launch_bounds__(kBlock) kb_channel(const SegDesc* __restrict__
                                                      BlockMap bm) {
     int b;
     int64_t lb, nb;
-    map_block(bm, b, lb, nb);
+    if (!map_block(bm, b, lb, nb)) return;
     if (!en[b]) return;
     const SegState* s = S + b;
     if (s->stalled) return;
@@ -1332,7 +1329,7 @@ __global__ void __launch_bounds__(kBlock) kb_expect(const SegDesc* __restrict__ 
     __shared__ double s_h[kBlock], s_l[kBlock];
     int b;
     int64_t lb, nb;
-    map_block(bm, b, lb, nb);
+    if (!map_block(bm, b, lb, nb)) return;
     const SegState* s = S + b;
     const int cur = s->cur ^ s->flip;
     const int64_t n = s->n[cur];
@@ -1370,7 +1367,7 @@ __global__ void __launch_bounds__(kBlock) kb_expect(const SegDesc* __restrict__ 
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) partial[blockIdx.x] = make_double2(s_h[0], s_l[0]);
+    if (threadIdx.x == 0) partial[bm.pre[b] + lb] = make_double2(s_h[0], s_l[0]);
 }
 
 __global__ void __launch_bounds__(kBlock) kb_expect_final(const double2* __restrict__ partial, BlockMap bm,
@@ -1402,7 +1399,7 @@ __global__ void __launch_bounds__(kBlock) kb_check(const SegDesc* __restrict__ D
                                                    BlockMap bm, int32_t* __restrict__ bad) {
     int b;
     int64_t lb, nb;
-    map_block(bm, b, lb, nb);
+    if (!map_block(bm, b, lb, nb)) return;
     const SegState* s = S + b;
     const int cur = s->cur ^ s->flip;
     const int64_t n = s->n[cur];
