@@ -387,6 +387,72 @@ def test_segment_summaries_mixed_levels(N, tile, mtile):
         assert np.array_equal(gc.view(np.uint64), oc.view(np.uint64)), f"vector {b} coefficients"
 
 
+def _batch_vs_oracle(N, init, gens, angs, strat, reserve=None, tiles=None):
+    from pauliprop import SparsePauliBatch
+    B = len(init)
+    bt = SparsePauliBatch([N] * B, [[((z, x), c) for z, x, c in t] for t in init])
+    if reserve:
+        bt.reserve(reserve)
+    if tiles:
+        bt.set_tiles(*tiles)
+    bt.evolve_(gens, angs, truncation=[strat] * B)
+    for b in range(B):
+        o = po.OracleSPV(N, init[b])
+        o.evolve([(p.z, p.x) for p in gens[b]], angs[b], strict=compile_filter(strat))
+        gz, gx, gc = bt.arrays(b)
+        oz, ox, oc = o.arrays()
+        assert np.array_equal(gz, oz) and np.array_equal(gx, ox), f"vector {b} keys"
+        assert np.array_equal(gc.view(np.uint64), oc.view(np.uint64)), f"vector {b} coefficients"
+
+
+def _two_qubit_circuits(N, sups, steps, rng):
+    gens, angs = [], []
+    for sup in sups:
+        g, a = [], []
+        for _ in range(steps):
+            gz, gx = random_two_qubit_pauli(N, sup, rng)
+            g.append(PauliBasis(N, gz, gx))
+            a.append(rng.uniform(-3, 3))
+        gens.append(g)
+        angs.append(a)
+    return gens, angs
+
+
+def test_large_vector_switches_every_vector_to_the_bounds_table():
+    """A call whose largest vector exceeds 256 sweep tiles ranks the children
+    of every vector (small ones included) from kb_bounds' key searches, with
+    kb_scan's tile offsets: the same terms as the oracle."""
+    rng = random.Random(404)
+    N = 40
+    sups = [spread_support(N, 11, rng) for _ in range(3)]
+    init = [random_terms(N, n, rng, sup) for n, sup in zip([160000, 3000, 700], sups)]
+    gens, angs = _two_qubit_circuits(N, sups, 8, rng)
+    log(f"large call: sizes {[len(t) for t in init]}")
+    _batch_vs_oracle(N, init, gens, angs, CoeffTruncation(1e-9))
+
+
+def test_large_capacity_uses_the_scan_kernel():
+    """Buffers sized beyond 512 sweep tiles make every step run kb_scan for
+    the child offsets instead of kb_emit's own sums: same terms."""
+    rng = random.Random(405)
+    N = 70
+    sups = [spread_support(N, 9, rng) for _ in range(2)]
+    init = [random_terms(N, n, rng, sup) for n, sup in zip([20000, 5000], sups)]
+    gens, angs = _two_qubit_circuits(N, sups, 12, rng)
+    _batch_vs_oracle(N, init, gens, angs, CoeffTruncation(1e-9), reserve=600000)
+
+
+def test_many_tiles_contiguous_block_layout():
+    """More tiles than the XCD-row layout takes (over 16 per CU): blocks are
+    laid out contiguously; two vectors of ~1.2e6 terms against the oracle."""
+    rng = random.Random(406)
+    N = 40
+    sups = [spread_support(N, 12, rng) for _ in range(2)]
+    init = [random_terms(N, n, rng, sup) for n, sup in zip([1200000, 1100000], sups)]
+    gens, angs = _two_qubit_circuits(N, sups, 4, rng)
+    _batch_vs_oracle(N, init, gens, angs, CoeffTruncation(1e-7), tiles=(512, 512))
+
+
 def test_batch_graph_replays_match_eager():
     """hipGraph-captured sequences: the same layer circuit replayed over
     several calls, with eager channel work and host synchronisation between
