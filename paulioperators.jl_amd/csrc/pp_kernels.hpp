@@ -1060,13 +1060,31 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
             j1 = j0 = 0;
         }
         const int nL = (int)(i1 - i0), nC = (int)(j1 - j0);
-        for (int k = tid; k < nL + nC; k += kBlock) {
-            if (k < nL) {
-                s_key[k] = load_key<W>(Lz, Lx, i0 + k);
-                s_c[k] = ldg(Lc, i0 + k);
-            } else {
-                s_key[k] = load_key<W>(Cz, Cx, j0 + (k - nL));
-                s_c[k] = ldg(Cc, j0 + (k - nL));
+        {
+            // every load of the tile first (one memory round trip), then LDS
+            Key<W> lk[E];
+            double lcv[E];
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int k = tid + e * kBlock;
+#pragma unroll
+                for (int w = 0; w < W; ++w) lk[e].z[w] = lk[e].x[w] = 0;
+                lcv[e] = 0.0;
+                if (k < nL) {
+                    lk[e] = load_key<W>(Lz, Lx, i0 + k);
+                    lcv[e] = ldg(Lc, i0 + k);
+                } else if (k < nL + nC) {
+                    lk[e] = load_key<W>(Cz, Cx, j0 + (k - nL));
+                    lcv[e] = ldg(Cc, j0 + (k - nL));
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int k = tid + e * kBlock;
+                if (k < nL + nC) {
+                    s_key[k] = lk[e];
+                    s_c[k] = lcv[e];
+                }
             }
         }
         // the neighbours across the tile's edges, loaded with the tile
