@@ -132,8 +132,9 @@ def main():
         print(json.dumps({"copy_bytes": 1 << 30, "launches": 3}))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU; runs on more than one GPU are not part of it "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py: measures one GPU only; --gpus must be 1 (the sweep's 16 theta_h vectors "
+              "would shard across GPUs as independent vectors, with no collective on the step path)",
+              file=sys.stderr)
         sys.exit(2)
 
     from pauliprop import PPCounters
