@@ -251,53 +251,113 @@ __device__ AggHit agg_fwd(const uint64_t* __restrict__ agg, const int32_t* __res
     return AggHit{ntiles, sum, 0};
 }
 
+// The summaries of 64 consecutive tiles at one level and their A-counts:
+// lane l holds tile base + dir * l (when that tile exists).  An edge scan
+// loads it together with the keys across the edge, so that its first step
+// shares their memory round trip.
+struct AggWin {
+    int64_t base;
+    int dir;
+    uint64_t a;
+    int64_t c;
+    bool in;
+};
+
+__device__ __forceinline__ AggWin agg_window(const uint64_t* __restrict__ agg, const int32_t* __restrict__ tilecnt,
+                                             int64_t base, int dir, int64_t ntiles, int li) {
+    const int lane = threadIdx.x % kWave;
+    AggWin w{base, dir, 0ull, 0, false};
+    const int64_t u = base + (int64_t)dir * lane;
+    if (u >= 0 && u < ntiles) {
+        w.a = ldg(agg, u * kAggLevels + li);
+        w.c = ldg(tilecnt, u);
+        w.in = true;
+    }
+    return w;
+}
+
+// agg_back (dir -1) or agg_fwd (dir +1) from tile u0, through the window
+// where it covers the start of the scan, then through memory.
+__device__ AggHit agg_scan(const AggWin& w, const uint64_t* __restrict__ agg, const int32_t* __restrict__ tilecnt,
+                           int64_t u0, int dir, int64_t ntiles, int li) {
+    const int lane = threadIdx.x % kWave;
+    const int64_t k0 = (u0 - w.base) * dir;  // window lane of tile u0
+    if (w.dir != dir || k0 < 0 || k0 >= kWave)
+        return dir < 0 ? agg_back(agg, tilecnt, u0, li) : agg_fwd(agg, tilecnt, u0, ntiles, li);
+    const bool cand = lane >= k0 && w.in;
+    const uint64_t hm = __ballot(cand && agg_heads(w.a) != 0);
+    if (hm) {
+        const int f = __ffsll((unsigned long long)hm) - 1;
+        const int64_t sum = wave_sum64(cand && lane < f ? w.c : 0);
+        return AggHit{w.base + (int64_t)dir * f, sum, __shfl(w.a, f, kWave)};
+    }
+    const int64_t sum = wave_sum64(cand ? w.c : 0);
+    const int64_t next = w.base + (int64_t)dir * kWave;  // first tile past the window
+    AggHit r = dir < 0 ? (next >= 0 ? agg_back(agg, tilecnt, next, li) : AggHit{-1, 0, 0})
+                       : (next < ntiles ? agg_fwd(agg, tilecnt, next, ntiles, li) : AggHit{ntiles, 0, 0});
+    r.sum += sum;
+    return r;
+}
+
 // The four cross-edge counts kb_bounds computes (same layout: pre_same,
 // pre_sib, post_same, post_sib), for one side of tile t at level lvl, from
 // the summaries.  Wave-uniform; `out` receives two values.
-template <int W>
+// h_edge receives hdb(K[te - 1], K[te]) on side 1 (kHdbNone when te = n).
+// WIN: load a window of summaries with the edge keys (costs registers; used
+// for small tiles, whose kernels are latency-bound).
+template <int W, bool WIN>
 __device__ void edge_counts(int side, int li, int lvl, int64_t t, int64_t ntiles, int64_t n, int64_t ts, int64_t te,
                             const uint64_t* __restrict__ Lz, const uint64_t* __restrict__ Lx,
                             const uint64_t* __restrict__ agg, const int32_t* __restrict__ tilecnt, int64_t& same,
-                            int64_t& sib) {
+                            int64_t& sib, int& h_edge) {
     same = 0;
     sib = 0;
+    h_edge = kHdbNone;
     if (side == 0) {
         if (ts == 0) return;
         const Key<W> ka = load_key<W>(Lz, Lx, ts - 1);
-        const int h = hdb<W>(ka, load_key<W>(Lz, Lx, ts));
+        const Key<W> kb = load_key<W>(Lz, Lx, ts);
+        const AggWin win = WIN ? agg_window(agg, tilecnt, t - 1, -1, ntiles, li)  // tiles t-1, t-2, ...
+                               : AggWin{0, 0, 0ull, 0, false};
+        const int h = hdb<W>(ka, kb);
         if (h > lvl) return;
-        const AggHit q = agg_back(agg, tilecnt, t - 1, li);        // q: last head <= ts - 1
+        const AggHit q = agg_scan(win, agg, tilecnt, t - 1, -1, ntiles, li);  // q: last head <= ts - 1
         same = agg_last(q.a) + q.sum;
         if (h < lvl && key_bit<W>(ka, lvl) == 1 && agg_eq_last(q.a)) {
             if (agg_heads(q.a) >= 2) {
                 sib = agg_l2(q.a);
             } else {
-                const AggHit q2 = agg_back(agg, tilecnt, q.u - 1, li);
+                const AggHit q2 = agg_scan(win, agg, tilecnt, q.u - 1, -1, ntiles, li);
                 sib = agg_first(q.a) + q2.sum + agg_last(q2.a);
             }
         }
     } else {
         if (te >= n) return;
         const Key<W> ka = load_key<W>(Lz, Lx, te);
-        const int h = hdb<W>(load_key<W>(Lz, Lx, te - 1), ka);
+        const Key<W> kp = load_key<W>(Lz, Lx, te - 1);
+        const AggWin win = WIN ? agg_window(agg, tilecnt, t + 1, 1, ntiles, li)  // tiles t+1, t+2, ...
+                               : AggWin{0, 0, 0ull, 0, false};
+        const int h = hdb<W>(kp, ka);
+        h_edge = h;
         if (h > lvl) return;
         if (h == lvl) {  // te is a head: the run [te, next head)
-            const uint64_t a = ldg(agg, (t + 1) * kAggLevels + li);
+            const uint64_t a = WIN ? __shfl(win.a, 0, kWave) : ldg(agg, (t + 1) * kAggLevels + li);  // tile t + 1
             if (agg_heads(a) >= 2) {
                 same = agg_12(a);
             } else {
-                const AggHit r = agg_fwd(agg, tilecnt, t + 2, ntiles, li);
-                same = (int64_t)ldg(tilecnt, t + 1) + r.sum + (r.u < ntiles ? agg_first(r.a) : 0);
+                const AggHit r = agg_scan(win, agg, tilecnt, t + 2, 1, ntiles, li);
+                same = (WIN ? __shfl(win.c, 0, kWave) : (int64_t)ldg(tilecnt, t + 1)) + r.sum +
+                       (r.u < ntiles ? agg_first(r.a) : 0);
             }
             return;
         }
-        const AggHit r = agg_fwd(agg, tilecnt, t + 1, ntiles, li);  // r: first head > te
+        const AggHit r = agg_scan(win, agg, tilecnt, t + 1, 1, ntiles, li);  // r: first head > te
         same = r.sum + (r.u < ntiles ? agg_first(r.a) : 0);
         if (r.u < ntiles && key_bit<W>(ka, lvl) == 0 && agg_eq_first(r.a)) {
             if (agg_heads(r.a) >= 2) {
                 sib = agg_12(r.a);
             } else {
-                const AggHit r2 = agg_fwd(agg, tilecnt, r.u + 1, ntiles, li);
+                const AggHit r2 = agg_scan(win, agg, tilecnt, r.u + 1, 1, ntiles, li);
                 sib = agg_last(r.a) + r2.sum + (r2.u < ntiles ? agg_first(r2.a) : 0);
             }
         }
@@ -792,17 +852,30 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
             for (int li = wave >> 1; li < nlev; li += 2) {
                 const int lvl = (int)((lev4 >> (8 * li)) & 255u);
                 int64_t same, sib;
-                edge_counts<W>(side, li, lvl, t, ntiles, n, ts, ts + Teff, Lz, Lx, agg, tilecnt, same, sib);
+                int h_edge;
+                edge_counts<W, (kTile <= 512)>(side, li, lvl, t, ntiles, n, ts, ts + Teff, Lz, Lx, agg, tilecnt,
+                                                 same, sib, h_edge);
                 if (lane == 0) {
                     s_bq[li][2 * side] = same;
                     s_bq[li][2 * side + 1] = sib;
+                    if (wave == 1 && li == 0) s_hdb[Teff] = (int16_t)h_edge;  // the end edge, read only by the level loop
                 }
             }
+            if (self_off && wave == 0) {
+                // the tile's A-count, from the prefetched tile counts
+                int32_t c = 0;
+#pragma unroll
+                for (int k = 0; k < kCW; ++k) c = (int64_t)k == (t >> 6) ? cpre[k] : c;
+                c = __shfl(c, (int)(t & 63), kWave);
+                if (lane == 0) s_aloc[Teff] = (int16_t)c;
+            }
         }
-        if (threadIdx.x == 0) {
+        if (threadIdx.x == 0 && !(summ && self_off)) {
             s_aloc[Teff] = (int16_t)(self_off ? ldg(tilecnt, t) : ldg(tileoff, t + 1) - toff_scan);
-            s_hdb[Teff] = (int16_t)(ts + Teff < n ? hdb<W>(load_key<W>(Lz, Lx, ts + Teff - 1), load_key<W>(Lz, Lx, ts + Teff))
-                                                  : kHdbNone);
+            if (!summ)
+                s_hdb[Teff] = (int16_t)(ts + Teff < n ? hdb<W>(load_key<W>(Lz, Lx, ts + Teff - 1),
+                                                               load_key<W>(Lz, Lx, ts + Teff))
+                                                      : kHdbNone);
         }
         __syncthreads();
         if (summ) {
