@@ -97,6 +97,33 @@ int pp_download(pp_handle* h, uint64_t* z, uint64_t* x, double* c, int64_t cap, 
 int pp_size(pp_handle* h, int64_t* n);
 int pp_capacities(pp_handle* h, int64_t* live_cap, int64_t* append_cap);
 
+/* Pauli-sum products (spv_ops.jl:255-484), computed on the device of h (whose
+ * N sets the word count; its live set is not touched):
+ *   PP_ALG_MUL             out = op(A) · op(B), op = adjoint when conj != 0
+ *                          (_mul_spv and Base.:* on SparsePauliVector and its
+ *                          Adjoint, spv_ops.jl:257-298; a single Pauli is a
+ *                          one-term operand)
+ *   PP_ALG_COMMUTATOR      out = [A, B] (commutator!, spv_ops.jl:418-460)
+ *   PP_ALG_ANTICOMMUTATOR  out = {A, B} (anticommutator!, spv_ops.jl:462-472)
+ * complex_coeffs = 1: coefficients are ComplexF64 (re, im arrays);
+ * complex_coeffs = 0: Float64 (im unused), where a product with a nonzero
+ * imaginary part fails as the reference's convert does (InexactError) and the
+ * commutator fails ("commutator coefficients are imaginary").  Output: keys
+ * strictly increasing, equal keys summed (zero sums kept, as
+ * _triples_to_live!), nout terms; cap < nout returns PP_EOVERFLOW with nout
+ * set and nothing written (cap = 0 asks for the size). */
+enum { PP_ALG_MUL = 0, PP_ALG_COMMUTATOR = 1, PP_ALG_ANTICOMMUTATOR = 2 };
+typedef struct {
+    const uint64_t* z;    /* n x W words */
+    const uint64_t* x;
+    const double* re;
+    const double* im;     /* complex_coeffs only */
+    int64_t n;
+    int32_t conj;         /* PP_ALG_MUL: use the adjoint (conjugated coefficients) */
+} pp_terms;
+int pp_algebra(pp_handle* h, int op, int complex_coeffs, const pp_terms* a, const pp_terms* b, uint64_t* oz,
+               uint64_t* ox, double* ore, double* oim, int64_t cap, int64_t* nout);
+
 /* Sequence evolution (spv_evolve.jl:312-387): L rotations exp(iθ/2 G) O exp(-iθ/2 G),
  * applied left to right.  gz, gx: L generators of W words each; cosv, sinv:
  * the caller's cos θ and sin θ (spv_evolve.jl:341-342).  strict: the compiled

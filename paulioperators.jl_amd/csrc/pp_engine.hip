@@ -28,6 +28,7 @@
 #include "pp_kernels.hpp"
 #include "pp_window.hpp"
 #include "pp_ops.hpp"
+#include "pp_algebra.hpp"
 
 using namespace pp;
 
@@ -125,6 +126,20 @@ bool filter_active(const pp_filter* f) {
     return f->thresh >= 0.0 || f->wmax != INT64_MAX || f->xwmax != INT64_MAX || f->mwmax != INT64_MAX ||
            f->dthresh_w >= 0.0 || f->dthresh_xw >= 0.0;
 }
+
+// Device allocations freed when the owner leaves scope (also on errors).
+struct DevTemps {
+    std::vector<void*> p;
+    template <typename T>
+    T* get(size_t n) {
+        T* q = dalloc<T>(n);
+        p.push_back(q);
+        return q;
+    }
+    ~DevTemps() {
+        for (void* q : p) dfree(q);
+    }
+};
 
 struct DevSet {
     uint64_t* z = nullptr;
@@ -1633,6 +1648,83 @@ struct Batch {
         HIPCHK(hipStreamSynchronize(stream));
     }
 
+    // Pauli-sum products (spv_ops.jl:255-484) on the windowed path's scratch
+    // (the append buffer is empty between calls): pairs in generation order,
+    // stable radix sort by key, run sums.  Host arrays in and out; returns
+    // the output count (terms written only when it fits `cap`).
+    template <int Wt>
+    int64_t algebra(int op, int cplx, const pp_terms& a, const pp_terms& b, uint64_t* oz, uint64_t* ox, double* ore,
+                    double* oim, int64_t cap) {
+        DevTemps tmp;
+        win_init();
+        WinBuf& w = win;
+        const int64_t na = a.n, nb = b.n;
+        if (na == 0 || nb == 0) return 0;
+        if (na > (int64_t)1 << 31 || nb > (int64_t)1 << 31 || na * nb > ((int64_t)1 << 40))
+            throw PPError(PP_EARG, "too many pairs");
+        auto upload = [&](const pp_terms& t, AlgOperand& d) {
+            uint64_t* z = tmp.get<uint64_t>((size_t)t.n * Wt);
+            uint64_t* x = tmp.get<uint64_t>((size_t)t.n * Wt);
+            double* re = tmp.get<double>((size_t)t.n);
+            double* im = cplx ? tmp.get<double>((size_t)t.n) : nullptr;
+            HIPCHK(hipMemcpyAsync(z, t.z, (size_t)t.n * Wt * 8, hipMemcpyHostToDevice, stream));
+            HIPCHK(hipMemcpyAsync(x, t.x, (size_t)t.n * Wt * 8, hipMemcpyHostToDevice, stream));
+            HIPCHK(hipMemcpyAsync(re, t.re, (size_t)t.n * 8, hipMemcpyHostToDevice, stream));
+            if (cplx) HIPCHK(hipMemcpyAsync(im, t.im, (size_t)t.n * 8, hipMemcpyHostToDevice, stream));
+            d = AlgOperand{z, x, re, im, t.n, t.conj};
+        };
+        AlgOperand A, Bo;
+        upload(a, A);
+        upload(b, Bo);
+        if (op != kAlgMul) A.conj = Bo.conj = 0;
+        HIPCHK(hipMemsetAsync(w.err, 0, sizeof(int32_t), stream));
+        // pairs: count, scan, write
+        const int64_t total = na * nb;
+        const int64_t nblk = cdiv(total, kRB);
+        win_blocks(nblk);
+        hipLaunchKernelGGL(ka_pairs<Wt>, dim3((unsigned)nblk), dim3(kBlock), 0, stream, A, Bo, op, cplx, 1, w.bcount,
+                           (const int64_t*)nullptr, (uint64_t*)nullptr, (uint64_t*)nullptr, (double*)nullptr,
+                           (double*)nullptr, (double*)nullptr, w.err);
+        scan_i64(w.bcount, nblk, w.boff, w.small);
+        const int64_t m = read_small(0);
+        if (m == 0) return 0;
+        grow_set(w.A, m, 0);
+        double* pre = tmp.get<double>((size_t)m);
+        double* pim = cplx ? tmp.get<double>((size_t)m) : nullptr;
+        hipLaunchKernelGGL(ka_pairs<Wt>, dim3((unsigned)nblk), dim3(kBlock), 0, stream, A, Bo, op, cplx, 0, w.bcount,
+                           (const int64_t*)w.boff, w.A.z, w.A.x, w.A.c, pre, pim, w.err);
+        HIPCHK(hipGetLastError());
+        int32_t e = 0;
+        HIPCHK(hipMemcpyAsync(&e, w.err, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        if (e & 16)
+            throw PPError(PP_EARG, "InexactError: a product coefficient is not real (real coefficient type)");
+        const DevSet* S = win_sort<Wt>(m);
+        // runs: count, scan, write
+        const int64_t mblk = cdiv(m, kRB);
+        win_blocks(mblk);
+        hipLaunchKernelGGL(ka_runs<Wt>, dim3((unsigned)mblk), dim3(kBlock), 0, stream, S->z, S->x, S->c, m,
+                           (const double*)pre, (const double*)pim, cplx, 1, w.bcount, (const int64_t*)nullptr,
+                           (uint64_t*)nullptr, (uint64_t*)nullptr, (double*)nullptr, (double*)nullptr);
+        scan_i64(w.bcount, mblk, w.boff, w.small);
+        const int64_t nout = read_small(0);
+        if (nout > cap) return nout;
+        uint64_t* dz = tmp.get<uint64_t>((size_t)nout * Wt);
+        uint64_t* dx = tmp.get<uint64_t>((size_t)nout * Wt);
+        double* dre = tmp.get<double>((size_t)nout);
+        double* dim = cplx ? tmp.get<double>((size_t)nout) : nullptr;
+        hipLaunchKernelGGL(ka_runs<Wt>, dim3((unsigned)mblk), dim3(kBlock), 0, stream, S->z, S->x, S->c, m,
+                           (const double*)pre, (const double*)pim, cplx, 0, w.bcount, (const int64_t*)w.boff, dz, dx,
+                           dre, dim);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(oz, dz, (size_t)nout * Wt * 8, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(ox, dx, (size_t)nout * Wt * 8, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(ore, dre, (size_t)nout * 8, hipMemcpyDeviceToHost, stream));
+        if (cplx) HIPCHK(hipMemcpyAsync(oim, dim, (size_t)nout * 8, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        return nout;
+    }
+
     int64_t download(int b, uint64_t* z, uint64_t* x, double* c, int64_t cap) {
         sync_state();
         const SegState& st = h_state[b];
@@ -1792,6 +1884,28 @@ int pp_upload(pp_handle* h, const uint64_t* z, const uint64_t* x, const double* 
     if (n < 0 || (n > 0 && (!z || !x || !c))) return fail(PP_EARG, "bad arrays");
     API_BEGIN
     h->bt.upload(0, z, x, c, n, sorted != 0);
+    API_END
+}
+
+int pp_algebra(pp_handle* h, int op, int complex_coeffs, const pp_terms* a, const pp_terms* b, uint64_t* oz,
+               uint64_t* ox, double* ore, double* oim, int64_t cap, int64_t* nout) {
+    if (!h || !a || !b || !nout) return fail(PP_EARG, "null argument");
+    if (op < PP_ALG_MUL || op > PP_ALG_ANTICOMMUTATOR) return fail(PP_EARG, "unknown operation");
+    if (op == PP_ALG_COMMUTATOR && !complex_coeffs)
+        return fail(PP_EARG, "commutator coefficients are imaginary; use a complex coefficient type (T=ComplexF64)");
+    for (const pp_terms* t : {a, b}) {
+        if (t->n < 0 || (t->n > 0 && (!t->z || !t->x || !t->re || (complex_coeffs && !t->im))))
+            return fail(PP_EARG, "bad operand arrays");
+    }
+    if (cap < 0 || (cap > 0 && (!oz || !ox || !ore || (complex_coeffs && !oim))))
+        return fail(PP_EARG, "bad output arrays");
+    API_BEGIN
+    int64_t n = 0;
+    dispatch_w(h->bt.W, [&](auto wc) {
+        n = h->bt.algebra<decltype(wc)::value>(op, complex_coeffs ? 1 : 0, *a, *b, oz, ox, ore, oim, cap);
+    });
+    *nout = n;
+    if (n > cap) return fail(PP_EOVERFLOW, "output capacity too small (" + std::to_string(n) + " terms)");
     API_END
 }
 

@@ -19,3 +19,4 @@ from .spv import (SparsePauliVector, bit_flip_channel, bit_flip_channel_, bit_ph
                   inner_product, isapprox, norm,
                   pauli_channel, pauli_channel_, phase_flip_channel_, stochastic_clip_, truncate_)
 from .batch import SparsePauliBatch
+from .algebra import anticommutator, commutator, multiply

@@ -19,6 +19,7 @@ PP_OK, PP_EARG, PP_ENOMEM, PP_EHIP, PP_EOVERFLOW, PP_EUNSUPPORTED = range(6)
 EXPORTS = ("pp_create", "pp_destroy", "pp_upload", "pp_download", "pp_size", "pp_capacities", "pp_evolve",
            "pp_evolve_ex", "pp_truncate", "pp_evolve_single", "pp_compact", "pp_lookup", "pp_norm",
            "pp_inner_product", "pp_diff_norm", "pp_setindex", "pp_delete", "pp_sum", "pp_scale", "pp_scale_by_popcount", "pp_expectation_ket", "pp_check",
+           "pp_algebra",
            "pp_batch_create", "pp_batch_destroy", "pp_batch_words", "pp_batch_upload", "pp_batch_download",
            "pp_batch_sizes", "pp_batch_evolve", "pp_batch_scale_by_popcount", "pp_batch_expectation_ket",
            "pp_batch_reserve", "pp_batch_set_tiles", "pp_batch_set_timing", "pp_batch_merge_time",
@@ -48,6 +49,17 @@ class PPStrategyOp(ctypes.Structure):
 
     _fields_ = [("kind", ctypes.c_int32), ("reserved", ctypes.c_int32), ("filter", PPFilter),
                 ("max_terms", ctypes.c_int64), ("value", ctypes.c_double), ("seed", ctypes.c_uint64)]
+
+
+PP_ALG_MUL, PP_ALG_COMMUTATOR, PP_ALG_ANTICOMMUTATOR = range(3)
+
+
+class PPTerms(ctypes.Structure):
+    """pp_terms: one operand of pp_algebra (n terms, W words per key)."""
+
+    _fields_ = [("z", ctypes.POINTER(ctypes.c_uint64)), ("x", ctypes.POINTER(ctypes.c_uint64)),
+                ("re", ctypes.POINTER(ctypes.c_double)), ("im", ctypes.POINTER(ctypes.c_double)),
+                ("n", ctypes.c_int64), ("conj", ctypes.c_int32)]
 
 
 class PPError(RuntimeError):
@@ -102,6 +114,7 @@ def lib():
     L.pp_scale_by_popcount.argtypes = [vp, ctypes.c_int, u64p, dp, dp, dp]
     L.pp_expectation_ket.argtypes = [vp, u64p, dp]
     L.pp_check.argtypes = [vp, P(ctypes.c_int)]
+    L.pp_algebra.argtypes = [vp, ctypes.c_int, ctypes.c_int, P(PPTerms), P(PPTerms), u64p, u64p, dp, dp, i64, P(i64)]
     ip = P(ctypes.c_int)
     L.pp_batch_create.argtypes = [ctypes.c_int, ip, P(i64), P(i64), P(vp)]
     L.pp_batch_destroy.argtypes = [vp]
