@@ -87,6 +87,61 @@ def merge_bandwidth_regime(count=256):
     return b / t / 1e9, 1e6 * t / n, b / n, n
 
 
+def algebra_commutator(reps=3, min_terms=20000, cpu=True):
+    """[H, O] on the 121-qubit lattice (pp_algebra, complex coefficients):
+    H = sum of Z_a Z_b over the 220 bonds + 0.5 sum of X_j (341 terms), O = the
+    mean-Z observable evolved at theta_h = 0.3 on the GPU until it holds at
+    least `min_terms` terms.  Returns the JSON record: pairs/s by wall time
+    (host arrays in and out), the CPU baseline (the numpy oracle, stable
+    order) and a bit-exact check of the GPU result against it."""
+    import numpy as np
+    from pauliprop import Pauli, SparsePauliVector, commutator
+    from pauliprop import workloads as wl
+    from pauliprop.spv import _pack
+    N = 121
+    w = wl.lattice121(0.3)
+    v = SparsePauliVector(N, {(p.z, p.x): c for p, c in w.observable})
+    layers = 0
+    for L in w.layers:
+        v.evolve_(L.generators, L.angles, truncation=w.truncation)
+        layers += 1
+        if len(v) >= min_terms:
+            break
+    oz, ox, oc = v.arrays()
+    O = (oz, ox, oc.astype(np.complex128))
+    hterms = [Pauli(N, Z=[a, b]) for a, b in wl.lattice_bonds(11, 11)] + [Pauli(N, X=[j]) for j in range(1, N + 1)]
+    hz = np.array([_pack(p.z, 2) for p in hterms], dtype=np.uint64)
+    hx = np.array([_pack(p.x, 2) for p in hterms], dtype=np.uint64)
+    hc = np.array([1.0] * 220 + [0.5] * N, dtype=np.complex128)
+    H = (hz, hx, hc)
+    pairs = len(hc) * len(oc)
+    out = commutator(N, H, O)  # warmup
+    best = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = commutator(N, H, O)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    rec = {"metric": "commutator pairs/s", "value": pairs / best, "unit": "pairs/s", "dtype": "c128",
+           "seconds": best,
+           "config": {"workload": f"[H, O] on the 121-qubit lattice: H = 220 ZZ bonds + 0.5 X on 121 sites, "
+                                  f"O = mean Z evolved {layers} layers at theta_h = 0.3",
+                      "pairs": pairs, "H_terms": len(hc), "O_terms": len(oc), "out_terms": len(out[2])},
+           "cpu_baseline": None}
+    if cpu:
+        import algebra_oracle as ao
+        t0 = time.perf_counter()
+        ref = ao.product(H, O, ao.COMMUTATOR)
+        cdt = time.perf_counter() - t0
+        same = (np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1])
+                and np.array_equal(out[2].view(np.uint64), ref[2].view(np.uint64)))
+        assert same, "pp_algebra differs from the oracle"
+        rec["cpu_baseline"] = {"value": pairs / cdt, "unit": "pairs/s", "cores": 1, "kind": "port",
+                               "sample": f"the same commutator ({cdt:.1f} s, numpy oracle)"}
+        rec["bitexact_vs_oracle"] = same
+    return rec
+
+
 def sweep_cpu(workloads, budget_s):
     import pyoracle as po
     from pauliprop import compile_filter
@@ -117,6 +172,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graphs", type=int, default=0, help="replay hipGraph-captured layer sequences (1/0)")
+    ap.add_argument("--algebra", action="store_true",
+                    help="run only the Pauli-sum product measurement ([H, O] on the 121-qubit lattice)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -130,6 +187,9 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": 1 << 30, "launches": 3}))
+        return
+    if args.algebra:
+        print(json.dumps(algebra_commutator(cpu=not args.no_cpu)), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: measures one GPU only; --gpus must be 1 (the sweep's 16 theta_h vectors "

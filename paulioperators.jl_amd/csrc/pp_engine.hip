@@ -188,6 +188,7 @@ struct WinBuf {
     int32_t* err = nullptr;
     uint32_t* ticket = nullptr;
     unsigned long long* orand = nullptr;  // [4 W]: OR then AND of the key words
+    unsigned long long* orpart = nullptr; // [kOrandBlocks][4 W]: per-block partials
     unsigned long long* qhist = nullptr;  // [256]
     double* out = nullptr;
     uint32_t epoch = 0;
@@ -898,7 +899,7 @@ struct Batch {
         free_set(w.A); free_set(w.T);
         dfree(w.bcount); dfree(w.boff); dfree(w.bsum); dfree(w.hist); dfree(w.hoff); dfree(w.status);
         dfree(w.partial); dfree(w.gens); dfree(w.filt); dfree(w.small); dfree(w.err); dfree(w.ticket);
-        dfree(w.orand); dfree(w.qhist); dfree(w.out);
+        dfree(w.orand); dfree(w.orpart); dfree(w.qhist); dfree(w.out);
         w = WinBuf();
     }
 
@@ -910,6 +911,7 @@ struct Batch {
         w.err = dalloc<int32_t>(1);
         w.ticket = dalloc<uint32_t>(1);
         w.orand = dalloc<unsigned long long>(4 * (size_t)W);
+        w.orpart = dalloc<unsigned long long>((size_t)kOrandBlocks * 4 * W);
         w.qhist = dalloc<unsigned long long>(kRadix);
         w.out = dalloc<double>(1);
         w.partial = dalloc<double2>(1024);
@@ -998,10 +1000,10 @@ struct Batch {
         WinBuf& w = win;
         if (m <= 1) return &w.A;
         grow_set(w.T, m, 0);
-        HIPCHK(hipMemsetAsync(w.orand, 0, 2 * (size_t)Wt * 8, stream));
-        HIPCHK(hipMemsetAsync(w.orand + 2 * Wt, 0xff, 2 * (size_t)Wt * 8, stream));
-        hipLaunchKernelGGL(ks_orand<Wt>, dim3((unsigned)std::min<int64_t>(cdiv(m, kBlock), 1024)), dim3(kBlock), 0,
-                           stream, w.A.z, w.A.x, m, w.orand);
+        const int onb = (int)std::min<int64_t>(cdiv(m, kBlock), kOrandBlocks);
+        hipLaunchKernelGGL(ks_orand<Wt>, dim3((unsigned)onb), dim3(kBlock), 0, stream, w.A.z, w.A.x, m, w.orpart);
+        hipLaunchKernelGGL(ks_orand_fin<Wt>, dim3(1), dim3(kBlock), 0, stream, (const unsigned long long*)w.orpart,
+                           onb, w.orand);
         unsigned long long oa[4 * 2];
         HIPCHK(hipMemcpyAsync(oa, w.orand, 4 * (size_t)Wt * 8, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));

@@ -173,10 +173,14 @@ __device__ __forceinline__ int key_digit(const Key<W>& k, int dd) {
     return (int)((key_word<W>(k, bit >> 6) >> (bit & 63)) & 0xffull);
 }
 
-// OR and AND of every key word (constant digits are skipped).
+// OR and AND of every key word (constant digits are skipped), in two
+// stages without atomics (same-address atomics from every wave of a large
+// grid serialise): per-block partials part[block][4 W], then ks_orand_fin.
+constexpr int kOrandBlocks = 1024;
 template <int W>
 __global__ void __launch_bounds__(kBlock) ks_orand(const uint64_t* __restrict__ Z, const uint64_t* __restrict__ X,
-                                                   int64_t m, unsigned long long* __restrict__ acc) {
+                                                   int64_t m, unsigned long long* __restrict__ part) {
+    __shared__ unsigned long long s_p[kBlock / kWave][4 * W];
     uint64_t o[2 * W], a[2 * W];
 #pragma unroll
     for (int w = 0; w < 2 * W; ++w) {
@@ -195,8 +199,63 @@ __global__ void __launch_bounds__(kBlock) ks_orand(const uint64_t* __restrict__ 
     }
 #pragma unroll
     for (int w = 0; w < 2 * W; ++w) {
-        atomicOr(acc + w, (unsigned long long)o[w]);
-        atomicAnd(acc + 2 * W + w, (unsigned long long)a[w]);
+#pragma unroll
+        for (int dd = kWave / 2; dd > 0; dd >>= 1) {
+            o[w] |= __shfl_xor(o[w], dd, kWave);
+            a[w] &= __shfl_xor(a[w], dd, kWave);
+        }
+    }
+    const int wave = threadIdx.x / kWave;
+    if (threadIdx.x % kWave == 0) {
+#pragma unroll
+        for (int w = 0; w < 2 * W; ++w) {
+            s_p[wave][w] = o[w];
+            s_p[wave][2 * W + w] = a[w];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 4 * W) {
+        const int k = threadIdx.x;
+        unsigned long long v = s_p[0][k];
+        for (int q = 1; q < kBlock / kWave; ++q) v = k < 2 * W ? (v | s_p[q][k]) : (v & s_p[q][k]);
+        part[(int64_t)blockIdx.x * 4 * W + k] = v;
+    }
+}
+
+// One workgroup: acc[0, 2W) = OR and acc[2W, 4W) = AND over nb partials.
+template <int W>
+__global__ void __launch_bounds__(kBlock) ks_orand_fin(const unsigned long long* __restrict__ part, int nb,
+                                                       unsigned long long* __restrict__ acc) {
+    __shared__ unsigned long long s_p[kBlock / kWave][4 * W];
+    unsigned long long v[4 * W];
+#pragma unroll
+    for (int k = 0; k < 4 * W; ++k) v[k] = k < 2 * W ? 0ull : ~0ull;
+    for (int b = threadIdx.x; b < nb; b += kBlock) {
+#pragma unroll
+        for (int k = 0; k < 4 * W; ++k) {
+            const unsigned long long x = part[(int64_t)b * 4 * W + k];
+            v[k] = k < 2 * W ? (v[k] | x) : (v[k] & x);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4 * W; ++k) {
+#pragma unroll
+        for (int dd = kWave / 2; dd > 0; dd >>= 1) {
+            const unsigned long long y = __shfl_xor(v[k], dd, kWave);
+            v[k] = k < 2 * W ? (v[k] | y) : (v[k] & y);
+        }
+    }
+    const int wave = threadIdx.x / kWave;
+    if (threadIdx.x % kWave == 0) {
+#pragma unroll
+        for (int k = 0; k < 4 * W; ++k) s_p[wave][k] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < 4 * W) {
+        const int k = threadIdx.x;
+        unsigned long long r = s_p[0][k];
+        for (int q = 1; q < kBlock / kWave; ++q) r = k < 2 * W ? (r | s_p[q][k]) : (r & s_p[q][k]);
+        acc[k] = r;
     }
 }
 
