@@ -142,6 +142,133 @@ def algebra_commutator(reps=3, min_terms=20000, cpu=True):
     return rec
 
 
+def secondary_rows(cpu=True, layers=10):
+    """The windowed evolution, device lookups and AdaptiveTruncation, each
+    timed on the GPU and beside its CPU path, on the 64-qubit chain
+    (RZZ/RX/RZ layers, depolarizing 1e-3, |c| <= 1e-6, weight <= 16) after
+    `layers` layers (6.8e6 live terms):
+      window    evolve! at window = 8 (and window = 1) over the first layers,
+                against the oracle at window = 8 (the reference's quicksort
+                order; <O> compared to 1e-10);
+      lookup    getindex of 1e6 keys (half present) by device binary search,
+                against numpy's searchsorted over the same sorted keys;
+      adaptive  truncate!(v, AdaptiveTruncation(1e6)) by device radix select,
+                against the oracle's sort-based restatement; terms compared
+                to the bit."""
+    import ctypes
+    import numpy as np
+    from pauliprop import AdaptiveTruncation, PPCounters, SparsePauliVector, compile_filter
+    from pauliprop import _lib
+    from pauliprop import workloads as wl
+    w = wl.chain64()
+    Ls = w.layers[:layers]
+    obs = {(p.z, p.x): c for p, c in w.observable}
+    recs = {}
+
+    def gpu_evolve(window):
+        v = SparsePauliVector(w.N, obs)
+        swept = 0
+        t0 = time.perf_counter()
+        for L in Ls:
+            cnt = PPCounters()
+            v.evolve_(L.generators, L.angles, window=window, truncation=w.truncation, counters=cnt)
+            v.depolarizing_channel_(w.channel[1])
+            swept += cnt.terms_swept
+        n = len(v)
+        return v, swept, time.perf_counter() - t0, n
+
+    gpu_evolve(8)  # warmup
+    v8, sw8, t8, n8 = gpu_evolve(8)
+    _, sw1, t1, n1 = gpu_evolve(1)
+    ev8 = v8.expectation_value(0)
+    rec = {"metric": "terms/s (windowed evolve!, window = 8)", "value": sw8 / t8, "unit": "terms/s",
+           "seconds": t8, "window1_terms_per_s": sw1 / t1, "live_terms": n8,
+           "config": {"workload": f"64-qubit chain, first {layers} layers, window 8", "term_rotations": sw8},
+           "cpu_baseline": None}
+    if cpu:
+        import pyoracle as po
+        o = po.OracleSPV(w.N, [(p.z, p.x, c) for p, c in w.observable])
+        f = compile_filter(w.truncation)
+        osw = 0
+        t0 = time.perf_counter()
+        for L in Ls:
+            osw += o.evolve([(p.z, p.x) for p in L.generators], L.angles, window=8, strict=f).terms_swept
+            o.channel(w.channel[0], w.channel[1])
+        ct = time.perf_counter() - t0
+        oev = o.expectation(0)
+        assert abs(oev - ev8) <= 1e-10, ("windowed <O> vs oracle", ev8, oev)
+        rec["cpu_baseline"] = {"value": osw / ct, "unit": "terms/s", "cores": 1, "kind": "port",
+                               "sample": f"the same {layers} layers at window 8 ({ct:.1f} s)"}
+        rec["expectation"] = ev8
+        rec["expectation_oracle"] = oev
+    recs["window"] = rec
+
+    # lookups: 1e6 keys, half taken from the vector, half random
+    z, x, c = v8.arrays()
+    rng = np.random.default_rng(5)
+    nq = 1_000_000
+    pick = rng.integers(0, len(c), size=nq // 2)
+    qz = np.concatenate([z[pick, 0], rng.integers(0, 2 ** 63, size=nq - nq // 2, dtype=np.uint64)])
+    qx = np.concatenate([x[pick, 0], rng.integers(0, 2 ** 63, size=nq - nq // 2, dtype=np.uint64)])
+    qz = np.ascontiguousarray(qz.reshape(-1, 1))
+    qx = np.ascontiguousarray(qx.reshape(-1, 1))
+    vals = np.zeros(nq)
+    found = np.zeros(nq, dtype=np.int32)
+    L = _lib.lib()
+    args = (v8._h, _lib.u64p(qz), _lib.u64p(qx), nq, _lib.dp(vals), found.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    _lib.check(L.pp_lookup(*args))
+    t0 = time.perf_counter()
+    _lib.check(L.pp_lookup(*args))
+    tl = time.perf_counter() - t0
+    rec = {"metric": "lookups/s (getindex by device binary search)", "value": nq / tl, "unit": "keys/s",
+           "seconds": tl, "config": {"workload": f"{nq} keys (half present) in {len(c)} live terms",
+                                     "found": int(found.sum())}, "cpu_baseline": None}
+    if cpu:
+        keys = np.zeros(len(c), dtype=[("z", "<u8"), ("x", "<u8")])
+        keys["z"], keys["x"] = z[:, 0], x[:, 0]
+        q = np.zeros(nq, dtype=keys.dtype)
+        q["z"], q["x"] = qz[:, 0], qx[:, 0]
+        t0 = time.perf_counter()
+        pos = np.searchsorted(keys, q)
+        hit = pos < len(c)
+        hit[hit] = keys[pos[hit]] == q[hit]
+        cv = np.where(hit, c[np.minimum(pos, len(c) - 1)], 0.0)
+        ct = time.perf_counter() - t0
+        assert np.array_equal(hit, found.astype(bool)) and np.array_equal(cv[hit], vals[hit]), "lookup mismatch"
+        rec["cpu_baseline"] = {"value": nq / ct, "unit": "keys/s", "cores": 1, "kind": "port",
+                               "sample": f"numpy searchsorted, same keys ({ct:.2f} s)"}
+    recs["lookup"] = rec
+
+    # AdaptiveTruncation to 1e6 terms
+    keep = 1_000_000
+    va = SparsePauliVector(w.N, (z, x, c))
+    len(va)
+    t0 = time.perf_counter()
+    va.truncate_(AdaptiveTruncation(keep, 1e-12))
+    na = len(va)
+    ta = time.perf_counter() - t0
+    rec = {"metric": "terms/s (AdaptiveTruncation by radix select)", "value": len(c) / ta, "unit": "terms/s",
+           "seconds": ta, "config": {"workload": f"{len(c)} live terms truncated to {keep}", "kept": na},
+           "cpu_baseline": None}
+    if cpu:
+        import pyoracle as po
+        o2 = po.OracleSPV(w.N)
+        zc, xc, cc = (np.ascontiguousarray(a) for a in (z, x, c))
+        po.lib().or_load(o2.h, po._u64p(zc), po._u64p(xc), po._dp(cc), len(cc))  # grows, sorts (already sorted)
+        t0 = time.perf_counter()
+        o2.adaptive(keep, 1e-12)
+        ct = time.perf_counter() - t0
+        gz, gx, gc = va.arrays()
+        oz, ox, oc = o2.arrays()
+        same = np.array_equal(gz, oz) and np.array_equal(gx, ox) and np.array_equal(gc.view(np.uint64), oc.view(np.uint64))
+        assert same, "adaptive truncation differs from the oracle"
+        rec["cpu_baseline"] = {"value": len(c) / ct, "unit": "terms/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle or_adaptive (sort-based), same vector ({ct:.2f} s)"}
+        rec["bitexact_vs_oracle"] = same
+    recs["adaptive"] = rec
+    return recs
+
+
 def sweep_cpu(workloads, budget_s):
     import pyoracle as po
     from pauliprop import compile_filter
@@ -172,6 +299,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graphs", type=int, default=0, help="replay hipGraph-captured layer sequences (1/0)")
+    ap.add_argument("--rows", action="store_true",
+                    help="run only the windowed-evolution, lookup and AdaptiveTruncation measurements")
     ap.add_argument("--algebra", action="store_true",
                     help="run only the Pauli-sum product measurement ([H, O] on the 121-qubit lattice)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -187,6 +316,9 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": 1 << 30, "launches": 3}))
+        return
+    if args.rows:
+        print(json.dumps(secondary_rows(cpu=not args.no_cpu)), flush=True)
         return
     if args.algebra:
         print(json.dumps(algebra_commutator(cpu=not args.no_cpu)), flush=True)
