@@ -35,6 +35,18 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def host_cpu() -> str:
+    """The host CPU model (the CPU baselines run on one of its cores)."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def sweep_gpu(workloads, timing=True, graphs=False):
     """All θ_h instances in one batch: per layer one evolve! per instance, then
     the dephasing channel.  Returns (seconds, total counters, <O> list, merge
@@ -136,7 +148,7 @@ def algebra_commutator(reps=3, min_terms=20000, cpu=True):
         same = (np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1])
                 and np.array_equal(out[2].view(np.uint64), ref[2].view(np.uint64)))
         assert same, "pp_algebra differs from the oracle"
-        rec["cpu_baseline"] = {"value": pairs / cdt, "unit": "pairs/s", "cores": 1, "kind": "port",
+        rec["cpu_baseline"] = {"value": pairs / cdt, "unit": "pairs/s", "cores": 1, "kind": "port", "host": host_cpu(),
                                "sample": f"the same commutator ({cdt:.1f} s, numpy oracle)"}
         rec["bitexact_vs_oracle"] = same
     return rec
@@ -197,7 +209,7 @@ def secondary_rows(cpu=True, layers=10):
         ct = time.perf_counter() - t0
         oev = o.expectation(0)
         assert abs(oev - ev8) <= 1e-10, ("windowed <O> vs oracle", ev8, oev)
-        rec["cpu_baseline"] = {"value": osw / ct, "unit": "terms/s", "cores": 1, "kind": "port",
+        rec["cpu_baseline"] = {"value": osw / ct, "unit": "terms/s", "cores": 1, "kind": "port", "host": host_cpu(),
                                "sample": f"the same {layers} layers at window 8 ({ct:.1f} s)"}
         rec["expectation"] = ev8
         rec["expectation_oracle"] = oev
@@ -235,7 +247,7 @@ def secondary_rows(cpu=True, layers=10):
         cv = np.where(hit, c[np.minimum(pos, len(c) - 1)], 0.0)
         ct = time.perf_counter() - t0
         assert np.array_equal(hit, found.astype(bool)) and np.array_equal(cv[hit], vals[hit]), "lookup mismatch"
-        rec["cpu_baseline"] = {"value": nq / ct, "unit": "keys/s", "cores": 1, "kind": "port",
+        rec["cpu_baseline"] = {"value": nq / ct, "unit": "keys/s", "cores": 1, "kind": "port", "host": host_cpu(),
                                "sample": f"numpy searchsorted, same keys ({ct:.2f} s)"}
     recs["lookup"] = rec
 
@@ -262,11 +274,52 @@ def secondary_rows(cpu=True, layers=10):
         oz, ox, oc = o2.arrays()
         same = np.array_equal(gz, oz) and np.array_equal(gx, ox) and np.array_equal(gc.view(np.uint64), oc.view(np.uint64))
         assert same, "adaptive truncation differs from the oracle"
-        rec["cpu_baseline"] = {"value": len(c) / ct, "unit": "terms/s", "cores": 1, "kind": "port",
+        rec["cpu_baseline"] = {"value": len(c) / ct, "unit": "terms/s", "cores": 1, "kind": "port", "host": host_cpu(),
                                "sample": f"oracle or_adaptive (sort-based), same vector ({ct:.2f} s)"}
         rec["bitexact_vs_oracle"] = same
     recs["adaptive"] = rec
     return recs
+
+
+def batched_with_cpu(max_depth=6, cap=1_000_000, budget_s=20.0, cpu=True):
+    """The batched workload (256 instances, seed 7) over its first `max_depth`
+    layers on the GPU (pauliprop.batched_bench.run: two key-width batches,
+    captured graphs, autotuned tiles, instances stopped past `cap` live
+    terms), and the oracle on a bounded sample of the same instances: one
+    instance at a time in seed order, the same 64-rotation chunks and the same
+    stop rule, until `budget_s` seconds of CPU work."""
+    from pauliprop import batched_bench, compile_filter
+    from pauliprop import workloads as wl
+    rec = batched_bench.run(256, cap, 64, True, True, max_depth, log=log)
+    rec = {"metric": "terms/s (batched, first layers)", "value": rec["terms_per_s"], "unit": "terms/s", **rec,
+           "cpu_baseline": None}
+    if cpu:
+        import pyoracle as po
+        ws = wl.batched_instances(256, seed=7, max_depth=max_depth)
+        swept = 0
+        done = 0
+        t0 = time.perf_counter()
+        for w in ws:
+            o = po.OracleSPV(w.N, [(p.z, p.x, c) for p, c in w.observable])
+            f = compile_filter(w.truncation)
+            stop = False
+            for L in w.layers:
+                g = [(p.z, p.x) for p in L.generators]
+                for s0 in range(0, len(g), 64):
+                    swept += o.evolve(g[s0:s0 + 64], L.angles[s0:s0 + 64], strict=f).terms_swept
+                    if len(o) > cap:
+                        stop = True
+                        break
+                if stop:
+                    break
+            done += 1
+            if time.perf_counter() - t0 > budget_s:
+                break
+        ct = time.perf_counter() - t0
+        rec["cpu_baseline"] = {"value": swept / ct, "unit": "terms/s", "cores": 1, "kind": "port", "host": host_cpu(),
+                               "sample": f"the first {done} of 256 instances, same layers and stop rule "
+                                         f"({swept} term-rotations, {ct:.1f} s)"}
+    return rec
 
 
 def sweep_cpu(workloads, budget_s):
@@ -299,6 +352,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graphs", type=int, default=0, help="replay hipGraph-captured layer sequences (1/0)")
+    ap.add_argument("--batched", action="store_true",
+                    help="run only the batched workload's first 6 layers with its CPU baseline")
     ap.add_argument("--rows", action="store_true",
                     help="run only the windowed-evolution, lookup and AdaptiveTruncation measurements")
     ap.add_argument("--algebra", action="store_true",
@@ -316,6 +371,9 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": 1 << 30, "launches": 3}))
+        return
+    if args.batched:
+        print(json.dumps(batched_with_cpu(cpu=not args.no_cpu)), flush=True)
         return
     if args.rows:
         print(json.dumps(secondary_rows(cpu=not args.no_cpu)), flush=True)
@@ -374,7 +432,7 @@ def main():
     if not args.no_cpu:
         log("cpu baseline (oracle, 1 thread)")
         cv, done, cs, cdt, cevs = sweep_cpu(workloads, args.cpu_budget)
-        cpu = {"value": cv, "unit": "terms/s", "cores": 1, "kind": "port",
+        cpu = {"value": cv, "unit": "terms/s", "cores": 1, "kind": "port", "host": host_cpu(),
                "sample": f"the same sweep, first {done} of 16 theta_h values ({cs} term-rotations, {cdt:.1f} s)"}
         for a, b in zip(evs, cevs):
             assert abs(a - b) <= 1e-12, ("<O> mismatch vs oracle", a, b)

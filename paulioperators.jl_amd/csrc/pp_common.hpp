@@ -53,7 +53,6 @@ struct SegDesc {
     int64_t* tileoff;     // exclusive scan of tilecnt (ntiles + 1 entries)
     int64_t* bnd;         // boundary segment counts, 4 per (tile, level)
     uint64_t* agg;        // segment summaries, 4 levels per sweep tile
-    int64_t* split;       // merge-path splits
     uint64_t* status;     // merge look-back words
     const struct FilterDev* fs;  // strict filter
     const struct FilterDev* fl;  // local filter

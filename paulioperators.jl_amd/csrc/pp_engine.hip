@@ -158,11 +158,10 @@ struct InstBuf {
     int64_t* tileoff = nullptr;
     int64_t* bnd = nullptr;
     uint64_t* agg = nullptr;
-    int64_t* split = nullptr;
     uint64_t* status = nullptr;
     int64_t cap_tiles = 0;   // sweep-tile capacity of flags/wpre/tilecnt/tileoff
     int64_t cap_bnd = 0;     // entries of bnd
-    int64_t cap_split = 0;   // entries of split/status
+    int64_t cap_split = 0;   // entries of status (merge tiles + 1)
     int maxlev = 1;
 };
 
@@ -327,7 +326,7 @@ struct Batch {
     }
     static void free_inst(InstBuf& b) {
         free_set(b.L[0]); free_set(b.L[1]); free_set(b.C);
-        dfree(b.flags); dfree(b.wpre); dfree(b.tilecnt); dfree(b.tileoff); dfree(b.bnd); dfree(b.agg); dfree(b.split); dfree(b.status);
+        dfree(b.flags); dfree(b.wpre); dfree(b.tilecnt); dfree(b.tileoff); dfree(b.bnd); dfree(b.agg); dfree(b.status);
         b = InstBuf();
     }
 
@@ -437,9 +436,8 @@ struct Batch {
         I.maxlev = ml;
         const int64_t sp = cdiv(I.L[0].cap + I.C.cap, 512) + 2;
         if (I.cap_split < sp) {
-            dfree(I.split); dfree(I.status);
+            dfree(I.status);
             I.cap_split = sp;
-            I.split = dalloc<int64_t>(sp);
             I.status = dalloc<uint64_t>(sp);
             // look-back words must start "not ready": freed memory may hold
             // words of another handle whose tags collide with this one's
@@ -460,7 +458,7 @@ struct Batch {
             }
             d.Cz = I.C.z; d.Cx = I.C.x; d.Cc = I.C.c;
             d.flags = I.flags; d.wpre = I.wpre; d.tilecnt = I.tilecnt; d.tileoff = I.tileoff;
-            d.bnd = I.bnd; d.agg = I.agg; d.split = I.split; d.status = I.status;
+            d.bnd = I.bnd; d.agg = I.agg; d.status = I.status;
             d.fs = d_filt + 2 * b;
             d.fl = d_filt + 2 * b + 1;
             d.cap = std::min(I.L[0].cap, I.L[1].cap);
