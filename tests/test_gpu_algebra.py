@@ -138,3 +138,37 @@ def test_empty_and_capacity():
                                ctypes.byref(n))
     assert rc == _lib.PP_EOVERFLOW and n.value == len(want[2]) > 1
     assert ore[0] == 7.0 and oim[0] == 7.0
+
+
+def test_single_run_of_all_pairs():
+    """Every pair lands on one key (A and B hold one basis string each, with
+    repeats): one run of nA x nB terms summed left to right in generation
+    order, as the reference's scan over its sorted workspace does."""
+    N = 50
+    rng = np.random.default_rng(21)
+    na, nb = 400, 400
+    A = (np.full((na, 1), 0b1011, np.uint64), np.full((na, 1), 0b0110, np.uint64),
+         rng.standard_normal(na) + 1j * rng.standard_normal(na))
+    B = (np.full((nb, 1), 0b0011, np.uint64), np.full((nb, 1), 0b0101, np.uint64),
+         rng.standard_normal(nb) * 1e8 + 1j * rng.standard_normal(nb))
+    log("pp_algebra one run of 160000 pairs")
+    got = product(N, A, B, ao.MUL)
+    assert len(got[2]) == 1
+    same_bits(got, ao.product(A, B, ao.MUL))
+
+
+def test_commuting_operands_and_full_width_keys():
+    """Z-only sums commute: an empty commutator; keys that use the top bits
+    of N = 128 (both words) give the same products as the oracle."""
+    N = 128
+    rng = np.random.default_rng(23)
+    n = 200
+    z = rng.integers(0, 2 ** 63, size=(n, 2), dtype=np.uint64) | np.uint64(1 << 63)
+    A = (z, np.zeros_like(z), rng.standard_normal(n) + 1j * rng.standard_normal(n))
+    B = (z[::-1].copy(), np.zeros_like(z), rng.standard_normal(n) + 0j)
+    log("pp_algebra commuting Z sums, N = 128")
+    assert len(product(N, A, B, ao.COMMUTATOR)[2]) == 0
+    x = rng.integers(0, 2 ** 63, size=(n, 2), dtype=np.uint64) | np.uint64(1 << 63)
+    C = (z, x, rng.standard_normal(n) + 1j * rng.standard_normal(n))
+    for op in (ao.MUL, ao.COMMUTATOR, ao.ANTICOMMUTATOR):
+        same_bits(product(N, A, C, op), ao.product(A, C, op))
