@@ -442,10 +442,53 @@ constexpr int kEpochShift = 40;
 constexpr uint64_t kEpochMask = (1ull << 22) - 1;
 constexpr uint64_t kCountMask = (1ull << kEpochShift) - 1;
 
+// Key order A[ia] <= B[ib] reading the z words first and the x words only
+// where the z words are equal: a search probe then fetches two scattered
+// cache lines instead of four unless its keys share z.
+template <int W>
+__device__ __forceinline__ bool key_le_zfirst(const uint64_t* __restrict__ Az, const uint64_t* __restrict__ Ax,
+                                              int64_t ia, const uint64_t* __restrict__ Bz,
+                                              const uint64_t* __restrict__ Bx, int64_t ib) {
+    uint64_t a[W], b[W];
+    if constexpr (W == 1) {
+        a[0] = ldg(Az, ia);
+        b[0] = ldg(Bz, ib);
+    } else {
+        const ulonglong2 u = ldg(reinterpret_cast<const ulonglong2*>(Az), ia);
+        const ulonglong2 v = ldg(reinterpret_cast<const ulonglong2*>(Bz), ib);
+        a[0] = u.x; a[1] = u.y;
+        b[0] = v.x; b[1] = v.y;
+    }
+    bool tie = true, lt = false;
+#pragma unroll
+    for (int w = W - 1; w >= 0; --w) {
+        if (tie && a[w] != b[w]) {
+            lt = a[w] < b[w];
+            tie = false;
+        }
+    }
+    if (!tie) return lt;
+    if constexpr (W == 1) {
+        a[0] = ldg(Ax, ia);
+        b[0] = ldg(Bx, ib);
+    } else {
+        const ulonglong2 u = ldg(reinterpret_cast<const ulonglong2*>(Ax), ia);
+        const ulonglong2 v = ldg(reinterpret_cast<const ulonglong2*>(Bx), ib);
+        a[0] = u.x; a[1] = u.y;
+        b[0] = v.x; b[1] = v.y;
+    }
+#pragma unroll
+    for (int w = W - 1; w >= 0; --w)
+        if (a[w] != b[w]) return a[w] < b[w];
+    return true;
+}
+
 // Merge-path split: the number of live terms among the first d outputs of
 // the merge of L (n) and C (m), live first on equal keys.  Wave-cooperative
-// 64-ary search over the monotone predicate L[i] <= C[d-1-i].
-template <int W>
+// 64-ary search over the monotone predicate L[i] <= C[d-1-i].  ZFIRST: probes
+// read the x words only on z ties (less traffic where the merge streams; one
+// more dependent load where keys tie, so latency-bound callers keep false).
+template <int W, bool ZFIRST = false>
 __device__ int64_t merge_split_wave(const uint64_t* __restrict__ Lz, const uint64_t* __restrict__ Lx, int64_t n,
                                     const uint64_t* __restrict__ Cz, const uint64_t* __restrict__ Cx, int64_t m,
                                     int64_t d) {
@@ -456,7 +499,11 @@ __device__ int64_t merge_split_wave(const uint64_t* __restrict__ Lz, const uint6
         const int64_t step = (hi - lo + kWave - 1) / kWave;
         const int64_t i = lo + step * (int64_t)lane;
         const bool valid = i < hi;
-        const bool pred = valid && key_le<W>(load_key<W>(Lz, Lx, i), load_key<W>(Cz, Cx, d - 1 - i));
+        bool pred = false;
+        if (valid) {
+            if constexpr (ZFIRST) pred = key_le_zfirst<W>(Lz, Lx, i, Cz, Cx, d - 1 - i);
+            else pred = key_le<W>(load_key<W>(Lz, Lx, i), load_key<W>(Cz, Cx, d - 1 - i));
+        }
         const uint64_t tm = __ballot(pred);
         const uint64_t vm = __ballot(valid);
         if (tm == vm) {
