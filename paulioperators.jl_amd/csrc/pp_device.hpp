@@ -314,6 +314,49 @@ __device__ int64_t seg_end(const uint64_t* __restrict__ Z, const uint64_t* __res
 }
 
 
+// K_q agrees with kp on every combined-key bit >= b (hdb(K_q, kp) < b).  The
+// x words (combined bits [0, 64 W)) are read only when b falls among them:
+// for a level in the z words, a probe fetches half the bytes.
+template <int W>
+__device__ __forceinline__ bool agrees_above(const uint64_t* __restrict__ Z, const uint64_t* __restrict__ X,
+                                             int64_t q, const Key<W>& kp, int b) {
+    uint64_t z[W], x[W];
+    const bool need_x = b < 64 * W;
+    if constexpr (W == 1) {
+        z[0] = ldg(Z, q);
+        x[0] = need_x ? ldg(X, q) : 0ull;
+    } else {
+        const ulonglong2 u = ldg(reinterpret_cast<const ulonglong2*>(Z), q);
+        z[0] = u.x; z[1] = u.y;
+        if (need_x) {
+            const ulonglong2 v = ldg(reinterpret_cast<const ulonglong2*>(X), q);
+            x[0] = v.x; x[1] = v.y;
+        } else {
+            x[0] = x[1] = 0ull;
+        }
+    }
+    bool ok = true;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        const int lo = 64 * W + 64 * w;  // combined position of bit 0 of z[w]
+        if (b < lo + 64) {
+            const uint64_t m = b <= lo ? ~0ull : (~0ull << (b - lo));
+            ok = ok && ((z[w] ^ kp.z[w]) & m) == 0;
+        }
+    }
+    if (need_x) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const int lo = 64 * w;
+            if (b < lo + 64) {
+                const uint64_t m = b <= lo ? ~0ull : (~0ull << (b - lo));
+                ok = ok && ((x[w] ^ kp.x[w]) & m) == 0;
+            }
+        }
+    }
+    return ok;
+}
+
 // Wave-cooperative versions: all 64 lanes call with the same arguments and
 // get the same result.  Each step probes 64 positions at once (a galloping
 // search with stride 64^k, then a 64-ary refinement), so a segment of length
@@ -327,7 +370,7 @@ __device__ int64_t seg_start_wave(const uint64_t* __restrict__ Z, const uint64_t
     int64_t in = p, out = -1, stride = 1;
     while (true) {
         const int64_t q = in - (int64_t)(lane + 1) * stride;
-        const bool inside = q >= 0 && hdb<W>(load_key<W>(Z, X, q), kp) < b;
+        const bool inside = q >= 0 && agrees_above<W>(Z, X, q, kp, b);
         const uint64_t outm = __ballot(!inside);
         if (outm == 0) {
             in -= (int64_t)kWave * stride;
@@ -344,7 +387,7 @@ __device__ int64_t seg_start_wave(const uint64_t* __restrict__ Z, const uint64_t
         const int64_t step = (in - out - 1 + kWave - 1) / kWave;
         const int64_t q = out + step * (int64_t)(lane + 1);
         const bool valid = q < in;
-        const bool inside = valid && hdb<W>(load_key<W>(Z, X, q), kp) < b;
+        const bool inside = valid && agrees_above<W>(Z, X, q, kp, b);
         const uint64_t inm = __ballot(inside);
         if (inm == 0) {
             int64_t kmax = (in - out - 1) / step;
@@ -368,7 +411,7 @@ __device__ int64_t seg_end_wave(const uint64_t* __restrict__ Z, const uint64_t* 
     int64_t in = p, out = n, stride = 1;
     while (true) {
         const int64_t q = in + (int64_t)(lane + 1) * stride;
-        const bool inside = q < n && hdb<W>(load_key<W>(Z, X, q), kp) < b;
+        const bool inside = q < n && agrees_above<W>(Z, X, q, kp, b);
         const uint64_t outm = __ballot(!inside);
         if (outm == 0) {
             in += (int64_t)kWave * stride;
@@ -385,7 +428,7 @@ __device__ int64_t seg_end_wave(const uint64_t* __restrict__ Z, const uint64_t* 
         const int64_t step = (out - in - 1 + kWave - 1) / kWave;
         const int64_t q = in + step * (int64_t)(lane + 1);
         const bool valid = q < out;
-        const bool outside = valid && !(hdb<W>(load_key<W>(Z, X, q), kp) < b);
+        const bool outside = valid && !(agrees_above<W>(Z, X, q, kp, b));
         const uint64_t om = __ballot(outside);
         if (om == 0) {
             int64_t kmax = (out - in - 1) / step;
