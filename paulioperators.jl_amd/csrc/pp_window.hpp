@@ -367,7 +367,7 @@ __global__ void __launch_bounds__(kBlock) kw_merge(const uint64_t* __restrict__ 
         const int64_t d0 = t * TM;
         const int64_t d1 = d0 + TM < tot ? d0 + TM : tot;
         if (wave < 2) {
-            const int64_t sp = merge_split_wave<W>(Lz, Lx, n, Cz, Cx, m, wave == 0 ? d0 : d1);
+            const int64_t sp = merge_split_wave<W, (TM >= 1024)>(Lz, Lx, n, Cz, Cx, m, wave == 0 ? d0 : d1);
             if (lane == 0) s_split[wave] = sp;
         }
         __syncthreads();
