@@ -766,6 +766,11 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
     // levels); otherwise from kb_flags' summaries (at most kAggLevels levels).
     // Two instantiations keep each variant's register use low.
     constexpr bool summ = !TABLE;
+    // the head-mask level loop serves every mask of at most kAggLevels levels
+    // (with the table's counts in the table variant); more levels, and the
+    // table variant's 2048-term tiles (whose registers the loop would push
+    // to two waves per SIMD), use scans
+    const bool maskp = summ || (kTile <= 512 && nlev <= kAggLevels);
     if (!TABLE && nlev > kAggLevels) {  // the host never launches this: report, do not write
         if (lb == 0 && threadIdx.x == 0) atomicOr(&s->err, 8);
         return;
@@ -804,7 +809,7 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
                 for (int w = 0; w < W; ++w) K[r].z[w] = K[r].x[w] = 0;
             }
         }
-        if (summ) {
+        if (maskp) {
             // head masks per level (a tile position 0 is a head of the tile's view)
 #pragma unroll
             for (int r = 0; r < R; ++r) {
@@ -870,6 +875,11 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
                 if (lane == 0) s_aloc[Teff] = (int16_t)c;
             }
         }
+        if (TABLE && maskp && threadIdx.x >= kWave - 4 * kAggLevels && threadIdx.x < kWave) {
+            // the table's cross-edge counts for the level loop
+            const int q = threadIdx.x - (kWave - 4 * kAggLevels), li = q >> 2;
+            if (li < nlev) s_bq[li][q & 3] = ldg(bnd, (t * maxlev + li) * 4 + (q & 3));
+        }
         if (threadIdx.x == 0 && !(summ && self_off)) {
             s_aloc[Teff] = (int16_t)(self_off ? ldg(tilecnt, t) : ldg(tileoff, t + 1) - toff_scan);
             if (!summ)
@@ -878,7 +888,7 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
                                                       : kHdbNone);
         }
         __syncthreads();
-        if (summ) {
+        if (maskp) {
             // the level loop on the head masks: no scans, no barriers
             for (int li = 0; li < nlev; ++li) {
                 const int lvl = (int)((lev4 >> (8 * li)) & 255u);
@@ -919,7 +929,7 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
                 }
             }
         }
-        for (int li = 0; li < (summ ? 0 : nlev); ++li) {
+        for (int li = 0; li < (maskp ? 0 : nlev); ++li) {
             const int lvl = li < 4 ? (int)((lev4 >> (8 * li)) & 255u) : (int)st->g.lev[li];
             int64_t bq[4];
 #pragma unroll
