@@ -203,6 +203,7 @@ void set_last_error(const std::string& msg) { g_last_error = msg; }
 // ---------------------------------------------------------------------------
 struct Batch {
     int W = 1, B = 0, ncu = 256;
+    int sweep_fold_above = 32;  // sweep workgroups take several tiles past this many tiles per CU
     hipStream_t stream = nullptr;
     std::vector<int> N;
     std::vector<InstBuf> ib;
@@ -476,15 +477,24 @@ struct Batch {
         const int T2 = mtile;
         const int kTile = tile;
         int64_t acc[4] = {0, 0, 0, 0};
+        // expected live count during the coming sequence: blocks beyond the
+        // real tiles only exit (sweeps) or contend for the ticket (merge)
+        auto expected = [&](int b) {
+            const int64_t cap = std::min(ib[b].L[0].cap, ib[b].L[1].cap);
+            return std::min<int64_t>(cap, std::max<int64_t>(n_known[b] + n_known[b] / 2, 1024));
+        };
+        // with many sweep tiles per CU, each sweep workgroup takes several
+        // tiles in turn (4 of 512 terms, 2 of 2048): its prologue's dependent
+        // loads are paid once for them
+        int64_t all_tiles = 0;
+        for (int b = 0; b < B; ++b) all_tiles += cdiv(expected(b), kTile);
+        const int64_t fold = all_tiles > (int64_t)sweep_fold_above * ncu ? (kTile <= 512 ? 4 : 2) : 1;
         for (int b = 0; b <= B; ++b) {
             for (int k = 0; k < 4; ++k) h_maps[(size_t)k * (B + 1) + b] = acc[k];
             if (b == B) break;
-            const int64_t cap = std::min(ib[b].L[0].cap, ib[b].L[1].cap);
-            // expected live count during the coming sequence: blocks beyond the
-            // real tiles only exit (sweeps) or contend for the ticket (merge)
-            const int64_t nest = std::min<int64_t>(cap, std::max<int64_t>(n_known[b] + n_known[b] / 2, 1024));
+            const int64_t nest = expected(b);
             const int64_t tiles = cdiv(nest, kTile);
-            acc[0] += std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)ncu * 8));
+            acc[0] += std::max<int64_t>(1, std::min<int64_t>(cdiv(tiles, fold), (int64_t)ncu * 8));
             acc[1] += std::max<int64_t>(1, std::min<int64_t>(cdiv(2 * tiles, kBlock / kWave), (int64_t)ncu * 4));
             acc[2] += 1;
             acc[3] += std::max<int64_t>(1, std::min<int64_t>(cdiv(nest + nest / 2, T2), (int64_t)ncu * 4));

@@ -453,6 +453,19 @@ def test_many_tiles_contiguous_block_layout():
     _batch_vs_oracle(N, init, gens, angs, CoeffTruncation(1e-7), tiles=(512, 512))
 
 
+def test_folded_sweep_workgroups():
+    """A call with more than 32 sweep tiles per CU gives each flags/emit
+    workgroup several tiles in turn (fewer workgroups than tiles by a fixed
+    factor): twenty vectors of 1.8e5 terms against the oracle."""
+    rng = random.Random(408)
+    N = 40
+    sups = [spread_support(N, 12, rng) for _ in range(20)]
+    init = [random_terms(N, 180000, rng, sup) for sup in sups]
+    gens, angs = _two_qubit_circuits(N, sups, 3, rng)
+    log("folded sweep call: 20 x 180000 terms")
+    _batch_vs_oracle(N, init, gens, angs, CoeffTruncation(1e-7), tiles=(512, 512))
+
+
 def test_batch_graph_replays_match_eager():
     """hipGraph-captured sequences: the same layer circuit replayed over
     several calls, with eager channel work and host synchronisation between
