@@ -489,6 +489,11 @@ struct Batch {
         int64_t all_tiles = 0;
         for (int b = 0; b < B; ++b) all_tiles += cdiv(expected(b), kTile);
         const int64_t fold = all_tiles > (int64_t)sweep_fold_above * ncu ? (kTile <= 512 ? 4 : 2) : 1;
+        // the same for the merge grid: workgroups then take merge tiles by
+        // ticket, 4 per workgroup on average
+        int64_t all_mtiles = 0;
+        for (int b = 0; b < B; ++b) all_mtiles += cdiv(expected(b) + expected(b) / 2, T2);
+        const int64_t mfold = all_mtiles > (int64_t)sweep_fold_above * ncu ? 4 : 1;
         for (int b = 0; b <= B; ++b) {
             for (int k = 0; k < 4; ++k) h_maps[(size_t)k * (B + 1) + b] = acc[k];
             if (b == B) break;
@@ -497,7 +502,7 @@ struct Batch {
             acc[0] += std::max<int64_t>(1, std::min<int64_t>(cdiv(tiles, fold), (int64_t)ncu * 8));
             acc[1] += std::max<int64_t>(1, std::min<int64_t>(cdiv(2 * tiles, kBlock / kWave), (int64_t)ncu * 4));
             acc[2] += 1;
-            acc[3] += std::max<int64_t>(1, std::min<int64_t>(cdiv(nest + nest / 2, T2), (int64_t)ncu * 4));
+            acc[3] += std::max<int64_t>(1, std::min<int64_t>(cdiv(nest + nest / 2, T2 * mfold), (int64_t)ncu * 4));
         }
         // XCD placement: vectors by decreasing size, each to the XCD row with
         // the least tile blocks so far; a map's grid is kXcds times its
