@@ -529,8 +529,9 @@ __device__ __forceinline__ bool key_le_zfirst(const uint64_t* __restrict__ Az, c
 // Merge-path split: the number of live terms among the first d outputs of
 // the merge of L (n) and C (m), live first on equal keys.  Wave-cooperative
 // 64-ary search over the monotone predicate L[i] <= C[d-1-i].  ZFIRST: probes
-// read the x words only on z ties (less traffic where the merge streams; one
-// more dependent load where keys tie, so latency-bound callers keep false).
+// read the x words only on z ties (half the scattered lines where z words
+// differ, one more dependent load where they tie; measured faster for every
+// merge tile size of kb_merge).
 template <int W, bool ZFIRST = false>
 __device__ int64_t merge_split_wave(const uint64_t* __restrict__ Lz, const uint64_t* __restrict__ Lx, int64_t n,
                                     const uint64_t* __restrict__ Cz, const uint64_t* __restrict__ Cx, int64_t m,

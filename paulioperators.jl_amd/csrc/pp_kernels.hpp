@@ -1131,7 +1131,7 @@ __global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D
         const int64_t d1 = d0 + TM < tot ? d0 + TM : tot;
         // merge-path splits of this tile's window: wave 0 the start, wave 1 the end
         if (wave < 2) {
-            const int64_t sp = merge_split_wave<W, (TM >= 1024)>(Lz, Lx, n, Cz, Cx, m, wave == 0 ? d0 : d1);
+            const int64_t sp = merge_split_wave<W, true>(Lz, Lx, n, Cz, Cx, m, wave == 0 ? d0 : d1);  // z words first
             if (lane == 0) s_split[wave] = sp;
         }
         __syncthreads();
