@@ -634,28 +634,28 @@ struct Batch {
 
     template <int Wt, int T, int TM>
     void launch_step(const StepDev* dsteps, int64_t step_index, uint32_t ep_off, bool bounds, bool scan) {
-        hipLaunchKernelGGL((kb_flags<Wt, T>), dim3(blocks(0)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps, map(0),
+        hipLaunchKernelGGL((kb_flags<Wt, T>), dim3(blocks(0)), dim3(kBlock), 0, stream, map(0), d_desc, d_state, dsteps,
                            bounds ? 1 : 0);
         if (scan) hipLaunchKernelGGL(kb_scan<T>, dim3(B), dim3(1024), 0, stream, d_desc, d_state, dsteps, step_index);
         if (bounds)
-            hipLaunchKernelGGL((kb_bounds<Wt, T>), dim3(blocks(1)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps,
-                               map(1));
+            hipLaunchKernelGGL((kb_bounds<Wt, T>), dim3(blocks(1)), dim3(kBlock), 0, stream, map(1), d_desc, d_state,
+                               dsteps);
         if (bounds)
-            hipLaunchKernelGGL((kb_emit<Wt, T, true>), dim3(blocks(0)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps,
-                               map(0), 0, step_index);
+            hipLaunchKernelGGL((kb_emit<Wt, T, true>), dim3(blocks(0)), dim3(kBlock), 0, stream, map(0), d_desc, d_state,
+                               dsteps, 0, step_index);
         else
-            hipLaunchKernelGGL((kb_emit<Wt, T, false>), dim3(blocks(0)), dim3(kBlock), 0, stream, d_desc, d_state,
-                               dsteps, map(0), scan ? 0 : 1, step_index);
+            hipLaunchKernelGGL((kb_emit<Wt, T, false>), dim3(blocks(0)), dim3(kBlock), 0, stream, map(0), d_desc,
+                               d_state, dsteps, scan ? 0 : 1, step_index);
         if (timing) {
             // events recorded by the launch itself at the kernel's start and
             // end (not around it), so the interval is the kernel's duration
             hipEvent_t e0 = next_event();
             hipEvent_t e1 = next_event();
-            hipExtLaunchKernelGGL((kb_merge<Wt, TM>), dim3(blocks(3)), dim3(kBlock), 0, stream, e0, e1, 0, d_desc,
-                                  d_state, dsteps, map(3), 0, step_index, (const uint32_t*)d_epoch, ep_off);
+            hipExtLaunchKernelGGL((kb_merge<Wt, TM>), dim3(blocks(3)), dim3(kBlock), 0, stream, e0, e1, 0, map(3),
+                                  d_desc, d_state, dsteps, 0, step_index, (const uint32_t*)d_epoch, ep_off);
         } else {
-            hipLaunchKernelGGL((kb_merge<Wt, TM>), dim3(blocks(3)), dim3(kBlock), 0, stream, d_desc, d_state, dsteps,
-                               map(3), 0, step_index, (const uint32_t*)d_epoch, ep_off);
+            hipLaunchKernelGGL((kb_merge<Wt, TM>), dim3(blocks(3)), dim3(kBlock), 0, stream, map(3), d_desc, d_state,
+                               dsteps, 0, step_index, (const uint32_t*)d_epoch, ep_off);
         }
     }
 
@@ -663,7 +663,7 @@ struct Batch {
     // a valid pointer: the caller has sized it to at least B entries).
     template <int Wt, int TM>
     void launch_compact(uint32_t ep_off) {
-        hipLaunchKernelGGL((kb_merge<Wt, TM>), dim3(blocks(3)), dim3(kBlock), 0, stream, d_desc, d_state, d_steps, map(3),
+        hipLaunchKernelGGL((kb_merge<Wt, TM>), dim3(blocks(3)), dim3(kBlock), 0, stream, map(3), d_desc, d_state, d_steps,
                            1, (int64_t)-1, (const uint32_t*)d_epoch, ep_off);
     }
 

@@ -366,8 +366,8 @@ __device__ void edge_counts(int side, int li, int lvl, int64_t t, int64_t ntiles
 
 // ---------------------------------------------------------------------------
 template <int W, int T>
-__global__ void __launch_bounds__(kBlock) kb_flags(const SegDesc* __restrict__ D, SegState* __restrict__ S,
-                                                   const StepDev* __restrict__ steps, BlockMap bm, int bnd_ok) {
+__global__ void __launch_bounds__(kBlock) kb_flags(BlockMap bm, const SegDesc* __restrict__ D, SegState* __restrict__ S,
+                                                   const StepDev* __restrict__ steps, int bnd_ok) {
     constexpr int kTile = T;
     constexpr int kTileWords = T / 64;
     __shared__ int s_wcnt[kTileWords];
@@ -584,8 +584,8 @@ __global__ void __launch_bounds__(1024) kb_scan(const SegDesc* __restrict__ D, S
 //   q=3 post_sib : A in the segment just after segend_b(te) when it is the
 //                  bit-1 sibling and K_te has bit b = 0
 template <int W, int T>
-__global__ void __launch_bounds__(kBlock) kb_bounds(const SegDesc* __restrict__ D, const SegState* __restrict__ S,
-                                                    const StepDev* __restrict__ steps, BlockMap bm) {
+__global__ void __launch_bounds__(kBlock) kb_bounds(BlockMap bm, const SegDesc* __restrict__ D,
+                                                    const SegState* __restrict__ S, const StepDev* __restrict__ steps) {
     constexpr int kTile = T;
     int b;
     int64_t lb, nb;
@@ -692,8 +692,8 @@ __device__ __forceinline__ int last_head(const uint64_t* hm, uint64_t words, int
 // Layout inside a tile: wave w owns elements [w T/4, (w+1) T/4), round r
 // covers 64 consecutive elements (coalesced loads), lane = element % 64.
 template <int W, int T, bool TABLE>
-__global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D, SegState* __restrict__ S,
-                                                  const StepDev* __restrict__ steps, BlockMap bm, int self_off,
+__global__ void __launch_bounds__(kBlock) kb_emit(BlockMap bm, const SegDesc* __restrict__ D, SegState* __restrict__ S,
+                                                  const StepDev* __restrict__ steps, int self_off,
                                                   int64_t step_index) {
     constexpr int kTile = T;
     constexpr int R = kTile / kBlock;  // rounds of 64 consecutive terms per wave
@@ -1046,8 +1046,8 @@ __global__ void __launch_bounds__(kBlock) kb_emit(const SegDesc* __restrict__ D,
 // workgroups that have started: with a workgroup per tile, tile = local
 // workgroup index; with fewer workgroups, tiles are claimed by ticket.
 template <int W, int TM>
-__global__ void __launch_bounds__(kBlock) kb_merge(const SegDesc* __restrict__ D, SegState* __restrict__ S,
-                                                   const StepDev* __restrict__ steps, BlockMap bm, int mode,
+__global__ void __launch_bounds__(kBlock) kb_merge(BlockMap bm, const SegDesc* __restrict__ D, SegState* __restrict__ S,
+                                                   const StepDev* __restrict__ steps, int mode,
                                                    int64_t step_index, const uint32_t* __restrict__ d_epoch,
                                                    uint32_t ep_off) {
     constexpr int E = TM / kBlock;
