@@ -1250,6 +1250,21 @@ __global__ void __launch_bounds__(kBlock) kb_merge(BlockMap bm, const SegDesc* _
             total += s_wsum[w];
         }
         const int myoff = wpref + incl - cnt;
+        // kept outputs staged in LDS at their tile offsets (every LDS read of
+        // the merge ended at the barrier above); waves 1-3 stage while wave 0
+        // runs the look-back, wave 0 after it
+        auto stage = [&]() {
+            int k = myoff;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                if (outE[e]) {
+                    s_key[k] = outK[e];
+                    s_c[k] = outC[e];
+                    ++k;
+                }
+            }
+        };
+        if (wave != 0) stage();
         // Decoupled look-back for the tile's base, one wave scanning 64
         // predecessors per step.  Each status word carries its own payload
         // (flag | count), so relaxed atomics suffice: no other memory is
@@ -1327,20 +1342,9 @@ __global__ void __launch_bounds__(kBlock) kb_merge(BlockMap bm, const SegDesc* _
                 }
             }
         }
+        if (wave == 0) stage();
         __syncthreads();
         const int64_t base = s_base;
-        {
-            int k = myoff;
-#pragma unroll
-            for (int e = 0; e < E; ++e) {
-                if (outE[e]) {
-                    s_key[k] = outK[e];
-                    s_c[k] = outC[e];
-                    ++k;
-                }
-            }
-        }
-        __syncthreads();
         if (base + total <= capOut) {
             for (int k = tid; k < total; k += kBlock) {
                 store_key<W>(Oz, Ox, base + k, s_key[k]);
