@@ -161,7 +161,7 @@ struct InstBuf {
     uint64_t* status = nullptr;
     int64_t cap_tiles = 0;   // sweep-tile capacity of flags/wpre/tilecnt/tileoff
     int64_t cap_bnd = 0;     // entries of bnd
-    int64_t cap_split = 0;   // entries of status (merge tiles + 1)
+    int64_t cap_status = 0;   // entries of status (merge tiles + 1)
     int maxlev = 1;
 };
 
@@ -436,9 +436,9 @@ struct Batch {
         }
         I.maxlev = ml;
         const int64_t sp = cdiv(I.L[0].cap + I.C.cap, 512) + 2;
-        if (I.cap_split < sp) {
+        if (I.cap_status < sp) {
             dfree(I.status);
-            I.cap_split = sp;
+            I.cap_status = sp;
             I.status = dalloc<uint64_t>(sp);
             // look-back words must start "not ready": freed memory may hold
             // words of another handle whose tags collide with this one's
@@ -605,7 +605,7 @@ struct Batch {
     void reserve_epochs(uint64_t count) {
         if (epoch_used + count + 1 > (uint64_t)kEpochMask) {
             for (auto& I : ib)
-                if (I.status) HIPCHK(hipMemsetAsync(I.status, 0, (size_t)I.cap_split * sizeof(uint64_t), stream));
+                if (I.status) HIPCHK(hipMemsetAsync(I.status, 0, (size_t)I.cap_status * sizeof(uint64_t), stream));
             static const uint32_t one = 1;
             HIPCHK(hipMemcpyAsync(d_epoch, &one, sizeof(uint32_t), hipMemcpyHostToDevice, stream));
             HIPCHK(hipStreamSynchronize(stream));
