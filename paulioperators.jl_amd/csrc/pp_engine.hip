@@ -128,16 +128,28 @@ bool filter_active(const pp_filter* f) {
 }
 
 // Device allocations freed when the owner leaves scope (also on errors).
-struct DevTemps {
-    std::vector<void*> p;
+// Grow-only device buffers in numbered slots, kept between calls (the
+// products reuse them instead of allocating and freeing per call).
+struct DevArena {
+    static constexpr int kSlots = 14;
+    void* p[kSlots] = {};
+    size_t bytes[kSlots] = {};
     template <typename T>
-    T* get(size_t n) {
-        T* q = dalloc<T>(n);
-        p.push_back(q);
-        return q;
+    T* get(int slot, size_t n) {
+        const size_t need = (n ? n : 1) * sizeof(T);
+        if (bytes[slot] < need) {
+            dfree(p[slot]);
+            p[slot] = nullptr;
+            bytes[slot] = 0;
+            const size_t grow = need + need / 4;
+            p[slot] = dalloc<char>(grow);
+            bytes[slot] = grow;
+        }
+        return static_cast<T*>(p[slot]);
     }
-    ~DevTemps() {
-        for (void* q : p) dfree(q);
+    void release() {
+        for (int i = 0; i < kSlots; ++i) dfree(p[i]);
+        *this = DevArena();
     }
 };
 
@@ -190,6 +202,7 @@ struct WinBuf {
     unsigned long long* orpart = nullptr; // [kOrandBlocks][4 W]: per-block partials
     unsigned long long* qhist = nullptr;  // [256]
     double* out = nullptr;
+    DevArena alg;                      // operands, pair coefficients and results of pp_algebra
     uint32_t epoch = 0;
     uint64_t rng_calls = 0;
 };
@@ -913,6 +926,7 @@ struct Batch {
         dfree(w.bcount); dfree(w.boff); dfree(w.bsum); dfree(w.hist); dfree(w.hoff); dfree(w.status);
         dfree(w.partial); dfree(w.gens); dfree(w.filt); dfree(w.small); dfree(w.err); dfree(w.ticket);
         dfree(w.orand); dfree(w.orpart); dfree(w.qhist); dfree(w.out);
+        w.alg.release();
         w = WinBuf();
     }
 
@@ -1670,18 +1684,18 @@ struct Batch {
     template <int Wt>
     int64_t algebra(int op, int cplx, const pp_terms& a, const pp_terms& b, uint64_t* oz, uint64_t* ox, double* ore,
                     double* oim, int64_t cap) {
-        DevTemps tmp;
         win_init();
         WinBuf& w = win;
         const int64_t na = a.n, nb = b.n;
         if (na == 0 || nb == 0) return 0;
         if (na > (int64_t)1 << 31 || nb > (int64_t)1 << 31 || na * nb > ((int64_t)1 << 40))
             throw PPError(PP_EARG, "too many pairs");
-        auto upload = [&](const pp_terms& t, AlgOperand& d) {
-            uint64_t* z = tmp.get<uint64_t>((size_t)t.n * Wt);
-            uint64_t* x = tmp.get<uint64_t>((size_t)t.n * Wt);
-            double* re = tmp.get<double>((size_t)t.n);
-            double* im = cplx ? tmp.get<double>((size_t)t.n) : nullptr;
+        DevArena& tmp = w.alg;
+        auto upload = [&](const pp_terms& t, AlgOperand& d, int slot) {
+            uint64_t* z = tmp.get<uint64_t>(slot, (size_t)t.n * Wt);
+            uint64_t* x = tmp.get<uint64_t>(slot + 1, (size_t)t.n * Wt);
+            double* re = tmp.get<double>(slot + 2, (size_t)t.n);
+            double* im = cplx ? tmp.get<double>(slot + 3, (size_t)t.n) : nullptr;
             HIPCHK(hipMemcpyAsync(z, t.z, (size_t)t.n * Wt * 8, hipMemcpyHostToDevice, stream));
             HIPCHK(hipMemcpyAsync(x, t.x, (size_t)t.n * Wt * 8, hipMemcpyHostToDevice, stream));
             HIPCHK(hipMemcpyAsync(re, t.re, (size_t)t.n * 8, hipMemcpyHostToDevice, stream));
@@ -1689,8 +1703,8 @@ struct Batch {
             d = AlgOperand{z, x, re, im, t.n, t.conj};
         };
         AlgOperand A, Bo;
-        upload(a, A);
-        upload(b, Bo);
+        upload(a, A, 0);
+        upload(b, Bo, 4);
         if (op != kAlgMul) A.conj = Bo.conj = 0;
         HIPCHK(hipMemsetAsync(w.err, 0, sizeof(int32_t), stream));
         // pairs: count, scan, write
@@ -1704,8 +1718,8 @@ struct Batch {
         const int64_t m = read_small(0);
         if (m == 0) return 0;
         grow_set(w.A, m, 0);
-        double* pre = tmp.get<double>((size_t)m);
-        double* pim = cplx ? tmp.get<double>((size_t)m) : nullptr;
+        double* pre = tmp.get<double>(8, (size_t)m);
+        double* pim = cplx ? tmp.get<double>(9, (size_t)m) : nullptr;
         hipLaunchKernelGGL(ka_pairs<Wt>, dim3((unsigned)nblk), dim3(kBlock), 0, stream, A, Bo, op, cplx, 0, w.bcount,
                            (const int64_t*)w.boff, w.A.z, w.A.x, w.A.c, pre, pim, w.err);
         HIPCHK(hipGetLastError());
@@ -1724,10 +1738,10 @@ struct Batch {
         scan_i64(w.bcount, mblk, w.boff, w.small);
         const int64_t nout = read_small(0);
         if (nout > cap) return nout;
-        uint64_t* dz = tmp.get<uint64_t>((size_t)nout * Wt);
-        uint64_t* dx = tmp.get<uint64_t>((size_t)nout * Wt);
-        double* dre = tmp.get<double>((size_t)nout);
-        double* dim = cplx ? tmp.get<double>((size_t)nout) : nullptr;
+        uint64_t* dz = tmp.get<uint64_t>(10, (size_t)nout * Wt);
+        uint64_t* dx = tmp.get<uint64_t>(11, (size_t)nout * Wt);
+        double* dre = tmp.get<double>(12, (size_t)nout);
+        double* dim = cplx ? tmp.get<double>(13, (size_t)nout) : nullptr;
         hipLaunchKernelGGL(ka_runs<Wt>, dim3((unsigned)mblk), dim3(kBlock), 0, stream, S->z, S->x, S->c, m,
                            (const double*)pre, (const double*)pim, cplx, 0, w.bcount, (const int64_t*)w.boff, dz, dx,
                            dre, dim);
