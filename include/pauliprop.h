@@ -111,7 +111,9 @@ int pp_capacities(pp_handle* h, int64_t* live_cap, int64_t* append_cap);
  * commutator fails ("commutator coefficients are imaginary").  Output: keys
  * strictly increasing, equal keys summed (zero sums kept, as
  * _triples_to_live!), nout terms; cap < nout returns PP_EOVERFLOW with nout
- * set and nothing written (cap = 0 asks for the size). */
+ * set and nothing written (cap = 0 asks for the size).  The device buffers of
+ * a call (operands, one coefficient per pair, the result) are kept on h and
+ * grow as needed; pp_destroy releases them. */
 enum { PP_ALG_MUL = 0, PP_ALG_COMMUTATOR = 1, PP_ALG_ANTICOMMUTATOR = 2 };
 typedef struct {
     const uint64_t* z;    /* n x W words */
