@@ -172,3 +172,16 @@ def test_commuting_operands_and_full_width_keys():
     C = (z, x, rng.standard_normal(n) + 1j * rng.standard_normal(n))
     for op in (ao.MUL, ao.COMMUTATOR, ao.ANTICOMMUTATOR):
         same_bits(product(N, A, C, op), ao.product(A, C, op))
+
+
+def test_buffers_reused_across_calls():
+    """The device buffers of a product stay on the handle between calls:
+    a large call, a smaller one, then a larger one than the first (each
+    buffer regrows) all equal the oracle."""
+    N = 90
+    rng = np.random.default_rng(31)
+    for na, nb, sb in ((900, 800, 7), (40, 30, 4), (1400, 1300, 8), (60, 50, 5)):
+        A = random_sum(N, na, rng, support_bits=sb)
+        B = random_sum(N, nb, rng, support_bits=sb)
+        log(f"pp_algebra reuse {na}x{nb}")
+        same_bits(product(N, A, B, ao.COMMUTATOR), ao.product(A, B, ao.COMMUTATOR))
