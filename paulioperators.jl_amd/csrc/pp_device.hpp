@@ -60,6 +60,27 @@ __device__ __forceinline__ Key<W> load_key(const uint64_t* __restrict__ Z, const
     return k;
 }
 
+// Streaming stores (nontemporal hint: the line is not kept in the caches),
+// for outputs far larger than the caches that the next kernel reads once.
+template <typename T>
+__device__ __forceinline__ void stg_nt(T* p, int64_t i, T v) {
+    __builtin_nontemporal_store(v, (PP_GLOBAL T*)p + i);
+}
+template <int W>
+__device__ __forceinline__ void store_key_nt(uint64_t* __restrict__ Z, uint64_t* __restrict__ X, int64_t i,
+                                             const Key<W>& k) {
+    if constexpr (W == 1) {
+        stg_nt(Z, i, k.z[0]);
+        stg_nt(X, i, k.x[0]);
+    } else {
+        pp_u64x2 a, b;
+        a.x = k.z[0]; a.y = k.z[1];
+        b.x = k.x[0]; b.y = k.x[1];
+        __builtin_nontemporal_store(a, (PP_GLOBAL pp_u64x2*)Z + i);
+        __builtin_nontemporal_store(b, (PP_GLOBAL pp_u64x2*)X + i);
+    }
+}
+
 template <int W>
 __device__ __forceinline__ void store_key(uint64_t* __restrict__ Z, uint64_t* __restrict__ X, int64_t i, const Key<W>& k) {
     if constexpr (W == 1) {

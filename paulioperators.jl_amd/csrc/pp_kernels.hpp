@@ -1347,8 +1347,13 @@ __global__ void __launch_bounds__(kBlock) kb_merge(BlockMap bm, const SegDesc* _
         const int64_t base = s_base;
         if (base + total <= capOut) {
             for (int k = tid; k < total; k += kBlock) {
-                store_key<W>(Oz, Ox, base + k, s_key[k]);
-                stg(Oc, base + k, s_c[k]);
+                if constexpr (TM >= 2048) {  // large vectors: stream past the caches
+                    store_key_nt<W>(Oz, Ox, base + k, s_key[k]);
+                    stg_nt(Oc, base + k, s_c[k]);
+                } else {
+                    store_key<W>(Oz, Ox, base + k, s_key[k]);
+                    stg(Oc, base + k, s_c[k]);
+                }
             }
         }
         __syncthreads();
