@@ -60,11 +60,30 @@ __device__ __forceinline__ Key<W> load_key(const uint64_t* __restrict__ Z, const
     return k;
 }
 
-// Streaming stores (nontemporal hint: the line is not kept in the caches),
-// for outputs far larger than the caches that the next kernel reads once.
+// Streaming loads and stores (nontemporal hint: the line is not kept in the
+// caches), for inputs read once and outputs far larger than the caches.
 template <typename T>
 __device__ __forceinline__ void stg_nt(T* p, int64_t i, T v) {
     __builtin_nontemporal_store(v, (PP_GLOBAL T*)p + i);
+}
+template <typename T>
+__device__ __forceinline__ T ldg_nt(const T* p, int64_t i) {
+    return __builtin_nontemporal_load((const PP_GLOBAL T*)p + i);
+}
+template <int W>
+__device__ __forceinline__ Key<W> load_key_nt(const uint64_t* __restrict__ Z, const uint64_t* __restrict__ X,
+                                              int64_t i) {
+    Key<W> k;
+    if constexpr (W == 1) {
+        k.z[0] = ldg_nt(Z, i);
+        k.x[0] = ldg_nt(X, i);
+    } else {
+        const pp_u64x2 a = __builtin_nontemporal_load((const PP_GLOBAL pp_u64x2*)Z + i);
+        const pp_u64x2 b = __builtin_nontemporal_load((const PP_GLOBAL pp_u64x2*)X + i);
+        k.z[0] = a.x; k.z[1] = a.y;
+        k.x[0] = b.x; k.x[1] = b.y;
+    }
+    return k;
 }
 template <int W>
 __device__ __forceinline__ void store_key_nt(uint64_t* __restrict__ Z, uint64_t* __restrict__ X, int64_t i,

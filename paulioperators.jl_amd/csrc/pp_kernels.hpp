@@ -1153,12 +1153,22 @@ __global__ void __launch_bounds__(kBlock) kb_merge(BlockMap bm, const SegDesc* _
 #pragma unroll
                 for (int w = 0; w < W; ++w) lk[e].z[w] = lk[e].x[w] = 0;
                 lcv[e] = 0.0;
-                if (k < nL) {
-                    lk[e] = load_key<W>(Lz, Lx, i0 + k);
-                    lcv[e] = ldg(Lc, i0 + k);
-                } else if (k < nL + nC) {
-                    lk[e] = load_key<W>(Cz, Cx, j0 + (k - nL));
-                    lcv[e] = ldg(Cc, j0 + (k - nL));
+                if constexpr (TM >= 2048) {  // large vectors: read once, past the caches
+                    if (k < nL) {
+                        lk[e] = load_key_nt<W>(Lz, Lx, i0 + k);
+                        lcv[e] = ldg_nt(Lc, i0 + k);
+                    } else if (k < nL + nC) {
+                        lk[e] = load_key_nt<W>(Cz, Cx, j0 + (k - nL));
+                        lcv[e] = ldg_nt(Cc, j0 + (k - nL));
+                    }
+                } else {
+                    if (k < nL) {
+                        lk[e] = load_key<W>(Lz, Lx, i0 + k);
+                        lcv[e] = ldg(Lc, i0 + k);
+                    } else if (k < nL + nC) {
+                        lk[e] = load_key<W>(Cz, Cx, j0 + (k - nL));
+                        lcv[e] = ldg(Cc, j0 + (k - nL));
+                    }
                 }
             }
 #pragma unroll
