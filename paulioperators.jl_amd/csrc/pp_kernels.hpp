@@ -431,34 +431,56 @@ __global__ void __launch_bounds__(kBlock) kb_flags(BlockMap bm, const SegDesc* _
         need_x |= G.gz[w] != 0;
     }
     if (local_active || summ) need_z = need_x = true;
+    constexpr int RW = kTileWords / 4;  // rounds of 64 terms per wave
     for (int64_t t = lb; t < ntiles; t += nb) {
+        // every load of the tile first (one memory round trip): the wave's
+        // keys, their coefficients when the local filter runs, and the key
+        // before the wave's first term (the later rounds take theirs from
+        // lane 63 of the round before)
+        Key<W> kk[RW];
+        double cc[RW];
 #pragma unroll
-        for (int r = 0; r < kTileWords / 4; ++r) {
-            const int wi = wave * (kTileWords / 4) + r;
-            const int64_t i = t * kTile + wi * 64 + lane;
-            const bool valid = i < n;
-            bool flag = false;
-            Key<W> k;
+        for (int r = 0; r < RW; ++r) {
+            const int64_t i = t * kTile + (wave * RW + r) * 64 + lane;
 #pragma unroll
-            for (int w = 0; w < W; ++w) k.z[w] = k.x[w] = 0;
-            if (valid) {
+            for (int w = 0; w < W; ++w) kk[r].z[w] = kk[r].x[w] = 0;
+            cc[r] = 0.0;
+            if (i < n) {
                 if constexpr (W == 1) {
-                    k.z[0] = need_z ? ldg(Lz, i) : 0;
-                    k.x[0] = need_x ? ldg(Lx, i) : 0;
+                    kk[r].z[0] = need_z ? ldg(Lz, i) : 0;
+                    kk[r].x[0] = need_x ? ldg(Lx, i) : 0;
                 } else {
                     if (need_z) {
                         const ulonglong2 a = ldg(reinterpret_cast<const ulonglong2*>(Lz), i);
-                        k.z[0] = a.x; k.z[1] = a.y;
+                        kk[r].z[0] = a.x; kk[r].z[1] = a.y;
                     }
                     if (need_x) {
                         const ulonglong2 c2 = ldg(reinterpret_cast<const ulonglong2*>(Lx), i);
-                        k.x[0] = c2.x; k.x[1] = c2.y;
+                        kk[r].x[0] = c2.x; kk[r].x[1] = c2.y;
                     }
                 }
+                if (local_active) cc[r] = ldg(Lc, i);
+            }
+        }
+        Key<W> kp0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) kp0.z[w] = kp0.x[w] = 0;
+        {
+            const int64_t i0 = t * kTile + wave * RW * 64;
+            if (summ && lane == 0 && i0 > 0 && i0 < n) kp0 = load_key<W>(Lz, Lx, i0 - 1);
+        }
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+            const int wi = wave * RW + r;
+            const int64_t i = t * kTile + wi * 64 + lane;
+            const bool valid = i < n;
+            bool flag = false;
+            const Key<W> k = kk[r];
+            if (valid) {
                 flag = G.anti(k);
                 if (flag && local_active) {
                     Key<W> ch;
-                    const double cn = G.child(k, ldg(Lc, i), ch);
+                    const double cn = G.child(k, cc[r], ch);
                     flag = !should_drop<W>(floc, ch, fabs(cn));
                 }
             }
@@ -469,14 +491,27 @@ __global__ void __launch_bounds__(kBlock) kb_flags(BlockMap bm, const SegDesc* _
                 s_am[wi] = mask;
             }
             if (summ) {
-                // hdb against the previous key: from the lane below, lane 0 loads it
+                // hdb against the previous key: from the lane below; lane 0
+                // from lane 63 of the round before, or the key loaded above
                 Key<W> kp;
 #pragma unroll
                 for (int w = 0; w < W; ++w) {
                     kp.z[w] = __shfl_up(k.z[w], 1, kWave);
                     kp.x[w] = __shfl_up(k.x[w], 1, kWave);
                 }
-                if (lane == 0 && valid && i > 0) kp = load_key<W>(Lz, Lx, i - 1);
+                if (r == 0) {
+                    if (lane == 0) kp = kp0;
+                } else {
+#pragma unroll
+                    for (int w = 0; w < W; ++w) {
+                        const uint64_t pz = __shfl(kk[r - 1].z[w], kWave - 1, kWave);
+                        const uint64_t px = __shfl(kk[r - 1].x[w], kWave - 1, kWave);
+                        if (lane == 0) {
+                            kp.z[w] = pz;
+                            kp.x[w] = px;
+                        }
+                    }
+                }
                 const int hd = i == 0 ? (int)kHdbNone : hdb<W>(kp, k);
 #pragma unroll
                 for (int li = 0; li < kAggLevels; ++li) {
