@@ -822,26 +822,100 @@ __global__ void __launch_bounds__(kBlock) kb_emit(BlockMap bm, const SegDesc* __
         int64_t rank[R];
         bool fl[R];
         const int64_t toff_scan = self_off ? 0 : ldg(tileoff, t);
+        // Large tiles of the table variant (long vectors, bandwidth-bound):
+        // every load of the tile is issued before the tests.  Elsewhere the
+        // rounds load in turn (fewer registers; the lattice-sized 512-term
+        // tiles and the summary variant measured faster that way).
+        if constexpr (TABLE && T >= 2048) {
+            // every load of the tile first (one memory round trip): keys, flag
+            // words, in-tile prefixes and coefficients (each coefficient line is
+            // fetched for the flagged terms anyway), and the key before the
+            // wave's first term (later rounds take theirs from lane 63 of the
+            // round before)
+            uint64_t wd[R];
+            int wp[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int j = wave * (kTile / 4) + r * kWave + lane;
-            const int64_t i = ts + j;
-            fl[r] = false;
-            rank[r] = 0;
-            C[r] = 0.0;
-            if (j < Teff) {
-                K[r] = load_key<W>(Lz, Lx, i);
-                const uint64_t word = ldg(flags, i >> 6);
-                fl[r] = (word >> lane) & 1ull;
-                const uint64_t lowmask = lane ? ((~0ull) >> (64 - lane)) : 0ull;
-                const int aloc = (int)ldg(wpre, i >> 6) + popc64(word & lowmask);
-                s_aloc[j] = (int16_t)aloc;
-                s_hdb[j] = (int16_t)(i == 0 ? kHdbNone : hdb<W>(load_key<W>(Lz, Lx, i - 1), K[r]));
-                rank[r] = aloc;
-                if (fl[r]) C[r] = ldg(Lc, i);
-            } else {
+            for (int r = 0; r < R; ++r) {
+                const int j = wave * (kTile / 4) + r * kWave + lane;
+                const int64_t i = ts + j;
+                C[r] = 0.0;
+                wd[r] = 0ull;
+                wp[r] = 0;
 #pragma unroll
                 for (int w = 0; w < W; ++w) K[r].z[w] = K[r].x[w] = 0;
+                if (j < Teff) {
+                    K[r] = load_key<W>(Lz, Lx, i);
+                    wd[r] = ldg(flags, i >> 6);
+                    wp[r] = (int)ldg(wpre, i >> 6);
+                    C[r] = ldg(Lc, i);
+                }
+            }
+            Key<W> kp0;
+#pragma unroll
+            for (int w = 0; w < W; ++w) kp0.z[w] = kp0.x[w] = 0;
+            {
+                const int64_t i0 = ts + wave * (kTile / 4);
+                if (lane == 0 && i0 > 0 && wave * (kTile / 4) < Teff) kp0 = load_key<W>(Lz, Lx, i0 - 1);
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int j = wave * (kTile / 4) + r * kWave + lane;
+                const int64_t i = ts + j;
+                // the key at i - 1: from the lane below, lane 0 as above
+                Key<W> kp;
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    kp.z[w] = __shfl_up(K[r].z[w], 1, kWave);
+                    kp.x[w] = __shfl_up(K[r].x[w], 1, kWave);
+                }
+                if (r == 0) {
+                    if (lane == 0) kp = kp0;
+                } else {
+#pragma unroll
+                    for (int w = 0; w < W; ++w) {
+                        const uint64_t pz = __shfl(K[r - 1].z[w], kWave - 1, kWave);
+                        const uint64_t px = __shfl(K[r - 1].x[w], kWave - 1, kWave);
+                        if (lane == 0) {
+                            kp.z[w] = pz;
+                            kp.x[w] = px;
+                        }
+                    }
+                }
+                fl[r] = false;
+                rank[r] = 0;
+                if (j < Teff) {
+                    const uint64_t word = wd[r];
+                    fl[r] = (word >> lane) & 1ull;
+                    const uint64_t lowmask = lane ? ((~0ull) >> (64 - lane)) : 0ull;
+                    const int aloc = wp[r] + popc64(word & lowmask);
+                    s_aloc[j] = (int16_t)aloc;
+                    s_hdb[j] = (int16_t)(i == 0 ? kHdbNone : hdb<W>(kp, K[r]));
+                    rank[r] = aloc;
+                    if (!fl[r]) C[r] = 0.0;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int j = wave * (kTile / 4) + r * kWave + lane;
+                const int64_t i = ts + j;
+                fl[r] = false;
+                rank[r] = 0;
+                C[r] = 0.0;
+                if (j < Teff) {
+                    K[r] = load_key<W>(Lz, Lx, i);
+                    const uint64_t word = ldg(flags, i >> 6);
+                    fl[r] = (word >> lane) & 1ull;
+                    const uint64_t lowmask = lane ? ((~0ull) >> (64 - lane)) : 0ull;
+                    const int aloc = (int)ldg(wpre, i >> 6) + popc64(word & lowmask);
+                    s_aloc[j] = (int16_t)aloc;
+                    s_hdb[j] = (int16_t)(i == 0 ? kHdbNone : hdb<W>(load_key<W>(Lz, Lx, i - 1), K[r]));
+                    rank[r] = aloc;
+                    if (fl[r]) C[r] = ldg(Lc, i);
+                } else {
+#pragma unroll
+                    for (int w = 0; w < W; ++w) K[r].z[w] = K[r].x[w] = 0;
+                }
             }
         }
         if (maskp) {
