@@ -247,6 +247,25 @@ int pp_batch_graph_stats(pp_batch* p, int64_t* captures, int64_t* launches);
 int pp_batch_set_timing(pp_batch* p, int on);
 int pp_batch_merge_time(pp_batch* p, double* seconds, int64_t* launches, int reset);
 
+/* Resident layer launches: an execution mode of pp_batch_evolve and
+ * pp_evolve (window = 1) with no counterpart in the reference, whose
+ * evolve! runs rotation by rotation (spv_evolve.jl:352-385).  One kernel
+ * launch runs every rotation of the call with each vector's terms held in the
+ * LDS of a cluster of workgroups on one XCD; results are identical to the
+ * per-step kernels.  mode 0: off; 1 (default): calls of at least 8 steps
+ * whose vectors fit the clusters with room to double; 2: every call the
+ * kernel can take (vectors that outgrow their cluster finish the call on the
+ * per-step kernels).  cap_limit > 0 lowers the chunk capacity per workgroup
+ * (exercises those fallbacks); 0 = the full LDS. */
+int pp_batch_set_resident(pp_batch* p, int mode, int cap_limit);
+int pp_set_resident(pp_handle* h, int mode, int cap_limit);
+/* Resident launches so far, vectors finished by the per-step kernels after a
+ * resident launch stopped them, and aborted launches (after an abort the
+ * batch uses the per-step kernels only). */
+int pp_batch_resident_stats(pp_batch* p, int64_t* launches, int64_t* fallbacks, int64_t* aborts);
+/* HIP-event time of resident launches while timing is on (pp_batch_set_timing). */
+int pp_batch_resident_time(pp_batch* p, double* seconds, int reset);
+
 /* ---- Learned surrogate of ⟨O⟩(θ) -------------------------------------------
  * No counterpart in the reference: an ensemble of E MLPs
  * din -> hidden -> ... (layers hidden layers, GELU) -> 1, trained with Adam on

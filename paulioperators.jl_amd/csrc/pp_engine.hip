@@ -29,6 +29,7 @@
 #include "pp_window.hpp"
 #include "pp_ops.hpp"
 #include "pp_algebra.hpp"
+#include "pp_resident.hpp"
 
 using namespace pp;
 
@@ -273,11 +274,32 @@ struct Batch {
     std::vector<std::pair<GraphKey, hipGraphExec_t>> gcache;
     int64_t graph_launches = 0, graph_captures = 0;
 
+    // Resident layer launches (kr_layer): one launch per evolve call with the
+    // vectors' terms in LDS.  res_mode 0 off, 1 where the plan fits, 2 for
+    // every call the kernel can take (tests); res_cap_limit < chunk capacity
+    // forces the overflow paths (tests).
+    int res_mode = 1;
+    int res_cap_limit = 0;
+    bool res_broken = false;        // a launch was aborted: per-step kernels from then on
+    int res_kx = 0;                 // roles per XCD (CUs per XCD)
+    ResCtl* d_rctl = nullptr;       // [B]
+    uint32_t* d_rroles = nullptr;   // [kXcdCount * res_kx]
+    uint32_t* d_rtick = nullptr;    // [kXcdCount]
+    std::vector<uint32_t> h_roles;
+    int64_t res_launches = 0, res_fallbacks = 0, res_aborts = 0;
+    double t_res = 0.0;             // HIP-event time of resident launches (timing mode)
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> res_ev;
+
     WinBuf win;
 
     ~Batch() {
         for (auto& b : ib) free_inst(b);
         free_win();
+        dfree(d_rctl); dfree(d_rroles); dfree(d_rtick);
+        for (auto& e : res_ev) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
         dfree(d_desc); dfree(d_state); dfree(d_maps); dfree(d_blk); dfree(d_filt); dfree(d_chan); dfree(d_tabs);
         dfree(d_en); dfree(d_steps); dfree(d_partial); dfree(d_out); dfree(d_kets); dfree(d_bad);
         if (h_state) (void)hipHostFree(h_state);
@@ -392,6 +414,15 @@ struct Batch {
         d_bad = dalloc<int32_t>(B);
         d_epoch = dalloc<uint32_t>(1);
         HIPCHK(hipMemsetAsync(d_epoch, 0, sizeof(uint32_t), stream));
+        // resident launches need whole XCDs of equal size, each XCD's workgroups
+        // at most kResMaxK per vector
+        if (ncu % kXcdCount == 0 && ncu / kXcdCount >= 1 && ncu / kXcdCount <= 64) {
+            res_kx = ncu / kXcdCount;
+            d_rctl = dalloc<ResCtl>((size_t)B);
+            d_rroles = dalloc<uint32_t>((size_t)kXcdCount * res_kx);
+            d_rtick = dalloc<uint32_t>(kXcdCount);
+            h_roles.assign((size_t)kXcdCount * res_kx, kResIdle);
+        }
         HIPCHK(hipHostMalloc((void**)&h_state, sizeof(SegState) * (size_t)B, hipHostMallocDefault));
         std::memset(h_state, 0, sizeof(SegState) * (size_t)B);
         for (int b = 0; b < B; ++b) {
@@ -764,12 +795,128 @@ struct Batch {
         }
     }
 
+    // ----------------------------------------------------------------------
+    // Resident layer launches (kr_layer).
+
+    int res_cap() const {
+        const int full = W == 1 ? ResGeom<1>::cap : ResGeom<2>::cap;
+        return res_cap_limit > 0 && res_cap_limit < full ? res_cap_limit : full;
+    }
+
+    // Roles of one resident launch: vectors to XCDs by decreasing size, each
+    // with K workgroups whose starting chunks fill at most 60% of the chunk
+    // capacity.  false: the call runs on the per-step kernels (auto mode also
+    // declines calls of few steps and vectors that could not double in place).
+    bool res_plan(const std::vector<StepDev>& steps, int64_t nsteps) {
+        if (res_mode == 0 || res_broken || res_kx == 0) return false;
+        if (B > kXcdCount * res_kx) return false;
+        if (res_mode == 1 && nsteps < 8) return false;
+        for (int64_t s = 0; s < nsteps; ++s)
+            for (int b = 0; b < B; ++b) {
+                const StepDev& sd = steps[(size_t)(s * B + b)];
+                if (sd.active && sd.g.nlev > kResLevels) return false;
+            }
+        const int cap = res_cap();
+        const int kmax = std::min(res_kx, kResMaxK);
+        std::vector<int> K((size_t)B, 1);
+        const int fill = std::max(1, cap * 3 / 5);  // starting chunks at most 60% full
+        auto fits = [&](int b) { return n_known[b] <= (int64_t)K[b] * fill; };
+        for (int b = 0; b < B; ++b) {
+            const int64_t want = cdiv(std::max<int64_t>(n_known[b], 1), fill);
+            K[b] = (int)std::min<int64_t>(want, kmax);
+            if (res_mode == 1 && !fits(b)) return false;
+        }
+        std::vector<int> order((size_t)B);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return n_known[a] > n_known[c]; });
+        int freeslots[kXcdCount];
+        int64_t load[kXcdCount];
+        for (int x = 0; x < kXcdCount; ++x) {
+            freeslots[x] = res_kx;
+            load[x] = 0;
+        }
+        std::vector<int> xcd((size_t)B, 0);
+        for (int b : order) {
+            int x = 0;
+            for (int y = 1; y < kXcdCount; ++y)
+                if (freeslots[y] > freeslots[x] || (freeslots[y] == freeslots[x] && load[y] < load[x])) x = y;
+            if (freeslots[x] < 1) return false;
+            if (K[b] > freeslots[x]) {
+                K[b] = freeslots[x];
+                if (res_mode == 1 && !fits(b)) return false;
+            }
+            xcd[(size_t)b] = x;
+            freeslots[x] -= K[b];
+            load[x] += n_known[b];
+        }
+        std::fill(h_roles.begin(), h_roles.end(), kResIdle);
+        int next[kXcdCount] = {0};
+        for (int b = 0; b < B; ++b)
+            for (int k = 0; k < K[b]; ++k)
+                h_roles[(size_t)xcd[b] * res_kx + next[xcd[b]]++] =
+                    (uint32_t)b | ((uint32_t)k << 20) | ((uint32_t)K[b] << 26);
+        HIPCHK(hipMemcpyAsync(d_rroles, h_roles.data(), h_roles.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                              stream));
+        return true;
+    }
+
+    // Sequence-start precheck and compaction, then one resident launch for
+    // steps [first, nsteps) of every vector and the commit of its results.
+    template <int Wt>
+    void enqueue_resident(const StepDev* dsteps, int64_t first, int64_t nsteps) {
+        reserve_epochs(1);
+        hipLaunchKernelGGL(kb_precheck, dim3((unsigned)cdiv(B, 256)), dim3(256), 0, stream, d_state,
+                           (const int32_t*)d_en, B);
+        with_tiles<Wt>([&](auto, auto TM) { launch_compact<Wt, decltype(TM)::value>(0); });
+        hipLaunchKernelGGL(kb_epoch_advance, dim3(1), dim3(1), 0, stream, d_epoch, 1u);
+        hipLaunchKernelGGL(kr_reset, dim3((unsigned)cdiv(std::max(B, kXcdCount), 256)), dim3(256), 0, stream, d_rctl,
+                           d_rtick, B);
+        ResArgs a;
+        a.roles = d_rroles;
+        a.tickets = d_rtick;
+        a.kx = res_kx;
+        a.B = B;
+        a.D = d_desc;
+        a.S = d_state;
+        a.steps = dsteps;
+        a.first = first;
+        a.nsteps = nsteps;
+        a.ctl = d_rctl;
+        a.cap = res_cap();
+        const dim3 grid((unsigned)(kXcdCount * res_kx)), block(kResThreads);
+        if (timing) {
+            hipEvent_t e0, e1;
+            HIPCHK(hipEventCreate(&e0));
+            HIPCHK(hipEventCreate(&e1));
+            res_ev.emplace_back(e0, e1);
+            hipExtLaunchKernelGGL((kr_layer<Wt>), grid, block, 0, stream, e0, e1, 0, a);
+        } else {
+            hipLaunchKernelGGL((kr_layer<Wt>), grid, block, 0, stream, a);
+        }
+        hipLaunchKernelGGL(kr_commit, dim3((unsigned)cdiv(B, 256)), dim3(256), 0, stream, (const ResCtl*)d_rctl, d_state,
+                           B, first);
+        HIPCHK(hipGetLastError());
+        res_launches += 1;
+    }
+
+    void harvest_res_timing() {
+        for (auto& e : res_ev) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, e.first, e.second));
+            t_res += ms * 1e-3;
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        res_ev.clear();
+    }
+
     void sync_state() {
         hipLaunchKernelGGL(kb_resolve, dim3((unsigned)cdiv(B, 256)), dim3(256), 0, stream, d_state, B);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(h_state, d_state, sizeof(SegState) * (size_t)B, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
         if (timing) harvest_timing();
+        if (!res_ev.empty()) harvest_res_timing();
         for (int b = 0; b < B; ++b) {
             if (h_state[b].err)
                 throw PPError(PP_EHIP, "internal consistency check failed in a kernel (vector " + std::to_string(b) +
@@ -814,7 +961,8 @@ struct Batch {
         HIPCHK(hipMemcpyAsync(d_steps, steps.data(), (size_t)(nsteps * B) * sizeof(StepDev), hipMemcpyHostToDevice,
                               stream));
         HIPCHK(hipMemcpyAsync(d_en, en.data(), (size_t)B * sizeof(int32_t), hipMemcpyHostToDevice, stream));
-        enqueue_sequence<Wt>(true, d_steps, 0, nsteps, graphs);
+        if (res_plan(steps, nsteps)) enqueue_resident<Wt>(d_steps, 0, nsteps);
+        else enqueue_sequence<Wt>(true, d_steps, 0, nsteps, graphs);
         sync_state();
         for (int round = 0;; ++round) {
             std::vector<int> stalled;
@@ -825,6 +973,18 @@ struct Batch {
             int64_t first = nsteps;
             for (int b : stalled) {
                 SegState& st = h_state[b];
+                if (st.stalled >= 3) {
+                    // a resident launch stopped this vector (a chunk outgrew its
+                    // LDS) or was aborted (4): the per-step kernels finish the
+                    // call from that step, with the buffers as they are
+                    res_fallbacks += 1;
+                    if (st.stalled == 4) {
+                        res_aborts += 1;
+                        res_broken = true;
+                    }
+                    first = std::min(first, st.stall_step);
+                    continue;
+                }
                 const int64_t cap = std::min(ib[b].L[0].cap, ib[b].L[1].cap);
                 const int64_t n = st.n[st.cur];
                 // a vector that outgrew its buffers mid-sequence usually keeps
@@ -2338,6 +2498,42 @@ int pp_batch_merge_time(pp_batch* p, double* seconds, int64_t* launches, int res
         p->bt.merge_launches = 0;
     }
     return PP_OK;
+}
+
+static int set_resident(Batch& bt, int mode, int cap_limit) {
+    if (mode < 0 || mode > 2) return fail(PP_EARG, "resident mode must be 0, 1 or 2");
+    if (cap_limit < 0) return fail(PP_EARG, "cap_limit must be >= 0");
+    bt.res_mode = mode;
+    bt.res_cap_limit = cap_limit;
+    g_last_error.clear();
+    return PP_OK;
+}
+
+int pp_batch_set_resident(pp_batch* p, int mode, int cap_limit) {
+    if (!p) return fail(PP_EARG, "null handle");
+    return set_resident(p->bt, mode, cap_limit);
+}
+
+int pp_set_resident(pp_handle* h, int mode, int cap_limit) {
+    if (!h) return fail(PP_EARG, "null handle");
+    return set_resident(h->bt, mode, cap_limit);
+}
+
+int pp_batch_resident_stats(pp_batch* p, int64_t* launches, int64_t* fallbacks, int64_t* aborts) {
+    if (!p || !launches || !fallbacks || !aborts) return fail(PP_EARG, "null argument");
+    *launches = p->bt.res_launches;
+    *fallbacks = p->bt.res_fallbacks;
+    *aborts = p->bt.res_aborts;
+    return PP_OK;
+}
+
+int pp_batch_resident_time(pp_batch* p, double* seconds, int reset) {
+    if (!p || !seconds) return fail(PP_EARG, "null argument");
+    API_BEGIN
+    p->bt.sync_state();
+    *seconds = p->bt.t_res;
+    if (reset) p->bt.t_res = 0.0;
+    API_END
 }
 
 int pp_batch_expectation_ket(pp_batch* p, const uint64_t* kets, double* out) {

@@ -236,6 +236,28 @@ class SparsePauliBatch:
         check(_lib.lib().pp_batch_graph_stats(self._h, ctypes.byref(c), ctypes.byref(n)))
         return c.value, n.value
 
+    def set_resident(self, mode: int = 1, cap_limit: int = 0):
+        """Resident layer launches (one kernel launch per evolve call with the
+        terms in LDS): 0 off, 1 where the plan fits (default), 2 every call the
+        kernel can take; cap_limit > 0 lowers the chunk capacity per
+        workgroup (exercises the fallback to the per-step kernels)."""
+        check(_lib.lib().pp_batch_set_resident(self._h, int(mode), int(cap_limit)))
+        return self
+
+    def resident_stats(self):
+        """(resident launches, vectors finished by the per-step kernels after a
+        resident stop, aborted launches)."""
+        a = ctypes.c_int64()
+        b = ctypes.c_int64()
+        c = ctypes.c_int64()
+        check(_lib.lib().pp_batch_resident_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def resident_time(self, reset: bool = False) -> float:
+        s = ctypes.c_double()
+        check(_lib.lib().pp_batch_resident_time(self._h, ctypes.byref(s), 1 if reset else 0))
+        return s.value
+
     def set_timing(self, on: bool):
         check(_lib.lib().pp_batch_set_timing(self._h, 1 if on else 0))
 

@@ -142,6 +142,12 @@ class SparsePauliVector:
                 pass
             self._h = None
 
+    def set_resident(self, mode: int = 1, cap_limit: int = 0):
+        """Resident layer launches for evolve_ at window = 1 (see
+        SparsePauliBatch.set_resident)."""
+        check(_lib.lib().pp_set_resident(self._h, int(mode), int(cap_limit)))
+        return self
+
     # --- size / conversion ------------------------------------------------------------
     def __len__(self) -> int:
         n = ctypes.c_int64()
