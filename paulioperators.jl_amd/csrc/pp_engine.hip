@@ -287,6 +287,10 @@ struct Batch {
     uint32_t* d_rtick = nullptr;    // [kXcdCount]
     std::vector<uint32_t> h_roles;
     int64_t res_launches = 0, res_fallbacks = 0, res_aborts = 0;
+    std::vector<double> res_boost;   // per vector: divisor of the starting fill after chunk overflows (this call)
+    std::vector<char> res_off;       // per vector: per-step kernels for the rest of this call
+    int32_t* d_rpart = nullptr;      // [B] vectors in the current resident launch
+    std::vector<int32_t> h_rpart;
     double t_res = 0.0;             // HIP-event time of resident launches (timing mode)
     std::vector<std::pair<hipEvent_t, hipEvent_t>> res_ev;
 
@@ -295,7 +299,7 @@ struct Batch {
     ~Batch() {
         for (auto& b : ib) free_inst(b);
         free_win();
-        dfree(d_rctl); dfree(d_rroles); dfree(d_rtick);
+        dfree(d_rctl); dfree(d_rroles); dfree(d_rtick); dfree(d_rpart);
         for (auto& e : res_ev) {
             (void)hipEventDestroy(e.first);
             (void)hipEventDestroy(e.second);
@@ -416,11 +420,15 @@ struct Batch {
         HIPCHK(hipMemsetAsync(d_epoch, 0, sizeof(uint32_t), stream));
         // resident launches need whole XCDs of equal size, each XCD's workgroups
         // at most kResMaxK per vector
+        res_boost.assign((size_t)B, 1.0);
+        res_off.assign((size_t)B, 0);
+        h_rpart.assign((size_t)B, 0);
         if (ncu % kXcdCount == 0 && ncu / kXcdCount >= 1 && ncu / kXcdCount <= 64) {
             res_kx = ncu / kXcdCount;
             d_rctl = dalloc<ResCtl>((size_t)B);
             d_rroles = dalloc<uint32_t>((size_t)kXcdCount * res_kx);
             d_rtick = dalloc<uint32_t>(kXcdCount);
+            d_rpart = dalloc<int32_t>((size_t)B);
             h_roles.assign((size_t)kXcdCount * res_kx, kResIdle);
         }
         HIPCHK(hipHostMalloc((void**)&h_state, sizeof(SegState) * (size_t)B, hipHostMallocDefault));
@@ -744,6 +752,12 @@ struct Batch {
         bh = (bh ^ (scan_needed(false) ? 2u : 0u) ^ (big_call() ? 4u : 0u)) * 1099511628211ull;
         GraphKey key{nsteps, with_precheck ? -1 - first : first, tile, mtile, Wt, blocks(0), blocks(1), blocks(3),
                      (const void*)dsteps, bh};
+        launch_captured(key, body);
+    }
+
+    // Replay the graph captured for `key`, or capture body() into one.
+    template <typename F>
+    void launch_captured(const GraphKey& key, F&& body) {
         for (auto& g : gcache)
             if (g.first == key) {
                 HIPCHK(hipGraphLaunch(g.second, stream));
@@ -803,31 +817,44 @@ struct Batch {
         return res_cap_limit > 0 && res_cap_limit < full ? res_cap_limit : full;
     }
 
-    // Roles of one resident launch: vectors to XCDs by decreasing size, each
-    // with K workgroups whose starting chunks fill at most 60% of the chunk
-    // capacity.  false: the call runs on the per-step kernels (auto mode also
-    // declines calls of few steps and vectors that could not double in place).
+    // Roles of one resident launch: the vectors with active steps go to XCDs
+    // by decreasing size, each with K workgroups whose starting chunks fill at
+    // most 60% of a chunk (divided by the vector's overflow boost: a vector
+    // whose chunks overflowed resumes with more workgroups).
+    // false: the call runs on the per-step kernels (auto mode also declines
+    // calls of few steps and vectors that do not fit one XCD that way).
     bool res_plan(const std::vector<StepDev>& steps, int64_t nsteps) {
         if (res_mode == 0 || res_broken || res_kx == 0) return false;
-        if (B > kXcdCount * res_kx) return false;
         if (res_mode == 1 && nsteps < 8) return false;
+        std::vector<char> part((size_t)B, 0);
         for (int64_t s = 0; s < nsteps; ++s)
             for (int b = 0; b < B; ++b) {
                 const StepDev& sd = steps[(size_t)(s * B + b)];
-                if (sd.active && sd.g.nlev > kResLevels) return false;
+                if (!sd.active) continue;
+                if (sd.g.nlev > kResLevels || res_off[(size_t)b]) return false;
+                part[(size_t)b] = 1;
             }
         const int cap = res_cap();
         const int kmax = std::min(res_kx, kResMaxK);
-        std::vector<int> K((size_t)B, 1);
-        const int fill = std::max(1, cap * 3 / 5);  // starting chunks at most 60% full
-        auto fits = [&](int b) { return n_known[b] <= (int64_t)K[b] * fill; };
+        std::vector<int> K((size_t)B, 0);
+        std::vector<int64_t> fill((size_t)B);
+        int nparts = 0;
+        const int64_t base = std::max<int64_t>(1, (int64_t)(0.6 * cap));
         for (int b = 0; b < B; ++b) {
-            const int64_t want = cdiv(std::max<int64_t>(n_known[b], 1), fill);
+            fill[(size_t)b] = std::max<int64_t>(1, (int64_t)(base / res_boost[(size_t)b]));
+            nparts += part[(size_t)b];
+        }
+        if (nparts > kXcdCount * res_kx) return false;
+        auto fits = [&](int b) { return n_known[b] <= (int64_t)K[b] * base; };
+        for (int b = 0; b < B; ++b) {
+            if (!part[(size_t)b]) continue;
+            const int64_t want = cdiv(std::max<int64_t>(n_known[b], 1), fill[(size_t)b]);
             K[b] = (int)std::min<int64_t>(want, kmax);
             if (res_mode == 1 && !fits(b)) return false;
         }
-        std::vector<int> order((size_t)B);
-        std::iota(order.begin(), order.end(), 0);
+        std::vector<int> order;
+        for (int b = 0; b < B; ++b)
+            if (part[(size_t)b]) order.push_back(b);
         std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return n_known[a] > n_known[c]; });
         int freeslots[kXcdCount];
         int64_t load[kXcdCount];
@@ -855,22 +882,20 @@ struct Batch {
             for (int k = 0; k < K[b]; ++k)
                 h_roles[(size_t)xcd[b] * res_kx + next[xcd[b]]++] =
                     (uint32_t)b | ((uint32_t)k << 20) | ((uint32_t)K[b] << 26);
+        for (int b = 0; b < B; ++b) h_rpart[(size_t)b] = part[(size_t)b];
         HIPCHK(hipMemcpyAsync(d_rroles, h_roles.data(), h_roles.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                              stream));
+        HIPCHK(hipMemcpyAsync(d_rpart, h_rpart.data(), h_rpart.size() * sizeof(int32_t), hipMemcpyHostToDevice,
                               stream));
         return true;
     }
 
-    // Sequence-start precheck and compaction, then one resident launch for
-    // steps [first, nsteps) of every vector and the commit of its results.
+    // (Sequence start: the precheck and compaction, then) one resident launch
+    // for steps [first, nsteps) of the planned vectors and the commit of its
+    // results.
     template <int Wt>
-    void enqueue_resident(const StepDev* dsteps, int64_t first, int64_t nsteps) {
-        reserve_epochs(1);
-        hipLaunchKernelGGL(kb_precheck, dim3((unsigned)cdiv(B, 256)), dim3(256), 0, stream, d_state,
-                           (const int32_t*)d_en, B);
-        with_tiles<Wt>([&](auto, auto TM) { launch_compact<Wt, decltype(TM)::value>(0); });
-        hipLaunchKernelGGL(kb_epoch_advance, dim3(1), dim3(1), 0, stream, d_epoch, 1u);
-        hipLaunchKernelGGL(kr_reset, dim3((unsigned)cdiv(std::max(B, kXcdCount), 256)), dim3(256), 0, stream, d_rctl,
-                           d_rtick, B);
+    void enqueue_resident(const StepDev* dsteps, int64_t first, int64_t nsteps, bool with_precheck) {
+        if (with_precheck) reserve_epochs(1);
         ResArgs a;
         a.roles = d_rroles;
         a.tickets = d_rtick;
@@ -884,19 +909,38 @@ struct Batch {
         a.ctl = d_rctl;
         a.cap = res_cap();
         const dim3 grid((unsigned)(kXcdCount * res_kx)), block(kResThreads);
-        if (timing) {
-            hipEvent_t e0, e1;
-            HIPCHK(hipEventCreate(&e0));
-            HIPCHK(hipEventCreate(&e1));
-            res_ev.emplace_back(e0, e1);
-            hipExtLaunchKernelGGL((kr_layer<Wt>), grid, block, 0, stream, e0, e1, 0, a);
-        } else {
-            hipLaunchKernelGGL((kr_layer<Wt>), grid, block, 0, stream, a);
-        }
-        hipLaunchKernelGGL(kr_commit, dim3((unsigned)cdiv(B, 256)), dim3(256), 0, stream, (const ResCtl*)d_rctl, d_state,
-                           B, first);
-        HIPCHK(hipGetLastError());
+        auto body = [&](bool timed) {
+            if (with_precheck) {
+                hipLaunchKernelGGL(kb_precheck, dim3((unsigned)cdiv(B, 256)), dim3(256), 0, stream, d_state,
+                                   (const int32_t*)d_en, B);
+                with_tiles<Wt>([&](auto, auto TM) { launch_compact<Wt, decltype(TM)::value>(0); });
+                hipLaunchKernelGGL(kb_epoch_advance, dim3(1), dim3(1), 0, stream, d_epoch, 1u);
+            }
+            hipLaunchKernelGGL(kr_reset, dim3((unsigned)cdiv(std::max(B, kXcdCount), 256)), dim3(256), 0, stream,
+                               d_rctl, d_rtick, B);
+            if (timed) {
+                hipEvent_t e0, e1;
+                HIPCHK(hipEventCreate(&e0));
+                HIPCHK(hipEventCreate(&e1));
+                res_ev.emplace_back(e0, e1);
+                hipExtLaunchKernelGGL((kr_layer<Wt>), grid, block, 0, stream, e0, e1, 0, a);
+            } else {
+                hipLaunchKernelGGL((kr_layer<Wt>), grid, block, 0, stream, a);
+            }
+            hipLaunchKernelGGL(kr_commit, dim3((unsigned)cdiv(B, 256)), dim3(256), 0, stream, (const ResCtl*)d_rctl,
+                               (const int32_t*)d_rpart, d_state, B, first);
+        };
         res_launches += 1;
+        if (!graphs || timing) {
+            body(timing);
+            HIPCHK(hipGetLastError());
+            return;
+        }
+        // the role table is data (uploaded before the launch), so one graph
+        // serves every call of the same step count and grids
+        GraphKey key{nsteps, -(int64_t(1) << 40) - first, tile, mtile, Wt, blocks(3), (unsigned)res_cap(),
+                     (unsigned)res_kx, (const void*)dsteps, 0x7265736964656e74ull ^ (with_precheck ? 1u : 0u)};
+        launch_captured(key, [&]() { body(false); });
     }
 
     void harvest_res_timing() {
@@ -935,6 +979,15 @@ struct Batch {
     // replayed until every vector has finished the sequence.
     template <int Wt>
     void run_sequence(std::vector<StepDev>& steps, int64_t nsteps, const std::vector<int32_t>& en) {
+        std::fill(res_off.begin(), res_off.end(), 0);
+        std::fill(res_boost.begin(), res_boost.end(), 1.0);  // boosts last one call
+        res_ovf.assign((size_t)B, 0);
+        run_sequence_body<Wt>(steps, nsteps, en);
+    }
+    std::vector<char> res_ovf;  // per vector: a chunk overflowed in this call
+
+    template <int Wt>
+    void run_sequence_body(std::vector<StepDev>& steps, int64_t nsteps, const std::vector<int32_t>& en) {
         int maxlev = 1;
         for (const StepDev& s : steps) maxlev = std::max(maxlev, (int)s.g.nlev);
         step_bounds.assign((size_t)nsteps, 0);
@@ -961,9 +1014,10 @@ struct Batch {
         HIPCHK(hipMemcpyAsync(d_steps, steps.data(), (size_t)(nsteps * B) * sizeof(StepDev), hipMemcpyHostToDevice,
                               stream));
         HIPCHK(hipMemcpyAsync(d_en, en.data(), (size_t)B * sizeof(int32_t), hipMemcpyHostToDevice, stream));
-        if (res_plan(steps, nsteps)) enqueue_resident<Wt>(d_steps, 0, nsteps);
+        if (res_plan(steps, nsteps)) enqueue_resident<Wt>(d_steps, 0, nsteps, true);
         else enqueue_sequence<Wt>(true, d_steps, 0, nsteps, graphs);
         sync_state();
+        std::vector<int64_t> last_stall((size_t)B, -1);
         for (int round = 0;; ++round) {
             std::vector<int> stalled;
             for (int b = 0; b < B; ++b)
@@ -974,13 +1028,20 @@ struct Batch {
             for (int b : stalled) {
                 SegState& st = h_state[b];
                 if (st.stalled >= 3) {
-                    // a resident launch stopped this vector (a chunk outgrew its
-                    // LDS) or was aborted (4): the per-step kernels finish the
-                    // call from that step, with the buffers as they are
+                    // a resident launch stopped this vector: a chunk outgrew
+                    // its LDS (3; the vector resumes in a resident launch with
+                    // twice the workgroups, or on the per-step kernels when it
+                    // overflows twice at one step) or the launch was aborted
+                    // (4; per-step kernels from then on)
                     res_fallbacks += 1;
                     if (st.stalled == 4) {
                         res_aborts += 1;
                         res_broken = true;
+                    } else {
+                        if (res_ovf[(size_t)b] && last_stall[(size_t)b] == st.stall_step) res_off[(size_t)b] = 1;
+                        res_ovf[(size_t)b] = 1;
+                        last_stall[(size_t)b] = st.stall_step;
+                        res_boost[(size_t)b] = std::min(32.0, res_boost[(size_t)b] * 2.0);
                     }
                     first = std::min(first, st.stall_step);
                     continue;
@@ -1017,7 +1078,8 @@ struct Batch {
             upload_desc();
             plan();
             HIPCHK(hipMemcpyAsync(d_steps, rs.data(), rs.size() * sizeof(StepDev), hipMemcpyHostToDevice, stream));
-            enqueue_sequence<Wt>(false, d_steps, first, nsteps, false);
+            if (res_plan(rs, nsteps - first)) enqueue_resident<Wt>(d_steps, first, nsteps, false);
+            else enqueue_sequence<Wt>(false, d_steps, first, nsteps, false);
             sync_state();
         }
     }

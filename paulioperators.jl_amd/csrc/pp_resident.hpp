@@ -854,7 +854,6 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
                 }
             }
         }
-        if (tid == kResThreads - 1) res_step_commit(A, b, nxt, &s_step[cs ^ 1]);
         int mk = (int)m_tot;
         if (K > 1) {
             __syncthreads();
@@ -970,6 +969,9 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
             }
             nj = total;
         }
+        // the next rotation's step (its loads were issued in phase 2); the
+        // barrier at the top of the loop publishes it
+        if (tid == kResThreads - 1) res_step_commit(A, b, nxt, &s_step[cs ^ 1]);
         cb ^= 1;  // (an overflowed chunk's state stays in the previous buffer)
         if (tid == 0) {
             s_bk[kBkPn] = n_tot;
@@ -1037,9 +1039,10 @@ __global__ void kr_reset(ResCtl* __restrict__ ctl, uint32_t* __restrict__ ticket
 // After a resident launch: apply the staged results of every vector whose
 // cluster finished; a vector whose launch was aborted (or never ran) is left
 // as it was and marked for replay from the call's first step (code 4).
-__global__ void kr_commit(const ResCtl* __restrict__ ctl, SegState* __restrict__ S, int B, int64_t first) {
+__global__ void kr_commit(const ResCtl* __restrict__ ctl, const int32_t* __restrict__ part, SegState* __restrict__ S,
+                          int B, int64_t first) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= B) return;
+    if (b >= B || !part[b]) return;
     const ResCtl* c = ctl + b;
     SegState* s = S + b;
     if (s->stalled) return;

@@ -142,12 +142,13 @@ def lattice_batch(cap, n_layers, thetas, mode=2):
 
 def test_resident_lattice121_first_layers():
     """The 121-qubit sweep's first 4 layers for 8 of its 16 θ_h: clusters of
-    several workgroups, vectors that outgrow them mid-layer (finished by the
-    per-step kernels), dephasing between layers."""
+    several workgroups, vectors that outgrow them mid-layer (resumed from
+    that rotation with more workgroups, or on the per-step kernels),
+    dephasing between layers."""
     ths = wl.theta_h_values()[::2]
     bt = lattice_batch(0, 4, ths)
     launches, fallbacks, aborts = bt.resident_stats()
-    assert launches == 4 and aborts == 0
+    assert launches >= 4 and aborts == 0
 
 
 @pytest.mark.parametrize("cap", [40, 256])
@@ -156,7 +157,7 @@ def test_resident_lattice121_small_chunks(cap):
     ths = wl.theta_h_values()[1:5]
     bt = lattice_batch(cap, 3, ths)
     launches, fallbacks, aborts = bt.resident_stats()
-    assert launches == 3 and aborts == 0 and fallbacks > 0
+    assert launches >= 3 and aborts == 0 and fallbacks > 0
 
 
 def test_resident_batched_instances():
