@@ -156,6 +156,14 @@ int pp_truncate(pp_handle* h, const pp_strategy_op* ops, int nops, const uint64_
  * query keys (W words each in z, x), found[i] = 1 and c[i] = the coefficient
  * when the key is live, else found[i] = 0 and c[i] = 0.  Device binary search. */
 int pp_lookup(pp_handle* h, const uint64_t* z, const uint64_t* x, int64_t count, double* c, int32_t* found);
+/* pp_lookup without device allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under hipGraph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * results as pp_lookup; the staged query keys and results live in a fixed
+ * per-GPU scratch array of 2.75 MiB, held until the process ends, so at most
+ * 65536 queries per call (more: PP_EARG).  Calls that overlap on one GPU (from
+ * several host threads or handles) race on that array. */
+int pp_lookup_noalloc(pp_handle* h, const uint64_t* z, const uint64_t* x, int64_t count, double* c, int32_t* found);
 /* setindex! for `count` keys (type_SparsePauliVector.jl:315-335): present
  * keys get the new coefficient, absent keys are inserted in key order (the
  * reference's live-capacity growth is applied per inserted key); for repeated

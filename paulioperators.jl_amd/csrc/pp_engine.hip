@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <type_traits>
@@ -215,8 +216,37 @@ void set_last_error(const std::string& msg) { g_last_error = msg; }
 }  // namespace pp
 
 // ---------------------------------------------------------------------------
+// Scratch of pp_lookup_noalloc: its staged query keys and results live in this
+// per-GPU array (held by each GPU the library runs on until the process ends),
+// so the call allocates nothing.  Host code uses the address kept per device
+// by scratch_setup(), never the array's name (a host-side stand-in).
+constexpr int64_t kLookupNoAllocMax = int64_t(1) << 16;  // queries per call
+constexpr size_t kScratchBytes = (size_t)kLookupNoAllocMax * (2 * 16 + 8 + 4);  // z, x (2 words each), c, found
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kScratchDevices] = {};
+
+// Once per device, with that device current (pp_create): the device address
+// of its copy of the scratch array.
+int scratch_setup(int dev) {
+    if (dev < 0 || dev >= kScratchDevices) return fail(PP_EUNSUPPORTED, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return PP_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(PP_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+    return PP_OK;
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
 struct Batch {
-    int W = 1, B = 0, ncu = 256;
+    int W = 1, B = 0, ncu = 256, dev = 0;
     int sweep_fold_above = 32;  // sweep workgroups take several tiles past this many tiles per CU
     hipStream_t stream = nullptr;
     std::vector<int> N;
@@ -375,7 +405,6 @@ struct Batch {
         W = W_;
         B = (int)Ns.size();
         N = Ns;
-        int dev = 0;
         HIPCHK(hipGetDevice(&dev));
         hipDeviceProp_t prop;
         HIPCHK(hipGetDeviceProperties(&prop, dev));
@@ -1611,6 +1640,29 @@ struct Batch {
         dfree(qz); dfree(qx); dfree(dout); dfree(dfound);
     }
 
+    // lookup with the staged queries and results in `base` (the scratch array
+    // of pp_lookup_noalloc; count <= kLookupNoAllocMax): no device allocation.
+    template <int Wt>
+    void lookup_noalloc(const uint64_t* z, const uint64_t* x, int64_t count, double* out, int32_t* found,
+                        unsigned char* base) {
+        if (count <= 0) return;
+        int64_t n = 0;
+        const DevSet& L = live0(n);
+        uint64_t* qz = reinterpret_cast<uint64_t*>(base);
+        uint64_t* qx = qz + kLookupNoAllocMax * 2;
+        double* dout = reinterpret_cast<double*>(qx + kLookupNoAllocMax * 2);
+        int32_t* dfound = reinterpret_cast<int32_t*>(dout + kLookupNoAllocMax);
+        HIPCHK(hipMemcpyAsync(qz, z, (size_t)count * Wt * 8, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(qx, x, (size_t)count * Wt * 8, hipMemcpyHostToDevice, stream));
+        const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(count, kBlock), 4096));
+        hipLaunchKernelGGL(ko_lookup<Wt>, dim3(grid), dim3(kBlock), 0, stream, L.z, L.x, L.c, n, qz, qx, count, dout,
+                           dfound, (int64_t*)nullptr);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out, dout, (size_t)count * 8, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(found, dfound, (size_t)count * 4, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
     // Device indices of `count` keys in the live set of vector 0 (-1: absent).
     template <int Wt>
     std::vector<int64_t> find_indices(const uint64_t* z, const uint64_t* x, int64_t count, int64_t* d_idx_out) {
@@ -2118,6 +2170,11 @@ int pp_create(int N, int64_t live_cap, int64_t append_cap, pp_handle** out) {
         delete h;
         return fail(e.code, e.msg);
     }
+    const int rc = scratch_setup(h->bt.dev);
+    if (rc != PP_OK) {
+        delete h;
+        return rc;
+    }
     *out = h;
     g_last_error.clear();
     return PP_OK;
@@ -2271,6 +2328,20 @@ int pp_lookup(pp_handle* h, const uint64_t* z, const uint64_t* x, int64_t count,
     if (!h || count < 0 || (count > 0 && (!z || !x || !c || !found))) return fail(PP_EARG, "bad arguments");
     API_BEGIN
     dispatch_w(h->bt.W, [&](auto wc) { h->bt.lookup<decltype(wc)::value>(z, x, count, c, found); });
+    API_END
+}
+
+int pp_lookup_noalloc(pp_handle* h, const uint64_t* z, const uint64_t* x, int64_t count, double* c,
+                      int32_t* found) {
+    if (!h || count < 0 || (count > 0 && (!z || !x || !c || !found))) return fail(PP_EARG, "bad arguments");
+    if (count > kLookupNoAllocMax)
+        return fail(PP_EARG, "pp_lookup_noalloc: count exceeds the scratch capacity of " +
+                                 std::to_string(kLookupNoAllocMax) + " queries per call");
+    const int dev = h->bt.dev;
+    unsigned char* base = dev >= 0 && dev < kScratchDevices ? static_cast<unsigned char*>(g_scratch_ptr[dev]) : nullptr;
+    if (!base) return fail(PP_EHIP, "pp_lookup_noalloc: no scratch address for this device");
+    API_BEGIN
+    dispatch_w(h->bt.W, [&](auto wc) { h->bt.lookup_noalloc<decltype(wc)::value>(z, x, count, c, found, base); });
     API_END
 }
 
