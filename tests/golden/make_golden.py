@@ -13,6 +13,9 @@
   sensitive key/coefficient digest after the first layers of the 64-qubit
   chain (2 layers) and of the 121-qubit lattice (2 layers, θ_h index 5 and
   15), and after 4 layers of the first 3 batched instances.
+* lattice121_full.json — for each of the 16 θ_h of the 121-qubit sweep, the
+  live-term count after every layer and, after all 20 layers, ⟨0|O|0⟩
+  (float.hex) and the digest of the final term arrays.
 
 The outputs come from the oracle (a C restatement of the reference), so the
 fixtures pin the oracle and the device path against silent drift; the
@@ -74,6 +77,23 @@ def workloads_json():
     }
 
 
+def lattice_full():
+    out = []
+    for th in wl.theta_h_values():
+        w = wl.lattice121(th)
+        o = po.OracleSPV(w.N, [(p.z, p.x, c) for p, c in w.observable])
+        f = compile_filter(w.truncation)
+        sizes = []
+        for L in w.layers:
+            o.evolve([(p.z, p.x) for p in L.generators], L.angles, strict=f)
+            o.channel(*w.channel)
+            sizes.append(len(o))
+        z, x, c = o.arrays()
+        out.append({"theta_h": float(th).hex(), "n_by_layer": sizes, "expectation": float(o.expectation(0)).hex(),
+                    "digest": digest(z, x, c)})
+    return out
+
+
 def main():
     with open(os.path.join(HERE, "workloads.json"), "w") as fh:
         json.dump(workloads_json(), fh)
@@ -97,6 +117,8 @@ def main():
         print(name, len(c), file=sys.stderr)
     with open(os.path.join(HERE, "prefixes.json"), "w") as fh:
         json.dump(pref, fh, indent=1)
+    with open(os.path.join(HERE, "lattice121_full.json"), "w") as fh:
+        json.dump(lattice_full(), fh, indent=1)
 
 
 if __name__ == "__main__":

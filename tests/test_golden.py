@@ -51,3 +51,9 @@ def test_oracle_prefixes_match_fixture(name):
     assert len(c) == fx["n"]
     assert float(o.expectation(0)).hex() == fx["expectation"]
     assert mg.digest(z, x, c) == fx["digest"]
+
+
+def test_oracle_lattice121_full_sweep_matches_fixture():
+    """All 16 θ_h x 20 layers of the 121-qubit sweep: per-layer term counts,
+    final ⟨O⟩ and final array digests (about 4 s of oracle time)."""
+    assert mg.lattice_full() == load("lattice121_full.json")
