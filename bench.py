@@ -7,17 +7,24 @@ ZZ(θ_J = -π/2) on the 220 nearest-neighbour bonds then X(θ_h) on all sites;
 dephasing p = 5e-4 per qubit after each layer; truncation |c| <= 1e-4 and
 weight <= 12 after every rotation; observable (1/121) Σ_i Z_i; FP64.  The 16
 initial operators are resident on the GPU when the timed region starts; the
-16 evolutions run as one batch (one kernel sequence per rotation step for
-all of them).
+16 evolutions run as one batch: each layer's 341 rotations in one resident
+launch (the terms held in LDS by a cluster of workgroups per vector) where the
+vectors fit, else one kernel sequence per rotation for all of them.  The
+final term sets of the last timed step are checked against the committed
+full-sweep fixture digests.
 
 value = Σ over all rotations of the live terms entering the rotation / wall
-time of the step.  The JSON line also carries the merge kernel's roofline
-(algorithmic bytes per launch = 40 B x (terms in + children + terms out)
-over its HIP-event time; this sweep is latency-bound), the same kernel's
-roofline where it is bandwidth-bound (256 circuits of a 36-qubit chain with
-up to ~1e6 live terms each, 24 B per term), and the CPU baseline: the
-single-threaded oracle (a C restatement of the reference algorithm) on the
-same sweep.
+time of the step.  Algorithmic bytes of a rotation = 40 B x (terms entering +
+terms leaving it).  The JSON line carries that figure over the whole step
+(roofline_rotation), per kernel over its HIP-event time (the resident layer
+kernel and the per-step merge kernel; `roofline` is the one with more time),
+the merge kernel where it is bandwidth-bound (256 circuits of a 36-qubit
+chain with up to ~1e6 live terms each, 24 B per term), and two CPU
+baselines: the single-threaded oracle (a C restatement of the reference
+algorithm) on the same sweep, and the oracle in 16 processes on copies of the
+16 sweeps; both final term sets are compared with the GPU's bit for bit.
+HBM traffic per launch is not measured in this run (null); the rocprofv3
+FETCH_SIZE/WRITE_SIZE figures are kept beside the kernel profiles.
 """
 import argparse
 import json
@@ -47,11 +54,13 @@ def host_cpu() -> str:
     return "unknown"
 
 
-def sweep_gpu(workloads, timing=True, graphs=False):
+def sweep_gpu(workloads, timing=True, graphs=False, digests=False):
     """All θ_h instances in one batch: per layer one evolve! per instance, then
-    the dephasing channel.  Returns (seconds, total counters, <O> list, merge
-    kernel seconds, merge launches).  timing: HIP events around every merge
-    launch; graphs: replay captured layer sequences (not combined with timing)."""
+    the dephasing channel, then ⟨O⟩ of every instance.  Returns a dict:
+    seconds (wall), counters (summed), evs, and when timing is on the HIP-event
+    seconds and launch counts of the per-step merge kernel and of the resident
+    layer kernel with the resident launches' share of the counters; digests:
+    the final term sets' SHA-256 (taken after the clock stops)."""
     from pauliprop import PPCounters, SparsePauliBatch
     bt = SparsePauliBatch([w.N for w in workloads],
                           [[((p.z, p.x), c) for p, c in w.observable] for w in workloads])
@@ -71,8 +80,26 @@ def sweep_gpu(workloads, timing=True, graphs=False):
                 setattr(tot, f, getattr(tot, f) + getattr(c, f))
     evs = bt.expectation_values()
     dt = time.perf_counter() - t0
-    tm, nm = bt.merge_time()
-    return dt, tot, evs, tm, nm
+    out = {"seconds": dt, "counters": tot, "evs": evs, "resident_stats": bt.resident_stats()}
+    if timing:
+        out["merge"] = bt.merge_time()
+        out["resident_seconds"] = bt.resident_time()
+        out["resident_work"] = bt.resident_work()
+    if digests:
+        out["digests"] = [term_set_digest(*bt.arrays(b)) for b in range(len(workloads))]
+    return out
+
+
+def term_set_digest(z, x, c) -> str:
+    """SHA-256 over the key words, then the coefficient bit patterns (the
+    digest of the committed full-sweep fixture)."""
+    import hashlib
+    import numpy as np
+    h = hashlib.sha256()
+    h.update(np.ascontiguousarray(z).tobytes())
+    h.update(np.ascontiguousarray(x).tobytes())
+    h.update(np.ascontiguousarray(c).view(np.uint64).tobytes())
+    return h.hexdigest()
 
 
 def merge_bandwidth_regime(count=256):
@@ -365,13 +392,16 @@ def batched_with_cpu(max_depth=6, cap=1_000_000, budget_s=20.0, cpu=True, worker
 
 
 def sweep_cpu(workloads, budget_s):
+    """The oracle on the sweep, one θ_h at a time in order, until budget_s
+    seconds.  Returns (terms/s, θ_h done, term-rotations, seconds, <O> list,
+    final digests); the digests are taken outside the timed loop."""
     import pyoracle as po
     from pauliprop import compile_filter
     swept = 0
-    done = 0
-    t0 = time.perf_counter()
-    evs = []
+    dt = 0.0
+    evs, digs = [], []
     for w in workloads:
+        t0 = time.perf_counter()
         o = po.OracleSPV(w.N, [(p.z, p.x, c) for p, c in w.observable])
         f = compile_filter(w.truncation)
         for L in w.layers:
@@ -379,20 +409,75 @@ def sweep_cpu(workloads, budget_s):
             o.channel(w.channel[0], w.channel[1])
             swept += c.terms_swept
         evs.append(o.expectation(0))
-        done += 1
-        if time.perf_counter() - t0 > budget_s:
+        dt += time.perf_counter() - t0
+        digs.append(term_set_digest(*o.arrays()))
+        if dt > budget_s:
             break
-    dt = time.perf_counter() - t0
-    return swept / dt, done, swept, dt, evs
+    return swept / dt, len(evs), swept, dt, evs, digs
+
+
+_SWEEP_WS = None
+
+
+def _sweep_worker_init(barrier):
+    """All-cores CPU baseline worker: build the 16 workloads and load the
+    oracle, then wait for every worker so the timed span starts together."""
+    global _SWEEP_WS
+    import pyoracle  # noqa: F401
+    from pauliprop import workloads as wl
+    _SWEEP_WS = [wl.lattice121(t) for t in wl.theta_h_values()]
+    barrier.wait(timeout=300)
+
+
+def _sweep_job(k):
+    """One full 20-layer sweep of θ_h index k on the oracle.  Returns
+    (k, term-rotations, wall start, wall end, final digest)."""
+    import pyoracle as po
+    from pauliprop import compile_filter
+    w = _SWEEP_WS[k]
+    t0 = time.time()
+    o = po.OracleSPV(w.N, [(p.z, p.x, c) for p, c in w.observable])
+    f = compile_filter(w.truncation)
+    swept = 0
+    for L in w.layers:
+        swept += o.evolve([(p.z, p.x) for p in L.generators], L.angles, strict=f).terms_swept
+        o.channel(w.channel[0], w.channel[1])
+    t1 = time.time()
+    return k, swept, t0, t1, term_set_digest(*o.arrays())
+
+
+def sweep_cpu_all_cores(reps, workers=16):
+    """The sweep on every host core: `reps` copies of the 16 θ_h sweeps as
+    independent jobs handed to `workers` single-threaded oracle processes
+    (one job at a time each).  Returns (terms/s over the span from the first
+    job's start to the last job's end, jobs, term-rotations, span, digests by
+    θ_h index)."""
+    from concurrent.futures import ProcessPoolExecutor
+    import multiprocessing as mp
+    ctx = mp.get_context("spawn")
+    barrier = ctx.Barrier(workers)
+    jobs = [k for _ in range(reps) for k in range(16)]
+    with ProcessPoolExecutor(max_workers=workers, mp_context=ctx, initializer=_sweep_worker_init,
+                             initargs=(barrier,)) as ex:
+        res = list(ex.map(_sweep_job, jobs, chunksize=1))
+    tot = sum(r[1] for r in res)
+    span = max(r[3] for r in res) - min(r[2] for r in res)
+    digs = {}
+    for k, _, _, _, d in res:
+        assert digs.setdefault(k, d) == d, "oracle runs of one θ_h differ"
+    return tot / span, len(jobs), tot, span, digs
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-workers", type=int, default=16, help="processes of the all-cores CPU baseline")
+    ap.add_argument("--cpu-all-seconds", type=float, default=6.0,
+                    help="target wall seconds of the all-cores CPU baseline")
     ap.add_argument("--graphs", type=int, default=0, help="replay hipGraph-captured layer sequences (1/0)")
     ap.add_argument("--batched", action="store_true",
                     help="run only the batched workload's first 6 layers with its CPU baseline")
@@ -435,6 +520,8 @@ def main():
     ths = wl.theta_h_values()
     workloads = [wl.lattice121(t) for t in ths]
     S = 40  # bytes per term: 2 x 16 B keys + 8 B coefficient (65 <= N <= 128)
+    with open(os.path.join(ROOT, "tests", "golden", "lattice121_full.json")) as fh:
+        fixture = json.load(fh)
 
     graphs = bool(args.graphs)
     for i in range(args.warmup):
@@ -443,42 +530,81 @@ def main():
 
     tot_time = 0.0
     agg = PPCounters()
-    evs = None
+    evs = digs = None
     for i in range(args.steps):
         log(f"step {i}")
-        dt, cnt, evs, _, _ = sweep_gpu(workloads, timing=False, graphs=graphs)
-        log(f"  {dt:.3f} s, {cnt.terms_swept} term-rotations, {cnt.rotations} rotations")
-        tot_time += dt
+        r = sweep_gpu(workloads, timing=False, graphs=graphs, digests=(i == args.steps - 1))
+        cnt = r["counters"]
+        log(f"  {r['seconds']:.3f} s, {cnt.terms_swept} term-rotations, {cnt.rotations} rotations, "
+            f"resident launches/fallbacks/aborts {r['resident_stats']}")
+        tot_time += r["seconds"]
         for f, _ in cnt._fields_:
             setattr(agg, f, getattr(agg, f) + getattr(cnt, f))
-    # merge-kernel roofline: one more (untimed) sweep with HIP events around
-    # every merge launch on the batch's stream
-    _, pcnt, pevs, t_merge, n_merge = sweep_gpu(workloads, timing=True, graphs=False)
-    assert pevs == evs, "profiled sweep differs from the timed one"
-    log(f"  profiled sweep: merge kernel {t_merge:.3f} s over {n_merge} launches")
+        evs, digs = r["evs"], r.get("digests", digs)
+    # every final term set against the committed full-sweep fixture
+    bad = [k for k, (d, fx) in enumerate(zip(digs, fixture)) if d != fx["digest"]]
+    assert not bad, f"final term sets differ from the fixture at theta_h indices {bad}"
+    log("  final term sets of all 16 theta_h match the fixture digests")
 
-    value = agg.terms_swept / tot_time
-    merge_bytes = S * (pcnt.merge_in + pcnt.merge_out)
-    achieved = merge_bytes / t_merge / 1e9 if t_merge > 0 else None
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "merge_traffic_lattice121.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
+    # kernel rooflines: one more (untimed) sweep with HIP events around every
+    # resident layer launch and every per-step merge launch
+    pr = sweep_gpu(workloads, timing=True, graphs=False)
+    assert pr["evs"] == evs, "profiled sweep differs from the timed one"
+    pc = pr["counters"]
+    t_merge, n_merge = pr["merge"]
+    t_res = pr["resident_seconds"]
+    n_res = pr["resident_stats"][0]
+    r_rot, r_swept, r_out, r_rebal = pr["resident_work"]
+    log(f"  profiled sweep: resident kernel {t_res:.3f} s over {n_res} launches ({r_rot} rotations, "
+        f"{r_rebal} chunk rebalances), "
+        f"per-step merge kernel {t_merge:.3f} s over {n_merge} launches")
+
+    def kernel_roofline(kernel, nbytes, seconds, launches, rotations, note):
+        achieved = nbytes / seconds / 1e9 if seconds > 0 else None
+        return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None, "kernel": kernel,
+                "launches": launches, "rotations": rotations, "seconds": seconds,
+                "avg_launch_us": 1e6 * seconds / max(launches, 1), "bytes_per_launch": nbytes / max(launches, 1),
+                "bytes": note}
+
+    note = "S x (terms entering + terms leaving each rotation), S = 40 B"
+    k_res = kernel_roofline("kr_layer<2>", S * (r_swept + r_out), t_res, n_res, r_rot, note)
+    k_merge = kernel_roofline("kb_merge<2,512>", S * (pc.terms_swept - r_swept + pc.merge_out - r_out), t_merge,
+                              n_merge, pc.rotations - r_rot, note)
+    dominant, other = (k_res, k_merge) if t_res >= t_merge else (k_merge, k_res)
+    step_bytes = S * (agg.terms_swept + agg.merge_out) / args.steps
+    step_s = tot_time / args.steps
+    rotation = {"bound": "hbm", "achieved": step_bytes / step_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": step_bytes / step_s / 1e9 / HBM_PEAK_GBS, "bytes_per_step": step_bytes,
+                "bytes": note + ", over the whole step's wall time"}
 
     log("merge kernel in the bandwidth-bound regime (36-qubit chain, 256 circuits)")
     bw_gbs, bw_us, bw_bytes, bw_n = merge_bandwidth_regime()
     log(f"  {bw_gbs:.0f} GB/s algorithmic, {bw_us:.1f} us per launch over {bw_n} launches")
 
-    cpu = None
+    cpu = cpu_all = None
     if not args.no_cpu:
         log("cpu baseline (oracle, 1 thread)")
-        cv, done, cs, cdt, cevs = sweep_cpu(workloads, args.cpu_budget)
+        cv, done, cs, cdt, cevs, cdigs = sweep_cpu(workloads, args.cpu_budget)
+        for k, (a, b) in enumerate(zip(evs, cevs)):
+            assert abs(a - b) <= 1e-12, ("<O> mismatch vs oracle", k, a, b)
+        for k, d in enumerate(cdigs):
+            assert d == digs[k], f"final term set differs from the oracle at theta_h index {k}"
         cpu = {"value": cv, "unit": "terms/s", "cores": 1, "kind": "port", "host": host_cpu(),
-               "sample": f"the same sweep, first {done} of 16 theta_h values ({cs} term-rotations, {cdt:.1f} s)"}
-        for a, b in zip(evs, cevs):
-            assert abs(a - b) <= 1e-12, ("<O> mismatch vs oracle", a, b)
+               "sample": f"the same sweep, first {done} of 16 theta_h values ({cs} term-rotations, {cdt:.1f} s); "
+                         f"final term sets bit-identical to the GPU's"}
+        per_sweep = cdt * 16 / done
+        reps = max(1, int(round(args.cpu_all_seconds * args.cpu_workers / per_sweep)))
+        log(f"cpu baseline (oracle, {args.cpu_workers} processes, {reps} x 16 sweeps)")
+        av, jobs, at, span, adigs = sweep_cpu_all_cores(reps, args.cpu_workers)
+        for k, d in adigs.items():
+            assert d == digs[k], f"all-cores oracle differs from the GPU at theta_h index {k}"
+        cpu_all = {"value": av, "unit": "terms/s", "cores": args.cpu_workers, "kind": "port", "host": host_cpu(),
+                   "sample": f"{jobs} full theta_h sweeps ({reps} x the 16 values) over {args.cpu_workers} "
+                             f"single-threaded oracle processes, one sweep at a time each ({at} term-rotations, "
+                             f"{span:.1f} s from the first start to the last end)"}
 
+    value = agg.terms_swept / tot_time
     out = {
         "metric": "Pauli terms/s per gate sweep",
         "value": value,
@@ -495,18 +621,19 @@ def main():
         "config": {"workload": "lattice121_sweep: 121-qubit 11x11 open lattice, 20 layers ZZ(-pi/2) on 220 bonds "
                                "+ X(theta_h) on 121 sites, 16 theta_h in [0, pi/2], dephasing 5e-4, |c|<=1e-4, "
                                "weight<=12, observable mean Z",
-                   "rotations": agg.rotations, "term_rotations": agg.terms_swept, "graphs": graphs,
-                   "expectation_by_theta_h": evs},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
-                     "kernel": "kb_merge<2>", "launches": n_merge,
-                     "avg_launch_us": 1e6 * t_merge / max(n_merge, 1),
-                     "bytes_per_launch": merge_bytes / max(n_merge, 1)},
+                   "rotations": agg.rotations // args.steps, "term_rotations": agg.terms_swept // args.steps,
+                   "graphs": graphs, "expectation_by_theta_h": evs,
+                   "final_terms_by_theta_h": [f["n_by_layer"][-1] for f in fixture],
+                   "final_term_sets_match_fixture": True},
+        "roofline": dominant,
+        "roofline_other_kernel": other,
+        "roofline_rotation": rotation,
         "roofline_bandwidth_regime": {"bound": "hbm", "achieved": bw_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                       "frac": bw_gbs / HBM_PEAK_GBS, "kernel": "kb_merge<1,2048>",
                                       "workload": "36-qubit chain, 10 layers, 256 circuits in one batch, |c|<=1e-6",
                                       "launches": bw_n, "avg_launch_us": bw_us, "bytes_per_launch": bw_bytes},
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_all,
     }
     print(json.dumps(out), flush=True)
 

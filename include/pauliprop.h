@@ -273,6 +273,12 @@ int pp_set_resident(pp_handle* h, int mode, int cap_limit);
 int pp_batch_resident_stats(pp_batch* p, int64_t* launches, int64_t* fallbacks, int64_t* aborts);
 /* HIP-event time of resident launches while timing is on (pp_batch_set_timing). */
 int pp_batch_resident_time(pp_batch* p, double* seconds, int reset);
+/* The resident launches' share of the counters since creation (or the last
+ * reset): rotations, terms entering them, terms leaving their merges, and
+ * chunk rebalances (a cluster whose chunk would outgrow its LDS redistributes
+ * its terms and redoes the rotation). */
+int pp_batch_resident_work(pp_batch* p, int64_t* rotations, int64_t* swept, int64_t* merge_out, int64_t* rebalances,
+                           int reset);
 
 /* ---- Learned surrogate of ⟨O⟩(θ) -------------------------------------------
  * No counterpart in the reference: an ensemble of E MLPs

@@ -112,24 +112,26 @@ __device__ __forceinline__ void store_key(uint64_t* __restrict__ Z, uint64_t* __
 }
 
 // spv_kernels.jl:14-16: z-major, high word first.
+// Lexicographic order (z words from the top, then x words), as one borrow
+// chain over the 4W 32-bit limbs of (z, x) read as a single unsigned number
+// with z's top word most significant: a < b iff a - b borrows.
 template <int W>
 __device__ __forceinline__ bool key_lt(const Key<W>& a, const Key<W>& b) {
+    unsigned long long br = 0;
 #pragma unroll
-    for (int w = W - 1; w >= 0; --w)
-        if (a.z[w] != b.z[w]) return a.z[w] < b.z[w];
+    for (int w = 0; w < W; ++w) (void)__builtin_subcll(a.x[w], b.x[w], br, &br);
 #pragma unroll
-    for (int w = W - 1; w >= 0; --w)
-        if (a.x[w] != b.x[w]) return a.x[w] < b.x[w];
-    return false;
+    for (int w = 0; w < W; ++w) (void)__builtin_subcll(a.z[w], b.z[w], br, &br);
+    return br != 0;
 }
 template <int W>
 __device__ __forceinline__ bool key_le(const Key<W>& a, const Key<W>& b) { return !key_lt<W>(b, a); }
 template <int W>
 __device__ __forceinline__ bool key_eq(const Key<W>& a, const Key<W>& b) {
-    bool e = true;
+    uint64_t d = 0;
 #pragma unroll
-    for (int w = 0; w < W; ++w) e = e && a.z[w] == b.z[w] && a.x[w] == b.x[w];
-    return e;
+    for (int w = 0; w < W; ++w) d |= (a.z[w] ^ b.z[w]) | (a.x[w] ^ b.x[w]);
+    return d == 0;
 }
 
 // Highest differing bit of the combined key (x bits at [0, 64W), z bits above);

@@ -320,6 +320,7 @@ struct Batch {
     std::vector<double> res_boost;   // per vector: divisor of the starting fill after chunk overflows (this call)
     std::vector<char> res_off;       // per vector: per-step kernels for the rest of this call
     int32_t* d_rpart = nullptr;      // [B] vectors in the current resident launch
+    unsigned long long* d_rwork = nullptr;  // [4] rotations, swept terms, merge outputs, rebalances of committed launches
     std::vector<int32_t> h_rpart;
     double t_res = 0.0;             // HIP-event time of resident launches (timing mode)
     std::vector<std::pair<hipEvent_t, hipEvent_t>> res_ev;
@@ -329,7 +330,7 @@ struct Batch {
     ~Batch() {
         for (auto& b : ib) free_inst(b);
         free_win();
-        dfree(d_rctl); dfree(d_rroles); dfree(d_rtick); dfree(d_rpart);
+        dfree(d_rctl); dfree(d_rroles); dfree(d_rtick); dfree(d_rpart); dfree(d_rwork);
         for (auto& e : res_ev) {
             (void)hipEventDestroy(e.first);
             (void)hipEventDestroy(e.second);
@@ -458,6 +459,8 @@ struct Batch {
             d_rroles = dalloc<uint32_t>((size_t)kXcdCount * res_kx);
             d_rtick = dalloc<uint32_t>(kXcdCount);
             d_rpart = dalloc<int32_t>((size_t)B);
+            d_rwork = dalloc<unsigned long long>(4);
+            HIPCHK(hipMemsetAsync(d_rwork, 0, 4 * sizeof(unsigned long long), stream));
             h_roles.assign((size_t)kXcdCount * res_kx, kResIdle);
         }
         HIPCHK(hipHostMalloc((void**)&h_state, sizeof(SegState) * (size_t)B, hipHostMallocDefault));
@@ -851,7 +854,7 @@ struct Batch {
     // most 60% of a chunk (divided by the vector's overflow boost: a vector
     // whose chunks overflowed resumes with more workgroups).
     // false: the call runs on the per-step kernels (auto mode also declines
-    // calls of few steps and vectors that do not fit one XCD that way).
+    // calls of few steps and vectors that do not fit one XCD at 90% fill).
     bool res_plan(const std::vector<StepDev>& steps, int64_t nsteps) {
         if (res_mode == 0 || res_broken || res_kx == 0) return false;
         if (res_mode == 1 && nsteps < 8) return false;
@@ -874,7 +877,11 @@ struct Batch {
             nparts += part[(size_t)b];
         }
         if (nparts > kXcdCount * res_kx) return false;
-        auto fits = [&](int b) { return n_known[b] <= (int64_t)K[b] * base; };
+        // a vector held to fewer workgroups than it asked for (the XCD's
+        // limit) still runs resident with its chunks up to 90% full: the
+        // largest vectors of a sweep grow little within a call
+        const int64_t tight = std::max<int64_t>(1, (int64_t)(0.9 * cap));
+        auto fits = [&](int b) { return n_known[b] <= (int64_t)K[b] * tight; };
         for (int b = 0; b < B; ++b) {
             if (!part[(size_t)b]) continue;
             const int64_t want = cdiv(std::max<int64_t>(n_known[b], 1), fill[(size_t)b]);
@@ -957,7 +964,7 @@ struct Batch {
                 hipLaunchKernelGGL((kr_layer<Wt>), grid, block, 0, stream, a);
             }
             hipLaunchKernelGGL(kr_commit, dim3((unsigned)cdiv(B, 256)), dim3(256), 0, stream, (const ResCtl*)d_rctl,
-                               (const int32_t*)d_rpart, d_state, B, first);
+                               (const int32_t*)d_rpart, d_state, B, first, d_rwork);
         };
         res_launches += 1;
         if (!graphs || timing) {
@@ -2666,6 +2673,24 @@ int pp_batch_resident_time(pp_batch* p, double* seconds, int reset) {
     p->bt.sync_state();
     *seconds = p->bt.t_res;
     if (reset) p->bt.t_res = 0.0;
+    API_END
+}
+
+int pp_batch_resident_work(pp_batch* p, int64_t* rotations, int64_t* swept, int64_t* merge_out, int64_t* rebalances,
+                           int reset) {
+    if (!p || !rotations || !swept || !merge_out || !rebalances) return fail(PP_EARG, "null argument");
+    API_BEGIN
+    Batch& bt = p->bt;
+    unsigned long long w[4] = {0, 0, 0, 0};
+    if (bt.d_rwork) {
+        HIPCHK(hipMemcpyAsync(w, bt.d_rwork, sizeof(w), hipMemcpyDeviceToHost, bt.stream));
+        if (reset) HIPCHK(hipMemsetAsync(bt.d_rwork, 0, sizeof(w), bt.stream));
+        HIPCHK(hipStreamSynchronize(bt.stream));
+    }
+    *rotations = (int64_t)w[0];
+    *swept = (int64_t)w[1];
+    *merge_out = (int64_t)w[2];
+    *rebalances = (int64_t)w[3];
     API_END
 }
 

@@ -2,7 +2,7 @@
 // layer call of evolve!, spv_evolve.jl:312-387 at window = 1) for every
 // vector of a batch, with the vector's terms resident in LDS.
 //
-// Each vector is served by a cluster of K <= 32 workgroups of 1024 threads on
+// Each vector is served by a cluster of K <= 32 workgroups of 512 threads on
 // one XCD; workgroup k of the cluster holds a contiguous, sorted chunk of the
 // live set in its LDS.  A rotation is three phases separated by two cluster
 // barriers:
@@ -16,10 +16,13 @@
 //      rank formula as kb_emit, with chunks in place of tiles) into the
 //      vector's child buffer, counting children per owning chunk (the chunk
 //      whose key range holds the child);
-//   3. after barrier 2: each member loads the children of its key range and
-//      merges them into its chunk in LDS (live term first on ties, cos-scaling
-//      of anticommuting live terms, the strict filter on every output,
-//      spv_kernels.jl:326-386).  Its outputs stay in LDS as its next chunk.
+//   3. after barrier 2: every member sums the whole per-owner table, so all
+//      of them know whether any chunk would outgrow its LDS before one of
+//      them merges; then each loads the children of its key range and merges
+//      them into its chunk in place (live term first on ties, cos-scaling of
+//      anticommuting live terms, the strict filter on every output,
+//      spv_kernels.jl:326-386): every thread takes its merge-path share into
+//      registers, and writes it back after the barrier that ends the reads.
 // The reference's capacity bookkeeping (precheck, early merges, append and
 // live lengths, WindowCounters) is replayed from the records exactly as the
 // per-step kernels do it.
@@ -36,10 +39,10 @@
 // the vector's control block and applied by kr_commit only when every member
 // finished without an abort, so an aborted launch leaves the vector exactly as
 // it was (the input buffer is never written) and the host replays the call
-// with the per-step kernels.  A chunk that outgrows its LDS (or a child/live
-// buffer that is too small) stops the vector at that rotation with its state
-// before it (each chunk is double-buffered in LDS: the merge writes the other
-// buffer); the host then finishes the call from that rotation.
+// with the per-step kernels.  A chunk that would outgrow its LDS (or a
+// child/live buffer that is too small) stops the vector before that rotation,
+// with its state unchanged; the host then finishes the call from that
+// rotation.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -50,21 +53,24 @@
 namespace pp {
 
 constexpr int kXcdCount = 8;  // XCDs of an MI355X
-constexpr int kResThreads = 1024;
-constexpr int kResWaves = kResThreads / kWave;  // 16
+// 512 threads: two waves per SIMD, so each lane may hold 256 registers (the
+// merge keeps E terms per thread in registers; spills would go to L2)
+constexpr int kResThreads = 512;
+constexpr int kResWaves = kResThreads / kWave;  // 8
 constexpr int kResMaxK = 32;                    // workgroups per vector (one XCD)
 constexpr int kResLevels = 4;                   // mask levels (more: the per-step kernels)
 constexpr int kResWords = 64;                   // chunk positions in 64-term words
-// Chunk capacity in terms: two LDS buffers of 16 W + 8 bytes per term (the
-// chunk with its incoming children, and the merge output), and the positions
-// each thread covers (capacity <= E * kResThreads).
+// Chunk capacity in terms: one LDS buffer of 16 W + 8 bytes per term holding
+// the chunk with its incoming children (the merge writes it in place), and
+// the positions each thread covers (capacity <= E * kResThreads, and at most
+// 64 positions per flag word).
 template <int W>
 struct ResGeom {
-    static constexpr int cap = W == 1 ? 3072 : 1856;
-    static constexpr int E = W == 1 ? 3 : 2;
+    static constexpr int cap = W == 1 ? 4096 : 3712;
+    static constexpr int E = 8;
 };
 static_assert(ResGeom<1>::cap <= ResGeom<1>::E * kResThreads && ResGeom<2>::cap <= ResGeom<2>::E * kResThreads &&
-                  ResGeom<1>::cap <= 64 * kResWords,
+                  ResGeom<1>::cap <= 64 * kResWords && ResGeom<2>::cap <= 64 * kResWords,
               "chunk geometry");
 constexpr uint32_t kResAbort = 1u << 30;  // barrier counter bit: a member gave up
 constexpr uint32_t kResSpin = 1u << 21;   // polls before giving up (~seconds)
@@ -78,7 +84,7 @@ constexpr uint32_t kResIdle = ~0u;        // role of a workgroup with no vector
 //   52-53 heads (0, 1, 2 = two or more), 54 first head's hdb == level,
 //   55 last head's hdb == level.
 struct alignas(16) ResRec {
-    int32_t n, a, ovf, pad;
+    int32_t n, a, pad0, pad1;
     uint64_t fz[2], fx[2], lz[2], lx[2];
     uint64_t lv[kResLevels];
 };
@@ -89,11 +95,12 @@ struct ResCtl {
     uint32_t bar;        // barrier arrivals (+ kResAbort)
     int32_t done;        // member 0 staged its results
     int32_t wrote;       // the output buffer holds the new live set
-    int32_t stall;       // SegState::stalled code (0, 1, 2, 3 = chunk overflow)
+    int32_t stall;       // SegState::stalled code (0, 1, 2, 3 = a chunk would outgrow its LDS)
     int32_t err;
     int32_t pad;
     int64_t n_final, stall_step, need;
     int64_t rotations, swept, created, merge_in, merge_out, early, live_cap_v, app_cap_v;
+    int64_t rebalances;  // chunk rebalances of the launch
     ResRec rec[2][kResMaxK];
     int32_t hpre[2][kResMaxK][kResMaxK + 1];  // per member: exclusive prefix of its children per owner
 };
@@ -152,6 +159,31 @@ __device__ __forceinline__ int wave_incl_max(int v) {  // inclusive prefix maxim
 __device__ __forceinline__ int wave_shr1(int v) { return dpp_src<0x138, 0xf>(v); }  // lane l gets lane l - 1 (0 at lane 0)
 __device__ __forceinline__ int lane_of(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 __device__ __forceinline__ int wave_total(int v) { return lane_of(wave_incl(v), kWave - 1); }
+
+// Workgroup-uniform values read from LDS, moved to scalar registers (the
+// vector registers go to the per-thread terms).  Uniform control flow only.
+__device__ __forceinline__ int res_uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t res_uni(uint64_t v) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ int64_t res_uni(int64_t v) { return (int64_t)res_uni((uint64_t)v); }
+__device__ __forceinline__ double res_uni(double v) {
+    return __longlong_as_double((long long)res_uni((uint64_t)__double_as_longlong(v)));
+}
+__device__ __forceinline__ FilterReg res_uni(const FilterReg& f) {
+    FilterReg r;
+    r.wmax = res_uni(f.wmax);
+    r.xwmax = res_uni(f.xwmax);
+    r.mwmax = res_uni(f.mwmax);
+    r.thresh = res_uni(f.thresh);
+    r.dthresh_w = res_uni(f.dthresh_w);
+    r.dthresh_xw = res_uni(f.dthresh_xw);
+    r.exptab_w = reinterpret_cast<const double*>(res_uni((uint64_t)f.exptab_w));
+    r.exptab_xw = reinterpret_cast<const double*>(res_uni((uint64_t)f.exptab_xw));
+    return r;
+}
 
 // Data other members wrote during the launch (records, per-owner counts,
 // children) is read with agent-scope relaxed atomic loads: they miss in the
@@ -462,8 +494,8 @@ template <int W>
 __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
     constexpr int Cap = ResGeom<W>::cap;
     constexpr int E = ResGeom<W>::E;
-    __shared__ Key<W> s_k[2][Cap];
-    __shared__ double s_c[2][Cap];
+    __shared__ Key<W> s_k[Cap];
+    __shared__ double s_c[Cap];
     __shared__ uint64_t s_am[kResWords];
     __shared__ int32_t s_apre[kResWords];
     __shared__ uint64_t s_hm[kResLevels][kResWords];
@@ -471,15 +503,15 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
     __shared__ int16_t s_lb[kResLevels][kResWords];
     __shared__ int16_t s_fa[kResLevels][kResWords];
     __shared__ ResRec s_rec[kResMaxK];
-    __shared__ int32_t s_nprev[kResMaxK];
+    __shared__ int32_t s_mk[kResMaxK + 1];  // children per member; [K]: this member's first rank
+    __shared__ int32_t s_rbw[kResMaxK], s_rbn[kResMaxK], s_rbp[kResMaxK + 1];  // rebalance prefixes, bounds
     __shared__ int32_t s_edge[kResLevels][8];
     __shared__ int32_t s_hist[kResMaxK + 1];
     __shared__ Key<W> s_ownk[kResMaxK];
     __shared__ int32_t s_owni[kResMaxK];
     __shared__ int32_t s_wsum[kResWaves];
-    __shared__ int32_t s_int[8];   // role, slot, error bits, barrier result, owners
-    __shared__ int32_t s_tot[4];   // n_tot, m_tot, A before this chunk, any overflow
-    __shared__ int64_t s_r[2];
+    __shared__ int32_t s_int[8];   // role, slot, error bits, barrier result, owners, overflow
+    __shared__ int32_t s_tot[3];   // n_tot, m_tot, A before this chunk
     __shared__ int64_t s_bk[kBk];
     __shared__ FilterReg s_fs, s_fl;
     __shared__ ResStep s_step[2];
@@ -523,7 +555,7 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
         res_step_commit(A, b, res_step_load(A, b, A.first), &s_step[0]);
     }
     int32_t* s_err = &s_int[2];
-    if (tid < kResWords) {  // words past E * 16 stay zero
+    if (tid < kResWords) {  // every flag word starts at zero
         s_am[tid] = 0ull;
         for (int li = 0; li < kResLevels; ++li) s_hm[li][tid] = s_em[li][tid] = 0ull;
     }
@@ -533,6 +565,7 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
             ctl->wrote = 0;
             ctl->stall = 0;
             ctl->rotations = ctl->swept = ctl->created = ctl->merge_in = ctl->merge_out = ctl->early = 0;
+            ctl->rebalances = 0;
             ctl->live_cap_v = st->live_cap_v;
             ctl->app_cap_v = st->app_cap_v;
             ctl->n_final = n0;
@@ -542,7 +575,7 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
     }
 
     // initial chunk: [k n0 / K, (k + 1) n0 / K)
-    int nj = 0, cb = 0;  // chunk size, current LDS buffer
+    int nj = 0;  // chunk size
     int stall = 0;
     int64_t stall_step = A.first, need = 0;
     {
@@ -555,45 +588,57 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
             const uint64_t* Lx = cur ? dp->Lx[1] : dp->Lx[0];
             const double* Lc = cur ? dp->Lc[1] : dp->Lc[0];
             for (int p = tid; p < nj; p += kResThreads) {
-                s_k[0][p] = load_key<W>(Lz, Lx, lo + p);
-                s_c[0][p] = ldg(Lc, lo + p);
+                s_k[p] = load_key<W>(Lz, Lx, lo + p);
+                s_c[p] = ldg(Lc, lo + p);
             }
         }
     }
     bool pend = false;
-    int64_t p_step = 0;
-    int ovf = 0;
-    bool prev_buf = false, aborted = false;  // prev_buf: the final state is the previous buffer
+    bool aborted = false;
+    int64_t rb_step = -1;  // step of the last rebalance
+    int nrb = 0;           // rebalances
+    int e_dirty = 0;  // flag words at e * kResWaves + wave, e < e_dirty, may be nonzero
     uint32_t nbar = 0;
     int par = 0, cs = 0;  // record parity, step slot
 
     while (stall == 0) {
         __syncthreads();  // the chunk and the prefetched step are complete
         const ResStep& S = s_step[cs];
-        const int64_t s = S.s;
+        const int64_t s = res_uni(S.s);
         const bool last = s >= A.nsteps;
         GenReg<W> G;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            G.gz[w] = S.gz[w];
-            G.gx[w] = S.gx[w];
+            G.gz[w] = res_uni(S.gz[w]);
+            G.gx[w] = res_uni(S.gx[w]);
         }
-        G.cosv = S.cosv;
-        G.sinv = S.sinv;
-        G.ng = S.ng;
-        const uint32_t lev4 = S.lev4;
-        const int precheck = S.precheck;
-        int nlev = last ? 0 : S.nlev;
+        G.cosv = res_uni(S.cosv);
+        G.sinv = res_uni(S.sinv);
+        G.ng = res_uni(S.ng);
+        const uint32_t lev4 = (uint32_t)res_uni((int)S.lev4);
+        const int precheck = res_uni(S.precheck);
+        int nlev = last ? 0 : res_uni(S.nlev);
         if (nlev > kResLevels) {  // the host never sends these
             if (tid == 0) atomicOr(s_err, 8);
             nlev = kResLevels;
         }
-        const Key<W>* sk = s_k[cb];
-        const double* sc = s_c[cb];
+        Key<W>* sk = s_k;
+        double* sc = s_c;
         // ---- phase 1: flags, run heads, summaries, record
+        const int e_need = (nj + kResThreads - 1) / kResThreads;  // position rows in use
         if (!last) {
+            if (e_dirty > e_need) {  // clear the rows the chunk no longer reaches
+                const int w0 = e_need * kResWaves, nw = (e_dirty - e_need) * kResWaves;
+                for (int q = tid; q < nw * (1 + 2 * kResLevels); q += kResThreads) {
+                    const int arr = q / nw, w = w0 + q % nw;
+                    if (arr == 0) s_am[w] = 0ull;
+                    else if (arr <= kResLevels) s_hm[arr - 1][w] = 0ull;
+                    else s_em[arr - 1 - kResLevels][w] = 0ull;
+                }
+            }
+            e_dirty = e_need;
 #pragma unroll 1
-            for (int e = 0; e < E; ++e) {
+            for (int e = 0; e < e_need; ++e) {
                 const int p = e * kResThreads + tid;
                 const int wi = e * kResWaves + wave;
                 const bool valid = p < nj;
@@ -603,7 +648,7 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
                     const Key<W> Kp = sk[p];
                     flag = G.anti(Kp);
                     if (flag && local_active) {
-                        const FilterReg fl = s_fl;
+                        const FilterReg fl = s_fl;  // (divergent here: stays in vector registers)
                         Key<W> ch;
                         const double cn = G.child(Kp, sc[p], ch);
                         flag = !should_drop<W>(fl, ch, fabs(cn));
@@ -641,11 +686,9 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
                 if (K == 1) {
                     myrec->n = nj;
                     myrec->a = a;
-                    myrec->ovf = ovf;
                 } else {
                     stg(&grec->n, 0, nj);
                     stg(&grec->a, 0, a);
-                    stg(&grec->ovf, 0, ovf);
                 }
             }
             if (lane < 2 * W && nj > 0) {  // first and last keys
@@ -676,9 +719,6 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
                 aborted = true;
                 break;
             }
-            // the previous sizes are kept for a rollback to the previous buffer
-            if (tid < kResMaxK) s_nprev[tid] = tid < K ? s_rec[tid].n : 0;
-            __syncthreads();
             if (tid < K * 7) {  // 7 x 16 bytes per record
                 const int r = tid / 7, q = tid % 7;
                 const uint64_t* src = reinterpret_cast<const uint64_t*>(&ctl->rec[par][r]) + 2 * q;
@@ -693,14 +733,12 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
         // the owner table (last wave)
         if (wave == 0) {
             const bool in = lane < K;
-            const int ni = in ? s_rec[lane].n : 0, ai = in ? s_rec[lane].a : 0, oi = in ? s_rec[lane].ovf : 0;
+            const int ni = in ? s_rec[lane].n : 0, ai = in ? s_rec[lane].a : 0;
             const int nt = wave_total(ni), mt = wave_total(ai), ab = wave_total(lane < k ? ai : 0);
-            const int ov = wave_total(oi);
             if (lane == 0) {
                 s_tot[0] = nt;
                 s_tot[1] = mt;
                 s_tot[2] = ab;
-                s_tot[3] = ov;
             }
         } else if (!last && wave <= nlev) {
             const int li = wave - 1;
@@ -719,17 +757,11 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
                 s_owni[pos] = lane;
             }
             if (lane == 0) s_int[4] = popc64(nem);
-            if (lane <= kResMaxK) s_hist[lane] = 0;
+            if (lane <= kResMaxK) s_hist[lane] = s_mk[lane] = 0;
         }
         __syncthreads();
         const int64_t n_tot = s_tot[0], m_tot = s_tot[1], abase = s_tot[2];
         if (pend) {
-            if (s_tot[3]) {  // a chunk outgrew its LDS at the pending rotation
-                stall = 3;
-                stall_step = p_step;
-                prev_buf = true;
-                break;
-            }
             if (tid == 0) {  // confirm the pending rotation (kb_merge's bookkeeping)
                 const int64_t pn = s_bk[kBkPn], pm = s_bk[kBkPm];
                 s_bk[kBkRot] += 1;
@@ -769,8 +801,8 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
             stall_step = s;
             break;
         }
-        Key<W>* skw = s_k[cb];  // children are staged after the chunk
-        double* scw = s_c[cb];
+        Key<W>* skw = s_k;  // children are staged after the chunk
+        double* scw = s_c;
         // ---- phase 2: children at their ranks (and the next rotation's step)
         ResStepLoad nxt;
         if (tid == kResThreads - 1) nxt = res_step_load(A, b, s + 1);
@@ -781,7 +813,7 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
             uint64_t* Cx = dp->Cx;
             double* Cc = dp->Cc;
 #pragma unroll 1
-            for (int e = 0; e < E; ++e) {
+            for (int e = 0; e < e_need; ++e) {
                 const int p = e * kResThreads + tid;
                 const int wi = e * kResWaves + wave;
                 if (p >= nj || !((s_am[wi] >> lane) & 1ull)) continue;
@@ -854,6 +886,9 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
                 }
             }
         }
+        // the next rotation's step (its loads were issued at the start of
+        // phase 2); the barrier at the top of the loop publishes it
+        if (tid == kResThreads - 1) res_step_commit(A, b, nxt, &s_step[cs ^ 1]);
         int mk = (int)m_tot;
         if (K > 1) {
             __syncthreads();
@@ -868,29 +903,109 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
                 aborted = true;
                 break;
             }
+            // every member's incoming children from the whole per-owner table,
+            // so that all members see the same overflow decision before any
+            // of them merges
+            for (int q = tid; q < K * K; q += kResThreads) {
+                const int i = q / K, j = q % K;
+                const int v0 = ld_l2(&ctl->hpre[par][i][j]), v1 = ld_l2(&ctl->hpre[par][i][j + 1]);
+                atomicAdd(&s_mk[j], v1 - v0);
+                if (j == k) atomicAdd(&s_mk[kResMaxK], v0);
+            }
+            __syncthreads();
             if (wave == 0) {
-                int v0 = 0, v1 = 0;
-                if (lane < K) {
-                    v0 = ld_l2(&ctl->hpre[par][lane][k]);
-                    v1 = ld_l2(&ctl->hpre[par][lane][k + 1]);
-                }
-                v0 = wave_total(v0);
-                v1 = wave_total(v1);
+                const int mj = lane < K ? s_mk[lane] : 0;
+                const bool over = lane < K && (mj < 0 || s_rec[lane].n + mj > cap);
+                const uint64_t ob = __ballot(over);
+                const int msum = wave_total(mj);
                 if (lane == 0) {
-                    s_r[0] = v0;
-                    s_r[1] = v1;
+                    const int R0 = s_mk[kResMaxK];
+                    int bad = ob != 0ull ? 1 : 0;
+                    if (msum != (int)m_tot || R0 < 0 || R0 + s_mk[k] > (int)m_tot) {  // never expected
+                        atomicOr(s_err, 2);
+                        bad = 2;
+                    }
+                    s_int[5] = bad;
                 }
             }
             __syncthreads();
-            const int64_t R0 = s_r[0], R1 = s_r[1];
-            mk = (int)(R1 - R0);
-            if (R0 < 0 || R1 < R0 || R1 > m_tot) {  // never expected
-                if (tid == 0) atomicOr(s_err, 2);
-                mk = -1;
+            if (s_int[5]) {  // a chunk would outgrow its LDS
+                // once per step, when the cluster has room: new chunk bounds
+                // that share live terms plus incoming children evenly (the
+                // children taken as spread evenly over each old chunk), the
+                // chunks restaged through the spare live buffer, and rotation
+                // s redone
+                bool rebalanced = false;
+                if (s_int[5] == 1 && rb_step != s && (n_tot + m_tot) * 16 <= (int64_t)K * cap * 15) {
+                    if (wave == 0) {
+                        const bool in = lane < K;
+                        const int nl = in ? s_rec[in ? lane : 0].n : 0, wl = in ? nl + s_mk[lane] : 0;
+                        const int pw = wave_incl(wl) - wl, pn = wave_incl(nl) - nl;
+                        if (in) {
+                            s_rbw[lane] = pw;
+                            s_rbn[lane] = pn;
+                        }
+                    }
+                    __syncthreads();
+                    if (wave == 0) {
+                        int P = 0;
+                        if (lane <= K) {
+                            const int64_t Wt = n_tot + m_tot;
+                            if (lane == K) {
+                                P = (int)n_tot;
+                            } else if (lane > 0) {
+                                const int64_t T = (int64_t)lane * Wt / K;
+                                int j = -1;
+                                for (int i = 0; i < K; ++i)
+                                    if (s_rec[i].n > 0 && s_rbw[i] <= T) j = i;
+                                if (j >= 0) {
+                                    const int64_t nj_ = s_rec[j].n, wj = nj_ + s_mk[j];
+                                    int64_t u = ((T - s_rbw[j]) * nj_ + wj - 1) / wj;
+                                    if (u > nj_) u = nj_;
+                                    P = s_rbn[j] + (int)u;
+                                }
+                            }
+                            s_rbp[lane] = P;
+                        }
+                        const int nxtP = __shfl_down(P, 1, kWave);
+                        const bool fitsk = lane >= K || (nxtP - P >= 0 && nxtP - P <= cap);
+                        if (lane == 0) s_int[6] = __ballot(!fitsk) == 0ull ? 1 : 0;
+                    }
+                    __syncthreads();
+                    if (s_int[6]) {
+                        uint64_t* Oz = cur ? dp->Lz[0] : dp->Lz[1];
+                        uint64_t* Ox = cur ? dp->Lx[0] : dp->Lx[1];
+                        double* Oc = cur ? dp->Lc[0] : dp->Lc[1];
+                        const int off = s_rbn[k];
+                        for (int p = tid; p < nj; p += kResThreads) {
+                            store_key<W>(Oz, Ox, off + p, sk[p]);
+                            stg(Oc, off + p, sc[p]);
+                        }
+                        nbar += 1;
+                        if (!res_barrier(&ctl->bar, nbar * (uint32_t)K, &s_int[3])) {
+                            aborted = true;
+                            break;
+                        }
+                        const int P0 = s_rbp[k], P1 = s_rbp[k + 1];
+                        for (int p = tid; p < P1 - P0; p += kResThreads) {
+                            sk[p] = load_key_l2<W>(Oz, Ox, P0 + p);
+                            sc[p] = ld_l2(Oc + P0 + p);
+                        }
+                        nj = P1 - P0;
+                        rb_step = s;
+                        nrb += 1;
+                        par ^= 1;
+                        rebalanced = true;
+                    }
+                }
+                if (rebalanced) continue;
+                stall = 3;
+                stall_step = s;
+                break;
             }
-            if (mk < 0 || nj + mk > cap) {
-                ovf = 1;  // published with the next record: the cluster rolls back
-            } else {
+            mk = s_mk[k];
+            {
+                const int64_t R0 = s_mk[kResMaxK];
                 const uint64_t* Cz = dp->Cz;
                 const uint64_t* Cx = dp->Cx;
                 const double* Cc = dp->Cc;
@@ -901,15 +1016,17 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
             }
         }
         __syncthreads();
-        // ---- phase 3: merge chunk [0, nj) with children [nj, nj + mk) into the other buffer
-        if (!ovf) {
-            const FilterReg fs = s_fs;
+        // ---- phase 3: merge chunk [0, nj) with children [nj, nj + mk) in
+        // place: each thread holds its outputs in registers across the barrier
+        // that ends every read of the buffer
+        {
+            const FilterReg fs = res_uni(s_fs);
             const int tot = nj + mk;
             const Key<W>* sL = sk;
             const Key<W>* sC = sk + nj;
-            const double* sLc = sc;
-            const double* sCc = sc + nj;
-            const int dk = tid * E < tot ? tid * E : tot;
+            // outputs per thread: the chunk spread over every wave
+            const int Ee = (tot + kResThreads - 1) / kResThreads;
+            const int dk = tid * Ee < tot ? tid * Ee : tot;
             int lo = dk - mk > 0 ? dk - mk : 0;
             int hi = dk < nj ? dk : nj;
             while (lo < hi) {
@@ -918,33 +1035,44 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
                 else hi = mid;
             }
             int ia = lo, jb = dk - lo;
-            int src[E];
+            const int stop = dk + Ee < tot ? dk + Ee : tot;
+            // a child equal to the live term before this share was absorbed
+            // by it (in the previous share); inside a share an equal child is
+            // consumed with its live term
+            bool skip_first = false;
+            if (dk < stop && ia > 0 && jb < mk) skip_first = key_eq<W>(sL[ia - 1], sC[jb]);
+            Key<W> outK[E];
             double outC[E];
+            uint32_t keep = 0;
             int cnt = 0;
-            const int stop = dk + E < tot ? dk + E : tot;
+            const int lmax = Cap - 1;
 #pragma unroll
             for (int e = 0; e < E; ++e) {
-                src[e] = -1;
                 outC[e] = 0.0;
-                if (dk + e >= stop) continue;
-                const bool takeL = ia < nj && (jb >= mk || key_le<W>(sL[ia], sC[jb]));
-                int q;
-                double acc;
-                bool emit = true;
-                if (takeL) {
-                    q = ia;
-                    acc = sLc[ia];
-                    if (G.anti(sL[ia])) acc = acc * G.cosv;
-                    if (jb < mk && key_eq<W>(sC[jb], sL[ia])) acc = acc + sCc[jb];  // the equal-key child follows
-                    ++ia;
-                } else {
-                    q = nj + jb;
-                    acc = sCc[jb];
-                    if (ia > 0 && key_eq<W>(sL[ia - 1], sC[jb])) emit = false;  // absorbed by the live term
-                    ++jb;
+                if (ia + jb >= stop) break;
+                const bool haveL = ia < nj, haveC = jb < mk;
+                const int il = ia < lmax ? ia : lmax;
+                const int ic = nj + jb < lmax ? nj + jb : lmax;
+                const Key<W> a = sk[il], c = sk[ic];
+                const double ca = sc[il], cc = sc[ic];
+                const bool takeL = haveL & (!haveC | key_le<W>(a, c));
+                const bool eq = takeL & haveC & key_eq<W>(a, c);
+                Key<W> kv;
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    kv.z[w] = takeL ? a.z[w] : c.z[w];
+                    kv.x[w] = takeL ? a.x[w] : c.x[w];
                 }
-                if (emit && !should_drop<W>(fs, sk[q], fabs(acc))) {
-                    src[e] = q;
+                double acc = ca;
+                if (G.anti(a)) acc = acc * G.cosv;
+                if (eq) acc = acc + cc;  // the equal-key child follows its live term
+                if (!takeL) acc = cc;
+                const bool emit = !(e == 0 && skip_first && !takeL);
+                ia += takeL ? 1 : 0;
+                jb += (!takeL | eq) ? 1 : 0;
+                outK[e] = kv;
+                if (emit && !should_drop<W>(fs, kv, fabs(acc))) {
+                    keep |= 1u << e;
                     outC[e] = acc;
                     ++cnt;
                 }
@@ -957,28 +1085,21 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
             const int wpref = lane_of(wincl - wv, wave);
             const int total = lane_of(wincl, kResWaves - 1);
             int o = wpref + incl - cnt;
-            Key<W>* dk_ = s_k[cb ^ 1];
-            double* dc_ = s_c[cb ^ 1];
 #pragma unroll
             for (int e = 0; e < E; ++e) {
-                if (src[e] >= 0) {
-                    dk_[o] = sk[src[e]];
-                    dc_[o] = outC[e];
+                if ((keep >> e) & 1u) {
+                    sk[o] = outK[e];
+                    sc[o] = outC[e];
                     ++o;
                 }
             }
             nj = total;
         }
-        // the next rotation's step (its loads were issued in phase 2); the
-        // barrier at the top of the loop publishes it
-        if (tid == kResThreads - 1) res_step_commit(A, b, nxt, &s_step[cs ^ 1]);
-        cb ^= 1;  // (an overflowed chunk's state stays in the previous buffer)
         if (tid == 0) {
             s_bk[kBkPn] = n_tot;
             s_bk[kBkPm] = m_tot;
         }
         pend = true;
-        p_step = s;
         par ^= 1;
         cs ^= 1;
     }
@@ -989,15 +1110,14 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
     const bool write = s_bk[kBkRot] > 0;
     if (write) {
         int64_t off = 0;
-        for (int i = 0; i < k; ++i) off += prev_buf ? s_nprev[i] : s_rec[i].n;
-        const int mine = prev_buf ? s_nprev[k] : nj;
-        const int src = prev_buf ? cb ^ 1 : cb;
+        for (int i = 0; i < k; ++i) off += s_rec[i].n;
+        const int mine = nj;
         uint64_t* Oz = cur ? dp->Lz[0] : dp->Lz[1];
         uint64_t* Ox = cur ? dp->Lx[0] : dp->Lx[1];
         double* Oc = cur ? dp->Lc[0] : dp->Lc[1];
         for (int p = tid; p < mine; p += kResThreads) {
-            store_key<W>(Oz, Ox, off + p, s_k[src][p]);
-            stg(Oc, off + p, s_c[src][p]);
+            store_key<W>(Oz, Ox, off + p, s_k[p]);
+            stg(Oc, off + p, s_c[p]);
         }
     }
     if (tid == 0) {
@@ -1006,7 +1126,7 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
     }
     if (k == 0 && tid == 0) {
         int64_t n_final = 0;
-        for (int i = 0; i < K; ++i) n_final += prev_buf ? s_nprev[i] : s_rec[i].n;
+        for (int i = 0; i < K; ++i) n_final += s_rec[i].n;
         ctl->wrote = write ? 1 : 0;
         ctl->stall = stall;
         ctl->stall_step = stall_step;
@@ -1020,6 +1140,7 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
         ctl->early = s_bk[kBkEarly];
         ctl->live_cap_v = s_bk[kBkLive];
         ctl->app_cap_v = s_bk[kBkApp];
+        ctl->rebalances = nrb;
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __hip_atomic_store(&ctl->done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -1039,8 +1160,11 @@ __global__ void kr_reset(ResCtl* __restrict__ ctl, uint32_t* __restrict__ ticket
 // After a resident launch: apply the staged results of every vector whose
 // cluster finished; a vector whose launch was aborted (or never ran) is left
 // as it was and marked for replay from the call's first step (code 4).
+// work[0..3] accumulate the rotations, swept terms and merge outputs of the
+// committed launches (the resident kernel's share of the counters) and their
+// chunk rebalances.
 __global__ void kr_commit(const ResCtl* __restrict__ ctl, const int32_t* __restrict__ part, SegState* __restrict__ S,
-                          int B, int64_t first) {
+                          int B, int64_t first, unsigned long long* __restrict__ work) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= B || !part[b]) return;
     const ResCtl* c = ctl + b;
@@ -1064,6 +1188,10 @@ __global__ void kr_commit(const ResCtl* __restrict__ ctl, const int32_t* __restr
     s->merge_in += c->merge_in;
     s->merge_out += c->merge_out;
     s->early += c->early;
+    atomicAdd(work + 0, (unsigned long long)c->rotations);
+    atomicAdd(work + 1, (unsigned long long)c->swept);
+    atomicAdd(work + 2, (unsigned long long)c->merge_out);
+    atomicAdd(work + 3, (unsigned long long)c->rebalances);
     s->live_cap_v = c->live_cap_v;
     s->app_cap_v = c->app_cap_v;
     if (c->stall) {

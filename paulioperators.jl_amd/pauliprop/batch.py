@@ -254,9 +254,21 @@ class SparsePauliBatch:
         return a.value, b.value, c.value
 
     def resident_time(self, reset: bool = False) -> float:
+        """HIP-event seconds of the resident launches while timing is on."""
         s = ctypes.c_double()
         check(_lib.lib().pp_batch_resident_time(self._h, ctypes.byref(s), 1 if reset else 0))
         return s.value
+
+    def resident_work(self, reset: bool = False):
+        """(rotations, swept terms, merge outputs, chunk rebalances) of the
+        resident launches."""
+        r = ctypes.c_int64()
+        s = ctypes.c_int64()
+        o = ctypes.c_int64()
+        q = ctypes.c_int64()
+        check(_lib.lib().pp_batch_resident_work(self._h, ctypes.byref(r), ctypes.byref(s), ctypes.byref(o),
+                                                ctypes.byref(q), 1 if reset else 0))
+        return r.value, s.value, o.value, q.value
 
     def set_timing(self, on: bool):
         check(_lib.lib().pp_batch_set_timing(self._h, 1 if on else 0))

@@ -7,7 +7,7 @@
   per-layer counts, final ⟨O⟩ and array digests equal the committed fixture
   tests/golden/lattice121_full.json.
 * The 64-qubit chain (depolarizing 1e-3, |c| <= 1e-6, weight <= 16) for 8
-  layers (2.6e6 live terms), layer by layer.
+  layers (9.0e5 live terms at the end), layer by layer.
 * The first 32 seed-7 batched instances as key-width batches: chains for 4
   layers, grids for 3 (grids reach 1e6-8e7 terms by layer 4, beyond what the
   oracle finishes in a test).

@@ -3,8 +3,9 @@ call, terms held in LDS across the rotations of the call).
 
 Every case forces the resident mode and compares with the CPU oracle bit for
 bit (keys, order, coefficient bit patterns) and counter for counter.  Small
-chunk capacities (cap_limit) force multi-workgroup clusters, chunk overflows
-and the rollback to the per-step kernels in the middle of a call.
+chunk capacities (cap_limit) force multi-workgroup clusters, chunk
+rebalances inside a launch, and vectors that outgrow their cluster and
+finish the call on the per-step kernels.
 """
 import math
 import random
@@ -133,7 +134,7 @@ def lattice_batch(cap, n_layers, thetas, mode=2):
                 f"layer {li} theta {b} counters"
         for b, o in enumerate(orcs):
             assert same_arrays(bt.arrays(b), o.arrays()), f"layer {li} theta {b}"
-        log(f"cap={cap} layer {li}: sizes {bt.sizes()} resident {bt.resident_stats()}")
+        log(f"cap={cap} layer {li}: sizes {bt.sizes()} resident {bt.resident_stats()} work {bt.resident_work()}")
     evs = bt.expectation_values()
     for e, o in zip(evs, orcs):
         assert abs(e - o.expectation(0)) <= 1e-12
@@ -149,6 +150,7 @@ def test_resident_lattice121_first_layers():
     bt = lattice_batch(0, 4, ths)
     launches, fallbacks, aborts = bt.resident_stats()
     assert launches >= 4 and aborts == 0
+    assert bt.resident_work()[3] > 0  # chunks rebalanced inside launches
 
 
 @pytest.mark.parametrize("cap", [40, 256])
@@ -158,6 +160,7 @@ def test_resident_lattice121_small_chunks(cap):
     bt = lattice_batch(cap, 3, ths)
     launches, fallbacks, aborts = bt.resident_stats()
     assert launches >= 3 and aborts == 0 and fallbacks > 0
+    assert bt.resident_work()[3] > 0
 
 
 def test_resident_batched_instances():
