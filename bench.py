@@ -572,6 +572,8 @@ def main():
     k_merge = kernel_roofline("kb_merge<2,512>", S * (pc.terms_swept - r_swept + pc.merge_out - r_out), t_merge,
                               n_merge, pc.rotations - r_rot, note)
     dominant, other = (k_res, k_merge) if t_res >= t_merge else (k_merge, k_res)
+    if not other["launches"]:
+        other = None  # every rotation ran in the other kernel
     step_bytes = S * (agg.terms_swept + agg.merge_out) / args.steps
     step_s = tot_time / args.steps
     rotation = {"bound": "hbm", "achieved": step_bytes / step_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",

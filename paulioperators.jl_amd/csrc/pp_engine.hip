@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -245,6 +246,17 @@ int scratch_setup(int dev) {
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// A step table (nsteps x B entries, step-major) in host memory.
+struct StepSpan {
+    StepDev* p = nullptr;
+    size_t n = 0;
+    StepDev& operator[](size_t i) const { return p[i]; }
+    StepDev* data() const { return p; }
+    size_t size() const { return n; }
+    StepDev* begin() const { return p; }
+    StepDev* end() const { return p + n; }
+};
+
 struct Batch {
     int W = 1, B = 0, ncu = 256, dev = 0;
     int sweep_fold_above = 32;  // sweep workgroups take several tiles past this many tiles per CU
@@ -274,6 +286,8 @@ struct Batch {
     uint64_t* d_kets = nullptr;
     int32_t* d_bad = nullptr;
     SegState* h_state = nullptr;   // pinned mirror
+    StepDev* h_steps = nullptr;    // pinned step table of the current call
+    size_t h_steps_cap = 0;
     // Optional HIP-event timing of every kb_merge launch (for roofline figures).
     bool timing = false;
     std::vector<hipEvent_t> evpool;
@@ -338,6 +352,7 @@ struct Batch {
         dfree(d_desc); dfree(d_state); dfree(d_maps); dfree(d_blk); dfree(d_filt); dfree(d_chan); dfree(d_tabs);
         dfree(d_en); dfree(d_steps); dfree(d_partial); dfree(d_out); dfree(d_kets); dfree(d_bad);
         if (h_state) (void)hipHostFree(h_state);
+        if (h_steps) (void)hipHostFree(h_steps);
         for (auto e : evpool) (void)hipEventDestroy(e);
         for (auto& g : gcache) (void)hipGraphExecDestroy(g.second);
         dfree(d_epoch);
@@ -855,7 +870,7 @@ struct Batch {
     // whose chunks overflowed resumes with more workgroups).
     // false: the call runs on the per-step kernels (auto mode also declines
     // calls of few steps and vectors that do not fit one XCD at 90% fill).
-    bool res_plan(const std::vector<StepDev>& steps, int64_t nsteps) {
+    bool res_plan(const StepSpan& steps, int64_t nsteps) {
         if (res_mode == 0 || res_broken || res_kx == 0) return false;
         if (res_mode == 1 && nsteps < 8) return false;
         std::vector<char> part((size_t)B, 0);
@@ -1014,7 +1029,7 @@ struct Batch {
     // the reference's capacity precheck.  Stalled vectors are regrown and
     // replayed until every vector has finished the sequence.
     template <int Wt>
-    void run_sequence(std::vector<StepDev>& steps, int64_t nsteps, const std::vector<int32_t>& en) {
+    void run_sequence(const StepSpan& steps, int64_t nsteps, const std::vector<int32_t>& en) {
         std::fill(res_off.begin(), res_off.end(), 0);
         std::fill(res_boost.begin(), res_boost.end(), 1.0);  // boosts last one call
         res_ovf.assign((size_t)B, 0);
@@ -1022,8 +1037,25 @@ struct Batch {
     }
     std::vector<char> res_ovf;  // per vector: a chunk overflowed in this call
 
+    // The pinned step table for n entries (its previous contents kept while
+    // it is large enough; an upload from it has finished by the next call,
+    // which starts after the stream was synchronised).
+    StepSpan host_steps(size_t n) {
+        if (h_steps_cap < n) {
+            HIPCHK(hipStreamSynchronize(stream));
+            if (h_steps) HIPCHK(hipHostFree(h_steps));
+            h_steps = nullptr;
+            h_steps_cap = 0;
+            const size_t cap = std::max<size_t>(n, 1024);
+            HIPCHK(hipHostMalloc((void**)&h_steps, cap * sizeof(StepDev), hipHostMallocDefault));
+            std::memset(h_steps, 0, cap * sizeof(StepDev));
+            h_steps_cap = cap;
+        }
+        return StepSpan{h_steps, n};
+    }
+
     template <int Wt>
-    void run_sequence_body(std::vector<StepDev>& steps, int64_t nsteps, const std::vector<int32_t>& en) {
+    void run_sequence_body(const StepSpan& steps, int64_t nsteps, const std::vector<int32_t>& en) {
         int maxlev = 1;
         for (const StepDev& s : steps) maxlev = std::max(maxlev, (int)s.g.nlev);
         step_bounds.assign((size_t)nsteps, 0);
@@ -1114,7 +1146,7 @@ struct Batch {
             upload_desc();
             plan();
             HIPCHK(hipMemcpyAsync(d_steps, rs.data(), rs.size() * sizeof(StepDev), hipMemcpyHostToDevice, stream));
-            if (res_plan(rs, nsteps - first)) enqueue_resident<Wt>(d_steps, first, nsteps, false);
+            if (res_plan(StepSpan{rs.data(), rs.size()}, nsteps - first)) enqueue_resident<Wt>(d_steps, first, nsteps, false);
             else enqueue_sequence<Wt>(false, d_steps, first, nsteps, false);
             sync_state();
         }
@@ -2054,9 +2086,8 @@ struct Batch {
 // ---------------------------------------------------------------------------
 // Rotation tables
 
-static GenDev make_gen(int W, const uint64_t* gz, const uint64_t* gx, double cosv, double sinv) {
-    GenDev g;
-    std::memset(&g, 0, sizeof(g));
+// g must arrive zeroed (levels past nlev stay zero).
+static void make_gen(GenDev& g, int W, const uint64_t* gz, const uint64_t* gx, double cosv, double sinv) {
     int ng = 0;
     for (int w = 0; w < W; ++w) {
         g.gz[w] = gz[w];
@@ -2068,29 +2099,28 @@ static GenDev make_gen(int W, const uint64_t* gz, const uint64_t* gx, double cos
     g.sinv = sinv;
     int nlev = 0;
     for (int w = 0; w < W; ++w)  // x bits: combined positions [0, 64 W)
-        for (int b = 0; b < 64; ++b)
-            if ((gx[w] >> b) & 1ull) g.lev[nlev++] = (uint8_t)(64 * w + b);
+        for (uint64_t m = gx[w]; m; m &= m - 1) g.lev[nlev++] = (uint8_t)(64 * w + __builtin_ctzll(m));
     for (int w = 0; w < W; ++w)  // z bits above
-        for (int b = 0; b < 64; ++b)
-            if ((gz[w] >> b) & 1ull) g.lev[nlev++] = (uint8_t)(64 * W + 64 * w + b);
+        for (uint64_t m = gz[w]; m; m &= m - 1) g.lev[nlev++] = (uint8_t)(64 * W + 64 * w + __builtin_ctzll(m));
     g.nlev = nlev;
-    return g;
 }
 
 // Steps for B vectors: generator i of vector b is gz[(i * B + b) * W]; active
 // may be null (all active).  The first active rotation of each vector skips
 // the per-step precheck (the sequence-start precheck covers it).
-static std::vector<StepDev> make_steps(int W, int B, int64_t nsteps, const uint64_t* gz, const uint64_t* gx,
-                                       const double* cosv, const double* sinv, const uint8_t* active,
-                                       std::vector<int32_t>& en) {
-    std::vector<StepDev> st((size_t)(nsteps * B));
+// st: nsteps * B entries that are zero or hold an earlier call's steps.
+static void make_steps(const StepSpan& st, int W, int B, int64_t nsteps, const uint64_t* gz, const uint64_t* gx,
+                       const double* cosv, const double* sinv, const uint8_t* active, std::vector<int32_t>& en) {
     en.assign(B, 0);
     std::vector<char> seen(B, 0);
     for (int64_t s = 0; s < nsteps; ++s)
         for (int b = 0; b < B; ++b) {
             const size_t k = (size_t)(s * B + b);
             StepDev& d = st[k];
-            d.g = make_gen(W, gz + k * W, gx + k * W, cosv[k], sinv[k]);
+            const int old = std::min<int>(std::max<int>(d.g.nlev, 0), kMaxLevels);
+            std::memset(&d.g, 0, offsetof(GenDev, lev));
+            std::memset(d.g.lev, 0, (size_t)old);  // levels past the new nlev stay zero
+            make_gen(d.g, W, gz + k * W, gx + k * W, cosv[k], sinv[k]);
             d.active = active ? (active[k] ? 1 : 0) : 1;
             d.precheck = 0;
             if (d.active) {
@@ -2099,7 +2129,6 @@ static std::vector<StepDev> make_steps(int W, int B, int64_t nsteps, const uint6
                 en[b] = 1;
             }
         }
-    return st;
 }
 
 static void add_counters(pp_counters* c, const SegState& a, const SegState& b) {
@@ -2142,8 +2171,9 @@ static void batch_evolve(Batch& bt, int64_t nsteps, const uint64_t* gz, const ui
                          const double* sinv, const uint8_t* active, const pp_filter* strict, const pp_filter* local,
                          pp_counters* counters) {
     std::vector<int32_t> en;
-    std::vector<StepDev> st = make_steps(bt.W, bt.B, nsteps, gz, gx, cosv, sinv, active, en);
     bt.sync_state();
+    const StepSpan st = bt.host_steps((size_t)(nsteps * bt.B));
+    make_steps(st, bt.W, bt.B, nsteps, gz, gx, cosv, sinv, active, en);
     std::vector<SegState> before(bt.h_state, bt.h_state + bt.B);
     bt.set_filters(strict, local);
     dispatch_w(bt.W, [&](auto wc) { bt.run_sequence<decltype(wc)::value>(st, nsteps, en); });
@@ -2308,8 +2338,8 @@ int pp_evolve_ex(pp_handle* h, const uint64_t* gz, const uint64_t* gx, const dou
     API_BEGIN
     check_ops(ops, nops);
     Batch& bt = h->bt;
-    std::vector<GenDev> gens((size_t)L);
-    for (int64_t i = 0; i < L; ++i) gens[i] = make_gen(bt.W, gz + i * bt.W, gx + i * bt.W, cosv[i], sinv[i]);
+    std::vector<GenDev> gens((size_t)L);  // value-initialised: zero
+    for (int64_t i = 0; i < L; ++i) make_gen(gens[(size_t)i], bt.W, gz + i * bt.W, gx + i * bt.W, cosv[i], sinv[i]);
     Batch::WinArgs a;
     a.ops = ops;
     a.nops = nops;
@@ -2426,7 +2456,9 @@ int pp_evolve_single(pp_handle* h, const uint64_t* gz, const uint64_t* gx, doubl
     if (n > st.app_cap_v) st.app_cap_v = std::max(2 * st.app_cap_v, n);
     bt.put_state();
     std::vector<int32_t> en;
-    std::vector<StepDev> steps = make_steps(bt.W, 1, 1, gz, gx, &cosv, &sinv, nullptr, en);
+    std::vector<StepDev> one(1);
+    const StepSpan steps{one.data(), 1};
+    make_steps(steps, bt.W, 1, 1, gz, gx, &cosv, &sinv, nullptr, en);
     bt.set_filters(nullptr, nullptr);
     // app_cap_v >= n now, so the sequence-start precheck cannot fire
     dispatch_w(bt.W, [&](auto wc) { bt.run_sequence<decltype(wc)::value>(steps, 1, en); });

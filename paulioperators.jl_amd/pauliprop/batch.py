@@ -17,10 +17,31 @@ import numpy as np
 from . import _lib
 from ._lib import ArgumentError, DimensionMismatch, PPCounters, check, dp, u64p
 from .basis import PauliBasis, word_count
-from .spv import _pack, _unpack_rows
+from .spv import _pack, _pow_table, _unpack_rows
 from .truncation import NoTruncation, PPFilter, compile_filter, is_compilable
 
 _KIND = {"depolarizing": 0, "dephasing": 1, "bit_flip": 2, "bit_phase_flip": 3, "pauli": 4}
+_FILTERS = {}  # tuple of strategies (frozen, hashable) -> (all compilable, PPFilter array)
+
+
+def _filter_array(strategies):
+    """(every strategy compilable, the PPFilter array of compile_filter per
+    vector); strategies are frozen dataclasses, so equal tuples compile to
+    equal arrays and the result is kept."""
+    key = tuple(strategies)
+    try:
+        hit = _FILTERS.get(key)
+    except TypeError:  # an unhashable strategy: no caching
+        hit, key = None, None
+    if hit is None:
+        ok = all(is_compilable(s) for s in strategies)
+        arr = (PPFilter * len(strategies))(*[compile_filter(s) for s in strategies]) if ok else None
+        hit = (ok, arr)
+        if key is not None:
+            if len(_FILTERS) > 256:
+                _FILTERS.clear()
+            _FILTERS[key] = hit
+    return hit
 
 
 class SparsePauliBatch:
@@ -144,13 +165,14 @@ class SparsePauliBatch:
         """``evolve_`` on arrays from ``pack`` (cos/sin already applied)."""
         strat = list(truncation) if isinstance(truncation, (list, tuple)) else [truncation] * self.B
         loc = list(local_truncation) if isinstance(local_truncation, (list, tuple)) else [local_truncation] * self.B
-        for s in strat:
-            if not is_compilable(s):
-                raise NotImplementedError(f"{type(s).__name__} at merge boundaries is not available in this build")
-        for s in loc:
-            if not is_compilable(s):
-                raise ArgumentError("local_truncation must be a deterministic (weight/coefficient) strategy "
-                                    "— it runs per append")
+        ok_s, fs = _filter_array(strat)
+        if not ok_s:
+            s = next(s for s in strat if not is_compilable(s))
+            raise NotImplementedError(f"{type(s).__name__} at merge boundaries is not available in this build")
+        ok_l, fl = _filter_array(loc)
+        if not ok_l:
+            raise ArgumentError("local_truncation must be a deterministic (weight/coefficient) strategy "
+                                "— it runs per append")
         gz, gx, cv, sv, act = packed
         nsteps = gz.shape[0]
         if nsteps == 0:
@@ -159,8 +181,6 @@ class SparsePauliBatch:
         if getattr(self, "_tuning", None):
             from .autotune import lookup
             self.set_tiles(*lookup(self._tuning, self.W, max(self.sizes())))
-        fs = (PPFilter * B)(*[compile_filter(s) for s in strat])
-        fl = (PPFilter * B)(*[compile_filter(s) for s in loc])
         cnt = (PPCounters * B)()
         check(_lib.lib().pp_batch_evolve(self._h, nsteps, u64p(gz), u64p(gx), dp(cv), dp(sv),
                                          act.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), fs, fl, cnt))
@@ -171,6 +191,14 @@ class SparsePauliBatch:
         channel name, or a list of names/None per vector)."""
         kinds = list(kind) if isinstance(kind, (list, tuple)) else [kind] * self.B
         ps = list(p) if isinstance(p, (list, tuple)) else [p] * self.B
+        qk = qubits if qubits is None or isinstance(qubits, int) else tuple(qubits)
+        key = (tuple(kinds), tuple(ps), pX, pY, pZ, qk)
+        cache = self.__dict__.setdefault("_chan_cache", {})
+        hit = cache.get(key)
+        if hit is not None:
+            karr, masks, tabs = hit
+            check(_lib.lib().pp_batch_scale_by_popcount(self._h, karr, u64p(masks), dp(tabs)))
+            return self
         L = _lib.lib()
         karr = (ctypes.c_int * self.B)()
         masks = np.zeros((self.B, self.W), dtype=np.uint64)
@@ -198,8 +226,10 @@ class SparsePauliBatch:
                     raise ArgumentError(f"{k} parameter p={pb} must be in [0,1]")
                 lams = (1 - 4 * pb / 3,) if k == "depolarizing" else (1 - 2 * pb,)
             for t, lam in enumerate(lams):
-                for j in range(N + 1):
-                    tabs[b, t, j] = L.pp_julia_powi(lam, j)
+                tabs[b, t, :N + 1] = _pow_table(lam, N)
+        if len(cache) > 64:
+            cache.clear()
+        cache[key] = (karr, masks, tabs)
         check(L.pp_batch_scale_by_popcount(self._h, karr, u64p(masks), dp(tabs)))
         return self
 

@@ -81,6 +81,24 @@ def _unpack_rows(a: np.ndarray) -> List[int]:
     return [int(lo) | (int(hi) << 64) for lo, hi in a]
 
 
+_POW_TABLES = {}  # (lambda, N) -> [lambda^0 .. lambda^N] by Julia's Float64^Int (pp_julia_powi)
+
+
+def _pow_table(lam: float, N: int) -> np.ndarray:
+    """The per-weight channel factors, computed once per (lambda, N); the
+    caller must not modify the returned array."""
+    key = (float(lam), int(N))
+    t = _POW_TABLES.get(key)
+    if t is None:
+        if len(_POW_TABLES) > 4096:
+            _POW_TABLES.clear()
+        L = _lib.lib()
+        t = np.array([L.pp_julia_powi(lam, j) for j in range(N + 1)], dtype=np.float64)
+        t.setflags(write=False)
+        _POW_TABLES[key] = t
+    return t
+
+
 class SparsePauliVector:
     """SparsePauliVector{N,W,Float64} (type_SparsePauliVector.jl:69-84), device-resident."""
 
@@ -436,7 +454,7 @@ class SparsePauliVector:
         return _pack(M, self.W)
 
     def _table(self, lam: float) -> np.ndarray:
-        return np.array([_lib.lib().pp_julia_powi(lam, k) for k in range(self.N + 1)], dtype=np.float64)
+        return _pow_table(lam, self.N)
 
     def _scale(self, kind: int, qubits, t0, t1=None, t2=None):
         M = self._mask(qubits)

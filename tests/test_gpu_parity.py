@@ -505,7 +505,7 @@ def test_batch_graph_replays_match_eager():
             assert np.array_equal(gz, oz) and np.array_equal(gc.view(np.uint64), oc.view(np.uint64))
         log(f"graph layer {layer}: sizes {g_bt.sizes()} stats {g_bt.graph_stats()}")
     caps, launches = g_bt.graph_stats()
-    assert launches == 6
+    assert launches >= 6  # one per layer call, plus replays of a call's tail after a vector outgrew its cluster
     # zero angles: every child has c = 0 and is cut, sizes and grids stay
     # fixed, so the captured sequence is replayed
     zero = [[0.0] * 127 for _ in range(B)]
@@ -517,7 +517,7 @@ def test_batch_graph_replays_match_eager():
             assert np.array_equal(g_bt.arrays(b)[2].view(np.uint64), e_bt.arrays(b)[2].view(np.uint64))
     caps2, launches2 = g_bt.graph_stats()
     log(f"graph stats after replays: {caps2} captures, {launches2} launches")
-    assert launches2 == 9 and caps2 <= caps + 1
+    assert launches2 == launches + 3 and caps2 <= caps + 1
 
 
 def test_batched_instances_bitexact_with_graphs_and_chunks():
