@@ -54,7 +54,7 @@ def host_cpu() -> str:
     return "unknown"
 
 
-def sweep_gpu(workloads, timing=True, graphs=False, digests=False):
+def sweep_gpu(workloads, timing=True, graphs=False, digests=False, resident=1):
     """All θ_h instances in one batch: per layer one evolve! per instance, then
     the dephasing channel, then ⟨O⟩ of every instance.  Returns a dict:
     seconds (wall), counters (summed), evs, and when timing is on the HIP-event
@@ -67,6 +67,7 @@ def sweep_gpu(workloads, timing=True, graphs=False, digests=False):
     bt.reserve(1 << 18)
     bt.set_timing(timing)
     bt.set_graphs(graphs)
+    bt.set_resident(resident, 0)
     tot = PPCounters()
     nl = len(workloads[0].layers)
     packed = [bt.pack([w.layers[li].generators for w in workloads], [w.layers[li].angles for w in workloads])
@@ -479,6 +480,8 @@ def main():
     ap.add_argument("--cpu-all-seconds", type=float, default=6.0,
                     help="target wall seconds of the all-cores CPU baseline")
     ap.add_argument("--graphs", type=int, default=0, help="replay hipGraph-captured layer sequences (1/0)")
+    ap.add_argument("--resident", type=int, default=1,
+                    help="resident layer launches: 1 where the vectors fit (default), 0 the per-step kernels only")
     ap.add_argument("--batched", action="store_true",
                     help="run only the batched workload's first 6 layers with its CPU baseline")
     ap.add_argument("--rows", action="store_true",
@@ -526,14 +529,14 @@ def main():
     graphs = bool(args.graphs)
     for i in range(args.warmup):
         log(f"warmup {i}")
-        sweep_gpu(workloads, timing=False, graphs=graphs)
+        sweep_gpu(workloads, timing=False, graphs=graphs, resident=args.resident)
 
     tot_time = 0.0
     agg = PPCounters()
     evs = digs = None
     for i in range(args.steps):
         log(f"step {i}")
-        r = sweep_gpu(workloads, timing=False, graphs=graphs, digests=(i == args.steps - 1))
+        r = sweep_gpu(workloads, timing=False, graphs=graphs, digests=(i == args.steps - 1), resident=args.resident)
         cnt = r["counters"]
         log(f"  {r['seconds']:.3f} s, {cnt.terms_swept} term-rotations, {cnt.rotations} rotations, "
             f"resident launches/fallbacks/aborts {r['resident_stats']}")
@@ -548,7 +551,7 @@ def main():
 
     # kernel rooflines: one more (untimed) sweep with HIP events around every
     # resident layer launch and every per-step merge launch
-    pr = sweep_gpu(workloads, timing=True, graphs=False)
+    pr = sweep_gpu(workloads, timing=True, graphs=False, resident=args.resident)
     assert pr["evs"] == evs, "profiled sweep differs from the timed one"
     pc = pr["counters"]
     t_merge, n_merge = pr["merge"]
@@ -624,7 +627,7 @@ def main():
                                "+ X(theta_h) on 121 sites, 16 theta_h in [0, pi/2], dephasing 5e-4, |c|<=1e-4, "
                                "weight<=12, observable mean Z",
                    "rotations": agg.rotations // args.steps, "term_rotations": agg.terms_swept // args.steps,
-                   "graphs": graphs, "expectation_by_theta_h": evs,
+                   "graphs": graphs, "resident": args.resident, "expectation_by_theta_h": evs,
                    "final_terms_by_theta_h": [f["n_by_layer"][-1] for f in fixture],
                    "final_term_sets_match_fixture": True},
         "roofline": dominant,
