@@ -280,6 +280,14 @@ int pp_batch_resident_time(pp_batch* p, double* seconds, int reset);
 int pp_batch_resident_work(pp_batch* p, int64_t* rotations, int64_t* swept, int64_t* merge_out, int64_t* rebalances,
                            int reset);
 
+/* Test hook, no counterpart in the reference: move the merge look-back tag
+ * base forward to `epoch` (1 .. 2^22 - 2, not below the tags already used),
+ * so that tests reach the 22-bit tag wrap, where every look-back array is
+ * cleared and the base restarts at 1, without 4 million launches.  pp_set_epoch
+ * also moves the windowed merge's tag. */
+int pp_batch_set_epoch(pp_batch* p, uint32_t epoch);
+int pp_set_epoch(pp_handle* h, uint32_t epoch);
+
 /* ---- Learned surrogate of ⟨O⟩(θ) -------------------------------------------
  * No counterpart in the reference: an ensemble of E MLPs
  * din -> hidden -> ... (layers hidden layers, GELU) -> 1, trained with Adam on

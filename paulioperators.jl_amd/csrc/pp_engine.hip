@@ -2691,6 +2691,29 @@ int pp_set_resident(pp_handle* h, int mode, int cap_limit) {
     return set_resident(h->bt, mode, cap_limit);
 }
 
+// Look-back tags: move the base forward (tests of the wrap near 2^22).
+static int set_epoch(Batch& bt, uint32_t e) {
+    if (e == 0 || (uint64_t)e >= kEpochMask) return fail(PP_EARG, "epoch must be in 1 .. 2^22 - 2");
+    API_BEGIN
+    bt.sync_state();
+    if ((uint64_t)e < bt.epoch_used) throw PPError(PP_EARG, "the look-back tag base only moves forward");
+    bt.epoch_used = e;
+    HIPCHK(hipMemcpyAsync(bt.d_epoch, &e, sizeof(uint32_t), hipMemcpyHostToDevice, bt.stream));
+    if (e > bt.win.epoch) bt.win.epoch = e;
+    HIPCHK(hipStreamSynchronize(bt.stream));
+    API_END
+}
+
+int pp_batch_set_epoch(pp_batch* p, uint32_t epoch) {
+    if (!p) return fail(PP_EARG, "null handle");
+    return set_epoch(p->bt, epoch);
+}
+
+int pp_set_epoch(pp_handle* h, uint32_t epoch) {
+    if (!h) return fail(PP_EARG, "null handle");
+    return set_epoch(h->bt, epoch);
+}
+
 int pp_batch_resident_stats(pp_batch* p, int64_t* launches, int64_t* fallbacks, int64_t* aborts) {
     if (!p || !launches || !fallbacks || !aborts) return fail(PP_EARG, "null argument");
     *launches = p->bt.res_launches;

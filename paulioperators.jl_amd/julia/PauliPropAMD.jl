@@ -195,6 +195,10 @@ function set_resident!(d::DeviceSPV, mode::Integer=1, cap_limit::Integer=0)
     return d
 end
 
+# test hook: the merge look-back tag base moved forward (reaches the 22-bit wrap)
+set_epoch!(d::DeviceSPV, epoch::Integer) =
+    (check(ccall((:pp_set_epoch, LIB), Cint, (Ptr{Cvoid}, UInt32), d.h, epoch)); d)
+
 # evolve!(O, generators, angles; window, truncation, local_truncation,
 # correction, counters) (spv_evolve.jl:312-387)
 function evolve!(d::DeviceSPV{N,W}, generators::Vector{PauliBasis{N}}, angles::Vector{<:Real};
@@ -605,6 +609,8 @@ set_graphs!(bt::DeviceSPVBatch, on::Bool) =
     (check(ccall((:pp_batch_set_graphs, LIB), Cint, (Ptr{Cvoid}, Cint), bt.h, on)); bt)
 set_timing!(bt::DeviceSPVBatch, on::Bool) =
     (check(ccall((:pp_batch_set_timing, LIB), Cint, (Ptr{Cvoid}, Cint), bt.h, on)); bt)
+set_epoch!(bt::DeviceSPVBatch, epoch::Integer) =
+    (check(ccall((:pp_batch_set_epoch, LIB), Cint, (Ptr{Cvoid}, UInt32), bt.h, epoch)); bt)
 set_resident!(bt::DeviceSPVBatch, mode::Integer=1, cap_limit::Integer=0) =
     (check(ccall((:pp_batch_set_resident, LIB), Cint, (Ptr{Cvoid}, Cint, Cint), bt.h, mode, cap_limit)); bt)
 

@@ -25,6 +25,7 @@ EXPORTS = ("pp_create", "pp_destroy", "pp_upload", "pp_download", "pp_size", "pp
            "pp_batch_reserve", "pp_batch_set_tiles", "pp_batch_set_timing", "pp_batch_merge_time",
            "pp_batch_set_graphs", "pp_batch_graph_stats", "pp_batch_set_resident", "pp_set_resident",
            "pp_batch_resident_stats", "pp_batch_resident_time", "pp_batch_resident_work",
+           "pp_batch_set_epoch", "pp_set_epoch",
            "pp_mlp_create", "pp_mlp_destroy", "pp_mlp_param_count", "pp_mlp_set_params", "pp_mlp_get_params",
            "pp_mlp_load_data", "pp_mlp_train_epoch", "pp_mlp_gradients", "pp_mlp_predict",
            "pp_julia_powi", "pp_last_error", "pp_version")
@@ -139,6 +140,8 @@ def lib():
     L.pp_batch_resident_stats.argtypes = [vp, P(i64), P(i64), P(i64)]
     L.pp_batch_resident_time.argtypes = [vp, dp, ctypes.c_int]
     L.pp_batch_resident_work.argtypes = [vp, P(i64), P(i64), P(i64), P(i64), ctypes.c_int]
+    L.pp_batch_set_epoch.argtypes = [vp, ctypes.c_uint32]
+    L.pp_set_epoch.argtypes = [vp, ctypes.c_uint32]
     fp = P(ctypes.c_float)
     L.pp_mlp_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P(vp)]
     L.pp_mlp_destroy.argtypes = [vp]

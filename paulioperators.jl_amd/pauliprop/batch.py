@@ -274,6 +274,12 @@ class SparsePauliBatch:
         check(_lib.lib().pp_batch_set_resident(self._h, int(mode), int(cap_limit)))
         return self
 
+    def set_epoch(self, epoch: int):
+        """Test hook: move the merge look-back tag base forward to `epoch`
+        < 2^22 - 1, to reach the tag wrap."""
+        check(_lib.lib().pp_batch_set_epoch(self._h, int(epoch)))
+        return self
+
     def resident_stats(self):
         """(resident launches, vectors finished by the per-step kernels after a
         resident stop, aborted launches)."""

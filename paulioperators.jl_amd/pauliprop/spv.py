@@ -166,6 +166,12 @@ class SparsePauliVector:
         check(_lib.lib().pp_set_resident(self._h, int(mode), int(cap_limit)))
         return self
 
+    def set_epoch(self, epoch: int):
+        """Test hook: move the merge look-back tag base (sequence and windowed
+        merges) forward to `epoch` < 2^22 - 1, to reach the tag wrap."""
+        check(_lib.lib().pp_set_epoch(self._h, int(epoch)))
+        return self
+
     # --- size / conversion ------------------------------------------------------------
     def __len__(self) -> int:
         n = ctypes.c_int64()
