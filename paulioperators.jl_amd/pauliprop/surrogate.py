@@ -32,9 +32,22 @@ _fp = ctypes.POINTER(ctypes.c_float)
 _dp = ctypes.POINTER(ctypes.c_double)
 
 
-def features(thetas) -> np.ndarray:
-    """Network input: θ / π, float32 [n][20]."""
-    return (np.asarray(thetas, dtype=np.float64) / math.pi).astype(np.float32)
+ENCODINGS = ("raw", "trig")
+
+
+def features(thetas, encoding: str = "raw") -> np.ndarray:
+    """Network input, float32: "raw" θ / π ([n][20]); "trig" cos θ and sin θ
+    of every angle ([n][40]: ⟨O⟩ is 2π-periodic in each angle)."""
+    t = np.asarray(thetas, dtype=np.float64)
+    if encoding == "raw":
+        return (t / math.pi).astype(np.float32)
+    if encoding == "trig":
+        return np.concatenate([np.cos(t), np.sin(t)], axis=1).astype(np.float32)
+    raise ArgumentError(f"encoding must be one of {ENCODINGS}")
+
+
+def feature_dim(encoding: str) -> int:
+    return 20 if encoding == "raw" else 40
 
 
 class MLPEnsemble:
@@ -99,27 +112,34 @@ class MLPEnsemble:
         return g, loss
 
     def fit(self, x, y, epochs: int = 60, batch: int = 1024, lr: float = 1e-3, lr_min: float = 1e-5,
-            seed: int = 1, log: Optional[Callable[[str], None]] = None):
+            seed: int = 1, log: Optional[Callable[[str], None]] = None,
+            callback: Optional[Callable[[int, np.ndarray], None]] = None, every: int = 10):
         """Adam with a cosine learning-rate decay from lr to lr_min; each member
-        sees its own shuffle of the data every epoch.  Returns the per-epoch
-        mean losses [epochs][E] and the wall time."""
+        sees its own shuffle of the data every epoch.  callback(epoch, losses)
+        runs after every `every`-th epoch and the last (outside the timed
+        training).  Returns the per-epoch mean losses [epochs][E] and the wall
+        time of the training epochs."""
         x = np.ascontiguousarray(x, dtype=np.float32)
         y = np.ascontiguousarray(y, dtype=np.float32)
         n = len(y)
         check(_lib.lib().pp_mlp_load_data(self._h, x.ctypes.data_as(_fp), y.ctypes.data_as(_fp), n))
         rng = np.random.default_rng(seed)
         hist = []
-        t0 = time.perf_counter()
+        t_train = 0.0
         for ep in range(epochs):
+            t0 = time.perf_counter()
             perm = np.stack([rng.permutation(n) for _ in range(self.E)]).astype(np.int32)
             rate = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * ep / max(epochs - 1, 1)))
             loss = np.empty(self.E, dtype=np.float32)
             check(_lib.lib().pp_mlp_train_epoch(self._h, perm.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                                                 batch, ctypes.c_float(rate), loss.ctypes.data_as(_fp)))
+            t_train += time.perf_counter() - t0
             hist.append(loss.copy())
             if log and (ep % 10 == 0 or ep == epochs - 1):
                 log(f"epoch {ep}: lr {rate:.2e} loss {np.array2string(loss, precision=3)}")
-        return np.array(hist), time.perf_counter() - t0
+            if callback and (ep % every == 0 or ep == epochs - 1):
+                callback(ep, loss.copy())
+        return np.array(hist), t_train
 
     def predict(self, x, members: bool = False):
         """(mean, std, members or None, device seconds) for x[n][din]."""
@@ -195,43 +215,95 @@ class GatedSurrogate:
     """Ensemble mean where the ensemble agrees, exact propagation elsewhere.
 
     A θ is served by the network when its ensemble standard deviation is at
-    most ``threshold``.  ``calibrate`` sets the threshold from labelled θ the
-    network was not trained on: the largest disagreement s (at most ``tol``)
-    such that every calibration point with std <= s is within ``tol`` of its
-    exact value; when even the most confident calibration point misses, the
-    threshold is 0 and every query goes to exact propagation.  (Ensemble
-    members that all collapse to the same wrong answer agree closely, so
-    disagreement alone is not a safe gate.)"""
+    most ``threshold`` (a disagreement, in units of ⟨O⟩'s spread over the
+    members; it is not the error tolerance).  ``calibrate`` sets it from
+    labelled θ the network was not trained on: the largest s such that every
+    calibration point with std <= s is within ``tol`` of its exact value
+    (ties at s count as above it), accepted only when at least
+    ``min_points`` calibration points lie at or below s (no miss among m
+    points bounds the miss rate on new θ below 3/m at 95% confidence);
+    otherwise nothing is served and every query goes to exact propagation.
+    (Ensemble members that all collapse to the same wrong answer agree
+    closely, so disagreement alone is not a safe gate.)"""
 
-    def __init__(self, ensemble: MLPEnsemble, exact: Callable = chain36_batch_expectations, tol: float = 1e-3):
-        self.ens, self.exact, self.tol = ensemble, exact, tol
-        self.threshold = tol
+    def __init__(self, ensemble: MLPEnsemble, exact: Callable = chain36_batch_expectations, tol: float = 1e-3,
+                 encoding: str = "raw", min_points: int = 256):
+        self.ens, self.exact, self.tol, self.encoding = ensemble, exact, tol, encoding
+        self.min_points = min_points
+        self.threshold = -1.0  # uncalibrated: serve nothing
 
     def calibrate(self, thetas, labels):
-        mean, sd, _, _ = self.ens.predict(features(thetas))
+        mean, sd, _, _ = self.ens.predict(features(thetas, self.encoding))
         err = np.abs(mean - np.asarray(labels))
         order = np.argsort(sd, kind="stable")
         bad = np.nonzero(err[order] > self.tol)[0]
         k = bad[0] if len(bad) else len(order)  # points [0, k) of the std order are all within tol
-        if k == 0:
-            self.threshold = 0.0
-        else:
-            nxt = sd[order[k]] if k < len(order) else np.inf
-            self.threshold = float(min(self.tol, sd[order[k - 1]] if nxt > sd[order[k - 1]] else 0.0))
+        self.threshold = -1.0
+        while k > 0:
+            s = float(sd[order[k - 1]])
+            if k == len(order) or sd[order[k]] > s:
+                break
+            k -= 1  # a tie across the first miss: step below it
+        if k >= max(1, self.min_points):
+            self.threshold = float(sd[order[k - 1]])
         return {"threshold": self.threshold, "calibration_points": int(len(order)), "within_tol_prefix": int(k),
-                "calibration_rmse": float(np.sqrt(np.mean(err ** 2)))}
+                "min_points": int(self.min_points), "calibration_rmse": float(np.sqrt(np.mean(err ** 2)))}
+
+    def serves(self, sd) -> np.ndarray:
+        return sd <= self.threshold if self.threshold >= 0 else np.zeros(len(sd), dtype=bool)
 
     def evaluate(self, thetas):
         thetas = np.asarray(thetas, dtype=np.float64)
         t0 = time.perf_counter()
-        mean, sd, _, dev_s = self.ens.predict(features(thetas))
+        mean, sd, _, dev_s = self.ens.predict(features(thetas, self.encoding))
         t1 = time.perf_counter()
-        fall = ~(sd <= self.threshold) if self.threshold > 0 else np.ones(len(thetas), dtype=bool)
+        fall = ~self.serves(sd)
         vals = mean.copy()
         if fall.any():
             vals[fall] = self.exact([list(t) for t in thetas[fall]])
         t2 = time.perf_counter()
         return vals, fall, {"surrogate_s": t1 - t0, "surrogate_device_s": dev_s, "exact_s": t2 - t1}
+
+
+# --- learnability of the labels ----------------------------------------------------
+
+def learning_study(labels, encodings=ENCODINGS, capacity_n: int = 4096, capacity_epochs: int = 300,
+                   sizes=(4096, 16384, 61440), epochs: int = 60, n_eval: int = 4096, E: int = 5,
+                   log: Optional[Callable[[str], None]] = None) -> dict:
+    """Training and held-out curves of the device ensemble on the chain36
+    labels.  capacity: the first `capacity_n` training θ for
+    `capacity_epochs` epochs (can the network fit them at all?); sizes: the
+    first n training θ for `epochs` epochs (does the fit carry over to θ it
+    has not seen?).  Held-out RMSE on the first `n_eval` held-out θ; the
+    reference line is the labels' standard deviation (predicting the mean)."""
+    tr, ho = training_thetas()
+    ytr, yho = labels[:len(tr)], labels[len(tr):len(tr) + n_eval]
+    ho = ho[:n_eval]
+    out = {"label_var_train": float(np.var(ytr)), "label_std_heldout": float(np.std(yho)), "runs": []}
+
+    def one(enc, n, ep_count, every):
+        ens = MLPEnsemble(E=E, din=feature_dim(enc), hidden=512, layers=3, seed=0)
+        xf, hf = features(tr[:n], enc), features(ho, enc)
+        curve = []
+
+        def cb(ep, loss):
+            mean, _, _, _ = ens.predict(hf)
+            curve.append({"epoch": ep, "train_loss": float(np.mean(loss)),
+                          "heldout_rmse": float(np.sqrt(np.mean((mean - yho) ** 2)))})
+
+        _, secs = ens.fit(xf, ytr[:n], epochs=ep_count, batch=1024, lr=1e-3, lr_min=1e-5, callback=cb, every=every)
+        mean, _, _, _ = ens.predict(xf)
+        run = {"encoding": enc, "n_train": n, "epochs": ep_count, "train_seconds": secs,
+               "train_rmse_final": float(np.sqrt(np.mean((mean - ytr[:n]) ** 2))), "curve": curve}
+        if log:
+            log(f"{enc} n={n}: train loss {curve[-1]['train_loss']:.5f}, held-out rmse {curve[-1]['heldout_rmse']:.4f}")
+        out["runs"].append(run)
+
+    for enc in encodings:
+        one(enc, capacity_n, capacity_epochs, max(1, capacity_epochs // 10))
+        for n in sizes:
+            one(enc, n, epochs, max(1, epochs // 6))
+    return out
 
 
 # --- labels --------------------------------------------------------------------------

@@ -18,7 +18,8 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run(n_plain: int = 8, n_heldout: int = 4096, epochs: int = 60, n_query: int = 1_000_000, tol: float = 1e-3):
+def run(n_plain: int = 8, n_heldout: int = 4096, epochs: int = 60, n_query: int = 1_000_000, tol: float = 1e-3,
+        encoding: str = "trig"):
     lab = np.load(sg.LABELS_PATH)["labels"]
     tr, ho = sg.training_thetas()
     ytr, yho = lab[:len(tr)], lab[len(tr):]
@@ -33,17 +34,18 @@ def run(n_plain: int = 8, n_heldout: int = 4096, epochs: int = 60, n_query: int 
     res["plain_evals_per_s"] = n_plain / t_plain
     res["plain_max_abs_err"] = float(np.max(np.abs(np.array(plain) - yho[:n_plain])))
     log(f"plain: {n_plain} in {t_plain:.2f} s")
-    # batched exact
+    # batched exact, on the held-out θ the gate evaluates below
     t0 = time.perf_counter()
-    bvals = sg.chain36_batch_expectations(ho[:1024], batch=256)
+    bvals = sg.chain36_batch_expectations(ho, batch=256)
     t_b = time.perf_counter() - t0
-    res["batched_evals_per_s"] = 1024 / t_b
-    res["batched_max_abs_err"] = float(np.max(np.abs(bvals - yho[:1024])))
-    log(f"batched: 1024 in {t_b:.2f} s")
+    res["batched_evals_per_s"] = len(ho) / t_b
+    res["batched_max_abs_err"] = float(np.max(np.abs(bvals - yho)))
+    log(f"batched: {len(ho)} in {t_b:.2f} s")
     # training (the last 4,096 training θ are kept back to calibrate the gate)
     ncal = 4096
-    ens = sg.MLPEnsemble(E=5, din=20, hidden=512, layers=3, seed=0)
-    hist, t_train = ens.fit(sg.features(tr[:-ncal]), ytr[:-ncal], epochs=epochs, batch=1024, lr=1e-3,
+    res["encoding"] = encoding
+    ens = sg.MLPEnsemble(E=5, din=sg.feature_dim(encoding), hidden=512, layers=3, seed=0)
+    hist, t_train = ens.fit(sg.features(tr[:-ncal], encoding), ytr[:-ncal], epochs=epochs, batch=1024, lr=1e-3,
                             lr_min=1e-5, log=log)
     res["train_seconds"] = t_train
     res["train_epochs"] = epochs
@@ -52,12 +54,13 @@ def run(n_plain: int = 8, n_heldout: int = 4096, epochs: int = 60, n_query: int 
     flop_step = 5 * 1024 * 2 * (64 * 512 + 2 * 512 * 512) * 3
     res["train_tflops"] = steps * flop_step / t_train / 1e12
     # ensemble quality on held-out
-    mean, sd, _, _ = ens.predict(sg.features(ho))
+    mean, sd, _, _ = ens.predict(sg.features(ho, encoding))
     err = np.abs(mean - yho)
     res["ensemble_rmse_heldout"] = float(np.sqrt(np.mean(err ** 2)))
     res["ensemble_std_median"] = float(np.median(sd))
     # gated evaluation of the held-out θ
-    gate = sg.GatedSurrogate(ens, exact=lambda th: sg.chain36_batch_expectations(th, batch=256), tol=tol)
+    gate = sg.GatedSurrogate(ens, exact=lambda th: sg.chain36_batch_expectations(th, batch=256), tol=tol,
+                             encoding=encoding)
     res["gate_calibration"] = gate.calibrate(tr[-ncal:], ytr[-ncal:])
     t0 = time.perf_counter()
     vals, fall, tm = gate.evaluate(ho)
@@ -72,17 +75,27 @@ def run(n_plain: int = 8, n_heldout: int = 4096, epochs: int = 60, n_query: int 
     # inference throughput on the query set
     q = sg.query_thetas(n_query)
     t0 = time.perf_counter()
-    _, qsd, _, dev = ens.predict(sg.features(q))
+    _, qsd, _, dev = ens.predict(sg.features(q, encoding))
     t_q = time.perf_counter() - t0
     res["inference_queries"] = n_query
     res["inference_evals_per_s_wall"] = n_query / t_q
     res["inference_evals_per_s_device"] = n_query / dev
     res["inference_tflops_device"] = n_query * 5 * 2 * (64 * 512 + 2 * 512 * 512 + 512) / dev / 1e12
-    res["query_fraction_served"] = float(np.mean(qsd <= gate.threshold)) if gate.threshold > 0 else 0.0
+    res["query_fraction_served"] = float(np.mean(gate.serves(qsd)))
     res["speedup_gated_vs_plain"] = res["gated_evals_per_s"] / res["plain_evals_per_s"]
     res["speedup_batched_vs_plain"] = res["batched_evals_per_s"] / res["plain_evals_per_s"]
+    res["speedup_gated_vs_batched_same_theta"] = res["gated_evals_per_s"] / res["batched_evals_per_s"]
     return res
 
 
 if __name__ == "__main__":
-    print(json.dumps(run()), flush=True)
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--study", action="store_true",
+                    help="only the training / held-out curves of the device ensemble (raw and cos/sin inputs)")
+    ap.add_argument("--encoding", default="trig", choices=sg.ENCODINGS)
+    a = ap.parse_args()
+    if a.study:
+        print(json.dumps(sg.learning_study(np.load(sg.LABELS_PATH)["labels"], log=log)), flush=True)
+    else:
+        print(json.dumps(run(encoding=a.encoding)), flush=True)

@@ -1,7 +1,9 @@
 """GPU tests of the learned surrogate: the BF16 matrix-core forward and
 backward kernels against the CPU oracle (oracle/mlp_oracle.py, device
 rounding mode), Adam training, the exact labels against the propagation
-oracle, and what the trained, gated ensemble gains over exact propagation."""
+oracle, the ensemble's capacity on the labels, the measured fact that it does
+not generalise on this workload, and the gate's guarantee that it never
+serves a value outside the tolerance."""
 import math
 import os
 import time
@@ -94,22 +96,42 @@ def test_labels_match_the_propagation_oracle():
 
 
 @pytest.mark.skipif(not os.path.exists(sg.LABELS_PATH), reason="chain36 labels not generated")
-def test_gated_surrogate_on_heldout():
-    """Train the 5-member ensemble on GPU labels (the last 4,096 training θ
-    kept back for calibrating the gate), then on 1,024 held-out θ: every
-    value the calibrated gate returns is within 1e-3 of exact propagation,
-    and the gated evaluation costs at most what exact batched propagation of
-    the same θ costs (plus the network pass)."""
+def test_ensemble_fits_a_training_subset_of_the_labels():
+    """Capacity: on 4,096 of the GPU labels the 5 x 512^3 ensemble (cos/sin
+    inputs) drives its training loss below 5% of the label variance, so the
+    kernels and the optimiser can fit this function where it has data."""
+    lab = np.load(sg.LABELS_PATH)["labels"]
+    tr, _ = sg.training_thetas()
+    n = 4096
+    y = lab[:n]
+    ens = sg.MLPEnsemble(E=5, din=sg.feature_dim("trig"), hidden=512, layers=3, seed=0)
+    hist, secs = ens.fit(sg.features(tr[:n], "trig"), y, epochs=300, batch=1024, lr=1e-3, lr_min=1e-5)
+    log(f"capacity: 300 epochs in {secs:.2f} s, final losses {hist[-1]}, label variance {np.var(y):.4f}")
+    assert np.all(hist[-1] < 0.05 * np.var(y))
+
+
+@pytest.mark.skipif(not os.path.exists(sg.LABELS_PATH), reason="chain36 labels not generated")
+def test_labels_do_not_generalize_and_the_gate_serves_nothing_wrong():
+    """The measured negative result: trained on 61,440 labels, the ensemble's
+    held-out RMSE is no better than predicting the mean (at least 0.9 of the
+    labels' spread), so no held-out θ can be served within 1e-3.  Every value
+    the calibrated gate returns is within 1e-3 of exact propagation, and on
+    the same 1,024 θ the gated path costs at most what exact batched
+    propagation costs (plus the network pass)."""
     lab = np.load(sg.LABELS_PATH)["labels"]
     tr, ho = sg.training_thetas()
     ytr, yho = lab[:len(tr)], lab[len(tr):]
     assert not np.isnan(lab).any()
     ncal = 4096
-    ens = sg.MLPEnsemble(E=5, din=20, hidden=512, layers=3, seed=0)
-    hist, secs = ens.fit(sg.features(tr[:-ncal]), ytr[:-ncal], epochs=30, batch=1024, lr=1e-3, lr_min=1e-5,
-                         log=log)
-    log(f"trained in {secs:.1f} s; final losses {hist[-1]}")
-    gate = sg.GatedSurrogate(ens, exact=lambda th: sg.chain36_batch_expectations(th, batch=512), tol=1e-3)
+    ens = sg.MLPEnsemble(E=5, din=sg.feature_dim("trig"), hidden=512, layers=3, seed=0)
+    hist, secs = ens.fit(sg.features(tr[:-ncal], "trig"), ytr[:-ncal], epochs=30, batch=1024, lr=1e-3,
+                         lr_min=1e-5, log=log)
+    mean, sd, _, _ = ens.predict(sg.features(ho, "trig"))
+    rmse = float(np.sqrt(np.mean((mean - yho) ** 2)))
+    log(f"trained in {secs:.1f} s; final losses {hist[-1]}; held-out rmse {rmse:.4f}, label std {np.std(yho):.4f}")
+    assert rmse >= 0.9 * np.std(yho)
+    gate = sg.GatedSurrogate(ens, exact=lambda th: sg.chain36_batch_expectations(th, batch=512), tol=1e-3,
+                             encoding="trig")
     cal = gate.calibrate(tr[-ncal:], ytr[-ncal:])
     log(f"calibration: {cal}")
     sub = ho[:1024]
@@ -117,13 +139,12 @@ def test_gated_surrogate_on_heldout():
     vals, fall, tm = gate.evaluate(sub)
     t_gate = time.perf_counter() - t0
     err = np.abs(vals - yho[:1024])
-    served = ~fall
-    log(f"gate: served {served.sum()}/1024 by the network, max err {err.max():.2e}, time {t_gate:.2f} s ({tm})")
+    log(f"gate: served {(~fall).sum()}/1024 by the network, max err {err.max():.2e}, time {t_gate:.2f} s ({tm})")
     assert err.max() <= 1e-3
     t0 = time.perf_counter()
     sg.chain36_batch_expectations(sub, batch=512)
     t_exact = time.perf_counter() - t0
-    log(f"exact batched: {t_exact:.2f} s")
+    log(f"exact batched, same 1,024 θ: {t_exact:.2f} s")
     assert t_gate <= t_exact * 1.1 + 0.05
 
 
@@ -143,6 +164,6 @@ def test_gate_calibration_rejects_a_collapsed_ensemble():
     assert np.all(np.abs(vals[~fall] - y[1024:2048][~fall]) <= 5e-2)
     assert (~fall).sum() > 0
     cal = gate.calibrate(th[:1024], y[:1024] + 1.0)  # labels the network cannot match
-    assert cal["threshold"] == 0.0
+    assert cal["threshold"] < 0 and cal["within_tol_prefix"] == 0
     vals, fall, _ = gate.evaluate(th[1024:2048])
     assert fall.all() and np.all(vals == 7.0)
