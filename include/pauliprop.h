@@ -143,14 +143,28 @@ int pp_evolve(pp_handle* h, const uint64_t* gz, const uint64_t* gx, const double
  * of _boundary!, spv_evolve.jl:258-284); nops == 0 uses `strict`.
  * energy_ket != NULL is EnergyCorrection(ψ) (truncation.jl:262-306): the
  * change of ⟨ψ|v|ψ⟩ across every boundary, measured before the merge (live +
- * pending appends) and after the strategy, is added to *energy_acc. */
+ * pending appends) and after the strategy, is added to *energy_acc.  With
+ * variance_acc != NULL as well it is EnergyVarianceCorrection(ψ)
+ * (truncation.jl:273-310): every boundary merges with NOFILTER, measures
+ * ⟨ψ|v|ψ⟩ and variance(v, ψ) on the merged state, applies the strict filter
+ * as a compaction (or the strategy), measures again and adds both changes
+ * (spv_evolve.jl:246-270). */
 int pp_evolve_ex(pp_handle* h, const uint64_t* gz, const uint64_t* gx, const double* cosv, const double* sinv,
                  int64_t L, int64_t window, const pp_filter* strict, const pp_strategy_op* ops, int nops,
-                 const pp_filter* local, const uint64_t* energy_ket, double* energy_acc, pp_counters* counters);
+                 const pp_filter* local, const uint64_t* energy_ket, double* energy_acc, double* variance_acc,
+                 pp_counters* counters);
 
 /* truncate!(v, strategy, corr) (truncation.jl:330-337) for any strategy given
- * as operations; energy_ket / energy_acc as in pp_evolve_ex (may be NULL). */
-int pp_truncate(pp_handle* h, const pp_strategy_op* ops, int nops, const uint64_t* energy_ket, double* energy_acc);
+ * as operations; energy_ket / energy_acc / variance_acc as in pp_evolve_ex
+ * (may be NULL). */
+int pp_truncate(pp_handle* h, const pp_strategy_op* ops, int nops, const uint64_t* energy_ket, double* energy_acc,
+                double* variance_acc);
+
+/* variance(v, ψ) (statistics.jl:6-22) for a computational-basis ket (W
+ * words): Σ over target kets k' of |⟨k'|v|ψ⟩|^2 minus ⟨ψ|v|ψ⟩^2, with each
+ * ⟨k'|v|ψ⟩ summed in key order as the reference accumulates it and the sum
+ * over k' in double-double (the reference's order over k' is its Dict's). */
+int pp_variance_ket(pp_handle* h, const uint64_t* ket, double* out);
 
 /* get / getindex / haskey (type_SparsePauliVector.jl:286-311): for `count`
  * query keys (W words each in z, x), found[i] = 1 and c[i] = the coefficient

@@ -69,7 +69,9 @@ def lib():
                                 P(OrFilter), P(OrCounters)]
         L.or_evolve_ex.argtypes = [ctypes.c_void_p, u64p, u64p, dp, dp, ctypes.c_int64, ctypes.c_int64,
                                    P(OrFilter), ctypes.c_int, ctypes.c_int64, ctypes.c_double,
-                                   P(OrFilter), u64p, P(ctypes.c_double), P(OrCounters)]
+                                   P(OrFilter), u64p, P(ctypes.c_double), P(ctypes.c_double), P(OrCounters)]
+        L.or_variance.restype = ctypes.c_double
+        L.or_variance.argtypes = [ctypes.c_void_p, u64p]
         L.or_set_sort_mode.argtypes = [ctypes.c_int]
         L.or_compact.argtypes = [ctypes.c_void_p, P(OrFilter)]
         L.or_adaptive.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_double]
@@ -209,10 +211,13 @@ class OracleSPV:
 
     def evolve(self, gens: Sequence[Tuple[int, int]], angles: Sequence[float], window: int = 1,
                strict=None, adaptive: Optional[Tuple[int, float]] = None, local=None,
-               cosv=None, sinv=None, energy: Optional[list] = None) -> OrCounters:
+               cosv=None, sinv=None, energy: Optional[list] = None,
+               variance: Optional[list] = None) -> OrCounters:
         """spv_evolve.jl:312-387.  ``energy``: a one-element list [ket] or
         [ket, accumulated]; on return energy[1] holds EnergyCorrection's
-        accumulated_energy (truncation.jl:262-310)."""
+        accumulated_energy (truncation.jl:262-310).  With ``variance`` (a list,
+        [] or [accumulated]) as well it is EnergyVarianceCorrection: on return
+        variance[0] holds accumulated_variance."""
         from_struct = lambda f: filter_from(f) if f is not None else None
         L = len(gens)
         gz = np.zeros((max(L, 1), self.W), dtype=np.uint64)
@@ -235,11 +240,15 @@ class OracleSPV:
         else:
             ket = pack_int(int(energy[0]), 2)
             acc = ctypes.c_double(energy[1] if len(energy) > 1 else 0.0)
+            vacc = ctypes.c_double(variance[0] if variance else 0.0)
             rc = lib().or_evolve_ex(self.h, _u64p(gz), _u64p(gx), _dp(cv), _dp(sv), L, window,
                                     ctypes.byref(fs), kind, mt, mth, ctypes.byref(fl), _u64p(ket),
-                                    ctypes.byref(acc), ctypes.byref(cnt))
+                                    ctypes.byref(acc), ctypes.byref(vacc) if variance is not None else None,
+                                    ctypes.byref(cnt))
             del energy[1:]
             energy.append(acc.value)
+            if variance is not None:
+                variance[:] = [vacc.value]
         if rc:
             raise RuntimeError(f"oracle evolve rc={rc}")
         return cnt
@@ -270,6 +279,10 @@ class OracleSPV:
 
     def expectation(self, ket: int = 0) -> float:
         return lib().or_expectation(self.h, _u64p(pack_int(ket, self.W)))
+
+    def variance(self, ket: int = 0) -> float:
+        """statistics.jl:6-22 on the merged live set."""
+        return lib().or_variance(self.h, _u64p(pack_int(ket, self.W)))
 
 
 def majorana_weight_bits(N: int, z: int, x: int) -> int:

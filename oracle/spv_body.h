@@ -438,12 +438,77 @@ static void F(apply_adaptive)(F(or_spv) *v, int64_t max_terms, double min_thresh
     F(compact)(v, &f);
 }
 
-/* spv_evolve.jl:258-284 — merge boundary, with the EnergyCorrection
- * accumulator (the non-linear EnergyVarianceCorrection is not restated). */
+/* statistics.jl:6-22 — variance(O, ψ) of the merged live set for a
+ * computational-basis ket: σ = O|ψ⟩ accumulated per target ket in the
+ * operator's iteration (key) order, P|k⟩ = i^((symplectic_phase + 2 sign) % 4)
+ * |k ⊻ x⟩ (multiplication.jl:48-52, type_Pauli.jl:64), then
+ * Σ_k' |σ[k']|^2 - σ[ψ]^2.  σ is a Dict whose iteration order is Julia's
+ * hash order; the sum over k' is taken here in increasing k' (= x ⊻ ψ order
+ * of the groups by x), a last-ulp difference. */
+typedef struct {
+    WT x;
+    int64_t i;
+} F(or_xi);
+static int F(cmp_xi)(const void *pa, const void *pb) {
+    const F(or_xi) *a = (const F(or_xi) *)pa, *b = (const F(or_xi) *)pb;
+    if (a->x != b->x) return a->x < b->x ? -1 : 1;
+    return a->i < b->i ? -1 : (a->i > b->i ? 1 : 0);
+}
+static double F(variance)(const F(or_spv) *v, WT kv) {
+    const int64_t n = v->n;
+    if (n == 0) return 0.0;
+    F(or_xi) *ix = (F(or_xi) *)malloc((size_t)n * sizeof(F(or_xi)));
+    for (int64_t i = 0; i < n; i++) {
+        ix[i].x = v->x[i];
+        ix[i].i = i;
+    }
+    qsort(ix, (size_t)n, sizeof(F(or_xi)), F(cmp_xi));  /* by x, then key order */
+    double e2 = 0.0, e1 = 0.0;
+    for (int64_t g = 0; g < n;) {
+        const WT x = ix[g].x;
+        double re = 0.0, im = 0.0;
+        int64_t j = g;
+        for (; j < n && ix[j].x == x; j++) {
+            const int64_t q = ix[j].i;
+            const WT z = v->z[q];
+            const int sign = F(popc)(z & (x ^ kv)) & 1;
+            const int sp = (4 - F(popc)(z & x) % 4) % 4;
+            const int ph = (sp + 2 * sign) % 4;  /* PHASE_TBL = [1, i, -1, -i] */
+            const double c = v->c[q];
+            if (ph == 0) re = re + c;
+            else if (ph == 1) im = im + c;
+            else if (ph == 2) re = re + (-c);
+            else im = im + (-c);
+        }
+        e2 = e2 + (re * re + im * im);
+        if (x == (WT)0) e1 = re;
+        g = j;
+    }
+    free(ix);
+    return e2 - e1 * e1;
+}
+
+/* spv_evolve.jl:246-284 — merge boundary with the correction accumulators.
+ * EnergyCorrection measures the pre-merge state (live + pending appends);
+ * EnergyVarianceCorrection needs the merged state: merge with NOFILTER,
+ * measure, then the strict filter as a compaction (or the strategy). */
 static void F(boundary)(F(or_spv) *v, const or_filter_t *f, or_strategy_t *s,
                         int have_mask, WT lz, WT lx, int64_t *n_in, int64_t *n_out) {
     const WT kv = (WT)((((u128)s->ket[1]) << 64) | (u128)s->ket[0]);
-    /* EnergyCorrection measures the pre-merge state (spv_evolve.jl:246-251) */
+    if (s->variance) {
+        or_filter_t nof;
+        or_filter_init_nofilter(&nof);
+        int64_t m = F(gather_append)(v);
+        if (have_mask) F(unshuffle_ws)(v, m, lz, lx);
+        else F(sort_pending)(v->ws, m);
+        F(merge_spv)(v, m, &nof, n_in, n_out);
+        const double e0 = F(expectation)(v, kv), v0 = F(variance)(v, kv);
+        if (s->kind == OR_STRAT_ADAPTIVE) F(apply_adaptive)(v, s->max_terms, s->min_thresh);
+        else F(compact)(v, f);
+        s->energy_acc += F(expectation)(v, kv) - e0;  /* truncation.jl:308-310 */
+        s->variance_acc += F(variance)(v, kv) - v0;
+        return;
+    }
     const double before = s->energy ? F(expectation)(v, kv) : 0.0;
     int64_t m = F(gather_append)(v);
     if (have_mask) F(unshuffle_ws)(v, m, lz, lx);

@@ -33,17 +33,21 @@ typedef struct {
 } or_filter_t;
 
 enum { OR_STRAT_COMPILED = 0, OR_STRAT_ADAPTIVE = 1 };
-/* The boundary strategy plus the EnergyCorrection accumulator
- * (truncation.jl:262-310): energy != 0 measures <ket|O|ket> over live +
- * pending appends before the merge and over the result after it, and adds
- * after - before to energy_acc. */
+/* The boundary strategy plus the correction accumulators
+ * (truncation.jl:262-310): energy != 0 (EnergyCorrection) measures
+ * <ket|O|ket> over live + pending appends before the merge and over the
+ * result after it, and adds after - before to energy_acc; variance != 0
+ * (EnergyVarianceCorrection) measures energy and variance on the merged state
+ * (spv_evolve.jl:246-270) and accumulates both. */
 typedef struct {
     int kind;
     int64_t max_terms;
     double min_thresh;
     int energy;
+    int variance;
     uint64_t ket[2];
     double energy_acc;
+    double variance_acc;
 } or_strategy_t;
 
 /* Sort used at merge boundaries with more than one contributing rotation.
@@ -256,10 +260,11 @@ static int or_evolve_strat(or_handle *h, const uint64_t *gz, const uint64_t *gx,
                            const double *sinv, int64_t L, int64_t window, const or_filter_t *strict,
                            or_strategy_t *sp, const or_filter_t *local, or_counters_t *cnt);
 
+/* variance_acc != NULL (with energy_ket) is EnergyVarianceCorrection. */
 int or_evolve_ex(or_handle *h, const uint64_t *gz, const uint64_t *gx, const double *cosv,
                  const double *sinv, int64_t L, int64_t window, const or_filter_t *strict,
                  int strat_kind, int64_t max_terms, double min_thresh, const or_filter_t *local,
-                 const uint64_t *energy_ket, double *energy_acc, or_counters_t *cnt) {
+                 const uint64_t *energy_ket, double *energy_acc, double *variance_acc, or_counters_t *cnt) {
     if (window < 1) return -1;
     if (L == 0) return 0;
     or_strategy_t s;
@@ -269,13 +274,22 @@ int or_evolve_ex(or_handle *h, const uint64_t *gz, const uint64_t *gx, const dou
     s.min_thresh = min_thresh;
     if (energy_ket) {
         s.energy = 1;
+        s.variance = variance_acc != NULL;
         s.ket[0] = energy_ket[0];
         s.ket[1] = h->W == 2 ? energy_ket[1] : 0;
         s.energy_acc = energy_acc ? *energy_acc : 0.0;
+        s.variance_acc = variance_acc ? *variance_acc : 0.0;
     }
     int rc = or_evolve_strat(h, gz, gx, cosv, sinv, L, window, strict, &s, local, cnt);
     if (energy_acc) *energy_acc = s.energy_acc;
+    if (variance_acc) *variance_acc = s.variance_acc;
     return rc;
+}
+
+/* variance(v, ψ) (statistics.jl:6-22) for a computational-basis ket. */
+double or_variance(or_handle *h, const uint64_t *ket) {
+    if (h->W == 1) return variance_w1((or_spv_w1 *)h->v, (uint64_t)ket[0]);
+    return variance_w2((or_spv_w2 *)h->v, ld2(ket));
 }
 
 void or_set_sort_mode(int mode) { or_sort_mode = mode; }
@@ -285,7 +299,7 @@ int or_evolve(or_handle *h, const uint64_t *gz, const uint64_t *gx, const double
               int strat_kind, int64_t max_terms, double min_thresh, const or_filter_t *local,
               or_counters_t *cnt) {
     return or_evolve_ex(h, gz, gx, cosv, sinv, L, window, strict, strat_kind, max_terms, min_thresh,
-                        local, NULL, NULL, cnt);
+                        local, NULL, NULL, NULL, cnt);
 }
 
 static int or_evolve_strat(or_handle *h, const uint64_t *gz, const uint64_t *gx, const double *cosv,

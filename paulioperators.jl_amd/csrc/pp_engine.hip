@@ -206,6 +206,7 @@ struct WinBuf {
     unsigned long long* qhist = nullptr;  // [256]
     double* out = nullptr;
     DevArena alg;                      // operands, pair coefficients and results of pp_algebra
+    DevArena vs;                       // variance: tile totals, head flags, carries
     uint32_t epoch = 0;
     uint64_t rng_calls = 0;
 };
@@ -1217,6 +1218,7 @@ struct Batch {
         dfree(w.partial); dfree(w.gens); dfree(w.filt); dfree(w.small); dfree(w.err); dfree(w.ticket);
         dfree(w.orand); dfree(w.orpart); dfree(w.qhist); dfree(w.out);
         w.alg.release();
+        w.vs.release();
         w = WinBuf();
     }
 
@@ -1230,7 +1232,7 @@ struct Batch {
         w.orand = dalloc<unsigned long long>(4 * (size_t)W);
         w.orpart = dalloc<unsigned long long>((size_t)kOrandBlocks * 4 * W);
         w.qhist = dalloc<unsigned long long>(kRadix);
-        w.out = dalloc<double>(1);
+        w.out = dalloc<double>(2);
         w.partial = dalloc<double2>(1024);
         HIPCHK(hipMemsetAsync(w.err, 0, sizeof(int32_t), stream));
     }
@@ -1510,6 +1512,42 @@ struct Batch {
         return v;
     }
 
+    // variance(O, ψ) of the merged live set (statistics.jl:6-22): the terms
+    // copied with x first into the append buffer (free at a merged state),
+    // radix-sorted by (x, z), then summed per x group.
+    template <int Wt>
+    double win_variance(int64_t n, int cur, const uint64_t* ket) {
+        WinBuf& w = win;
+        if (n == 0) return 0.0;
+        grow_set(w.A, n, 0);
+        const DevSet& Li = ib[0].L[cur];
+        const unsigned g1 = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(n, kBlock), 4096));
+        hipLaunchKernelGGL(kv_swap<Wt>, dim3(g1), dim3(kBlock), 0, stream, Li.z, Li.x, Li.c, n, w.A.z, w.A.x,
+                           w.A.c);
+        const DevSet* S = win_sort<Wt>(n);
+        Key<Wt> kk;
+        for (int q = 0; q < Wt; ++q) {
+            kk.z[q] = ket[q];
+            kk.x[q] = 0;
+        }
+        const int64_t ntiles = cdiv(n, kVarTile);
+        double2* tile_v = w.vs.get<double2>(0, (size_t)ntiles);
+        int32_t* tile_h = w.vs.get<int32_t>(1, (size_t)ntiles);
+        double2* carry = w.vs.get<double2>(2, (size_t)ntiles);
+        const unsigned grid = (unsigned)std::min<int64_t>(ntiles, 1024);
+        hipLaunchKernelGGL((kv_tiles<Wt, false>), dim3(grid), dim3(kBlock), 0, stream, S->z, S->x, S->c, n, kk,
+                           tile_v, tile_h, carry, w.partial, w.out);
+        hipLaunchKernelGGL(kv_carry, dim3(1), dim3(kBlock), 0, stream, tile_v, tile_h, ntiles, carry, w.out);
+        hipLaunchKernelGGL((kv_tiles<Wt, true>), dim3(grid), dim3(kBlock), 0, stream, S->z, S->x, S->c, n, kk,
+                           tile_v, tile_h, carry, w.partial, w.out);
+        hipLaunchKernelGGL(kw_expect_final, dim3(1), dim3(kBlock), 0, stream, w.partial, (int64_t)grid, w.out);
+        HIPCHK(hipGetLastError());
+        double v[2] = {0.0, 0.0};
+        HIPCHK(hipMemcpyAsync(v, w.out, 2 * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        return v[0] - v[1] * v[1];
+    }
+
     // _apply! of a non-compilable strategy: its operations in order.
     template <int Wt>
     int64_t win_apply_ops(int64_t n, int& cur, const pp_strategy_op* ops, int nops) {
@@ -1543,6 +1581,8 @@ struct Batch {
         int nops = 0;
         const uint64_t* ket = nullptr;        // EnergyCorrection ket (null: none)
         double energy = 0.0;
+        bool var = false;                     // EnergyVarianceCorrection (with ket)
+        double variance = 0.0;
     };
 
     template <int Wt>
@@ -1563,12 +1603,37 @@ struct Batch {
             w.gens = dalloc<GenDev>(L);
         }
         HIPCHK(hipMemcpyAsync(w.gens, gens.data(), (size_t)L * sizeof(GenDev), hipMemcpyHostToDevice, stream));
-        win_filter(0, a.nops > 0 ? nullptr : strict);
+        // EnergyVarianceCorrection merges with NOFILTER and applies the strict
+        // filter as a compaction afterwards (spv_evolve.jl:264-270)
+        win_filter(0, a.nops > 0 || a.var ? nullptr : strict);
         win_filter(1, local);
         pp_counters c{};
         double t_merge = 0.0;
         auto boundary = [&](bool early) {
             const auto t0 = std::chrono::steady_clock::now();
+            if (a.var) {
+                const int64_t m = an;
+                DevSet* S = win_sort<Wt>(m);
+                const int64_t out = win_merge<Wt>(n, cur, S, m, 0);
+                if (out > live_cap) live_cap = std::max(2 * live_cap, n + m);
+                c.merge_in += n + m;
+                c.merge_out += out;
+                cur ^= 1;
+                n = out;
+                an = 0;
+                const double e0 = win_expect<Wt>(n, cur, 0, a.ket), v0 = win_variance<Wt>(n, cur, a.ket);
+                if (a.nops > 0) {
+                    n = win_apply_ops<Wt>(n, cur, a.ops, a.nops);
+                } else if (strict) {
+                    win_filter(2, strict);
+                    n = win_compact<Wt>(n, cur, 0, 0.0, 0);
+                }
+                a.energy += win_expect<Wt>(n, cur, 0, a.ket) - e0;
+                a.variance += win_variance<Wt>(n, cur, a.ket) - v0;
+                if (early) c.early_merges += 1;
+                t_merge += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                return;
+            }
             const double before = a.ket ? win_expect<Wt>(n, cur, an, a.ket) : 0.0;
             const int64_t m = an;
             DevSet* S = win_sort<Wt>(m);
@@ -1629,15 +1694,17 @@ struct Batch {
 
     // truncate!(v, strategy, corr) for any strategy (truncation.jl:330-337).
     template <int Wt>
-    void truncate_ops(const pp_strategy_op* ops, int nops, const uint64_t* ket, double* energy) {
+    void truncate_ops(const pp_strategy_op* ops, int nops, const uint64_t* ket, double* energy, double* variance) {
         win_init();
         sync_state();
         SegState& st = h_state[0];
         int cur = st.cur;
         int64_t n = st.n[cur];
         const double before = ket ? win_expect<Wt>(n, cur, 0, ket) : 0.0;
+        const double vbefore = ket && variance ? win_variance<Wt>(n, cur, ket) : 0.0;
         n = win_apply_ops<Wt>(n, cur, ops, nops);
         if (ket) *energy += win_expect<Wt>(n, cur, 0, ket) - before;
+        if (ket && variance) *variance += win_variance<Wt>(n, cur, ket) - vbefore;
         st.cur = cur;
         st.n[cur] = n;
         st.n[cur ^ 1] = 0;
@@ -2287,7 +2354,7 @@ int pp_evolve(pp_handle* h, const uint64_t* gz, const uint64_t* gx, const double
     if (counters) std::memset(counters, 0, sizeof(*counters));
     if (L == 0) return PP_OK;
     if (window != 1) return pp_evolve_ex(h, gz, gx, cosv, sinv, L, window, strict, nullptr, 0, local, nullptr,
-                                         nullptr, counters);
+                                         nullptr, nullptr, counters);
     API_BEGIN
     Batch& bt = h->bt;
     bt.timing = counters != nullptr;
@@ -2327,12 +2394,14 @@ static void check_ops(const pp_strategy_op* ops, int nops) {
 
 int pp_evolve_ex(pp_handle* h, const uint64_t* gz, const uint64_t* gx, const double* cosv, const double* sinv,
                  int64_t L, int64_t window, const pp_filter* strict, const pp_strategy_op* ops, int nops,
-                 const pp_filter* local, const uint64_t* energy_ket, double* energy_acc, pp_counters* counters) {
+                 const pp_filter* local, const uint64_t* energy_ket, double* energy_acc, double* variance_acc,
+                 pp_counters* counters) {
     if (!h) return fail(PP_EARG, "null handle");
     if (window < 1) return fail(PP_EARG, "window must be >= 1");
     if (L < 0) return fail(PP_EARG, "L must be >= 0");
     if (L > 0 && (!gz || !gx || !cosv || !sinv)) return fail(PP_EARG, "null generator arrays");
     if (energy_ket && !energy_acc) return fail(PP_EARG, "energy_acc is NULL");
+    if (variance_acc && !energy_ket) return fail(PP_EARG, "the variance correction needs energy_ket");
     if (counters) std::memset(counters, 0, sizeof(*counters));
     if (L == 0) return PP_OK;
     API_BEGIN
@@ -2345,19 +2414,41 @@ int pp_evolve_ex(pp_handle* h, const uint64_t* gz, const uint64_t* gx, const dou
     a.nops = nops;
     a.ket = energy_ket;
     a.energy = energy_ket ? *energy_acc : 0.0;
+    a.var = variance_acc != nullptr;
+    a.variance = variance_acc ? *variance_acc : 0.0;
     dispatch_w(bt.W, [&](auto wc) { bt.evolve_window<decltype(wc)::value>(gens, window, strict, local, a, counters); });
     if (energy_ket) *energy_acc = a.energy;
+    if (variance_acc) *variance_acc = a.variance;
     API_END
 }
 
-int pp_truncate(pp_handle* h, const pp_strategy_op* ops, int nops, const uint64_t* energy_ket, double* energy_acc) {
+int pp_truncate(pp_handle* h, const pp_strategy_op* ops, int nops, const uint64_t* energy_ket, double* energy_acc,
+                double* variance_acc) {
     if (!h) return fail(PP_EARG, "null handle");
     if (energy_ket && !energy_acc) return fail(PP_EARG, "energy_acc is NULL");
+    if (variance_acc && !energy_ket) return fail(PP_EARG, "the variance correction needs energy_ket");
     API_BEGIN
     check_ops(ops, nops);
     double e = energy_ket ? *energy_acc : 0.0;
-    dispatch_w(h->bt.W, [&](auto wc) { h->bt.truncate_ops<decltype(wc)::value>(ops, nops, energy_ket, &e); });
+    double v = variance_acc ? *variance_acc : 0.0;
+    dispatch_w(h->bt.W, [&](auto wc) {
+        h->bt.truncate_ops<decltype(wc)::value>(ops, nops, energy_ket, &e, variance_acc ? &v : nullptr);
+    });
     if (energy_ket) *energy_acc = e;
+    if (variance_acc) *variance_acc = v;
+    API_END
+}
+
+int pp_variance_ket(pp_handle* h, const uint64_t* ket, double* out) {
+    if (!h || !ket || !out) return fail(PP_EARG, "null argument");
+    API_BEGIN
+    Batch& bt = h->bt;
+    bt.win_init();
+    bt.sync_state();
+    const SegState& st = bt.h_state[0];
+    const int64_t n = st.n[st.cur];
+    const int cur = st.cur;
+    dispatch_w(bt.W, [&](auto wc) { *out = bt.win_variance<decltype(wc)::value>(n, cur, ket); });
     API_END
 }
 

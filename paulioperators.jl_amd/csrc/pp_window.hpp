@@ -752,6 +752,177 @@ __global__ void __launch_bounds__(kBlock) kw_expect(const uint64_t* __restrict__
     if (threadIdx.x == 0) partial[blockIdx.x] = make_double2(s_h[0], s_l[0]);
 }
 
+// variance(O, ψ) of the live set (statistics.jl:6-22), step 1: the terms
+// with their words swapped (x first), so a sort by key orders them by x and,
+// within one x, by z (the operator's iteration order).
+template <int W>
+__global__ void __launch_bounds__(kBlock) kv_swap(const uint64_t* __restrict__ Lz, const uint64_t* __restrict__ Lx,
+                                                  const double* __restrict__ Lc, int64_t n, uint64_t* __restrict__ Az,
+                                                  uint64_t* __restrict__ Ax, double* __restrict__ Ac) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            Az[i * W + w] = Lx[i * W + w];
+            Ax[i * W + w] = Lz[i * W + w];
+        }
+        Ac[i] = Lc[i];
+    }
+}
+
+// Steps 2-4, on the terms sorted by (x, z): σ[ψ ⊻ x] = Σ i^((symplectic_phase
+// + 2 sign) % 4) c over each x group (multiplication.jl:48-52: sign =
+// popcount(z & (x ⊻ ψ)), symplectic_phase = (4 - popcount(z & x) % 4) % 4),
+// then Σ |σ|^2.  The group sums are a segmented scan over tiles of kVarTile
+// terms: kv_tiles<W, false> reduces each tile to (head seen, sum since the
+// tile's last group head), kv_carry scans those into the sum open at each
+// tile's start, and kv_tiles<W, true> rescans each tile from its carry, adding
+// |σ|^2 at the last term of every group to a double-double block partial.  The
+// group x = 0 (the first, when present) also gives σ[ψ] in out[1].  The
+// addition order is a fixed tree, so results repeat bit for bit.
+constexpr int kVarPer = 4;                     // consecutive terms per thread
+constexpr int kVarTile = kBlock * kVarPer;     // terms per tile
+
+struct VarSeg {
+    double re, im;
+    int head;  // a group starts inside the segment; (re, im) is the sum since its last start
+};
+
+__device__ __forceinline__ VarSeg var_join(const VarSeg& a, const VarSeg& b) {
+    return b.head ? b : VarSeg{a.re + b.re, a.im + b.im, a.head};
+}
+
+template <int W>
+__device__ __forceinline__ bool var_same_x(const uint64_t* __restrict__ Sz, int64_t i, int64_t j) {
+    bool same = true;
+#pragma unroll
+    for (int w = 0; w < W; ++w) same = same && Sz[i * W + w] == Sz[j * W + w];
+    return same;
+}
+
+// Term j's phased coefficient; Sz holds the x words, Sx the z words.
+template <int W>
+__device__ __forceinline__ VarSeg var_term(const uint64_t* __restrict__ Sz, const uint64_t* __restrict__ Sx,
+                                           const double* __restrict__ Sc, int64_t j, const Key<W>& ket) {
+    int sgn = 0, zx = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        const uint64_t x = Sz[j * W + w], z = Sx[j * W + w];
+        sgn += popc64(z & (x ^ ket.z[w]));
+        zx += popc64(z & x);
+    }
+    const int ph = ((4 - zx % 4) % 4 + 2 * (sgn & 1)) % 4;  // PHASE_TBL = [1, i, -1, -i]
+    const double c = Sc[j];
+    const int head = j == 0 || !var_same_x<W>(Sz, j, j - 1);
+    if (ph == 0) return VarSeg{c, 0.0, head};
+    if (ph == 1) return VarSeg{0.0, c, head};
+    if (ph == 2) return VarSeg{-c, 0.0, head};
+    return VarSeg{0.0, -c, head};
+}
+
+// Exclusive scan of the threads' segments in thread order, seeded by `seed`;
+// returns this thread's prefix and leaves the tile's total in `total`.
+__device__ __forceinline__ VarSeg var_block_scan(VarSeg mine, const VarSeg& seed, VarSeg& total, double* s_re,
+                                                 double* s_im, int* s_hd) {
+    const int t = threadIdx.x;
+    for (int off = 1; off < kBlock; off <<= 1) {
+        s_re[t] = mine.re;
+        s_im[t] = mine.im;
+        s_hd[t] = mine.head;
+        __syncthreads();
+        if (t >= off) mine = var_join(VarSeg{s_re[t - off], s_im[t - off], s_hd[t - off]}, mine);
+        __syncthreads();
+    }
+    s_re[t] = mine.re;
+    s_im[t] = mine.im;
+    s_hd[t] = mine.head;
+    __syncthreads();
+    const VarSeg prev = t > 0 ? VarSeg{s_re[t - 1], s_im[t - 1], s_hd[t - 1]} : VarSeg{0.0, 0.0, 0};
+    total = var_join(seed, VarSeg{s_re[kBlock - 1], s_im[kBlock - 1], s_hd[kBlock - 1]});
+    __syncthreads();
+    return t > 0 ? var_join(seed, prev) : seed;
+}
+
+template <int W, bool kFinal>
+__global__ void __launch_bounds__(kBlock) kv_tiles(const uint64_t* __restrict__ Sz, const uint64_t* __restrict__ Sx,
+                                                   const double* __restrict__ Sc, int64_t n, Key<W> ket,
+                                                   double2* __restrict__ tile_v, int32_t* __restrict__ tile_h,
+                                                   const double2* __restrict__ carry, double2* __restrict__ partial,
+                                                   double* __restrict__ out) {
+    __shared__ double s_re[kBlock], s_im[kBlock];
+    __shared__ int s_hd[kBlock];
+    __shared__ double s_h[kBlock], s_l[kBlock];
+    double h = 0.0, l = 0.0;
+    const int64_t ntiles = (n + kVarTile - 1) / kVarTile;
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int64_t j0 = tile * kVarTile + (int64_t)threadIdx.x * kVarPer;
+        VarSeg term[kVarPer];
+        VarSeg mine{0.0, 0.0, 0};
+#pragma unroll
+        for (int e = 0; e < kVarPer; ++e) {
+            term[e] = j0 + e < n ? var_term<W>(Sz, Sx, Sc, j0 + e, ket) : VarSeg{0.0, 0.0, 0};
+            mine = var_join(mine, term[e]);
+        }
+        const VarSeg seed = kFinal ? VarSeg{carry[tile].x, carry[tile].y, 0} : VarSeg{0.0, 0.0, 0};
+        VarSeg total;
+        VarSeg run = var_block_scan(mine, seed, total, s_re, s_im, s_hd);
+        if (!kFinal) {
+            if (threadIdx.x == 0) {
+                tile_v[tile] = make_double2(total.re, total.im);
+                tile_h[tile] = total.head;
+            }
+            continue;
+        }
+#pragma unroll
+        for (int e = 0; e < kVarPer; ++e) {
+            const int64_t j = j0 + e;
+            if (j >= n) break;
+            run = var_join(run, term[e]);
+            if (j + 1 == n || !var_same_x<W>(Sz, j, j + 1)) {
+                dd_add(h, l, run.re * run.re + run.im * run.im, 0.0);
+                bool zero = true;
+#pragma unroll
+                for (int w = 0; w < W; ++w) zero = zero && Sz[j * W + w] == 0;
+                if (zero) out[1] = run.re;
+            }
+        }
+    }
+    if (!kFinal) return;
+    s_h[threadIdx.x] = h;
+    s_l[threadIdx.x] = l;
+    __syncthreads();
+    for (int sh = kBlock / 2; sh > 0; sh >>= 1) {
+        if ((int)threadIdx.x < sh) {
+            double a = s_h[threadIdx.x], c = s_l[threadIdx.x];
+            dd_add(a, c, s_h[threadIdx.x + sh], s_l[threadIdx.x + sh]);
+            s_h[threadIdx.x] = a;
+            s_l[threadIdx.x] = c;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = make_double2(s_h[0], s_l[0]);
+}
+
+// One workgroup: carry[t] = the group sum open at tile t's start (the
+// exclusive segmented scan of the tile totals); also clears out[1], which the
+// final pass sets when a group x = 0 exists.
+__global__ void __launch_bounds__(kBlock) kv_carry(const double2* __restrict__ tile_v,
+                                                   const int32_t* __restrict__ tile_h, int64_t ntiles,
+                                                   double2* __restrict__ carry, double* __restrict__ out) {
+    __shared__ double s_re[kBlock], s_im[kBlock];
+    __shared__ int s_hd[kBlock];
+    const int64_t per = (ntiles + kBlock - 1) / kBlock;
+    const int64_t t0 = (int64_t)threadIdx.x * per, t1 = t0 + per < ntiles ? t0 + per : ntiles;
+    VarSeg mine{0.0, 0.0, 0};
+    for (int64_t t = t0; t < t1; ++t) mine = var_join(mine, VarSeg{tile_v[t].x, tile_v[t].y, tile_h[t]});
+    VarSeg total;
+    VarSeg run = var_block_scan(mine, VarSeg{0.0, 0.0, 0}, total, s_re, s_im, s_hd);
+    for (int64_t t = t0; t < t1; ++t) {
+        carry[t] = make_double2(run.re, run.im);
+        run = var_join(run, VarSeg{tile_v[t].x, tile_v[t].y, tile_h[t]});
+    }
+    if (threadIdx.x == 0) out[1] = 0.0;
+}
+
 __global__ void __launch_bounds__(kBlock) kw_expect_final(const double2* __restrict__ partial, int64_t np,
                                                           double* __restrict__ out) {
     __shared__ double s_h[kBlock], s_l[kBlock];

@@ -20,7 +20,7 @@ import PauliOperators: evolve!, expectation_value, truncate!, stochastic_clip!,
                        x_weight_damped_clip!, majorana_weight_clip!,
                        depolarizing_channel!, dephasing_channel!, bit_flip_channel!,
                        bit_phase_flip_channel!, pauli_channel!,
-                       commutator, anticommutator, inner_product, check_spv
+                       commutator, anticommutator, inner_product, check_spv, variance
 using PauliOperators: MergeFilter, NOFILTER, _compile_filter, _is_compilable, _pack, _spv_mask,
                       TruncationStrategy, NoTruncation, CoeffTruncation, WeightTruncation,
                       WeightDampedTruncation, XWeightTruncation, XWeightDampedTruncation,
@@ -109,12 +109,23 @@ function _strategy_ops!(ops::Vector{PPStrategyOp}, s::TruncationStrategy)
     return ops
 end
 
-# EnergyCorrection(ψ) as the (ket words, accumulator) pair of the ABI
+# EnergyCorrection(ψ) / EnergyVarianceCorrection(ψ) as the ABI's ket words and
+# accumulators (the variance accumulator is C_NULL for EnergyCorrection)
 _ket_words(::Type{W}, ψ::Ket) where {W} = Ref{W}((ψ.v % UInt128) % W)
 function _correction_args(::Type{W}, corr::CorrectionAccumulator) where {W}
-    corr isa NoCorrection && return (C_NULL, Ref{Float64}(0.0))
-    corr isa EnergyCorrection && return (_ket_words(W, corr.ψ), Ref{Float64}(corr.accumulated_energy))
-    throw(ArgumentError("$(typeof(corr)) is not available on the device (NoCorrection and EnergyCorrection are)"))
+    corr isa NoCorrection && return (C_NULL, Ref{Float64}(0.0), C_NULL)
+    corr isa EnergyCorrection &&
+        return (_ket_words(W, corr.ψ), Ref{Float64}(corr.accumulated_energy), C_NULL)
+    corr isa EnergyVarianceCorrection &&
+        return (_ket_words(W, corr.ψ), Ref{Float64}(corr.accumulated_energy), Ref{Float64}(corr.accumulated_variance))
+    throw(ArgumentError("$(typeof(corr)) has no device implementation"))
+end
+function _correction_back!(corr::CorrectionAccumulator, acc, vacc)
+    if corr isa EnergyCorrection || corr isa EnergyVarianceCorrection
+        corr.accumulated_energy = acc[]
+    end
+    corr isa EnergyVarianceCorrection && (corr.accumulated_variance = vacc[])
+    return corr
 end
 
 # pp_counters of a call into a WindowCounters: the device keeps call totals,
@@ -235,13 +246,13 @@ function evolve!(d::DeviceSPV{N,W}, generators::Vector{PauliBasis{N}}, angles::V
         compiled = _is_compilable(truncation)
         fs = Ref(compiled ? _compile_filter(truncation) : NOFILTER)
         ops = compiled ? PPStrategyOp[] : _strategy_ops!(PPStrategyOp[], truncation)
-        ket, acc = _correction_args(W, correction)
+        ket, acc, vacc = _correction_args(W, correction)
         check(ccall((:pp_evolve_ex, LIB), Cint,
                     (Ptr{Cvoid}, Ptr{W}, Ptr{W}, Ptr{Float64}, Ptr{Float64}, Int64, Int64,
                      Ref{MergeFilter}, Ptr{PPStrategyOp}, Cint, Ref{MergeFilter}, Ptr{Cvoid},
-                     Ref{Float64}, Ref{PPCounters}),
-                    d.h, gz, gx, cosv, sinv, L, window, fs, ops, length(ops), fl, ket, acc, cnt))
-        correction isa EnergyCorrection && (correction.accumulated_energy = acc[])
+                     Ref{Float64}, Ptr{Cvoid}, Ref{PPCounters}),
+                    d.h, gz, gx, cosv, sinv, L, window, fs, ops, length(ops), fl, ket, acc, vacc, cnt))
+        _correction_back!(correction, acc, vacc)
     end
     counters === nothing || _add_counters!(counters, cnt[])
     return d
@@ -263,10 +274,10 @@ function truncate!(d::DeviceSPV{N,W}, strategy::TruncationStrategy,
         return d
     end
     ops = _strategy_ops!(PPStrategyOp[], strategy)
-    ket, acc = _correction_args(W, corr)
-    check(ccall((:pp_truncate, LIB), Cint, (Ptr{Cvoid}, Ptr{PPStrategyOp}, Cint, Ptr{Cvoid}, Ref{Float64}),
-                d.h, ops, length(ops), ket, acc))
-    corr isa EnergyCorrection && (corr.accumulated_energy = acc[])
+    ket, acc, vacc = _correction_args(W, corr)
+    check(ccall((:pp_truncate, LIB), Cint, (Ptr{Cvoid}, Ptr{PPStrategyOp}, Cint, Ptr{Cvoid}, Ref{Float64}, Ptr{Cvoid}),
+                d.h, ops, length(ops), ket, acc, vacc))
+    _correction_back!(corr, acc, vacc)
     return d
 end
 
@@ -288,8 +299,8 @@ majorana_weight_clip!(d::DeviceSPV, max_weight::Int) =
 # counter-based generator seeded from rng
 function stochastic_clip!(d::DeviceSPV{N,W}, ε::Real; rng::AbstractRNG=Random.default_rng()) where {N,W}
     ops = [PPStrategyOp(PP_OP_STOCHASTIC_COEFF, 0, NOFILTER, 0, Float64(ε), rand(rng, UInt64))]
-    check(ccall((:pp_truncate, LIB), Cint, (Ptr{Cvoid}, Ptr{PPStrategyOp}, Cint, Ptr{Cvoid}, Ref{Float64}),
-                d.h, ops, 1, C_NULL, Ref{Float64}(0.0)))
+    check(ccall((:pp_truncate, LIB), Cint, (Ptr{Cvoid}, Ptr{PPStrategyOp}, Cint, Ptr{Cvoid}, Ref{Float64}, Ptr{Cvoid}),
+                d.h, ops, 1, C_NULL, Ref{Float64}(0.0), C_NULL))
     return d
 end
 
@@ -337,6 +348,13 @@ end
 function expectation_value(d::DeviceSPV{N,W}, ψ::Ket{N}) where {N,W}
     out = Ref{Float64}(0.0)
     check(ccall((:pp_expectation_ket, LIB), Cint, (Ptr{Cvoid}, Ref{W}, Ref{Float64}), d.h, _ket_words(W, ψ), out))
+    return out[]
+end
+
+# variance(v, ψ) (statistics.jl:6-22)
+function variance(d::DeviceSPV{N,W}, ψ::Ket{N}) where {N,W}
+    out = Ref{Float64}(0.0)
+    check(ccall((:pp_variance_ket, LIB), Cint, (Ptr{Cvoid}, Ref{W}, Ref{Float64}), d.h, _ket_words(W, ψ), out))
     return out[]
 end
 

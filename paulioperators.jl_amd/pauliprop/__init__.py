@@ -17,6 +17,6 @@ from .spv import (SparsePauliVector, bit_flip_channel, bit_flip_channel_, bit_ph
                   bit_phase_flip_channel_, dephasing_channel, dephasing_channel_, depolarizing_channel,
                   depolarizing_channel_, depolarizing_p_for_weight_decay, evolve, evolve_, expectation_value,
                   inner_product, isapprox, norm,
-                  pauli_channel, pauli_channel_, phase_flip_channel_, stochastic_clip_, truncate_)
+                  pauli_channel, pauli_channel_, phase_flip_channel_, stochastic_clip_, truncate_, variance)
 from .batch import SparsePauliBatch
 from .algebra import anticommutator, commutator, multiply

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(PKG_DIR, "_build", "libpauliprop.so")
 PP_OK, PP_EARG, PP_ENOMEM, PP_EHIP, PP_EOVERFLOW, PP_EUNSUPPORTED = range(6)
 
 EXPORTS = ("pp_create", "pp_destroy", "pp_upload", "pp_download", "pp_size", "pp_capacities", "pp_evolve",
-           "pp_evolve_ex", "pp_truncate", "pp_evolve_single", "pp_compact", "pp_lookup", "pp_lookup_noalloc", "pp_norm",
+           "pp_evolve_ex", "pp_truncate", "pp_variance_ket", "pp_evolve_single", "pp_compact", "pp_lookup", "pp_lookup_noalloc", "pp_norm",
            "pp_inner_product", "pp_diff_norm", "pp_setindex", "pp_delete", "pp_sum", "pp_scale", "pp_scale_by_popcount", "pp_expectation_ket", "pp_check",
            "pp_algebra",
            "pp_batch_create", "pp_batch_destroy", "pp_batch_words", "pp_batch_upload", "pp_batch_download",
@@ -101,8 +101,9 @@ def lib():
     L.pp_capacities.argtypes = [vp, P(i64), P(i64)]
     L.pp_evolve.argtypes = [vp, u64p, u64p, dp, dp, i64, i64, P(PPFilter), P(PPFilter), P(PPCounters)]
     L.pp_evolve_ex.argtypes = [vp, u64p, u64p, dp, dp, i64, i64, P(PPFilter), P(PPStrategyOp), ctypes.c_int,
-                               P(PPFilter), u64p, dp, P(PPCounters)]
-    L.pp_truncate.argtypes = [vp, P(PPStrategyOp), ctypes.c_int, u64p, dp]
+                               P(PPFilter), u64p, dp, dp, P(PPCounters)]
+    L.pp_truncate.argtypes = [vp, P(PPStrategyOp), ctypes.c_int, u64p, dp, dp]
+    L.pp_variance_ket.argtypes = [vp, u64p, dp]
     L.pp_evolve_single.argtypes = [vp, u64p, u64p, ctypes.c_double, ctypes.c_double]
     L.pp_lookup.argtypes = [vp, u64p, u64p, i64, dp, P(ctypes.c_int32)]
     L.pp_lookup_noalloc.argtypes = [vp, u64p, u64p, i64, dp, P(ctypes.c_int32)]

@@ -17,7 +17,8 @@ from . import _lib
 from ._lib import ArgumentError, DimensionMismatch, PPCounters, check, dp, u64p
 from .basis import Ket, PauliBasis, PauliSum, word_count
 from .truncation import (AdaptiveTruncation, CoeffTruncation, CompositeTruncation, CorrectionAccumulator,
-                         EnergyCorrection, MajoranaWeightTruncation, NoCorrection, NoTruncation,
+                         EnergyCorrection, EnergyVarianceCorrection, MajoranaWeightTruncation, NoCorrection,
+                         NoTruncation,
                          StochasticCoeffTruncation, strategy_steps,
                          StochasticSamplingTruncation, TruncationStrategy, WeightDampedTruncation,
                          WeightTruncation, XWeightDampedTruncation, XWeightTruncation, compile_filter,
@@ -63,12 +64,15 @@ def _strategy_ops(strategy):
 
 
 def _energy_args(correction, W):
+    """(ket words, energy accumulator, variance accumulator or None)."""
     if isinstance(correction, NoCorrection):
-        return None, None
-    if isinstance(correction, EnergyCorrection):
+        return None, None, None
+    if isinstance(correction, (EnergyCorrection, EnergyVarianceCorrection)):
         ket = _pack(correction.ket_int() & ((1 << (64 * W)) - 1), W)
-        return ket, ctypes.c_double(correction.accumulated_energy)
-    raise NotImplementedError(f"{type(correction).__name__} is not available on the device path in this build")
+        var = ctypes.c_double(correction.accumulated_variance) if isinstance(correction, EnergyVarianceCorrection) \
+            else None
+        return ket, ctypes.c_double(correction.accumulated_energy), var
+    raise ArgumentError(f"{type(correction).__name__} has no device implementation")
 
 
 def _pack(v: int, W: int) -> np.ndarray:
@@ -372,7 +376,7 @@ class SparsePauliVector:
         if not is_compilable(local_truncation):
             raise ArgumentError("local_truncation must be a deterministic (weight/coefficient) strategy "
                                 "— it runs per append")
-        ket, acc = _energy_args(correction, self.W)
+        ket, acc, vacc = _energy_args(correction, self.W)
         compiled = is_compilable(truncation)
         ops, nops = (None, 0) if compiled else _strategy_ops(truncation)
         if not gens:
@@ -390,9 +394,12 @@ class SparsePauliVector:
             check(_lib.lib().pp_evolve_ex(self._h, u64p(gz), u64p(gx), dp(cv), dp(sv), len(gens), int(window),
                                           ctypes.byref(fs), ops, nops, ctypes.byref(fl),
                                           u64p(ket) if ket is not None else None,
-                                          ctypes.byref(acc) if acc is not None else None, ctypes.byref(cnt)))
+                                          ctypes.byref(acc) if acc is not None else None,
+                                          ctypes.byref(vacc) if vacc is not None else None, ctypes.byref(cnt)))
             if acc is not None:
                 correction.accumulated_energy = acc.value
+            if vacc is not None:
+                correction.accumulated_variance = vacc.value
         return self
 
     def evolve(self, generators, angles, **kw) -> "SparsePauliVector":
@@ -428,14 +435,17 @@ class SparsePauliVector:
 
     def truncate_(self, strategy: TruncationStrategy, corr: CorrectionAccumulator = NoCorrection()):
         """``truncate!(v, strategy, corr)`` (truncation.jl:330-337)."""
-        ket, acc = _energy_args(corr, self.W)
+        ket, acc, vacc = _energy_args(corr, self.W)
         if is_compilable(strategy) and ket is None:
             return self._compact(compile_filter(strategy))
         ops, nops = _strategy_ops(strategy)
         check(_lib.lib().pp_truncate(self._h, ops, nops, u64p(ket) if ket is not None else None,
-                                     ctypes.byref(acc) if acc is not None else None))
+                                     ctypes.byref(acc) if acc is not None else None,
+                                     ctypes.byref(vacc) if vacc is not None else None))
         if acc is not None:
             corr.accumulated_energy = acc.value
+        if vacc is not None:
+            corr.accumulated_variance = vacc.value
         return self
 
     def stochastic_clip_(self, epsilon: float, rng=None):
@@ -510,6 +520,14 @@ class SparsePauliVector:
         check(_lib.lib().pp_expectation_ket(self._h, u64p(ket), ctypes.byref(out)))
         return out.value
 
+    def variance(self, psi: Union[Ket, int] = 0) -> float:
+        """``variance(v, ψ::Ket)`` (statistics.jl:6-22): ⟨ψ|v²|ψ⟩ - ⟨ψ|v|ψ⟩²."""
+        k = psi.v if isinstance(psi, Ket) else int(psi)
+        ket = _pack(k & ((1 << (64 * self.W)) - 1), self.W)
+        out = ctypes.c_double()
+        check(_lib.lib().pp_variance_ket(self._h, u64p(ket), ctypes.byref(out)))
+        return out.value
+
 
 # module-level spellings of the reference's free functions --------------------------------------
 
@@ -554,6 +572,10 @@ def stochastic_clip_(v: SparsePauliVector, epsilon: float, rng=None):
 
 def expectation_value(v: SparsePauliVector, psi) -> float:
     return v.expectation_value(psi)
+
+
+def variance(v: SparsePauliVector, psi) -> float:
+    return v.variance(psi)
 
 
 def depolarizing_channel_(v, p, qubits=None):

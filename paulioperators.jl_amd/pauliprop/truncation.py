@@ -200,13 +200,18 @@ class EnergyCorrection(CorrectionAccumulator):
 
 
 class EnergyVarianceCorrection(CorrectionAccumulator):
-    """truncation.jl:273-278.  Its variance measure needs the merged state and
-    O² (statistics.jl:6); not available on the device path in this build."""
+    """truncation.jl:273-278: accumulates the changes of ⟨ψ|O|ψ⟩ and of
+    variance(O, ψ) caused by every truncation, both measured on the merged
+    state (spv_evolve.jl:246-270)."""
 
-    def __init__(self, psi):
+    def __init__(self, psi, accumulated_energy: float = 0.0, accumulated_variance: float = 0.0):
         self.psi = psi
-        self.accumulated_energy = 0.0
-        self.accumulated_variance = 0.0
+        self.accumulated_energy = float(accumulated_energy)
+        self.accumulated_variance = float(accumulated_variance)
+
+    def ket_int(self) -> int:
+        v = getattr(self.psi, "v", self.psi)
+        return int(v)
 
 
 def strategy_steps(s: TruncationStrategy) -> list:
