@@ -3,7 +3,8 @@
  *
  * The reference (PauliOperators.jl v3.1.0) has no FFI: its interface is Julia
  * dispatch on SparsePauliVector{N,W,T} (src/PauliOperators.jl:42-110).  Each
- * entry point below replaces one of those methods for Float64 coefficients;
+ * entry point below replaces one of those methods for T = Float64 or
+ * T = ComplexF64 (the reference's default, type_SparsePauliVector.jl:138,167);
  * the Julia/Python wrappers validate arguments and raise the reference's
  * exception types before calling in.
  *
@@ -11,6 +12,10 @@
  *  - Keys cross the ABI as W little-endian uint64 words per term for z and
  *    for x (W = 1 for N <= 64, 2 for N <= 128), so a Julia Vector{UInt64} or
  *    Vector{UInt128} passes unchanged (type_SparsePauliVector.jl:23-34).
+ *  - Coefficients cross as `double` arrays: one value per term for a Float64
+ *    handle, two (re, im) per term for a ComplexF64 handle, so a Julia
+ *    Vector{Float64} or Vector{ComplexF64} passes unchanged.  Every `c`
+ *    argument of a single-vector call below follows the handle's type.
  *  - Host pointers are borrowed for the duration of a call; device state is
  *    owned by the handle.  One handle per host thread; calls are ordered on
  *    the handle's HIP stream.
@@ -86,6 +91,16 @@ typedef struct {
  * reference's buffer lengths, which decide when an early merge happens.  Device
  * memory is sized separately and grows on demand. */
 int pp_create(int N, int64_t live_cap, int64_t append_cap, pp_handle** out);
+/* SparsePauliVector(N, T) for T = Float64 (complex_coeffs = 0) or ComplexF64
+ * (complex_coeffs = 1; type_SparsePauliVector.jl:138-141).  ComplexF64 follows
+ * Julia's arithmetic bit for bit: the rotation child is (T(k-2) sinθ) c as a
+ * Complex * Complex product (spv_kernels.jl:163), |c| is Julia's hypot
+ * (abs(::Complex)) in every filter, sums and real scalings are componentwise.
+ * ComplexF64 vectors evolve on the per-step kernels (the resident layer
+ * launches hold Float64 coefficients). */
+int pp_create_typed(int N, int64_t live_cap, int64_t append_cap, int complex_coeffs, pp_handle** out);
+/* *complex_coeffs = 1 for a ComplexF64 handle, 0 for Float64. */
+int pp_coeff_type(pp_handle* h, int* complex_coeffs);
 int pp_destroy(pp_handle* h);
 
 /* Replace the live set by n terms.  sorted = 1: keys already strictly
@@ -174,7 +189,7 @@ int pp_lookup(pp_handle* h, const uint64_t* z, const uint64_t* x, int64_t count,
  * is for callers that cannot allocate device memory (code under hipGraph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
  * results as pp_lookup; the staged query keys and results live in a fixed
- * per-GPU scratch array of 2.75 MiB, held until the process ends, so at most
+ * per-GPU scratch array of 3.25 MiB, held until the process ends, so at most
  * 65536 queries per call (more: PP_EARG).  Calls that overlap on one GPU (from
  * several host threads or handles) race on that array. */
 int pp_lookup_noalloc(pp_handle* h, const uint64_t* z, const uint64_t* x, int64_t count, double* c, int32_t* found);
@@ -189,11 +204,17 @@ int pp_delete(pp_handle* h, const uint64_t* z, const uint64_t* x, int64_t count)
 /* sum!(a, b) (spv_ops.jl:182-193): a += b, a merge of b's sorted terms into
  * a with NOFILTER (a and b distinct handles of the same N). */
 int pp_sum(pp_handle* a, pp_handle* b);
-/* mul!(v, s) (spv_ops.jl:12-17): every coefficient times s. */
+/* mul!(v, s) (spv_ops.jl:12-17): every coefficient times the real s
+ * (componentwise for ComplexF64, as Complex * Real). */
 int pp_scale(pp_handle* h, double s);
+/* mul!(v, s) for a ComplexF64 handle and a ComplexF64 scalar (re, im):
+ * Julia's Complex * Complex.  A Float64 handle returns PP_EARG (InexactError). */
+int pp_scale_complex(pp_handle* h, double re, double im);
 /* norm(v, p) (spv_ops.jl:57-84): p = 2, 1, Inf or any p > 0. */
 int pp_norm(pp_handle* h, double p, double* out);
-/* inner_product(a, b) (spv_ops.jl:378-396): Σ over shared keys of c_a c_b. */
+/* inner_product(a, b) (spv_ops.jl:378-396): Σ over shared keys of
+ * conj(c_a) c_b; out[0] (and out[1], the imaginary part, for ComplexF64).
+ * a and b have the same coefficient type. */
 int pp_inner_product(pp_handle* a, pp_handle* b, double* out);
 /* The distance isapprox(a, b; atol, rtol) tests (spv_ops.jl:88-107):
  * sqrt(Σ over the union of keys of (c_a - c_b)^2), absent = 0. */
@@ -216,7 +237,8 @@ int pp_scale_by_popcount(pp_handle* h, int kind, const uint64_t* mask, const dou
                          const double* t2);
 
 /* ⟨ψ|v|ψ⟩ for a computational-basis ket (spv_evolve.jl:133-134 ->
- * spv_kernels.jl:441-452); ket is W words.  Synchronises. */
+ * spv_kernels.jl:441-452); ket is W words; out[0] (and out[1], the imaginary
+ * part, for ComplexF64).  Synchronises. */
 int pp_expectation_ket(pp_handle* h, const uint64_t* ket, double* out);
 
 /* check_spv key-order invariant (type_SparsePauliVector.jl:263-278): *ok = 1
@@ -230,7 +252,8 @@ int pp_check(pp_handle* h, int* ok);
  * which evolves one vector per call; each vector keeps exactly the semantics
  * of the single-vector calls above (one evolve! call per vector per
  * pp_batch_evolve, with the reference's capacity bookkeeping per vector).
- * live_cap / append_cap as in pp_create, per vector.
+ * live_cap / append_cap as in pp_create, per vector.  Batch vectors hold
+ * Float64 coefficients.
  */
 typedef struct pp_batch pp_batch;
 int pp_batch_create(int B, const int* N, const int64_t* live_cap, const int64_t* append_cap, pp_batch** out);

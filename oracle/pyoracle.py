@@ -55,6 +55,11 @@ def lib():
         dp = P(ctypes.c_double)
         L.or_create.restype = ctypes.c_void_p
         L.or_create.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int64]
+        L.or_create_typed.restype = ctypes.c_void_p
+        L.or_create_typed.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int]
+        L.or_expectation_c.argtypes = [ctypes.c_void_p, u64p, dp]
+        L.or_hypot.restype = ctypes.c_double
+        L.or_hypot.argtypes = [ctypes.c_double, ctypes.c_double]
         L.or_destroy.argtypes = [ctypes.c_void_p]
         L.or_load.argtypes = [ctypes.c_void_p, u64p, u64p, dp, ctypes.c_int64]
         L.or_size.restype = ctypes.c_int64
@@ -116,13 +121,14 @@ def pack_int(v: int, W: int) -> np.ndarray:
     return np.array([(v >> (64 * k)) & ((1 << 64) - 1) for k in range(W)], dtype=np.uint64)
 
 
-def pack_terms(N: int, terms: Sequence[Tuple[int, int, float]]):
-    """(z, x, c) triples with Python-int bitstrings -> word arrays."""
+def pack_terms(N: int, terms: Sequence[Tuple[int, int, float]], dtype=np.float64):
+    """(z, x, c) triples with Python-int bitstrings -> word arrays; c as
+    float64, or complex128 (interleaved re, im) for ComplexF64."""
     W = words(N)
     n = len(terms)
     z = np.zeros((max(n, 1), W), dtype=np.uint64)
     x = np.zeros((max(n, 1), W), dtype=np.uint64)
-    c = np.zeros(max(n, 1), dtype=np.float64)
+    c = np.zeros(max(n, 1), dtype=dtype)
     m = (1 << 64) - 1
     for i, (zi, xi, ci) in enumerate(terms):
         for k in range(W):
@@ -155,22 +161,31 @@ def julia_powi(x: float, n: int) -> float:
     return lib().or_julia_powi(float(x), int(n))
 
 
+def julia_hypot(x: float, y: float) -> float:
+    """The oracle's restatement of Julia's hypot (abs(::ComplexF64))."""
+    return lib().or_hypot(float(x), float(y))
+
+
 class OracleSPV:
-    """CPU restatement of SparsePauliVector{N,W,Float64}."""
+    """CPU restatement of SparsePauliVector{N,W,T}, T = Float64 (``T=float``)
+    or ComplexF64 (``T=complex``)."""
 
     def __init__(self, N: int, terms: Iterable[Tuple[int, int, float]] = (),
-                 capacity_factor: float = 2.0, append_factor: float = 1.0, min_capacity: int = 16):
+                 capacity_factor: float = 2.0, append_factor: float = 1.0, min_capacity: int = 16,
+                 T=float):
         terms = list(terms)
         self.N = N
         self.W = words(N)
+        self.complex = T is complex
+        self.dtype = np.complex128 if self.complex else np.float64
         # type_SparsePauliVector.jl:175-176
         live_cap = max(min_capacity, math.ceil(capacity_factor * max(len(terms), 1)))
         app_cap = max(min_capacity, math.ceil(append_factor * live_cap))
-        self.h = lib().or_create(N, live_cap, app_cap)
+        self.h = lib().or_create_typed(N, live_cap, app_cap, 1 if self.complex else 0)
         if not self.h:
             raise ValueError("oracle create failed")
         if terms:
-            z, x, c = pack_terms(N, terms)
+            z, x, c = pack_terms(N, terms, self.dtype)
             lib().or_load(self.h, _u64p(z), _u64p(x), _dp(c), len(terms))
 
     def __del__(self):
@@ -192,7 +207,7 @@ class OracleSPV:
         n = len(self)
         z = np.zeros((max(n, 1), self.W), dtype=np.uint64)
         x = np.zeros((max(n, 1), self.W), dtype=np.uint64)
-        c = np.zeros(max(n, 1), dtype=np.float64)
+        c = np.zeros(max(n, 1), dtype=self.dtype)
         lib().or_get(self.h, _u64p(z), _u64p(x), _dp(c))
         return z[:n], x[:n], c[:n]
 
@@ -277,8 +292,13 @@ class OracleSPV:
             lamZ = 1 - 2 * (pX + pY)
         lib().or_channel(self.h, self._KINDS[kind], _u64p(pack_int(M, self.W)), lam, lamX, lamY, lamZ)
 
-    def expectation(self, ket: int = 0) -> float:
-        return lib().or_expectation(self.h, _u64p(pack_int(ket, self.W)))
+    def expectation(self, ket: int = 0):
+        """⟨ket|O|ket⟩: a float for Float64, a complex for ComplexF64."""
+        if not self.complex:
+            return lib().or_expectation(self.h, _u64p(pack_int(ket, self.W)))
+        out = np.zeros(2, dtype=np.float64)
+        lib().or_expectation_c(self.h, _u64p(pack_int(ket, self.W)), _dp(out))
+        return complex(out[0], out[1])
 
     def variance(self, ket: int = 0) -> float:
         """statistics.jl:6-22 on the merged live set."""

@@ -6,8 +6,10 @@
  * checker for the HIP kernels and the CPU baseline timed beside them; the
  * product library never links or calls it.
  *
- * Instantiation: define WT (uint64_t or u128) and SFX (a name suffix), then
- * include this file.  Every function cites the reference file:line it restates.
+ * Instantiation: define WT (uint64_t or u128), the coefficient type CT with
+ * its arithmetic (C_ABS, C_ABS2, C_ROT, C_SCALE, C_ADD, C_RE, C_IM, C_ZERO,
+ * C_LD, C_ST: see spv_oracle.c) and SFX (a name suffix), then include this
+ * file.  Every function cites the reference file:line it restates.
  */
 
 #define CAT2(a, b) a##b
@@ -16,7 +18,7 @@
 
 typedef struct {
     WT z, x;
-    double c;
+    CT c;
 } F(or_trip);
 
 /* type_SparsePauliVector.jl:69-84 — live / append / scratch SoA buffers plus
@@ -24,13 +26,13 @@ typedef struct {
 typedef struct {
     int N;
     WT *z, *x;
-    double *c;
+    CT *c;
     int64_t n, live_cap;
     WT *az, *ax;
-    double *ac;
+    CT *ac;
     int64_t an, app_cap;
     WT *sz, *sx;
-    double *sc;
+    CT *sc;
     F(or_trip) *ws;
     int64_t ws_cap;
     F(or_trip) *ws2;
@@ -82,13 +84,13 @@ static F(or_spv) *F(or_alloc)(int N, int64_t live_cap, int64_t app_cap) {
     v->app_cap = app_cap;
     v->z = (WT *)calloc((size_t)live_cap, sizeof(WT));
     v->x = (WT *)calloc((size_t)live_cap, sizeof(WT));
-    v->c = (double *)calloc((size_t)live_cap, sizeof(double));
+    v->c = (CT *)calloc((size_t)live_cap, sizeof(CT));
     v->sz = (WT *)calloc((size_t)live_cap, sizeof(WT));
     v->sx = (WT *)calloc((size_t)live_cap, sizeof(WT));
-    v->sc = (double *)calloc((size_t)live_cap, sizeof(double));
+    v->sc = (CT *)calloc((size_t)live_cap, sizeof(CT));
     v->az = (WT *)calloc((size_t)app_cap, sizeof(WT));
     v->ax = (WT *)calloc((size_t)app_cap, sizeof(WT));
-    v->ac = (double *)calloc((size_t)app_cap, sizeof(double));
+    v->ac = (CT *)calloc((size_t)app_cap, sizeof(CT));
     v->ws_cap = live_cap > app_cap ? live_cap : app_cap;
     v->ws = (F(or_trip) *)malloc((size_t)v->ws_cap * sizeof(F(or_trip)));
     v->ws2 = NULL;
@@ -115,10 +117,10 @@ static void F(grow_live)(F(or_spv) *v, int64_t need) {
     int64_t newcap = 2 * v->live_cap > need ? 2 * v->live_cap : need;
     v->z = (WT *)F(xrealloc)(v->z, (size_t)newcap * sizeof(WT));
     v->x = (WT *)F(xrealloc)(v->x, (size_t)newcap * sizeof(WT));
-    v->c = (double *)F(xrealloc)(v->c, (size_t)newcap * sizeof(double));
+    v->c = (CT *)F(xrealloc)(v->c, (size_t)newcap * sizeof(CT));
     v->sz = (WT *)F(xrealloc)(v->sz, (size_t)newcap * sizeof(WT));
     v->sx = (WT *)F(xrealloc)(v->sx, (size_t)newcap * sizeof(WT));
-    v->sc = (double *)F(xrealloc)(v->sc, (size_t)newcap * sizeof(double));
+    v->sc = (CT *)F(xrealloc)(v->sc, (size_t)newcap * sizeof(CT));
     v->live_cap = newcap;
     if (v->ws_cap < newcap) {
         v->ws = (F(or_trip) *)F(xrealloc)(v->ws, (size_t)newcap * sizeof(F(or_trip)));
@@ -131,7 +133,7 @@ static void F(grow_append)(F(or_spv) *v, int64_t need) {
     int64_t newcap = 2 * v->app_cap > need ? 2 * v->app_cap : need;
     v->az = (WT *)F(xrealloc)(v->az, (size_t)newcap * sizeof(WT));
     v->ax = (WT *)F(xrealloc)(v->ax, (size_t)newcap * sizeof(WT));
-    v->ac = (double *)F(xrealloc)(v->ac, (size_t)newcap * sizeof(double));
+    v->ac = (CT *)F(xrealloc)(v->ac, (size_t)newcap * sizeof(CT));
     v->app_cap = newcap;
     if (v->ws_cap < newcap) {
         v->ws = (F(or_trip) *)F(xrealloc)(v->ws, (size_t)newcap * sizeof(F(or_trip)));
@@ -141,9 +143,9 @@ static void F(grow_append)(F(or_spv) *v, int64_t need) {
 
 /* spv_kernels.jl:146-177 — rotation sweep of terms [lo, hi) (0-based) of one
  * buffer set, appending sin-branch children at *cur. */
-static void F(rotate_range)(WT *z, WT *x, double *c, int64_t lo, int64_t hi,
+static void F(rotate_range)(WT *z, WT *x, CT *c, int64_t lo, int64_t hi,
                             WT gz, WT gx, int ng, double cosv, double sinv,
-                            WT *dz, WT *dx, double *dc, int64_t *cur, int64_t cap_end,
+                            WT *dz, WT *dx, CT *dc, int64_t *cur, int64_t cap_end,
                             const or_filter_t *f, int64_t *created, int *overflowed) {
     for (int64_t i = lo; i < hi; i++) {
         WT zi = z[i], xi = x[i];
@@ -153,9 +155,9 @@ static void F(rotate_range)(WT *z, WT *x, double *c, int64_t lo, int64_t hi,
         WT zp = gz ^ zi;
         WT xp = gx ^ xi;
         int k = (F(popc)(zp & xp) - ng - F(popc)(zi & xi) + 2 * m1) & 3;
-        double cnew = ((double)(k - 2) * sinv) * c[i];
-        c[i] = c[i] * cosv;
-        if (F(should_drop)(f, zp, xp, fabs(cnew))) continue;
+        CT cnew = C_ROT((double)(k - 2) * sinv, sinv, c[i]);
+        c[i] = C_SCALE(c[i], cosv);
+        if (F(should_drop)(f, zp, xp, C_ABS(cnew))) continue;
         if (*cur >= cap_end) {
             *overflowed = 1;
             continue;
@@ -312,11 +314,11 @@ static void F(unshuffle_ws)(F(or_spv) *v, int64_t m, WT gz, WT gx) {
 static int F(try_merge)(F(or_spv) *v, int64_t m, const or_filter_t *f, int64_t *out_n) {
     int64_t n = v->n, cap = v->live_cap, out = 0, i = 0, j = 0;
     WT *z = v->z, *x = v->x, *sz = v->sz, *sx = v->sx;
-    double *c = v->c, *sc = v->sc;
+    CT *c = v->c, *sc = v->sc;
     F(or_trip) *ws = v->ws;
     while (i < n || j < m) {
         WT kz, kx;
-        double acc;
+        CT acc;
         if (j >= m || (i < n && !F(key_lt)(ws[j].z, ws[j].x, z[i], x[i]))) {
             kz = z[i]; kx = x[i]; acc = c[i];
             i++;
@@ -325,10 +327,10 @@ static int F(try_merge)(F(or_spv) *v, int64_t m, const or_filter_t *f, int64_t *
             j++;
         }
         while (j < m && ws[j].z == kz && ws[j].x == kx) {
-            acc = acc + ws[j].c;
+            acc = C_ADD(acc, ws[j].c);
             j++;
         }
-        if (F(should_drop)(f, kz, kx, fabs(acc))) continue;
+        if (F(should_drop)(f, kz, kx, C_ABS(acc))) continue;
         out++;
         if (out > cap) { *out_n = out; return 1; }
         sz[out - 1] = kz;
@@ -356,7 +358,7 @@ static void F(merge_spv)(F(or_spv) *v, int64_t m, const or_filter_t *f, int64_t 
 static void F(compact)(F(or_spv) *v, const or_filter_t *f) {
     int64_t out = 0;
     for (int64_t i = 0; i < v->n; i++) {
-        if (F(should_drop)(f, v->z[i], v->x[i], fabs(v->c[i]))) continue;
+        if (F(should_drop)(f, v->z[i], v->x[i], C_ABS(v->c[i]))) continue;
         if (out != i) {
             v->z[out] = v->z[i];
             v->x[out] = v->x[i];
@@ -367,16 +369,17 @@ static void F(compact)(F(or_spv) *v, const or_filter_t *f) {
     v->n = out;
 }
 
-/* spv_kernels.jl:441-452 — sequential ⟨k|O|k⟩ over live then appends. */
-static double F(expectation)(const F(or_spv) *v, WT kv) {
-    double acc = 0.0;
+/* spv_kernels.jl:441-452 — sequential ⟨k|O|k⟩ over live then appends;
+ * (1 - 2 parity) * c is Int * T (componentwise for ComplexF64). */
+static CT F(expectation)(const F(or_spv) *v, WT kv) {
+    CT acc = C_ZERO;
     for (int64_t i = 0; i < v->n; i++) {
         if (v->x[i] != (WT)0) continue;
-        acc = acc + (double)(1 - 2 * (F(popc)(v->z[i] & kv) & 1)) * v->c[i];
+        acc = C_ADD(acc, C_SCALE(v->c[i], (double)(1 - 2 * (F(popc)(v->z[i] & kv) & 1))));
     }
     for (int64_t i = 0; i < v->an; i++) {
         if (v->ax[i] != (WT)0) continue;
-        acc = acc + (double)(1 - 2 * (F(popc)(v->az[i] & kv) & 1)) * v->ac[i];
+        acc = C_ADD(acc, C_SCALE(v->ac[i], (double)(1 - 2 * (F(popc)(v->az[i] & kv) & 1))));
     }
     return acc;
 }
@@ -427,7 +430,7 @@ static void F(apply_adaptive)(F(or_spv) *v, int64_t max_terms, double min_thresh
     or_filter_init_nofilter(&f);
     if (v->n > max_terms) {
         double *coeffs = (double *)malloc((size_t)v->n * sizeof(double));
-        for (int64_t i = 0; i < v->n; i++) coeffs[i] = fabs(v->c[i]);
+        for (int64_t i = 0; i < v->n; i++) coeffs[i] = C_ABS(v->c[i]);
         qsort(coeffs, (size_t)v->n, sizeof(double), F(cmp_dbl));
         /* Julia coeffs[end - max_terms] (1-based) = index n - max_terms - 1 (0-based) */
         f.thresh = coeffs[v->n - max_terms - 1];
@@ -463,7 +466,7 @@ static double F(variance)(const F(or_spv) *v, WT kv) {
         ix[i].i = i;
     }
     qsort(ix, (size_t)n, sizeof(F(or_xi)), F(cmp_xi));  /* by x, then key order */
-    double e2 = 0.0, e1 = 0.0;
+    double e2 = 0.0, e1r = 0.0, e1i = 0.0;
     for (int64_t g = 0; g < n;) {
         const WT x = ix[g].x;
         double re = 0.0, im = 0.0;
@@ -474,18 +477,20 @@ static double F(variance)(const F(or_spv) *v, WT kv) {
             const int sign = F(popc)(z & (x ^ kv)) & 1;
             const int sp = (4 - F(popc)(z & x) % 4) % 4;
             const int ph = (sp + 2 * sign) % 4;  /* PHASE_TBL = [1, i, -1, -i] */
-            const double c = v->c[q];
-            if (ph == 0) re = re + c;
-            else if (ph == 1) im = im + c;
-            else if (ph == 2) re = re + (-c);
-            else im = im + (-c);
+            /* phase * c: (1) c, (i) (-im, re), (-1) (-re, -im), (-i) (im, -re) */
+            const double cr = C_RE(v->c[q]), ci = C_IM(v->c[q]);
+            if (ph == 0) { re = re + cr; im = im + ci; }
+            else if (ph == 1) { re = re + (-ci); im = im + cr; }
+            else if (ph == 2) { re = re + (-cr); im = im + (-ci); }
+            else { re = re + ci; im = im + (-cr); }
         }
         e2 = e2 + (re * re + im * im);
-        if (x == (WT)0) e1 = re;
+        if (x == (WT)0) { e1r = re; e1i = im; }
         g = j;
     }
     free(ix);
-    return e2 - e1 * e1;
+    /* real(e2 - e1 * e1), statistics.jl:19-21 */
+    return e2 - (e1r * e1r - e1i * e1i);
 }
 
 /* spv_evolve.jl:246-284 — merge boundary with the correction accumulators.
@@ -502,21 +507,22 @@ static void F(boundary)(F(or_spv) *v, const or_filter_t *f, or_strategy_t *s,
         if (have_mask) F(unshuffle_ws)(v, m, lz, lx);
         else F(sort_pending)(v->ws, m);
         F(merge_spv)(v, m, &nof, n_in, n_out);
-        const double e0 = F(expectation)(v, kv), v0 = F(variance)(v, kv);
+        const double e0 = C_RE(F(expectation)(v, kv)), v0 = F(variance)(v, kv);
         if (s->kind == OR_STRAT_ADAPTIVE) F(apply_adaptive)(v, s->max_terms, s->min_thresh);
         else F(compact)(v, f);
-        s->energy_acc += F(expectation)(v, kv) - e0;  /* truncation.jl:308-310 */
+        s->energy_acc += C_RE(F(expectation)(v, kv)) - e0;  /* truncation.jl:308-310 */
         s->variance_acc += F(variance)(v, kv) - v0;
         return;
     }
-    const double before = s->energy ? F(expectation)(v, kv) : 0.0;
+    /* real(expectation_value(O, ψ)), truncation.jl:288 */
+    const double before = s->energy ? C_RE(F(expectation)(v, kv)) : 0.0;
     int64_t m = F(gather_append)(v);
     if (have_mask) F(unshuffle_ws)(v, m, lz, lx);
     else F(sort_pending)(v->ws, m);
     F(merge_spv)(v, m, f, n_in, n_out);
     if (s->kind == OR_STRAT_ADAPTIVE) F(apply_adaptive)(v, s->max_terms, s->min_thresh);
     if (s->energy) {
-        const double after = F(expectation)(v, kv);
+        const double after = C_RE(F(expectation)(v, kv));
         s->energy_acc += after - before;  /* truncation.jl:304-306 */
     }
 }
@@ -566,4 +572,133 @@ static int F(evolve_seq)(F(or_spv) *v, const WT *gz, const WT *gx, const double 
         }
     }
     return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Per-instance bodies of the exported API (spv_oracle.c dispatches on the
+ * handle's word count and coefficient type).  Keys cross as W little-endian
+ * uint64 words per term; coefficients as one double per term (Float64) or
+ * two, (re, im) (ComplexF64). */
+
+static inline WT F(ldw)(const uint64_t *p, int64_t i) {
+    if (sizeof(WT) == 8) return (WT)p[i];
+    return (WT)((((u128)p[2 * i + 1]) << 64) | (u128)p[2 * i]);
+}
+static inline void F(stw)(uint64_t *p, int64_t i, WT v) {
+    if (sizeof(WT) == 8) {
+        p[i] = (uint64_t)v;
+    } else {
+        p[2 * i] = (uint64_t)v;
+        p[2 * i + 1] = (uint64_t)((u128)v >> 64);
+    }
+}
+
+static void *F(api_create)(int N, int64_t live_cap, int64_t app_cap) { return F(or_alloc)(N, live_cap, app_cap); }
+static void F(api_destroy)(void *vv) { F(or_free)((F(or_spv) *)vv); }
+
+/* type_SparsePauliVector.jl:167-188: fill, then the construction sort. */
+static int F(api_load)(void *vv, const uint64_t *z, const uint64_t *x, const double *c, int64_t n) {
+    F(or_spv) *v = (F(or_spv) *)vv;
+    if (n > v->live_cap) F(grow_live)(v, n);
+    for (int64_t i = 0; i < n; i++) {
+        v->z[i] = F(ldw)(z, i);
+        v->x[i] = F(ldw)(x, i);
+        v->c[i] = C_LD(c, i);
+    }
+    v->n = n;
+    v->an = 0;
+    F(sort_live)(v);
+    return 0;
+}
+
+static int64_t F(api_size)(void *vv) { return ((F(or_spv) *)vv)->n; }
+static int64_t F(api_pending)(void *vv) { return ((F(or_spv) *)vv)->an; }
+static void F(api_capacities)(void *vv, int64_t *live_cap, int64_t *app_cap) {
+    *live_cap = ((F(or_spv) *)vv)->live_cap;
+    *app_cap = ((F(or_spv) *)vv)->app_cap;
+}
+
+static void F(api_get)(void *vv, uint64_t *z, uint64_t *x, double *c) {
+    F(or_spv) *v = (F(or_spv) *)vv;
+    for (int64_t i = 0; i < v->n; i++) {
+        F(stw)(z, i, v->z[i]);
+        F(stw)(x, i, v->x[i]);
+        C_ST(c, i, v->c[i]);
+    }
+}
+
+/* spv_evolve.jl:228-236 — single rotation, NOFILTER, unshuffle merge. */
+static int F(api_evolve_single)(void *vv, const uint64_t *gz, const uint64_t *gx, double cosv, double sinv) {
+    F(or_spv) *v = (F(or_spv) *)vv;
+    or_filter_t nof;
+    or_filter_init_nofilter(&nof);
+    int64_t n_in, n_out, created;
+    const WT z = F(ldw)(gz, 0), x = F(ldw)(gx, 0);
+    const int ng = F(popc)(z & x);
+    if (v->n + 2 * v->an > v->app_cap) F(grow_append)(v, v->n + 2 * v->an);
+    if (F(rotate_spv)(v, z, x, ng, cosv, sinv, &nof, &created)) return -2;
+    int64_t m = F(gather_append)(v);
+    F(unshuffle_ws)(v, m, z, x);
+    F(merge_spv)(v, m, &nof, &n_in, &n_out);
+    return 0;
+}
+
+static int F(api_evolve_seq)(void *vv, const uint64_t *gz, const uint64_t *gx, const double *cosv,
+                             const double *sinv, int64_t L, int64_t window, const or_filter_t *f,
+                             or_strategy_t *s, const or_filter_t *local, or_counters_t *cnt) {
+    WT *z = (WT *)malloc((size_t)(L > 0 ? L : 1) * sizeof(WT));
+    WT *x = (WT *)malloc((size_t)(L > 0 ? L : 1) * sizeof(WT));
+    for (int64_t i = 0; i < L; i++) {
+        z[i] = F(ldw)(gz, i);
+        x[i] = F(ldw)(gx, i);
+    }
+    int rc = F(evolve_seq)((F(or_spv) *)vv, z, x, cosv, sinv, L, window, f, s, local, cnt);
+    free(z);
+    free(x);
+    return rc;
+}
+
+static double F(api_variance)(void *vv, const uint64_t *ket) { return F(variance)((F(or_spv) *)vv, F(ldw)(ket, 0)); }
+static void F(api_compact)(void *vv, const or_filter_t *f) { F(compact)((F(or_spv) *)vv, f); }
+static void F(api_adaptive)(void *vv, int64_t max_terms, double min_thresh) {
+    F(apply_adaptive)((F(or_spv) *)vv, max_terms, min_thresh);
+}
+
+/* channels.jl:203-265 — c *= factor (Complex * Real componentwise).
+ * kind: 0 depolarizing (z|x), 1 dephasing (x), 2 bit flip (z), 3 bit-phase
+ * flip (x^z), 4 general Pauli channel (lamX^nX * lamY^nY) * lamZ^nZ. */
+static void F(api_channel)(void *vv, int kind, const uint64_t *mask, double lam, double lamX, double lamY,
+                           double lamZ) {
+    F(or_spv) *v = (F(or_spv) *)vv;
+    const WT M = F(ldw)(mask, 0);
+    for (int64_t i = 0; i < v->n; i++) {
+        const WT z = v->z[i], x = v->x[i];
+        double fac;
+        if (kind == 4) {
+            const WT xM = x & M, zM = z & M;
+            const int nX = F(popc)(xM & ~z), nY = F(popc)(xM & z), nZ = F(popc)(zM & ~x);
+            fac = or_julia_powi(lamX, nX) * or_julia_powi(lamY, nY);
+            fac = fac * or_julia_powi(lamZ, nZ);
+        } else {
+            const WT sel = kind == 0 ? (z | x) : kind == 1 ? x : kind == 2 ? z : (x ^ z);
+            fac = or_julia_powi(lam, F(popc)(sel & M));
+        }
+        v->c[i] = C_SCALE(v->c[i], fac);
+    }
+}
+
+/* spv_evolve.jl:133-134 -> spv_kernels.jl:441-452: out[0] real part, out[1]
+ * imaginary part (0 for Float64). */
+static void F(api_expectation)(void *vv, const uint64_t *ket, double *out) {
+    const CT e = F(expectation)((F(or_spv) *)vv, F(ldw)(ket, 0));
+    out[0] = C_RE(e);
+    out[1] = C_IM(e);
+}
+
+/* type_SparsePauliVector.jl:263-278 (key-order part of check_spv). */
+static int F(api_check)(void *vv) {
+    F(or_spv) *v = (F(or_spv) *)vv;
+    for (int64_t i = 1; i < v->n; i++)
+        if (!F(key_lt)(v->z[i - 1], v->x[i - 1], v->z[i], v->x[i])) return 0;
+    return 1;
 }

@@ -7,9 +7,10 @@
  * src/truncation.jl).  Only the tests, the smoke check and the bench's
  * cpu_baseline leg load it; the product library never does.
  *
- * The generic body lives in spv_body.h and is instantiated twice:
- *   _w1 : UInt64 words  (N <= 64),  type_SparsePauliVector.jl:23-26
- *   _w2 : UInt128 words (N <= 128)
+ * The generic body lives in spv_body.h and is instantiated four times:
+ *   _w1, _w1c : UInt64 words  (N <= 64),  type_SparsePauliVector.jl:23-26
+ *   _w2, _w2c : UInt128 words (N <= 128)
+ * with Float64 coefficients (_w1, _w2) or ComplexF64 (_w1c, _w2c).
  * Keys cross this ABI as little-endian uint64 words per term (W words for z,
  * W words for x), i.e. a Julia Vector{UInt128} passes unchanged.
  */
@@ -85,6 +86,83 @@ static inline int popc_w1(uint64_t v) { return __builtin_popcountll(v); }
 static inline int popc_w2(u128 v) {
     return __builtin_popcountll((uint64_t)v) + __builtin_popcountll((uint64_t)(v >> 64));
 }
+static inline int popc_w1c(uint64_t v) { return popc_w1(v); }
+static inline int popc_w2c(u128 v) { return popc_w2(v); }
+
+double or_julia_powi(double x, int64_t n);
+
+/* ------------------------------------------------------------------------ */
+/* ComplexF64 arithmetic as Julia performs it (base/complex.jl, base/math.jl). */
+typedef struct {
+    double re, im;
+} or_cplx;
+
+/* hypot(x, y) = abs(::ComplexF64) (base/math.jl, the branch taken on CPUs
+ * with a hardware fma: Borges' corrected hypot; muladd taken as fused). */
+double or_hypot(double x, double y) {
+    double ax = fabs(x), ay = fabs(y);
+    if (isinf(ax) || isinf(ay)) return INFINITY;
+    if (ay > ax) {
+        const double t = ax;
+        ax = ay;
+        ay = t;
+    }
+    if (ay <= ax * sqrt(0x1p-53)) return ax;            /* sqrt(eps(Float64) / 2) */
+    double scale = 0x1p-52 * sqrt(0x1p-1022);            /* eps * sqrt(floatmin) */
+    if (ax > sqrt(0x1.fffffffffffffp+1023 / 2)) {        /* sqrt(floatmax / 2) */
+        ax = ax * scale;
+        ay = ay * scale;
+        scale = 1.0 / scale;
+    } else if (ay < sqrt(0x1p-1022)) {                   /* sqrt(floatmin) */
+        ax = ax / scale;
+        ay = ay / scale;
+    } else {
+        scale = 1.0;
+    }
+    double h = sqrt(fma(ax, ax, ay * ay));
+    const double hsq = h * h;
+    const double axsq = ax * ax;
+    h -= (fma(-ay, ay, hsq - axsq) + fma(h, h, -hsq) - fma(ax, ax, -axsq)) / (2 * h);
+    return h * scale;
+}
+
+/* T(k - 2) * sinθ = Complex(a, 0.0 * sinθ) times c, Complex * Complex
+ * (spv_kernels.jl:163): (a re - b im, a im + b re). */
+static inline or_cplx cx_rot(double a, double s, or_cplx c) {
+    const double b = 0.0 * s;
+    or_cplx r = {a * c.re - b * c.im, a * c.im + b * c.re};
+    return r;
+}
+static inline or_cplx cx_scale(or_cplx c, double s) {  /* Complex * Real */
+    or_cplx r = {c.re * s, c.im * s};
+    return r;
+}
+static inline or_cplx cx_add(or_cplx a, or_cplx b) {
+    or_cplx r = {a.re + b.re, a.im + b.im};
+    return r;
+}
+static inline double cx_abs2(or_cplx c) { return c.re * c.re + c.im * c.im; }
+static inline or_cplx cx_ld(const double *p, int64_t i) {
+    or_cplx r = {p[2 * i], p[2 * i + 1]};
+    return r;
+}
+static inline void cx_st(double *p, int64_t i, or_cplx c) {
+    p[2 * i] = c.re;
+    p[2 * i + 1] = c.im;
+}
+
+/* Float64 */
+#define CT double
+#define C_ABS(c) fabs(c)
+#define C_ABS2(c) ((c) * (c))
+#define C_ROT(a, s, c) ((a) * (c))
+#define C_SCALE(c, s) ((c) * (s))
+#define C_ADD(a, b) ((a) + (b))
+#define C_RE(c) (c)
+#define C_IM(c) 0.0
+#define C_ZERO 0.0
+#define C_LD(p, i) ((p)[i])
+#define C_ST(p, i, v) ((p)[i] = (v))
 
 #define WT uint64_t
 #define SFX _w1
@@ -97,6 +175,50 @@ static inline int popc_w2(u128 v) {
 
 #define WT u128
 #define SFX _w2
+#include "spv_body.h"
+#undef WT
+#undef SFX
+#undef F
+#undef CAT
+#undef CAT2
+
+#undef CT
+#undef C_ABS
+#undef C_ABS2
+#undef C_ROT
+#undef C_SCALE
+#undef C_ADD
+#undef C_RE
+#undef C_IM
+#undef C_ZERO
+#undef C_LD
+#undef C_ST
+
+/* ComplexF64 */
+static const or_cplx or_czero = {0.0, 0.0};
+#define CT or_cplx
+#define C_ABS(c) or_hypot((c).re, (c).im)
+#define C_ABS2(c) cx_abs2(c)
+#define C_ROT(a, s, c) cx_rot((a), (s), (c))
+#define C_SCALE(c, s) cx_scale((c), (s))
+#define C_ADD(a, b) cx_add((a), (b))
+#define C_RE(c) ((c).re)
+#define C_IM(c) ((c).im)
+#define C_ZERO or_czero
+#define C_LD(p, i) cx_ld((p), (i))
+#define C_ST(p, i, v) cx_st((p), (i), (v))
+
+#define WT uint64_t
+#define SFX _w1c
+#include "spv_body.h"
+#undef WT
+#undef SFX
+#undef F
+#undef CAT
+#undef CAT2
+
+#define WT u128
+#define SFX _w2c
 #include "spv_body.h"
 #undef WT
 #undef SFX
@@ -139,20 +261,22 @@ double or_julia_powi(double x, int64_t n) {
 }
 
 /* ------------------------------------------------------------------------ */
-/* Exported oracle API (ctypes).  Handles carry the word count. */
+/* Exported oracle API (ctypes).  Handles carry the word count and the
+ * coefficient type; OR_CALL picks the instance. */
 
 typedef struct {
     int W;
+    int cplx;
     void *v;
 } or_handle;
+
+#define OR_CALL(h, fn, ...)                                                              \
+    ((h)->W == 1 ? ((h)->cplx ? fn##_w1c((h)->v, ##__VA_ARGS__) : fn##_w1((h)->v, ##__VA_ARGS__)) \
+                 : ((h)->cplx ? fn##_w2c((h)->v, ##__VA_ARGS__) : fn##_w2((h)->v, ##__VA_ARGS__)))
 
 static int words_for(int N) { return N <= 64 ? 1 : 2; }
 
 static inline u128 ld2(const uint64_t *p) { return ((u128)p[1] << 64) | (u128)p[0]; }
-static inline void st2(uint64_t *p, u128 v) {
-    p[0] = (uint64_t)v;
-    p[1] = (uint64_t)(v >> 64);
-}
 
 void or_filter_make(or_filter_t *f, int64_t wmax, int64_t xwmax, int64_t mwmax, double thresh,
                     double alpha_w, double dthresh_w, double alpha_xw, double dthresh_xw) {
@@ -169,98 +293,45 @@ void or_filter_make(or_filter_t *f, int64_t wmax, int64_t xwmax, int64_t mwmax, 
 
 size_t or_filter_size(void) { return sizeof(or_filter_t); }
 
-/* type_SparsePauliVector.jl:167-188: capacities, fill, construction sort. */
-or_handle *or_create(int N, int64_t live_cap, int64_t app_cap) {
+/* SparsePauliVector(N, T) with T = Float64 (cplx = 0) or ComplexF64 (cplx = 1):
+ * type_SparsePauliVector.jl:138-141,167-188. */
+or_handle *or_create_typed(int N, int64_t live_cap, int64_t app_cap, int cplx) {
     if (N < 1 || N > 128 || live_cap < 1 || app_cap < 1) return NULL;
     or_handle *h = (or_handle *)malloc(sizeof(or_handle));
     h->W = words_for(N);
-    h->v = h->W == 1 ? (void *)or_alloc_w1(N, live_cap, app_cap) : (void *)or_alloc_w2(N, live_cap, app_cap);
+    h->cplx = cplx ? 1 : 0;
+    if (h->W == 1) h->v = h->cplx ? api_create_w1c(N, live_cap, app_cap) : api_create_w1(N, live_cap, app_cap);
+    else h->v = h->cplx ? api_create_w2c(N, live_cap, app_cap) : api_create_w2(N, live_cap, app_cap);
     return h;
 }
 
+or_handle *or_create(int N, int64_t live_cap, int64_t app_cap) { return or_create_typed(N, live_cap, app_cap, 0); }
+
+int or_is_complex(or_handle *h) { return h->cplx; }
+
 void or_destroy(or_handle *h) {
     if (!h) return;
-    if (h->W == 1) or_free_w1((or_spv_w1 *)h->v);
-    else or_free_w2((or_spv_w2 *)h->v);
+    OR_CALL(h, api_destroy);
     free(h);
 }
 
 /* Load n terms (any order, no duplicates) and sort them as the constructor does. */
 int or_load(or_handle *h, const uint64_t *z, const uint64_t *x, const double *c, int64_t n) {
-    if (h->W == 1) {
-        or_spv_w1 *v = (or_spv_w1 *)h->v;
-        if (n > v->live_cap) grow_live_w1(v, n);
-        for (int64_t i = 0; i < n; i++) { v->z[i] = z[i]; v->x[i] = x[i]; v->c[i] = c[i]; }
-        v->n = n;
-        v->an = 0;
-        sort_live_w1(v);
-    } else {
-        or_spv_w2 *v = (or_spv_w2 *)h->v;
-        if (n > v->live_cap) grow_live_w2(v, n);
-        for (int64_t i = 0; i < n; i++) { v->z[i] = ld2(z + 2 * i); v->x[i] = ld2(x + 2 * i); v->c[i] = c[i]; }
-        v->n = n;
-        v->an = 0;
-        sort_live_w2(v);
-    }
-    return 0;
+    return OR_CALL(h, api_load, z, x, c, n);
 }
 
-int64_t or_size(or_handle *h) {
-    return h->W == 1 ? ((or_spv_w1 *)h->v)->n : ((or_spv_w2 *)h->v)->n;
-}
+int64_t or_size(or_handle *h) { return OR_CALL(h, api_size); }
+int64_t or_pending(or_handle *h) { return OR_CALL(h, api_pending); }
+void or_capacities(or_handle *h, int64_t *live_cap, int64_t *app_cap) { OR_CALL(h, api_capacities, live_cap, app_cap); }
+void or_get(or_handle *h, uint64_t *z, uint64_t *x, double *c) { OR_CALL(h, api_get, z, x, c); }
 
-int64_t or_pending(or_handle *h) {
-    return h->W == 1 ? ((or_spv_w1 *)h->v)->an : ((or_spv_w2 *)h->v)->an;
-}
-
-void or_capacities(or_handle *h, int64_t *live_cap, int64_t *app_cap) {
-    if (h->W == 1) { *live_cap = ((or_spv_w1 *)h->v)->live_cap; *app_cap = ((or_spv_w1 *)h->v)->app_cap; }
-    else { *live_cap = ((or_spv_w2 *)h->v)->live_cap; *app_cap = ((or_spv_w2 *)h->v)->app_cap; }
-}
-
-void or_get(or_handle *h, uint64_t *z, uint64_t *x, double *c) {
-    if (h->W == 1) {
-        or_spv_w1 *v = (or_spv_w1 *)h->v;
-        for (int64_t i = 0; i < v->n; i++) { z[i] = v->z[i]; x[i] = v->x[i]; c[i] = v->c[i]; }
-    } else {
-        or_spv_w2 *v = (or_spv_w2 *)h->v;
-        for (int64_t i = 0; i < v->n; i++) { st2(z + 2 * i, v->z[i]); st2(x + 2 * i, v->x[i]); c[i] = v->c[i]; }
-    }
-}
-
-/* spv_evolve.jl:228-236 — single rotation, NOFILTER, unshuffle merge. */
 int or_evolve_single(or_handle *h, const uint64_t *gz, const uint64_t *gx, double cosv, double sinv) {
-    or_filter_t nof;
-    or_filter_init_nofilter(&nof);
-    int64_t n_in, n_out, created;
-    if (h->W == 1) {
-        or_spv_w1 *v = (or_spv_w1 *)h->v;
-        int ng = popc_w1(gz[0] & gx[0]);
-        if (v->n + 2 * v->an > v->app_cap) grow_append_w1(v, v->n + 2 * v->an);
-        if (rotate_spv_w1(v, gz[0], gx[0], ng, cosv, sinv, &nof, &created)) return -2;
-        int64_t m = gather_append_w1(v);
-        unshuffle_ws_w1(v, m, gz[0], gx[0]);
-        merge_spv_w1(v, m, &nof, &n_in, &n_out);
-    } else {
-        or_spv_w2 *v = (or_spv_w2 *)h->v;
-        u128 z = ld2(gz), x = ld2(gx);
-        int ng = popc_w2(z & x);
-        if (v->n + 2 * v->an > v->app_cap) grow_append_w2(v, v->n + 2 * v->an);
-        if (rotate_spv_w2(v, z, x, ng, cosv, sinv, &nof, &created)) return -2;
-        int64_t m = gather_append_w2(v);
-        unshuffle_ws_w2(v, m, z, x);
-        merge_spv_w2(v, m, &nof, &n_in, &n_out);
-    }
-    return 0;
+    return OR_CALL(h, api_evolve_single, gz, gx, cosv, sinv);
 }
 
 /* spv_evolve.jl:312-387 — windowed sequence driver.  gz/gx hold L generators
- * of W words each; cosv/sinv are the caller's cos/sin of each angle. */
-static int or_evolve_strat(or_handle *h, const uint64_t *gz, const uint64_t *gx, const double *cosv,
-                           const double *sinv, int64_t L, int64_t window, const or_filter_t *strict,
-                           or_strategy_t *sp, const or_filter_t *local, or_counters_t *cnt);
-
-/* variance_acc != NULL (with energy_ket) is EnergyVarianceCorrection. */
+ * of W words each; cosv/sinv are the caller's cos/sin of each angle.
+ * variance_acc != NULL (with energy_ket) is EnergyVarianceCorrection. */
 int or_evolve_ex(or_handle *h, const uint64_t *gz, const uint64_t *gx, const double *cosv,
                  const double *sinv, int64_t L, int64_t window, const or_filter_t *strict,
                  int strat_kind, int64_t max_terms, double min_thresh, const or_filter_t *local,
@@ -280,17 +351,17 @@ int or_evolve_ex(or_handle *h, const uint64_t *gz, const uint64_t *gx, const dou
         s.energy_acc = energy_acc ? *energy_acc : 0.0;
         s.variance_acc = variance_acc ? *variance_acc : 0.0;
     }
-    int rc = or_evolve_strat(h, gz, gx, cosv, sinv, L, window, strict, &s, local, cnt);
+    or_filter_t nof;
+    or_filter_init_nofilter(&nof);
+    const or_filter_t *f = strat_kind == OR_STRAT_COMPILED ? strict : &nof;
+    int rc = OR_CALL(h, api_evolve_seq, gz, gx, cosv, sinv, L, window, f, &s, local, cnt);
     if (energy_acc) *energy_acc = s.energy_acc;
     if (variance_acc) *variance_acc = s.variance_acc;
     return rc;
 }
 
 /* variance(v, ψ) (statistics.jl:6-22) for a computational-basis ket. */
-double or_variance(or_handle *h, const uint64_t *ket) {
-    if (h->W == 1) return variance_w1((or_spv_w1 *)h->v, (uint64_t)ket[0]);
-    return variance_w2((or_spv_w2 *)h->v, ld2(ket));
-}
+double or_variance(or_handle *h, const uint64_t *ket) { return OR_CALL(h, api_variance, ket); }
 
 void or_set_sort_mode(int mode) { or_sort_mode = mode; }
 
@@ -302,86 +373,27 @@ int or_evolve(or_handle *h, const uint64_t *gz, const uint64_t *gx, const double
                         local, NULL, NULL, NULL, cnt);
 }
 
-static int or_evolve_strat(or_handle *h, const uint64_t *gz, const uint64_t *gx, const double *cosv,
-                           const double *sinv, int64_t L, int64_t window, const or_filter_t *strict,
-                           or_strategy_t *sp, const or_filter_t *local, or_counters_t *cnt) {
-    const int strat_kind = sp->kind;
-    or_strategy_t s = *sp;
-    or_filter_t nof;
-    or_filter_init_nofilter(&nof);
-    const or_filter_t *f = strat_kind == OR_STRAT_COMPILED ? strict : &nof;
-    if (h->W == 1) {
-        int rc1 = evolve_seq_w1((or_spv_w1 *)h->v, gz, gx, cosv, sinv, L, window, f, &s, local, cnt);
-        *sp = s;
-        return rc1;
-    }
-    u128 *z = (u128 *)malloc((size_t)L * sizeof(u128));
-    u128 *x = (u128 *)malloc((size_t)L * sizeof(u128));
-    for (int64_t i = 0; i < L; i++) { z[i] = ld2(gz + 2 * i); x[i] = ld2(gx + 2 * i); }
-    int rc = evolve_seq_w2((or_spv_w2 *)h->v, z, x, cosv, sinv, L, window, f, &s, local, cnt);
-    free(z);
-    free(x);
-    *sp = s;
-    return rc;
-}
-
 /* spv_kernels.jl:395-408 via the clip family (spv_evolve.jl:13-26). */
-void or_compact(or_handle *h, const or_filter_t *f) {
-    if (h->W == 1) compact_w1((or_spv_w1 *)h->v, f);
-    else compact_w2((or_spv_w2 *)h->v, f);
-}
+void or_compact(or_handle *h, const or_filter_t *f) { OR_CALL(h, api_compact, f); }
 
-void or_adaptive(or_handle *h, int64_t max_terms, double min_thresh) {
-    if (h->W == 1) apply_adaptive_w1((or_spv_w1 *)h->v, max_terms, min_thresh);
-    else apply_adaptive_w2((or_spv_w2 *)h->v, max_terms, min_thresh);
-}
+void or_adaptive(or_handle *h, int64_t max_terms, double min_thresh) { OR_CALL(h, api_adaptive, max_terms, min_thresh); }
 
-/* channels.jl:203-265 — per-term popcount damping.
- * kind: 0 depolarizing (z|x), 1 dephasing (x), 2 bit flip (z), 3 bit-phase
- * flip (x^z), 4 general Pauli channel (lamX^nX * lamY^nY * lamZ^nZ).
- * The λ values are the caller's; powers use Julia's Float64^Int. */
+/* channels.jl:203-265 — per-term popcount damping; the λ values are the
+ * caller's, powers use Julia's Float64^Int. */
 void or_channel(or_handle *h, int kind, const uint64_t *mask, double lam, double lamX,
                 double lamY, double lamZ) {
-    if (h->W == 1) {
-        or_spv_w1 *v = (or_spv_w1 *)h->v;
-        uint64_t M = mask[0];
-        for (int64_t i = 0; i < v->n; i++) {
-            uint64_t z = v->z[i], x = v->x[i];
-            if (kind == 4) {
-                uint64_t xM = x & M, zM = z & M;
-                int nX = popc_w1(xM & ~z), nY = popc_w1(xM & z), nZ = popc_w1(zM & ~x);
-                double fac = or_julia_powi(lamX, nX) * or_julia_powi(lamY, nY);
-                fac = fac * or_julia_powi(lamZ, nZ);
-                v->c[i] = v->c[i] * fac;
-            } else {
-                uint64_t sel = kind == 0 ? (z | x) : kind == 1 ? x : kind == 2 ? z : (x ^ z);
-                v->c[i] = v->c[i] * or_julia_powi(lam, popc_w1(sel & M));
-            }
-        }
-    } else {
-        or_spv_w2 *v = (or_spv_w2 *)h->v;
-        u128 M = ld2(mask);
-        for (int64_t i = 0; i < v->n; i++) {
-            u128 z = v->z[i], x = v->x[i];
-            if (kind == 4) {
-                u128 xM = x & M, zM = z & M;
-                int nX = popc_w2(xM & ~z), nY = popc_w2(xM & z), nZ = popc_w2(zM & ~x);
-                double fac = or_julia_powi(lamX, nX) * or_julia_powi(lamY, nY);
-                fac = fac * or_julia_powi(lamZ, nZ);
-                v->c[i] = v->c[i] * fac;
-            } else {
-                u128 sel = kind == 0 ? (z | x) : kind == 1 ? x : kind == 2 ? z : (x ^ z);
-                v->c[i] = v->c[i] * or_julia_powi(lam, popc_w2(sel & M));
-            }
-        }
-    }
+    OR_CALL(h, api_channel, kind, mask, lam, lamX, lamY, lamZ);
 }
 
-/* spv_evolve.jl:133-134 -> spv_kernels.jl:441-452 */
+/* spv_evolve.jl:133-134 -> spv_kernels.jl:441-452 (the real part). */
 double or_expectation(or_handle *h, const uint64_t *ket) {
-    if (h->W == 1) return expectation_w1((or_spv_w1 *)h->v, ket[0]);
-    return expectation_w2((or_spv_w2 *)h->v, ld2(ket));
+    double out[2];
+    OR_CALL(h, api_expectation, ket, out);
+    return out[0];
 }
+
+/* The same, out[0] real and out[1] imaginary part. */
+void or_expectation_c(or_handle *h, const uint64_t *ket, double *out) { OR_CALL(h, api_expectation, ket, out); }
 
 /* spv_kernels.jl:51-62, exported for the weight-measure known-answer tests. */
 int or_majorana_weight(int W, const uint64_t *z, const uint64_t *x) {
@@ -396,15 +408,4 @@ int or_should_drop(int W, const or_filter_t *f, const uint64_t *z, const uint64_
 }
 
 /* type_SparsePauliVector.jl:263-278 (key-order part of check_spv). */
-int or_check(or_handle *h) {
-    if (h->W == 1) {
-        or_spv_w1 *v = (or_spv_w1 *)h->v;
-        for (int64_t i = 1; i < v->n; i++)
-            if (!key_lt_w1(v->z[i - 1], v->x[i - 1], v->z[i], v->x[i])) return 0;
-    } else {
-        or_spv_w2 *v = (or_spv_w2 *)h->v;
-        for (int64_t i = 1; i < v->n; i++)
-            if (!key_lt_w2(v->z[i - 1], v->x[i - 1], v->z[i], v->x[i])) return 0;
-    }
-    return 1;
-}
+int or_check(or_handle *h) { return OR_CALL(h, api_check); }

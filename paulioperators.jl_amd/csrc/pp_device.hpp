@@ -258,6 +258,88 @@ __device__ __forceinline__ bool should_drop(const FilterReg& f, const Key<W>& k,
     return should_drop_f<W>(f, k, absc);
 }
 
+// ---------------------------------------------------------------------------
+// Coefficient types.  A vector's coefficients are Float64 (`double`) or
+// ComplexF64 (`pp_cplx`: (re, im), Julia's interleaved layout, 16 bytes).
+// Kernels are templated on the type; the vector operators of the clang
+// vector extension make `c * s` (s real) and `a + b` componentwise, which is
+// Julia's Complex * Real and Complex + Complex bit for bit.
+typedef double pp_cplx __attribute__((ext_vector_type(2)));
+
+template <typename CT>
+struct CoefTraits;
+template <>
+struct CoefTraits<double> {
+    static constexpr bool kComplex = false;
+};
+template <>
+struct CoefTraits<pp_cplx> {
+    static constexpr bool kComplex = true;
+};
+
+__device__ __forceinline__ double coef_re(double c) { return c; }
+__device__ __forceinline__ double coef_re(pp_cplx c) { return c.x; }
+__device__ __forceinline__ double coef_im(double) { return 0.0; }
+__device__ __forceinline__ double coef_im(pp_cplx c) { return c.y; }
+template <typename CT>
+__device__ __forceinline__ CT coef_zero() {
+    return CT(0.0);
+}
+
+// Julia's hypot(x, y) (base/math.jl, the branch taken where the CPU has a
+// hardware fma: Borges' corrected hypot, correctly rounded), which is
+// abs(::ComplexF64).  muladd is taken as fused, as Julia lowers it there.
+__host__ __device__ __forceinline__ double julia_hypot(double x, double y) {
+    double ax = fabs(x), ay = fabs(y);
+    if (isinf(ax) || isinf(ay)) return INFINITY;
+    if (ay > ax) {
+        const double t = ax;
+        ax = ay;
+        ay = t;
+    }
+    // sqrt(eps(Float64) / 2) = sqrt(2^-53), correctly rounded
+    if (ay <= ax * 0x1.6a09e667f3bcdp-27) return ax;
+    double scale = 0x1p-563;  // eps(Float64) * sqrt(floatmin(Float64)) = 2^-52 * 2^-511
+    if (ax > 0x1.6a09e667f3bccp+511) {  // sqrt(floatmax(Float64) / 2)
+        ax = ax * scale;
+        ay = ay * scale;
+        scale = 0x1p+563;  // inv(scale)
+    } else if (ay < 0x1p-511) {  // sqrt(floatmin(Float64))
+        ax = ax / scale;
+        ay = ay / scale;
+    } else {
+        scale = 1.0;
+    }
+    double h = sqrt(fma(ax, ax, ay * ay));
+    const double hsq = h * h;
+    const double axsq = ax * ax;
+    h -= (fma(-ay, ay, hsq - axsq) + fma(h, h, -hsq) - fma(ax, ax, -axsq)) / (2 * h);
+    return h * scale;
+}
+
+// abs(c) and abs2(c) (Julia's abs2(z) = real(z)*real(z) + imag(z)*imag(z)).
+__device__ __forceinline__ double coef_abs(double c) { return fabs(c); }
+__device__ __forceinline__ double coef_abs(pp_cplx c) { return julia_hypot(c.x, c.y); }
+__device__ __forceinline__ double coef_abs2(double c) { return c * c; }
+__device__ __forceinline__ double coef_abs2(pp_cplx c) { return c.x * c.x + c.y * c.y; }
+
+// The rotation child's coefficient (T(k - 2) * sinθ) * c (spv_kernels.jl:163).
+// For ComplexF64, T(k - 2) * sinθ is Complex((k - 2) sinθ, 0.0 sinθ) and the
+// product with c is Julia's Complex * Complex,
+// (a re - b im, a im + b re): the b terms are signed zeros, kept so that
+// zero components carry Julia's signs.
+__device__ __forceinline__ double coef_rot(double a, double s, double c) {
+    (void)s;
+    return a * c;
+}
+__device__ __forceinline__ pp_cplx coef_rot(double a, double s, pp_cplx c) {
+    const double b = 0.0 * s;
+    pp_cplx r;
+    r.x = a * c.x - b * c.y;
+    r.y = a * c.y + b * c.x;
+    return r;
+}
+
 template <int W>
 struct GenReg {
     uint64_t gz[W], gx[W];
@@ -281,7 +363,8 @@ struct GenReg {
         return (m & 1) != 0;
     }
     // spv_kernels.jl:160-163: child key and coefficient from the OLD c.
-    __device__ __forceinline__ double child(const Key<W>& k, double c, Key<W>& out) const {
+    template <typename CT>
+    __device__ __forceinline__ CT child(const Key<W>& k, CT c, Key<W>& out) const {
         int m1 = 0, nyp = 0, ny = 0;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
@@ -292,7 +375,7 @@ struct GenReg {
             ny += popc64(k.z[w] & k.x[w]);
         }
         const int kk = (nyp - ng - ny + 2 * m1) & 3;
-        return ((double)(kk - 2) * sinv) * c;
+        return coef_rot((double)(kk - 2) * sinv, sinv, c);
     }
 };
 

@@ -223,7 +223,7 @@ void set_last_error(const std::string& msg) { g_last_error = msg; }
 // so the call allocates nothing.  Host code uses the address kept per device
 // by scratch_setup(), never the array's name (a host-side stand-in).
 constexpr int64_t kLookupNoAllocMax = int64_t(1) << 16;  // queries per call
-constexpr size_t kScratchBytes = (size_t)kLookupNoAllocMax * (2 * 16 + 8 + 4);  // z, x (2 words each), c, found
+constexpr size_t kScratchBytes = (size_t)kLookupNoAllocMax * (2 * 16 + 16 + 4);  // z, x (2 words each), c (re, im), found
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 namespace {
@@ -258,8 +258,20 @@ struct StepSpan {
     StepDev* end() const { return p + n; }
 };
 
+// Coefficient arrays are `double*` in the descriptors and buffers; the
+// kernels see them as CT (double for Float64, pp_cplx for ComplexF64).
+template <typename CT>
+inline CT* cptr(double* p) {
+    return reinterpret_cast<CT*>(p);
+}
+template <typename CT>
+inline const CT* cptr(const double* p) {
+    return reinterpret_cast<const CT*>(p);
+}
+
 struct Batch {
     int W = 1, B = 0, ncu = 256, dev = 0;
+    int cw = 1;  // doubles per coefficient: 1 Float64, 2 ComplexF64 (re, im interleaved)
     int sweep_fold_above = 32;  // sweep workgroups take several tiles past this many tiles per CU
     hipStream_t stream = nullptr;
     std::vector<int> N;
@@ -381,13 +393,14 @@ struct Batch {
     }
 
     size_t kb() const { return sizeof(uint64_t) * (size_t)W; }
+    size_t cb() const { return sizeof(double) * (size_t)cw; }
 
     // Whether `terms` more terms of this word size fit in half of the free
     // device memory (growth beyond what is needed is only taken when cheap).
     bool fits_device(int64_t terms) const {
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) != hipSuccess) return false;
-        return (double)terms * (double)(2 * kb() + 8) < 0.5 * (double)fr;
+        return (double)terms * (double)(2 * kb() + cb()) < 0.5 * (double)fr;
     }
     // Tile shapes of the next launches (0 = choose from the live counts).
     int tile_req = 0, mtile_req = 0;
@@ -404,7 +417,7 @@ struct Batch {
     void alloc_set(DevSet& s, int64_t cap) {
         s.z = dalloc<uint64_t>((size_t)cap * W);
         s.x = dalloc<uint64_t>((size_t)cap * W);
-        s.c = dalloc<double>((size_t)cap);
+        s.c = dalloc<double>((size_t)cap * cw);
         s.cap = cap;
     }
     static void free_set(DevSet& s) {
@@ -418,8 +431,9 @@ struct Batch {
     }
 
     void init(int W_, const std::vector<int>& Ns, const std::vector<int64_t>& live_cap_v,
-              const std::vector<int64_t>& app_cap_v) {
+              const std::vector<int64_t>& app_cap_v, int cw_ = 1) {
         W = W_;
+        cw = cw_;
         B = (int)Ns.size();
         N = Ns;
         HIPCHK(hipGetDevice(&dev));
@@ -459,7 +473,7 @@ struct Batch {
         d_chan = dalloc<ChanDev>(B);
         d_tabs = dalloc<double>((size_t)B * 3 * kExpTab);
         d_en = dalloc<int32_t>(B);
-        d_out = dalloc<double>(B);
+        d_out = dalloc<double>(2 * (size_t)B);
         d_kets = dalloc<uint64_t>((size_t)B * W);
         d_bad = dalloc<int32_t>(B);
         d_epoch = dalloc<uint32_t>(1);
@@ -505,7 +519,7 @@ struct Batch {
                 alloc_set(keepset, nc);
                 HIPCHK(hipMemcpyAsync(keepset.z, I.L[cur].z, (size_t)keep * kb(), hipMemcpyDeviceToDevice, stream));
                 HIPCHK(hipMemcpyAsync(keepset.x, I.L[cur].x, (size_t)keep * kb(), hipMemcpyDeviceToDevice, stream));
-                HIPCHK(hipMemcpyAsync(keepset.c, I.L[cur].c, (size_t)keep * sizeof(double), hipMemcpyDeviceToDevice, stream));
+                HIPCHK(hipMemcpyAsync(keepset.c, I.L[cur].c, (size_t)keep * cb(), hipMemcpyDeviceToDevice, stream));
                 HIPCHK(hipStreamSynchronize(stream));
             }
             free_set(I.L[0]);
@@ -732,45 +746,45 @@ struct Batch {
         return false;
     }
 
-    template <int Wt, int T, int TM>
+    template <int Wt, typename CT, int T, int TM>
     void launch_step(const StepDev* dsteps, int64_t step_index, uint32_t ep_off, bool bounds, bool scan) {
-        hipLaunchKernelGGL((kb_flags<Wt, T>), dim3(blocks(0)), dim3(kBlock), 0, stream, map(0), d_desc, d_state, dsteps,
+        hipLaunchKernelGGL((kb_flags<Wt, T, CT>), dim3(blocks(0)), dim3(kBlock), 0, stream, map(0), d_desc, d_state, dsteps,
                            bounds ? 1 : 0);
         if (scan) hipLaunchKernelGGL(kb_scan<T>, dim3(B), dim3(1024), 0, stream, d_desc, d_state, dsteps, step_index);
         if (bounds)
             hipLaunchKernelGGL((kb_bounds<Wt, T>), dim3(blocks(1)), dim3(kBlock), 0, stream, map(1), d_desc, d_state,
                                dsteps);
         if (bounds)
-            hipLaunchKernelGGL((kb_emit<Wt, T, true>), dim3(blocks(0)), dim3(kBlock), 0, stream, map(0), d_desc, d_state,
+            hipLaunchKernelGGL((kb_emit<Wt, T, true, CT>), dim3(blocks(0)), dim3(kBlock), 0, stream, map(0), d_desc, d_state,
                                dsteps, 0, step_index);
         else
-            hipLaunchKernelGGL((kb_emit<Wt, T, false>), dim3(blocks(0)), dim3(kBlock), 0, stream, map(0), d_desc,
+            hipLaunchKernelGGL((kb_emit<Wt, T, false, CT>), dim3(blocks(0)), dim3(kBlock), 0, stream, map(0), d_desc,
                                d_state, dsteps, scan ? 0 : 1, step_index);
         if (timing) {
             // events recorded by the launch itself at the kernel's start and
             // end (not around it), so the interval is the kernel's duration
             hipEvent_t e0 = next_event();
             hipEvent_t e1 = next_event();
-            hipExtLaunchKernelGGL((kb_merge<Wt, TM>), dim3(blocks(3)), dim3(kBlock), 0, stream, e0, e1, 0, map(3),
+            hipExtLaunchKernelGGL((kb_merge<Wt, TM, CT>), dim3(blocks(3)), dim3(kBlock), 0, stream, e0, e1, 0, map(3),
                                   d_desc, d_state, dsteps, 0, step_index, (const uint32_t*)d_epoch, ep_off);
         } else {
-            hipLaunchKernelGGL((kb_merge<Wt, TM>), dim3(blocks(3)), dim3(kBlock), 0, stream, map(3), d_desc, d_state,
+            hipLaunchKernelGGL((kb_merge<Wt, TM, CT>), dim3(blocks(3)), dim3(kBlock), 0, stream, map(3), d_desc, d_state,
                                dsteps, 0, step_index, (const uint32_t*)d_epoch, ep_off);
         }
     }
 
     // Compaction launch (the step table is not read in this mode but must be
     // a valid pointer: the caller has sized it to at least B entries).
-    template <int Wt, int TM>
+    template <int Wt, typename CT, int TM>
     void launch_compact(uint32_t ep_off) {
-        hipLaunchKernelGGL((kb_merge<Wt, TM>), dim3(blocks(3)), dim3(kBlock), 0, stream, map(3), d_desc, d_state, d_steps,
+        hipLaunchKernelGGL((kb_merge<Wt, TM, CT>), dim3(blocks(3)), dim3(kBlock), 0, stream, map(3), d_desc, d_state, d_steps,
                            1, (int64_t)-1, (const uint32_t*)d_epoch, ep_off);
     }
 
     // Enqueue a sequence: optional precheck + compaction, steps [first,
     // nsteps) from the table at dsteps, then the tag advance.  In graph mode
     // the launches are captured once per key and replayed.
-    template <int Wt>
+    template <int Wt, typename CT>
     void enqueue_sequence(bool with_precheck, const StepDev* dsteps, int64_t first, int64_t nsteps, bool use_graph) {
         const uint32_t count = (uint32_t)(nsteps - first + (with_precheck ? 1 : 0));
         reserve_epochs(count);
@@ -780,11 +794,11 @@ struct Batch {
                                    (const int32_t*)d_en, B);
             with_tiles<Wt>([&](auto T, auto TM) {
                 uint32_t off = 0;
-                if (with_precheck) launch_compact<Wt, decltype(TM)::value>(off++);
+                if (with_precheck) launch_compact<Wt, CT, decltype(TM)::value>(off++);
                 const bool big = big_call();
                 for (int64_t s = first; s < nsteps; ++s) {
                     const bool bnd = step_bounds[(size_t)s] != 0 || big;
-                    launch_step<Wt, decltype(T)::value, decltype(TM)::value>(dsteps + (s - first) * B, s, off++, bnd,
+                    launch_step<Wt, CT, decltype(T)::value, decltype(TM)::value>(dsteps + (s - first) * B, s, off++, bnd,
                                                                              scan_needed(bnd));
                 }
             });
@@ -872,7 +886,9 @@ struct Batch {
     // false: the call runs on the per-step kernels (auto mode also declines
     // calls of few steps and vectors that do not fit one XCD at 90% fill).
     bool res_plan(const StepSpan& steps, int64_t nsteps) {
-        if (res_mode == 0 || res_broken || res_kx == 0) return false;
+        // the resident kernel holds Float64 coefficients; ComplexF64 vectors
+        // run on the per-step kernels
+        if (res_mode == 0 || res_broken || res_kx == 0 || cw != 1) return false;
         if (res_mode == 1 && nsteps < 8) return false;
         std::vector<char> part((size_t)B, 0);
         for (int64_t s = 0; s < nsteps; ++s)
@@ -945,7 +961,7 @@ struct Batch {
     // (Sequence start: the precheck and compaction, then) one resident launch
     // for steps [first, nsteps) of the planned vectors and the commit of its
     // results.
-    template <int Wt>
+    template <int Wt, typename CT>
     void enqueue_resident(const StepDev* dsteps, int64_t first, int64_t nsteps, bool with_precheck) {
         if (with_precheck) reserve_epochs(1);
         ResArgs a;
@@ -965,7 +981,7 @@ struct Batch {
             if (with_precheck) {
                 hipLaunchKernelGGL(kb_precheck, dim3((unsigned)cdiv(B, 256)), dim3(256), 0, stream, d_state,
                                    (const int32_t*)d_en, B);
-                with_tiles<Wt>([&](auto, auto TM) { launch_compact<Wt, decltype(TM)::value>(0); });
+                with_tiles<Wt>([&](auto, auto TM) { launch_compact<Wt, CT, decltype(TM)::value>(0); });
                 hipLaunchKernelGGL(kb_epoch_advance, dim3(1), dim3(1), 0, stream, d_epoch, 1u);
             }
             hipLaunchKernelGGL(kr_reset, dim3((unsigned)cdiv(std::max(B, kXcdCount), 256)), dim3(256), 0, stream,
@@ -1029,12 +1045,12 @@ struct Batch {
     // active != 0, in step order (one evolve! call per vector), preceded by
     // the reference's capacity precheck.  Stalled vectors are regrown and
     // replayed until every vector has finished the sequence.
-    template <int Wt>
+    template <int Wt, typename CT>
     void run_sequence(const StepSpan& steps, int64_t nsteps, const std::vector<int32_t>& en) {
         std::fill(res_off.begin(), res_off.end(), 0);
         std::fill(res_boost.begin(), res_boost.end(), 1.0);  // boosts last one call
         res_ovf.assign((size_t)B, 0);
-        run_sequence_body<Wt>(steps, nsteps, en);
+        run_sequence_body<Wt, CT>(steps, nsteps, en);
     }
     std::vector<char> res_ovf;  // per vector: a chunk overflowed in this call
 
@@ -1055,7 +1071,7 @@ struct Batch {
         return StepSpan{h_steps, n};
     }
 
-    template <int Wt>
+    template <int Wt, typename CT>
     void run_sequence_body(const StepSpan& steps, int64_t nsteps, const std::vector<int32_t>& en) {
         int maxlev = 1;
         for (const StepDev& s : steps) maxlev = std::max(maxlev, (int)s.g.nlev);
@@ -1083,8 +1099,8 @@ struct Batch {
         HIPCHK(hipMemcpyAsync(d_steps, steps.data(), (size_t)(nsteps * B) * sizeof(StepDev), hipMemcpyHostToDevice,
                               stream));
         HIPCHK(hipMemcpyAsync(d_en, en.data(), (size_t)B * sizeof(int32_t), hipMemcpyHostToDevice, stream));
-        if (res_plan(steps, nsteps)) enqueue_resident<Wt>(d_steps, 0, nsteps, true);
-        else enqueue_sequence<Wt>(true, d_steps, 0, nsteps, graphs);
+        if (res_plan(steps, nsteps)) enqueue_resident<Wt, CT>(d_steps, 0, nsteps, true);
+        else enqueue_sequence<Wt, CT>(true, d_steps, 0, nsteps, graphs);
         sync_state();
         std::vector<int64_t> last_stall((size_t)B, -1);
         for (int round = 0;; ++round) {
@@ -1147,24 +1163,24 @@ struct Batch {
             upload_desc();
             plan();
             HIPCHK(hipMemcpyAsync(d_steps, rs.data(), rs.size() * sizeof(StepDev), hipMemcpyHostToDevice, stream));
-            if (res_plan(StepSpan{rs.data(), rs.size()}, nsteps - first)) enqueue_resident<Wt>(d_steps, first, nsteps, false);
-            else enqueue_sequence<Wt>(false, d_steps, first, nsteps, false);
+            if (res_plan(StepSpan{rs.data(), rs.size()}, nsteps - first)) enqueue_resident<Wt, CT>(d_steps, first, nsteps, false);
+            else enqueue_sequence<Wt, CT>(false, d_steps, first, nsteps, false);
             sync_state();
         }
     }
 
-    template <int Wt>
+    template <int Wt, typename CT>
     void channel(const std::vector<ChanDev>& ch, const std::vector<double>& tabs, const std::vector<int32_t>& en) {
         HIPCHK(hipMemcpyAsync(d_tabs, tabs.data(), tabs.size() * sizeof(double), hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemcpyAsync(d_chan, ch.data(), ch.size() * sizeof(ChanDev), hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemcpyAsync(d_en, en.data(), en.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream));
         plan();
-        hipLaunchKernelGGL(kb_channel<Wt>, dim3(blocks(0)), dim3(kBlock), 0, stream, d_desc, d_state, d_chan, d_en, map(0));
+        hipLaunchKernelGGL((kb_channel<Wt, CT>), dim3(blocks(0)), dim3(kBlock), 0, stream, d_desc, d_state, d_chan, d_en, map(0));
         HIPCHK(hipGetLastError());
     }
 
     // Compaction of the selected vectors under their strict filter (clips).
-    template <int Wt>
+    template <int Wt, typename CT>
     void compact(const std::vector<int32_t>& sel) {
         sync_state();
         for (int b = 0; b < B; ++b) {
@@ -1177,25 +1193,28 @@ struct Batch {
         plan();
         ensure_steps(B);
         reserve_epochs(1);
-        with_tiles<Wt>([&](auto, auto TM) { launch_compact<Wt, decltype(TM)::value>(0); });
+        with_tiles<Wt>([&](auto, auto TM) { launch_compact<Wt, CT, decltype(TM)::value>(0); });
         hipLaunchKernelGGL(kb_epoch_advance, dim3(1), dim3(1), 0, stream, d_epoch, 1u);
         HIPCHK(hipGetLastError());
         sync_state();
     }
 
-    template <int Wt>
+    // out[b * cw + comp]: the real part (and for ComplexF64 the imaginary part)
+    template <int Wt, typename CT>
     void expectation(const uint64_t* kets, double* out) {
         HIPCHK(hipMemcpyAsync(d_kets, kets, (size_t)B * W * sizeof(uint64_t), hipMemcpyHostToDevice, stream));
         plan();
-        hipLaunchKernelGGL(kb_expect<Wt>, dim3(blocks(0)), dim3(kBlock), 0, stream, d_desc, d_state, d_kets, map(0),
-                           d_partial);
-        hipLaunchKernelGGL(kb_expect_final, dim3(B), dim3(kBlock), 0, stream, d_partial, map(0), d_out);
+        for (int comp = 0; comp < cw; ++comp) {
+            hipLaunchKernelGGL((kb_expect<Wt, CT>), dim3(blocks(0)), dim3(kBlock), 0, stream, d_desc, d_state, d_kets,
+                               map(0), d_partial, comp);
+            hipLaunchKernelGGL(kb_expect_final, dim3(B), dim3(kBlock), 0, stream, d_partial, map(0), d_out, cw, comp);
+        }
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(out, d_out, (size_t)B * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(out, d_out, (size_t)B * cb(), hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
     }
 
-    template <int Wt>
+    template <int Wt, typename CT>
     void check(int32_t* ok) {
         HIPCHK(hipMemsetAsync(d_bad, 0, (size_t)B * sizeof(int32_t), stream));
         plan();
@@ -1232,7 +1251,7 @@ struct Batch {
         w.orand = dalloc<unsigned long long>(4 * (size_t)W);
         w.orpart = dalloc<unsigned long long>((size_t)kOrandBlocks * 4 * W);
         w.qhist = dalloc<unsigned long long>(kRadix);
-        w.out = dalloc<double>(2);
+        w.out = dalloc<double>(3);
         w.partial = dalloc<double2>(1024);
         HIPCHK(hipMemsetAsync(w.err, 0, sizeof(int32_t), stream));
     }
@@ -1245,7 +1264,7 @@ struct Batch {
         if (keep > 0) {
             HIPCHK(hipMemcpyAsync(n2.z, s.z, (size_t)keep * kb(), hipMemcpyDeviceToDevice, stream));
             HIPCHK(hipMemcpyAsync(n2.x, s.x, (size_t)keep * kb(), hipMemcpyDeviceToDevice, stream));
-            HIPCHK(hipMemcpyAsync(n2.c, s.c, (size_t)keep * sizeof(double), hipMemcpyDeviceToDevice, stream));
+            HIPCHK(hipMemcpyAsync(n2.c, s.c, (size_t)keep * cb(), hipMemcpyDeviceToDevice, stream));
             HIPCHK(hipStreamSynchronize(stream));
         }
         free_set(s);
@@ -1295,7 +1314,7 @@ struct Batch {
     }
 
     // One rotation over [live (n) ; appends (an)]; returns the children appended.
-    template <int Wt>
+    template <int Wt, typename CT>
     int64_t win_rotate(const GenDev* g, int64_t n, int64_t an, int cur) {
         WinBuf& w = win;
         const int64_t tot = n + an;
@@ -1304,17 +1323,17 @@ struct Batch {
         const int64_t nb = cdiv(tot, kRB);
         win_blocks(nb);
         const DevSet& Ls = ib[0].L[cur];
-        hipLaunchKernelGGL(kw_rotate<Wt>, dim3((unsigned)nb), dim3(kBlock), 0, stream, Ls.z, Ls.x, Ls.c, n, w.A.z,
-                           w.A.x, w.A.c, an, g, w.filt + 1, 1, w.bcount, (const int64_t*)nullptr, w.A.cap);
+        hipLaunchKernelGGL((kw_rotate<Wt, CT>), dim3((unsigned)nb), dim3(kBlock), 0, stream, Ls.z, Ls.x, cptr<CT>(Ls.c), n, w.A.z,
+                           w.A.x, cptr<CT>(w.A.c), an, g, w.filt + 1, 1, w.bcount, (const int64_t*)nullptr, w.A.cap);
         scan_i64(w.bcount, nb, w.boff, w.small);
-        hipLaunchKernelGGL(kw_rotate<Wt>, dim3((unsigned)nb), dim3(kBlock), 0, stream, Ls.z, Ls.x, Ls.c, n, w.A.z,
-                           w.A.x, w.A.c, an, g, w.filt + 1, 0, w.bcount, (const int64_t*)w.boff, w.A.cap);
+        hipLaunchKernelGGL((kw_rotate<Wt, CT>), dim3((unsigned)nb), dim3(kBlock), 0, stream, Ls.z, Ls.x, cptr<CT>(Ls.c), n, w.A.z,
+                           w.A.x, cptr<CT>(w.A.c), an, g, w.filt + 1, 0, w.bcount, (const int64_t*)w.boff, w.A.cap);
         HIPCHK(hipGetLastError());
         return read_small(0);
     }
 
     // Stable LSD radix sort of the m appends; returns the buffer holding them.
-    template <int Wt>
+    template <int Wt, typename CT>
     DevSet* win_sort(int64_t m) {
         WinBuf& w = win;
         if (m <= 1) return &w.A;
@@ -1343,8 +1362,8 @@ struct Batch {
             hipLaunchKernelGGL(ks_hist<Wt>, dim3((unsigned)nb), dim3(kBlock), 0, stream, src->z, src->x, m, dd, w.hist,
                                nb);
             scan_i64(w.hist, nh, w.hoff, w.small);
-            hipLaunchKernelGGL(ks_scatter<Wt>, dim3((unsigned)nb), dim3(kBlock), 0, stream, src->z, src->x, src->c, m,
-                               dd, w.hoff, nb, dst->z, dst->x, dst->c);
+            hipLaunchKernelGGL((ks_scatter<Wt, CT>), dim3((unsigned)nb), dim3(kBlock), 0, stream, src->z, src->x, cptr<CT>(src->c), m,
+                               dd, w.hoff, nb, dst->z, dst->x, cptr<CT>(dst->c));
             std::swap(src, dst);
         }
         HIPCHK(hipGetLastError());
@@ -1353,7 +1372,7 @@ struct Batch {
 
     // Merge live (n, buffer cur) with the sorted appends under filter slot
     // fslot; the result lands in buffer cur ^ 1.  Returns the output count.
-    template <int Wt>
+    template <int Wt, typename CT>
     int64_t win_merge(int64_t n, int cur, const DevSet* S, int64_t m, int fslot) {
         constexpr int TM = 1024;
         WinBuf& w = win;
@@ -1374,8 +1393,8 @@ struct Batch {
         const DevSet& Li = ib[0].L[cur];
         const DevSet& Lo = ib[0].L[cur ^ 1];
         const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)ncu * 4));
-        hipLaunchKernelGGL((kw_merge<Wt, TM>), dim3(grid), dim3(kBlock), 0, stream, Li.z, Li.x, Li.c, n, S->z, S->x,
-                           S->c, m, w.filt + fslot, Lo.z, Lo.x, Lo.c, Lo.cap, w.status, w.ticket, w.epoch,
+        hipLaunchKernelGGL((kw_merge<Wt, TM, CT>), dim3(grid), dim3(kBlock), 0, stream, Li.z, Li.x, cptr<CT>(Li.c), n, S->z, S->x,
+                           cptr<CT>(S->c), m, w.filt + fslot, Lo.z, Lo.x, cptr<CT>(Lo.c), Lo.cap, w.status, w.ticket, w.epoch,
                            w.small + 1, w.err);
         HIPCHK(hipGetLastError());
         const int64_t out = read_small(1);
@@ -1385,7 +1404,7 @@ struct Batch {
 
     // Order-preserving compaction of the live set (mode 0: filter slot 2;
     // mode 1: stochastic clip).  Returns the new count; flips cur.
-    template <int Wt>
+    template <int Wt, typename CT>
     int64_t win_compact(int64_t n, int& cur, int mode, double eps, uint64_t seed, SampleSel ss = SampleSel{}) {
         WinBuf& w = win;
         if (n == 0) return 0;
@@ -1394,11 +1413,11 @@ struct Batch {
         win_blocks(nb);
         const DevSet& Li = ib[0].L[cur];
         const DevSet& Lo = ib[0].L[cur ^ 1];
-        hipLaunchKernelGGL(kw_compact<Wt>, dim3((unsigned)nb), dim3(kBlock), 0, stream, Li.z, Li.x, Li.c, n,
-                           w.filt + 2, mode, eps, seed, 1, w.bcount, (const int64_t*)nullptr, Lo.z, Lo.x, Lo.c, ss);
+        hipLaunchKernelGGL((kw_compact<Wt, CT>), dim3((unsigned)nb), dim3(kBlock), 0, stream, Li.z, Li.x, cptr<CT>(Li.c), n,
+                           w.filt + 2, mode, eps, seed, 1, w.bcount, (const int64_t*)nullptr, Lo.z, Lo.x, cptr<CT>(Lo.c), ss);
         scan_i64(w.bcount, nb, w.boff, w.small);
-        hipLaunchKernelGGL(kw_compact<Wt>, dim3((unsigned)nb), dim3(kBlock), 0, stream, Li.z, Li.x, Li.c, n,
-                           w.filt + 2, mode, eps, seed, 0, w.bcount, (const int64_t*)w.boff, Lo.z, Lo.x, Lo.c, ss);
+        hipLaunchKernelGGL((kw_compact<Wt, CT>), dim3((unsigned)nb), dim3(kBlock), 0, stream, Li.z, Li.x, cptr<CT>(Li.c), n,
+                           w.filt + 2, mode, eps, seed, 0, w.bcount, (const int64_t*)w.boff, Lo.z, Lo.x, cptr<CT>(Lo.c), ss);
         HIPCHK(hipGetLastError());
         cur ^= 1;
         return read_small(0);
@@ -1406,6 +1425,7 @@ struct Batch {
 
     // k-th smallest |c| (0-based) of the live set by an 8-pass radix select
     // over the IEEE bit patterns (non-negative doubles order as integers).
+    template <typename CT>
     double win_select_abs(int64_t n, int cur, int64_t k) {
         WinBuf& w = win;
         const DevSet& Li = ib[0].L[cur];
@@ -1414,7 +1434,7 @@ struct Batch {
         unsigned long long h[kRadix];
         for (int shift = 56; shift >= 0; shift -= 8) {
             HIPCHK(hipMemsetAsync(w.qhist, 0, kRadix * sizeof(unsigned long long), stream));
-            hipLaunchKernelGGL(kq_hist, dim3(grid), dim3(kBlock), 0, stream, Li.c, n, shift, prefix, pmask, w.qhist);
+            hipLaunchKernelGGL(kq_hist<CT>, dim3(grid), dim3(kBlock), 0, stream, cptr<CT>(Li.c), n, shift, prefix, pmask, w.qhist);
             HIPCHK(hipGetLastError());
             HIPCHK(hipMemcpyAsync(h, w.qhist, sizeof(h), hipMemcpyDeviceToHost, stream));
             HIPCHK(hipStreamSynchronize(stream));
@@ -1465,22 +1485,23 @@ struct Batch {
 
     // StochasticSamplingTruncation (spv_evolve.jl:69-99): keep n_keep terms
     // drawn by the exponential-key method, rescaled to the original Σ|c|^2.
-    template <int Wt>
+    template <int Wt, typename CT>
     int64_t win_sample(int64_t n, int& cur, int64_t n_keep, uint64_t seed) {
         if (n <= n_keep) return n;
-        const double norm_sq = norm_sum_of(ib[0].L[cur].c, n, 0, 2.0);
+        const double norm_sq = norm_sum_of<CT>(ib[0].L[cur].c, n, 0, 2.0);
         uint64_t* key = dalloc<uint64_t>((size_t)n);
         const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(n, kBlock), 2048));
-        hipLaunchKernelGGL(kw_sample_keys, dim3(grid), dim3(kBlock), 0, stream, ib[0].L[cur].c, n, seed, key);
+        hipLaunchKernelGGL(kw_sample_keys<CT>, dim3(grid), dim3(kBlock), 0, stream, cptr<CT>(ib[0].L[cur].c), n, seed, key);
         HIPCHK(hipGetLastError());
         const auto sel = win_select_sample(key, n, n_keep - 1);
         SampleSel ss{key, sel.first, sel.second, n};
-        n = win_compact<Wt>(n, cur, 2, 0.0, 0, ss);
+        n = win_compact<Wt, CT>(n, cur, 2, 0.0, 0, ss);
         HIPCHK(hipStreamSynchronize(stream));
         dfree(key);
-        const double kept = norm_sum_of(ib[0].L[cur].c, n, 0, 2.0);
+        const double kept = norm_sum_of<CT>(ib[0].L[cur].c, n, 0, 2.0);
         if (kept > 0.0) {
-            hipLaunchKernelGGL(kw_scale, dim3(grid), dim3(kBlock), 0, stream, ib[0].L[cur].c, n,
+            // c *= scale, componentwise for ComplexF64 (Complex * Real)
+            hipLaunchKernelGGL(kw_scale, dim3(grid), dim3(kBlock), 0, stream, ib[0].L[cur].c, n * cw,
                                std::sqrt(norm_sq / kept));
             HIPCHK(hipGetLastError());
         }
@@ -1492,7 +1513,9 @@ struct Batch {
         HIPCHK(hipMemcpyAsync(win.filt + slot, &d, sizeof(d), hipMemcpyHostToDevice, stream));
     }
 
-    template <int Wt>
+    // real(⟨ψ|v|ψ⟩) over live + pending appends: what EnergyCorrection measures
+    // (truncation.jl:288)
+    template <int Wt, typename CT>
     double win_expect(int64_t n, int cur, int64_t an, const uint64_t* ket) {
         WinBuf& w = win;
         Key<Wt> kk;
@@ -1502,8 +1525,8 @@ struct Batch {
         }
         const DevSet& Li = ib[0].L[cur];
         const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(n + an, kBlock), 1024));
-        hipLaunchKernelGGL(kw_expect<Wt>, dim3(grid), dim3(kBlock), 0, stream, Li.z, Li.x, Li.c, n, w.A.z, w.A.x,
-                           w.A.c, an, kk, w.partial);
+        hipLaunchKernelGGL((kw_expect<Wt, CT>), dim3(grid), dim3(kBlock), 0, stream, Li.z, Li.x, cptr<CT>(Li.c), n, w.A.z, w.A.x,
+                           cptr<CT>(w.A.c), an, kk, w.partial, 0);
         hipLaunchKernelGGL(kw_expect_final, dim3(1), dim3(kBlock), 0, stream, w.partial, (int64_t)grid, w.out);
         HIPCHK(hipGetLastError());
         double v = 0.0;
@@ -1515,16 +1538,16 @@ struct Batch {
     // variance(O, ψ) of the merged live set (statistics.jl:6-22): the terms
     // copied with x first into the append buffer (free at a merged state),
     // radix-sorted by (x, z), then summed per x group.
-    template <int Wt>
+    template <int Wt, typename CT>
     double win_variance(int64_t n, int cur, const uint64_t* ket) {
         WinBuf& w = win;
         if (n == 0) return 0.0;
         grow_set(w.A, n, 0);
         const DevSet& Li = ib[0].L[cur];
         const unsigned g1 = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(n, kBlock), 4096));
-        hipLaunchKernelGGL(kv_swap<Wt>, dim3(g1), dim3(kBlock), 0, stream, Li.z, Li.x, Li.c, n, w.A.z, w.A.x,
-                           w.A.c);
-        const DevSet* S = win_sort<Wt>(n);
+        hipLaunchKernelGGL((kv_swap<Wt, CT>), dim3(g1), dim3(kBlock), 0, stream, Li.z, Li.x, cptr<CT>(Li.c), n, w.A.z, w.A.x,
+                           cptr<CT>(w.A.c));
+        const DevSet* S = win_sort<Wt, CT>(n);
         Key<Wt> kk;
         for (int q = 0; q < Wt; ++q) {
             kk.z[q] = ket[q];
@@ -1535,40 +1558,41 @@ struct Batch {
         int32_t* tile_h = w.vs.get<int32_t>(1, (size_t)ntiles);
         double2* carry = w.vs.get<double2>(2, (size_t)ntiles);
         const unsigned grid = (unsigned)std::min<int64_t>(ntiles, 1024);
-        hipLaunchKernelGGL((kv_tiles<Wt, false>), dim3(grid), dim3(kBlock), 0, stream, S->z, S->x, S->c, n, kk,
+        hipLaunchKernelGGL((kv_tiles<Wt, false, CT>), dim3(grid), dim3(kBlock), 0, stream, S->z, S->x, cptr<CT>(S->c), n, kk,
                            tile_v, tile_h, carry, w.partial, w.out);
         hipLaunchKernelGGL(kv_carry, dim3(1), dim3(kBlock), 0, stream, tile_v, tile_h, ntiles, carry, w.out);
-        hipLaunchKernelGGL((kv_tiles<Wt, true>), dim3(grid), dim3(kBlock), 0, stream, S->z, S->x, S->c, n, kk,
+        hipLaunchKernelGGL((kv_tiles<Wt, true, CT>), dim3(grid), dim3(kBlock), 0, stream, S->z, S->x, cptr<CT>(S->c), n, kk,
                            tile_v, tile_h, carry, w.partial, w.out);
         hipLaunchKernelGGL(kw_expect_final, dim3(1), dim3(kBlock), 0, stream, w.partial, (int64_t)grid, w.out);
         HIPCHK(hipGetLastError());
-        double v[2] = {0.0, 0.0};
-        HIPCHK(hipMemcpyAsync(v, w.out, 2 * sizeof(double), hipMemcpyDeviceToHost, stream));
+        // real(e2 - e1 * e1) with e1 = σ[ψ] = (v[1], v[2]) (statistics.jl:19-21)
+        double v[3] = {0.0, 0.0, 0.0};
+        HIPCHK(hipMemcpyAsync(v, w.out, 3 * sizeof(double), hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
-        return v[0] - v[1] * v[1];
+        return v[0] - (v[1] * v[1] - v[2] * v[2]);
     }
 
     // _apply! of a non-compilable strategy: its operations in order.
-    template <int Wt>
+    template <int Wt, typename CT>
     int64_t win_apply_ops(int64_t n, int& cur, const pp_strategy_op* ops, int nops) {
         for (int o = 0; o < nops; ++o) {
             const pp_strategy_op& op = ops[o];
             if (op.kind == PP_OP_FILTER) {
                 win_filter(2, &op.filter);
-                n = win_compact<Wt>(n, cur, 0, 0.0, 0);
+                n = win_compact<Wt, CT>(n, cur, 0, 0.0, 0);
             } else if (op.kind == PP_OP_ADAPTIVE) {
                 // spv_evolve.jl:103-114: threshold = the (n - max_terms)-th
                 // smallest |c| (1-based), then coeff_clip!
                 pp_filter f = {INT64_MAX, INT64_MAX, INT64_MAX, op.value, 0.0, -1.0, 0.0, -1.0};
-                if (n > op.max_terms) f.thresh = win_select_abs(n, cur, n - op.max_terms - 1);
+                if (n > op.max_terms) f.thresh = win_select_abs<CT>(n, cur, n - op.max_terms - 1);
                 win_filter(2, &f);
-                n = win_compact<Wt>(n, cur, 0, 0.0, 0);
+                n = win_compact<Wt, CT>(n, cur, 0, 0.0, 0);
             } else if (op.kind == PP_OP_STOCHASTIC_COEFF) {
                 const uint64_t seed = op.seed ^ (0x9e3779b97f4a7c15ull * (++win.rng_calls));
-                n = win_compact<Wt>(n, cur, 1, op.value, seed);
+                n = win_compact<Wt, CT>(n, cur, 1, op.value, seed);
             } else if (op.kind == PP_OP_STOCHASTIC_SAMPLING) {
                 const uint64_t seed = op.seed ^ (0x9e3779b97f4a7c15ull * (++win.rng_calls));
-                n = win_sample<Wt>(n, cur, op.max_terms, seed);
+                n = win_sample<Wt, CT>(n, cur, op.max_terms, seed);
             } else {
                 throw PPError(PP_EARG, "unknown strategy operation kind " + std::to_string(op.kind));
             }
@@ -1585,7 +1609,7 @@ struct Batch {
         double variance = 0.0;
     };
 
-    template <int Wt>
+    template <int Wt, typename CT>
     void evolve_window(const std::vector<GenDev>& gens, int64_t window, const pp_filter* strict, const pp_filter* local,
                        WinArgs& a, pp_counters* cnt) {
         win_init();
@@ -1613,31 +1637,31 @@ struct Batch {
             const auto t0 = std::chrono::steady_clock::now();
             if (a.var) {
                 const int64_t m = an;
-                DevSet* S = win_sort<Wt>(m);
-                const int64_t out = win_merge<Wt>(n, cur, S, m, 0);
+                DevSet* S = win_sort<Wt, CT>(m);
+                const int64_t out = win_merge<Wt, CT>(n, cur, S, m, 0);
                 if (out > live_cap) live_cap = std::max(2 * live_cap, n + m);
                 c.merge_in += n + m;
                 c.merge_out += out;
                 cur ^= 1;
                 n = out;
                 an = 0;
-                const double e0 = win_expect<Wt>(n, cur, 0, a.ket), v0 = win_variance<Wt>(n, cur, a.ket);
+                const double e0 = win_expect<Wt, CT>(n, cur, 0, a.ket), v0 = win_variance<Wt, CT>(n, cur, a.ket);
                 if (a.nops > 0) {
-                    n = win_apply_ops<Wt>(n, cur, a.ops, a.nops);
+                    n = win_apply_ops<Wt, CT>(n, cur, a.ops, a.nops);
                 } else if (strict) {
                     win_filter(2, strict);
-                    n = win_compact<Wt>(n, cur, 0, 0.0, 0);
+                    n = win_compact<Wt, CT>(n, cur, 0, 0.0, 0);
                 }
-                a.energy += win_expect<Wt>(n, cur, 0, a.ket) - e0;
-                a.variance += win_variance<Wt>(n, cur, a.ket) - v0;
+                a.energy += win_expect<Wt, CT>(n, cur, 0, a.ket) - e0;
+                a.variance += win_variance<Wt, CT>(n, cur, a.ket) - v0;
                 if (early) c.early_merges += 1;
                 t_merge += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
                 return;
             }
-            const double before = a.ket ? win_expect<Wt>(n, cur, an, a.ket) : 0.0;
+            const double before = a.ket ? win_expect<Wt, CT>(n, cur, an, a.ket) : 0.0;
             const int64_t m = an;
-            DevSet* S = win_sort<Wt>(m);
-            const int64_t out = win_merge<Wt>(n, cur, S, m, 0);
+            DevSet* S = win_sort<Wt, CT>(m);
+            const int64_t out = win_merge<Wt, CT>(n, cur, S, m, 0);
             // _try_merge! overflow -> _grow_live!(v, n + m) (spv_kernels.jl:377-386)
             if (out > live_cap) live_cap = std::max(2 * live_cap, n + m);
             c.merge_in += n + m;
@@ -1645,8 +1669,8 @@ struct Batch {
             cur ^= 1;
             n = out;
             an = 0;
-            if (a.nops > 0) n = win_apply_ops<Wt>(n, cur, a.ops, a.nops);
-            if (a.ket) a.energy += win_expect<Wt>(n, cur, 0, a.ket) - before;
+            if (a.nops > 0) n = win_apply_ops<Wt, CT>(n, cur, a.ops, a.nops);
+            if (a.ket) a.energy += win_expect<Wt, CT>(n, cur, 0, a.ket) - before;
             if (early) c.early_merges += 1;
             t_merge += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         };
@@ -1660,7 +1684,7 @@ struct Batch {
             }
             c.terms_swept += n + an;
             c.rotations += 1;
-            const int64_t created = win_rotate<Wt>(w.gens + i, n, an, cur);
+            const int64_t created = win_rotate<Wt, CT>(w.gens + i, n, an, cur);
             c.terms_created += created;
             an += created;
             since += 1;
@@ -1693,18 +1717,18 @@ struct Batch {
     }
 
     // truncate!(v, strategy, corr) for any strategy (truncation.jl:330-337).
-    template <int Wt>
+    template <int Wt, typename CT>
     void truncate_ops(const pp_strategy_op* ops, int nops, const uint64_t* ket, double* energy, double* variance) {
         win_init();
         sync_state();
         SegState& st = h_state[0];
         int cur = st.cur;
         int64_t n = st.n[cur];
-        const double before = ket ? win_expect<Wt>(n, cur, 0, ket) : 0.0;
-        const double vbefore = ket && variance ? win_variance<Wt>(n, cur, ket) : 0.0;
-        n = win_apply_ops<Wt>(n, cur, ops, nops);
-        if (ket) *energy += win_expect<Wt>(n, cur, 0, ket) - before;
-        if (ket && variance) *variance += win_variance<Wt>(n, cur, ket) - vbefore;
+        const double before = ket ? win_expect<Wt, CT>(n, cur, 0, ket) : 0.0;
+        const double vbefore = ket && variance ? win_variance<Wt, CT>(n, cur, ket) : 0.0;
+        n = win_apply_ops<Wt, CT>(n, cur, ops, nops);
+        if (ket) *energy += win_expect<Wt, CT>(n, cur, 0, ket) - before;
+        if (ket && variance) *variance += win_variance<Wt, CT>(n, cur, ket) - vbefore;
         st.cur = cur;
         st.n[cur] = n;
         st.n[cur ^ 1] = 0;
@@ -1725,22 +1749,22 @@ struct Batch {
         return ib[0].L[st.cur];
     }
 
-    template <int Wt>
+    template <int Wt, typename CT>
     void lookup(const uint64_t* z, const uint64_t* x, int64_t count, double* out, int32_t* found) {
         if (count <= 0) return;
         int64_t n = 0;
         const DevSet& L = live0(n);
         uint64_t* qz = dalloc<uint64_t>((size_t)count * Wt);
         uint64_t* qx = dalloc<uint64_t>((size_t)count * Wt);
-        double* dout = dalloc<double>((size_t)count);
+        double* dout = dalloc<double>((size_t)count * cw);
         int32_t* dfound = dalloc<int32_t>((size_t)count);
         HIPCHK(hipMemcpyAsync(qz, z, (size_t)count * Wt * 8, hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemcpyAsync(qx, x, (size_t)count * Wt * 8, hipMemcpyHostToDevice, stream));
         const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(count, kBlock), 4096));
-        hipLaunchKernelGGL(ko_lookup<Wt>, dim3(grid), dim3(kBlock), 0, stream, L.z, L.x, L.c, n, qz, qx, count, dout,
-                           dfound, (int64_t*)nullptr);
+        hipLaunchKernelGGL((ko_lookup<Wt, CT>), dim3(grid), dim3(kBlock), 0, stream, L.z, L.x, cptr<CT>(L.c), n, qz, qx,
+                           count, cptr<CT>(dout), dfound, (int64_t*)nullptr);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(out, dout, (size_t)count * 8, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(out, dout, (size_t)count * cb(), hipMemcpyDeviceToHost, stream));
         HIPCHK(hipMemcpyAsync(found, dfound, (size_t)count * 4, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
         dfree(qz); dfree(qx); dfree(dout); dfree(dfound);
@@ -1748,7 +1772,7 @@ struct Batch {
 
     // lookup with the staged queries and results in `base` (the scratch array
     // of pp_lookup_noalloc; count <= kLookupNoAllocMax): no device allocation.
-    template <int Wt>
+    template <int Wt, typename CT>
     void lookup_noalloc(const uint64_t* z, const uint64_t* x, int64_t count, double* out, int32_t* found,
                         unsigned char* base) {
         if (count <= 0) return;
@@ -1757,32 +1781,32 @@ struct Batch {
         uint64_t* qz = reinterpret_cast<uint64_t*>(base);
         uint64_t* qx = qz + kLookupNoAllocMax * 2;
         double* dout = reinterpret_cast<double*>(qx + kLookupNoAllocMax * 2);
-        int32_t* dfound = reinterpret_cast<int32_t*>(dout + kLookupNoAllocMax);
+        int32_t* dfound = reinterpret_cast<int32_t*>(dout + kLookupNoAllocMax * 2);
         HIPCHK(hipMemcpyAsync(qz, z, (size_t)count * Wt * 8, hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemcpyAsync(qx, x, (size_t)count * Wt * 8, hipMemcpyHostToDevice, stream));
         const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(count, kBlock), 4096));
-        hipLaunchKernelGGL(ko_lookup<Wt>, dim3(grid), dim3(kBlock), 0, stream, L.z, L.x, L.c, n, qz, qx, count, dout,
-                           dfound, (int64_t*)nullptr);
+        hipLaunchKernelGGL((ko_lookup<Wt, CT>), dim3(grid), dim3(kBlock), 0, stream, L.z, L.x, cptr<CT>(L.c), n, qz, qx,
+                           count, cptr<CT>(dout), dfound, (int64_t*)nullptr);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(out, dout, (size_t)count * 8, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(out, dout, (size_t)count * cb(), hipMemcpyDeviceToHost, stream));
         HIPCHK(hipMemcpyAsync(found, dfound, (size_t)count * 4, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
     }
 
     // Device indices of `count` keys in the live set of vector 0 (-1: absent).
-    template <int Wt>
+    template <int Wt, typename CT>
     std::vector<int64_t> find_indices(const uint64_t* z, const uint64_t* x, int64_t count, int64_t* d_idx_out) {
         int64_t n = 0;
         const DevSet& L = live0(n);
         uint64_t* qz = dalloc<uint64_t>((size_t)count * Wt);
         uint64_t* qx = dalloc<uint64_t>((size_t)count * Wt);
-        double* dout = dalloc<double>((size_t)count);
+        double* dout = dalloc<double>((size_t)count * cw);
         int32_t* dfound = dalloc<int32_t>((size_t)count);
         HIPCHK(hipMemcpyAsync(qz, z, (size_t)count * Wt * 8, hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemcpyAsync(qx, x, (size_t)count * Wt * 8, hipMemcpyHostToDevice, stream));
         const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(count, kBlock), 4096));
-        hipLaunchKernelGGL(ko_lookup<Wt>, dim3(grid), dim3(kBlock), 0, stream, L.z, L.x, L.c, n, qz, qx, count, dout,
-                           dfound, d_idx_out);
+        hipLaunchKernelGGL((ko_lookup<Wt, CT>), dim3(grid), dim3(kBlock), 0, stream, L.z, L.x, cptr<CT>(L.c), n, qz, qx,
+                           count, cptr<CT>(dout), dfound, d_idx_out);
         HIPCHK(hipGetLastError());
         std::vector<int64_t> idx((size_t)count);
         HIPCHK(hipMemcpyAsync(idx.data(), d_idx_out, (size_t)count * 8, hipMemcpyDeviceToHost, stream));
@@ -1806,7 +1830,7 @@ struct Batch {
 
     // setindex! for many keys (type_SparsePauliVector.jl:315-335); the last
     // of repeated keys wins, as with sequential calls.
-    template <int Wt>
+    template <int Wt, typename CT>
     void setindex(const uint64_t* z, const uint64_t* x, const double* c, int64_t count) {
         if (count <= 0) return;
         // last occurrence of each key
@@ -1825,16 +1849,16 @@ struct Batch {
             if (i + 1 == order.size() || kless(order[i], order[i + 1])) uniq.push_back(order[i]);
         const int64_t u = (int64_t)uniq.size();
         std::vector<uint64_t> uz((size_t)u * Wt), ux((size_t)u * Wt);
-        std::vector<double> uc((size_t)u);
+        std::vector<double> uc((size_t)u * cw);
         for (int64_t i = 0; i < u; ++i) {
             for (int w = 0; w < Wt; ++w) {
                 uz[i * Wt + w] = z[uniq[i] * Wt + w];
                 ux[i * Wt + w] = x[uniq[i] * Wt + w];
             }
-            uc[i] = c[uniq[i]];
+            for (int q = 0; q < cw; ++q) uc[i * cw + q] = c[uniq[i] * cw + q];
         }
         int64_t* didx = dalloc<int64_t>((size_t)u);
-        const std::vector<int64_t> idx = find_indices<Wt>(uz.data(), ux.data(), u, didx);
+        const std::vector<int64_t> idx = find_indices<Wt, CT>(uz.data(), ux.data(), u, didx);
         dfree(didx);
         std::vector<int64_t> fi;
         std::vector<double> fv;
@@ -1843,13 +1867,13 @@ struct Batch {
         for (int64_t i = 0; i < u; ++i) {
             if (idx[i] >= 0) {
                 fi.push_back(idx[i]);
-                fv.push_back(uc[i]);
+                for (int q = 0; q < cw; ++q) fv.push_back(uc[i * cw + q]);
             } else {
                 for (int w = 0; w < Wt; ++w) {
                     nz.push_back(uz[i * Wt + w]);
                     nx.push_back(ux[i * Wt + w]);
                 }
-                nc.push_back(uc[i]);
+                for (int q = 0; q < cw; ++q) nc.push_back(uc[i * cw + q]);
             }
         }
         SegState& st = h_state[0];
@@ -1862,14 +1886,14 @@ struct Batch {
             HIPCHK(hipMemcpyAsync(di, fi.data(), fi.size() * 8, hipMemcpyHostToDevice, stream));
             HIPCHK(hipMemcpyAsync(dv, fv.data(), fv.size() * 8, hipMemcpyHostToDevice, stream));
             const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv((int64_t)fi.size(), kBlock), 4096));
-            hipLaunchKernelGGL(ko_scatter, dim3(grid), dim3(kBlock), 0, stream, ib[0].L[cur].c, (const int64_t*)di,
-                               (const double*)dv, (int64_t)fi.size());
+            hipLaunchKernelGGL(ko_scatter<CT>, dim3(grid), dim3(kBlock), 0, stream, cptr<CT>(ib[0].L[cur].c),
+                               (const int64_t*)di, cptr<CT>((const double*)dv), (int64_t)fi.size());
             HIPCHK(hipGetLastError());
             HIPCHK(hipStreamSynchronize(stream));
             dfree(di);
             dfree(dv);
         }
-        const int64_t m = (int64_t)nc.size();
+        const int64_t m = (int64_t)nc.size() / cw;
         if (m > 0) {
             // the reference inserts one key at a time, growing when full
             for (int64_t i = 0; i < m; ++i) {
@@ -1880,10 +1904,10 @@ struct Batch {
             grow_set(w.A, m, 0);
             HIPCHK(hipMemcpyAsync(w.A.z, nz.data(), (size_t)m * Wt * 8, hipMemcpyHostToDevice, stream));
             HIPCHK(hipMemcpyAsync(w.A.x, nx.data(), (size_t)m * Wt * 8, hipMemcpyHostToDevice, stream));
-            HIPCHK(hipMemcpyAsync(w.A.c, nc.data(), (size_t)m * 8, hipMemcpyHostToDevice, stream));
-            DevSet* S = win_sort<Wt>(m);
+            HIPCHK(hipMemcpyAsync(w.A.c, nc.data(), (size_t)m * cb(), hipMemcpyHostToDevice, stream));
+            DevSet* S = win_sort<Wt, CT>(m);
             win_filter(0, nullptr);
-            n = win_merge<Wt>(n, cur, S, m, 0);
+            n = win_merge<Wt, CT>(n, cur, S, m, 0);
             cur ^= 1;
         }
         commit_live0(cur, n, live_cap);
@@ -1891,7 +1915,7 @@ struct Batch {
 
     // sum!(v1, v2) (spv_ops.jl:182-193): v2's sorted live terms merged into
     // vector 0 with NOFILTER; _try_merge! growth when the union overflows.
-    template <int Wt>
+    template <int Wt, typename CT>
     void sum_from(Batch& other) {
         int64_t n2 = 0;
         const DevSet& L2 = other.live0(n2);
@@ -1906,31 +1930,35 @@ struct Batch {
         grow_set(w.A, n2, 0);
         HIPCHK(hipMemcpyAsync(w.A.z, L2.z, (size_t)n2 * kb(), hipMemcpyDeviceToDevice, stream));
         HIPCHK(hipMemcpyAsync(w.A.x, L2.x, (size_t)n2 * kb(), hipMemcpyDeviceToDevice, stream));
-        HIPCHK(hipMemcpyAsync(w.A.c, L2.c, (size_t)n2 * sizeof(double), hipMemcpyDeviceToDevice, stream));
+        HIPCHK(hipMemcpyAsync(w.A.c, L2.c, (size_t)n2 * cb(), hipMemcpyDeviceToDevice, stream));
         win_filter(0, nullptr);
-        const int64_t out = win_merge<Wt>(n, cur, &w.A, n2, 0);
+        const int64_t out = win_merge<Wt, CT>(n, cur, &w.A, n2, 0);
         if (out > live_cap) live_cap = std::max(2 * live_cap, n + n2);
         commit_live0(cur ^ 1, out, live_cap);
     }
 
-    // mul!(v, a) (spv_ops.jl:12-17)
-    void scale(double a) {
+    // mul!(v, a) (spv_ops.jl:12-17): a real a scales every component; a
+    // complex a (ComplexF64 coefficients only) multiplies as Complex * Complex.
+    void scale(double ar, double ai, bool complex_scalar) {
         int64_t n = 0;
         const DevSet& L = live0(n);
         if (n == 0) return;
         const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(n, kBlock), 2048));
-        hipLaunchKernelGGL(kw_scale, dim3(grid), dim3(kBlock), 0, stream, L.c, n, a);
+        if (complex_scalar)
+            hipLaunchKernelGGL(kw_scale_c, dim3(grid), dim3(kBlock), 0, stream, cptr<pp_cplx>(L.c), n, ar, ai);
+        else
+            hipLaunchKernelGGL(kw_scale, dim3(grid), dim3(kBlock), 0, stream, L.c, n * cw, ar);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(stream));
     }
 
     // delete! for many keys (type_SparsePauliVector.jl:345-358); absent keys
     // are ignored.
-    template <int Wt>
+    template <int Wt, typename CT>
     void delete_keys(const uint64_t* z, const uint64_t* x, int64_t count) {
         if (count <= 0) return;
         int64_t* didx = dalloc<int64_t>((size_t)count);
-        const std::vector<int64_t> idx = find_indices<Wt>(z, x, count, didx);
+        const std::vector<int64_t> idx = find_indices<Wt, CT>(z, x, count, didx);
         std::vector<int64_t> hit;
         for (int64_t v : idx)
             if (v >= 0) hit.push_back(v);
@@ -1949,7 +1977,7 @@ struct Batch {
             win_init();
             SampleSel ss{};
             ss.mark = mark;
-            n = win_compact<Wt>(n, cur, 3, 0.0, 0, ss);
+            n = win_compact<Wt, CT>(n, cur, 3, 0.0, 0, ss);
             HIPCHK(hipStreamSynchronize(stream));
             dfree(mark);
             dfree(di);
@@ -1969,17 +1997,19 @@ struct Batch {
         return v;
     }
 
-    // kind 0: Σ c^2, 1: Σ |c|, 2: max |c|, 3: Σ |c|^p
+    // kind 0: Σ abs2(c), 1: Σ |c|, 2: max |c|, 3: Σ |c|^p
+    template <typename CT>
     double norm_sum(int kind, double p) {
         int64_t n = 0;
         const DevSet& L = live0(n);
-        return norm_sum_of(L.c, n, kind, p);
+        return norm_sum_of<CT>(L.c, n, kind, p);
     }
 
+    template <typename CT>
     double norm_sum_of(const double* c, int64_t n, int kind, double p) {
         const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(n, kBlock), 1024));
         double2* part = dalloc<double2>(grid);
-        hipLaunchKernelGGL(ko_norm, dim3(grid), dim3(kBlock), 0, stream, c, n, kind, p, part);
+        hipLaunchKernelGGL(ko_norm<CT>, dim3(grid), dim3(kBlock), 0, stream, cptr<CT>(c), n, kind, p, part);
         HIPCHK(hipGetLastError());
         const double v = reduce_partials(part, grid, kind == 2);
         dfree(part);
@@ -1987,15 +2017,15 @@ struct Batch {
     }
 
     // Σ over this vector's terms of ko_join's mode term against `other`.
-    template <int Wt>
-    double join(Batch& other, int mode) {
+    template <int Wt, typename CT>
+    double join(Batch& other, int mode, int comp = 0) {
         int64_t na = 0, nb = 0;
         const DevSet& A = live0(na);
         const DevSet& Bs = other.live0(nb);
         const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(na, kBlock), 1024));
         double2* part = dalloc<double2>(grid);
-        hipLaunchKernelGGL(ko_join<Wt>, dim3(grid), dim3(kBlock), 0, stream, A.z, A.x, A.c, na, Bs.z, Bs.x, Bs.c, nb,
-                           mode, part);
+        hipLaunchKernelGGL((ko_join<Wt, CT>), dim3(grid), dim3(kBlock), 0, stream, A.z, A.x, cptr<CT>(A.c), na, Bs.z, Bs.x, cptr<CT>(Bs.c), nb,
+                           mode, comp, part);
         HIPCHK(hipGetLastError());
         const double v = reduce_partials(part, grid, false);
         dfree(part);
@@ -2005,7 +2035,8 @@ struct Batch {
     void upload(int b, const uint64_t* z, const uint64_t* x, const double* c, int64_t n, bool sorted) {
         const int Wl = W;
         std::vector<uint64_t> hz(z, z + n * Wl), hx(x, x + n * Wl);
-        std::vector<double> hc(c, c + n);
+        const int C = cw;
+        std::vector<double> hc(c, c + n * C);
         auto less = [&](int64_t a, int64_t q) {
             for (int w = Wl - 1; w >= 0; --w)
                 if (hz[a * Wl + w] != hz[q * Wl + w]) return hz[a * Wl + w] < hz[q * Wl + w];
@@ -2018,13 +2049,13 @@ struct Batch {
             std::iota(perm.begin(), perm.end(), 0);
             std::sort(perm.begin(), perm.end(), less);
             std::vector<uint64_t> sz((size_t)n * Wl), sx((size_t)n * Wl);
-            std::vector<double> sc((size_t)n);
+            std::vector<double> sc((size_t)n * C);
             for (int64_t i = 0; i < n; ++i) {
                 for (int w = 0; w < Wl; ++w) {
                     sz[i * Wl + w] = hz[perm[i] * Wl + w];
                     sx[i * Wl + w] = hx[perm[i] * Wl + w];
                 }
-                sc[i] = hc[perm[i]];
+                for (int q = 0; q < C; ++q) sc[i * C + q] = hc[perm[i] * C + q];
             }
             hz.swap(sz);
             hx.swap(sx);
@@ -2040,7 +2071,7 @@ struct Batch {
         if (n > 0) {
             HIPCHK(hipMemcpyAsync(s.z, hz.data(), (size_t)n * Wl * 8, hipMemcpyHostToDevice, stream));
             HIPCHK(hipMemcpyAsync(s.x, hx.data(), (size_t)n * Wl * 8, hipMemcpyHostToDevice, stream));
-            HIPCHK(hipMemcpyAsync(s.c, hc.data(), (size_t)n * 8, hipMemcpyHostToDevice, stream));
+            HIPCHK(hipMemcpyAsync(s.c, hc.data(), (size_t)n * cb(), hipMemcpyHostToDevice, stream));
         }
         SegState& st = h_state[b];
         st.n[0] = n;
@@ -2061,7 +2092,7 @@ struct Batch {
     // (the append buffer is empty between calls): pairs in generation order,
     // stable radix sort by key, run sums.  Host arrays in and out; returns
     // the output count (terms written only when it fits `cap`).
-    template <int Wt>
+    template <int Wt, typename CT>
     int64_t algebra(int op, int cplx, const pp_terms& a, const pp_terms& b, uint64_t* oz, uint64_t* ox, double* ore,
                     double* oim, int64_t cap) {
         win_init();
@@ -2108,7 +2139,8 @@ struct Batch {
         HIPCHK(hipStreamSynchronize(stream));
         if (e & 16)
             throw PPError(PP_EARG, "InexactError: a product coefficient is not real (real coefficient type)");
-        const DevSet* S = win_sort<Wt>(m);
+        // the pair records carry one double each, whatever the handle's type
+        const DevSet* S = win_sort<Wt, double>(m);
         // runs: count, scan, write
         const int64_t mblk = cdiv(m, kRB);
         win_blocks(mblk);
@@ -2143,7 +2175,7 @@ struct Batch {
         if (n > 0) {
             HIPCHK(hipMemcpyAsync(z, s.z, (size_t)n * W * 8, hipMemcpyDeviceToHost, stream));
             HIPCHK(hipMemcpyAsync(x, s.x, (size_t)n * W * 8, hipMemcpyDeviceToHost, stream));
-            HIPCHK(hipMemcpyAsync(c, s.c, (size_t)n * 8, hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipMemcpyAsync(c, s.c, (size_t)n * cb(), hipMemcpyDeviceToHost, stream));
         }
         HIPCHK(hipStreamSynchronize(stream));
         return n;
@@ -2228,10 +2260,23 @@ struct pp_batch {
     g_last_error.clear();                   \
     return PP_OK;
 
+template <typename T>
+struct TypeTag {
+    using type = T;
+};
+
+// f(words per key, coefficient type) for a batch's layout.
 template <typename F>
-static void dispatch_w(int W, F&& f) {
-    if (W == 1) f(std::integral_constant<int, 1>());
-    else f(std::integral_constant<int, 2>());
+static void dispatch_wc(const Batch& bt, F&& f) {
+    using W1 = std::integral_constant<int, 1>;
+    using W2 = std::integral_constant<int, 2>;
+    if (bt.cw == 1) {
+        if (bt.W == 1) f(W1(), TypeTag<double>());
+        else f(W2(), TypeTag<double>());
+    } else {
+        if (bt.W == 1) f(W1(), TypeTag<pp_cplx>());
+        else f(W2(), TypeTag<pp_cplx>());
+    }
 }
 
 static void batch_evolve(Batch& bt, int64_t nsteps, const uint64_t* gz, const uint64_t* gx, const double* cosv,
@@ -2243,7 +2288,7 @@ static void batch_evolve(Batch& bt, int64_t nsteps, const uint64_t* gz, const ui
     make_steps(st, bt.W, bt.B, nsteps, gz, gx, cosv, sinv, active, en);
     std::vector<SegState> before(bt.h_state, bt.h_state + bt.B);
     bt.set_filters(strict, local);
-    dispatch_w(bt.W, [&](auto wc) { bt.run_sequence<decltype(wc)::value>(st, nsteps, en); });
+    dispatch_wc(bt, [&](auto wc, auto ct) { bt.run_sequence<decltype(wc)::value, typename decltype(ct)::type>(st, nsteps, en); });
     if (counters)
         for (int b = 0; b < bt.B; ++b) {
             std::memset(counters + b, 0, sizeof(pp_counters));
@@ -2262,14 +2307,19 @@ double pp_julia_powi(double x, int64_t n) { return julia_powi(x, n); }
 // --- single vector ------------------------------------------------------------
 
 int pp_create(int N, int64_t live_cap, int64_t append_cap, pp_handle** out) {
+    return pp_create_typed(N, live_cap, append_cap, 0, out);
+}
+
+int pp_create_typed(int N, int64_t live_cap, int64_t append_cap, int complex_coeffs, pp_handle** out) {
     if (!out) return fail(PP_EARG, "out is NULL");
     *out = nullptr;
     if (N < 1 || N > 128) return fail(PP_EARG, "N must be in 1:128");
     if (live_cap < 1 || append_cap < 1) return fail(PP_EARG, "capacities must be >= 1");
+    if (complex_coeffs != 0 && complex_coeffs != 1) return fail(PP_EARG, "complex_coeffs must be 0 or 1");
     pp_handle* h = new pp_handle();
     try {
         h->N = N;
-        h->bt.init(N <= 64 ? 1 : 2, {N}, {live_cap}, {append_cap});
+        h->bt.init(N <= 64 ? 1 : 2, {N}, {live_cap}, {append_cap}, complex_coeffs ? 2 : 1);
     } catch (const PPError& e) {
         delete h;
         return fail(e.code, e.msg);
@@ -2280,6 +2330,13 @@ int pp_create(int N, int64_t live_cap, int64_t append_cap, pp_handle** out) {
         return rc;
     }
     *out = h;
+    g_last_error.clear();
+    return PP_OK;
+}
+
+int pp_coeff_type(pp_handle* h, int* complex_coeffs) {
+    if (!h || !complex_coeffs) return fail(PP_EARG, "null argument");
+    *complex_coeffs = h->bt.cw == 2 ? 1 : 0;
     g_last_error.clear();
     return PP_OK;
 }
@@ -2313,8 +2370,8 @@ int pp_algebra(pp_handle* h, int op, int complex_coeffs, const pp_terms* a, cons
         return fail(PP_EARG, "bad output arrays");
     API_BEGIN
     int64_t n = 0;
-    dispatch_w(h->bt.W, [&](auto wc) {
-        n = h->bt.algebra<decltype(wc)::value>(op, complex_coeffs ? 1 : 0, *a, *b, oz, ox, ore, oim, cap);
+    dispatch_wc(h->bt, [&](auto wc, auto ct) {
+        n = h->bt.algebra<decltype(wc)::value, typename decltype(ct)::type>(op, complex_coeffs ? 1 : 0, *a, *b, oz, ox, ore, oim, cap);
     });
     *nout = n;
     if (n > cap) return fail(PP_EOVERFLOW, "output capacity too small (" + std::to_string(n) + " terms)");
@@ -2416,7 +2473,7 @@ int pp_evolve_ex(pp_handle* h, const uint64_t* gz, const uint64_t* gx, const dou
     a.energy = energy_ket ? *energy_acc : 0.0;
     a.var = variance_acc != nullptr;
     a.variance = variance_acc ? *variance_acc : 0.0;
-    dispatch_w(bt.W, [&](auto wc) { bt.evolve_window<decltype(wc)::value>(gens, window, strict, local, a, counters); });
+    dispatch_wc(bt, [&](auto wc, auto ct) { bt.evolve_window<decltype(wc)::value, typename decltype(ct)::type>(gens, window, strict, local, a, counters); });
     if (energy_ket) *energy_acc = a.energy;
     if (variance_acc) *variance_acc = a.variance;
     API_END
@@ -2431,8 +2488,8 @@ int pp_truncate(pp_handle* h, const pp_strategy_op* ops, int nops, const uint64_
     check_ops(ops, nops);
     double e = energy_ket ? *energy_acc : 0.0;
     double v = variance_acc ? *variance_acc : 0.0;
-    dispatch_w(h->bt.W, [&](auto wc) {
-        h->bt.truncate_ops<decltype(wc)::value>(ops, nops, energy_ket, &e, variance_acc ? &v : nullptr);
+    dispatch_wc(h->bt, [&](auto wc, auto ct) {
+        h->bt.truncate_ops<decltype(wc)::value, typename decltype(ct)::type>(ops, nops, energy_ket, &e, variance_acc ? &v : nullptr);
     });
     if (energy_ket) *energy_acc = e;
     if (variance_acc) *variance_acc = v;
@@ -2448,14 +2505,14 @@ int pp_variance_ket(pp_handle* h, const uint64_t* ket, double* out) {
     const SegState& st = bt.h_state[0];
     const int64_t n = st.n[st.cur];
     const int cur = st.cur;
-    dispatch_w(bt.W, [&](auto wc) { *out = bt.win_variance<decltype(wc)::value>(n, cur, ket); });
+    dispatch_wc(bt, [&](auto wc, auto ct) { *out = bt.win_variance<decltype(wc)::value, typename decltype(ct)::type>(n, cur, ket); });
     API_END
 }
 
 int pp_lookup(pp_handle* h, const uint64_t* z, const uint64_t* x, int64_t count, double* c, int32_t* found) {
     if (!h || count < 0 || (count > 0 && (!z || !x || !c || !found))) return fail(PP_EARG, "bad arguments");
     API_BEGIN
-    dispatch_w(h->bt.W, [&](auto wc) { h->bt.lookup<decltype(wc)::value>(z, x, count, c, found); });
+    dispatch_wc(h->bt, [&](auto wc, auto ct) { h->bt.lookup<decltype(wc)::value, typename decltype(ct)::type>(z, x, count, c, found); });
     API_END
 }
 
@@ -2469,21 +2526,21 @@ int pp_lookup_noalloc(pp_handle* h, const uint64_t* z, const uint64_t* x, int64_
     unsigned char* base = dev >= 0 && dev < kScratchDevices ? static_cast<unsigned char*>(g_scratch_ptr[dev]) : nullptr;
     if (!base) return fail(PP_EHIP, "pp_lookup_noalloc: no scratch address for this device");
     API_BEGIN
-    dispatch_w(h->bt.W, [&](auto wc) { h->bt.lookup_noalloc<decltype(wc)::value>(z, x, count, c, found, base); });
+    dispatch_wc(h->bt, [&](auto wc, auto ct) { h->bt.lookup_noalloc<decltype(wc)::value, typename decltype(ct)::type>(z, x, count, c, found, base); });
     API_END
 }
 
 int pp_setindex(pp_handle* h, const uint64_t* z, const uint64_t* x, const double* c, int64_t count) {
     if (!h || count < 0 || (count > 0 && (!z || !x || !c))) return fail(PP_EARG, "bad arguments");
     API_BEGIN
-    dispatch_w(h->bt.W, [&](auto wc) { h->bt.setindex<decltype(wc)::value>(z, x, c, count); });
+    dispatch_wc(h->bt, [&](auto wc, auto ct) { h->bt.setindex<decltype(wc)::value, typename decltype(ct)::type>(z, x, c, count); });
     API_END
 }
 
 int pp_delete(pp_handle* h, const uint64_t* z, const uint64_t* x, int64_t count) {
     if (!h || count < 0 || (count > 0 && (!z || !x))) return fail(PP_EARG, "bad arguments");
     API_BEGIN
-    dispatch_w(h->bt.W, [&](auto wc) { h->bt.delete_keys<decltype(wc)::value>(z, x, count); });
+    dispatch_wc(h->bt, [&](auto wc, auto ct) { h->bt.delete_keys<decltype(wc)::value, typename decltype(ct)::type>(z, x, count); });
     API_END
 }
 
@@ -2491,15 +2548,25 @@ int pp_sum(pp_handle* a, pp_handle* b) {
     if (!a || !b) return fail(PP_EARG, "null argument");
     if (a == b) return fail(PP_EARG, "sum!(v, v) needs two distinct vectors");
     if (a->N != b->N) return fail(PP_EARG, "qubit counts differ");
+    if (a->bt.cw != b->bt.cw) return fail(PP_EARG, "coefficient types differ");
     API_BEGIN
-    dispatch_w(a->bt.W, [&](auto wc) { a->bt.sum_from<decltype(wc)::value>(b->bt); });
+    dispatch_wc(a->bt, [&](auto wc, auto ct) { a->bt.sum_from<decltype(wc)::value, typename decltype(ct)::type>(b->bt); });
     API_END
 }
 
 int pp_scale(pp_handle* h, double a) {
     if (!h) return fail(PP_EARG, "null handle");
     API_BEGIN
-    h->bt.scale(a);
+    h->bt.scale(a, 0.0, false);
+    API_END
+}
+
+int pp_scale_complex(pp_handle* h, double re, double im) {
+    if (!h) return fail(PP_EARG, "null handle");
+    if (h->bt.cw != 2)
+        return fail(PP_EARG, "InexactError: a complex scalar needs ComplexF64 coefficients");
+    API_BEGIN
+    h->bt.scale(re, im, true);
     API_END
 }
 
@@ -2507,29 +2574,39 @@ int pp_norm(pp_handle* h, double p, double* out) {
     if (!h || !out) return fail(PP_EARG, "null argument");
     if (!(p > 0)) return fail(PP_EARG, "p must be > 0");
     API_BEGIN
-    if (p == 2.0) *out = std::sqrt(h->bt.norm_sum(0, 2.0));
-    else if (p == 1.0) *out = h->bt.norm_sum(1, 1.0);
-    else if (std::isinf(p)) *out = h->bt.norm_sum(2, p);
-    else *out = std::pow(h->bt.norm_sum(3, p), 1.0 / p);
+    dispatch_wc(h->bt, [&](auto, auto ct) {
+        using CT = typename decltype(ct)::type;
+        Batch& bt = h->bt;
+        if (p == 2.0) *out = std::sqrt(bt.norm_sum<CT>(0, 2.0));
+        else if (p == 1.0) *out = bt.norm_sum<CT>(1, 1.0);
+        else if (std::isinf(p)) *out = bt.norm_sum<CT>(2, p);
+        else *out = std::pow(bt.norm_sum<CT>(3, p), 1.0 / p);
+    });
     API_END
 }
 
 int pp_inner_product(pp_handle* a, pp_handle* b, double* out) {
     if (!a || !b || !out) return fail(PP_EARG, "null argument");
     if (a->N != b->N) return fail(PP_EARG, "qubit counts differ");
+    if (a->bt.cw != b->bt.cw) return fail(PP_EARG, "coefficient types differ");
     API_BEGIN
-    dispatch_w(a->bt.W, [&](auto wc) { *out = a->bt.join<decltype(wc)::value>(b->bt, 0); });
+    dispatch_wc(a->bt, [&](auto wc, auto ct) {
+        for (int comp = 0; comp < a->bt.cw; ++comp)
+            out[comp] = a->bt.join<decltype(wc)::value, typename decltype(ct)::type>(b->bt, 0, comp);
+    });
     API_END
 }
 
 int pp_diff_norm(pp_handle* a, pp_handle* b, double* out) {
     if (!a || !b || !out) return fail(PP_EARG, "null argument");
     if (a->N != b->N) return fail(PP_EARG, "qubit counts differ");
+    if (a->bt.cw != b->bt.cw) return fail(PP_EARG, "coefficient types differ");
     API_BEGIN
-    dispatch_w(a->bt.W, [&](auto wc) {
+    dispatch_wc(a->bt, [&](auto wc, auto ct) {
         constexpr int Wt = decltype(wc)::value;
-        const double s1 = a->bt.join<Wt>(b->bt, 1);
-        const double s2 = b->bt.join<Wt>(a->bt, 2);
+        using CT = typename decltype(ct)::type;
+        const double s1 = a->bt.join<Wt, CT>(b->bt, 1);
+        const double s2 = b->bt.join<Wt, CT>(a->bt, 2);
         *out = std::sqrt(s1 + s2);
     });
     API_END
@@ -2552,7 +2629,7 @@ int pp_evolve_single(pp_handle* h, const uint64_t* gz, const uint64_t* gx, doubl
     make_steps(steps, bt.W, 1, 1, gz, gx, &cosv, &sinv, nullptr, en);
     bt.set_filters(nullptr, nullptr);
     // app_cap_v >= n now, so the sequence-start precheck cannot fire
-    dispatch_w(bt.W, [&](auto wc) { bt.run_sequence<decltype(wc)::value>(steps, 1, en); });
+    dispatch_wc(bt, [&](auto wc, auto ct) { bt.run_sequence<decltype(wc)::value, typename decltype(ct)::type>(steps, 1, en); });
     API_END
 }
 
@@ -2560,7 +2637,7 @@ int pp_compact(pp_handle* h, const pp_filter* f) {
     if (!h || !f) return fail(PP_EARG, "null argument");
     API_BEGIN
     h->bt.set_filters(f, nullptr);
-    dispatch_w(h->bt.W, [&](auto wc) { h->bt.compact<decltype(wc)::value>({1}); });
+    dispatch_wc(h->bt, [&](auto wc, auto ct) { h->bt.compact<decltype(wc)::value, typename decltype(ct)::type>({1}); });
     API_END
 }
 
@@ -2585,7 +2662,7 @@ int pp_scale_by_popcount(pp_handle* h, int kind, const uint64_t* mask, const dou
     ch.t0 = bt.d_tabs;
     ch.t1 = bt.d_tabs + kExpTab;
     ch.t2 = bt.d_tabs + 2 * kExpTab;
-    dispatch_w(bt.W, [&](auto wc) { bt.channel<decltype(wc)::value>({ch}, tabs, {1}); });
+    dispatch_wc(bt, [&](auto wc, auto ct) { bt.channel<decltype(wc)::value, typename decltype(ct)::type>({ch}, tabs, {1}); });
     HIPCHK(hipStreamSynchronize(bt.stream));
     API_END
 }
@@ -2593,7 +2670,7 @@ int pp_scale_by_popcount(pp_handle* h, int kind, const uint64_t* mask, const dou
 int pp_expectation_ket(pp_handle* h, const uint64_t* ket, double* out) {
     if (!h || !ket || !out) return fail(PP_EARG, "null argument");
     API_BEGIN
-    dispatch_w(h->bt.W, [&](auto wc) { h->bt.expectation<decltype(wc)::value>(ket, out); });
+    dispatch_wc(h->bt, [&](auto wc, auto ct) { h->bt.expectation<decltype(wc)::value, typename decltype(ct)::type>(ket, out); });
     API_END
 }
 
@@ -2601,7 +2678,7 @@ int pp_check(pp_handle* h, int* ok) {
     if (!h || !ok) return fail(PP_EARG, "null argument");
     API_BEGIN
     int32_t o = 0;
-    dispatch_w(h->bt.W, [&](auto wc) { h->bt.check<decltype(wc)::value>(&o); });
+    dispatch_wc(h->bt, [&](auto wc, auto ct) { h->bt.check<decltype(wc)::value, typename decltype(ct)::type>(&o); });
     *ok = o;
     API_END
 }
@@ -2701,7 +2778,7 @@ int pp_batch_scale_by_popcount(pp_batch* p, const int* kind, const uint64_t* mas
         ch[b].t1 = bt.d_tabs + ((size_t)b * 3 + 1) * kExpTab;
         ch[b].t2 = bt.d_tabs + ((size_t)b * 3 + 2) * kExpTab;
     }
-    dispatch_w(bt.W, [&](auto wc) { bt.channel<decltype(wc)::value>(ch, tabs, en); });
+    dispatch_wc(bt, [&](auto wc, auto ct) { bt.channel<decltype(wc)::value, typename decltype(ct)::type>(ch, tabs, en); });
     HIPCHK(hipStreamSynchronize(bt.stream));
     API_END
 }
@@ -2843,7 +2920,7 @@ int pp_batch_resident_work(pp_batch* p, int64_t* rotations, int64_t* swept, int6
 int pp_batch_expectation_ket(pp_batch* p, const uint64_t* kets, double* out) {
     if (!p || !kets || !out) return fail(PP_EARG, "null argument");
     API_BEGIN
-    dispatch_w(p->bt.W, [&](auto wc) { p->bt.expectation<decltype(wc)::value>(kets, out); });
+    dispatch_wc(p->bt, [&](auto wc, auto ct) { p->bt.expectation<decltype(wc)::value, typename decltype(ct)::type>(kets, out); });
     API_END
 }
 

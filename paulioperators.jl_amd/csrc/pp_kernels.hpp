@@ -365,7 +365,7 @@ __device__ void edge_counts(int side, int li, int lvl, int64_t t, int64_t ntiles
 }
 
 // ---------------------------------------------------------------------------
-template <int W, int T>
+template <int W, int T, typename CT>
 __global__ void __launch_bounds__(kBlock) kb_flags(BlockMap bm, const SegDesc* __restrict__ D, SegState* __restrict__ S,
                                                    const StepDev* __restrict__ steps, int bnd_ok) {
     constexpr int kTile = T;
@@ -387,7 +387,7 @@ __global__ void __launch_bounds__(kBlock) kb_flags(BlockMap bm, const SegDesc* _
     const int precheck = st->precheck;
     const int64_t n0 = s->n[0], n1 = s->n[1], appcap = s->app_cap_v;
     const uint64_t *Lz0 = d.Lz[0], *Lz1 = d.Lz[1], *Lx0 = d.Lx[0], *Lx1 = d.Lx[1];
-    const double *Lc0 = d.Lc[0], *Lc1 = d.Lc[1];
+    const CT *Lc0 = reinterpret_cast<const CT*>(d.Lc[0]), *Lc1 = reinterpret_cast<const CT*>(d.Lc[1]);
     uint64_t* __restrict__ flags = d.flags;
     uint16_t* __restrict__ wpre = d.wpre;
     int32_t* __restrict__ tilecnt = d.tilecnt;
@@ -417,7 +417,7 @@ __global__ void __launch_bounds__(kBlock) kb_flags(BlockMap bm, const SegDesc* _
     }
     const uint64_t* __restrict__ Lz = cur ? Lz1 : Lz0;
     const uint64_t* __restrict__ Lx = cur ? Lx1 : Lx0;
-    const double* __restrict__ Lc = cur ? Lc1 : Lc0;
+    const CT* __restrict__ Lc = cur ? Lc1 : Lc0;
     const int64_t ntiles = (n + kTile - 1) / kTile;
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     FilterReg floc;  // the local filter, when it can drop children
@@ -438,13 +438,13 @@ __global__ void __launch_bounds__(kBlock) kb_flags(BlockMap bm, const SegDesc* _
         // before the wave's first term (the later rounds take theirs from
         // lane 63 of the round before)
         Key<W> kk[RW];
-        double cc[RW];
+        CT cc[RW];
 #pragma unroll
         for (int r = 0; r < RW; ++r) {
             const int64_t i = t * kTile + (wave * RW + r) * 64 + lane;
 #pragma unroll
             for (int w = 0; w < W; ++w) kk[r].z[w] = kk[r].x[w] = 0;
-            cc[r] = 0.0;
+            cc[r] = coef_zero<CT>();
             if (i < n) {
                 if constexpr (W == 1) {
                     kk[r].z[0] = need_z ? ldg(Lz, i) : 0;
@@ -480,8 +480,8 @@ __global__ void __launch_bounds__(kBlock) kb_flags(BlockMap bm, const SegDesc* _
                 flag = G.anti(k);
                 if (flag && local_active) {
                     Key<W> ch;
-                    const double cn = G.child(k, cc[r], ch);
-                    flag = !should_drop<W>(floc, ch, fabs(cn));
+                    const CT cn = G.template child<CT>(k, cc[r], ch);
+                    flag = !should_drop<W>(floc, ch, coef_abs(cn));
                 }
             }
             const uint64_t mask = __ballot(flag);
@@ -726,7 +726,7 @@ __device__ __forceinline__ int last_head(const uint64_t* hm, uint64_t words, int
 // kb_emit: children written at their rank in key order.
 // Layout inside a tile: wave w owns elements [w T/4, (w+1) T/4), round r
 // covers 64 consecutive elements (coalesced loads), lane = element % 64.
-template <int W, int T, bool TABLE>
+template <int W, int T, bool TABLE, typename CT>
 __global__ void __launch_bounds__(kBlock) kb_emit(BlockMap bm, const SegDesc* __restrict__ D, SegState* __restrict__ S,
                                                   const StepDev* __restrict__ steps, int self_off,
                                                   int64_t step_index) {
@@ -751,7 +751,7 @@ __global__ void __launch_bounds__(kBlock) kb_emit(BlockMap bm, const SegDesc* __
               maxlev = d.maxlev;
     const int64_t n0 = s->n[0], n1 = s->n[1], m_scan = s->m, capC = d.capC;
     const uint64_t *Lz0 = d.Lz[0], *Lz1 = d.Lz[1], *Lx0 = d.Lx[0], *Lx1 = d.Lx[1];
-    const double *Lc0 = d.Lc[0], *Lc1 = d.Lc[1];
+    const CT *Lc0 = reinterpret_cast<const CT*>(d.Lc[0]), *Lc1 = reinterpret_cast<const CT*>(d.Lc[1]);
     const uint64_t* __restrict__ flags = d.flags;
     const uint16_t* __restrict__ wpre = d.wpre;
     const int64_t* __restrict__ tileoff = d.tileoff;
@@ -760,7 +760,7 @@ __global__ void __launch_bounds__(kBlock) kb_emit(BlockMap bm, const SegDesc* __
     const int32_t* __restrict__ tilecnt = d.tilecnt;
     uint64_t* __restrict__ Cz = d.Cz;
     uint64_t* __restrict__ Cx = d.Cx;
-    double* __restrict__ Cc = d.Cc;
+    CT* __restrict__ Cc = reinterpret_cast<CT*>(d.Cc);
     GenReg<W> G;
     G.load(&st->g);
     const uint32_t lev4 = *reinterpret_cast<const uint32_t*>(st->g.lev);  // the first four levels
@@ -793,7 +793,7 @@ __global__ void __launch_bounds__(kBlock) kb_emit(BlockMap bm, const SegDesc* __
     }
     const uint64_t* __restrict__ Lz = cur ? Lz1 : Lz0;
     const uint64_t* __restrict__ Lx = cur ? Lx1 : Lx0;
-    const double* __restrict__ Lc = cur ? Lc1 : Lc0;
+    const CT* __restrict__ Lc = cur ? Lc1 : Lc0;
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     // cross-edge segment counts from kb_flags' summaries (1..kAggLevels
     // levels) or from kb_bounds' table
@@ -818,7 +818,7 @@ __global__ void __launch_bounds__(kBlock) kb_emit(BlockMap bm, const SegDesc* __
         const int64_t ts = t * kTile;
         const int Teff = (int)((n - ts) < kTile ? (n - ts) : kTile);
         Key<W> K[R];
-        double C[R];
+        CT C[R];
         int64_t rank[R];
         bool fl[R];
         const int64_t toff_scan = self_off ? 0 : ldg(tileoff, t);
@@ -838,7 +838,7 @@ __global__ void __launch_bounds__(kBlock) kb_emit(BlockMap bm, const SegDesc* __
             for (int r = 0; r < R; ++r) {
                 const int j = wave * (kTile / 4) + r * kWave + lane;
                 const int64_t i = ts + j;
-                C[r] = 0.0;
+                C[r] = coef_zero<CT>();
                 wd[r] = 0ull;
                 wp[r] = 0;
 #pragma unroll
@@ -891,7 +891,7 @@ __global__ void __launch_bounds__(kBlock) kb_emit(BlockMap bm, const SegDesc* __
                     s_aloc[j] = (int16_t)aloc;
                     s_hdb[j] = (int16_t)(i == 0 ? kHdbNone : hdb<W>(kp, K[r]));
                     rank[r] = aloc;
-                    if (!fl[r]) C[r] = 0.0;
+                    if (!fl[r]) C[r] = coef_zero<CT>();
                 }
             }
         } else {
@@ -901,7 +901,7 @@ __global__ void __launch_bounds__(kBlock) kb_emit(BlockMap bm, const SegDesc* __
                 const int64_t i = ts + j;
                 fl[r] = false;
                 rank[r] = 0;
-                C[r] = 0.0;
+                C[r] = coef_zero<CT>();
                 if (j < Teff) {
                     K[r] = load_key<W>(Lz, Lx, i);
                     const uint64_t word = ldg(flags, i >> 6);
@@ -1139,7 +1139,7 @@ __global__ void __launch_bounds__(kBlock) kb_emit(BlockMap bm, const SegDesc* __
                 continue;
             }
             Key<W> ch;
-            const double cn = G.child(K[r], C[r], ch);
+            const CT cn = G.template child<CT>(K[r], C[r], ch);
             store_key<W>(Cz, Cx, rank[r], ch);
             stg(Cc, rank[r], cn);
         }
@@ -1154,19 +1154,19 @@ __global__ void __launch_bounds__(kBlock) kb_emit(BlockMap bm, const SegDesc* __
 // kb_precheck flagged.  A tile only ever waits in the look-back on tiles of
 // workgroups that have started: with a workgroup per tile, tile = local
 // workgroup index; with fewer workgroups, tiles are claimed by ticket.
-template <int W, int TM>
+template <int W, int TM, typename CT>
 __global__ void __launch_bounds__(kBlock) kb_merge(BlockMap bm, const SegDesc* __restrict__ D, SegState* __restrict__ S,
                                                    const StepDev* __restrict__ steps, int mode,
                                                    int64_t step_index, const uint32_t* __restrict__ d_epoch,
                                                    uint32_t ep_off) {
     constexpr int E = TM / kBlock;
     __shared__ Key<W> s_key[TM];
-    __shared__ double s_c[TM];
+    __shared__ CT s_c[TM];
     __shared__ int64_t s_tile;
     __shared__ int64_t s_base;
     __shared__ int64_t s_split[2];
     __shared__ Key<W> s_edge[2];   // C[j1] (first child after the tile), L[i0 - 1] (last live term before it)
-    __shared__ double s_edge_c;    // c of C[j1]
+    __shared__ CT s_edge_c;    // c of C[j1]
     __shared__ int s_wsum[4];
     int b;
     int64_t lb, nb;
@@ -1180,10 +1180,10 @@ __global__ void __launch_bounds__(kBlock) kb_merge(BlockMap bm, const SegDesc* _
     const uint32_t ep0 = *d_epoch;
     const int64_t nstep = s->nstep, m0 = s->m, capOut = d.cap;
     uint64_t *Lz0 = d.Lz[0], *Lz1 = d.Lz[1], *Lx0 = d.Lx[0], *Lx1 = d.Lx[1];
-    double *Lc0 = d.Lc[0], *Lc1 = d.Lc[1];
+    CT *Lc0 = reinterpret_cast<CT*>(d.Lc[0]), *Lc1 = reinterpret_cast<CT*>(d.Lc[1]);
     const uint64_t* __restrict__ Cz = d.Cz;
     const uint64_t* __restrict__ Cx = d.Cx;
-    const double* __restrict__ Cc = d.Cc;
+    const CT* __restrict__ Cc = reinterpret_cast<const CT*>(d.Cc);
     uint64_t* __restrict__ status = d.status;
     FilterReg fs;
     fs.load(d.fs);
@@ -1207,10 +1207,10 @@ __global__ void __launch_bounds__(kBlock) kb_merge(BlockMap bm, const SegDesc* _
     const int64_t ntm = (tot + TM - 1) / TM;
     const uint64_t* __restrict__ Lz = cur ? Lz1 : Lz0;
     const uint64_t* __restrict__ Lx = cur ? Lx1 : Lx0;
-    const double* __restrict__ Lc = cur ? Lc1 : Lc0;
+    const CT* __restrict__ Lc = cur ? Lc1 : Lc0;
     uint64_t* __restrict__ Oz = cur ? Lz0 : Lz1;
     uint64_t* __restrict__ Ox = cur ? Lx0 : Lx1;
-    double* __restrict__ Oc = cur ? Lc0 : Lc1;
+    CT* __restrict__ Oc = cur ? Lc0 : Lc1;
     const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
     if (ntm == 0) {  // nothing to merge: the output is empty
         if (lb == 0 && tid == 0) {
@@ -1255,13 +1255,13 @@ __global__ void __launch_bounds__(kBlock) kb_merge(BlockMap bm, const SegDesc* _
         {
             // every load of the tile first (one memory round trip), then LDS
             Key<W> lk[E];
-            double lcv[E];
+            CT lcv[E];
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const int k = tid + e * kBlock;
 #pragma unroll
                 for (int w = 0; w < W; ++w) lk[e].z[w] = lk[e].x[w] = 0;
-                lcv[e] = 0.0;
+                lcv[e] = coef_zero<CT>();
                 if constexpr (TM >= 2048) {  // large vectors: read once, past the caches
                     if (k < nL) {
                         lk[e] = load_key_nt<W>(Lz, Lx, i0 + k);
@@ -1298,8 +1298,8 @@ __global__ void __launch_bounds__(kBlock) kb_merge(BlockMap bm, const SegDesc* _
         __syncthreads();
         const Key<W>* sL = s_key;
         const Key<W>* sC = s_key + nL;
-        const double* sLc = s_c;
-        const double* sCc = s_c + nL;
+        const CT* sLc = s_c;
+        const CT* sCc = s_c + nL;
         const int dk = tid * E < nL + nC ? tid * E : nL + nC;
         int lo = dk - nC > 0 ? dk - nC : 0;
         int hi = dk < nL ? dk : nL;
@@ -1312,7 +1312,7 @@ __global__ void __launch_bounds__(kBlock) kb_merge(BlockMap bm, const SegDesc* _
         // outputs in registers indexed by the unrolled position (a run-time
         // index would place the arrays in scratch memory)
         Key<W> outK[E];
-        double outC[E];
+        CT outC[E];
         bool outE[E];
         const int stop = dk + E < nL + nC ? dk + E : nL + nC;
 #pragma unroll
@@ -1320,11 +1320,11 @@ __global__ void __launch_bounds__(kBlock) kb_merge(BlockMap bm, const SegDesc* _
             outE[e] = false;
 #pragma unroll
             for (int w = 0; w < W; ++w) outK[e].z[w] = outK[e].x[w] = 0;
-            outC[e] = 0.0;
+            outC[e] = coef_zero<CT>();
             if (dk + e >= stop) continue;
             const bool takeL = ia < nL && (jb >= nC || key_le<W>(sL[ia], sC[jb]));
             Key<W> K;
-            double acc;
+            CT acc;
             bool emit = true;
             if (takeL) {
                 K = sL[ia];
@@ -1350,7 +1350,7 @@ __global__ void __launch_bounds__(kBlock) kb_merge(BlockMap bm, const SegDesc* _
             }
             outK[e] = K;
             outC[e] = acc;
-            outE[e] = emit && !should_drop<W>(fs, K, fabs(acc));
+            outE[e] = emit && !should_drop<W>(fs, K, coef_abs(acc));
         }
         int cnt = 0;
 #pragma unroll
@@ -1523,7 +1523,7 @@ __global__ void kb_resolve(SegState* __restrict__ S, int B) {
 
 // ---------------------------------------------------------------------------
 // Channels (channels.jl:203-265): c *= table[popcount] in place.
-template <int W>
+template <int W, typename CT>
 __global__ void __launch_bounds__(kBlock) kb_channel(const SegDesc* __restrict__ D, const SegState* __restrict__ S,
                                                      const ChanDev* __restrict__ chans, const int32_t* __restrict__ en,
                                                      BlockMap bm) {
@@ -1540,7 +1540,7 @@ __global__ void __launch_bounds__(kBlock) kb_channel(const SegDesc* __restrict__
     const SegDesc& d = D[b];
     const uint64_t* __restrict__ Lz = d.Lz[cur];
     const uint64_t* __restrict__ Lx = d.Lx[cur];
-    double* __restrict__ Lc = d.Lc[cur];
+    CT* __restrict__ Lc = reinterpret_cast<CT*>(d.Lc[cur]);
     for (int64_t i = lb * kBlock + threadIdx.x; i < n; i += nb * kBlock) {
         const Key<W> k = load_key<W>(Lz, Lx, i);
         Lc[i] = Lc[i] * chan_factor<W>(ch, k);
@@ -1551,10 +1551,12 @@ __global__ void __launch_bounds__(kBlock) kb_channel(const SegDesc* __restrict__
 // <k|O|k> (spv_kernels.jl:441-452) with a double-double accumulator so the
 // parallel sum is accurate to ~2^-106 of sum |c| before its one rounding.
 // Partials per block, reduced per vector in block order (deterministic).
-template <int W>
+// ComplexF64 coefficients: one launch per component (comp 0 real, 1 imaginary;
+// the sum of (±1) c is componentwise).
+template <int W, typename CT>
 __global__ void __launch_bounds__(kBlock) kb_expect(const SegDesc* __restrict__ D, const SegState* __restrict__ S,
                                                     const uint64_t* __restrict__ kets, BlockMap bm,
-                                                    double2* __restrict__ partial) {
+                                                    double2* __restrict__ partial, int comp) {
     __shared__ double s_h[kBlock], s_l[kBlock];
     int b;
     int64_t lb, nb;
@@ -1565,7 +1567,7 @@ __global__ void __launch_bounds__(kBlock) kb_expect(const SegDesc* __restrict__ 
     const SegDesc& d = D[b];
     const uint64_t* __restrict__ Lz = d.Lz[cur];
     const uint64_t* __restrict__ Lx = d.Lx[cur];
-    const double* __restrict__ Lc = d.Lc[cur];
+    const CT* __restrict__ Lc = reinterpret_cast<const CT*>(d.Lc[cur]);
     uint64_t kw[W];
 #pragma unroll
     for (int w = 0; w < W; ++w) kw[w] = kets[(int64_t)b * W + w];
@@ -1580,7 +1582,9 @@ __global__ void __launch_bounds__(kBlock) kb_expect(const SegDesc* __restrict__ 
             par += popc64(k.z[w] & kw[w]);
         }
         if (diag) {
-            const double v = (par & 1) ? -Lc[i] : Lc[i];
+            const CT c = Lc[i];
+            const double cv = comp ? coef_im(c) : coef_re(c);
+            const double v = (par & 1) ? -cv : cv;
             dd_add(h, l, v, 0.0);
         }
     }
@@ -1600,7 +1604,7 @@ __global__ void __launch_bounds__(kBlock) kb_expect(const SegDesc* __restrict__ 
 }
 
 __global__ void __launch_bounds__(kBlock) kb_expect_final(const double2* __restrict__ partial, BlockMap bm,
-                                                          double* __restrict__ out) {
+                                                          double* __restrict__ out, int stride, int comp) {
     __shared__ double s_h[kBlock], s_l[kBlock];
     const int b = blockIdx.x;
     const int64_t p0 = bm.pre[b], p1 = bm.pre[b + 1];
@@ -1618,7 +1622,7 @@ __global__ void __launch_bounds__(kBlock) kb_expect_final(const double2* __restr
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[b] = s_h[0] + s_l[0];
+    if (threadIdx.x == 0) out[(int64_t)b * stride + comp] = s_h[0] + s_l[0];
 }
 
 // check_spv key order (type_SparsePauliVector.jl:273-276): bad[b] = 1 if any

@@ -28,23 +28,24 @@ __device__ __forceinline__ int64_t find_key(const uint64_t* __restrict__ Z, cons
     return -1;
 }
 
-template <int W>
+template <int W, typename CT>
 __global__ void __launch_bounds__(kBlock) ko_lookup(const uint64_t* __restrict__ Z, const uint64_t* __restrict__ X,
-                                                    const double* __restrict__ C, int64_t n,
+                                                    const CT* __restrict__ C, int64_t n,
                                                     const uint64_t* __restrict__ Qz, const uint64_t* __restrict__ Qx,
-                                                    int64_t count, double* __restrict__ out,
+                                                    int64_t count, CT* __restrict__ out,
                                                     int32_t* __restrict__ found, int64_t* __restrict__ idx) {
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < count; i += (int64_t)gridDim.x * kBlock) {
         const int64_t j = find_key<W>(Z, X, n, load_key<W>(Qz, Qx, i));
         found[i] = j >= 0 ? 1 : 0;
-        out[i] = j >= 0 ? C[j] : 0.0;
+        out[i] = j >= 0 ? C[j] : coef_zero<CT>();
         if (idx) idx[i] = j;
     }
 }
 
 // setindex! on present keys: C[idx[i]] = v[i]
-__global__ void __launch_bounds__(kBlock) ko_scatter(double* __restrict__ C, const int64_t* __restrict__ idx,
-                                                     const double* __restrict__ v, int64_t count) {
+template <typename CT>
+__global__ void __launch_bounds__(kBlock) ko_scatter(CT* __restrict__ C, const int64_t* __restrict__ idx,
+                                                     const CT* __restrict__ v, int64_t count) {
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < count; i += (int64_t)gridDim.x * kBlock)
         C[idx[i]] = v[i];
 }
@@ -74,43 +75,50 @@ __device__ __forceinline__ void block_dd_reduce(double h, double l, double2* __r
 }
 
 // Σ over the terms of A of:
-//   mode 0 (inner product)   c_a * c_b              for keys also in B
-//   mode 1 (difference, A)   (c_a - c_b)^2 if in B, else c_a^2
-//   mode 2 (difference, B)   c_a^2                  for keys not in B
-template <int W>
+//   mode 0 (inner product)   conj(c_a) c_b           for keys also in B
+//                            (component comp: 0 real, 1 imaginary)
+//   mode 1 (difference, A)   abs2(c_a - c_b) if in B, else abs2(c_a)
+//   mode 2 (difference, B)   abs2(c_a)               for keys not in B
+// conj(a) b is Julia's Complex * Complex of (a.re, -a.im) and b.
+template <int W, typename CT>
 __global__ void __launch_bounds__(kBlock) ko_join(const uint64_t* __restrict__ Az, const uint64_t* __restrict__ Ax,
-                                                  const double* __restrict__ Ac, int64_t na,
+                                                  const CT* __restrict__ Ac, int64_t na,
                                                   const uint64_t* __restrict__ Bz, const uint64_t* __restrict__ Bx,
-                                                  const double* __restrict__ Bc, int64_t nb, int mode,
+                                                  const CT* __restrict__ Bc, int64_t nb, int mode, int comp,
                                                   double2* __restrict__ partial) {
     double h = 0.0, l = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < na; i += (int64_t)gridDim.x * kBlock) {
         const int64_t j = find_key<W>(Bz, Bx, nb, load_key<W>(Az, Ax, i));
-        const double a = Ac[i];
+        const CT a = Ac[i];
         double v = 0.0;
         if (mode == 0) {
-            if (j >= 0) v = a * Bc[j];
+            if (j >= 0) {
+                const CT b = Bc[j];
+                const double ar = coef_re(a), ai = -coef_im(a), br = coef_re(b), bi = coef_im(b);
+                if constexpr (CoefTraits<CT>::kComplex) v = comp ? ar * bi + ai * br : ar * br - ai * bi;
+                else v = ar * br;
+            }
         } else if (mode == 1) {
-            const double dlt = j >= 0 ? a - Bc[j] : a;
-            v = dlt * dlt;
+            v = coef_abs2(j >= 0 ? a - Bc[j] : a);
         } else {
-            if (j < 0) v = a * a;
+            if (j < 0) v = coef_abs2(a);
         }
         dd_add(h, l, v, 0.0);
     }
     block_dd_reduce(h, l, partial);
 }
 
-// kind 0: Σ c^2, 1: Σ |c|, 2: max |c|, 3: Σ |c|^p
-__global__ void __launch_bounds__(kBlock) ko_norm(const double* __restrict__ C, int64_t n, int kind, double p,
+// kind 0: Σ abs2(c), 1: Σ |c|, 2: max |c|, 3: Σ |c|^p
+template <typename CT>
+__global__ void __launch_bounds__(kBlock) ko_norm(const CT* __restrict__ C, int64_t n, int kind, double p,
                                                   double2* __restrict__ partial) {
     double h = 0.0, l = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-        const double a = fabs(C[i]);
+        const double a = kind == 0 ? 0.0 : coef_abs(C[i]);
         if (kind == 2) {
             h = a > h ? a : h;
         } else {
-            const double v = kind == 0 ? a * a : (kind == 1 ? a : pow(a, p));
+            const double v = kind == 0 ? coef_abs2(C[i]) : (kind == 1 ? a : pow(a, p));
             dd_add(h, l, v, 0.0);
         }
     }

@@ -108,10 +108,10 @@ __global__ void __launch_bounds__(kBlock) kx_scan_add(int64_t* __restrict__ out,
 // of children kept by the local filter.  Pass 2 (count=0): given the block
 // offsets, cos-scales anticommuting terms in place and writes the children
 // at appends[an + offset].
-template <int W>
+template <int W, typename CT>
 __global__ void __launch_bounds__(kBlock) kw_rotate(uint64_t* __restrict__ Lz, uint64_t* __restrict__ Lx,
-                                                    double* __restrict__ Lc, int64_t n, uint64_t* __restrict__ Az,
-                                                    uint64_t* __restrict__ Ax, double* __restrict__ Ac, int64_t an,
+                                                    CT* __restrict__ Lc, int64_t n, uint64_t* __restrict__ Az,
+                                                    uint64_t* __restrict__ Ax, CT* __restrict__ Ac, int64_t an,
                                                     const GenDev* __restrict__ g, const FilterDev* __restrict__ floc,
                                                     int count_only, int64_t* __restrict__ bcount,
                                                     const int64_t* __restrict__ boff, int64_t cap_app) {
@@ -127,17 +127,17 @@ __global__ void __launch_bounds__(kBlock) kw_rotate(uint64_t* __restrict__ Lz, u
         const int64_t i = base + (int64_t)r * kBlock + tid;
         bool keep = false;
         Key<W> k, ch;
-        double cn = 0.0;
+        CT cn = coef_zero<CT>();
         if (i < total) {
             const bool live = i < n;
             const int64_t q = live ? i : i - n;
             k = live ? load_key<W>(Lz, Lx, q) : load_key<W>(Az, Ax, q);
             if (G.anti(k)) {
-                double* cp = live ? Lc + q : Ac + q;
-                const double c = *cp;
+                CT* cp = live ? Lc + q : Ac + q;
+                const CT c = *cp;
                 cn = G.child(k, c, ch);
                 if (!count_only) *cp = c * G.cosv;
-                keep = !should_drop<W>(floc, ch, fabs(cn));
+                keep = !should_drop<W>(floc, ch, coef_abs(cn));
             }
         }
         // ordered offsets inside the block: waves in order, lanes in order
@@ -277,12 +277,12 @@ __global__ void __launch_bounds__(kBlock) ks_hist(const uint64_t* __restrict__ Z
 
 // Stable scatter: within a block, elements keep their order per digit
 // (waves in order, lanes in order, rounds in order).
-template <int W>
+template <int W, typename CT>
 __global__ void __launch_bounds__(kBlock) ks_scatter(const uint64_t* __restrict__ Z, const uint64_t* __restrict__ X,
-                                                     const double* __restrict__ C, int64_t m, int dd,
+                                                     const CT* __restrict__ C, int64_t m, int dd,
                                                      const int64_t* __restrict__ off, int64_t nblocks,
                                                      uint64_t* __restrict__ OZ, uint64_t* __restrict__ OX,
-                                                     double* __restrict__ OC) {
+                                                     CT* __restrict__ OC) {
     __shared__ int64_t s_base[kRadix];
     __shared__ int s_cnt[4][kRadix];
     const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
@@ -293,7 +293,7 @@ __global__ void __launch_bounds__(kBlock) ks_scatter(const uint64_t* __restrict_
         const int64_t i = base + (int64_t)r * kBlock + tid;
         const bool valid = i < m;
         Key<W> k;
-        double c = 0.0;
+        CT c = coef_zero<CT>();
         int dg = -1;
         if (valid) {
             k = load_key<W>(Z, X, i);
@@ -334,19 +334,19 @@ __global__ void __launch_bounds__(kBlock) ks_scatter(const uint64_t* __restrict_
 // keys) -> O.  A run's value is L's coefficient (if present) plus the run's
 // appends left to right; the thread that holds the run's first element sums
 // it (reading past its range when needed).
-template <int W, int TM>
+template <int W, int TM, typename CT>
 __global__ void __launch_bounds__(kBlock) kw_merge(const uint64_t* __restrict__ Lz, const uint64_t* __restrict__ Lx,
-                                                   const double* __restrict__ Lc, int64_t n,
+                                                   const CT* __restrict__ Lc, int64_t n,
                                                    const uint64_t* __restrict__ Cz, const uint64_t* __restrict__ Cx,
-                                                   const double* __restrict__ Cc, int64_t m,
+                                                   const CT* __restrict__ Cc, int64_t m,
                                                    const FilterDev* __restrict__ f, uint64_t* __restrict__ Oz,
-                                                   uint64_t* __restrict__ Ox, double* __restrict__ Oc, int64_t capOut,
+                                                   uint64_t* __restrict__ Ox, CT* __restrict__ Oc, int64_t capOut,
                                                    uint64_t* __restrict__ status, uint32_t* __restrict__ ticket,
                                                    uint32_t epoch, int64_t* __restrict__ d_nout,
                                                    int32_t* __restrict__ d_err) {
     constexpr int E = TM / kBlock;
     __shared__ Key<W> s_key[TM];
-    __shared__ double s_c[TM];
+    __shared__ CT s_c[TM];
     __shared__ int64_t s_tile;
     __shared__ int64_t s_base;
     __shared__ int64_t s_split[2];
@@ -381,13 +381,13 @@ __global__ void __launch_bounds__(kBlock) kw_merge(const uint64_t* __restrict__ 
         {
             // every load of the tile first (one memory round trip), then LDS
             Key<W> lk[E];
-            double lcv[E];
+            CT lcv[E];
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const int k = tid + e * kBlock;
 #pragma unroll
                 for (int w = 0; w < W; ++w) lk[e].z[w] = lk[e].x[w] = 0;
-                lcv[e] = 0.0;
+                lcv[e] = coef_zero<CT>();
                 if (k < nL) {
                     lk[e] = load_key<W>(Lz, Lx, i0 + k);
                     lcv[e] = ldg(Lc, i0 + k);
@@ -408,8 +408,8 @@ __global__ void __launch_bounds__(kBlock) kw_merge(const uint64_t* __restrict__ 
         __syncthreads();
         const Key<W>* sL = s_key;
         const Key<W>* sC = s_key + nL;
-        const double* sLc = s_c;
-        const double* sCc = s_c + nL;
+        const CT* sLc = s_c;
+        const CT* sCc = s_c + nL;
         const int dk = tid * E < nL + nC ? tid * E : nL + nC;
         int lo = dk - nC > 0 ? dk - nC : 0;
         int hi = dk < nL ? dk : nL;
@@ -422,7 +422,7 @@ __global__ void __launch_bounds__(kBlock) kw_merge(const uint64_t* __restrict__ 
         // outputs in registers indexed by the unrolled position (a run-time
         // index would place the arrays in scratch memory)
         Key<W> outK[E];
-        double outC[E];
+        CT outC[E];
         bool outE[E];
         const int stop = dk + E < nL + nC ? dk + E : nL + nC;
 #pragma unroll
@@ -430,11 +430,11 @@ __global__ void __launch_bounds__(kBlock) kw_merge(const uint64_t* __restrict__ 
             outE[e] = false;
 #pragma unroll
             for (int w = 0; w < W; ++w) outK[e].z[w] = outK[e].x[w] = 0;
-            outC[e] = 0.0;
+            outC[e] = coef_zero<CT>();
             if (dk + e >= stop) continue;
             const bool takeL = ia < nL && (jb >= nC || key_le<W>(sL[ia], sC[jb]));
             Key<W> K;
-            double acc;
+            CT acc;
             bool head = true;
             int64_t jg;  // global index of the next append after this element
             if (takeL) {
@@ -473,7 +473,7 @@ __global__ void __launch_bounds__(kBlock) kw_merge(const uint64_t* __restrict__ 
             }
             outK[e] = K;
             outC[e] = acc;
-            outE[e] = !should_drop<W>(f, K, fabs(acc));
+            outE[e] = !should_drop<W>(f, K, coef_abs(acc));
         }
         int cnt = 0;
 #pragma unroll
@@ -566,13 +566,14 @@ __global__ void __launch_bounds__(kBlock) kw_merge(const uint64_t* __restrict__ 
 // ---------------------------------------------------------------------------
 // Radix select over |c| bit patterns (non-negative doubles order as uint64):
 // histogram of one byte among the values whose higher bytes equal `prefix`.
-__global__ void __launch_bounds__(kBlock) kq_hist(const double* __restrict__ C, int64_t n, int shift, uint64_t prefix,
+template <typename CT>
+__global__ void __launch_bounds__(kBlock) kq_hist(const CT* __restrict__ C, int64_t n, int shift, uint64_t prefix,
                                                   uint64_t pmask, unsigned long long* __restrict__ hist) {
     __shared__ int s_h[kRadix];
     for (int i = threadIdx.x; i < kRadix; i += kBlock) s_h[i] = 0;
     __syncthreads();
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const uint64_t u = (uint64_t)__double_as_longlong(fabs(C[i]));
+        const uint64_t u = (uint64_t)__double_as_longlong(coef_abs(C[i]));
         if ((u & pmask) == prefix) atomicAdd(&s_h[(int)((u >> shift) & 0xffull)], 1);
     }
     __syncthreads();
@@ -609,16 +610,16 @@ struct SampleSel {
     const uint8_t* mark;  // mode 3: terms to delete
 };
 
-template <int W>
+template <int W, typename CT>
 __device__ __forceinline__ bool compact_keep(int mode, const FilterDev* f, double eps, uint64_t seed, int64_t i,
-                                             const Key<W>& k, double& c, const SampleSel& ss) {
-    if (mode == 0) return !should_drop<W>(f, k, fabs(c));
+                                             const Key<W>& k, CT& c, const SampleSel& ss) {
+    if (mode == 0) return !should_drop<W>(f, k, coef_abs(c));
     if (mode == 2) {
         const uint64_t e = ss.key[i];
         return e > ss.eT || (e == ss.eT && ss.n - 1 - i >= ss.jT);
     }
     if (mode == 3) return ss.mark[i] == 0;
-    const double ac = fabs(c);
+    const double ac = coef_abs(c);
     if (ac < eps) {
         if (!(counter_uniform(seed, (uint64_t)i) < ac / eps)) return false;
         c = eps * (c / ac);
@@ -626,13 +627,13 @@ __device__ __forceinline__ bool compact_keep(int mode, const FilterDev* f, doubl
     return true;
 }
 
-template <int W>
+template <int W, typename CT>
 __global__ void __launch_bounds__(kBlock) kw_compact(const uint64_t* __restrict__ Iz, const uint64_t* __restrict__ Ix,
-                                                     const double* __restrict__ Ic, int64_t n,
+                                                     const CT* __restrict__ Ic, int64_t n,
                                                      const FilterDev* __restrict__ f, int mode, double eps,
                                                      uint64_t seed, int count_only, int64_t* __restrict__ bcount,
                                                      const int64_t* __restrict__ boff, uint64_t* __restrict__ Oz,
-                                                     uint64_t* __restrict__ Ox, double* __restrict__ Oc, SampleSel ss) {
+                                                     uint64_t* __restrict__ Ox, CT* __restrict__ Oc, SampleSel ss) {
     __shared__ int s_w[4];
     const int64_t base = (int64_t)blockIdx.x * kRB;
     const int tid = threadIdx.x, lane = tid % kWave, wave = tid / kWave;
@@ -642,11 +643,11 @@ __global__ void __launch_bounds__(kBlock) kw_compact(const uint64_t* __restrict_
         const int64_t i = base + (int64_t)r * kBlock + tid;
         bool keep = false;
         Key<W> k;
-        double c = 0.0;
+        CT c = coef_zero<CT>();
         if (i < n) {
             k = load_key<W>(Iz, Ix, i);
             c = Ic[i];
-            keep = compact_keep<W>(mode, f, eps, seed, i, k, c, ss);
+            keep = compact_keep<W, CT>(mode, f, eps, seed, i, k, c, ss);
         }
         const uint64_t mask = __ballot(keep);
         const int before = __popcll(mask & (lane ? (~0ull >> (64 - lane)) : 0ull));
@@ -672,10 +673,11 @@ __global__ void __launch_bounds__(kBlock) kw_compact(const uint64_t* __restrict_
 // StochasticSamplingTruncation keys (spv_evolve.jl:69-99): term i gets
 // log(u_i) / |c_i|^2, which orders the terms as u_i^(1/|c_i|^2) does (zero
 // weight: -inf), encoded as an unsigned integer with the same order.
-__global__ void __launch_bounds__(kBlock) kw_sample_keys(const double* __restrict__ C, int64_t n, uint64_t seed,
+template <typename CT>
+__global__ void __launch_bounds__(kBlock) kw_sample_keys(const CT* __restrict__ C, int64_t n, uint64_t seed,
                                                          uint64_t* __restrict__ key) {
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-        const double w = C[i] * C[i];
+        const double w = coef_abs2(C[i]);
         const double u = counter_uniform(seed, (uint64_t)i);
         const double k = w > 0.0 ? log(u) / w : -INFINITY;
         const uint64_t b = (uint64_t)__double_as_longlong(k);
@@ -710,14 +712,28 @@ __global__ void __launch_bounds__(kBlock) kw_scale(double* __restrict__ C, int64
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) C[i] *= f;
 }
 
+// mul!(v, a) with a ComplexF64 scalar on ComplexF64 coefficients
+// (spv_ops.jl:12-17): Julia's Complex * Complex, (c.re a.re - c.im a.im,
+// c.re a.im + c.im a.re).  A real scalar scales componentwise (kw_scale over
+// 2n values), as Complex * Real does.
+__global__ void __launch_bounds__(kBlock) kw_scale_c(pp_cplx* __restrict__ C, int64_t n, double ar, double ai) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        const pp_cplx c = C[i];
+        pp_cplx r;
+        r.x = c.x * ar - c.y * ai;
+        r.y = c.x * ai + c.y * ar;
+        C[i] = r;
+    }
+}
+
 // <ket|O|ket> partial sums over [A (na) ; B (nb)] (live then pending appends,
 // spv_kernels.jl:441-452), double-double per block.
-template <int W>
+template <int W, typename CT>
 __global__ void __launch_bounds__(kBlock) kw_expect(const uint64_t* __restrict__ Az, const uint64_t* __restrict__ Ax,
-                                                    const double* __restrict__ Ac, int64_t na,
+                                                    const CT* __restrict__ Ac, int64_t na,
                                                     const uint64_t* __restrict__ Bz, const uint64_t* __restrict__ Bx,
-                                                    const double* __restrict__ Bc, int64_t nb, Key<W> ket,
-                                                    double2* __restrict__ partial) {
+                                                    const CT* __restrict__ Bc, int64_t nb, Key<W> ket,
+                                                    double2* __restrict__ partial, int comp) {
     __shared__ double s_h[kBlock], s_l[kBlock];
     double h = 0.0, l = 0.0;
     const int64_t tot = na + nb;
@@ -733,7 +749,8 @@ __global__ void __launch_bounds__(kBlock) kw_expect(const uint64_t* __restrict__
             par += popc64(k.z[w] & ket.z[w]);
         }
         if (diag) {
-            const double c = inA ? Ac[q] : Bc[q];
+            const CT cv = inA ? Ac[q] : Bc[q];
+            const double c = comp ? coef_im(cv) : coef_re(cv);
             dd_add(h, l, (par & 1) ? -c : c, 0.0);
         }
     }
@@ -755,10 +772,10 @@ __global__ void __launch_bounds__(kBlock) kw_expect(const uint64_t* __restrict__
 // variance(O, ψ) of the live set (statistics.jl:6-22), step 1: the terms
 // with their words swapped (x first), so a sort by key orders them by x and,
 // within one x, by z (the operator's iteration order).
-template <int W>
+template <int W, typename CT>
 __global__ void __launch_bounds__(kBlock) kv_swap(const uint64_t* __restrict__ Lz, const uint64_t* __restrict__ Lx,
-                                                  const double* __restrict__ Lc, int64_t n, uint64_t* __restrict__ Az,
-                                                  uint64_t* __restrict__ Ax, double* __restrict__ Ac) {
+                                                  const CT* __restrict__ Lc, int64_t n, uint64_t* __restrict__ Az,
+                                                  uint64_t* __restrict__ Ax, CT* __restrict__ Ac) {
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
 #pragma unroll
         for (int w = 0; w < W; ++w) {
@@ -800,9 +817,9 @@ __device__ __forceinline__ bool var_same_x(const uint64_t* __restrict__ Sz, int6
 }
 
 // Term j's phased coefficient; Sz holds the x words, Sx the z words.
-template <int W>
+template <int W, typename CT>
 __device__ __forceinline__ VarSeg var_term(const uint64_t* __restrict__ Sz, const uint64_t* __restrict__ Sx,
-                                           const double* __restrict__ Sc, int64_t j, const Key<W>& ket) {
+                                           const CT* __restrict__ Sc, int64_t j, const Key<W>& ket) {
     int sgn = 0, zx = 0;
 #pragma unroll
     for (int w = 0; w < W; ++w) {
@@ -811,12 +828,14 @@ __device__ __forceinline__ VarSeg var_term(const uint64_t* __restrict__ Sz, cons
         zx += popc64(z & x);
     }
     const int ph = ((4 - zx % 4) % 4 + 2 * (sgn & 1)) % 4;  // PHASE_TBL = [1, i, -1, -i]
-    const double c = Sc[j];
+    const CT c = Sc[j];
+    const double cr = coef_re(c), ci = coef_im(c);
     const int head = j == 0 || !var_same_x<W>(Sz, j, j - 1);
-    if (ph == 0) return VarSeg{c, 0.0, head};
-    if (ph == 1) return VarSeg{0.0, c, head};
-    if (ph == 2) return VarSeg{-c, 0.0, head};
-    return VarSeg{0.0, -c, head};
+    // phase * c: (1) c, (i) (-im, re), (-1) (-re, -im), (-i) (im, -re)
+    if (ph == 0) return VarSeg{cr, ci, head};
+    if (ph == 1) return VarSeg{-ci, cr, head};
+    if (ph == 2) return VarSeg{-cr, -ci, head};
+    return VarSeg{ci, -cr, head};
 }
 
 // Exclusive scan of the threads' segments in thread order, seeded by `seed`;
@@ -842,9 +861,9 @@ __device__ __forceinline__ VarSeg var_block_scan(VarSeg mine, const VarSeg& seed
     return t > 0 ? var_join(seed, prev) : seed;
 }
 
-template <int W, bool kFinal>
+template <int W, bool kFinal, typename CT>
 __global__ void __launch_bounds__(kBlock) kv_tiles(const uint64_t* __restrict__ Sz, const uint64_t* __restrict__ Sx,
-                                                   const double* __restrict__ Sc, int64_t n, Key<W> ket,
+                                                   const CT* __restrict__ Sc, int64_t n, Key<W> ket,
                                                    double2* __restrict__ tile_v, int32_t* __restrict__ tile_h,
                                                    const double2* __restrict__ carry, double2* __restrict__ partial,
                                                    double* __restrict__ out) {
@@ -859,7 +878,7 @@ __global__ void __launch_bounds__(kBlock) kv_tiles(const uint64_t* __restrict__ 
         VarSeg mine{0.0, 0.0, 0};
 #pragma unroll
         for (int e = 0; e < kVarPer; ++e) {
-            term[e] = j0 + e < n ? var_term<W>(Sz, Sx, Sc, j0 + e, ket) : VarSeg{0.0, 0.0, 0};
+            term[e] = j0 + e < n ? var_term<W, CT>(Sz, Sx, Sc, j0 + e, ket) : VarSeg{0.0, 0.0, 0};
             mine = var_join(mine, term[e]);
         }
         const VarSeg seed = kFinal ? VarSeg{carry[tile].x, carry[tile].y, 0} : VarSeg{0.0, 0.0, 0};
@@ -882,7 +901,10 @@ __global__ void __launch_bounds__(kBlock) kv_tiles(const uint64_t* __restrict__ 
                 bool zero = true;
 #pragma unroll
                 for (int w = 0; w < W; ++w) zero = zero && Sz[j * W + w] == 0;
-                if (zero) out[1] = run.re;
+                if (zero) {
+                    out[1] = run.re;
+                    out[2] = run.im;
+                }
             }
         }
     }
@@ -920,7 +942,10 @@ __global__ void __launch_bounds__(kBlock) kv_carry(const double2* __restrict__ t
         carry[t] = make_double2(run.re, run.im);
         run = var_join(run, VarSeg{tile_v[t].x, tile_v[t].y, tile_h[t]});
     }
-    if (threadIdx.x == 0) out[1] = 0.0;
+    if (threadIdx.x == 0) {
+        out[1] = 0.0;
+        out[2] = 0.0;
+    }
 }
 
 __global__ void __launch_bounds__(kBlock) kw_expect_final(const double2* __restrict__ partial, int64_t np,

@@ -1,12 +1,14 @@
 # PauliPropAMD — Julia binding of libpauliprop.so (the MI355X SparsePauliVector
 # engine) for PauliOperators.jl v3.1.0.
 #
-# A `DeviceSPV{N,W}` is a device-resident twin of the reference's
-# `SparsePauliVector{N,W,Float64}` (src/type_SparsePauliVector.jl:69-84); the
+# A `DeviceSPV{N,W,T}` is a device-resident twin of the reference's
+# `SparsePauliVector{N,W,T}` for T = Float64 or ComplexF64
+# (src/type_SparsePauliVector.jl:69-84); the
 # methods below keep the reference's names, keyword arguments and exception
 # types and forward to the C ABI declared in include/pauliprop.h.  Keys pass
 # unchanged: a Vector{UInt64} (N <= 64) or Vector{UInt128} (N <= 128) is already
-# W little-endian 64-bit words per term, and `MergeFilter`
+# W little-endian 64-bit words per term, a Vector{Float64} or
+# Vector{ComplexF64} is already the ABI's coefficient layout, and `MergeFilter`
 # (src/spv_kernels.jl:33-42) has the byte layout of `pp_filter`.  Julia
 # computes cos θ, sin θ and the channels' λ^k itself and passes them in, so
 # results match the Julia reference to the bit.
@@ -142,7 +144,9 @@ end
 
 # ---- one device vector ----------------------------------------------------------
 
-mutable struct DeviceSPV{N,W}
+const DeviceCoeff = Union{Float64,ComplexF64}
+
+mutable struct DeviceSPV{N,W,T<:DeviceCoeff}
     h::Ptr{Cvoid}
 end
 
@@ -153,27 +157,39 @@ function _destroy!(d::DeviceSPV)
     return nothing
 end
 
-# SparsePauliVector(N, Float64; capacity) on the device, with the reference's
-# virtual buffer lengths (type_SparsePauliVector.jl:94-104)
-function DeviceSPV{N,W}(live_cap::Integer=16, append_cap::Integer=16) where {N,W}
+# SparsePauliVector(N, T; capacity) on the device, with the reference's
+# virtual buffer lengths (type_SparsePauliVector.jl:94-104,138-141)
+function DeviceSPV{N,W,T}(live_cap::Integer=16, append_cap::Integer=16) where {N,W,T<:DeviceCoeff}
     h = Ref{Ptr{Cvoid}}(C_NULL)
-    check(ccall((:pp_create, LIB), Cint, (Cint, Int64, Int64, Ref{Ptr{Cvoid}}), N, live_cap, append_cap, h))
-    d = DeviceSPV{N,W}(h[])
+    rc = T <: Complex ?
+        ccall((:pp_create_typed, LIB), Cint, (Cint, Int64, Int64, Cint, Ref{Ptr{Cvoid}}),
+              N, live_cap, append_cap, 1, h) :
+        ccall((:pp_create, LIB), Cint, (Cint, Int64, Int64, Ref{Ptr{Cvoid}}), N, live_cap, append_cap, h)
+    check(rc)
+    d = DeviceSPV{N,W,T}(h[])
     finalizer(_destroy!, d)
     return d
 end
+DeviceSPV{N,W}(live_cap::Integer=16, append_cap::Integer=16) where {N,W} = DeviceSPV{N,W,Float64}(live_cap, append_cap)
 
-function DeviceSPV(v::SparsePauliVector{N,W,Float64}) where {N,W}
-    d = DeviceSPV{N,W}(length(v.z), length(v.az))
+function DeviceSPV(v::SparsePauliVector{N,W,T}) where {N,W,T<:DeviceCoeff}
+    d = DeviceSPV{N,W,T}(length(v.z), length(v.az))
     upload!(d, v)
     return d
 end
 
-function upload!(d::DeviceSPV{N,W}, v::SparsePauliVector{N,W,Float64}) where {N,W}
+function upload!(d::DeviceSPV{N,W,T}, v::SparsePauliVector{N,W,T}) where {N,W,T}
     v.an == 0 || throw(ArgumentError("merge the pending appends first (merge_pending!)"))
-    check(ccall((:pp_upload, LIB), Cint, (Ptr{Cvoid}, Ptr{W}, Ptr{W}, Ptr{Float64}, Int64, Cint),
+    check(ccall((:pp_upload, LIB), Cint, (Ptr{Cvoid}, Ptr{W}, Ptr{W}, Ptr{T}, Int64, Cint),
                 d.h, v.z, v.x, v.c, v.n, 1))
     return d
+end
+
+# the coefficient type the library holds for d (pp_coeff_type)
+function coeff_type(d::DeviceSPV)
+    c = Ref{Cint}(0)
+    check(ccall((:pp_coeff_type, LIB), Cint, (Ptr{Cvoid}, Ref{Cint}), d.h, c))
+    return c[] == 1 ? ComplexF64 : Float64
 end
 
 function Base.length(d::DeviceSPV)
@@ -190,11 +206,11 @@ function capacities(d::DeviceSPV)
 end
 
 # PauliSum(d) goes through the host copy: PauliSum(SparsePauliVector(d))
-function PauliOperators.SparsePauliVector(d::DeviceSPV{N,W}) where {N,W}
+function PauliOperators.SparsePauliVector(d::DeviceSPV{N,W,T}) where {N,W,T}
     n = length(d)
-    v = SparsePauliVector(N, Float64; capacity=max(n, 1))
+    v = SparsePauliVector(N, T; capacity=max(n, 1))
     got = Ref{Int64}(0)
-    check(ccall((:pp_download, LIB), Cint, (Ptr{Cvoid}, Ptr{W}, Ptr{W}, Ptr{Float64}, Int64, Ref{Int64}),
+    check(ccall((:pp_download, LIB), Cint, (Ptr{Cvoid}, Ptr{W}, Ptr{W}, Ptr{T}, Int64, Ref{Int64}),
                 d.h, v.z, v.x, v.c, length(v.z), got))
     v.n = Int(got[])
     return v
@@ -344,10 +360,10 @@ end
 
 # ---- observables and reductions --------------------------------------------------
 
-# ⟨ψ|O|ψ⟩ for a computational-basis ket (spv_ops.jl:490-497)
-function expectation_value(d::DeviceSPV{N,W}, ψ::Ket{N}) where {N,W}
-    out = Ref{Float64}(0.0)
-    check(ccall((:pp_expectation_ket, LIB), Cint, (Ptr{Cvoid}, Ref{W}, Ref{Float64}), d.h, _ket_words(W, ψ), out))
+# ⟨ψ|O|ψ⟩ for a computational-basis ket (spv_ops.jl:490-497), of type T
+function expectation_value(d::DeviceSPV{N,W,T}, ψ::Ket{N}) where {N,W,T}
+    out = Ref{T}(zero(T))
+    check(ccall((:pp_expectation_ket, LIB), Cint, (Ptr{Cvoid}, Ref{W}, Ref{T}), d.h, _ket_words(W, ψ), out))
     return out[]
 end
 
@@ -364,9 +380,10 @@ function LinearAlgebra.norm(d::DeviceSPV, p::Real=2)
     return out[]
 end
 
-function inner_product(a::DeviceSPV{N,W}, b::DeviceSPV{N,W}) where {N,W}
-    out = Ref{Float64}(0.0)
-    check(ccall((:pp_inner_product, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ref{Float64}), a.h, b.h, out))
+# Σ over shared keys of conj(c_a) c_b (spv_ops.jl:378-396)
+function inner_product(a::DeviceSPV{N,W,T}, b::DeviceSPV{N,W,T}) where {N,W,T}
+    out = Ref{T}(zero(T))
+    check(ccall((:pp_inner_product, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ref{T}), a.h, b.h, out))
     return out[]
 end
 
@@ -386,14 +403,14 @@ end
 
 # ---- Dict idiom (type_SparsePauliVector.jl:286-358) -------------------------------
 
-function _lookup(sym::Symbol, d::DeviceSPV{N,W}, p::PauliBasis{N}) where {N,W}
+function _lookup(sym::Symbol, d::DeviceSPV{N,W,T}, p::PauliBasis{N}) where {N,W,T}
     z, x = _pack(W, p)
-    c = Ref{Float64}(0.0)
+    c = Ref{T}(zero(T))
     f = Ref{Int32}(0)
     rc = sym === :pp_lookup ?
-        ccall((:pp_lookup, LIB), Cint, (Ptr{Cvoid}, Ref{W}, Ref{W}, Int64, Ref{Float64}, Ref{Int32}),
+        ccall((:pp_lookup, LIB), Cint, (Ptr{Cvoid}, Ref{W}, Ref{W}, Int64, Ref{T}, Ref{Int32}),
               d.h, z, x, 1, c, f) :
-        ccall((:pp_lookup_noalloc, LIB), Cint, (Ptr{Cvoid}, Ref{W}, Ref{W}, Int64, Ref{Float64}, Ref{Int32}),
+        ccall((:pp_lookup_noalloc, LIB), Cint, (Ptr{Cvoid}, Ref{W}, Ref{W}, Int64, Ref{T}, Ref{Int32}),
               d.h, z, x, 1, c, f)
     check(rc)
     return f[] != 0, c[]
@@ -412,10 +429,10 @@ Base.haskey(d::DeviceSPV{N}, p::PauliBasis{N}) where {N} = _lookup(:pp_lookup, d
 lookup_noalloc(d::DeviceSPV{N}, p::PauliBasis{N}, default) where {N} =
     (r = _lookup(:pp_lookup_noalloc, d, p); r[1] ? r[2] : default)
 
-function Base.setindex!(d::DeviceSPV{N,W}, val, p::PauliBasis{N}) where {N,W}
+function Base.setindex!(d::DeviceSPV{N,W,T}, val, p::PauliBasis{N}) where {N,W,T}
     z, x = _pack(W, p)
-    check(ccall((:pp_setindex, LIB), Cint, (Ptr{Cvoid}, Ref{W}, Ref{W}, Ref{Float64}, Int64),
-                d.h, z, x, Float64(val), 1))
+    check(ccall((:pp_setindex, LIB), Cint, (Ptr{Cvoid}, Ref{W}, Ref{W}, Ref{T}, Int64),
+                d.h, z, x, convert(T, val), 1))
     return d
 end
 
@@ -433,6 +450,13 @@ function Base.sum!(a::DeviceSPV{N,W}, b::DeviceSPV{N,W}) where {N,W}
 end
 function LinearAlgebra.mul!(d::DeviceSPV, s::Real)
     check(ccall((:pp_scale, LIB), Cint, (Ptr{Cvoid}, Float64), d.h, Float64(s)))
+    return d
+end
+# a complex scalar: Complex * Complex on ComplexF64 coefficients; a Float64
+# vector takes it only when it is real, as convert(Float64, c * s) requires
+function LinearAlgebra.mul!(d::DeviceSPV{N,W,T}, s::Complex) where {N,W,T}
+    T <: Complex || return LinearAlgebra.mul!(d, convert(Float64, s))
+    check(ccall((:pp_scale_complex, LIB), Cint, (Ptr{Cvoid}, Float64, Float64), d.h, real(s), imag(s)))
     return d
 end
 

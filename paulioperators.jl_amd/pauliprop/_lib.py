@@ -16,9 +16,9 @@ LIB_PATH = os.path.join(PKG_DIR, "_build", "libpauliprop.so")
 
 PP_OK, PP_EARG, PP_ENOMEM, PP_EHIP, PP_EOVERFLOW, PP_EUNSUPPORTED = range(6)
 
-EXPORTS = ("pp_create", "pp_destroy", "pp_upload", "pp_download", "pp_size", "pp_capacities", "pp_evolve",
+EXPORTS = ("pp_create", "pp_create_typed", "pp_coeff_type", "pp_destroy", "pp_upload", "pp_download", "pp_size", "pp_capacities", "pp_evolve",
            "pp_evolve_ex", "pp_truncate", "pp_variance_ket", "pp_evolve_single", "pp_compact", "pp_lookup", "pp_lookup_noalloc", "pp_norm",
-           "pp_inner_product", "pp_diff_norm", "pp_setindex", "pp_delete", "pp_sum", "pp_scale", "pp_scale_by_popcount", "pp_expectation_ket", "pp_check",
+           "pp_inner_product", "pp_diff_norm", "pp_setindex", "pp_delete", "pp_sum", "pp_scale", "pp_scale_complex", "pp_scale_by_popcount", "pp_expectation_ket", "pp_check",
            "pp_algebra",
            "pp_batch_create", "pp_batch_destroy", "pp_batch_words", "pp_batch_upload", "pp_batch_download",
            "pp_batch_sizes", "pp_batch_evolve", "pp_batch_scale_by_popcount", "pp_batch_expectation_ket",
@@ -94,6 +94,8 @@ def lib():
     dp = P(ctypes.c_double)
     i64 = ctypes.c_int64
     L.pp_create.argtypes = [ctypes.c_int, i64, i64, P(vp)]
+    L.pp_create_typed.argtypes = [ctypes.c_int, i64, i64, ctypes.c_int, P(vp)]
+    L.pp_coeff_type.argtypes = [vp, P(ctypes.c_int)]
     L.pp_destroy.argtypes = [vp]
     L.pp_upload.argtypes = [vp, u64p, u64p, dp, i64, ctypes.c_int]
     L.pp_download.argtypes = [vp, u64p, u64p, dp, i64, P(i64)]
@@ -111,6 +113,7 @@ def lib():
     L.pp_setindex.argtypes = [vp, u64p, u64p, dp, i64]
     L.pp_sum.argtypes = [vp, vp]
     L.pp_scale.argtypes = [vp, ctypes.c_double]
+    L.pp_scale_complex.argtypes = [vp, ctypes.c_double, ctypes.c_double]
     L.pp_delete.argtypes = [vp, u64p, u64p, i64]
     L.pp_inner_product.argtypes = [vp, vp, dp]
     L.pp_diff_norm.argtypes = [vp, vp, dp]

@@ -2,7 +2,8 @@
 
 Each method mirrors a reference method (cited) with the same argument meaning
 and error behaviour; Julia's ``f!`` is spelled ``f_``.  Coefficients are
-Float64 (the reference's ``T = Float64`` case, which all workloads use).
+Float64 (``T=float``, what the workloads use) or ComplexF64 (``T=complex``,
+the reference's default element type, type_SparsePauliVector.jl:138,167).
 The live terms stay in device memory between calls.
 """
 from __future__ import annotations
@@ -104,18 +105,25 @@ def _pow_table(lam: float, N: int) -> np.ndarray:
 
 
 class SparsePauliVector:
-    """SparsePauliVector{N,W,Float64} (type_SparsePauliVector.jl:69-84), device-resident."""
+    """SparsePauliVector{N,W,T} (type_SparsePauliVector.jl:69-84), T = Float64
+    or ComplexF64, device-resident."""
 
-    def __init__(self, N: int, terms=None, *, capacity_factor: float = 2.0, append_factor: float = 1.0,
+    def __init__(self, N: int, terms=None, *, T=float, capacity_factor: float = 2.0, append_factor: float = 1.0,
                  min_capacity: int = 16, imag_tol: float = 1e-10, _caps: Optional[Tuple[int, int]] = None):
-        """``SparsePauliVector(O::PauliSum; capacity_factor, append_factor, min_capacity, imag_tol)``
+        """``SparsePauliVector(O::PauliSum; T, capacity_factor, append_factor, min_capacity, imag_tol)``
         (type_SparsePauliVector.jl:167-188).  ``terms`` is a PauliSum/dict
         {PauliBasis: coeff}, a list of (PauliBasis, coeff), or a tuple of word
-        arrays (z, x, c) already in key order."""
+        arrays (z, x, c) already in key order.  ``T`` is ``float`` (Float64) or
+        ``complex`` (ComplexF64)."""
         if capacity_factor < 1:
             raise ArgumentError("capacity_factor must be >= 1")
+        if T not in (float, complex):
+            raise ArgumentError("T must be float (Float64) or complex (ComplexF64)")
         self.N = int(N)
         self.W = word_count(self.N)
+        self.T = T
+        self.complex = T is complex
+        self.dtype = np.complex128 if self.complex else np.float64
         arrays = None
         if isinstance(terms, tuple) and len(terms) == 3 and isinstance(terms[0], np.ndarray):
             arrays = terms
@@ -130,12 +138,12 @@ class SparsePauliVector:
         if _caps is not None:
             live_cap, app_cap = _caps
         h = ctypes.c_void_p()
-        check(_lib.lib().pp_create(self.N, live_cap, app_cap, ctypes.byref(h)))
+        check(_lib.lib().pp_create_typed(self.N, live_cap, app_cap, 1 if self.complex else 0, ctypes.byref(h)))
         self._h = h
         if arrays is None:
             z = np.zeros((max(nterms, 1), self.W), dtype=np.uint64)
             x = np.zeros((max(nterms, 1), self.W), dtype=np.uint64)
-            c = np.zeros(max(nterms, 1), dtype=np.float64)
+            c = np.zeros(max(nterms, 1), dtype=self.dtype)
             for i, (p, coef) in enumerate(items):
                 if isinstance(p, PauliBasis):
                     if p.N != self.N:
@@ -144,15 +152,16 @@ class SparsePauliVector:
                 else:
                     pz, px = p
                 cc = complex(coef)
-                # type_SparsePauliVector.jl:144-149
-                if abs(cc.imag) > imag_tol * max(1.0, abs(cc)):
+                # _engine_coeff (type_SparsePauliVector.jl:143-149)
+                if not self.complex and abs(cc.imag) > imag_tol * max(1.0, abs(cc)):
                     raise ValueError(f"coefficient {coef} has an imaginary part; Float64 storage needs real input")
                 z[i] = _pack(pz, self.W)
                 x[i] = _pack(px, self.W)
-                c[i] = cc.real
+                c[i] = cc if self.complex else cc.real
             arrays = (z[:nterms], x[:nterms], c[:nterms])
         if nterms:
-            z, x, c = (np.ascontiguousarray(a) for a in arrays)
+            z, x = (np.ascontiguousarray(a) for a in arrays[:2])
+            c = np.ascontiguousarray(arrays[2], dtype=self.dtype)
             check(_lib.lib().pp_upload(self._h, u64p(z), u64p(x), dp(c), nterms, 1 if presorted else 0))
 
     def __del__(self):
@@ -189,11 +198,12 @@ class SparsePauliVector:
         return a.value, b.value
 
     def arrays(self) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
-        """Live (z, x, c) in key order: z, x as (n, W) uint64, c as float64."""
+        """Live (z, x, c) in key order: z, x as (n, W) uint64, c as float64
+        (complex128 for ComplexF64)."""
         n = len(self)
         z = np.zeros((max(n, 1), self.W), dtype=np.uint64)
         x = np.zeros((max(n, 1), self.W), dtype=np.uint64)
-        c = np.zeros(max(n, 1), dtype=np.float64)
+        c = np.zeros(max(n, 1), dtype=self.dtype)
         got = ctypes.c_int64()
         check(_lib.lib().pp_download(self._h, u64p(z), u64p(x), dp(c), max(n, 1), ctypes.byref(got)))
         return z[:n], x[:n], c[:n]
@@ -211,7 +221,7 @@ class SparsePauliVector:
 
     def copy(self) -> "SparsePauliVector":
         """``copy(v)`` (type_SparsePauliVector.jl:249-254)."""
-        return SparsePauliVector(self.N, self.arrays(), _caps=self.capacities())
+        return SparsePauliVector(self.N, self.arrays(), T=self.T, _caps=self.capacities())
 
     def check(self) -> bool:
         """``check_spv`` key-order invariant (type_SparsePauliVector.jl:263-278)."""
@@ -235,7 +245,7 @@ class SparsePauliVector:
         (type_SparsePauliVector.jl:286-299 _find): (values, found) arrays."""
         keys = list(keys)
         z, x = self._key_arrays(keys)
-        vals = np.zeros(max(len(keys), 1), dtype=np.float64)
+        vals = np.zeros(max(len(keys), 1), dtype=self.dtype)
         found = np.zeros(max(len(keys), 1), dtype=np.int32)
         check(_lib.lib().pp_lookup(self._h, u64p(z), u64p(x), len(keys), dp(vals),
                                    found.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
@@ -248,12 +258,12 @@ class SparsePauliVector:
         v, f = self.lookup([p])
         if not f[0]:
             raise KeyError(p)
-        return float(v[0])
+        return self.T(v[0])
 
     def get(self, p: PauliBasis, default=None):
         """``get(v, p, default)`` (type_SparsePauliVector.jl:301-305)."""
         v, f = self.lookup([p])
-        return float(v[0]) if f[0] else default
+        return self.T(v[0]) if f[0] else default
 
     def haskey(self, p: PauliBasis) -> bool:
         return bool(self.lookup([p])[1][0])
@@ -267,7 +277,7 @@ class SparsePauliVector:
         given twice the last value wins."""
         items = list(items.items() if isinstance(items, dict) else items)
         z, x = self._key_arrays([k for k, _ in items])
-        c = np.array([float(v) for _, v in items] or [0.0], dtype=np.float64)
+        c = np.array([self._coef(v) for _, v in items] or [0.0], dtype=self.dtype)
         check(_lib.lib().pp_setindex(self._h, u64p(z), u64p(x), dp(c), len(items)))
         return self
 
@@ -294,15 +304,35 @@ class SparsePauliVector:
     def filter(self, f) -> "SparsePauliVector":
         return self.copy().filter_(f)
 
-    def mul_(self, a: float):
-        """``mul!(v, a)`` (spv_ops.jl:12-17)."""
-        check(_lib.lib().pp_scale(self._h, float(a)))
+    def _coef(self, v):
+        """convert(T, v): a Float64 vector rejects a nonzero imaginary part
+        (Julia's InexactError)."""
+        if self.complex:
+            return complex(v)
+        cv = complex(v)
+        if cv.imag != 0:
+            raise ArgumentError(f"InexactError: {v} is not a Float64")
+        return cv.real
+
+    def mul_(self, a):
+        """``mul!(v, a)`` (spv_ops.jl:12-17): a real a scales componentwise; a
+        complex a multiplies as Complex * Complex (ComplexF64 vectors)."""
+        if isinstance(a, complex):
+            if not self.complex:
+                self._coef(a)
+                check(_lib.lib().pp_scale(self._h, float(a.real)))
+            else:
+                check(_lib.lib().pp_scale_complex(self._h, float(a.real), float(a.imag)))
+        else:
+            check(_lib.lib().pp_scale(self._h, float(a)))
         return self
 
     def sum_(self, other: "SparsePauliVector"):
         """``sum!(v, other)`` (spv_ops.jl:182-193): v += other."""
         if other.N != self.N:
             raise DimensionMismatch("qubit counts differ")
+        if other.T is not self.T:
+            raise ArgumentError("coefficient types differ")
         if other is self:
             return self.mul_(2.0)
         check(_lib.lib().pp_sum(self._h, other._h))
@@ -334,9 +364,9 @@ class SparsePauliVector:
         check(_lib.lib().pp_norm(self._h, float(p), ctypes.byref(out)))
         return out.value
 
-    def tr(self) -> float:
+    def tr(self):
         """``tr(v)`` (spv_ops.jl:52-55): the identity coefficient times 2^N."""
-        return (self.get((0, 0), 0.0)) * 2.0 ** self.N
+        return (self.get((0, 0), self.T(0.0))) * 2.0 ** self.N
 
     # --- evolution -----------------------------------------------------------------------
     def _gen_arrays(self, gens: Sequence[PauliBasis]):
@@ -512,13 +542,14 @@ class SparsePauliVector:
         return self._scale(4, qubits, self._table(lX), self._table(lY), self._table(lZ))
 
     # --- observables ------------------------------------------------------------------------------
-    def expectation_value(self, psi: Union[Ket, int] = 0) -> float:
-        """``expectation_value(v, ψ::Ket)`` (spv_evolve.jl:133-134)."""
+    def expectation_value(self, psi: Union[Ket, int] = 0):
+        """``expectation_value(v, ψ::Ket)`` (spv_evolve.jl:133-134): a float
+        (a complex for ComplexF64)."""
         k = psi.v if isinstance(psi, Ket) else int(psi)
         ket = _pack(k & ((1 << (64 * self.W)) - 1), self.W)
-        out = ctypes.c_double()
-        check(_lib.lib().pp_expectation_ket(self._h, u64p(ket), ctypes.byref(out)))
-        return out.value
+        out = np.zeros(2, dtype=np.float64)
+        check(_lib.lib().pp_expectation_ket(self._h, u64p(ket), dp(out)))
+        return complex(out[0], out[1]) if self.complex else float(out[0])
 
     def variance(self, psi: Union[Ket, int] = 0) -> float:
         """``variance(v, ψ::Ket)`` (statistics.jl:6-22): ⟨ψ|v²|ψ⟩ - ⟨ψ|v|ψ⟩²."""
@@ -547,13 +578,16 @@ def norm(v: SparsePauliVector, p: float = 2) -> float:
     return v.norm(p)
 
 
-def inner_product(v1: SparsePauliVector, v2: SparsePauliVector) -> float:
-    """``inner_product(v1, v2)`` (spv_ops.jl:378-396): Σ over shared keys of c1 c2."""
+def inner_product(v1: SparsePauliVector, v2: SparsePauliVector):
+    """``inner_product(v1, v2)`` (spv_ops.jl:378-396): Σ over shared keys of
+    conj(c1) c2 (a complex for ComplexF64)."""
     if v1.N != v2.N:
         raise DimensionMismatch("qubit counts differ")
-    out = ctypes.c_double()
-    check(_lib.lib().pp_inner_product(v1._h, v2._h, ctypes.byref(out)))
-    return out.value
+    if v1.T is not v2.T:
+        raise ArgumentError("coefficient types differ")
+    out = np.zeros(2, dtype=np.float64)
+    check(_lib.lib().pp_inner_product(v1._h, v2._h, dp(out)))
+    return complex(out[0], out[1]) if v1.complex else float(out[0])
 
 
 def isapprox(v1: SparsePauliVector, v2: SparsePauliVector, atol: float = 1e-14, rtol: float = 0.0) -> bool:
@@ -561,6 +595,8 @@ def isapprox(v1: SparsePauliVector, v2: SparsePauliVector, atol: float = 1e-14, 
     difference over the union of keys against atol + rtol max(norm)."""
     if v1.N != v2.N:
         raise DimensionMismatch("qubit counts differ")
+    if v1.T is not v2.T:
+        raise ArgumentError("coefficient types differ")
     out = ctypes.c_double()
     check(_lib.lib().pp_diff_norm(v1._h, v2._h, ctypes.byref(out)))
     return out.value <= atol + rtol * max(v1.norm(), v2.norm())

@@ -58,7 +58,7 @@ def anticommute(a: Key, b: Key) -> bool:
 
 
 def dict_evolve_single(N: int, O: Dict[Key, float], G: Key, theta: float, cosv=None, sinv=None) -> None:
-    """evolve.jl:32-46 on a real-coefficient dict (in place)."""
+    """evolve.jl:32-46 on a dict of real or complex coefficients (in place)."""
     c_ = math.cos(theta) if cosv is None else cosv
     s_ = math.sin(theta) if sinv is None else sinv
     sin_branch: Dict[Key, float] = {}
@@ -76,6 +76,72 @@ def dict_evolve_single(N: int, O: Dict[Key, float], G: Key, theta: float, cosv=N
             O[k] = O[k] + v
         else:
             O[k] = v
+
+
+def _dyadic(v: float):
+    """v = n / 2^k exactly (k may be negative)."""
+    n, d = float(v).as_integer_ratio()
+    return n, d.bit_length() - 1
+
+
+def _dy_add_half(a, b):
+    """(a + b) / 2 of two dyadic rationals (n, k)."""
+    (an, ak), (bn, bk) = a, b
+    m = max(ak, bk)
+    return (an << (m - ak)) + (bn << (m - bk)), m + 1
+
+
+def _dy_sq_cmp(a, s):
+    """Sign of a^2 - s for dyadic a = (n, k) and s = (sn, sk) (s already a square sum)."""
+    (an, ak), (sn, sk) = a, s
+    an2, ak2 = an * an, 2 * ak
+    m = max(ak2, sk)
+    l, r = an2 << (m - ak2), sn << (m - sk)
+    return (l > r) - (l < r)
+
+
+def exact_hypot(x: float, y: float) -> float:
+    """The correctly rounded sqrt(x^2 + y^2), decided in exact (integer,
+    dyadic) arithmetic: the double whose rounding interval holds the exact
+    value (candidates around math.hypot, which is within one ulp)."""
+    if math.isinf(x) or math.isinf(y):
+        return math.inf
+    if x == 0 and y == 0:
+        return 0.0
+    (xn, xk), (yn, yk) = _dyadic(x), _dyadic(y)
+    m = max(xk, yk)
+    S = ((xn * xn) << (2 * (m - xk))) + ((yn * yn) << (2 * (m - yk))), 2 * m
+    big = 1.7976931348623157e308
+    top = _dy_add_half(_dyadic(big), (1 << 1024, 0))  # rounds to inf from here
+    if _dy_sq_cmp(top, S) <= 0:
+        return math.inf
+    h = math.hypot(x, y)
+    if math.isinf(h):
+        h = big
+    cands = [h]
+    for _ in range(3):
+        cands = [math.nextafter(cands[0], 0.0)] + cands + [math.nextafter(cands[-1], math.inf)]
+
+    def up(c):  # the next double above c (2^1024 past the largest finite one)
+        n = math.nextafter(c, math.inf)
+        return (1 << 1024, 0) if math.isinf(n) else _dyadic(n)
+
+    for c in cands:
+        if math.isinf(c):
+            continue
+        lo = _dy_add_half(_dyadic(math.nextafter(c, 0.0)), _dyadic(c))
+        hi = _dy_add_half(_dyadic(c), up(c))
+        if _dy_sq_cmp(lo, S) <= 0 <= _dy_sq_cmp(hi, S):
+            return c
+    raise AssertionError(f"no correctly rounded candidate for hypot({x!r}, {y!r})")
+
+
+def cabs(c) -> float:
+    """abs(c): |c| for Float64, the correctly rounded hypot for ComplexF64
+    (the rounding Julia's hypot gives on FMA hardware)."""
+    if isinstance(c, complex):
+        return exact_hypot(c.real, c.imag)
+    return abs(c)
 
 
 def weight(k: Key) -> int:
@@ -115,7 +181,7 @@ def dict_truncate(O: Dict[Key, float], strategy) -> None:
     if isinstance(s, T.NoTruncation):
         return
     if isinstance(s, T.CoeffTruncation):
-        for k in [k for k, c in O.items() if not abs(c) > s.thresh]:
+        for k in [k for k, c in O.items() if not cabs(c) > s.thresh]:
             del O[k]
     elif isinstance(s, T.WeightTruncation):
         for k in [k for k in O if not weight(k) <= s.max_weight]:
@@ -127,14 +193,14 @@ def dict_truncate(O: Dict[Key, float], strategy) -> None:
         for k in [k for k in O if not majorana_weight(k) <= s.max_weight]:
             del O[k]
     elif isinstance(s, T.WeightDampedTruncation):
-        for k in [k for k, c in O.items() if not abs(c) * math.exp(-s.alpha * weight(k)) > s.thresh]:
+        for k in [k for k, c in O.items() if not cabs(c) * math.exp(-s.alpha * weight(k)) > s.thresh]:
             del O[k]
     elif isinstance(s, T.XWeightDampedTruncation):
-        for k in [k for k, c in O.items() if not abs(c) * math.exp(-s.alpha * x_weight(k)) > s.thresh]:
+        for k in [k for k, c in O.items() if not cabs(c) * math.exp(-s.alpha * x_weight(k)) > s.thresh]:
             del O[k]
     elif isinstance(s, T.AdaptiveTruncation):
         if len(O) > s.max_terms:
-            coeffs = sorted(abs(c) for c in O.values())
+            coeffs = sorted(cabs(c) for c in O.values())
             thresh = coeffs[len(coeffs) - 1 - s.max_terms]
             dict_truncate(O, T.CoeffTruncation(thresh))
         else:
@@ -267,11 +333,11 @@ def window_evolve(N: int, terms: Sequence[Tuple[Key, float]], gens: Sequence[Key
             while j < len(srt) and srt[j][0] == k:
                 a = a + srt[j][1]
                 j += 1
-            if not drop_strict(k, abs(a)):
+            if not drop_strict(k, cabs(a)):
                 out.append((k, a))
         live, app = out, []
         if energy_ket is not None:
-            acc_e += expect(live) - before
+            acc_e += (expect(live) - before).real
 
     for i, (g, th) in enumerate(zip(gens, angs)):
         c_, s_ = math.cos(th), math.sin(th)
@@ -285,7 +351,7 @@ def window_evolve(N: int, terms: Sequence[Tuple[Key, float]], gens: Sequence[Key
                 f = 1j * phase
                 v = c * s_ if f.real > 0 else -(c * s_)
                 seq[idx] = (p, c * c_)
-                if not drop_local(k, abs(v)):
+                if not drop_local(k, cabs(v)):
                     new_children.append((k, v))
         app.extend(new_children)
         if (i + 1) % window == 0 or i + 1 == len(gens):
