@@ -449,25 +449,11 @@ struct Batch {
         d_maps = dalloc<int64_t>(4 * (size_t)(B + 1));
         h_maps.assign(4 * (size_t)(B + 1), 0);
         {
-            // fixed table layout (a captured graph keeps these addresses):
-            // each map's per-vector block count is bounded by plan()'s caps
-            auto p2 = [](int64_t v) {
-                int64_t r = 1;
-                while (r < v) r <<= 1;
-                return r;
-            };
-            const int64_t per[4] = {p2((int64_t)ncu * 8), p2((int64_t)ncu * 4), 1, p2((int64_t)ncu * 4)};
-            // a map's table: twice the largest contiguous grid (per-vector
-            // maximum times B, a power of two); plan() falls back to the
-            // contiguous layout when the XCD rows would not fit
-            int64_t off = 0;
-            for (int k = 0; k < 4; ++k) {
-                blk_off[k] = off;
-                blk_cap[k] = 2 * p2(per[k] * B);
-                off += blk_cap[k];
-            }
-            d_blk = dalloc<uint64_t>((size_t)off);
-            h_blk.assign((size_t)off, 0);
+            // block tables start at a few blocks per vector and grow with
+            // plan()'s grids (ensure_blk)
+            const int64_t first[4] = {16 * (int64_t)B + 8 * ncu, 16 * (int64_t)B + 8 * ncu, 2 * (int64_t)B + 8,
+                                      16 * (int64_t)B + 8 * ncu};
+            ensure_blk(first);
         }
         d_filt = dalloc<FilterDev>(2 * (size_t)B);
         d_chan = dalloc<ChanDev>(B);
@@ -634,39 +620,44 @@ struct Batch {
                 load[x] += h_maps[(size_t)0 * (B + 1) + b + 1] - h_maps[(size_t)0 * (B + 1) + b];
             }
         }
+        auto round = [&](int64_t g) {
+            // with captured graphs each grid is rounded up to a power of
+            // two, so that a graph, whose grids are fixed, is replayed by
+            // most calls; the block table itself is data read at run time
+            g = std::max<int64_t>(1, g);
+            if (!graphs) return g;
+            int64_t r = 1;
+            while (r < g) r <<= 1;
+            return r;
+        };
+        // rows pay off where kernels are latency-bound (L2 hits between the
+        // kernels of a step); a call with more tiles than a few per CU is
+        // bandwidth-bound and wants every XCD on every vector
+        const bool rows = acc[0] <= 16 * (int64_t)ncu;
+        std::vector<int64_t> st((size_t)4 * B);
+        int64_t grids[4];
         for (int k = 0; k < 4; ++k) {
             const int64_t* pre = h_maps.data() + (size_t)k * (B + 1);
             int64_t start[kXcds] = {0};
-            std::vector<int64_t> st((size_t)B);
             for (int b : order) {
-                st[(size_t)b] = start[row[(size_t)b]];
+                st[(size_t)k * B + b] = start[row[(size_t)b]];
                 start[row[(size_t)b]] += pre[b + 1] - pre[b];
             }
             int64_t len = 0;
             for (int x = 0; x < kXcds; ++x) len = std::max(len, start[x]);
-            auto round = [&](int64_t g) {
-                // with captured graphs each grid is rounded up to a power of
-                // two, so that a graph, whose grids are fixed, is replayed by
-                // most calls; the block table itself is data read at run time
-                g = std::max<int64_t>(1, g);
-                if (!graphs) return g;
-                int64_t r = 1;
-                while (r < g) r <<= 1;
-                return r;
-            };
-            int64_t grid = round((int64_t)kXcds * len);
-            // rows pay off where kernels are latency-bound (L2 hits between
-            // the kernels of a step); a call with more tiles than a few per CU
-            // is bandwidth-bound and wants every XCD on every vector
-            const bool rows = grid <= blk_cap[k] && acc[0] <= 16 * (int64_t)ncu;
-            if (!rows) grid = round(pre[B]);  // contiguous layout: pre[B] <= per-vector maximum x B
+            grids[k] = rows ? round((int64_t)kXcds * len) : round(pre[B]);  // contiguous layout: pre[B] entries
+        }
+        ensure_blk(grids);
+        for (int k = 0; k < 4; ++k) {
+            const int64_t* pre = h_maps.data() + (size_t)k * (B + 1);
+            const int64_t grid = grids[k];
             h_grid[k] = grid;
             uint64_t* tab = h_blk.data() + blk_off[k];
             std::fill(tab, tab + grid, (uint64_t)kBlkNone);
             for (int b = 0; b < B; ++b) {
                 const int64_t nbk = pre[b + 1] - pre[b];
                 for (int64_t lb = 0; lb < nbk; ++lb) {
-                    const int64_t g = rows ? kXcds * (st[(size_t)b] + lb) + row[(size_t)b] : pre[b] + lb;
+                    const int64_t g = rows ? kXcds * (st[(size_t)k * B + b] + lb) + row[(size_t)b] : pre[b] + lb;
                     tab[(size_t)g] =
                         (uint64_t)b | ((uint64_t)lb << kBlkVecBits) | ((uint64_t)nbk << (kBlkVecBits + kBlkLocBits));
                 }
@@ -678,6 +669,38 @@ struct Batch {
             dfree(d_partial);
             cap_partial = acc[0];
             d_partial = dalloc<double2>(cap_partial);
+        }
+    }
+
+    // Block tables of at least need[k] entries per map.  Growing moves the
+    // tables: in-flight kernels finish first, and the captured graphs, which
+    // hold the old address, are dropped (they are captured again on use).
+    void ensure_blk(const int64_t need[4]) {
+        bool fits = d_blk != nullptr;
+        for (int k = 0; k < 4; ++k) fits = fits && need[k] <= blk_cap[k];
+        if (fits) return;
+        auto p2 = [](int64_t v) {
+            int64_t r = 1;
+            while (r < v) r <<= 1;
+            return r;
+        };
+        int64_t cap[4], off = 0;
+        for (int k = 0; k < 4; ++k) {
+            cap[k] = std::max(blk_cap[k], 2 * p2(std::max<int64_t>(need[k], 1)));
+            off += cap[k];
+        }
+        if (stream) HIPCHK(hipStreamSynchronize(stream));
+        for (auto& g : gcache) (void)hipGraphExecDestroy(g.second);
+        gcache.clear();
+        dfree(d_blk);
+        d_blk = nullptr;
+        d_blk = dalloc<uint64_t>((size_t)off);
+        h_blk.assign((size_t)off, 0);
+        off = 0;
+        for (int k = 0; k < 4; ++k) {
+            blk_off[k] = off;
+            blk_cap[k] = cap[k];
+            off += cap[k];
         }
     }
 
