@@ -241,6 +241,30 @@ int pp_scale_by_popcount(pp_handle* h, int kind, const uint64_t* mask, const dou
  * part, for ComplexF64).  Synchronises. */
 int pp_expectation_ket(pp_handle* h, const uint64_t* ket, double* out);
 
+/* copy(v) (type_SparsePauliVector.jl:249-254): dst takes src's live set and
+ * the reference's virtual capacities, device to device (same N and T). */
+int pp_copy(pp_handle* dst, pp_handle* src);
+/* otimes(v1, v2) = v1 ⊗ v2 (spv_ops.jl:342-358): out (N = N1 + N2, same T,
+ * a third handle) becomes the tensor product, keys (z1 | z2 << N1,
+ * x1 | x2 << N1) (type_PauliBasis.jl:130-132), coefficients c1 * c2, sorted;
+ * out's virtual capacities become the reference's (live max(16, n), append 16).
+ * osum (spv_ops.jl:360-366) is v1 ⊗ I + I ⊗ v2: two otimes and a pp_sum. */
+int pp_otimes(pp_handle* out, pp_handle* a, pp_handle* b);
+/* largest(v) / largest_diag(v) (spv_ops.jl:520-553): the first term with the
+ * largest |c| (diag_only: among the x == 0 terms): its key (W words each),
+ * coefficient (cw doubles) and 0-based index.  An empty vector (or no
+ * diagonal term) is PP_EARG ("collection must be non-empty"). */
+int pp_largest(pp_handle* h, int diag_only, uint64_t* z, uint64_t* x, double* c, int64_t* index);
+/* diag(v) (offdiag = 0) / offdiag(v) (offdiag = 1) in place (spv_ops.jl:555-587):
+ * keep the terms with x == 0 (x != 0), order preserved.  The reference's
+ * functions copy first (pp_copy). */
+int pp_select_diag(pp_handle* h, int offdiag);
+/* Matrix(v) (spv_ops.jl:109-115) for N <= 13: the 2^N x 2^N matrix,
+ * column-major (Julia's layout), one double (Float64) or two (re, im) per
+ * entry, Σ over terms in key order of Matrix(op) .* c with site 1 the least
+ * significant index bit (type_PauliBasis.jl:71-95).  A Float64 vector with an
+ * imaginary entry (a term with an odd number of Y) is PP_EARG (InexactError). */
+int pp_matrix(pp_handle* h, double* out);
 /* check_spv key-order invariant (type_SparsePauliVector.jl:263-278): *ok = 1
  * if the live keys are strictly increasing. */
 int pp_check(pp_handle* h, int* ok);

@@ -2055,6 +2055,150 @@ struct Batch {
         return v;
     }
 
+    // copy(v) (type_SparsePauliVector.jl:249-254): vector 0 takes src's live
+    // set and the reference's virtual capacities, device to device.
+    void copy_from(Batch& src) {
+        int64_t n = 0;
+        const DevSet& S = src.live0(n);
+        const int64_t lcap = src.h_state[0].live_cap_v, acap = src.h_state[0].app_cap_v;
+        sync_state();
+        ensure_caps(0, std::max<int64_t>(2 * n + 1024, 4096), std::max<int64_t>(n + 1024, ib[0].C.cap), ib[0].maxlev,
+                    0, 0);
+        put_live0(S.z, S.x, S.c, n, lcap, acap);
+    }
+
+    // The live set of vector 0 := n device terms (sorted, unique), with the
+    // given virtual capacities; buffers already hold n terms.
+    void put_live0(const uint64_t* z, const uint64_t* x, const double* c, int64_t n, int64_t live_cap,
+                   int64_t app_cap) {
+        DevSet& d = ib[0].L[0];
+        if (n > 0) {
+            HIPCHK(hipMemcpyAsync(d.z, z, (size_t)n * kb(), hipMemcpyDeviceToDevice, stream));
+            HIPCHK(hipMemcpyAsync(d.x, x, (size_t)n * kb(), hipMemcpyDeviceToDevice, stream));
+            HIPCHK(hipMemcpyAsync(d.c, c, (size_t)n * cb(), hipMemcpyDeviceToDevice, stream));
+        }
+        SegState& st = h_state[0];
+        st.n[0] = n;
+        st.n[1] = 0;
+        st.cur = 0;
+        st.flip = 0;
+        st.m = 0;
+        st.stalled = 0;
+        st.compact = 0;
+        st.live_cap_v = live_cap;
+        st.app_cap_v = app_cap;
+        n_known[0] = n;
+        put_state();
+        upload_desc();
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    // otimes(v1, v2) (spv_ops.jl:342-358) into vector 0 (N = N1 + N2): every
+    // pair's key and product, radix-sorted (the keys are distinct, so the
+    // reference's sort + _triples_to_live! is the sorted copy); the capacities
+    // of the reference's result, live max(16, n) and append 16.
+    template <int Wt, typename CT>
+    void otimes_from(Batch& a, int Na, Batch& b) {
+        int64_t na = 0, nb = 0;
+        const DevSet& A = a.live0(na);
+        const DevSet& Bv = b.live0(nb);
+        sync_state();
+        const int64_t m = na * nb;
+        if (m > 0) {
+            win_init();
+            grow_set(win.A, m, 0);
+            const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(m, kBlock), 8192));
+            hipLaunchKernelGGL(ko_otimes<CT>, dim3(grid), dim3(kBlock), 0, stream, A.z, A.x, cptr<CT>(A.c), na, a.W,
+                               Bv.z, Bv.x, cptr<CT>(Bv.c), nb, b.W, Na, win.A.z, win.A.x, cptr<CT>(win.A.c), W);
+            HIPCHK(hipGetLastError());
+        }
+        const DevSet* S = m > 0 ? win_sort<Wt, CT>(m) : nullptr;
+        ensure_caps(0, std::max<int64_t>(2 * m + 1024, 4096), std::max<int64_t>(m + 1024, ib[0].C.cap), ib[0].maxlev,
+                    0, 0);
+        put_live0(S ? S->z : nullptr, S ? S->x : nullptr, S ? S->c : nullptr, m, std::max<int64_t>(16, m), 16);
+    }
+
+    // largest / largest_diag (spv_ops.jl:520-553): index of the first term with
+    // the largest |c| (among x == 0 terms for diag_only); -1 if none.
+    template <int Wt, typename CT>
+    int64_t largest(int diag_only) {
+        int64_t n = 0;
+        const DevSet& L = live0(n);
+        if (n == 0) return -1;
+        const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(n, kBlock), 1024));
+        double2* part = dalloc<double2>(grid);
+        int64_t* dout = dalloc<int64_t>(1);
+        hipLaunchKernelGGL((ko_argmax<Wt, CT>), dim3(grid), dim3(kBlock), 0, stream, L.x, cptr<CT>(L.c), n, diag_only,
+                           part);
+        hipLaunchKernelGGL(ko_argmax_final, dim3(1), dim3(kBlock), 0, stream, part, (int64_t)grid, dout);
+        HIPCHK(hipGetLastError());
+        int64_t idx = -1;
+        HIPCHK(hipMemcpyAsync(&idx, dout, sizeof(int64_t), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        dfree(part);
+        dfree(dout);
+        return idx;
+    }
+
+    // Term i of the live set of vector 0 to the host.
+    void fetch_term(int64_t i, uint64_t* z, uint64_t* x, double* c) {
+        int64_t n = 0;
+        const DevSet& L = live0(n);
+        if (i < 0 || i >= n) throw PPError(PP_EARG, "term index out of range");
+        HIPCHK(hipMemcpyAsync(z, L.z + i * W, kb(), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(x, L.x + i * W, kb(), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(c, L.c + i * cw, cb(), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    // diag(v) / offdiag(v) in place (spv_ops.jl:555-587): the terms with
+    // x == 0 (or x != 0), order kept.
+    template <int Wt, typename CT>
+    void select_diag(int offdiag) {
+        win_init();
+        sync_state();
+        SegState& st = h_state[0];
+        int cur = st.cur;
+        int64_t n = st.n[cur];
+        n = win_compact<Wt, CT>(n, cur, offdiag ? 5 : 4, 0.0, 0);
+        commit_live0(cur, n, st.live_cap_v);
+    }
+
+    // Matrix(v) (spv_ops.jl:109-115) for N <= 13: 2^N x 2^N entries,
+    // column-major, cw doubles each, into host memory.
+    template <int Wt, typename CT>
+    void matrix(int N, double* out) {
+        int64_t n = 0;
+        const DevSet& L = live0(n);
+        const int64_t D = (int64_t)1 << N;
+        const size_t bytes = (size_t)(D * D) * cb();
+        win_init();
+        WinBuf& w = win;
+        double* dout = dalloc<double>((size_t)(D * D) * cw);
+        const DevSet* S = nullptr;
+        if (n > 0) {
+            grow_set(w.A, n, 0);
+            const unsigned g1 = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(n, kBlock), 4096));
+            hipLaunchKernelGGL((kv_swap<Wt, CT>), dim3(g1), dim3(kBlock), 0, stream, L.z, L.x, cptr<CT>(L.c), n,
+                               w.A.z, w.A.x, cptr<CT>(w.A.c));
+            S = win_sort<Wt, CT>(n);
+        }
+        HIPCHK(hipMemsetAsync(w.err, 0, sizeof(int32_t), stream));
+        const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(D * D, kBlock), 16384));
+        hipLaunchKernelGGL(ko_matrix<CT>, dim3(grid), dim3(kBlock), 0, stream, S ? S->z : w.A.z, S ? S->x : w.A.x,
+                           cptr<CT>(S ? S->c : w.A.c), n, Wt, N, dout, w.err);
+        HIPCHK(hipGetLastError());
+        int32_t e = 0;
+        HIPCHK(hipMemcpyAsync(out, dout, bytes, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(&e, w.err, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        dfree(dout);
+        HIPCHK(hipMemsetAsync(w.err, 0, sizeof(int32_t), stream));
+        if (e)
+            throw PPError(PP_EARG, "InexactError: Matrix(v) of a Float64 vector with an imaginary entry "
+                                   "(a term with an odd number of Y factors); use ComplexF64 coefficients");
+    }
+
     void upload(int b, const uint64_t* z, const uint64_t* x, const double* c, int64_t n, bool sorted) {
         const int Wl = W;
         std::vector<uint64_t> hz(z, z + n * Wl), hx(x, x + n * Wl);
@@ -2694,6 +2838,68 @@ int pp_expectation_ket(pp_handle* h, const uint64_t* ket, double* out) {
     if (!h || !ket || !out) return fail(PP_EARG, "null argument");
     API_BEGIN
     dispatch_wc(h->bt, [&](auto wc, auto ct) { h->bt.expectation<decltype(wc)::value, typename decltype(ct)::type>(ket, out); });
+    API_END
+}
+
+int pp_copy(pp_handle* dst, pp_handle* src) {
+    if (!dst || !src) return fail(PP_EARG, "null argument");
+    if (dst == src) return PP_OK;
+    if (dst->N != src->N) return fail(PP_EARG, "qubit counts differ");
+    if (dst->bt.cw != src->bt.cw) return fail(PP_EARG, "coefficient types differ");
+    API_BEGIN
+    dst->bt.copy_from(src->bt);
+    API_END
+}
+
+int pp_otimes(pp_handle* out, pp_handle* a, pp_handle* b) {
+    if (!out || !a || !b) return fail(PP_EARG, "null argument");
+    if (out == a || out == b) return fail(PP_EARG, "the result must be a third vector");
+    if (out->N != a->N + b->N) return fail(PP_EARG, "the result needs N1 + N2 qubits");
+    if (a->bt.cw != b->bt.cw || out->bt.cw != a->bt.cw) return fail(PP_EARG, "coefficient types differ");
+    API_BEGIN
+    dispatch_wc(out->bt, [&](auto wc, auto ct) {
+        out->bt.otimes_from<decltype(wc)::value, typename decltype(ct)::type>(a->bt, a->N, b->bt);
+    });
+    API_END
+}
+
+int pp_largest(pp_handle* h, int diag_only, uint64_t* z, uint64_t* x, double* c, int64_t* index) {
+    if (!h || !z || !x || !c || !index) return fail(PP_EARG, "null argument");
+    API_BEGIN
+    int64_t idx = -1;
+    dispatch_wc(h->bt, [&](auto wc, auto ct) {
+        idx = h->bt.largest<decltype(wc)::value, typename decltype(ct)::type>(diag_only ? 1 : 0);
+    });
+    if (idx < 0) {
+        // largest: a vector of only NaN coefficients gives its first term (the
+        // reference's bi = 1); an empty vector or no diagonal term is an error
+        int64_t n = 0;
+        h->bt.sync_state();
+        n = h->bt.n_known[0];
+        if (diag_only || n == 0) throw PPError(PP_EARG, "collection must be non-empty");
+        idx = 0;
+    }
+    h->bt.fetch_term(idx, z, x, c);
+    *index = idx;
+    API_END
+}
+
+int pp_select_diag(pp_handle* h, int offdiag) {
+    if (!h) return fail(PP_EARG, "null handle");
+    API_BEGIN
+    dispatch_wc(h->bt, [&](auto wc, auto ct) {
+        h->bt.select_diag<decltype(wc)::value, typename decltype(ct)::type>(offdiag ? 1 : 0);
+    });
+    API_END
+}
+
+int pp_matrix(pp_handle* h, double* out) {
+    if (!h || !out) return fail(PP_EARG, "null argument");
+    if (h->N > 13) return fail(PP_EARG, "Matrix(v) is limited to N <= 13 (2^26 entries)");
+    API_BEGIN
+    dispatch_wc(h->bt, [&](auto wc, auto ct) {
+        h->bt.matrix<decltype(wc)::value, typename decltype(ct)::type>(h->N, out);
+    });
     API_END
 }
 

@@ -589,6 +589,9 @@ __global__ void __launch_bounds__(kBlock) kq_hist(const CT* __restrict__ C, int6
 //   mode 1: stochastic clip: |c| < eps survives with probability |c| / eps
 //           and becomes eps * sign(c); the uniform variate of term i is
 //           counter_uniform(seed, i)
+//   mode 2: sampling selection (SampleSel); mode 3: delete marked terms
+//   mode 4: diag (x == 0, spv_ops.jl:571-587); mode 5: offdiag (x != 0,
+//           spv_ops.jl:555-569)
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
     // SplitMix64 finaliser (Steele, Lea, Flood 2014)
     z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
@@ -619,6 +622,12 @@ __device__ __forceinline__ bool compact_keep(int mode, const FilterDev* f, doubl
         return e > ss.eT || (e == ss.eT && ss.n - 1 - i >= ss.jT);
     }
     if (mode == 3) return ss.mark[i] == 0;
+    if (mode == 4 || mode == 5) {
+        uint64_t xo = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) xo |= k.x[w];
+        return (xo == 0) == (mode == 4);
+    }
     const double ac = coef_abs(c);
     if (ac < eps) {
         if (!(counter_uniform(seed, (uint64_t)i) < ac / eps)) return false;

@@ -22,7 +22,8 @@ import PauliOperators: evolve!, expectation_value, truncate!, stochastic_clip!,
                        x_weight_damped_clip!, majorana_weight_clip!,
                        depolarizing_channel!, dephasing_channel!, bit_flip_channel!,
                        bit_phase_flip_channel!, pauli_channel!,
-                       commutator, anticommutator, inner_product, check_spv, variance
+                       commutator, anticommutator, inner_product, check_spv, variance,
+                       otimes, osum, largest, largest_diag, offdiag
 using PauliOperators: MergeFilter, NOFILTER, _compile_filter, _is_compilable, _pack, _spv_mask,
                       TruncationStrategy, NoTruncation, CoeffTruncation, WeightTruncation,
                       WeightDampedTruncation, XWeightTruncation, XWeightDampedTruncation,
@@ -458,6 +459,68 @@ function LinearAlgebra.mul!(d::DeviceSPV{N,W,T}, s::Complex) where {N,W,T}
     T <: Complex || return LinearAlgebra.mul!(d, convert(Float64, s))
     check(ccall((:pp_scale_complex, LIB), Cint, (Ptr{Cvoid}, Float64, Float64), d.h, real(s), imag(s)))
     return d
+end
+
+# ---- copy, tensor products, selections, Matrix (spv_ops.jl:109-115,342-366,520-587) --
+
+# copy(v) (type_SparsePauliVector.jl:249-254), device to device
+function Base.copy(d::DeviceSPV{N,W,T}) where {N,W,T}
+    lc, ac = capacities(d)
+    out = DeviceSPV{N,W,T}(lc, ac)
+    check(ccall((:pp_copy, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), out.h, d.h))
+    return out
+end
+
+# otimes(v1, v2) = v1 ⊗ v2: v2 on the high sites, coefficients c1 * c2
+function otimes(a::DeviceSPV{N,W1,T}, b::DeviceSPV{M,W2,T}) where {N,M,W1,W2,T}
+    N + M <= 128 || throw(ArgumentError("N1 + N2 must be <= 128"))
+    Wo = N + M <= 64 ? UInt64 : UInt128
+    out = DeviceSPV{N + M,Wo,T}(16, 16)
+    check(ccall((:pp_otimes, LIB), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}), out.h, a.h, b.h))
+    return out
+end
+
+# osum(v1, v2) = v1 ⊗ I_M + I_N ⊗ v2
+function osum(a::DeviceSPV{N,W1,T}, b::DeviceSPV{M,W2,T}) where {N,M,W1,W2,T}
+    I_N = DeviceSPV{N,W1,T}(16, 16)
+    I_N[PauliBasis{N}(Int128(0), Int128(0))] = one(T)
+    I_M = DeviceSPV{M,W2,T}(16, 16)
+    I_M[PauliBasis{M}(Int128(0), Int128(0))] = one(T)
+    out = otimes(a, I_M)
+    return sum!(out, otimes(I_N, b))
+end
+
+function _largest(d::DeviceSPV{N,W,T}, diag_only::Bool) where {N,W,T}
+    z = Ref{W}(zero(W))
+    x = Ref{W}(zero(W))
+    c = Ref{T}(zero(T))
+    idx = Ref{Int64}(0)
+    check(ccall((:pp_largest, LIB), Cint, (Ptr{Cvoid}, Cint, Ref{W}, Ref{W}, Ref{T}, Ref{Int64}),
+                d.h, diag_only, z, x, c, idx))
+    return PauliBasis{N}(Int128(z[] % UInt128), Int128(x[] % UInt128)), c[]
+end
+# largest(v): the first term with the largest |c|, as a one-term vector
+function largest(d::DeviceSPV{N,W,T}) where {N,W,T}
+    p, c = _largest(d, false)
+    out = DeviceSPV{N,W,T}(16, 16)
+    out[p] = c
+    return out
+end
+# largest_diag(v): PauliBasis => c of the diagonal term with the largest |c|
+largest_diag(d::DeviceSPV) = Pair(_largest(d, true)...)
+
+# diag(v) / offdiag(v): copies holding the x == 0 (x != 0) terms
+LinearAlgebra.diag(d::DeviceSPV) =
+    (out = copy(d); check(ccall((:pp_select_diag, LIB), Cint, (Ptr{Cvoid}, Cint), out.h, 0)); out)
+offdiag(d::DeviceSPV) =
+    (out = copy(d); check(ccall((:pp_select_diag, LIB), Cint, (Ptr{Cvoid}, Cint), out.h, 1)); out)
+
+# Matrix(v) for N <= 13, Julia's column-major layout
+function Base.Matrix(d::DeviceSPV{N,W,T}) where {N,W,T}
+    N <= 13 || throw(ArgumentError("Matrix(v) is limited to N <= 13"))
+    out = Matrix{T}(undef, 2^N, 2^N)
+    check(ccall((:pp_matrix, LIB), Cint, (Ptr{Cvoid}, Ptr{T}), d.h, out))
+    return out
 end
 
 # ---- products (spv_ops.jl:255-484) on host SparsePauliVectors ---------------------

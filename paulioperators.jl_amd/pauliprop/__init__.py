@@ -16,7 +16,7 @@ from ._lib import ArgumentError, DimensionMismatch, PPCounters, PPError, PPStrat
 from .spv import (SparsePauliVector, bit_flip_channel, bit_flip_channel_, bit_phase_flip_channel,
                   bit_phase_flip_channel_, dephasing_channel, dephasing_channel_, depolarizing_channel,
                   depolarizing_channel_, depolarizing_p_for_weight_decay, evolve, evolve_, expectation_value,
-                  inner_product, isapprox, norm,
+                  inner_product, isapprox, norm, otimes, osum, largest, largest_diag, diag, offdiag, Matrix,
                   pauli_channel, pauli_channel_, phase_flip_channel_, stochastic_clip_, truncate_, variance)
 from .batch import SparsePauliBatch
 from .algebra import anticommutator, commutator, multiply

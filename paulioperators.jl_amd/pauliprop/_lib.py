@@ -19,6 +19,7 @@ PP_OK, PP_EARG, PP_ENOMEM, PP_EHIP, PP_EOVERFLOW, PP_EUNSUPPORTED = range(6)
 EXPORTS = ("pp_create", "pp_create_typed", "pp_coeff_type", "pp_destroy", "pp_upload", "pp_download", "pp_size", "pp_capacities", "pp_evolve",
            "pp_evolve_ex", "pp_truncate", "pp_variance_ket", "pp_evolve_single", "pp_compact", "pp_lookup", "pp_lookup_noalloc", "pp_norm",
            "pp_inner_product", "pp_diff_norm", "pp_setindex", "pp_delete", "pp_sum", "pp_scale", "pp_scale_complex", "pp_scale_by_popcount", "pp_expectation_ket", "pp_check",
+           "pp_copy", "pp_otimes", "pp_largest", "pp_select_diag", "pp_matrix",
            "pp_algebra",
            "pp_batch_create", "pp_batch_destroy", "pp_batch_words", "pp_batch_upload", "pp_batch_download",
            "pp_batch_sizes", "pp_batch_evolve", "pp_batch_scale_by_popcount", "pp_batch_expectation_ket",
@@ -121,6 +122,11 @@ def lib():
     L.pp_scale_by_popcount.argtypes = [vp, ctypes.c_int, u64p, dp, dp, dp]
     L.pp_expectation_ket.argtypes = [vp, u64p, dp]
     L.pp_check.argtypes = [vp, P(ctypes.c_int)]
+    L.pp_copy.argtypes = [vp, vp]
+    L.pp_otimes.argtypes = [vp, vp, vp]
+    L.pp_largest.argtypes = [vp, ctypes.c_int, u64p, u64p, dp, P(i64)]
+    L.pp_select_diag.argtypes = [vp, ctypes.c_int]
+    L.pp_matrix.argtypes = [vp, dp]
     L.pp_algebra.argtypes = [vp, ctypes.c_int, ctypes.c_int, P(PPTerms), P(PPTerms), u64p, u64p, dp, dp, i64, P(i64)]
     ip = P(ctypes.c_int)
     L.pp_batch_create.argtypes = [ctypes.c_int, ip, P(i64), P(i64), P(vp)]

@@ -220,8 +220,57 @@ class SparsePauliVector:
         return out
 
     def copy(self) -> "SparsePauliVector":
-        """``copy(v)`` (type_SparsePauliVector.jl:249-254)."""
-        return SparsePauliVector(self.N, self.arrays(), T=self.T, _caps=self.capacities())
+        """``copy(v)`` (type_SparsePauliVector.jl:249-254), device to device."""
+        out = SparsePauliVector(self.N, None, T=self.T, _caps=self.capacities())
+        check(_lib.lib().pp_copy(out._h, self._h))
+        return out
+
+    # --- spv_ops.jl:520-587 ------------------------------------------------------------
+    def largest(self) -> "SparsePauliVector":
+        """``largest(v)`` (spv_ops.jl:520-536): the term with the largest |c| (the
+        first such in key order) as a one-term vector."""
+        z, x, c, _ = self._largest(0)
+        out = SparsePauliVector(self.N, None, T=self.T, _caps=(16, 16))
+        out[PauliBasis(self.N, z, x)] = c
+        return out
+
+    def largest_diag(self):
+        """``largest_diag(v)`` (spv_ops.jl:538-553): (PauliBasis, c) of the
+        diagonal (x == 0) term with the largest |c|."""
+        z, x, c, _ = self._largest(1)
+        return PauliBasis(self.N, z, x), c
+
+    def _largest(self, diag_only: int):
+        zw = np.zeros(self.W, dtype=np.uint64)
+        xw = np.zeros(self.W, dtype=np.uint64)
+        cw = np.zeros(1, dtype=self.dtype)
+        idx = ctypes.c_int64()
+        check(_lib.lib().pp_largest(self._h, diag_only, u64p(zw), u64p(xw), dp(cw), ctypes.byref(idx)))
+        return _unpack_rows(zw[None, :])[0], _unpack_rows(xw[None, :])[0], self.T(cw[0]), idx.value
+
+    def diag(self) -> "SparsePauliVector":
+        """``diag(v)`` (spv_ops.jl:571-587): a copy with the x == 0 terms."""
+        out = self.copy()
+        check(_lib.lib().pp_select_diag(out._h, 0))
+        return out
+
+    def offdiag(self) -> "SparsePauliVector":
+        """``offdiag(v)`` (spv_ops.jl:555-569): a copy with the x != 0 terms."""
+        out = self.copy()
+        check(_lib.lib().pp_select_diag(out._h, 1))
+        return out
+
+    def matrix(self) -> np.ndarray:
+        """``Matrix(v)`` (spv_ops.jl:109-115) for N <= 13: the dense 2^N x 2^N
+        matrix (site 1 the least significant index bit), float64 for Float64
+        coefficients (an imaginary entry raises, as Julia's InexactError),
+        complex128 for ComplexF64."""
+        if self.N > 13:
+            raise ArgumentError("Matrix(v) is limited to N <= 13")
+        D = 1 << self.N
+        buf = np.zeros(D * D, dtype=self.dtype)
+        check(_lib.lib().pp_matrix(self._h, dp(buf)))
+        return buf.reshape(D, D).T.copy()  # column-major entries
 
     def check(self) -> bool:
         """``check_spv`` key-order invariant (type_SparsePauliVector.jl:263-278)."""
@@ -588,6 +637,46 @@ def inner_product(v1: SparsePauliVector, v2: SparsePauliVector):
     out = np.zeros(2, dtype=np.float64)
     check(_lib.lib().pp_inner_product(v1._h, v2._h, dp(out)))
     return complex(out[0], out[1]) if v1.complex else float(out[0])
+
+
+def otimes(v1: SparsePauliVector, v2: SparsePauliVector) -> SparsePauliVector:
+    """``otimes(v1, v2)`` = v1 ⊗ v2 (spv_ops.jl:342-358): N1 + N2 qubits, v2 on
+    the high sites, coefficients c1 c2."""
+    if v1.T is not v2.T:
+        raise ArgumentError("coefficient types differ")
+    if v1.N + v2.N > 128:
+        raise ArgumentError("N1 + N2 must be <= 128")
+    out = SparsePauliVector(v1.N + v2.N, None, T=v1.T, _caps=(16, 16))
+    check(_lib.lib().pp_otimes(out._h, v1._h, v2._h))
+    return out
+
+
+def osum(v1: SparsePauliVector, v2: SparsePauliVector) -> SparsePauliVector:
+    """``osum(v1, v2)`` = v1 ⊗ I + I ⊗ v2 (spv_ops.jl:360-366)."""
+    one = v1.T(1.0)
+    I_N = SparsePauliVector(v1.N, [((0, 0), one)], T=v1.T, _caps=(16, 16))
+    I_M = SparsePauliVector(v2.N, [((0, 0), one)], T=v2.T, _caps=(16, 16))
+    return otimes(v1, I_M) + otimes(I_N, v2)
+
+
+def largest(v: SparsePauliVector) -> SparsePauliVector:
+    return v.largest()
+
+
+def largest_diag(v: SparsePauliVector):
+    return v.largest_diag()
+
+
+def diag(v: SparsePauliVector) -> SparsePauliVector:
+    return v.diag()
+
+
+def offdiag(v: SparsePauliVector) -> SparsePauliVector:
+    return v.offdiag()
+
+
+def Matrix(v: SparsePauliVector) -> np.ndarray:
+    return v.matrix()
 
 
 def isapprox(v1: SparsePauliVector, v2: SparsePauliVector, atol: float = 1e-14, rtol: float = 0.0) -> bool:
