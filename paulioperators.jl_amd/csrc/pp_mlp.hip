@@ -19,6 +19,7 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include <type_traits>
 
 #include "../../include/pauliprop.h"
 
@@ -74,6 +75,8 @@ struct GemmArgs {
     int64_t sbias;
     const float* Zp;
     int64_t ldz, sze;
+    float* db;  // EPI 2: db[col] += the column's sum of the bf16 outputs (bias gradient)
+    int64_t sdb;
 };
 
 // Stage a 64 x 32 tile (rows r0.., k0..) of a bf16 operand into LDS as
@@ -123,6 +126,7 @@ __global__ void __launch_bounds__(256) kg_gemm(GemmArgs g) {
     }
     // accumulator r of lane l: row 8 (r div 4) + 4 (l div 32) + (r mod 4), column l mod 32
     const int col = n0 + wn * 32 + (lane & 31);
+    float dbs = 0.0f;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int row = m0 + wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
@@ -135,7 +139,187 @@ __global__ void __launch_bounds__(256) kg_gemm(GemmArgs g) {
             g.H[e * g.she + (int64_t)row * g.ldh + col] = f2bf(gelu(z));
         } else {
             const float zp = g.Zp[e * g.sze + (int64_t)row * g.ldz + col];
-            g.H[e * g.she + (int64_t)row * g.ldh + col] = f2bf(v * gelu_grad(zp));
+            const uint16_t h = f2bf(v * gelu_grad(zp));
+            g.H[e * g.she + (int64_t)row * g.ldh + col] = h;
+            dbs += bf2f(h);
+        }
+    }
+    if (EPI == 2 && g.db) {
+        dbs += __shfl_xor(dbs, 32, 64);  // lanes l and l + 32 hold the same column
+        if (lane < 32) atomicAdd(g.db + e * g.sdb + col, dbs);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// kg_gemm2: the same product and epilogues for shapes that are multiples of
+// 128 (M), 64 WNB (N) and 64 (K).  A workgroup computes a 128 x (64 WNB) tile
+// with four waves of 64 x (32 WNB) (2 x WNB blocks of
+// v_mfma_f32_32x32x16_bf16: each A fragment used WNB times, each B fragment
+// twice); K advances 64 per stage through two LDS buffers: the next stage's
+// global loads are issued into registers before the current stage's MFMAs and
+// written to the other buffer after them, one barrier per stage.  AK / BK: the
+// operand's k stride is 1 (rows of 8 contiguous k per 16-byte load);
+// otherwise its row stride is 1 and each thread loads a block of 8 rows x 4 k
+// (four 16-byte loads), transposes it in registers and writes 8-byte rows to
+// LDS, the 16 lanes of a group covering one row's 64 k (no bank conflict).
+constexpr int kTM = 128, kTK = 64, kLDT = kTK + 8;  // LDS row: 72 bf16 (144 B)
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+// One operand tile of ROWS x 64 in flight: ROWS / 32 (k-contiguous) or
+// ROWS / 128 (row-contiguous, four loads each) 16-byte loads per thread.
+template <int ROWS>
+struct Stage {
+    u32x4 v[ROWS / 32];
+};
+
+template <bool KC, int ROWS>
+__device__ __forceinline__ void stage_load(const uint16_t* __restrict__ src, int64_t srow, int64_t sk, int r0, int k0,
+                                           Stage<ROWS>& st) {
+    const int t = threadIdx.x;
+    if constexpr (KC) {
+#pragma unroll
+        for (int i = 0; i < ROWS / 32; ++i) {
+            const int c = t + 256 * i;  // ROWS x 8 chunks of 8 k
+            const int row = c >> 3, kc = (c & 7) * 8;
+            st.v[i] = *reinterpret_cast<const u32x4*>(src + (int64_t)(r0 + row) * srow + k0 + kc);
+        }
+    } else {
+#pragma unroll
+        for (int h = 0; h < ROWS / 128; ++h) {
+            const int rb = ((t >> 4) + 16 * h) * 8, kb = (t & 15) * 4;  // row blocks of 8, k quads
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                st.v[4 * h + i] = *reinterpret_cast<const u32x4*>(src + (int64_t)(k0 + kb + i) * sk + r0 + rb);
+        }
+    }
+}
+
+template <bool KC, int ROWS>
+__device__ __forceinline__ void stage_store(const Stage<ROWS>& st, uint16_t* __restrict__ dst) {
+    const int t = threadIdx.x;
+    if constexpr (KC) {
+#pragma unroll
+        for (int i = 0; i < ROWS / 32; ++i) {
+            const int c = t + 256 * i;
+            const int row = c >> 3, kc = (c & 7) * 8;
+            *reinterpret_cast<u32x4*>(dst + row * kLDT + kc) = st.v[i];
+        }
+    } else {
+#pragma unroll
+        for (int h = 0; h < ROWS / 128; ++h) {
+            const int rb = ((t >> 4) + 16 * h) * 8, kb = (t & 15) * 4;
+            // st.v[4h + i] holds rows rb..rb+7 at k = kb + i; row r of the
+            // output is (k0, k1) (k2, k3) as two 32-bit words
+#pragma unroll
+            for (int r = 0; r < 8; ++r) {
+                const int q = r >> 1, sh = (r & 1) * 16;
+                const uint32_t a0 = (st.v[4 * h][q] >> sh) & 0xffffu, a1 = (st.v[4 * h + 1][q] >> sh) & 0xffffu;
+                const uint32_t a2 = (st.v[4 * h + 2][q] >> sh) & 0xffffu, a3 = (st.v[4 * h + 3][q] >> sh) & 0xffffu;
+                u32x2 o;
+                o.x = a0 | (a1 << 16);
+                o.y = a2 | (a3 << 16);
+                *reinterpret_cast<u32x2*>(dst + (rb + r) * kLDT + kb) = o;
+            }
+        }
+    }
+}
+
+// One output element; returns the stored bf16 value as a float for EPI 2
+// (the bias gradient sums it), 0 otherwise.
+template <int EPI>
+__device__ __forceinline__ float gemm_epilogue(const GemmArgs& g, int e, int row, int col, float v) {
+    if (EPI == 0) {
+        g.C[e * g.sce + (int64_t)row * g.ldc + col] = v;
+    } else if (EPI == 1 || EPI == 3) {
+        const float z = v + g.bias[e * g.sbias + col];
+        if (EPI == 1) g.C[e * g.sce + (int64_t)row * g.ldc + col] = z;
+        g.H[e * g.she + (int64_t)row * g.ldh + col] = f2bf(gelu(z));
+    } else {
+        const float zp = g.Zp[e * g.sze + (int64_t)row * g.ldz + col];
+        const uint16_t h = f2bf(v * gelu_grad(zp));
+        g.H[e * g.she + (int64_t)row * g.ldh + col] = h;
+        return bf2f(h);
+    }
+    return 0.0f;
+}
+
+template <int EPI, bool AK, bool BK, int WNB>
+__global__ void __launch_bounds__(256, WNB == 2 ? 2 : 1) kg_gemm2(GemmArgs g) {
+    constexpr int TN = 64 * WNB;
+    __shared__ __attribute__((aligned(16))) uint16_t sA[2][kTM * kLDT];
+    __shared__ __attribute__((aligned(16))) uint16_t sB[2][TN * kLDT];
+    const int e = blockIdx.z;
+    const int m0 = blockIdx.y * kTM, n0 = blockIdx.x * TN;
+    const uint16_t* A = g.A + e * g.sae;
+    const uint16_t* B = g.B + e * g.sbe;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave >> 1, wn = wave & 1;  // the wave's 64 x (32 WNB) quadrant
+    floatx16 acc[2][WNB];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < WNB; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+    const int nk = g.K / kTK;
+    Stage<kTM> pa;
+    Stage<TN> pb;
+    stage_load<AK, kTM>(A, g.sam, g.sak, m0, 0, pa);
+    stage_load<BK, TN>(B, g.sbn, g.sbk, n0, 0, pb);
+    stage_store<AK, kTM>(pa, sA[0]);
+    stage_store<BK, TN>(pb, sB[0]);
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        if (kt + 1 < nk) {
+            stage_load<AK, kTM>(A, g.sam, g.sak, m0, (kt + 1) * kTK, pa);
+            stage_load<BK, TN>(B, g.sbn, g.sbk, n0, (kt + 1) * kTK, pb);
+        }
+        const uint16_t* a_s = sA[cur] + (wm * 64 + (lane & 31)) * kLDT + 8 * (lane >> 5);
+        const uint16_t* b_s = sB[cur] + (wn * 32 * WNB + (lane & 31)) * kLDT + 8 * (lane >> 5);
+#pragma unroll
+        for (int kk = 0; kk < kTK; kk += 16) {
+            // lane l supplies row (l mod 32), k = kk + 8 (l div 32) .. +7
+            bf16x8 a[2], b[WNB];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) a[i] = *reinterpret_cast<const bf16x8*>(a_s + i * 32 * kLDT + kk);
+#pragma unroll
+            for (int j = 0; j < WNB; ++j) b[j] = *reinterpret_cast<const bf16x8*>(b_s + j * 32 * kLDT + kk);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < WNB; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        if (kt + 1 < nk) {
+            stage_store<AK, kTM>(pa, sA[cur ^ 1]);
+            stage_store<BK, TN>(pb, sB[cur ^ 1]);
+        }
+        __syncthreads();
+    }
+    // accumulator r of lane l: row 8 (r div 4) + 4 (l div 32) + (r mod 4), column l mod 32
+    float dbs[WNB];
+#pragma unroll
+    for (int j = 0; j < WNB; ++j) dbs[j] = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < WNB; ++j) {
+            const int col = n0 + wn * 32 * WNB + j * 32 + (lane & 31);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                dbs[j] += gemm_epilogue<EPI>(g, e, row, col, acc[i][j][r]);
+            }
+        }
+    if (EPI == 2 && g.db) {
+#pragma unroll
+        for (int j = 0; j < WNB; ++j) {
+            const int col = n0 + wn * 32 * WNB + j * 32 + (lane & 31);
+            const float d = dbs[j] + __shfl_xor(dbs[j], 32, 64);  // lanes l and l + 32 hold the same column
+            if (lane < 32) atomicAdd(g.db + e * g.sdb + col, d);
         }
     }
 }
@@ -168,62 +352,66 @@ __global__ void kg_out_fwd(const uint16_t* __restrict__ A, int H, int Bsz, const
     if (lane == 0) y[(int64_t)e * Bsz + b] = s + bo[e * sbo];
 }
 
-// dy = 2 (y - t) / B (mean squared error per member), loss += (y - t)^2 / B.
+// dy = 2 (y - t) / B (mean squared error per member), loss += (y - t)^2 / B,
+// the output bias gradient db_o += sum_b dy and the epoch's loss sum.
 __global__ void kg_loss(const float* __restrict__ y, const float* __restrict__ t, int Bsz, float* __restrict__ dy,
-                        float* __restrict__ loss) {
+                        float* __restrict__ loss, float* __restrict__ dbo, int64_t sdbo, double* __restrict__ epoch) {
     const int e = blockIdx.y;
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    float l = 0.0f;
+    float l = 0.0f, d = 0.0f;
     if (b < Bsz) {
         const float r = y[(int64_t)e * Bsz + b] - t[(int64_t)e * Bsz + b];
-        dy[(int64_t)e * Bsz + b] = 2.0f * r / (float)Bsz;
+        d = 2.0f * r / (float)Bsz;
+        dy[(int64_t)e * Bsz + b] = d;
         l = r * r / (float)Bsz;
     }
 #pragma unroll
-    for (int d = 32; d > 0; d >>= 1) l += __shfl_xor(l, d, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(loss + e, l);
+    for (int s = 32; s > 0; s >>= 1) {
+        l += __shfl_xor(l, s, 64);
+        d += __shfl_xor(d, s, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(loss + e, l);
+        atomicAdd(dbo + e * sdbo, d);
+        atomicAdd(epoch + e, (double)l);
+    }
 }
 
-// dZ[e][b][o] = bf16(dy[e][b] * w[e][o] * gelu'(Z[e][b][o]))
-__global__ void kg_dz_out(const float* __restrict__ dy, const float* __restrict__ w, int64_t swe,
-                          const float* __restrict__ Z, int H, int Bsz, uint16_t* __restrict__ dZ) {
-    const int e = blockIdx.y;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)Bsz * H) return;
-    const int b = (int)(i / H), o = (int)(i % H);
-    const int64_t q = (int64_t)e * Bsz * H + i;
-    dZ[q] = f2bf(dy[(int64_t)e * Bsz + b] * w[e * swe + o] * gelu_grad(Z[q]));
-}
-
-// out[e][o] += sum over rows b in this block's chunk of U[e][b][o] * (v ? v[e][b] : 1)
-__global__ void kg_colsum(const uint16_t* __restrict__ U, const float* __restrict__ v, int H, int Bsz, int rows_per,
-                          float* __restrict__ out, int64_t soe) {
+// The output layer's backward in one pass over [B][H] per member: dZ_L =
+// bf16(dy w_o gelu'(Z_L)), dw_o[o] += sum_b dy[b] A_L[b][o] and the last
+// hidden bias gradient db[o] += sum_b dZ_L[b][o] (of the stored bf16 values).
+__global__ void kg_out_bwd(const float* __restrict__ dy, const float* __restrict__ w, int64_t swe,
+                           const float* __restrict__ Z, const uint16_t* __restrict__ A, int H, int Bsz, int rows_per,
+                           uint16_t* __restrict__ dZ, float* __restrict__ gwo, float* __restrict__ gb, int64_t sg) {
     const int e = blockIdx.z;
     const int o = blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= H) return;
     const int b0 = blockIdx.y * rows_per;
     const int b1 = min(Bsz, b0 + rows_per);
-    float s = 0.0f;
+    const float wo = w[e * swe + o];
+    float sw = 0.0f, sb = 0.0f;
     for (int b = b0; b < b1; ++b) {
-        const float u = bf2f(U[((int64_t)e * Bsz + b) * H + o]);
-        s += v ? u * v[(int64_t)e * Bsz + b] : u;
+        const float d = dy[(int64_t)e * Bsz + b];
+        const int64_t q = ((int64_t)e * Bsz + b) * H + o;
+        sw += bf2f(A[q]) * d;
+        const uint16_t h = f2bf(d * wo * gelu_grad(Z[q]));
+        dZ[q] = h;
+        sb += bf2f(h);
     }
-    atomicAdd(out + e * soe + o, s);
+    atomicAdd(gwo + e * sg + o, sw);
+    atomicAdd(gb + e * sg + o, sb);
 }
 
-// out[e] += sum_b v[e][b]
-__global__ void kg_sum(const float* __restrict__ v, int Bsz, float* __restrict__ out, int64_t soe) {
-    const int e = blockIdx.y;
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    float s = b < Bsz ? v[(int64_t)e * Bsz + b] : 0.0f;
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(out + e * soe, s);
-}
-
-// Adam (Kingma & Ba 2015) over the flat FP32 parameter array.
-__global__ void kg_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
-                        int64_t n, float lr, float b1, float b2, float eps, float c1, float c2) {
+// Adam and the bf16 copies of the hidden weight matrices in one pass: the
+// master layout of W_l[H][K_l] per member is the bf16 copy's layout.
+struct CastMap {
+    int L;
+    int64_t off[16], size[16];
+    uint16_t* dst[16];
+};
+__global__ void kg_adam_cast(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                             float* __restrict__ v, int64_t n, int64_t P, float lr, float b1, float b2, float eps,
+                             float c1, float c2, CastMap cm) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float gi = g[i];
@@ -231,7 +419,11 @@ __global__ void kg_adam(float* __restrict__ p, const float* __restrict__ g, floa
     const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
     m[i] = mi;
     v[i] = vi;
-    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    p[i] = pi;
+    const int64_t e = i / P, off = i - e * P;
+    for (int l = 0; l < cm.L; ++l)
+        if (off >= cm.off[l] && off < cm.off[l] + cm.size[l]) cm.dst[l][e * cm.size[l] + (off - cm.off[l])] = f2bf(pi);
 }
 
 // bf16 copy of a weight block [rows][cols] (row stride ld in the master).
@@ -312,6 +504,7 @@ struct pp_mlp {
     float* y = nullptr;
     float* dy = nullptr;
     float* loss = nullptr;
+    double* loss_acc = nullptr;  // [E] per-epoch sum of the minibatch losses
     // inference workspace
     int64_t cap_q = 0;
     uint16_t* Xq = nullptr;
@@ -333,7 +526,7 @@ struct pp_mlp {
         F(X); F(Y); F(perm); F(Xb); F(Yb);
         for (auto q : Z) F(q);
         for (auto q : A) F(q);
-        F(dZa); F(dZb); F(y); F(dy); F(loss);
+        F(dZa); F(dZb); F(y); F(dy); F(loss); F(loss_acc);
         F(Xq); F(Qa); F(Qb); F(yq); F(xq_f); F(mq); F(sq);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -362,6 +555,7 @@ struct pp_mlp {
         MCHK(hipMemsetAsync(v, 0, sizeof(float) * E * P, stream));
         for (int l = 0; l < L; ++l) Wb.push_back(mal<uint16_t>((size_t)E * H * K(l)));
         loss = mal<float>(E);
+        loss_acc = mal<double>(E);
         MCHK(hipStreamSynchronize(stream));
     }
 
@@ -401,11 +595,31 @@ struct pp_mlp {
     template <int EPI>
     void gemm(const uint16_t* Aop, int64_t sam, int64_t sak, int64_t sae, const uint16_t* Bop, int64_t sbn, int64_t sbk,
               int64_t sbe, int M, int N, int Kd, float* C, int64_t ldc, int64_t sce, uint16_t* Hout, int64_t ldh,
-              int64_t she, const float* bias, int64_t sbias, const float* Zp, int64_t ldz, int64_t sze) {
+              int64_t she, const float* bias, int64_t sbias, const float* Zp, int64_t ldz, int64_t sze,
+              float* db = nullptr, int64_t sdb = 0) {
         if (M % kBM || N % kBN || Kd % kBK) throw MlpError{PP_EARG, "GEMM shape not a multiple of the tile"};
         if ((sak != 1 && sam != 1) || (sbk != 1 && sbn != 1)) throw MlpError{PP_EARG, "GEMM operand not contiguous"};
-        GemmArgs a{Aop, sam, sak, sae, Bop, sbn, sbk, sbe, M, N, Kd, C, ldc, sce, Hout, ldh, she, bias, sbias, Zp, ldz, sze};
-        hipLaunchKernelGGL(kg_gemm<EPI>, dim3(N / kBN, M / kBM, E), dim3(256), 0, stream, a);
+        GemmArgs a{Aop, sam, sak, sae, Bop, sbn, sbk, sbe, M, N, Kd, C, ldc, sce, Hout, ldh, she, bias, sbias, Zp, ldz, sze,
+                   db, sdb};
+        if (M % kTM == 0 && N % 128 == 0 && Kd % kTK == 0) {
+            // 128 x 128 tiles: 128 x 256 (WNB = 4) holds one workgroup per CU
+            // and measured 20% slower on 65536-row inference
+            auto launch = [&](auto wnb) {
+                constexpr int WNB = decltype(wnb)::value;
+                const dim3 grid(N / (64 * WNB), M / kTM, E);
+                if (sak == 1 && sbk == 1)
+                    hipLaunchKernelGGL((kg_gemm2<EPI, true, true, WNB>), grid, dim3(256), 0, stream, a);
+                else if (sak == 1)
+                    hipLaunchKernelGGL((kg_gemm2<EPI, true, false, WNB>), grid, dim3(256), 0, stream, a);
+                else if (sbk == 1)
+                    hipLaunchKernelGGL((kg_gemm2<EPI, false, true, WNB>), grid, dim3(256), 0, stream, a);
+                else
+                    hipLaunchKernelGGL((kg_gemm2<EPI, false, false, WNB>), grid, dim3(256), 0, stream, a);
+            };
+            launch(std::integral_constant<int, 2>());
+        } else {
+            hipLaunchKernelGGL(kg_gemm<EPI>, dim3(N / kBN, M / kBM, E), dim3(256), 0, stream, a);
+        }
         MCHK(hipGetLastError());
     }
 
@@ -428,15 +642,15 @@ struct pp_mlp {
     void backward(int Bsz) {
         MCHK(hipMemsetAsync(g, 0, sizeof(float) * E * P, stream));
         MCHK(hipMemsetAsync(loss, 0, sizeof(float) * E, stream));
-        hipLaunchKernelGGL(kg_loss, dim3((unsigned)cdiv64(Bsz, 256), E), dim3(256), 0, stream, y, Yb, Bsz, dy, loss);
-        // output layer: dw_o = sum_b dy A_L, db_o = sum_b dy
+        // dy, loss, db_o and the epoch loss; then the output layer's dZ with
+        // dw_o and the last hidden layer's bias gradient
+        hipLaunchKernelGGL(kg_loss, dim3((unsigned)cdiv64(Bsz, 256), E), dim3(256), 0, stream, y, Yb, Bsz, dy, loss,
+                           g + offbo, P, loss_acc);
         const int rows = 64;
-        hipLaunchKernelGGL(kg_colsum, dim3((unsigned)cdiv64(H, 256), (unsigned)cdiv64(Bsz, rows), E), dim3(256), 0,
-                           stream, A[L - 1], dy, H, Bsz, rows, g + offwo, P);
-        hipLaunchKernelGGL(kg_sum, dim3((unsigned)cdiv64(Bsz, 256), E), dim3(256), 0, stream, dy, Bsz, g + offbo, P);
         const int64_t act = (int64_t)Bsz * H;
-        hipLaunchKernelGGL(kg_dz_out, dim3((unsigned)cdiv64(act, 256), E), dim3(256), 0, stream, dy, p + offwo, P,
-                           Z[L - 1], H, Bsz, dZa);
+        hipLaunchKernelGGL(kg_out_bwd, dim3((unsigned)cdiv64(H, 256), (unsigned)cdiv64(Bsz, rows), E), dim3(256), 0,
+                           stream, dy, p + offwo, P, Z[L - 1], A[L - 1], H, Bsz, rows, dZa, g + offwo,
+                           g + offb[L - 1], P);
         MCHK(hipGetLastError());
         uint16_t* dz = dZa;
         uint16_t* dzn = dZb;
@@ -446,13 +660,11 @@ struct pp_mlp {
             // dW_l[o][i] = sum_b dZ[b][o] in[b][i]: M = H (o), N = K_l (i), K = B
             gemm<0>(dz, 1, H, act, in, 1, ld_in, (int64_t)Bsz * ld_in, H, K(l), Bsz, g + offW[l], K(l), P, nullptr, 0,
                     0, nullptr, 0, nullptr, 0, 0);
-            hipLaunchKernelGGL(kg_colsum, dim3((unsigned)cdiv64(H, 256), (unsigned)cdiv64(Bsz, rows), E), dim3(256), 0,
-                               stream, dz, (const float*)nullptr, H, Bsz, rows, g + offb[l], P);
-            MCHK(hipGetLastError());
             if (l > 0) {
-                // dZ_{l-1}[b][i] = (sum_o dZ[b][o] W_l[o][i]) gelu'(Z_{l-1}[b][i]): M = B, N = H (i), K = H (o)
+                // dZ_{l-1}[b][i] = (sum_o dZ[b][o] W_l[o][i]) gelu'(Z_{l-1}[b][i]): M = B, N = H (i), K = H (o);
+                // its epilogue adds db_{l-1} = sum_b dZ_{l-1}[b][:]
                 gemm<2>(dz, H, 1, act, Wb[l], 1, K(l), (int64_t)H * K(l), Bsz, H, H, nullptr, 0, 0, dzn, H, act,
-                        nullptr, 0, Z[l - 1], H, act);
+                        nullptr, 0, Z[l - 1], H, act, g + offb[l - 1], P);
                 std::swap(dz, dzn);
             }
         }
@@ -464,10 +676,16 @@ struct pp_mlp {
         const float c1 = 1.0f - (float)std::pow((double)b1, (double)step);
         const float c2 = 1.0f - (float)std::pow((double)b2, (double)step);
         const int64_t cnt = (int64_t)E * P;
-        hipLaunchKernelGGL(kg_adam, dim3((unsigned)cdiv64(cnt, 256)), dim3(256), 0, stream, p, g, m, v, cnt, lr, b1, b2,
-                           eps, c1, c2);
+        CastMap cm{};
+        cm.L = L;
+        for (int l = 0; l < L; ++l) {
+            cm.off[l] = offW[l];
+            cm.size[l] = (int64_t)H * K(l);
+            cm.dst[l] = Wb[l];
+        }
+        hipLaunchKernelGGL(kg_adam_cast, dim3((unsigned)cdiv64(cnt, 256)), dim3(256), 0, stream, p, g, m, v, cnt, P, lr,
+                           b1, b2, eps, c1, c2, cm);
         MCHK(hipGetLastError());
-        cast_weights();
     }
 
     void ensure_query(int64_t cq) {
@@ -654,17 +872,18 @@ int pp_mlp_train_epoch(pp_mlp* h, const int32_t* perm, int batch, float lr, floa
     h->ensure_batch(batch);
     const int64_t steps = h->n / batch;
     std::vector<double> acc((size_t)h->E, 0.0);
-    std::vector<float> l((size_t)h->E);
+    // the steps are enqueued back to back; the losses are summed on the device
+    // and read once at the end of the epoch
+    MCHK(hipMemsetAsync(h->loss_acc, 0, sizeof(double) * h->E, h->stream));
     for (int64_t s = 0; s < steps; ++s) {
         hipLaunchKernelGGL(kg_gather, dim3((unsigned)cdiv64(batch, 4), h->E), dim3(256), 0, h->stream, h->X, h->Y,
                            h->perm, h->n, s * batch, batch, h->Xb, h->Yb);
         h->forward_train(batch);
         h->backward(batch);
         h->adam(lr);
-        MCHK(hipMemcpyAsync(l.data(), h->loss, sizeof(float) * h->E, hipMemcpyDeviceToHost, h->stream));
-        MCHK(hipStreamSynchronize(h->stream));
-        for (int e = 0; e < h->E; ++e) acc[e] += l[e];
     }
+    MCHK(hipMemcpyAsync(acc.data(), h->loss_acc, sizeof(double) * h->E, hipMemcpyDeviceToHost, h->stream));
+    MCHK(hipStreamSynchronize(h->stream));
     for (int e = 0; e < h->E; ++e) loss_out[e] = (float)(acc[e] / (double)std::max<int64_t>(steps, 1));
     MLP_END
 }

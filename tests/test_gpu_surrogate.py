@@ -28,7 +28,8 @@ def rel(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
 
 
-@pytest.mark.parametrize("E,din,H,L,B", [(2, 20, 64, 3, 128), (5, 20, 512, 3, 256), (1, 7, 128, 1, 64)])
+@pytest.mark.parametrize("E,din,H,L,B", [(2, 20, 64, 3, 128), (5, 20, 512, 3, 256), (1, 7, 128, 1, 64),
+                                         (3, 33, 256, 2, 384)])
 def test_forward_and_gradients_match_oracle(E, din, H, L, B):
     log(f"mlp parity E={E} din={din} H={H} L={L} B={B}")
     ens = sg.MLPEnsemble(E=E, din=din, hidden=H, layers=L, seed=11)
@@ -57,6 +58,22 @@ def test_forward_and_gradients_match_oracle(E, din, H, L, B):
         assert abs(bod - bor) <= 1e-4 * max(abs(bor), 1e-6)
     assert np.allclose(mean, mem.astype(np.float64).mean(axis=0), atol=1e-6)
     assert np.allclose(sd, mem.astype(np.float64).std(axis=0), atol=1e-6)
+
+
+def test_large_batch_inference_matches_oracle():
+    """8192 query rows take the wide (128 x 256) GEMM tiles; every member's
+    prediction against the oracle's forward pass."""
+    E, din, H, L = 5, 40, 512, 3
+    ens = sg.MLPEnsemble(E=E, din=din, hidden=H, layers=L, seed=3)
+    orc = mo.MLPOracle(din, H, L)
+    p = ens.get_params()
+    x = np.random.default_rng(8).uniform(-1, 1, size=(8192, din)).astype(np.float32)
+    mean, sd, mem, _ = ens.predict(x, members=True)
+    for e in (0, E - 1):
+        y_ref, _ = orc.forward(p[e], x)
+        scale = max(1e-3, float(np.max(np.abs(y_ref))))
+        assert np.max(np.abs(mem[e] - y_ref)) <= 2e-3 * scale, e
+    assert np.allclose(mean, mem.astype(np.float64).mean(axis=0), atol=1e-6)
 
 
 def test_set_get_params_roundtrip_and_padding():
