@@ -56,9 +56,29 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+// GELU through erf by Abramowitz & Stegun, Handbook of Mathematical
+// Functions, 7.1.26 (|error| <= 1.5e-7 on the real line): for u >= 0,
+// erf(u) = 1 - P(t) exp(-u^2), P(t) = a1 t + a2 t^2 + a3 t^3 + a4 t^4 + a5 t^5,
+// t = 1 / (1 + p u); erf is odd.  With u = |x| / sqrt 2 and E = exp(-x^2 / 2):
+//   gelu(x) = 0.5 x (1 + erf(x / sqrt 2)) = max(x, 0) - 0.5 |x| P E
+//   gelu'(x) = 0.5 (1 + erf(x / sqrt 2)) + x E / sqrt(2 pi)
+//            = [x >= 0] - sign(x) 0.5 P E + x E / sqrt(2 pi)
+// (no cancellation for negative x).  The error is far below the bf16
+// rounding (2^-9 relative) that every activation and gradient goes through.
+// half_p(x) = 0.5 P(t) at u = |x| / sqrt 2 (coefficients pre-halved).
+__device__ __forceinline__ float half_p(float x) {
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f * 0.70710678118654752f, fabsf(x), 1.0f));
+    float q = fmaf(0.5f * 1.061405429f, t, 0.5f * -1.453152027f);
+    q = fmaf(q, t, 0.5f * 1.421413741f);
+    q = fmaf(q, t, 0.5f * -0.284496736f);
+    q = fmaf(q, t, 0.5f * 0.254829592f);
+    return q * t;
+}
+__device__ __forceinline__ float gauss(float x) { return __expf(-0.5f * x * x); }  // exp(-x^2 / 2)
+__device__ __forceinline__ float gelu(float x) { return fmaf(-fabsf(x) * half_p(x), gauss(x), fmaxf(x, 0.0f)); }
 __device__ __forceinline__ float gelu_grad(float x) {
-    return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
+    const float hp = half_p(x);
+    return fmaf(fmaf(x, 0.39894228040143268f, copysignf(hp, -x)), gauss(x), x >= 0.0f ? 1.0f : 0.0f);
 }
 
 struct GemmArgs {
@@ -67,9 +87,13 @@ struct GemmArgs {
     const uint16_t* B;
     int64_t sbn, sbk, sbe;
     int M, N, K;
-    float* C;  // EPI 0: C = acc; EPI 1: C = acc + bias (pre-activation)
+    // EPI 0: C = acc (weight gradient).  EPI 1 (training forward) and 5 (its
+    // last hidden layer): C = acc + bias (pre-activation), H = gelu of it.
+    // EPI 2: H = acc * gelu'(Zp) (back-propagated dZ).  EPI 3 (inference):
+    // H only.  EPI 4 (inference, last hidden layer): neither.
+    float* C;
     int64_t ldc, sce;
-    uint16_t* H;  // EPI 1: gelu(acc + bias); EPI 2: acc * gelu'(Zp); EPI 3: gelu(acc + bias), no C
+    uint16_t* H;
     int64_t ldh, she;
     const float* bias;
     int64_t sbias;
@@ -77,7 +101,37 @@ struct GemmArgs {
     int64_t ldz, sze;
     float* db;  // EPI 2: db[col] += the column's sum of the bf16 outputs (bias gradient)
     int64_t sdb;
+    // EPI 4 / 5: the output layer fused in, yp[e][slot][row] = sum over the
+    // slot's columns of bf16(gelu(z)) wo[col], one slot per wave's column
+    // range
+    float* yp;
+    int64_t syp;
+    const float* wo;
+    int64_t swo;
 };
+
+// Sums N per-lane values over the 32 lanes of each half wave (N = 16 or 32)
+// by halving the held values at each exchange with lane ^ M (M = 16, 8, ..,
+// 1): afterwards v[0] of lane l is the total of value index (l mod 32) >>
+// (5 - log2 N), complete in the lanes whose low 5 - log2 N bits are zero.
+template <int N, int M>
+__device__ __forceinline__ void half_wave_transpose_sum(float* v, int lane) {
+    if constexpr (M >= 1) {
+        if constexpr (N > 1) {
+            const bool hi = (lane & M) != 0;
+#pragma unroll
+            for (int k = 0; k < N / 2; ++k) {
+                const float keep = hi ? v[k + N / 2] : v[k];
+                const float send = hi ? v[k] : v[k + N / 2];
+                v[k] = keep + __shfl_xor(send, M, 64);
+            }
+            half_wave_transpose_sum<N / 2, M / 2>(v, lane);
+        } else {
+            v[0] += __shfl_xor(v[0], M, 64);
+            half_wave_transpose_sum<1, M / 2>(v, lane);
+        }
+    }
+}
 
 // Stage a 64 x 32 tile (rows r0.., k0..) of a bf16 operand into LDS as
 // dst[row][k].  Either the k stride or the row stride is 1 (checked by the
@@ -97,6 +151,11 @@ __device__ __forceinline__ void stage_tile(const uint16_t* __restrict__ src, int
         for (int j = 0; j < 8; ++j) dst[(rc + j) * kLDK + k] = h[j];
     }
 }
+
+constexpr bool epi_act(int epi) { return epi == 1 || epi >= 3; }  // bias + GELU
+constexpr bool epi_keep_z(int epi) { return epi == 1 || epi == 5; }
+constexpr bool epi_keep_h(int epi) { return epi == 1 || epi == 3 || epi == 5; }
+constexpr bool epi_out(int epi) { return epi >= 4; }
 
 template <int EPI>
 __global__ void __launch_bounds__(256) kg_gemm(GemmArgs g) {
@@ -127,16 +186,21 @@ __global__ void __launch_bounds__(256) kg_gemm(GemmArgs g) {
     // accumulator r of lane l: row 8 (r div 4) + 4 (l div 32) + (r mod 4), column l mod 32
     const int col = n0 + wn * 32 + (lane & 31);
     float dbs = 0.0f;
+    float ys[16];
+    const float wcol = epi_out(EPI) ? g.wo[e * g.swo + col] : 0.0f;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int row = m0 + wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
         const float v = acc[r];
+        ys[r] = 0.0f;
         if (EPI == 0) {
             g.C[e * g.sce + (int64_t)row * g.ldc + col] = v;
-        } else if (EPI == 1 || EPI == 3) {
+        } else if (epi_act(EPI)) {
             const float z = v + g.bias[e * g.sbias + col];
-            if (EPI == 1) g.C[e * g.sce + (int64_t)row * g.ldc + col] = z;
-            g.H[e * g.she + (int64_t)row * g.ldh + col] = f2bf(gelu(z));
+            if (epi_keep_z(EPI)) g.C[e * g.sce + (int64_t)row * g.ldc + col] = z;
+            const uint16_t h = f2bf(gelu(z));
+            if (epi_keep_h(EPI)) g.H[e * g.she + (int64_t)row * g.ldh + col] = h;
+            ys[r] = bf2f(h) * wcol;
         } else {
             const float zp = g.Zp[e * g.sze + (int64_t)row * g.ldz + col];
             const uint16_t h = f2bf(v * gelu_grad(zp));
@@ -148,14 +212,24 @@ __global__ void __launch_bounds__(256) kg_gemm(GemmArgs g) {
         dbs += __shfl_xor(dbs, 32, 64);  // lanes l and l + 32 hold the same column
         if (lane < 32) atomicAdd(g.db + e * g.sdb + col, dbs);
     }
+    if (epi_out(EPI)) {
+        // slot = the wave's 32 columns; value r of lane l is row 8 (r div 4) + 4 (l div 32) + (r mod 4)
+        half_wave_transpose_sum<16, 16>(ys, lane);
+        const float t = ys[0];
+        if ((lane & 1) == 0) {
+            const int r = (lane & 31) >> 1;
+            const int row = m0 + wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            g.yp[e * g.syp + (int64_t)((n0 + wn * 32) / 32) * g.M + row] = t;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
 // kg_gemm2: the same product and epilogues for shapes that are multiples of
-// 128 (M), 64 WNB (N) and 64 (K).  A workgroup computes a 128 x (64 WNB) tile
-// with four waves of 64 x (32 WNB) (2 x WNB blocks of
-// v_mfma_f32_32x32x16_bf16: each A fragment used WNB times, each B fragment
-// twice); K advances 64 per stage through two LDS buffers: the next stage's
+// 128 (M), 128 (N) and 64 (K).  A workgroup computes a 128 x 128 tile with
+// four waves of 64 x 64 (2 x 2 blocks of v_mfma_f32_32x32x16_bf16, each
+// fragment used twice; a 128 x 256 tile holds one workgroup per CU and
+// measured 20% slower on 65536-row inference); K advances 64 per stage through two LDS buffers: the next stage's
 // global loads are issued into registers before the current stage's MFMAs and
 // written to the other buffer after them, one barrier per stage.  AK / BK: the
 // operand's k stride is 1 (rows of 8 contiguous k per 16-byte load);
@@ -226,28 +300,12 @@ __device__ __forceinline__ void stage_store(const Stage<ROWS>& st, uint16_t* __r
     }
 }
 
-// One output element; returns the stored bf16 value as a float for EPI 2
-// (the bias gradient sums it), 0 otherwise.
-template <int EPI>
-__device__ __forceinline__ float gemm_epilogue(const GemmArgs& g, int e, int row, int col, float v) {
-    if (EPI == 0) {
-        g.C[e * g.sce + (int64_t)row * g.ldc + col] = v;
-    } else if (EPI == 1 || EPI == 3) {
-        const float z = v + g.bias[e * g.sbias + col];
-        if (EPI == 1) g.C[e * g.sce + (int64_t)row * g.ldc + col] = z;
-        g.H[e * g.she + (int64_t)row * g.ldh + col] = f2bf(gelu(z));
-    } else {
-        const float zp = g.Zp[e * g.sze + (int64_t)row * g.ldz + col];
-        const uint16_t h = f2bf(v * gelu_grad(zp));
-        g.H[e * g.she + (int64_t)row * g.ldh + col] = h;
-        return bf2f(h);
-    }
-    return 0.0f;
-}
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
-template <int EPI, bool AK, bool BK, int WNB>
-__global__ void __launch_bounds__(256, WNB == 2 ? 2 : 1) kg_gemm2(GemmArgs g) {
-    constexpr int TN = 64 * WNB;
+template <int EPI, bool AK, bool BK>
+__global__ void __launch_bounds__(256, 2) kg_gemm2(GemmArgs g) {
+    constexpr int TN = 128;
     __shared__ __attribute__((aligned(16))) uint16_t sA[2][kTM * kLDT];
     __shared__ __attribute__((aligned(16))) uint16_t sB[2][TN * kLDT];
     const int e = blockIdx.z;
@@ -255,12 +313,12 @@ __global__ void __launch_bounds__(256, WNB == 2 ? 2 : 1) kg_gemm2(GemmArgs g) {
     const uint16_t* A = g.A + e * g.sae;
     const uint16_t* B = g.B + e * g.sbe;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = wave >> 1, wn = wave & 1;  // the wave's 64 x (32 WNB) quadrant
-    floatx16 acc[2][WNB];
+    const int wm = wave >> 1, wn = wave & 1;  // the wave's 64 x 64 quadrant
+    floatx16 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < WNB; ++j)
+        for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
     const int nk = g.K / kTK;
@@ -278,20 +336,22 @@ __global__ void __launch_bounds__(256, WNB == 2 ? 2 : 1) kg_gemm2(GemmArgs g) {
             stage_load<BK, TN>(B, g.sbn, g.sbk, n0, (kt + 1) * kTK, pb);
         }
         const uint16_t* a_s = sA[cur] + (wm * 64 + (lane & 31)) * kLDT + 8 * (lane >> 5);
-        const uint16_t* b_s = sB[cur] + (wn * 32 * WNB + (lane & 31)) * kLDT + 8 * (lane >> 5);
+        const uint16_t* b_s = sB[cur] + (wn * 64 + (lane & 31)) * kLDT + 8 * (lane >> 5);
 #pragma unroll
         for (int kk = 0; kk < kTK; kk += 16) {
             // lane l supplies row (l mod 32), k = kk + 8 (l div 32) .. +7
-            bf16x8 a[2], b[WNB];
+            bf16x8 a[2], b[2];
 #pragma unroll
             for (int i = 0; i < 2; ++i) a[i] = *reinterpret_cast<const bf16x8*>(a_s + i * 32 * kLDT + kk);
 #pragma unroll
-            for (int j = 0; j < WNB; ++j) b[j] = *reinterpret_cast<const bf16x8*>(b_s + j * 32 * kLDT + kk);
+            for (int j = 0; j < 2; ++j) b[j] = *reinterpret_cast<const bf16x8*>(b_s + j * 32 * kLDT + kk);
+            // B rows first: the product block is C transposed, so a lane's
+            // accumulators run along a row of C (four adjacent columns each)
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int j = 0; j < WNB; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[j], a[i], acc[i][j], 0, 0, 0);
         }
         if (kt + 1 < nk) {
             stage_store<AK, kTM>(pa, sA[cur ^ 1]);
@@ -299,27 +359,87 @@ __global__ void __launch_bounds__(256, WNB == 2 ? 2 : 1) kg_gemm2(GemmArgs g) {
         }
         __syncthreads();
     }
-    // accumulator r of lane l: row 8 (r div 4) + 4 (l div 32) + (r mod 4), column l mod 32
-    float dbs[WNB];
+    // accumulator 4 c + q of lane l in block (i, j): row 64 wm + 32 i + (l mod
+    // 32), column 64 wn + 32 j + 8 c + 4 (l div 32) + q of the tile.  bf16
+    // outputs go through LDS (the A buffers, free after the last barrier) so
+    // that the global stores are whole 256-byte rows.
+    constexpr bool kOut = epi_out(EPI);
+    constexpr bool kH = epi_keep_h(EPI) || EPI == 2;
+    constexpr int kLDH = 136;  // bf16 per LDS row of the output tile (272 B)
+    static_assert(kTM * kLDH <= 2 * kTM * kLDT, "output tile exceeds the A buffers");
+    uint16_t* sH = &sA[0][0];
+    float ys[2] = {0.0f, 0.0f};  // fused output layer: this lane's row sums
+    float cs[32];                // EPI 2: column sums over this lane's rows
 #pragma unroll
-    for (int j = 0; j < WNB; ++j) dbs[j] = 0.0f;
+    for (int q = 0; q < 32; ++q) cs[q] = 0.0f;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 2; ++i) {
+        const int row = m0 + wm * 64 + i * 32 + (lane & 31);
 #pragma unroll
-        for (int j = 0; j < WNB; ++j) {
-            const int col = n0 + wn * 32 * WNB + j * 32 + (lane & 31);
+        for (int j = 0; j < 2; ++j)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int row = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                dbs[j] += gemm_epilogue<EPI>(g, e, row, col, acc[i][j][r]);
+            for (int c = 0; c < 4; ++c) {
+                const int col = n0 + wn * 64 + j * 32 + 8 * c + 4 * (lane >> 5);
+                floatx4 v;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) v[q] = acc[i][j][4 * c + q];
+                if constexpr (EPI == 0) {
+                    *reinterpret_cast<floatx4*>(g.C + e * g.sce + (int64_t)row * g.ldc + col) = v;
+                } else if constexpr (epi_act(EPI)) {
+                    const floatx4 z = v + *reinterpret_cast<const floatx4*>(g.bias + e * g.sbias + col);
+                    if constexpr (epi_keep_z(EPI))
+                        *reinterpret_cast<floatx4*>(g.C + e * g.sce + (int64_t)row * g.ldc + col) = z;
+                    floatx4 a;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) a[q] = gelu(z[q]);
+                    const bf16x4 h = __builtin_convertvector(a, bf16x4);
+                    if constexpr (epi_keep_h(EPI))
+                        *reinterpret_cast<bf16x4*>(sH + (row - m0) * kLDH + col - n0) = h;
+                    if constexpr (kOut) {
+                        const floatx4 w = *reinterpret_cast<const floatx4*>(g.wo + e * g.swo + col);
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) ys[i] += (float)h[q] * w[q];
+                    }
+                } else {
+                    const floatx4 zp = *reinterpret_cast<const floatx4*>(g.Zp + e * g.sze + (int64_t)row * g.ldz + col);
+                    floatx4 d;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) d[q] = v[q] * gelu_grad(zp[q]);
+                    const bf16x4 h = __builtin_convertvector(d, bf16x4);
+                    *reinterpret_cast<bf16x4*>(sH + (row - m0) * kLDH + col - n0) = h;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) cs[16 * j + 4 * c + q] += (float)h[q];
+                }
             }
-        }
-    if (EPI == 2 && g.db) {
+    }
+    if constexpr (kH) {
+        __syncthreads();
+        const int t = threadIdx.x;
 #pragma unroll
-        for (int j = 0; j < WNB; ++j) {
-            const int col = n0 + wn * 32 * WNB + j * 32 + (lane & 31);
-            const float d = dbs[j] + __shfl_xor(dbs[j], 32, 64);  // lanes l and l + 32 hold the same column
-            if (lane < 32) atomicAdd(g.db + e * g.sdb + col, d);
+        for (int pass = 0; pass < 8; ++pass) {
+            const int row = pass * 16 + (t >> 4), cc = (t & 15) * 8;
+            *reinterpret_cast<u32x4*>(g.H + e * g.she + (int64_t)(m0 + row) * g.ldh + n0 + cc) =
+                *reinterpret_cast<const u32x4*>(sH + row * kLDH + cc);
+        }
+    }
+    if constexpr (kOut) {
+        // lanes l and l + 32 hold the two halves of the wave's 64 columns;
+        // slot = the wave's column range
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const float t = ys[i] + __shfl_xor(ys[i], 32, 64);
+            if (lane < 32)
+                g.yp[e * g.syp + (int64_t)((n0 + wn * 64) / 64) * g.M + m0 + wm * 64 + i * 32 + lane] = t;
+        }
+    }
+    if constexpr (EPI == 2) {
+        if (g.db) {
+            // value 16 j + 4 c + q: column 64 wn + 32 j + 8 c + 4 (l div 32) + q;
+            // summed over the 32 rows of each half wave
+            half_wave_transpose_sum<32, 16>(cs, lane);
+            const int v = lane & 31;
+            const int col = n0 + wn * 64 + 32 * (v >> 4) + 8 * ((v >> 2) & 3) + 4 * (lane >> 5) + (v & 3);
+            atomicAdd(g.db + e * g.sdb + col, cs[0]);
         }
     }
 }
@@ -336,31 +456,25 @@ __global__ void kg_gather(const uint16_t* __restrict__ X, const float* __restric
     if (c == 0) Yb[(int64_t)e * Bsz + i] = Y[src];
 }
 
-// y[e][b] = sum_o A[e][b][o] w[e][o] + bo[e]: one wave per row.
-__global__ void kg_out_fwd(const uint16_t* __restrict__ A, int H, int Bsz, const float* __restrict__ w, int64_t swe,
-                           const float* __restrict__ bo, int64_t sbo, float* __restrict__ y) {
-    const int e = blockIdx.y;
-    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (b >= Bsz) return;
-    const uint16_t* a = A + ((int64_t)e * Bsz + b) * H;
-    const float* we = w + e * swe;
+// The output layer's value from the last hidden GEMM's partial sums:
+// y = sum over slots (in slot order) of yp[e][slot][row], + bo[e].
+__device__ __forceinline__ float out_value(const float* __restrict__ yp, int nslot, int64_t syp, int64_t M,
+                                           const float* __restrict__ bo, int64_t sbo, int e, int64_t row) {
     float s = 0.0f;
-    for (int o = lane; o < H; o += 64) s += bf2f(a[o]) * we[o];
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d, 64);
-    if (lane == 0) y[(int64_t)e * Bsz + b] = s + bo[e * sbo];
+    for (int q = 0; q < nslot; ++q) s += yp[e * syp + q * M + row];
+    return s + bo[e * sbo];
 }
 
 // dy = 2 (y - t) / B (mean squared error per member), loss += (y - t)^2 / B,
 // the output bias gradient db_o += sum_b dy and the epoch's loss sum.
-__global__ void kg_loss(const float* __restrict__ y, const float* __restrict__ t, int Bsz, float* __restrict__ dy,
+__global__ void kg_loss(const float* __restrict__ yp, int nslot, int64_t syp, const float* __restrict__ bo,
+                        int64_t sbo, const float* __restrict__ t, int Bsz, float* __restrict__ dy,
                         float* __restrict__ loss, float* __restrict__ dbo, int64_t sdbo, double* __restrict__ epoch) {
     const int e = blockIdx.y;
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     float l = 0.0f, d = 0.0f;
     if (b < Bsz) {
-        const float r = y[(int64_t)e * Bsz + b] - t[(int64_t)e * Bsz + b];
+        const float r = out_value(yp, nslot, syp, Bsz, bo, sbo, e, b) - t[(int64_t)e * Bsz + b];
         d = 2.0f * r / (float)Bsz;
         dy[(int64_t)e * Bsz + b] = d;
         l = r * r / (float)Bsz;
@@ -445,13 +559,19 @@ __global__ void kg_pack_input(const float* __restrict__ x, int64_t n, int din, u
     X[i] = c < din ? f2bf(x[r * din + c]) : (uint16_t)0;
 }
 
-// mean and (population) standard deviation over the E members of each row
-__global__ void kg_ensemble(const float* __restrict__ y, int E, int64_t n, int64_t stride, double* __restrict__ mean,
-                            double* __restrict__ sd) {
+// Each member's output y[e][i] (from the fused partial sums), then mean
+// and (population) standard deviation over the E members of each row.
+__global__ void kg_ensemble(const float* __restrict__ yp, int nslot, int64_t syp, const float* __restrict__ bo,
+                            int64_t sbo, int E, int64_t n, int64_t stride, float* __restrict__ y,
+                            double* __restrict__ mean, double* __restrict__ sd) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double s = 0.0;
-    for (int e = 0; e < E; ++e) s += (double)y[e * stride + i];
+    for (int e = 0; e < E; ++e) {
+        const float v = out_value(yp, nslot, syp, n, bo, sbo, e, i);
+        y[e * stride + i] = v;
+        s += (double)v;
+    }
     const double mu = s / E;
     double q = 0.0;
     for (int e = 0; e < E; ++e) {
@@ -501,7 +621,7 @@ struct pp_mlp {
     std::vector<uint16_t*> A;    // [E][B][H] per layer
     uint16_t* dZa = nullptr;     // [E][B][H] ping
     uint16_t* dZb = nullptr;     // [E][B][H] pong
-    float* y = nullptr;
+    float* y = nullptr;  // [E][H / 32][cap_b] output-layer partial sums
     float* dy = nullptr;
     float* loss = nullptr;
     double* loss_acc = nullptr;  // [E] per-epoch sum of the minibatch losses
@@ -511,6 +631,7 @@ struct pp_mlp {
     uint16_t* Qa = nullptr;
     uint16_t* Qb = nullptr;
     float* yq = nullptr;
+    float* ypq = nullptr;  // [E][H / 32][cap_q] output-layer partial sums
     float* xq_f = nullptr;
     double* mq = nullptr;
     double* sq = nullptr;
@@ -527,7 +648,7 @@ struct pp_mlp {
         for (auto q : Z) F(q);
         for (auto q : A) F(q);
         F(dZa); F(dZb); F(y); F(dy); F(loss); F(loss_acc);
-        F(Xq); F(Qa); F(Qb); F(yq); F(xq_f); F(mq); F(sq);
+        F(Xq); F(Qa); F(Qb); F(yq); F(ypq); F(xq_f); F(mq); F(sq);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -579,7 +700,7 @@ struct pp_mlp {
         }
         dZa = mal<uint16_t>(act);
         dZb = mal<uint16_t>(act);
-        y = mal<float>((size_t)E * Bsz);
+        y = mal<float>((size_t)E * (H / 32) * Bsz);  // output-layer partial sums [E][slot][Bsz]
         dy = mal<float>((size_t)E * Bsz);
     }
 
@@ -592,31 +713,32 @@ struct pp_mlp {
         MCHK(hipGetLastError());
     }
 
+    // kg_gemm2 takes the shape (else kg_gemm)
+    static bool wide_tiles(int M, int N, int Kd) { return M % kTM == 0 && N % 128 == 0 && Kd % kTK == 0; }
+    // output-layer partial sums per row of the last hidden GEMM (M x H):
+    // one per wave's column range, 64 columns in kg_gemm2, 32 in kg_gemm
+    int out_slots(int M) const { return wide_tiles(M, H, K(L - 1)) ? H / 64 : H / 32; }
+
     template <int EPI>
     void gemm(const uint16_t* Aop, int64_t sam, int64_t sak, int64_t sae, const uint16_t* Bop, int64_t sbn, int64_t sbk,
               int64_t sbe, int M, int N, int Kd, float* C, int64_t ldc, int64_t sce, uint16_t* Hout, int64_t ldh,
               int64_t she, const float* bias, int64_t sbias, const float* Zp, int64_t ldz, int64_t sze,
-              float* db = nullptr, int64_t sdb = 0) {
+              float* db = nullptr, int64_t sdb = 0, float* yp = nullptr, int64_t syp = 0, const float* wo = nullptr,
+              int64_t swo = 0) {
         if (M % kBM || N % kBN || Kd % kBK) throw MlpError{PP_EARG, "GEMM shape not a multiple of the tile"};
         if ((sak != 1 && sam != 1) || (sbk != 1 && sbn != 1)) throw MlpError{PP_EARG, "GEMM operand not contiguous"};
         GemmArgs a{Aop, sam, sak, sae, Bop, sbn, sbk, sbe, M, N, Kd, C, ldc, sce, Hout, ldh, she, bias, sbias, Zp, ldz, sze,
-                   db, sdb};
-        if (M % kTM == 0 && N % 128 == 0 && Kd % kTK == 0) {
-            // 128 x 128 tiles: 128 x 256 (WNB = 4) holds one workgroup per CU
-            // and measured 20% slower on 65536-row inference
-            auto launch = [&](auto wnb) {
-                constexpr int WNB = decltype(wnb)::value;
-                const dim3 grid(N / (64 * WNB), M / kTM, E);
-                if (sak == 1 && sbk == 1)
-                    hipLaunchKernelGGL((kg_gemm2<EPI, true, true, WNB>), grid, dim3(256), 0, stream, a);
-                else if (sak == 1)
-                    hipLaunchKernelGGL((kg_gemm2<EPI, true, false, WNB>), grid, dim3(256), 0, stream, a);
-                else if (sbk == 1)
-                    hipLaunchKernelGGL((kg_gemm2<EPI, false, true, WNB>), grid, dim3(256), 0, stream, a);
-                else
-                    hipLaunchKernelGGL((kg_gemm2<EPI, false, false, WNB>), grid, dim3(256), 0, stream, a);
-            };
-            launch(std::integral_constant<int, 2>());
+                   db, sdb, yp, syp, wo, swo};
+        if (wide_tiles(M, N, Kd)) {
+            const dim3 grid(N / 128, M / kTM, E);
+            if (sak == 1 && sbk == 1)
+                hipLaunchKernelGGL((kg_gemm2<EPI, true, true>), grid, dim3(256), 0, stream, a);
+            else if (sak == 1)
+                hipLaunchKernelGGL((kg_gemm2<EPI, true, false>), grid, dim3(256), 0, stream, a);
+            else if (sbk == 1)
+                hipLaunchKernelGGL((kg_gemm2<EPI, false, true>), grid, dim3(256), 0, stream, a);
+            else
+                hipLaunchKernelGGL((kg_gemm2<EPI, false, false>), grid, dim3(256), 0, stream, a);
         } else {
             hipLaunchKernelGGL(kg_gemm<EPI>, dim3(N / kBN, M / kBM, E), dim3(256), 0, stream, a);
         }
@@ -628,14 +750,18 @@ struct pp_mlp {
         const uint16_t* in = Xb;
         int64_t ld_in = kKin;
         for (int l = 0; l < L; ++l) {
-            gemm<1>(in, ld_in, 1, (int64_t)Bsz * ld_in, Wb[l], K(l), 1, (int64_t)H * K(l), Bsz, H, K(l), Z[l], H,
-                    (int64_t)Bsz * H, A[l], H, (int64_t)Bsz * H, p + offb[l], P, nullptr, 0, 0);
+            auto run = [&](auto epi) {
+                gemm<decltype(epi)::value>(in, ld_in, 1, (int64_t)Bsz * ld_in, Wb[l], K(l), 1, (int64_t)H * K(l), Bsz,
+                                           H, K(l), Z[l], H, (int64_t)Bsz * H, A[l], H, (int64_t)Bsz * H, p + offb[l],
+                                           P, nullptr, 0, 0, nullptr, 0, y, (int64_t)out_slots(Bsz) * Bsz, p + offwo,
+                                           P);
+            };
+            // the last hidden layer's epilogue also forms the output layer's partial sums
+            if (l == L - 1) run(std::integral_constant<int, 5>());
+            else run(std::integral_constant<int, 1>());
             in = A[l];
             ld_in = H;
         }
-        hipLaunchKernelGGL(kg_out_fwd, dim3((unsigned)cdiv64(Bsz, 4), E), dim3(256), 0, stream, A[L - 1], H, Bsz,
-                           p + offwo, P, p + offbo, P, y);
-        MCHK(hipGetLastError());
     }
 
     // Loss, gradients into g (zeroed first), for the minibatch in Xb / Yb.
@@ -644,8 +770,9 @@ struct pp_mlp {
         MCHK(hipMemsetAsync(loss, 0, sizeof(float) * E, stream));
         // dy, loss, db_o and the epoch loss; then the output layer's dZ with
         // dw_o and the last hidden layer's bias gradient
-        hipLaunchKernelGGL(kg_loss, dim3((unsigned)cdiv64(Bsz, 256), E), dim3(256), 0, stream, y, Yb, Bsz, dy, loss,
-                           g + offbo, P, loss_acc);
+        const int ns = out_slots(Bsz);
+        hipLaunchKernelGGL(kg_loss, dim3((unsigned)cdiv64(Bsz, 256), E), dim3(256), 0, stream, y, ns,
+                           (int64_t)ns * Bsz, p + offbo, P, Yb, Bsz, dy, loss, g + offbo, P, loss_acc);
         const int rows = 64;
         const int64_t act = (int64_t)Bsz * H;
         hipLaunchKernelGGL(kg_out_bwd, dim3((unsigned)cdiv64(H, 256), (unsigned)cdiv64(Bsz, rows), E), dim3(256), 0,
@@ -693,33 +820,39 @@ struct pp_mlp {
         auto F = [](void* q) {
             if (q) (void)hipFree(q);
         };
-        F(Xq); F(Qa); F(Qb); F(yq); F(xq_f); F(mq); F(sq);
+        F(Xq); F(Qa); F(Qb); F(yq); F(ypq); F(xq_f); F(mq); F(sq);
         cap_q = cq;
         Xq = mal<uint16_t>((size_t)cq * kKin);
         Qa = mal<uint16_t>((size_t)E * cq * H);
         Qb = mal<uint16_t>((size_t)E * cq * H);
         yq = mal<float>((size_t)E * cq);
+        ypq = mal<float>((size_t)E * (H / 32) * cq);
         xq_f = mal<float>((size_t)cq * din);
         mq = mal<double>((size_t)cq);
         sq = mal<double>((size_t)cq);
     }
 
-    // Forward of `cnt` rows (multiple of 64) of Xq into yq[E][cnt].
+    // Forward of `cnt` rows (multiple of 64) of Xq; the output layer's
+    // partial sums land in ypq[E][slot][cnt].
     void forward_query(int64_t cnt) {
         const uint16_t* in = Xq;
         int64_t ld_in = kKin, se_in = 0;  // the input is shared by every member
         uint16_t* outs[2] = {Qa, Qb};
         for (int l = 0; l < L; ++l) {
             uint16_t* o = outs[l & 1];
-            gemm<3>(in, ld_in, 1, se_in, Wb[l], K(l), 1, (int64_t)H * K(l), (int)cnt, H, K(l), nullptr, 0, 0, o, H,
-                    cnt * H, p + offb[l], P, nullptr, 0, 0);
+            auto run = [&](auto epi) {
+                gemm<decltype(epi)::value>(in, ld_in, 1, se_in, Wb[l], K(l), 1, (int64_t)H * K(l), (int)cnt, H, K(l),
+                                           nullptr, 0, 0, o, H, cnt * H, p + offb[l], P, nullptr, 0, 0, nullptr, 0,
+                                           ypq, (int64_t)out_slots((int)cnt) * cnt, p + offwo, P);
+            };
+            // the last hidden layer stores no activations: its epilogue forms
+            // the output layer's partial sums in ypq
+            if (l == L - 1) run(std::integral_constant<int, 4>());
+            else run(std::integral_constant<int, 3>());
             in = o;
             ld_in = H;
             se_in = cnt * H;
         }
-        hipLaunchKernelGGL(kg_out_fwd, dim3((unsigned)cdiv64(cnt, 4), E), dim3(256), 0, stream, in, H, (int)cnt,
-                           p + offwo, P, p + offbo, P, yq);
-        MCHK(hipGetLastError());
     }
 };
 
@@ -942,8 +1075,9 @@ int pp_mlp_predict(pp_mlp* h, const float* x, int64_t n, float* members, double*
         hipLaunchKernelGGL(kg_pack_input, dim3((unsigned)cdiv64(cpad * kKin, 256)), dim3(256), 0, h->stream, h->xq_f,
                            cpad, h->din, h->Xq);
         h->forward_query(cpad);
-        hipLaunchKernelGGL(kg_ensemble, dim3((unsigned)cdiv64(cpad, 256)), dim3(256), 0, h->stream, h->yq, h->E, cpad,
-                           cpad, h->mq, h->sq);
+        const int ns = h->out_slots((int)cpad);
+        hipLaunchKernelGGL(kg_ensemble, dim3((unsigned)cdiv64(cpad, 256)), dim3(256), 0, h->stream, h->ypq, ns,
+                           (int64_t)ns * cpad, h->p + h->offbo, h->P, h->E, cpad, cpad, h->yq, h->mq, h->sq);
         MCHK(hipGetLastError());
         MCHK(hipEventRecord(e1, h->stream));
         MCHK(hipMemcpyAsync(mean + r0, h->mq, sizeof(double) * cnt, hipMemcpyDeviceToHost, h->stream));
