@@ -788,10 +788,10 @@ struct Batch {
             // end (not around it), so the interval is the kernel's duration
             hipEvent_t e0 = next_event();
             hipEvent_t e1 = next_event();
-            hipExtLaunchKernelGGL((kb_merge<Wt, TM, CT>), dim3(blocks(3)), dim3(kBlock), 0, stream, e0, e1, 0, map(3),
+            hipExtLaunchKernelGGL((kb_merge<Wt, TM, CT>), dim3(blocks(3)), dim3(merge_threads(TM)), 0, stream, e0, e1, 0, map(3),
                                   d_desc, d_state, dsteps, 0, step_index, (const uint32_t*)d_epoch, ep_off);
         } else {
-            hipLaunchKernelGGL((kb_merge<Wt, TM, CT>), dim3(blocks(3)), dim3(kBlock), 0, stream, map(3), d_desc, d_state,
+            hipLaunchKernelGGL((kb_merge<Wt, TM, CT>), dim3(blocks(3)), dim3(merge_threads(TM)), 0, stream, map(3), d_desc, d_state,
                                dsteps, 0, step_index, (const uint32_t*)d_epoch, ep_off);
         }
     }
@@ -800,7 +800,7 @@ struct Batch {
     // a valid pointer: the caller has sized it to at least B entries).
     template <int Wt, typename CT, int TM>
     void launch_compact(uint32_t ep_off) {
-        hipLaunchKernelGGL((kb_merge<Wt, TM, CT>), dim3(blocks(3)), dim3(kBlock), 0, stream, map(3), d_desc, d_state, d_steps,
+        hipLaunchKernelGGL((kb_merge<Wt, TM, CT>), dim3(blocks(3)), dim3(merge_threads(TM)), 0, stream, map(3), d_desc, d_state, d_steps,
                            1, (int64_t)-1, (const uint32_t*)d_epoch, ep_off);
     }
 

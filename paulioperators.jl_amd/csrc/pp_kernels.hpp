@@ -1148,6 +1148,12 @@ __global__ void __launch_bounds__(kBlock) kb_emit(BlockMap bm, const SegDesc* __
     }
 }
 
+// Threads per kb_merge workgroup: E = TM / threads outputs each.
+constexpr int merge_threads(int TM) { return TM >= 2048 ? 2 * kBlock : kBlock; }
+// Waves per SIMD the register budget must allow: three 512-thread
+// workgroups per CU where their LDS (48 KiB each at W = 1) fits.
+constexpr int merge_waves(int TM) { return TM >= 2048 ? 6 : 1; }
+
 // ---------------------------------------------------------------------------
 // kb_merge (see the file header).  mode 0: rotation (live + children);
 // mode 1: early-merge compaction (live only, no rotation) for the vectors
@@ -1155,11 +1161,12 @@ __global__ void __launch_bounds__(kBlock) kb_emit(BlockMap bm, const SegDesc* __
 // workgroups that have started: with a workgroup per tile, tile = local
 // workgroup index; with fewer workgroups, tiles are claimed by ticket.
 template <int W, int TM, typename CT>
-__global__ void __launch_bounds__(kBlock) kb_merge(BlockMap bm, const SegDesc* __restrict__ D, SegState* __restrict__ S,
+__global__ void __launch_bounds__(merge_threads(TM), merge_waves(TM)) kb_merge(BlockMap bm, const SegDesc* __restrict__ D, SegState* __restrict__ S,
                                                    const StepDev* __restrict__ steps, int mode,
                                                    int64_t step_index, const uint32_t* __restrict__ d_epoch,
                                                    uint32_t ep_off) {
-    constexpr int E = TM / kBlock;
+    constexpr int NT = merge_threads(TM), NW = NT / kWave;
+    constexpr int E = TM / NT;
     __shared__ Key<W> s_key[TM];
     __shared__ CT s_c[TM];
     __shared__ int64_t s_tile;
@@ -1167,7 +1174,7 @@ __global__ void __launch_bounds__(kBlock) kb_merge(BlockMap bm, const SegDesc* _
     __shared__ int64_t s_split[2];
     __shared__ Key<W> s_edge[2];   // C[j1] (first child after the tile), L[i0 - 1] (last live term before it)
     __shared__ CT s_edge_c;    // c of C[j1]
-    __shared__ int s_wsum[4];
+    __shared__ int s_wsum[NW];
     int b;
     int64_t lb, nb;
     if (!map_block(bm, b, lb, nb)) return;
@@ -1258,7 +1265,7 @@ __global__ void __launch_bounds__(kBlock) kb_merge(BlockMap bm, const SegDesc* _
             CT lcv[E];
 #pragma unroll
             for (int e = 0; e < E; ++e) {
-                const int k = tid + e * kBlock;
+                const int k = tid + e * NT;
 #pragma unroll
                 for (int w = 0; w < W; ++w) lk[e].z[w] = lk[e].x[w] = 0;
                 lcv[e] = coef_zero<CT>();
@@ -1282,7 +1289,7 @@ __global__ void __launch_bounds__(kBlock) kb_merge(BlockMap bm, const SegDesc* _
             }
 #pragma unroll
             for (int e = 0; e < E; ++e) {
-                const int k = tid + e * kBlock;
+                const int k = tid + e * NT;
                 if (k < nL + nC) {
                     s_key[k] = lk[e];
                     s_c[k] = lcv[e];
@@ -1290,11 +1297,11 @@ __global__ void __launch_bounds__(kBlock) kb_merge(BlockMap bm, const SegDesc* _
             }
         }
         // the neighbours across the tile's edges, loaded with the tile
-        if (tid == kBlock - 1 && j1 < m) {
+        if (tid == NT - 1 && j1 < m) {
             s_edge[0] = load_key<W>(Cz, Cx, j1);
             s_edge_c = ldg(Cc, j1);
         }
-        if (tid == kBlock - 2 && i0 > 0) s_edge[1] = load_key<W>(Lz, Lx, i0 - 1);
+        if (tid == NT - 2 && i0 > 0) s_edge[1] = load_key<W>(Lz, Lx, i0 - 1);
         __syncthreads();
         const Key<W>* sL = s_key;
         const Key<W>* sC = s_key + nL;
@@ -1364,7 +1371,7 @@ __global__ void __launch_bounds__(kBlock) kb_merge(BlockMap bm, const SegDesc* _
         if (lane == kWave - 1) s_wsum[wave] = incl;
         __syncthreads();
         int wpref = 0, total = 0;
-        for (int w = 0; w < 4; ++w) {
+        for (int w = 0; w < NW; ++w) {
             if (w < wave) wpref += s_wsum[w];
             total += s_wsum[w];
         }
@@ -1465,7 +1472,7 @@ __global__ void __launch_bounds__(kBlock) kb_merge(BlockMap bm, const SegDesc* _
         __syncthreads();
         const int64_t base = s_base;
         if (base + total <= capOut) {
-            for (int k = tid; k < total; k += kBlock) {
+            for (int k = tid; k < total; k += NT) {
                 if constexpr (TM >= 2048) {  // large vectors: stream past the caches
                     store_key_nt<W>(Oz, Ox, base + k, s_key[k]);
                     stg_nt(Oc, base + k, s_c[k]);
