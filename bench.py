@@ -488,6 +488,8 @@ def main():
                     help="run only the windowed-evolution, lookup and AdaptiveTruncation measurements")
     ap.add_argument("--algebra", action="store_true",
                     help="run only the Pauli-sum product measurement ([H, O] on the 121-qubit lattice)")
+    ap.add_argument("--bandwidth", action="store_true",
+                    help="run only the bandwidth-regime merge measurement (256 circuits of the 36-qubit chain)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -504,6 +506,12 @@ def main():
         return
     if args.batched:
         print(json.dumps(batched_with_cpu(cpu=not args.no_cpu)), flush=True)
+        return
+    if args.bandwidth:
+        gbs, us, nbytes, launches = merge_bandwidth_regime()
+        print(json.dumps({"workload": "256 circuits of the 36-qubit chain, 10 layers, |c| <= 1e-6",
+                          "merge_gbs": gbs, "merge_avg_launch_us": us, "merge_bytes_per_launch": nbytes,
+                          "merge_launches": launches, "frac_of_8tbs": gbs / HBM_PEAK_GBS}), flush=True)
         return
     if args.rows:
         print(json.dumps(secondary_rows(cpu=not args.no_cpu)), flush=True)

@@ -610,6 +610,7 @@ struct Batch {
         std::vector<int> order((size_t)B), row((size_t)B);
         std::iota(order.begin(), order.end(), 0);
         std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return n_known[a] > n_known[c]; });
+        int64_t row_max = 0;  // sweep blocks of the fullest XCD row
         {
             int64_t load[kXcds] = {0};
             for (int b : order) {
@@ -619,6 +620,7 @@ struct Batch {
                 row[(size_t)b] = x;
                 load[x] += h_maps[(size_t)0 * (B + 1) + b + 1] - h_maps[(size_t)0 * (B + 1) + b];
             }
+            for (int x = 0; x < kXcds; ++x) row_max = std::max(row_max, load[x]);
         }
         auto round = [&](int64_t g) {
             // with captured graphs each grid is rounded up to a power of
@@ -632,8 +634,13 @@ struct Batch {
         };
         // rows pay off where kernels are latency-bound (L2 hits between the
         // kernels of a step); a call with more tiles than a few per CU is
-        // bandwidth-bound and wants every XCD on every vector
-        const bool rows = acc[0] <= 16 * (int64_t)ncu;
+        // bandwidth-bound and wants every XCD on every vector.  So does a
+        // small batch whose fullest row holds more than a few tiles per CU of
+        // its XCD for each of its vectors: one large vector alone would run
+        // on one XCD (measured on the 36-qubit chain one vector at a time:
+        // 45 evals/s with rows, 59 without)
+        const int64_t row_ok = 16 * std::max<int64_t>(1, ncu / kXcds) * (int64_t)B;
+        const bool rows = acc[0] <= 16 * (int64_t)ncu && row_max <= row_ok;
         std::vector<int64_t> st((size_t)4 * B);
         int64_t grids[4];
         for (int k = 0; k < 4; ++k) {
