@@ -108,6 +108,10 @@ struct GemmArgs {
     int64_t syp;
     const float* wo;
     int64_t swo;
+    // split K (EPI 0): blockIdx.z = part * ... + e over kparts parts of K / kparts
+    // each; part p's product goes to C + p scp
+    int kparts;
+    int64_t scp;
 };
 
 // Sums N per-lane values over the 32 lanes of each half wave (N = 16 or 32)
@@ -161,7 +165,8 @@ template <int EPI>
 __global__ void __launch_bounds__(256) kg_gemm(GemmArgs g) {
     __shared__ __attribute__((aligned(16))) uint16_t sA[kBM * kLDK];
     __shared__ __attribute__((aligned(16))) uint16_t sB[kBN * kLDK];
-    const int e = blockIdx.z;
+    const int e = blockIdx.z % (gridDim.z / g.kparts), part = blockIdx.z / (gridDim.z / g.kparts);
+    const int kl = g.K / g.kparts, kb = part * kl;
     const int m0 = blockIdx.y * kBM, n0 = blockIdx.x * kBN;
     const uint16_t* A = g.A + e * g.sae;
     const uint16_t* B = g.B + e * g.sbe;
@@ -170,7 +175,7 @@ __global__ void __launch_bounds__(256) kg_gemm(GemmArgs g) {
     floatx16 acc;
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
-    for (int k0 = 0; k0 < g.K; k0 += kBK) {
+    for (int k0 = kb; k0 < kb + kl; k0 += kBK) {
         stage_tile(A, g.sam, g.sak, m0, k0, sA);
         stage_tile(B, g.sbn, g.sbk, n0, k0, sB);
         __syncthreads();
@@ -194,7 +199,7 @@ __global__ void __launch_bounds__(256) kg_gemm(GemmArgs g) {
         const float v = acc[r];
         ys[r] = 0.0f;
         if (EPI == 0) {
-            g.C[e * g.sce + (int64_t)row * g.ldc + col] = v;
+            g.C[part * g.scp + e * g.sce + (int64_t)row * g.ldc + col] = v;
         } else if (epi_act(EPI)) {
             const float z = v + g.bias[e * g.sbias + col];
             if (epi_keep_z(EPI)) g.C[e * g.sce + (int64_t)row * g.ldc + col] = z;
@@ -308,7 +313,11 @@ __global__ void __launch_bounds__(256, 2) kg_gemm2(GemmArgs g) {
     constexpr int TN = 128;
     __shared__ __attribute__((aligned(16))) uint16_t sA[2][kTM * kLDT];
     __shared__ __attribute__((aligned(16))) uint16_t sB[2][TN * kLDT];
-    const int e = blockIdx.z;
+    // split K only for the weight gradient (EPI 0)
+    const int kparts = EPI == 0 ? g.kparts : 1;
+    const int e = EPI == 0 ? blockIdx.z % (gridDim.z / kparts) : blockIdx.z;
+    const int part = EPI == 0 ? blockIdx.z / (gridDim.z / kparts) : 0;
+    const int kb = part * (g.K / kparts);
     const int m0 = blockIdx.y * kTM, n0 = blockIdx.x * TN;
     const uint16_t* A = g.A + e * g.sae;
     const uint16_t* B = g.B + e * g.sbe;
@@ -321,19 +330,19 @@ __global__ void __launch_bounds__(256, 2) kg_gemm2(GemmArgs g) {
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
-    const int nk = g.K / kTK;
+    const int nk = g.K / kparts / kTK;
     Stage<kTM> pa;
     Stage<TN> pb;
-    stage_load<AK, kTM>(A, g.sam, g.sak, m0, 0, pa);
-    stage_load<BK, TN>(B, g.sbn, g.sbk, n0, 0, pb);
+    stage_load<AK, kTM>(A, g.sam, g.sak, m0, kb, pa);
+    stage_load<BK, TN>(B, g.sbn, g.sbk, n0, kb, pb);
     stage_store<AK, kTM>(pa, sA[0]);
     stage_store<BK, TN>(pb, sB[0]);
     __syncthreads();
     for (int kt = 0; kt < nk; ++kt) {
         const int cur = kt & 1;
         if (kt + 1 < nk) {
-            stage_load<AK, kTM>(A, g.sam, g.sak, m0, (kt + 1) * kTK, pa);
-            stage_load<BK, TN>(B, g.sbn, g.sbk, n0, (kt + 1) * kTK, pb);
+            stage_load<AK, kTM>(A, g.sam, g.sak, m0, kb + (kt + 1) * kTK, pa);
+            stage_load<BK, TN>(B, g.sbn, g.sbk, n0, kb + (kt + 1) * kTK, pb);
         }
         const uint16_t* a_s = sA[cur] + (wm * 64 + (lane & 31)) * kLDT + 8 * (lane >> 5);
         const uint16_t* b_s = sB[cur] + (wn * 64 + (lane & 31)) * kLDT + 8 * (lane >> 5);
@@ -384,7 +393,7 @@ __global__ void __launch_bounds__(256, 2) kg_gemm2(GemmArgs g) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) v[q] = acc[i][j][4 * c + q];
                 if constexpr (EPI == 0) {
-                    *reinterpret_cast<floatx4*>(g.C + e * g.sce + (int64_t)row * g.ldc + col) = v;
+                    *reinterpret_cast<floatx4*>(g.C + part * g.scp + e * g.sce + (int64_t)row * g.ldc + col) = v;
                 } else if constexpr (epi_act(EPI)) {
                     const floatx4 z = v + *reinterpret_cast<const floatx4*>(g.bias + e * g.sbias + col);
                     if constexpr (epi_keep_z(EPI))
@@ -454,6 +463,32 @@ __global__ void kg_gather(const uint16_t* __restrict__ X, const float* __restric
     const int64_t src = perm[(int64_t)e * n + off + i];
     Xb[((int64_t)e * Bsz + i) * kKin + c] = X[src * kKin + c];
     if (c == 0) Yb[(int64_t)e * Bsz + i] = Y[src];
+}
+
+// Split-K sum: dst[e][i] = sum over parts (in order) of ws[part][e][i], i < cnt
+// (a multiple of 4; dst rows contiguous).
+__global__ void kg_splitk_sum(const float* __restrict__ ws, int parts, int E, int64_t cnt, float* __restrict__ dst,
+                              int64_t sdst) {
+    const int e = blockIdx.y;
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    if (i >= cnt) return;
+    floatx4 acc = *reinterpret_cast<const floatx4*>(ws + e * cnt + i);
+    for (int q = 1; q < parts; ++q) acc += *reinterpret_cast<const floatx4*>(ws + ((int64_t)q * E + e) * cnt + i);
+    *reinterpret_cast<floatx4*>(dst + e * sdst + i) = acc;
+}
+
+// Zero the gradient slots the backward pass accumulates into with atomics
+// (every hidden bias, the output weights and bias) and the minibatch loss;
+// the weight gradients are stored whole by their GEMMs.
+struct AccSlots {
+    int n;
+    int64_t off[17], len[17];
+};
+__global__ void kg_zero_acc(float* __restrict__ g, int64_t P, AccSlots sl, float* __restrict__ loss) {
+    const int e = blockIdx.y, r = blockIdx.x;
+    if (r < sl.n)
+        for (int64_t i = threadIdx.x; i < sl.len[r]; i += blockDim.x) g[e * P + sl.off[r] + i] = 0.0f;
+    if (r == 0 && threadIdx.x == 0) loss[e] = 0.0f;
 }
 
 // The output layer's value from the last hidden GEMM's partial sums:
@@ -632,6 +667,9 @@ struct pp_mlp {
     uint16_t* Qb = nullptr;
     float* yq = nullptr;
     float* ypq = nullptr;  // [E][H / 32][cap_q] output-layer partial sums
+    float* ws = nullptr;   // split-K partial products of the weight gradients
+    size_t ws_cap = 0;
+    int ncu = 256;         // compute units of the device
     float* xq_f = nullptr;
     double* mq = nullptr;
     double* sq = nullptr;
@@ -648,13 +686,20 @@ struct pp_mlp {
         for (auto q : Z) F(q);
         for (auto q : A) F(q);
         F(dZa); F(dZb); F(y); F(dy); F(loss); F(loss_acc);
-        F(Xq); F(Qa); F(Qb); F(yq); F(ypq); F(xq_f); F(mq); F(sq);
+        F(Xq); F(Qa); F(Qb); F(yq); F(ypq); F(xq_f); F(mq); F(sq); F(ws);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
     void init(int E_, int din_, int H_, int L_) {
         E = E_; din = din_; H = H_; L = L_;
         MCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        {
+            int dev = 0;
+            MCHK(hipGetDevice(&dev));
+            hipDeviceProp_t prop;
+            MCHK(hipGetDeviceProperties(&prop, dev));
+            if (prop.multiProcessorCount > 0) ncu = prop.multiProcessorCount;
+        }
         int64_t off = 0;
         for (int l = 0; l < L; ++l) {
             offW.push_back(off);
@@ -672,6 +717,7 @@ struct pp_mlp {
         m = mal<float>((size_t)E * P);
         v = mal<float>((size_t)E * P);
         MCHK(hipMemsetAsync(p, 0, sizeof(float) * E * P, stream));
+        MCHK(hipMemsetAsync(g, 0, sizeof(float) * E * P, stream));  // padding stays zero
         MCHK(hipMemsetAsync(m, 0, sizeof(float) * E * P, stream));
         MCHK(hipMemsetAsync(v, 0, sizeof(float) * E * P, stream));
         for (int l = 0; l < L; ++l) Wb.push_back(mal<uint16_t>((size_t)E * H * K(l)));
@@ -728,9 +774,32 @@ struct pp_mlp {
         if (M % kBM || N % kBN || Kd % kBK) throw MlpError{PP_EARG, "GEMM shape not a multiple of the tile"};
         if ((sak != 1 && sam != 1) || (sbk != 1 && sbn != 1)) throw MlpError{PP_EARG, "GEMM operand not contiguous"};
         GemmArgs a{Aop, sam, sak, sae, Bop, sbn, sbk, sbe, M, N, Kd, C, ldc, sce, Hout, ldh, she, bias, sbias, Zp, ldz, sze,
-                   db, sdb, yp, syp, wo, swo};
-        if (wide_tiles(M, N, Kd)) {
-            const dim3 grid(N / 128, M / kTM, E);
+                   db, sdb, yp, syp, wo, swo, 1, 0};
+        const bool wide = wide_tiles(M, N, Kd);
+        // a weight gradient with fewer workgroups than CUs (K = the batch)
+        // splits K into parts written to a workspace and summed in order
+        int parts = 1;
+        if (EPI == 0 && ldc == N) {
+            const int64_t wgs = wide ? (int64_t)(N / 128) * (M / kTM) * E : (int64_t)(N / kBN) * (M / kBM) * E;
+            const int kt = wide ? kTK : kBK;
+            while (wgs * parts < ncu && Kd % (2 * parts * kt) == 0 && Kd / (2 * parts) >= 2 * kt) parts *= 2;
+        }
+        if (parts > 1) {
+            const size_t need = (size_t)parts * E * M * N;
+            if (ws_cap < need) {
+                if (ws) (void)hipFree(ws);
+                ws = nullptr;
+                ws_cap = 0;
+                ws = mal<float>(need);
+                ws_cap = need;
+            }
+            a.C = ws;
+            a.sce = (int64_t)M * N;
+            a.scp = (int64_t)E * M * N;
+            a.kparts = parts;
+        }
+        if (wide) {
+            const dim3 grid(N / 128, M / kTM, E * parts);
             if (sak == 1 && sbk == 1)
                 hipLaunchKernelGGL((kg_gemm2<EPI, true, true>), grid, dim3(256), 0, stream, a);
             else if (sak == 1)
@@ -740,7 +809,12 @@ struct pp_mlp {
             else
                 hipLaunchKernelGGL((kg_gemm2<EPI, false, false>), grid, dim3(256), 0, stream, a);
         } else {
-            hipLaunchKernelGGL(kg_gemm<EPI>, dim3(N / kBN, M / kBM, E), dim3(256), 0, stream, a);
+            hipLaunchKernelGGL(kg_gemm<EPI>, dim3(N / kBN, M / kBM, E * parts), dim3(256), 0, stream, a);
+        }
+        if (parts > 1) {
+            const int64_t cnt = (int64_t)M * N;
+            hipLaunchKernelGGL(kg_splitk_sum, dim3((unsigned)cdiv64(cnt / 4, 256), E), dim3(256), 0, stream, ws, parts, E,
+                               cnt, C, sce);
         }
         MCHK(hipGetLastError());
     }
@@ -764,16 +838,26 @@ struct pp_mlp {
         }
     }
 
-    // Loss, gradients into g (zeroed first), for the minibatch in Xb / Yb.
+    // Loss and gradients into g (its atomically accumulated slots zeroed
+    // first, the weight gradients stored whole), for the minibatch in Xb / Yb.
     void backward(int Bsz) {
-        MCHK(hipMemsetAsync(g, 0, sizeof(float) * E * P, stream));
-        MCHK(hipMemsetAsync(loss, 0, sizeof(float) * E, stream));
+        {
+            AccSlots sl{};
+            sl.n = L + 1;
+            for (int l = 0; l < L; ++l) {
+                sl.off[l] = offb[l];
+                sl.len[l] = H;
+            }
+            sl.off[L] = offwo;  // w_o and b_o are adjacent
+            sl.len[L] = offbo + 1 - offwo;
+            hipLaunchKernelGGL(kg_zero_acc, dim3((unsigned)sl.n, E), dim3(256), 0, stream, g, P, sl, loss);
+        }
         // dy, loss, db_o and the epoch loss; then the output layer's dZ with
         // dw_o and the last hidden layer's bias gradient
         const int ns = out_slots(Bsz);
         hipLaunchKernelGGL(kg_loss, dim3((unsigned)cdiv64(Bsz, 256), E), dim3(256), 0, stream, y, ns,
                            (int64_t)ns * Bsz, p + offbo, P, Yb, Bsz, dy, loss, g + offbo, P, loss_acc);
-        const int rows = 64;
+        const int rows = 16;
         const int64_t act = (int64_t)Bsz * H;
         hipLaunchKernelGGL(kg_out_bwd, dim3((unsigned)cdiv64(H, 256), (unsigned)cdiv64(Bsz, rows), E), dim3(256), 0,
                            stream, dy, p + offwo, P, Z[L - 1], A[L - 1], H, Bsz, rows, dZa, g + offwo,
