@@ -309,10 +309,12 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
 template <int EPI, bool AK, bool BK>
-__global__ void __launch_bounds__(256, 2) kg_gemm2(GemmArgs g) {
+__global__ void __launch_bounds__(256, 3) kg_gemm2(GemmArgs g) {
     constexpr int TN = 128;
-    __shared__ __attribute__((aligned(16))) uint16_t sA[2][kTM * kLDT];
-    __shared__ __attribute__((aligned(16))) uint16_t sB[2][TN * kLDT];
+    // one LDS stage (A then B, 36 KiB): three workgroups per CU
+    __shared__ __attribute__((aligned(16))) uint16_t smem[(kTM + TN) * kLDT];
+    uint16_t* const sA = smem;
+    uint16_t* const sB = smem + kTM * kLDT;
     // split K only for the weight gradient (EPI 0)
     const int kparts = EPI == 0 ? g.kparts : 1;
     const int e = EPI == 0 ? blockIdx.z % (gridDim.z / kparts) : blockIdx.z;
@@ -335,17 +337,16 @@ __global__ void __launch_bounds__(256, 2) kg_gemm2(GemmArgs g) {
     Stage<TN> pb;
     stage_load<AK, kTM>(A, g.sam, g.sak, m0, kb, pa);
     stage_load<BK, TN>(B, g.sbn, g.sbk, n0, kb, pb);
-    stage_store<AK, kTM>(pa, sA[0]);
-    stage_store<BK, TN>(pb, sB[0]);
+    stage_store<AK, kTM>(pa, sA);
+    stage_store<BK, TN>(pb, sB);
     __syncthreads();
     for (int kt = 0; kt < nk; ++kt) {
-        const int cur = kt & 1;
         if (kt + 1 < nk) {
             stage_load<AK, kTM>(A, g.sam, g.sak, m0, kb + (kt + 1) * kTK, pa);
             stage_load<BK, TN>(B, g.sbn, g.sbk, n0, kb + (kt + 1) * kTK, pb);
         }
-        const uint16_t* a_s = sA[cur] + (wm * 64 + (lane & 31)) * kLDT + 8 * (lane >> 5);
-        const uint16_t* b_s = sB[cur] + (wn * 64 + (lane & 31)) * kLDT + 8 * (lane >> 5);
+        const uint16_t* a_s = sA + (wm * 64 + (lane & 31)) * kLDT + 8 * (lane >> 5);
+        const uint16_t* b_s = sB + (wn * 64 + (lane & 31)) * kLDT + 8 * (lane >> 5);
 #pragma unroll
         for (int kk = 0; kk < kTK; kk += 16) {
             // lane l supplies row (l mod 32), k = kk + 8 (l div 32) .. +7
@@ -363,8 +364,9 @@ __global__ void __launch_bounds__(256, 2) kg_gemm2(GemmArgs g) {
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[j], a[i], acc[i][j], 0, 0, 0);
         }
         if (kt + 1 < nk) {
-            stage_store<AK, kTM>(pa, sA[cur ^ 1]);
-            stage_store<BK, TN>(pb, sB[cur ^ 1]);
+            __syncthreads();  // every wave has read the stage
+            stage_store<AK, kTM>(pa, sA);
+            stage_store<BK, TN>(pb, sB);
         }
         __syncthreads();
     }
@@ -375,8 +377,8 @@ __global__ void __launch_bounds__(256, 2) kg_gemm2(GemmArgs g) {
     constexpr bool kOut = epi_out(EPI);
     constexpr bool kH = epi_keep_h(EPI) || EPI == 2;
     constexpr int kLDH = 136;  // bf16 per LDS row of the output tile (272 B)
-    static_assert(kTM * kLDH <= 2 * kTM * kLDT, "output tile exceeds the A buffers");
-    uint16_t* sH = &sA[0][0];
+    static_assert(kTM * kLDH <= (kTM + TN) * kLDT, "output tile exceeds the LDS stage");
+    uint16_t* sH = smem;
     float ys[2] = {0.0f, 0.0f};  // fused output layer: this lane's row sums
     float cs[32];                // EPI 2: column sums over this lane's rows
 #pragma unroll
