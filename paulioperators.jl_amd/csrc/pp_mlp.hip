@@ -234,9 +234,11 @@ __global__ void __launch_bounds__(256) kg_gemm(GemmArgs g) {
 // 128 (M), 128 (N) and 64 (K).  A workgroup computes a 128 x 128 tile with
 // four waves of 64 x 64 (2 x 2 blocks of v_mfma_f32_32x32x16_bf16, each
 // fragment used twice; a 128 x 256 tile holds one workgroup per CU and
-// measured 20% slower on 65536-row inference); K advances 64 per stage through two LDS buffers: the next stage's
-// global loads are issued into registers before the current stage's MFMAs and
-// written to the other buffer after them, one barrier per stage.  AK / BK: the
+// measured 20% slower on 65536-row inference); K advances 64 per stage
+// through one LDS stage of 36 KiB (three workgroups per CU; with two stages
+// only two fit, measured 16% slower): the next stage's global loads are
+// issued into registers before the current stage's MFMAs and written after
+// a barrier that ends the stage's reads.  AK / BK: the
 // operand's k stride is 1 (rows of 8 contiguous k per 16-byte load);
 // otherwise its row stride is 1 and each thread loads a block of 8 rows x 4 k
 // (four 16-byte loads), transposes it in registers and writes 8-byte rows to
