@@ -1151,8 +1151,9 @@ __global__ void __launch_bounds__(kBlock) kb_emit(BlockMap bm, const SegDesc* __
 // Threads per kb_merge workgroup: E = TM / threads outputs each.
 constexpr int merge_threads(int TM) { return TM >= 2048 ? 2 * kBlock : kBlock; }
 // Waves per SIMD the register budget must allow: three 512-thread
-// workgroups per CU where their LDS (48 KiB each at W = 1) fits.
-constexpr int merge_waves(int TM) { return TM >= 2048 ? 6 : 1; }
+// workgroups per CU where their LDS fits (48 KiB each at W = 1 with FP64
+// coefficients; two with complex ones, 64 KiB).
+constexpr int merge_waves(int TM, int coef_bytes) { return TM >= 2048 ? (coef_bytes == 8 ? 6 : 4) : 1; }
 
 // ---------------------------------------------------------------------------
 // kb_merge (see the file header).  mode 0: rotation (live + children);
@@ -1161,7 +1162,7 @@ constexpr int merge_waves(int TM) { return TM >= 2048 ? 6 : 1; }
 // workgroups that have started: with a workgroup per tile, tile = local
 // workgroup index; with fewer workgroups, tiles are claimed by ticket.
 template <int W, int TM, typename CT>
-__global__ void __launch_bounds__(merge_threads(TM), merge_waves(TM)) kb_merge(BlockMap bm, const SegDesc* __restrict__ D, SegState* __restrict__ S,
+__global__ void __launch_bounds__(merge_threads(TM), merge_waves(TM, (int)sizeof(CT))) kb_merge(BlockMap bm, const SegDesc* __restrict__ D, SegState* __restrict__ S,
                                                    const StepDev* __restrict__ steps, int mode,
                                                    int64_t step_index, const uint32_t* __restrict__ d_epoch,
                                                    uint32_t ep_off) {

@@ -88,25 +88,26 @@ def run(n_plain: int = 8, n_heldout: int = 4096, epochs: int = 60, n_query: int 
     return res
 
 
-def kernels(epochs: int = 4, n_train: int = 61440, n_query: int = 1_000_000, seed: int = 1):
+def kernels(epochs: int = 4, n_train: int = 61440, n_query: int = 1_000_000, seed: int = 1, batch: int = 1024):
     """Device throughput of the ensemble's kernels alone, on uniform synthetic
-    inputs of the workload's shape (5 members of 40-512-512-512-1, batch
-    1024): training steps per second and inference rows per second, with the
-    BF16 FLOP counts of the GEMMs (input layer padded to 64)."""
+    inputs of the workload's shape (5 members of 40-512-512-512-1, training
+    batch 1024 unless given): training steps per second and inference rows
+    per second, with the BF16 FLOP counts of the GEMMs (input layer padded
+    to 64)."""
     rng = np.random.default_rng(seed)
     din = sg.feature_dim("trig")
     ens = sg.MLPEnsemble(E=5, din=din, hidden=512, layers=3, seed=0)
     x = rng.uniform(-1, 1, size=(n_train, din)).astype(np.float32)
     y = rng.uniform(-1, 1, size=n_train).astype(np.float32)
-    ens.fit(x[:4096], y[:4096], epochs=1, batch=1024)
-    _, t = ens.fit(x, y, epochs=epochs, batch=1024)
-    steps = epochs * (n_train // 1024)
-    flop_step = 5 * 1024 * 2 * (64 * 512 + 2 * 512 * 512) * 3
+    ens.fit(x[:max(4096, batch)], y[:max(4096, batch)], epochs=1, batch=batch)
+    _, t = ens.fit(x, y, epochs=epochs, batch=batch)
+    steps = epochs * (n_train // batch)
+    flop_step = 5 * batch * 2 * (64 * 512 + 2 * 512 * 512) * 3
     q = rng.uniform(-1, 1, size=(n_query, din)).astype(np.float32)
     ens.predict(q[:65536])
     _, _, _, dev = ens.predict(q)
     flop_row = 5 * 2 * (64 * 512 + 2 * 512 * 512 + 512)
-    return {"train_steps": steps, "train_seconds": t, "train_us_per_step": t / steps * 1e6,
+    return {"train_batch": batch, "train_steps": steps, "train_seconds": t, "train_us_per_step": t / steps * 1e6,
             "train_tflops": steps * flop_step / t / 1e12, "inference_rows": n_query,
             "inference_seconds_device": dev, "inference_tflops_device": n_query * flop_row / dev / 1e12}
 
@@ -121,9 +122,10 @@ if __name__ == "__main__":
                     help="only the ensemble's training and inference throughput on synthetic inputs")
     ap.add_argument("--epochs", type=int, default=4, help="training epochs of --kernels")
     ap.add_argument("--queries", type=int, default=1_000_000, help="inference rows of --kernels")
+    ap.add_argument("--batch", type=int, default=1024, help="training batch of --kernels")
     a = ap.parse_args()
     if a.kernels:
-        print(json.dumps(kernels(epochs=a.epochs, n_query=a.queries)), flush=True)
+        print(json.dumps(kernels(epochs=a.epochs, n_query=a.queries, batch=a.batch)), flush=True)
     elif a.study:
         print(json.dumps(sg.learning_study(np.load(sg.LABELS_PATH)["labels"], log=log)), flush=True)
     else:
