@@ -911,7 +911,7 @@ struct Batch {
 
     // Roles of one resident launch: the vectors with active steps go to XCDs
     // by decreasing size, each with K workgroups whose starting chunks fill at
-    // most 25% (else 60%) of a chunk, divided by the vector's overflow boost
+    // most 25% (else 40%, else 60%) of a chunk, divided by the vector's overflow boost
     // (a vector whose chunks overflowed resumes with more workgroups).
     // false: the call runs on the per-step kernels (auto mode also declines
     // calls of few steps and vectors that do not fit one XCD at 90% fill).
@@ -981,8 +981,9 @@ struct Batch {
         // CUs and leave room to grow: on the 121-qubit sweep 25-30% starting
         // fill ran 7-11% faster than 60%, with fewer vectors handed back to
         // the per-step kernels.  Where the XCDs cannot hold every vector at
-        // that fill, 60% is tried before declining.
-        if (!assign(0.25) && !assign(0.6)) return false;
+        // that fill, 40% and then 60% are tried before declining (declining
+        // sends the whole call to the per-step kernels).
+        if (!assign(0.25) && !assign(0.4) && !assign(0.6)) return false;
         std::fill(h_roles.begin(), h_roles.end(), kResIdle);
         int next[kXcdCount] = {0};
         for (int b = 0; b < B; ++b)
