@@ -163,10 +163,14 @@ def test_resident_lattice121_small_chunks(cap):
     assert bt.resident_work()[3] > 0
 
 
-def test_resident_batched_instances():
-    """The first 12 seed-7 batched instances (uneven N, chain and grid
-    topologies, RZZ/RX/RZ) for 2 layers, per key-width group."""
-    ws = wl.batched_instances(12, seed=7, max_depth=2)
+@pytest.mark.parametrize("count", [12, 64])
+def test_resident_batched_instances(count):
+    """The first `count` seed-7 batched instances (uneven N, chain and grid
+    topologies, RZZ/RX/RZ) for 2 layers, per key-width group.  With 64
+    instances the XCDs cannot give every vector the workgroups a
+    quarter-full start asks for, so the planner falls back to fuller
+    chunks."""
+    ws = wl.batched_instances(count, seed=7, max_depth=2)
     for sel in ([w for w in ws if w.N <= 64], [w for w in ws if w.N > 64]):
         if not sel:
             continue
