@@ -96,8 +96,8 @@ int pp_create(int N, int64_t live_cap, int64_t append_cap, pp_handle** out);
  * Julia's arithmetic bit for bit: the rotation child is (T(k-2) sinθ) c as a
  * Complex * Complex product (spv_kernels.jl:163), |c| is Julia's hypot
  * (abs(::Complex)) in every filter, sums and real scalings are componentwise.
- * ComplexF64 vectors evolve on the per-step kernels (the resident layer
- * launches hold Float64 coefficients). */
+ * The resident layer launches hold either coefficient type (ComplexF64
+ * chunks of at most 3072 terms at N > 64, 16-byte coefficients in LDS). */
 int pp_create_typed(int N, int64_t live_cap, int64_t append_cap, int complex_coeffs, pp_handle** out);
 /* *complex_coeffs = 1 for a ComplexF64 handle, 0 for Float64. */
 int pp_coeff_type(pp_handle* h, int* complex_coeffs);

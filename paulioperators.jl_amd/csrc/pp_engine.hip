@@ -905,7 +905,7 @@ struct Batch {
     // Resident layer launches (kr_layer).
 
     int res_cap() const {
-        const int full = W == 1 ? ResGeom<1>::cap : ResGeom<2>::cap;
+        const int full = W == 1 ? ResGeom<1>::cap : (cw == 1 ? ResGeom<2, double>::cap : ResGeom<2, pp_cplx>::cap);
         return res_cap_limit > 0 && res_cap_limit < full ? res_cap_limit : full;
     }
 
@@ -916,9 +916,7 @@ struct Batch {
     // false: the call runs on the per-step kernels (auto mode also declines
     // calls of few steps and vectors that do not fit one XCD at 90% fill).
     bool res_plan(const StepSpan& steps, int64_t nsteps) {
-        // the resident kernel holds Float64 coefficients; ComplexF64 vectors
-        // run on the per-step kernels
-        if (res_mode == 0 || res_broken || res_kx == 0 || cw != 1) return false;
+        if (res_mode == 0 || res_broken || res_kx == 0) return false;
         if (res_mode == 1 && nsteps < 8) return false;
         std::vector<char> part((size_t)B, 0);
         for (int64_t s = 0; s < nsteps; ++s)
@@ -1031,9 +1029,9 @@ struct Batch {
                 HIPCHK(hipEventCreate(&e0));
                 HIPCHK(hipEventCreate(&e1));
                 res_ev.emplace_back(e0, e1);
-                hipExtLaunchKernelGGL((kr_layer<Wt>), grid, block, 0, stream, e0, e1, 0, a);
+                hipExtLaunchKernelGGL((kr_layer<Wt, CT>), grid, block, 0, stream, e0, e1, 0, a);
             } else {
-                hipLaunchKernelGGL((kr_layer<Wt>), grid, block, 0, stream, a);
+                hipLaunchKernelGGL((kr_layer<Wt, CT>), grid, block, 0, stream, a);
             }
             hipLaunchKernelGGL(kr_commit, dim3((unsigned)cdiv(B, 256)), dim3(256), 0, stream, (const ResCtl*)d_rctl,
                                (const int32_t*)d_rpart, d_state, B, first, d_rwork);

@@ -60,17 +60,23 @@ constexpr int kResWaves = kResThreads / kWave;  // 8
 constexpr int kResMaxK = 32;                    // workgroups per vector (one XCD)
 constexpr int kResLevels = 4;                   // mask levels (more: the per-step kernels)
 constexpr int kResWords = 64;                   // chunk positions in 64-term words
-// Chunk capacity in terms: one LDS buffer of 16 W + 8 bytes per term holding
-// the chunk with its incoming children (the merge writes it in place), and
-// the positions each thread covers (capacity <= E * kResThreads, and at most
-// 64 positions per flag word).
-template <int W>
+// Chunk capacity in terms: one LDS buffer of 16 W + sizeof(CT) bytes per term
+// holding the chunk with its incoming children (the merge writes it in
+// place), and the positions each thread covers (capacity <= E * kResThreads,
+// and at most 64 positions per flag word).  ComplexF64 coefficients take 16
+// bytes, so at W = 2 the chunk holds 3072 terms (6 per thread).
+template <int W, typename CT = double>
 struct ResGeom {
-    static constexpr int cap = W == 1 ? 4096 : 3712;
-    static constexpr int E = 8;
+    static constexpr bool kCplx = sizeof(CT) == 16;
+    static constexpr int cap = W == 1 ? 4096 : (kCplx ? 3072 : 3712);
+    static constexpr int E = (cap + kResThreads - 1) / kResThreads;
 };
-static_assert(ResGeom<1>::cap <= ResGeom<1>::E * kResThreads && ResGeom<2>::cap <= ResGeom<2>::E * kResThreads &&
-                  ResGeom<1>::cap <= 64 * kResWords && ResGeom<2>::cap <= 64 * kResWords,
+template <int W, typename CT>
+constexpr bool res_geom_ok() {
+    return ResGeom<W, CT>::cap <= ResGeom<W, CT>::E * kResThreads && ResGeom<W, CT>::cap <= 64 * kResWords;
+}
+static_assert(res_geom_ok<1, double>() && res_geom_ok<2, double>() && res_geom_ok<1, pp_cplx>() &&
+                  res_geom_ok<2, pp_cplx>(),
               "chunk geometry");
 constexpr uint32_t kResAbort = 1u << 30;  // barrier counter bit: a member gave up
 constexpr uint32_t kResSpin = 1u << 21;   // polls before giving up (~seconds)
@@ -197,6 +203,13 @@ __device__ __forceinline__ int32_t ld_l2(const int32_t* p) {
 }
 __device__ __forceinline__ double ld_l2(const double* p) {
     return __longlong_as_double((long long)ld_l2(reinterpret_cast<const uint64_t*>(p)));
+}
+__device__ __forceinline__ pp_cplx ld_l2(const pp_cplx* p) {
+    const uint64_t* q = reinterpret_cast<const uint64_t*>(p);
+    pp_cplx r;
+    r.x = __longlong_as_double((long long)ld_l2(q));
+    r.y = __longlong_as_double((long long)ld_l2(q + 1));
+    return r;
 }
 template <int W>
 __device__ __forceinline__ Key<W> load_key_l2(const uint64_t* Z, const uint64_t* X, int64_t i) {
@@ -490,12 +503,12 @@ __device__ __forceinline__ void res_step_commit(const ResArgs& A, int b, ResStep
 constexpr int kBkRot = 0, kBkSwept = 1, kBkCreated = 2, kBkMin = 3, kBkMout = 4, kBkEarly = 5, kBkLive = 6,
               kBkApp = 7, kBkPn = 8, kBkPm = 9, kBk = 10;
 
-template <int W>
+template <int W, typename CT>
 __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
-    constexpr int Cap = ResGeom<W>::cap;
-    constexpr int E = ResGeom<W>::E;
+    constexpr int Cap = ResGeom<W, CT>::cap;
+    constexpr int E = ResGeom<W, CT>::E;
     __shared__ Key<W> s_k[Cap];
-    __shared__ double s_c[Cap];
+    __shared__ CT s_c[Cap];
     __shared__ uint64_t s_am[kResWords];
     __shared__ int32_t s_apre[kResWords];
     __shared__ uint64_t s_hm[kResLevels][kResWords];
@@ -586,7 +599,7 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
             nj = (int)(hi - lo);
             const uint64_t* Lz = cur ? dp->Lz[1] : dp->Lz[0];
             const uint64_t* Lx = cur ? dp->Lx[1] : dp->Lx[0];
-            const double* Lc = cur ? dp->Lc[1] : dp->Lc[0];
+            const CT* Lc = reinterpret_cast<const CT*>(cur ? dp->Lc[1] : dp->Lc[0]);
             for (int p = tid; p < nj; p += kResThreads) {
                 s_k[p] = load_key<W>(Lz, Lx, lo + p);
                 s_c[p] = ldg(Lc, lo + p);
@@ -623,7 +636,7 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
             nlev = kResLevels;
         }
         Key<W>* sk = s_k;
-        double* sc = s_c;
+        CT* sc = s_c;
         // ---- phase 1: flags, run heads, summaries, record
         const int e_need = (nj + kResThreads - 1) / kResThreads;  // position rows in use
         if (!last) {
@@ -650,8 +663,8 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
                     if (flag && local_active) {
                         const FilterReg fl = s_fl;  // (divergent here: stays in vector registers)
                         Key<W> ch;
-                        const double cn = G.child(Kp, sc[p], ch);
-                        flag = !should_drop<W>(fl, ch, fabs(cn));
+                        const CT cn = G.child(Kp, sc[p], ch);
+                        flag = !should_drop<W>(fl, ch, coef_abs(cn));
                     }
                     if (p > 0) hd = hdb<W>(sk[p - 1], Kp);
                 }
@@ -802,7 +815,7 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
             break;
         }
         Key<W>* skw = s_k;  // children are staged after the chunk
-        double* scw = s_c;
+        CT* scw = s_c;
         // ---- phase 2: children at their ranks (and the next rotation's step)
         ResStepLoad nxt;
         if (tid == kResThreads - 1) nxt = res_step_load(A, b, s + 1);
@@ -811,7 +824,7 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
             const int Aj = s_rec[k].a;
             uint64_t* Cz = dp->Cz;
             uint64_t* Cx = dp->Cx;
-            double* Cc = dp->Cc;
+            CT* Cc = reinterpret_cast<CT*>(dp->Cc);
 #pragma unroll 1
             for (int e = 0; e < e_need; ++e) {
                 const int p = e * kResThreads + tid;
@@ -865,7 +878,7 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
                     rank += delta;
                 }
                 Key<W> ch;
-                const double cn = G.child(Kp, sc[p], ch);
+                const CT cn = G.child(Kp, sc[p], ch);
                 if (rank < 0 || rank >= m_tot) {  // never expected
                     atomicOr(s_err, 1);
                     continue;
@@ -975,7 +988,7 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
                     if (s_int[6]) {
                         uint64_t* Oz = cur ? dp->Lz[0] : dp->Lz[1];
                         uint64_t* Ox = cur ? dp->Lx[0] : dp->Lx[1];
-                        double* Oc = cur ? dp->Lc[0] : dp->Lc[1];
+                        CT* Oc = reinterpret_cast<CT*>(cur ? dp->Lc[0] : dp->Lc[1]);
                         const int off = s_rbn[k];
                         for (int p = tid; p < nj; p += kResThreads) {
                             store_key<W>(Oz, Ox, off + p, sk[p]);
@@ -1008,7 +1021,7 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
                 const int64_t R0 = s_mk[kResMaxK];
                 const uint64_t* Cz = dp->Cz;
                 const uint64_t* Cx = dp->Cx;
-                const double* Cc = dp->Cc;
+                const CT* Cc = reinterpret_cast<const CT*>(dp->Cc);
                 for (int p = tid; p < mk; p += kResThreads) {
                     skw[nj + p] = load_key_l2<W>(Cz, Cx, R0 + p);
                     scw[nj + p] = ld_l2(Cc + R0 + p);
@@ -1042,19 +1055,19 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
             bool skip_first = false;
             if (dk < stop && ia > 0 && jb < mk) skip_first = key_eq<W>(sL[ia - 1], sC[jb]);
             Key<W> outK[E];
-            double outC[E];
+            CT outC[E];
             uint32_t keep = 0;
             int cnt = 0;
             const int lmax = Cap - 1;
 #pragma unroll
             for (int e = 0; e < E; ++e) {
-                outC[e] = 0.0;
+                outC[e] = coef_zero<CT>();
                 if (ia + jb >= stop) break;
                 const bool haveL = ia < nj, haveC = jb < mk;
                 const int il = ia < lmax ? ia : lmax;
                 const int ic = nj + jb < lmax ? nj + jb : lmax;
                 const Key<W> a = sk[il], c = sk[ic];
-                const double ca = sc[il], cc = sc[ic];
+                const CT ca = sc[il], cc = sc[ic];
                 const bool takeL = haveL & (!haveC | key_le<W>(a, c));
                 const bool eq = takeL & haveC & key_eq<W>(a, c);
                 Key<W> kv;
@@ -1063,7 +1076,7 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
                     kv.z[w] = takeL ? a.z[w] : c.z[w];
                     kv.x[w] = takeL ? a.x[w] : c.x[w];
                 }
-                double acc = ca;
+                CT acc = ca;
                 if (G.anti(a)) acc = acc * G.cosv;
                 if (eq) acc = acc + cc;  // the equal-key child follows its live term
                 if (!takeL) acc = cc;
@@ -1071,7 +1084,7 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
                 ia += takeL ? 1 : 0;
                 jb += (!takeL | eq) ? 1 : 0;
                 outK[e] = kv;
-                if (emit && !should_drop<W>(fs, kv, fabs(acc))) {
+                if (emit && !should_drop<W>(fs, kv, coef_abs(acc))) {
                     keep |= 1u << e;
                     outC[e] = acc;
                     ++cnt;
@@ -1114,7 +1127,7 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
         const int mine = nj;
         uint64_t* Oz = cur ? dp->Lz[0] : dp->Lz[1];
         uint64_t* Ox = cur ? dp->Lx[0] : dp->Lx[1];
-        double* Oc = cur ? dp->Lc[0] : dp->Lc[1];
+        CT* Oc = reinterpret_cast<CT*>(cur ? dp->Lc[0] : dp->Lc[1]);
         for (int p = tid; p < mine; p += kResThreads) {
             store_key<W>(Oz, Ox, off + p, s_k[p]);
             stg(Oc, off + p, s_c[p]);

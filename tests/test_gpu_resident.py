@@ -74,6 +74,55 @@ def test_resident_single_vector_sequences(strat, cap):
         assert g.capacities() == o.capacities()
 
 
+@pytest.mark.parametrize("cap", [0, 48, 200])
+@pytest.mark.parametrize("strat", [NoTruncation(), CoeffTruncation(1e-4), WeightTruncation(3),
+                                   CompositeTruncation(WeightTruncation(4), CoeffTruncation(1e-6))],
+                         ids=lambda s: type(s).__name__)
+def test_resident_complex_single_vector_sequences(strat, cap):
+    """ComplexF64 coefficients in the resident launch (16-byte coefficients
+    in LDS; 3072-term chunks at two key words): the Heisenberg sequences with
+    a complex observable, against the ComplexF64 oracle bit for bit (both
+    parts) and counter for counter."""
+    for N in (8, 70):
+        gens, angs = heisenberg_gens(8, nsteps=3)
+        gens = [PauliBasis(N, p.z, p.x) for p in gens]
+        p2 = Pauli(N, X=[2], Y=[3])
+        O0 = [(Pauli(N, Z=[4]).z, 0, complex(0.8, 0.6)), (p2.z, p2.x, complex(0.3, -0.2))]
+        g = SparsePauliVector(N, [((z, x), c) for z, x, c in O0], T=complex).set_resident(2, cap)
+        o = po.OracleSPV(N, O0, T=complex)
+        cnt = PPCounters()
+        g.evolve_(gens, angs, truncation=strat, counters=cnt)
+        oc = o.evolve([(p.z, p.x) for p in gens], angs, strict=compile_filter(strat))
+        log(f"complex {type(strat).__name__} N={N} cap={cap}: n={len(g)}")
+        assert g.arrays()[2].dtype == np.complex128
+        assert same_arrays(g.arrays(), o.arrays()), f"{type(strat).__name__} N={N} cap={cap}"
+        assert (cnt.rotations, cnt.terms_swept, cnt.terms_created, cnt.merge_in, cnt.merge_out,
+                cnt.early_merges) == (oc.rotations, oc.terms_swept, oc.terms_created, oc.merge_in, oc.merge_out,
+                                      oc.early_merges)
+        assert g.capacities() == o.capacities()
+
+
+@pytest.mark.parametrize("cap", [0, 256])
+def test_resident_complex_lattice121(cap):
+    """The 121-qubit lattice (two key words) as one ComplexF64 vector with a
+    complex observable, three layers with dephasing: multi-workgroup
+    clusters at cap 256, against the ComplexF64 oracle."""
+    w = wl.lattice121(0.3, n_layers=3)
+    terms = [(p.z, p.x, complex(c, 0.5 * c)) for p, c in w.observable]
+    g = SparsePauliVector(w.N, [((z, x), c) for z, x, c in terms], T=complex).set_resident(2, cap)
+    o = po.OracleSPV(w.N, terms, T=complex)
+    f = compile_filter(w.truncation)
+    for li in range(3):
+        L = w.layers[li]
+        g.evolve_(L.generators, L.angles, truncation=w.truncation)
+        assert w.channel[0] == "dephasing"
+        g.dephasing_channel_(w.channel[1])
+        o.evolve([(p.z, p.x) for p in L.generators], L.angles, strict=f)
+        o.channel(*w.channel)
+        assert same_arrays(g.arrays(), o.arrays()), f"layer {li}"
+        log(f"complex lattice cap={cap} layer {li}: n={len(g)}")
+
+
 @pytest.mark.parametrize("cap", [0, 64, 300])
 def test_resident_dense_collisions(cap):
     """N = 6 (4096 keys): most children land on live keys; random generators."""
