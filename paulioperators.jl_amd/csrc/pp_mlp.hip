@@ -374,8 +374,8 @@ __global__ void __launch_bounds__(256, 3) kg_gemm2(GemmArgs g) {
     }
     // accumulator 4 c + q of lane l in block (i, j): row 64 wm + 32 i + (l mod
     // 32), column 64 wn + 32 j + 8 c + 4 (l div 32) + q of the tile.  bf16
-    // outputs go through LDS (the A buffers, free after the last barrier) so
-    // that the global stores are whole 256-byte rows.
+    // outputs go through LDS (the stage's buffer, free after the last
+    // barrier) so that the global stores are whole 256-byte rows.
     constexpr bool kOut = epi_out(EPI);
     constexpr bool kH = epi_keep_h(EPI) || EPI == 2;
     constexpr int kLDH = 136;  // bf16 per LDS row of the output tile (272 B)
