@@ -127,6 +127,40 @@ def merge_bandwidth_regime(count=256):
     return b / t / 1e9, 1e6 * t / n, b / n, n
 
 
+def complex_resident(theta_h=0.3, reps=2):
+    """One 121-qubit lattice vector (θ_h = 0.3, all 20 layers with dephasing)
+    as Float64 and as ComplexF64 (the reference's default element type), each
+    with resident layer launches and on the per-step kernels: wall ms per run
+    (best of `reps`), and whether the ComplexF64 run's real parts equal the
+    Float64 run bit for bit (the observable's coefficients are real)."""
+    import numpy as np
+    from pauliprop import SparsePauliVector, workloads as wl
+    w = wl.lattice121(theta_h)
+    out = {"workload": f"lattice121 theta_h={theta_h}, 20 layers, one vector", "ms": {}}
+    finals = {}
+    for T in (float, complex):
+        for mode in (1, 0):
+            best = None
+            for _ in range(reps):
+                g = SparsePauliVector(w.N, [((p.z, p.x), T(c)) for p, c in w.observable], T=T).set_resident(mode, 0)
+                t0 = time.perf_counter()
+                for L in w.layers:
+                    g.evolve_(L.generators, L.angles, truncation=w.truncation)
+                    g.dephasing_channel_(w.channel[1])
+                n = len(g)
+                dt = time.perf_counter() - t0
+                best = dt if best is None else min(best, dt)
+            out["ms"][f"{T.__name__}_{'resident' if mode else 'per_step'}"] = best * 1e3
+            finals[(T, mode)] = g.arrays()
+    zf, xf, cf = finals[(float, 1)]
+    zc, xc, cc = finals[(complex, 1)]
+    out["terms"] = int(len(cf))
+    out["complex_real_parts_equal_float64"] = bool(np.array_equal(zf, zc) and np.array_equal(xf, xc)
+                                                   and np.array_equal(cf.view(np.uint64), cc.real.view(np.uint64))
+                                                   and not np.any(cc.imag))
+    return out
+
+
 def algebra_commutator(reps=3, min_terms=20000, cpu=True):
     """[H, O] on the 121-qubit lattice (pp_algebra, complex coefficients):
     H = sum of Z_a Z_b over the 220 bonds + 0.5 sum of X_j (341 terms), O = the
@@ -488,6 +522,8 @@ def main():
                     help="run only the windowed-evolution, lookup and AdaptiveTruncation measurements")
     ap.add_argument("--algebra", action="store_true",
                     help="run only the Pauli-sum product measurement ([H, O] on the 121-qubit lattice)")
+    ap.add_argument("--complex", action="store_true",
+                    help="run only the Float64 / ComplexF64 single-vector comparison (resident and per-step)")
     ap.add_argument("--bandwidth", action="store_true",
                     help="run only the bandwidth-regime merge measurement (256 circuits of the 36-qubit chain)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -506,6 +542,9 @@ def main():
         return
     if args.batched:
         print(json.dumps(batched_with_cpu(cpu=not args.no_cpu)), flush=True)
+        return
+    if args.complex:
+        print(json.dumps(complex_resident()), flush=True)
         return
     if args.bandwidth:
         gbs, us, nbytes, launches = merge_bandwidth_regime()
