@@ -525,6 +525,7 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
     __shared__ int32_t s_wsum[kResWaves];
     __shared__ int32_t s_int[8];   // role, slot, error bits, barrier result, owners, overflow
     __shared__ int32_t s_tot[3];   // n_tot, m_tot, A before this chunk
+    __shared__ int64_t s_al[2];    // a chunk's edges moved to changes of the z words
     __shared__ int64_t s_bk[kBk];
     __shared__ FilterReg s_fs, s_fl;
     __shared__ ResStep s_step[2];
@@ -592,14 +593,40 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
     int stall = 0;
     int64_t stall_step = A.first, need = 0;
     {
-        const int64_t lo = (n0 * k) / K, hi = (n0 * (k + 1)) / K;
+        int64_t lo = (n0 * k) / K, hi = (n0 * (k + 1)) / K;
         if ((n0 + K - 1) / K > cap) {  // the largest chunk (the same test in every member)
             stall = 3;
         } else {
-            nj = (int)(hi - lo);
             const uint64_t* Lz = cur ? dp->Lz[1] : dp->Lz[0];
             const uint64_t* Lx = cur ? dp->Lx[1] : dp->Lx[0];
             const CT* Lc = reinterpret_cast<const CT*>(cur ? dp->Lc[1] : dp->Lc[0]);
+            // Each edge moves to the next change of the z words within 64
+            // terms (the same function of the same index in both members
+            // that share it) when the larger chunks still fit: a rotation
+            // whose mask lies in the x words then keeps every child in its
+            // parent's chunk (see the local rotations below).
+            if (K > 1 && (n0 + K - 1) / K + kWave <= cap) {
+                if (wave < 2) {
+                    const int64_t e0 = wave == 0 ? lo : hi;
+                    int64_t ne = e0;
+                    if (e0 > 0 && e0 < n0) {
+                        const int64_t i = e0 + lane;
+                        bool chg = false;
+                        if (i < n0) {
+                            const Key<W> a = load_key<W>(Lz, Lx, i - 1), c = load_key<W>(Lz, Lx, i);
+#pragma unroll
+                            for (int w = 0; w < W; ++w) chg = chg || a.z[w] != c.z[w];
+                        }
+                        const uint64_t bm = __ballot(chg);
+                        if (bm) ne = e0 + __ffsll((unsigned long long)bm) - 1;
+                    }
+                    if (lane == 0) s_al[wave] = ne;
+                }
+                __syncthreads();
+                lo = s_al[0];
+                hi = s_al[1];
+            }
+            nj = (int)(hi - lo);
             for (int p = tid; p < nj; p += kResThreads) {
                 s_k[p] = load_key<W>(Lz, Lx, lo + p);
                 s_c[p] = ldg(Lc, lo + p);
@@ -748,10 +775,39 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
             const bool in = lane < K;
             const int ni = in ? s_rec[lane].n : 0, ai = in ? s_rec[lane].a : 0;
             const int nt = wave_total(ni), mt = wave_total(ai), ab = wave_total(lane < k ? ai : 0);
+            // Local rotation: every pair of neighbouring non-empty chunks
+            // differs above the mask's top bit, so a child (which differs from
+            // its parent only at the mask's bits) stays between the chunk's
+            // neighbours, in its parent's chunk, at rank - (A before the
+            // chunk) among the chunk's children; and every chunk keeps room for
+            // its own children.  The same records give every member the same
+            // answer.
+            const int topm = nlev > 0 ? (int)((lev4 >> (8 * (nlev - 1))) & 255u) : -1;
+            const bool ne = in && ni > 0;
+            const uint64_t nem = __ballot(ne);
+            bool ok = true;
+            if (ne) {
+                ok = ni + ai <= cap;
+                const uint64_t below = nem & res_low(lane);
+                if (below) {
+                    const int pv = 63 - __clzll((long long)below);
+                    Key<W> a, c;
+#pragma unroll
+                    for (int w = 0; w < W; ++w) {
+                        a.z[w] = s_rec[pv].lz[w];
+                        a.x[w] = s_rec[pv].lx[w];
+                        c.z[w] = s_rec[lane].fz[w];
+                        c.x[w] = s_rec[lane].fx[w];
+                    }
+                    ok = ok && hdb<W>(a, c) > topm;
+                }
+            }
+            const bool loc = K > 1 && !last && __ballot(!ok) == 0ull;
             if (lane == 0) {
                 s_tot[0] = nt;
                 s_tot[1] = mt;
                 s_tot[2] = ab;
+                s_int[7] = loc ? 1 : 0;
             }
         } else if (!last && wave <= nlev) {
             const int li = wave - 1;
@@ -819,6 +875,7 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
         // ---- phase 2: children at their ranks (and the next rotation's step)
         ResStepLoad nxt;
         if (tid == kResThreads - 1) nxt = res_step_load(A, b, s + 1);
+        const bool loc = s_int[7] != 0;
         {
             const int nown = s_int[4];
             const int Aj = s_rec[k].a;
@@ -886,6 +943,14 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
                 if (K == 1) {
                     skw[nj + rank] = ch;
                     scw[nj + rank] = cn;
+                } else if (loc) {
+                    const int r = rank - (int)abase;
+                    if (r < 0 || r >= Aj) {  // never expected
+                        atomicOr(s_err, 1);
+                        continue;
+                    }
+                    skw[nj + r] = ch;
+                    scw[nj + r] = cn;
                 } else {
                     store_key<W>(Cz, Cx, rank, ch);
                     stg(Cc, rank, cn);
@@ -895,6 +960,22 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
                         if (key_le<W>(s_ownk[mid], ch)) lo = mid;
                         else hi = mid;
                     }
+                    // ... or the next one when the child has its first key's z
+                    // words and no chunk holds part of that z group: z groups
+                    // stay whole, which keeps x-word rotations local (owners
+                    // stay monotone in the key, and the owner's keys all have
+                    // smaller z words)
+                    if (lo + 1 < nown) {
+                        const Key<W>& nf = s_ownk[lo + 1];
+                        const ResRec& ro = s_rec[s_owni[lo]];
+                        bool samez = true, split = true;
+#pragma unroll
+                        for (int w = 0; w < W; ++w) {
+                            samez = samez && ch.z[w] == nf.z[w];
+                            split = split && ro.lz[w] == nf.z[w];
+                        }
+                        if (samez && !split) lo += 1;
+                    }
                     atomicAdd(&s_hist[s_owni[lo]], 1);
                 }
             }
@@ -903,7 +984,9 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
         // phase 2); the barrier at the top of the loop publishes it
         if (tid == kResThreads - 1) res_step_commit(A, b, nxt, &s_step[cs ^ 1]);
         int mk = (int)m_tot;
-        if (K > 1) {
+        if (K > 1 && loc) {
+            mk = s_rec[k].a;  // the chunk's own children, staged in LDS above
+        } else if (K > 1) {
             __syncthreads();
             if (wave == 0) {  // this member's children per owner, as an exclusive prefix row
                 const int v = lane < K ? s_hist[lane] : 0;
@@ -982,7 +1065,10 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
                         }
                         const int nxtP = __shfl_down(P, 1, kWave);
                         const bool fitsk = lane >= K || (nxtP - P >= 0 && nxtP - P <= cap);
-                        if (lane == 0) s_int[6] = __ballot(!fitsk) == 0ull ? 1 : 0;
+                        const bool roomk = lane >= K || nxtP - P + kWave <= cap;
+                        // 2: the bounds may also move to changes of the z words
+                        const uint64_t bad = __ballot(!fitsk), tight = __ballot(!roomk);
+                        if (lane == 0) s_int[6] = bad != 0ull ? 0 : (tight == 0ull ? 2 : 1);
                     }
                     __syncthreads();
                     if (s_int[6]) {
@@ -999,7 +1085,28 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
                             aborted = true;
                             break;
                         }
-                        const int P0 = s_rbp[k], P1 = s_rbp[k + 1];
+                        int P0 = s_rbp[k], P1 = s_rbp[k + 1];
+                        if (s_int[6] == 2) {  // as for the initial chunks
+                            if (wave < 2) {
+                                const int e0 = wave == 0 ? P0 : P1;
+                                int ne = e0;
+                                if (e0 > 0 && e0 < (int)n_tot) {
+                                    const int i = e0 + lane;
+                                    bool chg = false;
+                                    if (i < (int)n_tot) {
+                                        const Key<W> a = load_key_l2<W>(Oz, Ox, i - 1), c = load_key_l2<W>(Oz, Ox, i);
+#pragma unroll
+                                        for (int w = 0; w < W; ++w) chg = chg || a.z[w] != c.z[w];
+                                    }
+                                    const uint64_t bm = __ballot(chg);
+                                    if (bm) ne = e0 + __ffsll((unsigned long long)bm) - 1;
+                                }
+                                if (lane == 0) s_al[wave] = ne;
+                            }
+                            __syncthreads();
+                            P0 = (int)s_al[0];
+                            P1 = (int)s_al[1];
+                        }
                         for (int p = tid; p < P1 - P0; p += kResThreads) {
                             sk[p] = load_key_l2<W>(Oz, Ox, P0 + p);
                             sc[p] = ld_l2(Oc + P0 + p);
