@@ -165,6 +165,7 @@ __device__ __forceinline__ int wave_incl_max(int v) {  // inclusive prefix maxim
 __device__ __forceinline__ int wave_shr1(int v) { return dpp_src<0x138, 0xf>(v); }  // lane l gets lane l - 1 (0 at lane 0)
 __device__ __forceinline__ int lane_of(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 __device__ __forceinline__ int wave_total(int v) { return lane_of(wave_incl(v), kWave - 1); }
+__device__ __forceinline__ int wave_max(int v) { return lane_of(wave_incl_max(v), kWave - 1); }  // values >= 0
 
 // Workgroup-uniform values read from LDS, moved to scalar registers (the
 // vector registers go to the per-thread terms).  Uniform control flow only.
@@ -499,6 +500,47 @@ __device__ __forceinline__ void res_step_commit(const ResArgs& A, int b, ResStep
     *out = r;
 }
 
+// Chunk edges moved to the first position of the highest differing bit
+// within kAlign terms: for the edge at e of a sorted sequence of n keys, the
+// smallest i in [e, e + kAlign) ∩ [1, n) that maximises hdb(key[i-1], key[i])
+// (e itself when e is 0 or n).  The same function of the same position in
+// both members that share an edge, and monotone in it.  Waves [0, kAlignWaves)
+// take e_lo, the next kAlignWaves waves e_hi; the whole workgroup calls it,
+// and the results are in out[0], out[1] after its closing barrier.
+constexpr int kAlignWaves = 1, kAlign = kAlignWaves * kWave;
+static_assert(2 * kAlignWaves <= kResWaves, "one wave per 64 positions of each edge");
+template <int W, bool L2>
+__device__ __forceinline__ void align_edges(const uint64_t* Z, const uint64_t* X, int64_t n, int64_t e_lo,
+                                            int64_t e_hi, int32_t (*s_v)[kAlignWaves], int64_t* out) {
+    const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
+    if (wave < 2 * kAlignWaves) {
+        const int side = wave / kAlignWaves, part = wave % kAlignWaves;
+        const int64_t e0 = side ? e_hi : e_lo;
+        int v = 0;  // (level + 1) << 12 | (4095 - offset); 0: none
+        if (e0 > 0 && e0 < n) {
+            const int64_t i = e0 + part * kWave + lane;
+            int hd = -1;
+            if (i < n) {
+                const Key<W> a = L2 ? load_key_l2<W>(Z, X, i - 1) : load_key<W>(Z, X, i - 1);
+                const Key<W> c = L2 ? load_key_l2<W>(Z, X, i) : load_key<W>(Z, X, i);
+                hd = hdb<W>(a, c);
+            }
+            const int mx = wave_max(hd + 1);  // every lane active
+            const uint64_t bm = __ballot(hd >= 0 && hd + 1 == mx);
+            if (bm) v = (mx << 12) | (4095 - (part * kWave + __ffsll((unsigned long long)bm) - 1));
+        }
+        if (lane == 0) s_v[side][part] = v;
+    }
+    __syncthreads();
+    if (tid < 2) {
+        const int64_t e0 = tid ? e_hi : e_lo;
+        int best = 0;
+        for (int q = 0; q < kAlignWaves; ++q) best = s_v[tid][q] > best ? s_v[tid][q] : best;
+        out[tid] = best ? e0 + (4095 - (best & 4095)) : e0;
+    }
+    __syncthreads();
+}
+
 // Bookkeeping slots (thread 0 of each member; member 0 stages them).
 constexpr int kBkRot = 0, kBkSwept = 1, kBkCreated = 2, kBkMin = 3, kBkMout = 4, kBkEarly = 5, kBkLive = 6,
               kBkApp = 7, kBkPn = 8, kBkPm = 9, kBk = 10;
@@ -525,7 +567,8 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
     __shared__ int32_t s_wsum[kResWaves];
     __shared__ int32_t s_int[8];   // role, slot, error bits, barrier result, owners, overflow
     __shared__ int32_t s_tot[3];   // n_tot, m_tot, A before this chunk
-    __shared__ int64_t s_al[2];    // a chunk's edges moved to changes of the z words
+    __shared__ int64_t s_al[2];    // a chunk's edges, moved by align_edges
+    __shared__ int32_t s_alv[2][kAlignWaves];
     __shared__ int64_t s_bk[kBk];
     __shared__ FilterReg s_fs, s_fl;
     __shared__ ResStep s_step[2];
@@ -600,29 +643,12 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
             const uint64_t* Lz = cur ? dp->Lz[1] : dp->Lz[0];
             const uint64_t* Lx = cur ? dp->Lx[1] : dp->Lx[0];
             const CT* Lc = reinterpret_cast<const CT*>(cur ? dp->Lc[1] : dp->Lc[0]);
-            // Each edge moves to the next change of the z words within 64
-            // terms (the same function of the same index in both members
-            // that share it) when the larger chunks still fit: a rotation
-            // whose mask lies in the x words then keeps every child in its
-            // parent's chunk (see the local rotations below).
-            if (K > 1 && (n0 + K - 1) / K + kWave <= cap) {
-                if (wave < 2) {
-                    const int64_t e0 = wave == 0 ? lo : hi;
-                    int64_t ne = e0;
-                    if (e0 > 0 && e0 < n0) {
-                        const int64_t i = e0 + lane;
-                        bool chg = false;
-                        if (i < n0) {
-                            const Key<W> a = load_key<W>(Lz, Lx, i - 1), c = load_key<W>(Lz, Lx, i);
-#pragma unroll
-                            for (int w = 0; w < W; ++w) chg = chg || a.z[w] != c.z[w];
-                        }
-                        const uint64_t bm = __ballot(chg);
-                        if (bm) ne = e0 + __ffsll((unsigned long long)bm) - 1;
-                    }
-                    if (lane == 0) s_al[wave] = ne;
-                }
-                __syncthreads();
+            // Edges at the highest differing bit within reach, when the
+            // larger chunks still fit: a rotation whose mask lies below every
+            // edge's level then keeps every child in its parent's chunk (see
+            // the local rotations below).
+            if (K > 1 && (n0 + K - 1) / K + kAlign <= cap) {
+                align_edges<W, false>(Lz, Lx, n0, lo, hi, s_alv, s_al);
                 lo = s_al[0];
                 hi = s_al[1];
             }
@@ -960,21 +986,23 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
                         if (key_le<W>(s_ownk[mid], ch)) lo = mid;
                         else hi = mid;
                     }
-                    // ... or the next one when the child has its first key's z
-                    // words and no chunk holds part of that z group: z groups
-                    // stay whole, which keeps x-word rotations local (owners
-                    // stay monotone in the key, and the owner's keys all have
-                    // smaller z words)
+                    // ... or the next one when the child agrees with that
+                    // chunk's first key above the level of the edge between
+                    // them (the highest bit where the owner's last key and
+                    // that first key differ): the child then lies above every
+                    // key of the owner, owners stay monotone in the key, and
+                    // no edge's level falls, which keeps the rotations whose
+                    // masks lie below every edge local
                     if (lo + 1 < nown) {
                         const Key<W>& nf = s_ownk[lo + 1];
                         const ResRec& ro = s_rec[s_owni[lo]];
-                        bool samez = true, split = true;
+                        Key<W> la;
 #pragma unroll
                         for (int w = 0; w < W; ++w) {
-                            samez = samez && ch.z[w] == nf.z[w];
-                            split = split && ro.lz[w] == nf.z[w];
+                            la.z[w] = ro.lz[w];
+                            la.x[w] = ro.lx[w];
                         }
-                        if (samez && !split) lo += 1;
+                        if (hdb<W>(ch, nf) < hdb<W>(la, nf)) lo += 1;
                     }
                     atomicAdd(&s_hist[s_owni[lo]], 1);
                 }
@@ -1065,7 +1093,7 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
                         }
                         const int nxtP = __shfl_down(P, 1, kWave);
                         const bool fitsk = lane >= K || (nxtP - P >= 0 && nxtP - P <= cap);
-                        const bool roomk = lane >= K || nxtP - P + kWave <= cap;
+                        const bool roomk = lane >= K || nxtP - P + kAlign <= cap;
                         // 2: the bounds may also move to changes of the z words
                         const uint64_t bad = __ballot(!fitsk), tight = __ballot(!roomk);
                         if (lane == 0) s_int[6] = bad != 0ull ? 0 : (tight == 0ull ? 2 : 1);
@@ -1087,23 +1115,7 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
                         }
                         int P0 = s_rbp[k], P1 = s_rbp[k + 1];
                         if (s_int[6] == 2) {  // as for the initial chunks
-                            if (wave < 2) {
-                                const int e0 = wave == 0 ? P0 : P1;
-                                int ne = e0;
-                                if (e0 > 0 && e0 < (int)n_tot) {
-                                    const int i = e0 + lane;
-                                    bool chg = false;
-                                    if (i < (int)n_tot) {
-                                        const Key<W> a = load_key_l2<W>(Oz, Ox, i - 1), c = load_key_l2<W>(Oz, Ox, i);
-#pragma unroll
-                                        for (int w = 0; w < W; ++w) chg = chg || a.z[w] != c.z[w];
-                                    }
-                                    const uint64_t bm = __ballot(chg);
-                                    if (bm) ne = e0 + __ffsll((unsigned long long)bm) - 1;
-                                }
-                                if (lane == 0) s_al[wave] = ne;
-                            }
-                            __syncthreads();
+                            align_edges<W, true>(Oz, Ox, n_tot, P0, P1, s_alv, s_al);
                             P0 = (int)s_al[0];
                             P1 = (int)s_al[1];
                         }
