@@ -1094,7 +1094,7 @@ __global__ void __launch_bounds__(kResThreads) kr_layer(ResArgs A) {
                         const int nxtP = __shfl_down(P, 1, kWave);
                         const bool fitsk = lane >= K || (nxtP - P >= 0 && nxtP - P <= cap);
                         const bool roomk = lane >= K || nxtP - P + kAlign <= cap;
-                        // 2: the bounds may also move to changes of the z words
+                        // 2: every chunk also has room for its edges to move (align_edges)
                         const uint64_t bad = __ballot(!fitsk), tight = __ballot(!roomk);
                         if (lane == 0) s_int[6] = bad != 0ull ? 0 : (tight == 0ull ? 2 : 1);
                     }
